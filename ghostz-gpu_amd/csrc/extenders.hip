@@ -1,0 +1,827 @@
+// extenders.hip -- the search-path kernels and the extender classes of the
+// C-ABI (DistanceCalculatorGpu, UngappedExtenderGpu, GappedExtenderGpu).
+//
+// Kernels are integer-only and written for gfx950 (wave64):
+//  * PackKernel           -- PackAlphabetCodeSequence on the device
+//  * DistanceKernel       -- reduced-alphabet mismatches over a window,
+//                            one pair per lane, reduced map in LDS
+//  * UngappedTriggerKernel-- reverse X-drop, forward continuation while the
+//                            sum is <= trigger; one seed per lane, the score
+//                            table in LDS, codes streamed from packed words
+//  * SeedScatter/Gather   -- SetQuerySeeds expansion via own max-scan
+//  * GappedKernel<W>      -- the CPU's row-major affine X-drop
+//                            (src/gapped_extender.cpp:306-471), one seed per
+//                            lane, DP row kept in a W-column LDS ring; seeds
+//                            whose band outgrows W go to a wide-ring pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gz_common.h"
+#include "gz_device.h"
+#include "gz_scan.h"
+
+namespace gz {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void SetError(const std::string &msg) { g_last_error = msg; }
+const char *LastError() { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// packing
+// ---------------------------------------------------------------------------
+__global__ void PackKernel(const uint8_t *codes, size_t n, uint64_t *packed,
+                           size_t words) {
+  const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= words) return;
+  uint64_t word = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kCodesPerWord; ++j) {
+    const size_t ci = w * kCodesPerWord + j;
+    uint64_t c = kDelimiter;
+    if (ci >= kPackPadCodes && ci - kPackPadCodes < n) c = codes[ci - kPackPadCodes];
+    word |= c << (kCodeBits * j);
+  }
+  packed[w] = word;
+}
+
+// ---------------------------------------------------------------------------
+// distance (src/distance_calculator_gpu.cu:187-286 semantics)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    DistanceKernel(size_t size, const uint32_t *qcenters,
+                   const uint32_t *dcenters, const uint64_t *queries,
+                   const uint64_t *database, const uint8_t *reduced_map,
+                   uint32_t length, char *out) {
+  __shared__ uint8_t map[kTableDim];
+  if (threadIdx.x < kTableDim) {
+    map[threadIdx.x] = (threadIdx.x < kNumberLetters)
+                           ? reduced_map[threadIdx.x]
+                           : (uint8_t)(100 + threadIdx.x);
+  }
+  __syncthreads();
+  const uint32_t half = length / 2;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < size;
+       i += (size_t)gridDim.x * blockDim.x) {
+    ForwardCursor a, b;
+    a.Init(queries, qcenters[i] - half);
+    b.Init(database, dcenters[i] - half);
+    int count = 0;
+    for (uint32_t k = 0; k < length; ++k) {
+      count += (map[a.Next()] != map[b.Next()]) ? 1 : 0;
+    }
+    out[i] = (char)count;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ungapped X-drop with trigger (src/seed_searcher.h:133-158,
+// src/ungapped_extender.h:34-82)
+// ---------------------------------------------------------------------------
+template <class Cursor>
+__device__ __forceinline__ int UngappedSide(Cursor &q, Cursor &d,
+                                            const int8_t *sm, uint32_t delim,
+                                            int cutoff) {
+  int best = 0;
+  uint32_t qc = q.Next(), dc = d.Next();
+  if (dc != delim && qc != delim) {
+    int score = 0;
+    do {
+      if (dc == delim || qc == delim) break;
+      score += sm[dc * kTableDim + qc];
+      qc = q.Next();
+      dc = d.Next();
+      if (score > 0) {
+        do {
+          best += score;
+          if (dc == delim || qc == delim) break;
+          score = sm[dc * kTableDim + qc];
+          qc = q.Next();
+          dc = d.Next();
+        } while (score > 0);
+      }
+    } while (score > -cutoff);
+  }
+  return best;
+}
+
+__device__ __forceinline__ int UngappedWithTrigger(
+    const uint64_t *queries, uint32_t qpos, const uint64_t *database,
+    uint32_t dpos, const int8_t *sm, uint32_t delim, int cutoff, int trigger) {
+  ReverseCursor rq, rd;
+  rq.Init(queries, qpos - 1);
+  rd.Init(database, dpos - 1);
+  int sum = UngappedSide(rq, rd, sm, delim, cutoff);
+  if (sum <= trigger) {
+    ForwardCursor fq, fd;
+    fq.Init(queries, qpos);
+    fd.Init(database, dpos);
+    sum += UngappedSide(fq, fd, sm, delim, cutoff);
+  }
+  return sum;
+}
+
+__global__ void __launch_bounds__(256)
+    UngappedTriggerKernel(size_t size, const uint32_t *qids,
+                          const uint32_t *qpos, const uint32_t *dpos,
+                          const uint64_t *queries, const uint64_t *database,
+                          const int *matrix, uint32_t nl, uint32_t delim,
+                          const int *cutoffs, const int *triggers,
+                          char *flags) {
+  __shared__ int8_t sm[kTableDim * kTableDim];
+  StageScoreTable(sm, matrix, nl);
+  __syncthreads();
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < size;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t q = qids[i];
+    const int trigger = triggers[q];
+    const int s = UngappedWithTrigger(queries, qpos[i], database, dpos[i], sm,
+                                      delim, cutoffs[q], trigger);
+    flags[i] = (s > trigger) ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SetQuerySeeds expansion (src/ungapped_extender_gpu.cu:332-408, 818-852)
+// ---------------------------------------------------------------------------
+__global__ void SeedScatterKernel(size_t runs, size_t size,
+                                  const uint32_t *starts, uint32_t *temp) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < runs && starts[i] < size) temp[starts[i]] = (uint32_t)i;
+}
+
+__global__ void SeedGatherKernel(size_t size, const uint32_t *run_index,
+                                 const uint32_t *seed_ids,
+                                 const uint32_t *seed_query_ids,
+                                 const uint32_t *seed_query_positions,
+                                 uint32_t *qids, uint32_t *qpos) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= size) return;
+  const uint32_t s = seed_ids[run_index[i]];
+  qids[i] = seed_query_ids[s];
+  qpos[i] = seed_query_positions[s];
+}
+
+// ---------------------------------------------------------------------------
+// gapped X-drop, one seed per lane
+// ---------------------------------------------------------------------------
+struct LaneRingLds {
+  int2 *cells;     // [W][64]
+  uint8_t *codes;  // [W][64]
+  uint32_t mask;
+  int lane;
+  __device__ __forceinline__ int2 &Cell(int i) {
+    return cells[(i & mask) * kWave + lane];
+  }
+  __device__ __forceinline__ uint8_t &Code(int i) {
+    return codes[(i & mask) * kWave + lane];
+  }
+};
+
+struct LaneRingGlobal {
+  int2 *cells;     // [W]
+  uint8_t *codes;  // [W]
+  uint32_t mask;
+  __device__ __forceinline__ int2 &Cell(int i) { return cells[i & mask]; }
+  __device__ __forceinline__ uint8_t &Code(int i) { return codes[i & mask]; }
+};
+
+struct GappedParams {
+  const uint64_t *queries;
+  const uint64_t *database;
+  uint32_t delim;
+  int gap_open;
+  int gap_ext;
+  int cutoff;
+};
+
+// Returns false when the band would not fit the ring (width >= W).
+template <bool kReverse, class Ring>
+__device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
+                              uint32_t qpos, uint32_t dpos, int *best_score,
+                              int *best_p0, int *best_p1, uint32_t *ncells) {
+  const int inc = kReverse ? -1 : 1;
+  const int W = (int)R.mask + 1;
+  *best_score = 0;
+  *best_p0 = -inc;
+  *best_p1 = -inc;
+  *ncells = 0;
+  // column c (c >= 1) holds seq0 code at qpos + (c - 1) * inc
+  auto load_code = [&](int col) -> uint32_t {
+    return PackedCodeAt(P.queries, (uint32_t)((int)qpos + (col - 1) * inc));
+  };
+  typename std::conditional<kReverse, ReverseCursor, ForwardCursor>::type dc;
+  dc.Init(P.database, dpos);
+  uint32_t d = dc.Next();
+  uint32_t c = load_code(1);
+  if (c == P.delim || d == P.delim) return true;
+
+  const int gap_init = P.gap_open + P.gap_ext;
+  const int cutoff = P.cutoff;
+  int array_start = 0, array_end = 0, ai;
+  int score = -gap_init;
+  R.Cell(0) = make_int2(0, -gap_init);
+  int col = 1;
+  for (ai = 1; c != P.delim; ++ai) {
+    if (score < -cutoff) break;
+    if (ai >= W) return false;
+    R.Cell(ai) = make_int2(score, score - gap_init);
+    R.Code(ai) = (uint8_t)c;
+    score -= P.gap_ext;
+    c = load_code(++col);
+  }
+  if (c != P.delim) {
+    if (ai >= W) return false;
+    R.Cell(ai) = make_int2(kInitScore, kInitScore);
+    R.Code(ai) = (uint8_t)c;
+    array_end = ai + 1;
+  } else {
+    array_end = ai;
+  }
+  int max_score = 0, max_p0 = -inc, max_p1 = -inc;
+  uint32_t cells = 0;
+  for (int row = 0; d != P.delim; ++row, d = dc.Next()) {
+    const int8_t *mrow = sm + d * kTableDim;
+    int2 first = R.Cell(array_start);
+    int prev = first.x;
+    first.x = (max_score - first.y > cutoff) ? kInitScore : first.y;
+    int gap_row = first.x - gap_init;
+    first.y -= P.gap_ext;
+    R.Cell(array_start) = first;
+    int last = array_start;
+    const int as0 = array_start;
+    for (ai = as0 + 1; ai < array_end; ++ai) {
+      ++cells;
+      int s = prev + mrow[R.Code(ai)];
+      int2 cell = R.Cell(ai);
+      prev = cell.x;
+      const int gap_col = cell.y;
+      s = (s < gap_col) ? gap_col : s;
+      s = (s < gap_row) ? gap_row : s;
+      if (max_score - s > cutoff) {
+        if (array_start + 1 == ai) ++array_start;
+        R.Cell(ai) = make_int2(kInitScore, kInitScore);
+        gap_row = kInitScore;
+      } else {
+        last = ai;
+        if (s > max_score) {
+          max_score = s;
+          max_p0 = (ai - 1) * inc;
+          max_p1 = row * inc;
+        }
+        const int ng = s - gap_init;
+        R.Cell(ai) = make_int2(s, (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext);
+        gap_row = (ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext;
+      }
+    }
+    if (array_start + 1 == array_end) break;
+    // seq0 position after the row loop is column array_end (old value)
+    int p0col = array_end;
+    if (last < array_end - 1) {
+      array_end = last + 1;
+      c = load_code(p0col);
+    } else {
+      c = load_code(p0col);
+      while (gap_row >= (max_score - cutoff) && c != P.delim) {
+        if (array_end - array_start >= W) return false;
+        R.Cell(array_end) = make_int2(gap_row, gap_row - gap_init);
+        R.Code(array_end) = (uint8_t)c;
+        gap_row -= P.gap_ext;
+        ++array_end;
+        c = load_code(++p0col);
+      }
+    }
+    if (c != P.delim) {
+      if (array_end - array_start >= W) return false;
+      // the appended column's code: cached if it was inside the band,
+      // otherwise it is the column just inspected
+      if (array_end == p0col) R.Code(array_end) = (uint8_t)c;
+      R.Cell(array_end) = make_int2(kInitScore, kInitScore);
+      ++array_end;
+    }
+  }
+  *best_score = max_score;
+  *best_p0 = max_p0;
+  *best_p1 = max_p1;
+  *ncells = cells;
+  return true;
+}
+
+constexpr int kGappedBlock = 64;
+
+template <bool kReverse, int W>
+__global__ void __launch_bounds__(kGappedBlock)
+    GappedKernel(size_t size, GappedParams P, const int *matrix, uint32_t nl,
+                 uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
+                 uint32_t *overflow_count, uint32_t *overflow_list) {
+  __shared__ int8_t sm[kTableDim * kTableDim];
+  __shared__ int2 ring_cells[W * kWave];
+  __shared__ uint8_t ring_codes[W * kWave];
+  StageScoreTable(sm, matrix, nl);
+  __syncthreads();
+  LaneRingLds R;
+  R.cells = ring_cells;
+  R.codes = ring_codes;
+  R.mask = W - 1;
+  R.lane = threadIdx.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < size;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t q = qpos[i], d = dpos[i];
+    int s, p0, p1;
+    uint32_t nc;
+    if (GappedOneSide<kReverse>(R, P, sm, q, d, &s, &p0, &p1, &nc)) {
+      scores[i] = s;
+      qpos[i] = (uint32_t)((int)q + p0);
+      dpos[i] = (uint32_t)((int)d + p1);
+      if (cells) cells[i] = nc;
+    } else {
+      const uint32_t slot = atomicAdd(overflow_count, 1u);
+      overflow_list[slot] = (uint32_t)i;
+    }
+  }
+}
+
+template <bool kReverse>
+__global__ void __launch_bounds__(kGappedBlock)
+    GappedWideKernel(GappedParams P, const int *matrix, uint32_t nl,
+                     uint32_t *qpos, uint32_t *dpos, int *scores,
+                     uint32_t *cells, const uint32_t *overflow_count,
+                     const uint32_t *overflow_list, int2 *scratch_cells,
+                     uint8_t *scratch_codes, uint32_t wide, uint32_t *failed) {
+  __shared__ int8_t sm[kTableDim * kTableDim];
+  StageScoreTable(sm, matrix, nl);
+  __syncthreads();
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t slots = gridDim.x * blockDim.x;
+  LaneRingGlobal R;
+  R.cells = scratch_cells + (size_t)slot * wide;
+  R.codes = scratch_codes + (size_t)slot * wide;
+  R.mask = wide - 1;
+  const uint32_t n = *overflow_count;
+  for (uint32_t j = slot; j < n; j += slots) {
+    const uint32_t i = overflow_list[j];
+    const uint32_t q = qpos[i], d = dpos[i];
+    int s, p0, p1;
+    uint32_t nc;
+    if (GappedOneSide<kReverse>(R, P, sm, q, d, &s, &p0, &p1, &nc)) {
+      scores[i] = s;
+      qpos[i] = (uint32_t)((int)q + p0);
+      dpos[i] = (uint32_t)((int)d + p1);
+      if (cells) cells[i] = nc;
+    } else {
+      atomicAdd(failed, 1u);
+    }
+  }
+}
+
+__global__ void ConvertSeedsKernel(size_t size, const uint32_t *seed_positions,
+                                   const uint32_t *qin, const uint32_t *din,
+                                   uint32_t *qout, uint32_t *dout) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= size) return;
+  const uint32_t s = seed_positions[i];
+  qout[i] = qin[s];
+  dout[i] = din[s];
+}
+
+inline unsigned GridFor(size_t n, unsigned block, unsigned cap = 65535u * 4) {
+  size_t g = (n + block - 1) / block;
+  if (g == 0) g = 1;
+  return (unsigned)std::min<size_t>(g, cap);
+}
+
+}  // namespace gz
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using namespace gz;
+
+struct gz_stream_s {
+  hipStream_t s;
+};
+
+static inline hipStream_t S(gz_stream_t st) { return st ? st->s : 0; }
+
+extern "C" {
+
+const char *gz_last_error(void) { return LastError(); }
+
+gz_status gz_set_device(int device) {
+  GZ_HIP_CHECK(hipSetDevice(device));
+  return GZ_OK;
+}
+gz_status gz_device_malloc(void **ptr, size_t bytes) {
+  GZ_REQUIRE(ptr != nullptr, "gz_device_malloc: null out pointer");
+  if (bytes == 0) bytes = 1;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    SetError(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return GZ_ERROR_OUT_OF_MEMORY;
+  }
+  return GZ_OK;
+}
+gz_status gz_device_free(void *ptr) {
+  if (ptr) GZ_HIP_CHECK(hipFree(ptr));
+  return GZ_OK;
+}
+gz_status gz_host_malloc(void **ptr, size_t bytes) {
+  GZ_REQUIRE(ptr != nullptr, "gz_host_malloc: null out pointer");
+  if (bytes == 0) bytes = 1;
+  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    SetError(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    return GZ_ERROR_OUT_OF_MEMORY;
+  }
+  return GZ_OK;
+}
+gz_status gz_host_free(void *ptr) {
+  if (ptr) GZ_HIP_CHECK(hipHostFree(ptr));
+  return GZ_OK;
+}
+gz_status gz_stream_create(gz_stream_t *stream) {
+  GZ_REQUIRE(stream != nullptr, "gz_stream_create: null out pointer");
+  gz_stream_s *s = new gz_stream_s();
+  hipError_t e = hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    SetError(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    return GZ_ERROR_HIP;
+  }
+  *stream = s;
+  return GZ_OK;
+}
+gz_status gz_stream_destroy(gz_stream_t stream) {
+  if (!stream) return GZ_OK;
+  hipError_t e = hipStreamDestroy(stream->s);
+  delete stream;
+  GZ_HIP_CHECK(e);
+  return GZ_OK;
+}
+gz_status gz_stream_synchronize(gz_stream_t stream) {
+  GZ_HIP_CHECK(hipStreamSynchronize(S(stream)));
+  return GZ_OK;
+}
+gz_status gz_memcpy_async(void *dst, const void *src, size_t bytes,
+                          gz_stream_t stream) {
+  if (bytes == 0) return GZ_OK;
+  GZ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, S(stream)));
+  return GZ_OK;
+}
+gz_status gz_memset_async(void *dst, int value, size_t bytes,
+                          gz_stream_t stream) {
+  if (bytes == 0) return GZ_OK;
+  GZ_HIP_CHECK(hipMemsetAsync(dst, value, bytes, S(stream)));
+  return GZ_OK;
+}
+
+size_t gz_packed_words(size_t codes) { return PackedWords(codes); }
+
+gz_status gz_pack_sequence_async(const uint8_t *d_codes, size_t n,
+                                 uint64_t *d_packed, gz_stream_t stream) {
+  GZ_REQUIRE(d_packed != nullptr, "gz_pack_sequence_async: null output");
+  const size_t words = PackedWords(n);
+  hipLaunchKernelGGL(PackKernel, dim3(GridFor(words, 256, 1u << 30)), dim3(256), 0,
+                     S(stream), d_codes, n, d_packed, words);
+  GZ_HIP_CHECK(hipGetLastError());
+  return GZ_OK;
+}
+
+uint8_t gz_packed_code_at(const uint64_t *packed, size_t position) {
+  const size_t p = position + kPackPadCodes;
+  return (uint8_t)((packed[p / kCodesPerWord] >> (kCodeBits * (p % kCodesPerWord))) & 31u);
+}
+
+// ---- DistanceCalculatorGpu ----
+struct gz_distance_calculator_s {
+  const uint64_t *queries = nullptr;
+  const uint64_t *database = nullptr;
+  const uint8_t *reduced_map = nullptr;
+  uint32_t length = 0;
+};
+
+gz_status gz_distance_calculator_create(gz_distance_calculator_t *h) {
+  GZ_REQUIRE(h != nullptr, "null handle pointer");
+  *h = new gz_distance_calculator_s();
+  return GZ_OK;
+}
+gz_status gz_distance_calculator_destroy(gz_distance_calculator_t h) {
+  delete h;
+  return GZ_OK;
+}
+gz_status gz_distance_calculator_set_queries(gz_distance_calculator_t h,
+                                             const uint64_t *d) {
+  GZ_REQUIRE(h, "null handle");
+  h->queries = d;
+  return GZ_OK;
+}
+gz_status gz_distance_calculator_set_database(gz_distance_calculator_t h,
+                                              const uint64_t *d) {
+  GZ_REQUIRE(h, "null handle");
+  h->database = d;
+  return GZ_OK;
+}
+gz_status gz_distance_calculator_set_reduced_code_map(gz_distance_calculator_t h,
+                                                      const uint8_t *d) {
+  GZ_REQUIRE(h, "null handle");
+  h->reduced_map = d;
+  return GZ_OK;
+}
+gz_status gz_distance_calculator_set_subsequence_length(gz_distance_calculator_t h,
+                                                        uint32_t length) {
+  GZ_REQUIRE(h, "null handle");
+  h->length = length;
+  return GZ_OK;
+}
+gz_status gz_distance_calculator_calculate_distances_async(
+    gz_distance_calculator_t h, size_t size, const uint32_t *h_qc,
+    const uint32_t *h_dc, char *h_dist, uint32_t *d_qc, uint32_t *d_dc,
+    char *d_dist, gz_stream_t stream) {
+  GZ_REQUIRE(h && h->queries && h->database && h->reduced_map,
+             "distance calculator: queries, database and reduced map must be set");
+  if (size == 0) return GZ_OK;
+  GZ_REQUIRE(d_qc && d_dc && d_dist, "distance calculator: null device buffer");
+  hipStream_t s = S(stream);
+  if (h_qc) GZ_HIP_CHECK(hipMemcpyAsync(d_qc, h_qc, size * 4, hipMemcpyDefault, s));
+  if (h_dc) GZ_HIP_CHECK(hipMemcpyAsync(d_dc, h_dc, size * 4, hipMemcpyDefault, s));
+  hipLaunchKernelGGL(DistanceKernel, dim3(GridFor(size, 256)), dim3(256), 0, s,
+                     size, d_qc, d_dc, h->queries, h->database, h->reduced_map,
+                     h->length, d_dist);
+  GZ_HIP_CHECK(hipGetLastError());
+  if (h_dist) GZ_HIP_CHECK(hipMemcpyAsync(h_dist, d_dist, size, hipMemcpyDefault, s));
+  return GZ_OK;
+}
+
+// ---- UngappedExtenderGpu ----
+struct gz_ungapped_extender_s {
+  uint32_t delimiter = kDelimiter;
+  const uint32_t *seed_query_ids = nullptr;
+  const uint32_t *seed_query_positions = nullptr;
+  const uint64_t *queries = nullptr;
+  const uint64_t *database = nullptr;
+  const int *cutoffs = nullptr;
+  const int *triggers = nullptr;
+  const int *matrix = nullptr;
+  uint32_t number_letters = 0;
+};
+
+gz_status gz_ungapped_extender_create(gz_ungapped_extender_t *h) {
+  GZ_REQUIRE(h != nullptr, "null handle pointer");
+  *h = new gz_ungapped_extender_s();
+  return GZ_OK;
+}
+gz_status gz_ungapped_extender_destroy(gz_ungapped_extender_t h) {
+  delete h;
+  return GZ_OK;
+}
+gz_status gz_ungapped_extender_set_query_seed_data_list(gz_ungapped_extender_t h,
+                                                        const uint32_t *ids,
+                                                        const uint32_t *pos) {
+  GZ_REQUIRE(h, "null handle");
+  h->seed_query_ids = ids;
+  h->seed_query_positions = pos;
+  return GZ_OK;
+}
+gz_status gz_ungapped_extender_set_queries(gz_ungapped_extender_t h,
+                                           uint8_t delim, const uint64_t *q,
+                                           const int *cutoffs,
+                                           const int *triggers) {
+  GZ_REQUIRE(h, "null handle");
+  h->delimiter = delim;
+  h->queries = q;
+  h->cutoffs = cutoffs;
+  h->triggers = triggers;
+  return GZ_OK;
+}
+gz_status gz_ungapped_extender_set_database(gz_ungapped_extender_t h,
+                                            const uint64_t *d) {
+  GZ_REQUIRE(h, "null handle");
+  h->database = d;
+  return GZ_OK;
+}
+gz_status gz_ungapped_extender_set_score_matrix(gz_ungapped_extender_t h,
+                                                const int *m, uint32_t nl) {
+  GZ_REQUIRE(h, "null handle");
+  GZ_REQUIRE(nl <= kTableDim, "score matrix larger than 32 letters");
+  h->matrix = m;
+  h->number_letters = nl;
+  return GZ_OK;
+}
+
+size_t gz_ungapped_extender_query_seeds_temp_bytes(size_t size) {
+  return DeviceScanTempBytes<uint32_t>(size);
+}
+
+gz_status gz_ungapped_extender_set_query_seeds(
+    gz_ungapped_extender_t h, size_t runs, size_t size, const uint32_t *h_ids,
+    const uint32_t *h_starts, size_t temp_bytes, void *d_temp,
+    uint32_t *d_ids, uint32_t *d_starts, uint32_t *d_qids, uint32_t *d_qpos,
+    uint32_t *d_tmp, gz_stream_t stream) {
+  GZ_REQUIRE(h && h->seed_query_ids && h->seed_query_positions,
+             "ungapped extender: query seed data list not set");
+  if (size == 0) return GZ_OK;
+  GZ_REQUIRE(temp_bytes >= DeviceScanTempBytes<uint32_t>(size),
+             "ungapped extender: temporary storage too small");
+  hipStream_t s = S(stream);
+  if (h_ids) GZ_HIP_CHECK(hipMemcpyAsync(d_ids, h_ids, runs * 4, hipMemcpyDefault, s));
+  if (h_starts) GZ_HIP_CHECK(hipMemcpyAsync(d_starts, h_starts, runs * 4, hipMemcpyDefault, s));
+  GZ_HIP_CHECK(hipMemsetAsync(d_tmp, 0, size * 4, s));
+  hipLaunchKernelGGL(SeedScatterKernel, dim3(GridFor(runs, 256, 1u << 30)), dim3(256), 0,
+                     s, runs, size, d_starts, d_tmp);
+  GZ_HIP_CHECK(hipGetLastError());
+  GZ_HIP_CHECK((DeviceScan<uint32_t, true>(d_tmp, d_tmp, size, true, d_temp, s)));
+  hipLaunchKernelGGL(SeedGatherKernel, dim3(GridFor(size, 256, 1u << 30)), dim3(256), 0,
+                     s, size, d_tmp, d_ids, h->seed_query_ids,
+                     h->seed_query_positions, d_qids, d_qpos);
+  GZ_HIP_CHECK(hipGetLastError());
+  return GZ_OK;
+}
+
+gz_status gz_ungapped_extender_extend_with_trigger_async(
+    gz_ungapped_extender_t h, size_t size, const uint32_t *h_qids,
+    const uint32_t *h_qpos, const uint32_t *h_dpos, char *h_flags,
+    uint32_t *d_qids, uint32_t *d_qpos, uint32_t *d_dpos, char *d_flags,
+    int *d_temp, gz_stream_t stream) {
+  (void)d_temp;
+  GZ_REQUIRE(h && h->queries && h->database && h->matrix && h->cutoffs &&
+                 h->triggers,
+             "ungapped extender: queries, database and score matrix must be set");
+  if (size == 0) return GZ_OK;
+  hipStream_t s = S(stream);
+  if (h_qids) GZ_HIP_CHECK(hipMemcpyAsync(d_qids, h_qids, size * 4, hipMemcpyDefault, s));
+  if (h_qpos) GZ_HIP_CHECK(hipMemcpyAsync(d_qpos, h_qpos, size * 4, hipMemcpyDefault, s));
+  if (h_dpos) GZ_HIP_CHECK(hipMemcpyAsync(d_dpos, h_dpos, size * 4, hipMemcpyDefault, s));
+  hipLaunchKernelGGL(UngappedTriggerKernel, dim3(GridFor(size, 256)), dim3(256), 0,
+                     s, size, d_qids, d_qpos, d_dpos, h->queries, h->database,
+                     h->matrix, h->number_letters, (uint32_t)h->delimiter,
+                     h->cutoffs, h->triggers, d_flags);
+  GZ_HIP_CHECK(hipGetLastError());
+  if (h_flags) GZ_HIP_CHECK(hipMemcpyAsync(h_flags, d_flags, size, hipMemcpyDefault, s));
+  return GZ_OK;
+}
+
+// ---- GappedExtenderGpu ----
+constexpr int kGappedRing = 64;
+constexpr uint32_t kWideSlots = 256;
+
+struct gz_gapped_extender_s {
+  uint32_t delimiter = kDelimiter;
+  const uint64_t *queries = nullptr;
+  const uint64_t *database = nullptr;
+  const int *matrix = nullptr;
+  uint32_t number_letters = 0;
+  int gap_open = 0, gap_ext = 0, cutoff = 0;
+  uint32_t wide = 1u << 16;  // wide-ring columns (>= longest query + 2)
+  // workspace
+  uint32_t *counters = nullptr;        // [0] overflow count, [1] failures
+  uint32_t *overflow_list = nullptr;
+  size_t overflow_capacity = 0;
+  int2 *wide_cells = nullptr;
+  uint8_t *wide_codes = nullptr;
+  uint32_t wide_allocated = 0;
+  ~gz_gapped_extender_s() {
+    if (counters) (void)hipFree(counters);
+    if (overflow_list) (void)hipFree(overflow_list);
+    if (wide_cells) (void)hipFree(wide_cells);
+    if (wide_codes) (void)hipFree(wide_codes);
+  }
+};
+
+gz_status gz_gapped_extender_create(gz_gapped_extender_t *h) {
+  GZ_REQUIRE(h != nullptr, "null handle pointer");
+  *h = new gz_gapped_extender_s();
+  return GZ_OK;
+}
+gz_status gz_gapped_extender_destroy(gz_gapped_extender_t h) {
+  delete h;
+  return GZ_OK;
+}
+gz_status gz_gapped_extender_set_queries(gz_gapped_extender_t h, uint8_t delim,
+                                         const uint64_t *q) {
+  GZ_REQUIRE(h, "null handle");
+  h->delimiter = delim;
+  h->queries = q;
+  return GZ_OK;
+}
+gz_status gz_gapped_extender_set_database(gz_gapped_extender_t h,
+                                          const uint64_t *d) {
+  GZ_REQUIRE(h, "null handle");
+  h->database = d;
+  return GZ_OK;
+}
+gz_status gz_gapped_extender_set_score_parameters(gz_gapped_extender_t h,
+                                                  const int *m, uint32_t nl,
+                                                  int gap_open, int gap_ext,
+                                                  int cutoff) {
+  GZ_REQUIRE(h, "null handle");
+  GZ_REQUIRE(nl <= kTableDim, "score matrix larger than 32 letters");
+  h->matrix = m;
+  h->number_letters = nl;
+  h->gap_open = gap_open;
+  h->gap_ext = gap_ext;
+  h->cutoff = cutoff;
+  return GZ_OK;
+}
+
+gz_status gz_gapped_extender_convert_to_gapped_extension_seeds_async(
+    gz_gapped_extender_t h, size_t size, int reverse, const uint32_t *h_seed,
+    const uint32_t *h_q, const uint32_t *h_d, uint32_t *d_seed,
+    uint32_t *d_temp, uint32_t *d_q, uint32_t *d_d, gz_stream_t stream) {
+  (void)reverse;
+  GZ_REQUIRE(h, "null handle");
+  if (size == 0) return GZ_OK;
+  hipStream_t s = S(stream);
+  // d_temp holds the un-permuted query positions, then database positions
+  uint32_t *tq = d_temp, *td = d_temp + size;
+  if (h_seed) GZ_HIP_CHECK(hipMemcpyAsync(d_seed, h_seed, size * 4, hipMemcpyDefault, s));
+  GZ_HIP_CHECK(hipMemcpyAsync(tq, h_q ? h_q : d_q, size * 4, hipMemcpyDefault, s));
+  GZ_HIP_CHECK(hipMemcpyAsync(td, h_d ? h_d : d_d, size * 4, hipMemcpyDefault, s));
+  hipLaunchKernelGGL(ConvertSeedsKernel, dim3(GridFor(size, 256, 1u << 30)), dim3(256), 0,
+                     s, size, d_seed, tq, td, d_q, d_d);
+  GZ_HIP_CHECK(hipGetLastError());
+  return GZ_OK;
+}
+
+static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
+                           uint32_t *d_q, uint32_t *d_d, int *d_scores,
+                           uint32_t *d_cells, hipStream_t s) {
+  GZ_REQUIRE(h->queries && h->database && h->matrix,
+             "gapped extender: queries, database and score parameters must be set");
+  if (!h->counters) GZ_HIP_CHECK(hipMalloc(&h->counters, 2 * sizeof(uint32_t)));
+  if (h->overflow_capacity < size) {
+    if (h->overflow_list) GZ_HIP_CHECK(hipFree(h->overflow_list));
+    size_t cap = std::max<size_t>(size, 1 << 16);
+    GZ_HIP_CHECK(hipMalloc(&h->overflow_list, cap * sizeof(uint32_t)));
+    h->overflow_capacity = cap;
+  }
+  if (h->wide_allocated < h->wide) {
+    if (h->wide_cells) GZ_HIP_CHECK(hipFree(h->wide_cells));
+    if (h->wide_codes) GZ_HIP_CHECK(hipFree(h->wide_codes));
+    GZ_HIP_CHECK(hipMalloc(&h->wide_cells, (size_t)kWideSlots * h->wide * sizeof(int2)));
+    GZ_HIP_CHECK(hipMalloc(&h->wide_codes, (size_t)kWideSlots * h->wide));
+    h->wide_allocated = h->wide;
+  }
+  GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, 2 * sizeof(uint32_t), s));
+  GappedParams P;
+  P.queries = h->queries;
+  P.database = h->database;
+  P.delim = h->delimiter;
+  P.gap_open = h->gap_open;
+  P.gap_ext = h->gap_ext;
+  P.cutoff = h->cutoff;
+  const unsigned grid = GridFor(size, kGappedBlock, 256u * 32);
+  if (reverse) {
+    hipLaunchKernelGGL((GappedKernel<true, kGappedRing>), dim3(grid), dim3(kGappedBlock), 0, s,
+                       size, P, h->matrix, h->number_letters, d_q, d_d, d_scores,
+                       d_cells, h->counters, h->overflow_list);
+    GZ_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((GappedWideKernel<true>), dim3(kWideSlots / kGappedBlock),
+                       dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q,
+                       d_d, d_scores, d_cells, h->counters, h->overflow_list,
+                       h->wide_cells, h->wide_codes, h->wide, h->counters + 1);
+  } else {
+    hipLaunchKernelGGL((GappedKernel<false, kGappedRing>), dim3(grid), dim3(kGappedBlock), 0, s,
+                       size, P, h->matrix, h->number_letters, d_q, d_d, d_scores,
+                       d_cells, h->counters, h->overflow_list);
+    GZ_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((GappedWideKernel<false>), dim3(kWideSlots / kGappedBlock),
+                       dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q,
+                       d_d, d_scores, d_cells, h->counters, h->overflow_list,
+                       h->wide_cells, h->wide_codes, h->wide, h->counters + 1);
+  }
+  GZ_HIP_CHECK(hipGetLastError());
+  return GZ_OK;
+}
+
+gz_status gz_gapped_extender_extend_one_side_score_only_async(
+    gz_gapped_extender_t h, size_t size, int reverse, uint32_t *h_q,
+    uint32_t *h_d, int *h_scores, uint32_t *d_q, uint32_t *d_d, int *d_scores,
+    gz_stream_t stream) {
+  GZ_REQUIRE(h, "null handle");
+  if (size == 0) return GZ_OK;
+  hipStream_t s = S(stream);
+  if (h_q) GZ_HIP_CHECK(hipMemcpyAsync(d_q, h_q, size * 4, hipMemcpyDefault, s));
+  if (h_d) GZ_HIP_CHECK(hipMemcpyAsync(d_d, h_d, size * 4, hipMemcpyDefault, s));
+  gz_status st = GappedRun(h, size, reverse, d_q, d_d, d_scores, nullptr, s);
+  if (st != GZ_OK) return st;
+  if (h_q) GZ_HIP_CHECK(hipMemcpyAsync(h_q, d_q, size * 4, hipMemcpyDefault, s));
+  if (h_d) GZ_HIP_CHECK(hipMemcpyAsync(h_d, d_d, size * 4, hipMemcpyDefault, s));
+  if (h_scores) GZ_HIP_CHECK(hipMemcpyAsync(h_scores, d_scores, size * 4, hipMemcpyDefault, s));
+  return GZ_OK;
+}
+
+gz_status gz_gapped_extender_extend_one_side_score_only_cells_async(
+    gz_gapped_extender_t h, size_t size, int reverse, uint32_t *d_q,
+    uint32_t *d_d, int *d_scores, uint32_t *d_cells, gz_stream_t stream) {
+  GZ_REQUIRE(h, "null handle");
+  if (size == 0) return GZ_OK;
+  return GappedRun(h, size, reverse, d_q, d_d, d_scores, d_cells, S(stream));
+}
+
+}  // extern "C"
