@@ -1,0 +1,441 @@
+// capi.hip -- C ABI of the pipeline objects: queries, database, aligner.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "aligner.h"
+#include "alphabet.h"
+#include "cost_model_gpu.h"
+#include "database.h"
+#include "gz_common.h"
+#include "queries.h"
+#include "seg.h"
+
+using namespace gz;
+
+struct gz_queries_s {
+  QuerySet q;
+  std::vector<const char *> name_ptrs;
+};
+
+struct gz_database_s {
+  Database db;
+  std::vector<std::vector<const char *> > name_ptrs;
+};
+
+struct gz_aligner_s {
+  std::vector<int> matrix;
+  AlignerOptions opt;
+  std::unique_ptr<Aligner> aligner;
+  std::unique_ptr<GpuCostModel> model;
+};
+
+namespace {
+
+std::vector<FastaRecord> Records(const char *const *names, const char *residues,
+                                 const uint64_t *offsets, uint64_t n) {
+  std::vector<FastaRecord> r(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    r[i].name = names ? names[i] : std::to_string(i);
+    r[i].sequence.assign(residues + offsets[i], residues + offsets[i + 1]);
+    for (char &c : r[i].sequence) c = (char)toupper((unsigned char)c);
+  }
+  return r;
+}
+
+bool ReadAll(const char *path, std::vector<FastaRecord> *out) {
+  std::ifstream in(path);
+  if (!in) return false;
+  FastaRecord rec;
+  while (ReadFasta(in, &rec)) out->push_back(rec);
+  return true;
+}
+
+std::vector<int> Blosum() {
+  std::vector<int> m;
+  ParseScoreMatrix(Blosum62Text(), m);
+  return m;
+}
+
+void FinishQueries(gz_queries_s *h) {
+  h->name_ptrs.clear();
+  for (auto &s : h->q.names) h->name_ptrs.push_back(s.c_str());
+}
+
+void FinishDatabase(gz_database_s *h) {
+  h->name_ptrs.clear();
+  for (auto &c : h->db.chunks) {
+    h->name_ptrs.emplace_back();
+    for (auto &s : c.names) h->name_ptrs.back().push_back(s.c_str());
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- queries ----
+gz_status gz_queries_create(const char *const *names, const char *residues,
+                            const uint64_t *offsets, uint64_t n,
+                            const gz_query_options *o, gz_queries_t *out) {
+  GZ_REQUIRE(out && (n == 0 || (residues && offsets)), "gz_queries_create: null argument");
+  QueryOptions opt;
+  if (o) {
+    opt.translated = o->translated != 0;
+    opt.filter = o->filter != 0;
+    opt.threads = o->threads;
+  }
+  std::vector<int> m = Blosum();
+  gz_queries_s *h = new gz_queries_s();
+  BuildQueries(Records(names, residues, offsets, n), opt, m.data(), kNumberLetters, &h->q);
+  FinishQueries(h);
+  *out = h;
+  return GZ_OK;
+}
+
+gz_status gz_queries_read_fasta(const char *path, const gz_query_options *o,
+                                gz_queries_t *out) {
+  GZ_REQUIRE(path && out, "gz_queries_read_fasta: null argument");
+  std::vector<FastaRecord> recs;
+  if (!ReadAll(path, &recs)) {
+    SetError(std::string("cannot read ") + path);
+    return GZ_ERROR_IO;
+  }
+  QueryOptions opt;
+  if (o) {
+    opt.translated = o->translated != 0;
+    opt.filter = o->filter != 0;
+    opt.threads = o->threads;
+  }
+  std::vector<int> m = Blosum();
+  gz_queries_s *h = new gz_queries_s();
+  BuildQueries(recs, opt, m.data(), kNumberLetters, &h->q);
+  FinishQueries(h);
+  *out = h;
+  return GZ_OK;
+}
+
+gz_status gz_queries_destroy(gz_queries_t h) {
+  delete h;
+  return GZ_OK;
+}
+
+gz_status gz_queries_view(gz_queries_t h, gz_queries_view_t *v) {
+  GZ_REQUIRE(h && v, "gz_queries_view: null argument");
+  v->number_queries = h->q.size();
+  v->codes = h->q.codes.data();
+  v->offsets = h->q.offsets.data();
+  v->names = h->name_ptrs.data();
+  v->lambda = h->q.lambda.data();
+  v->K = h->q.K.data();
+  v->translated = h->q.translated ? 1 : 0;
+  v->dna_lengths = h->q.translated ? h->q.dna_lengths.data() : nullptr;
+  return GZ_OK;
+}
+
+// ---- database ----
+static DatabaseOptions DbOptions(const gz_database_options *o) {
+  DatabaseOptions opt;
+  if (o) {
+    opt.clustering = o->clustering != 0;
+    if (o->subsequence_length) opt.subsequence_length = o->subsequence_length;
+    if (o->seed_threshold) opt.seed_threshold = o->seed_threshold;
+    if (o->chunk_size) opt.chunk_size = o->chunk_size;
+    opt.threads = o->threads;
+  }
+  return opt;
+}
+
+gz_status gz_database_create(const char *const *names, const char *residues,
+                             const uint64_t *offsets, uint64_t n,
+                             const gz_database_options *o, gz_database_t *out) {
+  GZ_REQUIRE(out && (n == 0 || (residues && offsets)), "gz_database_create: null argument");
+  std::vector<int> m = Blosum();
+  gz_database_s *h = new gz_database_s();
+  BuildDatabase(Records(names, residues, offsets, n), DbOptions(o), m.data(), kNumberLetters,
+                &h->db);
+  FinishDatabase(h);
+  *out = h;
+  return GZ_OK;
+}
+
+gz_status gz_database_read_fasta(const char *path, const gz_database_options *o,
+                                 gz_database_t *out) {
+  GZ_REQUIRE(path && out, "gz_database_read_fasta: null argument");
+  std::vector<FastaRecord> recs;
+  if (!ReadAll(path, &recs)) {
+    SetError(std::string("cannot read ") + path);
+    return GZ_ERROR_IO;
+  }
+  std::vector<int> m = Blosum();
+  gz_database_s *h = new gz_database_s();
+  BuildDatabase(recs, DbOptions(o), m.data(), kNumberLetters, &h->db);
+  FinishDatabase(h);
+  *out = h;
+  return GZ_OK;
+}
+
+gz_status gz_database_destroy(gz_database_t h) {
+  delete h;
+  return GZ_OK;
+}
+
+gz_status gz_database_info(gz_database_t h, gz_database_info_t *info) {
+  GZ_REQUIRE(h && info, "gz_database_info: null argument");
+  const Database &d = h->db;
+  info->number_chunks = (uint32_t)d.chunks.size();
+  info->total_length = d.total_length;
+  info->total_sequences = d.total_sequences;
+  info->max_length = d.max_length;
+  info->subsequence_length = d.subsequence_length;
+  memcpy(info->reduced_code_map, d.reduced_map, 25);
+  memcpy(info->hash_code_map, d.hash.code_map, 25);
+  info->hash_max_code = d.hash.max_code;
+  info->hash_score_threshold = d.hash.threshold;
+  info->hash_max_length = d.hash.max_length;
+  info->hash_shift = d.hash.shift;
+  for (size_t i = 0; i < 16; ++i)
+    info->hash_code_scores[i] = i < d.hash.scores.size() ? d.hash.scores[i] : 0;
+  info->max_mismatches = d.max_mismatches;
+  return GZ_OK;
+}
+
+gz_status gz_database_chunk(gz_database_t h, uint32_t i, gz_database_chunk_view_t *v) {
+  GZ_REQUIRE(h && v && i < h->db.chunks.size(), "gz_database_chunk: bad argument");
+  const DbChunk &c = h->db.chunks[i];
+  v->sequence = c.seq.data();
+  v->sequence_length = (uint32_t)c.seq.size();
+  v->number_sequences = c.NumberSequences();
+  v->offsets = c.offsets.data();
+  v->names = h->name_ptrs[i].data();
+  v->number_hashes = (uint32_t)c.hashes.size();
+  v->hashes = c.hashes.data();
+  v->rep_offsets = c.rep_off.data();
+  v->rep_positions = c.rep_pos.data();
+  v->rep_member_offsets = c.mem_off.data();
+  v->member_positions = c.mem_pos.data();
+  v->value_offsets = c.val_off.data();
+  v->value_positions = c.val_pos.data();
+  v->number_reps = c.rep_pos.size();
+  v->number_members = c.mem_pos.size();
+  v->number_values = c.val_pos.size();
+  v->clustering = c.clustering ? 1 : 0;
+  return GZ_OK;
+}
+
+// ---- aligner ----
+void gz_aligner_default_parameters(gz_aligner_parameters *p) {
+  memset(p, 0, sizeof(*p));
+  p->gap_open = 11;
+  p->gap_extension = 1;
+  p->normalized_ungapped_cutoff = 7.0f;
+  p->normalized_gapped_trigger = 22.0f;
+  p->normalized_presearch_cutoff = 15.0f;
+  p->normalized_result_cutoff = 25.0f;
+  p->max_results = 10;
+  p->max_results_per_subject = 1;
+  p->queries_chunk_size = 1ull << 27;
+  p->threads = 0;
+  p->schedule = GZ_SCHEDULE_PLAIN;
+  p->record_traces = 0;
+}
+
+gz_status gz_aligner_create(const gz_aligner_parameters *p, gz_aligner_t *out) {
+  GZ_REQUIRE(out, "gz_aligner_create: null argument");
+  gz_aligner_parameters d;
+  gz_aligner_default_parameters(&d);
+  if (!p) p = &d;
+  gz_aligner_s *h = new gz_aligner_s();
+  h->matrix = Blosum();
+  AlignerOptions &o = h->opt;
+  o.search.matrix = h->matrix.data();
+  o.search.nl = kNumberLetters;
+  o.search.gap_open = p->gap_open;
+  o.search.gap_extension = p->gap_extension;
+  o.search.normalized_ungapped_cutoff = p->normalized_ungapped_cutoff;
+  o.search.normalized_gapped_trigger = p->normalized_gapped_trigger;
+  o.search.normalized_presearch_cutoff = p->normalized_presearch_cutoff;
+  o.search.normalized_result_cutoff = p->normalized_result_cutoff;
+  o.search.max_results = p->max_results;
+  o.search.max_results_per_subject = p->max_results_per_subject;
+  o.queries_chunk_size = p->queries_chunk_size ? p->queries_chunk_size : (1ull << 27);
+  o.threads = p->threads;
+  o.schedule = (Schedule)p->schedule;
+  o.record_cells = p->record_traces != 0;
+  h->aligner.reset(new Aligner(o));
+  *out = h;
+  return GZ_OK;
+}
+
+gz_status gz_aligner_destroy(gz_aligner_t h) {
+  delete h;
+  return GZ_OK;
+}
+
+gz_status gz_aligner_set_database(gz_aligner_t h, gz_database_t db) {
+  GZ_REQUIRE(h && db, "gz_aligner_set_database: null argument");
+  GZ_HIP_CHECK(h->aligner->SetDatabase(&db->db));
+  return GZ_OK;
+}
+
+gz_status gz_aligner_set_cost_model(gz_aligner_t h, const float *weights, size_t count) {
+  GZ_REQUIRE(h && weights, "gz_aligner_set_cost_model: null argument");
+  CostModelWeights w;
+  if (!UnflattenWeights(weights, count, &w)) {
+    SetError("gz_aligner_set_cost_model: wrong weight count");
+    return GZ_ERROR_INVALID_ARGUMENT;
+  }
+  h->model.reset(new GpuCostModel(w));
+  h->opt.cost_model = h->model.get();
+  h->aligner.reset(new Aligner(h->opt));
+  return GZ_OK;
+}
+
+gz_status gz_aligner_align(gz_aligner_t h, gz_queries_t q, char **output,
+                           size_t *output_length, gz_align_stats *stats) {
+  GZ_REQUIRE(h && q && output && output_length, "gz_aligner_align: null argument");
+  std::string out;
+  AlignStats st;
+  GZ_HIP_CHECK(h->aligner->Align(q->q, &out, &st));
+  char *buf = (char *)malloc(out.size() + 1);
+  if (!buf) return GZ_ERROR_OUT_OF_MEMORY;
+  memcpy(buf, out.data(), out.size());
+  buf[out.size()] = 0;
+  *output = buf;
+  *output_length = out.size();
+  if (stats) {
+    stats->seconds_total = st.seconds_total;
+    stats->seconds_query_lists = st.seconds_query_lists;
+    stats->seconds_upload = st.seconds_upload;
+    stats->seconds_seed = st.seconds_seed;
+    stats->seconds_chain = st.seconds_chain;
+    stats->seconds_gapped = st.seconds_gapped;
+    stats->seconds_add = st.seconds_add;
+    stats->seconds_build = st.seconds_build;
+    stats->seconds_output = st.seconds_output;
+    stats->query_chunks = st.query_chunks;
+    stats->lists = st.lists;
+    stats->units_rep = st.units_rep;
+    stats->units_value = st.units_value;
+    stats->seed_hits = st.seed_hits;
+    stats->chained_hits = st.chained_hits;
+    stats->gapped_extensions = st.gapped_extensions;
+    stats->gapped_cells = st.gapped_cells;
+    stats->results = st.results;
+    stats->sequential_build_chunks = st.sequential_build_chunks;
+    stats->gapped_kernel_ms = st.gapped_kernel_ms;
+  }
+  return GZ_OK;
+}
+
+uint64_t gz_aligner_traces(gz_aligner_t h, gz_gapped_trace *out, uint64_t capacity) {
+  if (!h) return 0;
+  const std::vector<GappedTrace> &t = h->aligner->Traces();
+  if (out) {
+    for (uint64_t i = 0; i < t.size() && i < capacity; ++i) {
+      out[i].query_position = t[i].qpos;
+      out[i].database_position = t[i].dpos;
+      out[i].chunk = t[i].chunk;
+      out[i].reverse = t[i].reverse;
+      out[i].cells = t[i].cells;
+    }
+  }
+  return t.size();
+}
+
+gz_status gz_aligner_last_concatenated_queries(gz_aligner_t h, const uint8_t **codes,
+                                               uint64_t *length) {
+  GZ_REQUIRE(h && codes && length, "null argument");
+  *codes = h->aligner->LastConcatenatedQueries().data();
+  *length = h->aligner->LastConcatenatedQueries().size();
+  return GZ_OK;
+}
+
+gz_status gz_align_fasta(const char *queries_path, const char *database_path,
+                         const char *output_path, int translated,
+                         const gz_aligner_parameters *p,
+                         const gz_database_options *dbo) {
+  GZ_REQUIRE(queries_path && database_path && output_path, "gz_align_fasta: null argument");
+  gz_query_options qo = {translated, 1, 0};
+  gz_queries_t q = nullptr;
+  gz_database_t db = nullptr;
+  gz_aligner_t a = nullptr;
+  gz_status st = gz_database_read_fasta(database_path, dbo, &db);
+  if (st == GZ_OK) st = gz_queries_read_fasta(queries_path, &qo, &q);
+  if (st == GZ_OK) st = gz_aligner_create(p, &a);
+  if (st == GZ_OK) st = gz_aligner_set_database(a, db);
+  char *out = nullptr;
+  size_t len = 0;
+  if (st == GZ_OK) st = gz_aligner_align(a, q, &out, &len, nullptr);
+  if (st == GZ_OK) {
+    std::ofstream os(output_path, std::ios::binary);
+    if (!os) {
+      SetError(std::string("cannot write ") + output_path);
+      st = GZ_ERROR_IO;
+    } else {
+      os.write(out, (std::streamsize)len);
+    }
+  }
+  free(out);
+  gz_aligner_destroy(a);
+  gz_queries_destroy(q);
+  gz_database_destroy(db);
+  return st;
+}
+
+void gz_free(void *p) { free(p); }
+
+// ---- host-side preparation pieces exposed for tests ----
+int gz_seg_mask(const char *seq, char *out) {
+  static thread_local Seg seg(12);
+  std::string masked;
+  const int r = seg.Mask(seq, &masked);
+  if (r == 0) memcpy(out, masked.data(), masked.size());
+  return r;
+}
+
+double gz_seg_ln_factorial(int n) { return SegLnFactorial(n); }
+
+int gz_translate(const char *dna, char *out, uint32_t *lengths) {
+  std::vector<std::string> f;
+  TranslateSixFrames(dna, f);
+  size_t o = 0;
+  for (size_t i = 0; i < 6; ++i) {
+    if (i < f.size()) {
+      memcpy(out + o, f[i].data(), f[i].size());
+      lengths[i] = (uint32_t)f[i].size();
+      o += f[i].size();
+    } else {
+      lengths[i] = 0;
+    }
+  }
+  return (int)f.size();
+}
+
+void gz_karlin_block(const double *p_at_min, int min, int max, float *lambda, float *K,
+                     float *H) {
+  KarlinBlockCalc(p_at_min, min, max, lambda, K, H);
+}
+
+int gz_query_karlin(const uint8_t *seq, uint32_t len, float *lambda, float *K, float *H) {
+  std::vector<int> m = Blosum();
+  Statistics s;
+  KarlinParameters p;
+  const bool ok = s.QueryUngapped(seq, len, m.data(), kNumberLetters, &p);
+  *lambda = p.lambda;
+  *K = p.K;
+  *H = p.H;
+  return ok ? 1 : 0;
+}
+
+uint64_t gz_search_space(uint64_t qlen, uint64_t dblen, uint64_t dbn) {
+  return Statistics::SearchSpace(qlen, dblen, dbn, Statistics::GappedBlosum62(), 1.9, -30);
+}
+
+}  // extern "C"

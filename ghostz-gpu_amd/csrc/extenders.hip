@@ -23,6 +23,7 @@
 
 #include "gz_common.h"
 #include "gz_device.h"
+#include "gz_runtime.h"
 #include "gz_scan.h"
 
 namespace gz {
@@ -402,10 +403,6 @@ inline unsigned GridFor(size_t n, unsigned block, unsigned cap = 65535u * 4) {
 // C ABI
 // ---------------------------------------------------------------------------
 using namespace gz;
-
-struct gz_stream_s {
-  hipStream_t s;
-};
 
 static inline hipStream_t S(gz_stream_t st) { return st ? st->s : 0; }
 
@@ -822,6 +819,26 @@ gz_status gz_gapped_extender_extend_one_side_score_only_cells_async(
   GZ_REQUIRE(h, "null handle");
   if (size == 0) return GZ_OK;
   return GappedRun(h, size, reverse, d_q, d_d, d_scores, d_cells, S(stream));
+}
+
+gz_status gz_gapped_extender_set_max_sequence_length(gz_gapped_extender_t h,
+                                                     uint32_t length) {
+  GZ_REQUIRE(h, "null handle");
+  uint32_t w = 64;
+  while (w < length + 2) w <<= 1;
+  h->wide = w;
+  return GZ_OK;
+}
+
+gz_status gz_gapped_extender_failures(gz_gapped_extender_t h, uint32_t *failures) {
+  GZ_REQUIRE(h && failures, "null argument");
+  *failures = 0;
+  if (h->counters) {
+    uint32_t c[2];
+    GZ_HIP_CHECK(hipMemcpy(c, h->counters, sizeof(c), hipMemcpyDeviceToHost));
+    *failures = c[1];
+  }
+  return GZ_OK;
 }
 
 }  // extern "C"

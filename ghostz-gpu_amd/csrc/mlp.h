@@ -1,0 +1,61 @@
+// mlp.h -- the cost-model MLP on the device: embedding 21x8 per residue,
+// 1024 -> 256 -> 64 -> 1 with GELU, bf16 operands, fp32 accumulation (MFMA),
+// fp32 master weights and Adam for training.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "cost_model.h"
+
+namespace gz {
+
+class MlpDevice {
+ public:
+  MlpDevice();
+  ~MlpDevice();
+  void SetWeights(const CostModelWeights &w);
+  void GetWeights(CostModelWeights *w);
+  // Predicted log2 cells for n seeds given their 128 feature codes each.
+  hipError_t Forward(const uint8_t *codes, size_t n, float *out);
+  // One training step on a batch: mean squared error against targets;
+  // computes gradients, applies Adam (lr) unless lr == 0, and optionally
+  // returns the flat gradient (same layout as the flat weights).
+  hipError_t TrainStep(const uint8_t *codes, const float *targets, size_t n, float lr,
+                       float *loss, float *flat_grad);
+  // Kernel-only forward time of the last Forward call (HIP events).
+  float last_forward_ms = 0;
+
+ private:
+  hipError_t Reserve(size_t batch);
+  hipError_t ForwardDevice(size_t n, size_t padded);
+  void SyncBf16();
+
+  hipStream_t stream_ = nullptr;
+  size_t cap_ = 0;
+  // fp32 master weights + gradients + Adam moments, flat
+  float *master_ = nullptr, *grad_ = nullptr, *m_ = nullptr, *v_ = nullptr;
+  uint16_t *w1h_ = nullptr, *w2h_ = nullptr, *embh_ = nullptr;  // bf16 copies
+  size_t count_ = 0;
+  long step_ = 0;
+  // activations
+  uint8_t *codes_ = nullptr;
+  uint16_t *x_ = nullptr;       // [B][1024] bf16
+  float *z1_ = nullptr;         // [B][256]
+  uint16_t *h1_ = nullptr;      // [B][256] bf16
+  float *z2_ = nullptr;         // [B][64]
+  uint16_t *h2_ = nullptr;      // [B][64] bf16
+  float *y_ = nullptr, *t_ = nullptr, *loss_ = nullptr;
+  float *dz2_ = nullptr;        // [B][64]
+  uint16_t *dz2h_ = nullptr;
+  float *dz1_ = nullptr;        // [B][256]
+  uint16_t *dz1h_ = nullptr;
+  float *dh1_ = nullptr;        // [B][256]
+  float *dx_ = nullptr;         // [B][1024]
+};
+
+}  // namespace gz

@@ -1,0 +1,539 @@
+// results.cpp -- see results.h.
+#include "results.h"
+
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cstring>
+#include <sstream>
+#include <thread>
+
+#include "gz_common.h"
+
+namespace gz {
+
+// ---------------------------------------------------------------------------
+// chain filter
+// ---------------------------------------------------------------------------
+namespace {
+struct DbOrder {
+  bool operator()(const Hit &a, const Hit &b) const { return a.d < b.d; }
+};
+
+// ChainFilter::Connected (src/chain_filter.cpp:62-99): does the reverse
+// ungapped walk from the hit reach the chain's end before dropping?
+bool Connected(const uint8_t *q, const uint8_t *d, const int *sm, uint32_t nl,
+               uint32_t chain_end, const Hit &h, int cutoff) {
+  if (h.q <= chain_end + 1) return true;
+  uint32_t qo = h.q, doff = h.d;
+  int score = 0, best = 0, c = -cutoff;
+  do {
+    --doff;
+    --qo;
+    if (d[doff] == kDelimiter || q[qo] == kDelimiter || qo <= chain_end) break;
+    score += sm[d[doff] * nl + q[qo]];
+    if (best < score) {
+      best = score;
+      c = best - cutoff;
+    }
+  } while (score > c);
+  return qo <= chain_end;
+}
+}  // namespace
+
+void ChainFilter(const uint8_t *q, uint32_t qlen, int cutoff, const uint8_t *d,
+                 const int *sm, uint32_t nl, std::vector<Hit> &hits,
+                 std::vector<uint64_t> &chains) {
+  // per diagonal slot: (query_end << 32) | diagonal
+  chains.assign(qlen, 0);
+  std::sort(hits.begin(), hits.end(), DbOrder());
+  size_t n = 0;
+  for (size_t i = 0; i < hits.size(); ++i) {
+    const Hit h = hits[i];
+    const uint32_t diagonal = qlen - 1 + h.d - h.q;
+    const uint32_t slot = diagonal % qlen;
+    const uint32_t cdiag = (uint32_t)chains[slot];
+    const uint32_t cend = (uint32_t)(chains[slot] >> 32);
+    if (cdiag != diagonal || !Connected(q, d, sm, nl, cend, h, cutoff)) {
+      chains[slot] = ((uint64_t)h.q << 32) | diagonal;
+      hits[n++] = h;
+    } else {
+      chains[slot] = ((uint64_t)h.q << 32) | cdiag;
+    }
+  }
+  hits.resize(n);
+}
+
+// ---------------------------------------------------------------------------
+// result merge
+// ---------------------------------------------------------------------------
+namespace {
+// AlignerCommon::AlignmentComp (src/aligner_common.cpp:12-56)
+int Compare(int as, const Coordinate &a0, const Coordinate &a1, int bs,
+            const Coordinate &b0, const Coordinate &b1) {
+  if (as != bs) return as > bs ? 1 : -1;
+  const uint32_t aq = a1.q - a0.q, bq = b1.q - b0.q;
+  if (aq != bq) return aq > bq ? 1 : -1;
+  const uint32_t ad = a1.d - a0.d, bd = b1.d - b0.d;
+  if (ad != bd) return ad > bd ? 1 : -1;
+  return a0.d < b0.d ? 1 : -1;
+}
+bool Contains(const Coordinate &a0, const Coordinate &a1, const Coordinate &b0,
+              const Coordinate &b1) {
+  return (a0.q == b0.q && a0.d == b0.d) || (a1.q == b1.q && a1.d == b1.d) ||
+         (a0.q <= b0.q && b1.q <= a1.q && a0.d <= b0.d && b1.d <= a1.d);
+}
+struct ByScore {
+  bool operator()(const Presearched &a, const Presearched &b) const {
+    return Compare(a.score, a.start, a.end, b.score, b.start, b.end) > 0;
+  }
+};
+struct ResultByScore {
+  bool operator()(const AlignmentResult &a, const AlignmentResult &b) const {
+    return Compare(a.score, a.start, a.end, b.score, b.start, b.end) > 0;
+  }
+};
+struct PositionLess {
+  bool operator()(const AlignmentPosition &a, const AlignmentPosition &b) const {
+    return a.position < b.position;
+  }
+};
+}  // namespace
+
+void AddResults(const DbChunk &chunk, int chunk_id, const SearchParameters &p,
+                std::vector<std::vector<AlignmentPosition> > &starts,
+                std::vector<Presearched> &added, std::vector<Presearched> &results) {
+  std::vector<Presearched> merged;
+  if (starts.size() < chunk.NumberSequences()) starts.resize(chunk.NumberSequences());
+  std::stable_sort(added.begin(), added.end(), ByScore());
+  size_t ri = 0, ai = 0;
+  while (merged.size() < p.max_results && (ai < added.size() || ri < results.size())) {
+    Presearched a;
+    if (ai == added.size()) {
+      a = results[ri++];
+    } else if (ri == results.size() ||
+               Compare(results[ri].score, results[ri].start, results[ri].end,
+                       added[ai].score, added[ai].start, added[ai].end) < 0) {
+      a = added[ai++];
+      a.chunk = chunk_id;
+      a.subject = chunk.SubjectOf(a.start.d);
+    } else {
+      a = results[ri++];
+    }
+    if (a.chunk == chunk_id) {
+      std::vector<AlignmentPosition> &sp = starts[a.subject];
+      if (sp.size() < p.max_results_per_subject) {
+        AlignmentPosition key = {0, a.start.d};
+        auto it = std::lower_bound(sp.begin(), sp.end(), key, PositionLess());
+        const auto insert_at = it;
+        key.position = a.end.d;
+        auto ie = std::lower_bound(it, sp.end(), key, PositionLess());
+        bool keep = true;
+        for (; it != ie; ++it) {
+          const Presearched &r = merged[it->result_id];
+          if (Contains(r.start, r.end, a.start, a.end)) {
+            keep = false;
+            break;
+          }
+        }
+        if (keep) {
+          AlignmentPosition ap = {(uint32_t)merged.size(), a.start.q};
+          sp.insert(insert_at, ap);
+          merged.push_back(a);
+        }
+      }
+    } else {
+      merged.push_back(a);
+    }
+  }
+  results.swap(merged);
+  for (size_t i = 0; i < results.size(); ++i) starts[results[i].subject].clear();
+}
+
+// ---------------------------------------------------------------------------
+// traceback re-extension
+// ---------------------------------------------------------------------------
+namespace {
+
+enum : uint8_t { kSub = 0, kGap0 = 1, kGap1 = 2, kExt0 = 4, kExt1 = 8 };
+enum : uint8_t { kEditGap0 = 0, kEditSub = 1, kEditGap1 = 2 };  // EditBlocks order
+
+struct Blocks {
+  std::vector<std::pair<uint8_t, int> > b;
+  void Add(uint8_t op, int n) {
+    if (n == 0) return;
+    if (!b.empty() && b.back().first == op) b.back().second += n;
+    else b.push_back(std::make_pair(op, n));
+  }
+};
+
+// GappedExtender with traceback (src/gapped_extender.cpp:70-304,473-531).
+// The op store mirrors gap_states_ (:25-68): rows are laid out back to back
+// in a byte arena, and the traceback may read bytes outside a row's band;
+// such bytes come from neighbouring rows of the same extension.  A read of
+// a byte that the current extension did not write would depend on earlier
+// extensions run through the same object; those reads are counted so the
+// caller can replay the chunk in the reference's order.
+class Traceback {
+ public:
+  uint64_t stale = 0;
+
+  void Extend(const uint8_t *s0, uint32_t s0len, const uint8_t *s1, bool reversed,
+              const int *sm, uint32_t nl, int gap_open, int gap_ext, int cutoff,
+              int *best_score, int *best_p0, int *best_p1, Blocks *eb) {
+    *best_score = 0;
+    const int inc = reversed ? -1 : 1;
+    *best_p0 = -inc;
+    *best_p1 = -inc;
+    eb->b.clear();
+    if (s0[0] == kDelimiter || s1[0] == kDelimiter) return;
+    uint32_t n0 = 0;
+    while (s0[(int)n0 * inc] != kDelimiter) ++n0;
+    if (dp_.size() < std::max<size_t>(s0len + 1, n0 + 2)) dp_.resize(std::max<size_t>(s0len + 1, n0 + 2));
+    rows_arr_.clear();
+    rows_base_.clear();
+    rows_off_.clear();
+    ++serial_;
+    for (auto &a : arenas_) a.used = 0;
+    const uint32_t extra = (gap_ext > 0 && cutoff != INT_MAX) ? cutoff / gap_ext + 3 : s0len + 3;
+    uint32_t arr = Pick(extra);
+    Put(arr, 0, kSub);
+
+    const int gi = gap_open + gap_ext;
+    int as = 0, ae = 0, ai = 0, last = 0;
+    int score = -gi;
+    dp_[0] = {0, -gi};
+    int p0 = 0;
+    for (ai = 1; s0[p0] != kDelimiter; ++ai, p0 += inc) {
+      if (score < -cutoff) break;
+      dp_[ai] = {score, score - gi};
+      score -= gap_ext;
+    }
+    if (s0[p0] != kDelimiter) {
+      dp_[ai] = {kInitScore, kInitScore};
+      ae = ai + 1;
+    } else {
+      ae = ai;
+    }
+    int max_score = 0, mp0 = -inc, mp1 = -inc;
+    for (int p1 = 0; s1[p1] != kDelimiter; p1 += inc) {
+      arr = (gap_ext > 0) ? Pick((uint32_t)(ae - as) + extra) : Pick(extra - (uint32_t)as);
+      const int64_t base = (int64_t)arenas_[arr].used + 1;
+      const int off = as;
+      rows_arr_.push_back(arr);
+      rows_base_.push_back(base);
+      rows_off_.push_back(off);
+      const int64_t zero = base - off;
+      const int *row = sm + s1[p1] * nl;
+      p0 = as * inc;
+      int prev = dp_[as].best;
+      dp_[as].best = (max_score - dp_[as].gap > cutoff) ? kInitScore : dp_[as].gap;
+      int gap_row = dp_[as].best - gi;
+      dp_[as].gap -= gap_ext;
+      Put(arr, zero + as, kGap0 + kExt0);
+      last = as;
+      for (ai = as + 1; ai < ae; ++ai, p0 += inc) {
+        int s = prev + row[s0[p0]];
+        prev = dp_[ai].best;
+        int gap_col = dp_[ai].gap;
+        uint8_t op = kSub;
+        if (s < gap_col) {
+          op = kGap0;
+          s = gap_col;
+        }
+        if (s < gap_row) {
+          op = kGap1;
+          s = gap_row;
+        }
+        if (max_score - s > cutoff) {
+          if (as + 1 == ai) ++as;
+          dp_[ai] = {kInitScore, kInitScore};
+          gap_row = kInitScore;
+        } else {
+          last = ai;
+          if (s > max_score) {
+            max_score = s;
+            mp0 = p0;
+            mp1 = p1;
+          }
+          gap_col -= gap_ext;
+          gap_row -= gap_ext;
+          if (gap_col < s - gi) {
+            dp_[ai].gap = s - gi;
+          } else {
+            dp_[ai].gap = gap_col;
+            op += kExt0;
+          }
+          if (gap_row < s - gi) gap_row = s - gi; else op += kExt1;
+          Put(arr, zero + ai, op);
+          dp_[ai].best = s;
+        }
+      }
+      if (as + 1 == ae) break;
+      if (last < ae - 1) {
+        ae = last + 1;
+      } else {
+        while (gap_row >= (max_score - cutoff) && s0[p0] != kDelimiter) {
+          dp_[ae] = {gap_row, gap_row - gi};
+          Put(arr, zero + ae, kGap0 + kExt0);
+          gap_row -= gap_ext;
+          ++ae;
+          p0 += inc;
+        }
+      }
+      if (s0[p0] != kDelimiter) {
+        dp_[ae] = {kInitScore, kInitScore};
+        ++ae;
+      }
+      arenas_[arr].used += (uint32_t)(ae - off);
+    }
+    *best_score = max_score;
+    *best_p0 = mp0;
+    *best_p1 = mp1;
+    // PickOptimalPath
+    int q = mp0, d = mp1;
+    uint8_t op = kSub;
+    while (d * inc >= 0 && q * inc >= 0) {
+      const int r = d * inc;
+      const uint8_t next = Get(rows_arr_[r], rows_base_[r] + (q * inc) - rows_off_[r] + 1);
+      if (op == kGap0) {
+        op = next & 3;
+        if (next & kExt0) op = kGap0;
+      } else if (op == kGap1) {
+        op = next & 3;
+        if (next & kExt1) op = kGap1;
+      } else {
+        op = next & 3;
+      }
+      if (op == kGap0) {
+        eb->Add(kEditGap0, 1);
+        d -= inc;
+      } else if (op == kGap1) {
+        eb->Add(kEditGap1, 1);
+        q -= inc;
+      } else {
+        eb->Add(kEditSub, 1);
+        d -= inc;
+        q -= inc;
+      }
+    }
+    if (d * inc >= 0) eb->Add(kEditGap0, d * inc + 1);
+    else if (q * inc >= 0) eb->Add(kEditGap1, q * inc + 1);
+    if (inc > 0) std::reverse(eb->b.begin(), eb->b.end());
+  }
+
+ private:
+  struct Cell {
+    int best;
+    int gap;
+  };
+  struct Arena {
+    std::vector<uint8_t> a;
+    std::vector<uint32_t> stamp;
+    uint32_t used = 0;
+  };
+  static constexpr uint32_t kArena = 1u << 21;
+
+  uint32_t Pick(uint32_t length) {  // GetGapStateIndex
+    const uint32_t l = length + length / 3;
+    const uint32_t alen = std::max(kArena, l);
+    if (arenas_.empty()) {
+      arenas_.emplace_back();
+      Resize(arenas_[0], alen);
+      return 0;
+    }
+    for (uint32_t i = 0; i < arenas_.size(); ++i) {
+      if (l < arenas_[i].a.size() - arenas_[i].used) return i;
+      if (arenas_[i].used == 0) {
+        Resize(arenas_[i], alen);
+        return i;
+      }
+    }
+    arenas_.emplace_back();
+    Resize(arenas_.back(), alen);
+    return (uint32_t)arenas_.size() - 1;
+  }
+  static void Resize(Arena &a, uint32_t n) {
+    a.a.resize(n);
+    a.stamp.resize(n, 0);
+  }
+  void Put(uint32_t arr, int64_t idx, uint8_t v) {
+    arenas_[arr].a[idx] = v;
+    arenas_[arr].stamp[idx] = serial_;
+  }
+  uint8_t Get(uint32_t arr, int64_t idx) {
+    Arena &a = arenas_[arr];
+    if (idx < 0 || idx >= (int64_t)a.a.size()) {
+      ++stale;
+      return 0;
+    }
+    if (a.stamp[idx] != serial_) ++stale;
+    return a.a[idx];
+  }
+
+  std::vector<Cell> dp_;
+  std::vector<Arena> arenas_;
+  std::vector<uint32_t> rows_arr_;
+  std::vector<int64_t> rows_base_;
+  std::vector<int> rows_off_;
+  uint32_t serial_ = 0;
+};
+
+// BuildResults for one presearched result (aligner_build_results_thread.h
+// :97-150 + AlignerCommon::BuildResult, src/aligner_common.h:276-343).
+void BuildOne(Traceback &tb, const QuerySet &q, uint32_t qi, const DbChunk &c,
+              const Presearched &pr, const SearchParameters &p, int cutoff,
+              AlignmentResult *out) {
+  const uint8_t *qs = q.Sequence(qi);
+  const uint32_t qlen = q.Length(qi);
+  const uint8_t *ds = c.seq.data();
+  Blocks rev, fwd;
+  int s1, s2, p0, p1;
+  const Coordinate hit = pr.hit;
+  tb.Extend(qs + hit.q - 1, hit.q - 1, ds + hit.d - 1, true, p.matrix, p.nl, p.gap_open,
+            p.gap_extension, cutoff, &s1, &p0, &p1, &rev);
+  Coordinate start = {hit.q - 1 + p0, hit.d - 1 + p1};
+  std::reverse(rev.b.begin(), rev.b.end());
+  tb.Extend(qs + hit.q, qlen - hit.q, ds + hit.d, false, p.matrix, p.nl, p.gap_open,
+            p.gap_extension, cutoff, &s2, &p0, &p1, &fwd);
+  Coordinate end = {hit.q + (uint32_t)p0, hit.d + (uint32_t)p1};
+  Blocks all;
+  for (auto &b : rev.b) all.Add(b.first, b.second);
+  for (auto &b : fwd.b) all.Add(b.first, b.second);
+  uint32_t qp = start.q, dp = start.d, m = 0, o = 0, e = 0, len = 0;
+  uint8_t prev = kEditSub;
+  for (auto &b : all.b) {
+    for (int k = 0; k < b.second; ++k) {
+      const uint8_t op = b.first;
+      if (op == kEditSub) {
+        if (qs[qp] != ds[dp]) ++m;
+        ++qp;
+        ++dp;
+      } else if (op == kEditGap0) {
+        if (prev != kEditGap0) ++o; else ++e;
+        ++dp;
+      } else {
+        if (prev != kEditGap1) ++o; else ++e;
+        ++qp;
+      }
+      prev = op;
+      ++len;
+    }
+  }
+  out->chunk = pr.chunk;
+  out->subject = pr.subject;
+  out->score = s1 + s2;
+  out->length = len;
+  out->mismatches = m;
+  out->gap_openings = o;
+  out->gap_extensions = e;
+  out->hit = hit;
+  out->start = start;
+  out->end = end;
+}
+
+}  // namespace
+
+void BuildResults(const QuerySet &q, const Database &db, const SearchParameters &p,
+                  int cutoff, std::vector<std::vector<Presearched> > &pre,
+                  std::vector<std::vector<AlignmentResult> > &results, int threads,
+                  uint64_t *sequential_chunks) {
+  const size_t nq = q.size();
+  results.assign(nq, std::vector<AlignmentResult>());
+  for (size_t i = 0; i < nq; ++i) results[i].resize(pre[i].size());
+  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  threads = std::max(1, threads);
+  for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
+    std::vector<std::pair<uint32_t, uint32_t> > ids;
+    for (uint32_t qi = 0; qi < nq; ++qi)
+      for (uint32_t r = 0; r < pre[qi].size(); ++r)
+        if (pre[qi][r].chunk == (int)ci) ids.push_back(std::make_pair(qi, r));
+    if (ids.empty()) continue;
+    const DbChunk &c = db.chunks[ci];
+    std::atomic<size_t> next(0);
+    std::atomic<uint64_t> stale(0);
+    const int T = std::max(1, std::min<int>(threads, (int)(ids.size() / 64) + 1));
+    auto work = [&]() {
+      Traceback tb;
+      for (;;) {
+        const size_t b = next.fetch_add(256);
+        if (b >= ids.size()) break;
+        for (size_t k = b; k < std::min(ids.size(), b + 256); ++k) {
+          const uint32_t qi = ids[k].first, r = ids[k].second;
+          BuildOne(tb, q, qi, c, pre[qi][r], p, cutoff, &results[qi][r]);
+        }
+      }
+      stale += tb.stale;
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    if (stale.load() != 0) {
+      // history-dependent bytes were read: replay the chunk exactly as the
+      // reference's one thread does -- one extender object, pairs handed out
+      // in blocks of 32 and taken from the back of each block
+      if (sequential_chunks) ++*sequential_chunks;
+      Traceback tb;
+      for (size_t b = 0; b < ids.size(); b += 32) {
+        const size_t e = std::min(ids.size(), b + 32);
+        for (size_t k = e; k-- > b;) {
+          const uint32_t qi = ids[k].first, r = ids[k].second;
+          BuildOne(tb, q, qi, c, pre[qi][r], p, cutoff, &results[qi][r]);
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < nq; ++i)
+    std::sort(results[i].begin(), results[i].end(), ResultByScore());
+}
+
+// ---------------------------------------------------------------------------
+// output
+// ---------------------------------------------------------------------------
+void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,
+                 const std::vector<std::vector<AlignmentResult> > &results) {
+  const KarlinParameters gk = Statistics::GappedBlosum62();
+  const double alpha = 1.9, beta = -30;
+  const size_t nq = q.size();
+  const size_t block = 2048;
+  const size_t nblocks = (nq + block - 1) / block;
+  std::vector<std::string> text(nblocks);
+  std::atomic<size_t> next(0);
+  auto work = [&]() {
+    for (;;) {
+      const size_t b = next.fetch_add(1);
+      if (b >= nblocks) break;
+      std::ostringstream s;
+      for (size_t qi = b * block; qi < std::min(nq, (b + 1) * block); ++qi) {
+        if (results[qi].empty()) continue;
+        const uint64_t space = Statistics::SearchSpace(q.RealLength(qi), db.total_length,
+                                                       db.total_sequences, gk, alpha, beta);
+        for (const AlignmentResult &r : results[qi]) {
+          const DbChunk &c = db.chunks[r.chunk];
+          const uint32_t base = c.offsets[r.subject] - 1;
+          s << q.names[qi] << "\t" << c.names[r.subject] << "\t";
+          s << (1.0 - (static_cast<float>(r.mismatches + r.gap_openings + r.gap_extensions) /
+                       static_cast<float>(r.length))) *
+                   100
+            << "\t";
+          s << r.length << "\t" << r.mismatches << "\t" << r.gap_openings << "\t";
+          s << q.RealStart(qi, r.start.q) << "\t" << q.RealEnd(qi, r.end.q) << "\t";
+          s << r.start.d - base << "\t" << r.end.d - base << "\t";
+          s << Statistics::Nominal2EValue(r.score, space, gk) << "\t";
+          s << Statistics::Nominal2Normalized(r.score, gk) << '\n';
+        }
+      }
+      text[b] = s.str();
+    }
+  };
+  const int T = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(),
+                                          (int)nblocks));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  for (auto &t : text) os << t;
+}
+
+}  // namespace gz

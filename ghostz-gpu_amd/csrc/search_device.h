@@ -1,0 +1,91 @@
+// search_device.h -- the device-resident seed stage of the presearch.
+//
+// Replaces the host enumeration of SeedSearcherGpu
+// (src/seed_searcher_gpu.cpp:129-534), the DistanceCalculationSeedList /
+// UngappedExtensionWithTriggerSeedList batches and the host flag compaction
+// (src/aligner_gpu_presearch_thread.cpp:822-896) with device kernels:
+//   lookup:   query hash list -> database representatives / values
+//   units:    (list, representative, entry) and (list, value, entry) work
+//             units, prefix-summed and mapped to lanes by load-balanced search
+//   seeds:    per unit: reduced-alphabet distance, ungapped X-drop with
+//             trigger for the representative, its members, or the value
+//   append:   wave-aggregated (ballot/popcount) append of passing seeds
+// Each passing seed carries the key (list rank, representative or value
+// index, member index + 1 or 0, entry index) that orders it exactly as the
+// one-thread CPU search enumerates it.
+#pragma once
+
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace gz {
+
+struct SeedHit {
+  uint32_t qid;
+  uint32_t qpos;   // concatenated query position
+  uint32_t dpos;   // database chunk position
+  uint32_t list;   // list rank
+  uint32_t rv;     // representative index, or number of reps + value index
+  uint32_t member; // 0 for a representative / value, 1 + member index
+  uint32_t entry;  // entry index within the list
+};
+
+struct DeviceDbChunk {
+  const uint64_t *packed = nullptr;
+  uint32_t nhash = 0;
+  const uint32_t *hashes = nullptr, *rep_off = nullptr, *rep_pos = nullptr,
+                 *mem_off = nullptr, *mem_pos = nullptr, *val_off = nullptr,
+                 *val_pos = nullptr;
+};
+
+struct DeviceQueryLists {
+  const uint64_t *packed = nullptr;  // concatenated queries
+  const int *cutoffs = nullptr;
+  const int *triggers = nullptr;
+  uint32_t nlists = 0;
+  const uint32_t *hash = nullptr, *entry_off = nullptr, *entry_pos = nullptr,
+                 *entry_qid = nullptr;
+  const uint8_t *entry_check = nullptr;
+};
+
+struct SeedStageParams {
+  const int *matrix = nullptr;  // device nl x nl
+  uint32_t nl = 25;
+  uint32_t delimiter = 25;
+  const uint8_t *reduced_map = nullptr;  // device, 25 entries
+  uint32_t subsequence_length = 10;
+  int distance_threshold = 2;  // rep extension when d <= threshold
+  int max_mismatches = 1;      // members when max(d - mm, 0) <= threshold
+  bool clustering = true;
+};
+
+struct SeedStageStats {
+  uint64_t units_rep = 0;      // (list, rep, entry) units
+  uint64_t units_value = 0;    // (list, value, entry) units
+  uint64_t distance_pairs = 0;
+  uint64_t hits = 0;
+};
+
+// Workspace owned by the caller (grown on demand).
+class SeedStage {
+ public:
+  ~SeedStage();
+  // Runs the seed stage of one query chunk against one database chunk and
+  // returns the passing seeds (host vector, unordered).
+  hipError_t Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
+                 const SeedStageParams &p, hipStream_t stream,
+                 std::vector<SeedHit> *hits, SeedStageStats *stats);
+
+ private:
+  hipError_t Reserve(void **p, size_t *cap, size_t bytes);
+  void *lookup_ = nullptr; size_t lookup_cap_ = 0;
+  void *units_ = nullptr; size_t units_cap_ = 0;
+  void *scan_tmp_ = nullptr; size_t scan_tmp_cap_ = 0;
+  void *active_ = nullptr; size_t active_cap_ = 0;
+  void *hits_ = nullptr; size_t hits_cap_ = 0;
+  uint32_t *counters_ = nullptr;
+  void *pinned_ = nullptr; size_t pinned_cap_ = 0;
+};
+
+}  // namespace gz

@@ -1,0 +1,409 @@
+// search_device.hip -- see search_device.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "gz_common.h"
+#include "gz_device.h"
+#include "gz_scan.h"
+#include "search_device.h"
+
+namespace gz {
+
+namespace {
+
+constexpr int kSeedThreads = 256;
+constexpr int kSeedItems = 4;
+constexpr uint32_t kSeedTile = kSeedThreads * kSeedItems;
+constexpr uint64_t kUnitsPerLaunch = 1ull << 28;
+constexpr size_t kInitialHitCapacity = 1u << 22;
+
+struct ListInfo {
+  uint32_t rep_begin, nrep, val_begin, nval;
+};
+
+// counters: [0] hits, [1] distance pairs, [2] ungapped extensions
+constexpr int kCounters = 4;
+
+__global__ void LookupKernel(DeviceQueryLists q, DeviceDbChunk db, int clustering,
+                             ListInfo *info, uint64_t *units_rep,
+                             uint64_t *units_val) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= q.nlists) return;
+  const uint32_t h = q.hash[l];
+  const uint64_t P = q.entry_off[l + 1] - q.entry_off[l];
+  uint32_t lo = 0, hi = db.nhash;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (db.hashes[mid] < h) lo = mid + 1; else hi = mid;
+  }
+  ListInfo li = {0, 0, 0, 0};
+  if (lo < db.nhash && db.hashes[lo] == h) {
+    if (clustering) {
+      li.rep_begin = db.rep_off[lo];
+      li.nrep = db.rep_off[lo + 1] - li.rep_begin;
+    }
+    li.val_begin = db.val_off[lo];
+    li.nval = db.val_off[lo + 1] - li.val_begin;
+  }
+  info[l] = li;
+  units_rep[l] = (uint64_t)li.nrep * P;
+  units_val[l] = (uint64_t)li.nval * P;
+}
+
+__global__ void ActiveFlagsKernel(uint32_t n, const uint64_t *units, uint32_t *flags) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < n) flags[l] = units[l] > 0 ? 1u : 0u;
+}
+
+__global__ void ActiveScatterKernel(uint32_t n, const uint64_t *units,
+                                    const uint64_t *offsets, const uint32_t *slot,
+                                    uint32_t *active, uint64_t *active_off) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  if (units[l] > 0) {
+    active[slot[l]] = l;
+    active_off[slot[l]] = offsets[l];
+  }
+  if (l == n - 1) {
+    // sentinel: total units after the last active list
+    const uint32_t count = slot[l] + (units[l] > 0 ? 1u : 0u);
+    active_off[count] = offsets[l] + units[l];
+  }
+}
+
+__device__ __forceinline__ void WaveAppend(bool pred, const SeedHit &h,
+                                           SeedHit *out, uint32_t *counter,
+                                           uint32_t capacity) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, kWave);
+  if (pred) {
+    const uint32_t off = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (off < capacity) out[off] = h;
+  }
+}
+
+__device__ __forceinline__ void WaveCount(bool pred, uint32_t *counter) {
+  const uint64_t mask = __ballot(pred);
+  if (mask != 0 && __lane_id() == __ffsll((unsigned long long)mask) - 1)
+    atomicAdd(counter, (uint32_t)__popcll(mask));
+}
+
+template <class Cursor>
+__device__ __forceinline__ int Side(Cursor &q, Cursor &d, const int8_t *sm,
+                                    uint32_t delim, int cutoff) {
+  int best = 0;
+  uint32_t qc = q.Next(), dc = d.Next();
+  if (dc != delim && qc != delim) {
+    int score = 0;
+    do {
+      if (dc == delim || qc == delim) break;
+      score += sm[dc * kTableDim + qc];
+      qc = q.Next();
+      dc = d.Next();
+      if (score > 0) {
+        do {
+          best += score;
+          if (dc == delim || qc == delim) break;
+          score = sm[dc * kTableDim + qc];
+          qc = q.Next();
+          dc = d.Next();
+        } while (score > 0);
+      }
+    } while (score > -cutoff);
+  }
+  return best;
+}
+
+// SeedSearcher::UngappedExtend (src/seed_searcher.h:133-158)
+__device__ __forceinline__ int Ungapped(const uint64_t *queries, uint32_t qpos,
+                                        const uint64_t *database, uint32_t dpos,
+                                        const int8_t *sm, uint32_t delim,
+                                        int cutoff, int trigger) {
+  ReverseCursor rq, rd;
+  rq.Init(queries, qpos - 1);
+  rd.Init(database, dpos - 1);
+  int sum = Side(rq, rd, sm, delim, cutoff);
+  if (sum <= trigger) {
+    ForwardCursor fq, fd;
+    fq.Init(queries, qpos);
+    fd.Init(database, dpos);
+    sum += Side(fq, fd, sm, delim, cutoff);
+  }
+  return sum;
+}
+
+__device__ __forceinline__ int Distance(const uint64_t *queries, uint32_t qc,
+                                        const uint64_t *database, uint32_t dc,
+                                        uint32_t length, const uint8_t *map) {
+  ForwardCursor a, b;
+  const uint32_t half = length / 2;
+  a.Init(queries, qc - half);
+  b.Init(database, dc - half);
+  int count = 0;
+  for (uint32_t k = 0; k < length; ++k) count += (map[a.Next()] != map[b.Next()]) ? 1 : 0;
+  return count;
+}
+
+template <bool kValues>
+__global__ void __launch_bounds__(kSeedThreads)
+    SeedKernel(DeviceQueryLists q, DeviceDbChunk db, SeedStageParams p,
+               const ListInfo *info, const uint32_t *active,
+               const uint64_t *active_off, uint32_t nactive, uint64_t unit_begin,
+               uint64_t unit_end, SeedHit *hits, uint32_t capacity,
+               uint32_t *counters) {
+  __shared__ int8_t sm[kTableDim * kTableDim];
+  __shared__ uint8_t rmap[kTableDim];
+  __shared__ uint64_t loff[kSeedTile + 1];
+  __shared__ uint32_t lfirst, lcount;
+  StageScoreTable(sm, p.matrix, p.nl);
+  if (threadIdx.x < kTableDim)
+    rmap[threadIdx.x] = threadIdx.x < 25 ? p.reduced_map[threadIdx.x]
+                                         : (uint8_t)(100 + threadIdx.x);
+  __syncthreads();
+  const uint32_t delim = p.delimiter;
+  for (uint64_t tile = unit_begin + (uint64_t)blockIdx.x * kSeedTile; tile < unit_end;
+       tile += (uint64_t)gridDim.x * kSeedTile) {
+    const uint64_t tile_end = min(tile + (uint64_t)kSeedTile, unit_end);
+    if (threadIdx.x == 0) {
+      // active lists covering [tile, tile_end): upper_bound - 1
+      uint32_t lo = 0, hi = nactive + 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (active_off[mid] <= tile) lo = mid + 1; else hi = mid;
+      }
+      const uint32_t a = lo - 1;
+      lo = a;
+      hi = nactive + 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (active_off[mid] <= tile_end - 1) lo = mid + 1; else hi = mid;
+      }
+      lfirst = a;
+      lcount = lo - a;  // lists a .. lo-1
+    }
+    __syncthreads();
+    const uint32_t first = lfirst, count = lcount;
+    for (uint32_t i = threadIdx.x; i <= count; i += kSeedThreads) loff[i] = active_off[first + i];
+    __syncthreads();
+#pragma unroll 1
+    for (int j = 0; j < kSeedItems; ++j) {
+      const uint64_t u = tile + (uint64_t)j * kSeedThreads + threadIdx.x;
+      const bool valid = u < tile_end;
+      SeedHit h = {0, 0, 0, 0, 0, 0, 0};
+      bool pass = false;
+      bool member_ok = false;
+      uint32_t mb = 0, me = 0;
+      int trig = 0, cut = 0;
+      if (valid) {
+        uint32_t lo = 0, hi = count;  // largest i < count with loff[i] <= u
+        while (lo + 1 < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (loff[mid] <= u) lo = mid; else hi = mid;
+        }
+        const uint32_t l = active[first + lo];
+        const uint64_t local = u - loff[lo];
+        const uint32_t eb = q.entry_off[l];
+        const uint32_t P = q.entry_off[l + 1] - eb;
+        const uint32_t idx = (uint32_t)(local / P);
+        const uint32_t e = (uint32_t)(local - (uint64_t)idx * P);
+        const uint32_t ent = eb + e;
+        const uint32_t qpos = q.entry_pos[ent];
+        const uint32_t qid = q.entry_qid[ent];
+        const ListInfo li = info[l];
+        cut = q.cutoffs[qid];
+        trig = q.triggers[qid];
+        h.qid = qid;
+        h.qpos = qpos;
+        h.list = l;
+        h.entry = e;
+        if (kValues) {
+          h.dpos = db.val_pos[li.val_begin + idx];
+          h.rv = li.nrep + idx;
+          pass = Ungapped(q.packed, qpos, db.packed, h.dpos, sm, delim, cut, trig) > trig;
+        } else {
+          const uint32_t ri = li.rep_begin + idx;
+          h.dpos = db.rep_pos[ri];
+          h.rv = idx;
+          const bool check = q.entry_check[ent] != 0;
+          int d = 0;
+          if (check) d = Distance(q.packed, qpos, db.packed, h.dpos, p.subsequence_length, rmap);
+          if (d <= p.distance_threshold)
+            pass = Ungapped(q.packed, qpos, db.packed, h.dpos, sm, delim, cut, trig) > trig;
+          member_ok = !check || max(d - p.max_mismatches, 0) <= p.distance_threshold;
+          if (member_ok) {
+            mb = db.mem_off[ri];
+            me = db.mem_off[ri + 1];
+          }
+        }
+      }
+      WaveAppend(pass, h, hits, counters, capacity);
+      if (!kValues) {
+        for (uint32_t it = 0;; ++it) {
+          const bool act = member_ok && (mb + it < me);
+          if (!__any(act)) break;
+          bool mp = false;
+          SeedHit mh = h;
+          if (act) {
+            mh.dpos = db.mem_pos[mb + it];
+            mh.member = 1 + it;
+            mp = Ungapped(q.packed, mh.qpos, db.packed, mh.dpos, sm, delim, cut, trig) > trig;
+          }
+          WaveAppend(mp, mh, hits, counters, capacity);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+SeedStage::~SeedStage() {
+  void *ps[] = {lookup_, units_, scan_tmp_, active_, hits_};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  if (counters_) (void)hipFree(counters_);
+  if (pinned_) (void)hipHostFree(pinned_);
+}
+
+hipError_t SeedStage::Reserve(void **p, size_t *cap, size_t bytes) {
+  if (*cap >= bytes && *p) return hipSuccess;
+  if (*p) {
+    hipError_t e = hipFree(*p);
+    if (e != hipSuccess) return e;
+    *p = nullptr;
+  }
+  bytes = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess) *cap = bytes;
+  return e;
+}
+
+#define GZ_TRY(x)                      \
+  do {                                 \
+    hipError_t _e = (x);               \
+    if (_e != hipSuccess) return _e;   \
+  } while (0)
+
+hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
+                          const SeedStageParams &p, hipStream_t s,
+                          std::vector<SeedHit> *out, SeedStageStats *stats) {
+  out->clear();
+  const uint32_t n = q.nlists;
+  if (n == 0) return hipSuccess;
+  if (!counters_) GZ_TRY(hipMalloc(&counters_, kCounters * sizeof(uint32_t)));
+  if (!pinned_) {
+    GZ_TRY(hipHostMalloc(&pinned_, 64, hipHostMallocDefault));
+    pinned_cap_ = 64;
+  }
+  // layout: info | units_rep | units_val | off_rep | off_val | flags | slots
+  const size_t sz_info = AlignUp(n * sizeof(ListInfo), 256);
+  const size_t sz_u64 = AlignUp((size_t)(n + 1) * 8, 256);
+  const size_t sz_u32 = AlignUp((size_t)(n + 1) * 4, 256);
+  GZ_TRY(Reserve(&lookup_, &lookup_cap_, sz_info + 4 * sz_u64 + 2 * sz_u32));
+  char *base = (char *)lookup_;
+  ListInfo *info = (ListInfo *)base;
+  uint64_t *units[2] = {(uint64_t *)(base + sz_info), (uint64_t *)(base + sz_info + sz_u64)};
+  uint64_t *offs[2] = {(uint64_t *)(base + sz_info + 2 * sz_u64),
+                       (uint64_t *)(base + sz_info + 3 * sz_u64)};
+  uint32_t *flags = (uint32_t *)(base + sz_info + 4 * sz_u64);
+  uint32_t *slots = (uint32_t *)(base + sz_info + 4 * sz_u64 + sz_u32);
+  const size_t tmp = std::max(DeviceScanTempBytes<uint64_t>(n), DeviceScanTempBytes<uint32_t>(n));
+  GZ_TRY(Reserve(&scan_tmp_, &scan_tmp_cap_, tmp));
+  GZ_TRY(Reserve(&active_, &active_cap_, 2 * (AlignUp((size_t)(n + 1) * 4, 256) +
+                                              AlignUp((size_t)(n + 1) * 8, 256))));
+  if (!hits_) GZ_TRY(Reserve(&hits_, &hits_cap_, kInitialHitCapacity * sizeof(SeedHit)));
+
+  const unsigned grid_n = (n + 255) / 256;
+  hipLaunchKernelGGL(LookupKernel, dim3(grid_n), dim3(256), 0, s, q, db,
+                     p.clustering ? 1 : 0, info, units[0], units[1]);
+  GZ_TRY(hipGetLastError());
+
+  uint32_t *act[2];
+  uint64_t *act_off[2];
+  {
+    char *a = (char *)active_;
+    const size_t s32 = AlignUp((size_t)(n + 1) * 4, 256), s64 = AlignUp((size_t)(n + 1) * 8, 256);
+    act[0] = (uint32_t *)a;
+    act_off[0] = (uint64_t *)(a + s32);
+    act[1] = (uint32_t *)(a + s32 + s64);
+    act_off[1] = (uint64_t *)(a + 2 * s32 + s64);
+  }
+  uint64_t totals[2];
+  uint32_t nact[2];
+  for (int k = 0; k < 2; ++k) {
+    GZ_TRY((DeviceScan<uint64_t, false>(units[k], offs[k], n, false, scan_tmp_, s)));
+    hipLaunchKernelGGL(ActiveFlagsKernel, dim3(grid_n), dim3(256), 0, s, n, units[k], flags);
+    GZ_TRY(hipGetLastError());
+    GZ_TRY((DeviceScan<uint32_t, false>(flags, slots, n, false, scan_tmp_, s)));
+    hipLaunchKernelGGL(ActiveScatterKernel, dim3(grid_n), dim3(256), 0, s, n, units[k],
+                       offs[k], slots, act[k], act_off[k]);
+    GZ_TRY(hipGetLastError());
+    // totals: last offset + last units, last slot + last flag
+    uint64_t *hp = (uint64_t *)pinned_;
+    GZ_TRY(hipMemcpyAsync(hp, offs[k] + (n - 1), 8, hipMemcpyDeviceToHost, s));
+    GZ_TRY(hipMemcpyAsync(hp + 1, units[k] + (n - 1), 8, hipMemcpyDeviceToHost, s));
+    GZ_TRY(hipMemcpyAsync((uint32_t *)(hp + 2), slots + (n - 1), 4, hipMemcpyDeviceToHost, s));
+    GZ_TRY(hipMemcpyAsync((uint32_t *)(hp + 2) + 1, flags + (n - 1), 4, hipMemcpyDeviceToHost, s));
+    GZ_TRY(hipStreamSynchronize(s));
+    totals[k] = hp[0] + hp[1];
+    nact[k] = ((uint32_t *)(hp + 2))[0] + ((uint32_t *)(hp + 2))[1];
+  }
+  if (stats) {
+    stats->units_rep += totals[0];
+    stats->units_value += totals[1];
+  }
+  int dev = 0;
+  GZ_TRY(hipGetDevice(&dev));
+  int cus = 256;
+  GZ_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  for (int k = 0; k < 2; ++k) {
+    if (totals[k] == 0) continue;
+    for (uint64_t u0 = 0; u0 < totals[k];) {
+      const uint64_t u1 = std::min(totals[k], u0 + kUnitsPerLaunch);
+      const uint32_t cap = (uint32_t)(hits_cap_ / sizeof(SeedHit));
+      GZ_TRY(hipMemsetAsync(counters_, 0, kCounters * sizeof(uint32_t), s));
+      const uint64_t tiles = (u1 - u0 + kSeedTile - 1) / kSeedTile;
+      const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)cus * 8);
+      if (k == 0) {
+        hipLaunchKernelGGL((SeedKernel<false>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
+                           info, act[0], act_off[0], nact[0], u0, u1, (SeedHit *)hits_, cap,
+                           counters_);
+      } else {
+        hipLaunchKernelGGL((SeedKernel<true>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
+                           info, act[1], act_off[1], nact[1], u0, u1, (SeedHit *)hits_, cap,
+                           counters_);
+      }
+      GZ_TRY(hipGetLastError());
+      uint32_t *hc = (uint32_t *)pinned_;
+      GZ_TRY(hipMemcpyAsync(hc, counters_, kCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      GZ_TRY(hipStreamSynchronize(s));
+      const uint32_t count = hc[0];
+      if (count > cap) {
+        // too many passing seeds for the buffer: grow it and redo the range
+        size_t want = (size_t)count * 2 * sizeof(SeedHit);
+        GZ_TRY(Reserve(&hits_, &hits_cap_, want));
+        continue;
+      }
+      const size_t old = out->size();
+      out->resize(old + count);
+      if (count)
+        GZ_TRY(hipMemcpyAsync(out->data() + old, hits_, count * sizeof(SeedHit),
+                              hipMemcpyDeviceToHost, s));
+      GZ_TRY(hipStreamSynchronize(s));
+      if (stats) stats->hits += count;
+      u0 = u1;
+    }
+  }
+  return hipSuccess;
+}
+
+}  // namespace gz

@@ -172,6 +172,155 @@ gz_status gz_gapped_extender_extend_one_side_score_only_cells_async(
     gz_gapped_extender_t h, size_t size, int reverse,
     uint32_t *d_query_concatenated_positions, uint32_t *d_database_positions,
     int *d_scores, uint32_t *d_cells, gz_stream_t stream);
+/* Longest query the extender must handle (sizes the wide-band fallback
+ * ring; default 65534 residues).  No counterpart in the reference, whose
+ * GPU kernel caps the query side at 128 residues
+ * (src/gapped_extender_gpu.h:25). */
+gz_status gz_gapped_extender_set_max_sequence_length(gz_gapped_extender_t h,
+                                                     uint32_t length);
+/* Extensions of the last call that no ring could hold (must be 0; a
+ * non-zero count means those results are invalid).  Synchronizes. */
+gz_status gz_gapped_extender_failures(gz_gapped_extender_t h,
+                                      uint32_t *failures);
+
+/* ---- queries (Queries / ProteinQuery / TranslatedDnaQuery,
+ *      src/queries.cpp:183-259, src/protein_query.cpp:22-60,
+ *      src/translated_dna_query.cpp:27-85) ---- */
+typedef struct gz_queries_s *gz_queries_t;
+typedef struct {
+  int translated; /* -q d -t p */
+  int filter;     /* -F T: SEG */
+  int threads;    /* 0 = all cores */
+} gz_query_options;
+typedef struct {
+  uint64_t number_queries;
+  const uint8_t *codes;    /* framed queries back to back */
+  const uint64_t *offsets; /* number_queries + 1 */
+  const char *const *names;
+  const float *lambda;     /* per-query ungapped Karlin */
+  const float *K;
+  int translated;
+  const uint32_t *dna_lengths;
+} gz_queries_view_t;
+gz_status gz_queries_create(const char *const *names, const char *residues,
+                            const uint64_t *offsets, uint64_t n,
+                            const gz_query_options *options, gz_queries_t *out);
+gz_status gz_queries_read_fasta(const char *path, const gz_query_options *options,
+                                gz_queries_t *out);
+gz_status gz_queries_destroy(gz_queries_t q);
+gz_status gz_queries_view(gz_queries_t q, gz_queries_view_t *view);
+
+/* ---- database (Database / DatabaseChunk / SeedSearcherDatabaseParameters,
+ *      src/database.h:440-514, src/database_chunk.h:188-210,
+ *      src/seed_searcher_database_parameters.cpp:41-319) ---- */
+typedef struct gz_database_s *gz_database_t;
+typedef struct {
+  int clustering;              /* -C T */
+  uint32_t subsequence_length; /* -L, 0 = 10 */
+  int seed_threshold;          /* -s, 0 = 39 */
+  uint64_t chunk_size;         /* -l residues, 0 = 1 GiB */
+  int threads;
+} gz_database_options;
+typedef struct {
+  uint32_t number_chunks;
+  uint64_t total_length;
+  uint64_t total_sequences;
+  uint32_t max_length;
+  uint32_t subsequence_length;
+  uint8_t reduced_code_map[25];
+  uint8_t hash_code_map[25];   /* shifted by one */
+  int hash_code_scores[16];
+  uint32_t hash_max_code;
+  int hash_score_threshold;
+  uint32_t hash_max_length;
+  uint32_t hash_shift;
+  int max_mismatches;
+} gz_database_info_t;
+typedef struct {
+  const uint8_t *sequence;
+  uint32_t sequence_length;
+  uint32_t number_sequences;
+  const uint32_t *offsets;
+  const char *const *names;
+  uint32_t number_hashes;
+  const uint32_t *hashes, *rep_offsets, *rep_positions, *rep_member_offsets,
+      *member_positions, *value_offsets, *value_positions;
+  uint64_t number_reps, number_members, number_values;
+  int clustering;
+} gz_database_chunk_view_t;
+gz_status gz_database_create(const char *const *names, const char *residues,
+                             const uint64_t *offsets, uint64_t n,
+                             const gz_database_options *options, gz_database_t *out);
+gz_status gz_database_read_fasta(const char *path, const gz_database_options *options,
+                                 gz_database_t *out);
+gz_status gz_database_destroy(gz_database_t db);
+gz_status gz_database_info(gz_database_t db, gz_database_info_t *info);
+gz_status gz_database_chunk(gz_database_t db, uint32_t chunk,
+                            gz_database_chunk_view_t *view);
+
+/* ---- aligner (AlignerGpu, src/aligner_gpu.h:22-62; one device, one
+ *      GpuStreamController-style pipeline) ---- */
+typedef struct gz_aligner_s *gz_aligner_t;
+#define GZ_SCHEDULE_PLAIN 0         /* gapped extensions in hit order */
+#define GZ_SCHEDULE_LENGTH_SORTED 1 /* by distance to the delimiter */
+#define GZ_SCHEDULE_LEARNED 2       /* by the learned cost model */
+typedef struct {
+  int gap_open, gap_extension;               /* 11 / 1 */
+  float normalized_ungapped_cutoff;          /* 7 bits */
+  float normalized_gapped_trigger;           /* 22 bits */
+  float normalized_presearch_cutoff;         /* 15 bits */
+  float normalized_result_cutoff;            /* 25 bits */
+  uint32_t max_results;                      /* -b */
+  uint32_t max_results_per_subject;          /* -v */
+  uint64_t queries_chunk_size;               /* -l */
+  int threads;                               /* host threads, 0 = all */
+  int schedule;                              /* GZ_SCHEDULE_* */
+  int record_traces;                         /* keep per-extension DP cells */
+} gz_aligner_parameters;
+typedef struct {
+  double seconds_total, seconds_query_lists, seconds_upload, seconds_seed,
+      seconds_chain, seconds_gapped, seconds_add, seconds_build, seconds_output;
+  uint64_t query_chunks, lists, units_rep, units_value, seed_hits, chained_hits,
+      gapped_extensions, gapped_cells, results, sequential_build_chunks;
+  double gapped_kernel_ms;
+} gz_align_stats;
+typedef struct {
+  uint32_t query_position; /* concatenated seed position */
+  uint32_t database_position;
+  uint32_t chunk;
+  uint32_t reverse;
+  uint32_t cells;          /* DP cells the extension evaluated */
+} gz_gapped_trace;
+void gz_aligner_default_parameters(gz_aligner_parameters *p);
+gz_status gz_aligner_create(const gz_aligner_parameters *p, gz_aligner_t *out);
+gz_status gz_aligner_destroy(gz_aligner_t a);
+gz_status gz_aligner_set_database(gz_aligner_t a, gz_database_t db);
+/* Flat fp32 weights of the cost model (embed, w1, b1, w2, b2, w3, b3). */
+gz_status gz_aligner_set_cost_model(gz_aligner_t a, const float *weights, size_t count);
+/* AlignerGpu::Align for in-memory queries: presearch, BuildResults and
+ * WriteOutput; *output is malloc'd (gz_free). */
+gz_status gz_aligner_align(gz_aligner_t a, gz_queries_t q, char **output,
+                           size_t *output_length, gz_align_stats *stats);
+uint64_t gz_aligner_traces(gz_aligner_t a, gz_gapped_trace *out, uint64_t capacity);
+gz_status gz_aligner_last_concatenated_queries(gz_aligner_t a, const uint8_t **codes,
+                                               uint64_t *length);
+/* `ghostz-gpu aln` on FASTA inputs (the database is built in memory). */
+gz_status gz_align_fasta(const char *queries_path, const char *database_path,
+                         const char *output_path, int translated,
+                         const gz_aligner_parameters *p,
+                         const gz_database_options *database_options);
+void gz_free(void *p);
+
+/* ---- host preparation pieces (tests) ---- */
+int gz_seg_mask(const char *sequence, char *masked);    /* SequenceSegFilter */
+double gz_seg_ln_factorial(int n);
+int gz_translate(const char *dna, char *frames, uint32_t *lengths); /* Translator */
+void gz_karlin_block(const double *p_at_min, int min, int max, float *lambda,
+                     float *K, float *H);                /* BlastKarlinBlkCalc */
+int gz_query_karlin(const uint8_t *framed, uint32_t length, float *lambda, float *K,
+                    float *H);
+uint64_t gz_search_space(uint64_t query_length, uint64_t db_length,
+                         uint64_t db_sequences);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,252 @@
+"""HIP extender kernels vs the CPU oracle, through the C ABI.
+
+Every comparison is bit-exact: scores, best positions, flags and distances
+are integers.  Inputs are seeded: homologous pairs (with indels) so that
+extensions are long and gapped, plus unrelated pairs for early X-drop stops.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import gz_native as gz
+import oracle_py
+from helpers import DELIM, concat, random_protein, related_pair_set
+
+pytestmark = pytest.mark.gpu
+INT_MAX = 2**31 - 1
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    gz.check(gz.lib().gz_set_device(0))
+    m, nl = gz.blosum62()
+    return {"matrix": m, "nl": nl, "d_matrix": gz.DeviceBuffer.from_array(m)}
+
+
+def test_pack_round_trip(dev):
+    rng = np.random.default_rng(1)
+    for n in (0, 1, 11, 12, 13, 47, 48, 49, 1000, 12345):
+        say("pack n", n)
+        codes = rng.integers(0, 26, n).astype(np.uint8)
+        _, packed = gz.pack_on_device(codes)
+        words = gz.lib().gz_packed_words(n)
+        host = packed.to_array(np.uint64, words)
+        got = np.array([gz.lib().gz_packed_code_at(gz.ptr(host), i) for i in range(n)], np.uint8)
+        assert np.array_equal(got, codes)
+        # padding reads as delimiter in front and behind
+        if n:
+            assert gz.lib().gz_packed_code_at(gz.ptr(host), n) == DELIM
+
+
+def test_distance_kernel(dev):
+    rng = np.random.default_rng(2)
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 400, min_len=30)
+    rmap, _ = gz.reduced_map()
+    # centres at least L/2 inside their sequences
+    qc, dc = [], []
+    for i in range(len(qpos)):
+        for _ in range(8):
+            a = int(rng.integers(qo[i] + 5, qo[i + 1] - 6)) if qo[i + 1] - qo[i] > 12 else None
+            b = int(rng.integers(do[i] + 5, do[i + 1] - 6)) if do[i + 1] - do[i] > 12 else None
+            if a is None or b is None:
+                continue
+            qc.append(a)
+            dc.append(b if rng.random() < 0.5 else min(max(a - qo[i] + do[i], do[i] + 5),
+                                                       do[i + 1] - 6))
+    qc, dc = np.array(qc, np.uint32), np.array(dc, np.uint32)
+    n = qc.size
+    say("distance pairs", n)
+    _, qp = gz.pack_on_device(q)
+    _, dp = gz.pack_on_device(d)
+    d_map = gz.DeviceBuffer.from_array(rmap)
+    h = ctypes.c_void_p()
+    L = gz.lib()
+    gz.check(L.gz_distance_calculator_create(ctypes.byref(h)))
+    gz.check(L.gz_distance_calculator_set_queries(h, qp.ptr))
+    gz.check(L.gz_distance_calculator_set_database(h, dp.ptr))
+    gz.check(L.gz_distance_calculator_set_reduced_code_map(h, d_map.ptr))
+    gz.check(L.gz_distance_calculator_set_subsequence_length(h, 10))
+    d_qc, d_dc, d_out = gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n)
+    out = np.zeros(n, np.int8)
+    s = gz.Stream()
+    gz.check(L.gz_distance_calculator_calculate_distances_async(
+        h, n, gz.ptr(qc), gz.ptr(dc), gz.ptr(out), d_qc.ptr, d_dc.ptr, d_out.ptr, s.handle))
+    s.sync()
+    O = oracle_py.oracle()
+    want = np.array([O.gzo_distance(gz.ptr(q), int(a), gz.ptr(d), int(b), 10, gz.ptr(rmap))
+                     for a, b in zip(qc, dc)], np.int8)
+    assert np.array_equal(out, want)
+    L.gz_distance_calculator_destroy(h)
+
+
+def _ungapped_case(dev, rng, nseq, related=True):
+    if related:
+        q, qo, d, do, qpos, dpos = related_pair_set(rng, nseq, min_len=10, max_len=400)
+    else:
+        qs = [random_protein(rng, int(rng.integers(5, 300)), 0.01) for _ in range(nseq)]
+        ds = [random_protein(rng, int(rng.integers(5, 300)), 0.01) for _ in range(nseq)]
+        q, qo = concat(qs)
+        d, do = concat(ds)
+        qpos = np.array([rng.integers(qo[i], qo[i + 1] - 1) for i in range(nseq)], np.uint32)
+        dpos = np.array([rng.integers(do[i], do[i + 1] - 1) for i in range(nseq)], np.uint32)
+    qids = np.arange(nseq, dtype=np.uint32)
+    cut = rng.integers(5, 30, nseq).astype(np.int32)
+    trig = rng.integers(10, 60, nseq).astype(np.int32)
+    return q, d, qids, qpos, dpos, cut, trig
+
+
+@pytest.mark.parametrize("related", [True, False])
+def test_ungapped_with_trigger_kernel(dev, related):
+    rng = np.random.default_rng(3 if related else 4)
+    q, d, qids, qpos, dpos, cut, trig = _ungapped_case(dev, rng, 3000, related)
+    n = qids.size
+    say("ungapped seeds", n, "related", related)
+    _, qp = gz.pack_on_device(q)
+    _, dp = gz.pack_on_device(d)
+    d_cut, d_trig = gz.DeviceBuffer.from_array(cut), gz.DeviceBuffer.from_array(trig)
+    L = gz.lib()
+    h = ctypes.c_void_p()
+    gz.check(L.gz_ungapped_extender_create(ctypes.byref(h)))
+    gz.check(L.gz_ungapped_extender_set_queries(h, DELIM, qp.ptr, d_cut.ptr, d_trig.ptr))
+    gz.check(L.gz_ungapped_extender_set_database(h, dp.ptr))
+    gz.check(L.gz_ungapped_extender_set_score_matrix(h, dev["d_matrix"].ptr, dev["nl"]))
+    bufs = [gz.DeviceBuffer(n * 4) for _ in range(3)]
+    d_flags = gz.DeviceBuffer(n)
+    flags = np.zeros(n, np.int8)
+    s = gz.Stream()
+    gz.check(L.gz_ungapped_extender_extend_with_trigger_async(
+        h, n, gz.ptr(qids), gz.ptr(qpos), gz.ptr(dpos), gz.ptr(flags), bufs[0].ptr, bufs[1].ptr,
+        bufs[2].ptr, d_flags.ptr, None, s.handle))
+    s.sync()
+    O = oracle_py.oracle()
+    m = dev["matrix"]
+    want = np.array([O.gzo_ungapped_with_trigger(gz.ptr(q), int(a), gz.ptr(d), int(b), DELIM,
+                                                 gz.ptr(m), 25, int(c), int(t)) > int(t)
+                     for a, b, c, t in zip(qpos, dpos, cut, trig)], np.int8)
+    assert np.array_equal(flags, want)
+    if related:
+        assert want.sum() > n // 10  # the test exercises passing seeds
+    L.gz_ungapped_extender_destroy(h)
+
+
+def test_set_query_seeds_expansion(dev):
+    """Run-length seeds expand to per-seed query id / position (max-scan)."""
+    rng = np.random.default_rng(5)
+    L = gz.lib()
+    nseed_table = 5000
+    table_ids = rng.integers(0, 1000, nseed_table).astype(np.uint32)
+    table_pos = rng.integers(0, 10**7, nseed_table).astype(np.uint32)
+    d_tid, d_tpos = gz.DeviceBuffer.from_array(table_ids), gz.DeviceBuffer.from_array(table_pos)
+    h = ctypes.c_void_p()
+    gz.check(L.gz_ungapped_extender_create(ctypes.byref(h)))
+    gz.check(L.gz_ungapped_extender_set_query_seed_data_list(h, d_tid.ptr, d_tpos.ptr))
+    for runs in (1, 7, 1000, 70000):
+        lens = rng.integers(1, 9, runs)
+        starts = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint32)
+        size = int(lens.sum())
+        ids = rng.integers(0, nseed_table, runs).astype(np.uint32)
+        say("expansion runs", runs, "seeds", size)
+        tb = L.gz_ungapped_extender_query_seeds_temp_bytes(size)
+        d_temp = gz.DeviceBuffer(tb)
+        d_ids, d_starts = gz.DeviceBuffer(runs * 4), gz.DeviceBuffer(runs * 4)
+        d_qid, d_qpos, d_tmp = gz.DeviceBuffer(size * 4), gz.DeviceBuffer(size * 4), \
+            gz.DeviceBuffer(size * 4)
+        s = gz.Stream()
+        gz.check(L.gz_ungapped_extender_set_query_seeds(
+            h, runs, size, gz.ptr(ids), gz.ptr(starts), tb, d_temp.ptr, d_ids.ptr, d_starts.ptr,
+            d_qid.ptr, d_qpos.ptr, d_tmp.ptr, s.handle))
+        s.sync()
+        run_of = np.repeat(np.arange(runs), lens)
+        assert np.array_equal(d_qid.to_array(np.uint32, size), table_ids[ids[run_of]])
+        assert np.array_equal(d_qpos.to_array(np.uint32, size), table_pos[ids[run_of]])
+    L.gz_ungapped_extender_destroy(h)
+
+
+def _gapped_gpu(dev, q, d, qpos, dpos, reverse, cutoff, gap_open=11, gap_ext=1, matrix=None,
+                nl=None, delim=DELIM):
+    L = gz.lib()
+    n = qpos.size
+    _, qp = gz.pack_on_device(q)
+    _, dp = gz.pack_on_device(d)
+    d_m = dev["d_matrix"] if matrix is None else gz.DeviceBuffer.from_array(matrix)
+    h = ctypes.c_void_p()
+    gz.check(L.gz_gapped_extender_create(ctypes.byref(h)))
+    gz.check(L.gz_gapped_extender_set_queries(h, delim, qp.ptr))
+    gz.check(L.gz_gapped_extender_set_database(h, dp.ptr))
+    gz.check(L.gz_gapped_extender_set_score_parameters(h, d_m.ptr, nl or dev["nl"], gap_open,
+                                                       gap_ext, cutoff))
+    hq, hd = qpos.copy(), dpos.copy()
+    hs = np.zeros(n, np.int32)
+    d_q, d_d, d_s = gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n * 4)
+    s = gz.Stream()
+    gz.check(L.gz_gapped_extender_extend_one_side_score_only_async(
+        h, n, int(reverse), gz.ptr(hq), gz.ptr(hd), gz.ptr(hs), d_q.ptr, d_d.ptr, d_s.ptr,
+        s.handle))
+    s.sync()
+    L.gz_gapped_extender_destroy(h)
+    return hs, hq, hd
+
+
+def _gapped_oracle(q, d, qpos, dpos, reverse, cutoff, matrix, nl, gap_open=11, gap_ext=1,
+                   delim=DELIM):
+    out = []
+    for a, b in zip(qpos, dpos):
+        s, p0, p1, _, _ = oracle_py.gapped_one_side("oracle", q, int(a), q.size, d, int(b), delim,
+                                                    reverse, matrix, nl, gap_open, gap_ext,
+                                                    cutoff)
+        out.append((s, int(a) + p0, int(b) + p1))
+    return np.array(out, np.int64)
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("cutoff", [26, 52, 8])
+def test_gapped_kernel_vs_oracle(dev, reverse, cutoff):
+    rng = np.random.default_rng(10 + cutoff + int(reverse))
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 1500, min_len=10, max_len=600,
+                                                identity=(0.25, 0.95), indel=0.04)
+    if reverse:
+        qpos, dpos = qpos - 1, dpos - 1  # reverse seeds start one before the hit
+    say("gapped seeds", qpos.size, "reverse", reverse, "cutoff", cutoff)
+    hs, hq, hd = _gapped_gpu(dev, q, d, qpos, dpos, reverse, cutoff)
+    want = _gapped_oracle(q, d, qpos, dpos, reverse, cutoff, dev["matrix"], dev["nl"])
+    got = np.stack([hs, hq.astype(np.int64), hd.astype(np.int64)], 1)
+    bad = np.nonzero((got != want).any(1))[0]
+    assert bad.size == 0, (bad[:5], got[bad[:5]], want[bad[:5]])
+
+
+def test_gapped_kernel_wide_band(dev):
+    """cutoff = INT_MAX disables X-drop: bands span whole sequences, so the
+    LDS ring overflows and the wide-ring pass must produce the answers."""
+    rng = np.random.default_rng(21)
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 200, min_len=60, max_len=300)
+    say("wide gapped seeds", qpos.size)
+    for reverse in (False, True):
+        a, b = (qpos - 1, dpos - 1) if reverse else (qpos, dpos)
+        hs, hq, hd = _gapped_gpu(dev, q, d, a, b, reverse, INT_MAX)
+        want = _gapped_oracle(q, d, a, b, reverse, INT_MAX, dev["matrix"], dev["nl"])
+        got = np.stack([hs, hq.astype(np.int64), hd.astype(np.int64)], 1)
+        assert np.array_equal(got, want)
+
+
+def test_gapped_kernel_reference_gpu_test_answers(dev):
+    """test/gapped_extender_gpu_test.cu:40-253 on the DNA test alphabet."""
+    D = {"A": 0, "C": 1, "G": 2, "T": 3}
+    mat = np.array([[2 if i == j else -1 for j in range(4)] for i in range(4)], np.int32).ravel()
+
+    def enc(s):
+        return np.array([5] + [D[c] for c in s] + [5], np.uint8)
+
+    s1 = enc("AGCAG")
+    hs, hq, hd = _gapped_gpu(dev, enc("AGCAC"), s1, np.array([1], np.uint32),
+                             np.array([1], np.uint32), False, 0, 0, 1, mat, 4, 5)
+    assert (hs[0], hq[0], hd[0]) == (8, 4, 4)
+    hs, hq, hd = _gapped_gpu(dev, enc("AGTCAC"), s1, np.array([1], np.uint32),
+                             np.array([1], np.uint32), False, INT_MAX, 0, 1, mat, 4, 5)
+    assert (hs[0], hq[0], hd[0]) == (7, 5, 4)
