@@ -291,15 +291,17 @@ gz_status gz_aligner_set_cost_model(gz_aligner_t h, const float *weights, size_t
     SetError("gz_aligner_set_cost_model: wrong weight count");
     return GZ_ERROR_INVALID_ARGUMENT;
   }
-  h->model.reset(new GpuCostModel(w));
-  h->opt.cost_model = h->model.get();
-  h->aligner.reset(new Aligner(h->opt));
+  std::unique_ptr<GpuCostModel> model(new GpuCostModel(w));
+  h->aligner->SetCostModel(model.get());
+  h->opt.cost_model = model.get();
+  h->model = std::move(model);
   return GZ_OK;
 }
 
 gz_status gz_aligner_align(gz_aligner_t h, gz_queries_t q, char **output,
                            size_t *output_length, gz_align_stats *stats) {
   GZ_REQUIRE(h && q && output && output_length, "gz_aligner_align: null argument");
+  GZ_REQUIRE(h->aligner->HasDatabase(), "gz_aligner_align: no database set");
   std::string out;
   AlignStats st;
   GZ_HIP_CHECK(h->aligner->Align(q->q, &out, &st));
@@ -330,6 +332,11 @@ gz_status gz_aligner_align(gz_aligner_t h, gz_queries_t q, char **output,
     stats->results = st.results;
     stats->sequential_build_chunks = st.sequential_build_chunks;
     stats->gapped_kernel_ms = st.gapped_kernel_ms;
+    stats->seed_rep_kernel_ms = st.seed_kernel_ms[0];
+    stats->seed_value_kernel_ms = st.seed_kernel_ms[1];
+    stats->seed_rep_launches = st.seed_launches[0];
+    stats->seed_value_launches = st.seed_launches[1];
+    stats->gapped_launches = st.gapped_launches;
   }
   return GZ_OK;
 }

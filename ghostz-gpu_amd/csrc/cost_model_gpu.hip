@@ -4,6 +4,7 @@
 // kernels of mlp.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -50,7 +51,10 @@ void GpuCostModel::Predict(const uint8_t *queries, const uint8_t *database,
   std::vector<uint8_t> codes(n * kCostResidues);
   for (size_t i = 0; i < n; ++i)
     CostFeatures(queries, database, qpos[i], dpos[i], reverse, &codes[i * kCostResidues]);
-  impl_->mlp.Forward(codes.data(), n, out);
+  // a failed prediction only loses the ordering: equal costs keep the
+  // plain order (the schedule never changes results)
+  if (impl_->mlp.Forward(codes.data(), n, out) != hipSuccess)
+    std::fill(out, out + n, 0.f);
 }
 
 }  // namespace gz

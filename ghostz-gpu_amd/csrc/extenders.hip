@@ -28,12 +28,6 @@
 
 namespace gz {
 
-namespace {
-thread_local std::string g_last_error;
-}
-void SetError(const std::string &msg) { g_last_error = msg; }
-const char *LastError() { return g_last_error.c_str(); }
-
 // ---------------------------------------------------------------------------
 // packing
 // ---------------------------------------------------------------------------

@@ -1,0 +1,305 @@
+// presearch.cpp -- see presearch.h.
+#include "presearch.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <numeric>
+#include <sstream>
+#include <thread>
+
+#include "cost_model.h"
+#include "gz_common.h"
+
+namespace gz {
+
+namespace {
+
+double Now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+template <class F>
+void ParallelBlocks(size_t n, int threads, size_t grain, F f) {
+  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  threads = std::max(1, std::min<int>(threads, (int)((n + grain - 1) / std::max<size_t>(grain, 1))));
+  std::atomic<size_t> next(0);
+  auto work = [&]() {
+    for (;;) {
+      const size_t b = next.fetch_add(grain);
+      if (b >= n) break;
+      f(b, std::min(n, b + grain));
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+}
+
+// the one-thread CPU enumeration order of passing seeds within one query
+struct KeyOrder {
+  const SeedHit *h;
+  bool operator()(uint32_t a, uint32_t b) const {
+    const SeedHit &x = h[a], &y = h[b];
+    if (x.list != y.list) return x.list < y.list;
+    if (x.rv != y.rv) return x.rv < y.rv;
+    if (x.member != y.member) return x.member < y.member;
+    return x.entry < y.entry;
+  }
+};
+
+#define GZ_STEP(x)            \
+  do {                        \
+    const int _r = (x);       \
+    if (_r != 0) return _r;   \
+  } while (0)
+
+int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions &opt,
+                    PresearchDevice *dev, std::string *output, AlignStats *st,
+                    std::vector<GappedTrace> *traces, std::vector<uint8_t> &concat) {
+  const SearchParameters &P = opt.search;
+  const size_t nq = q.size();
+  const int threads = opt.threads > 0 ? opt.threads : (int)std::thread::hardware_concurrency();
+  // cutoffs (AlignerGpu::SetQueriesData, src/aligner_gpu.cpp:213-242)
+  const KarlinParameters gk = Statistics::GappedBlosum62();
+  const int presearch_cutoff = Statistics::Normalized2Nominal(P.normalized_presearch_cutoff, gk);
+  const int result_cutoff = Statistics::Normalized2Nominal(P.normalized_result_cutoff, gk);
+  std::vector<int> cut(nq), trig(nq);
+  for (size_t k = 0; k < nq; ++k) {
+    KarlinParameters kp = {q.lambda[k], q.K[k], 0};
+    cut[k] = Statistics::NormalizedCutoff2NominalCutoff(P.normalized_ungapped_cutoff, kp);
+    trig[k] = Statistics::Normalized2Nominal(P.normalized_gapped_trigger, kp);
+  }
+  double t = Now();
+  std::vector<uint64_t> coff;
+  ConcatenatedOffsets(q, &coff, &concat);
+  QueryLists lists;
+  BuildQueryLists(q, coff, db, threads, &lists);
+  st->seconds_query_lists += Now() - t;
+  st->lists += lists.NumberLists();
+
+  t = Now();
+  GZ_STEP(dev->BeginQueryChunk(q, concat, lists, cut, trig, presearch_cutoff, st));
+  st->seconds_upload += Now() - t;
+
+  std::vector<std::vector<Presearched> > pre(nq);
+  for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
+    const DbChunk &chunk = db.chunks[ci];
+    // 1. seed stage
+    t = Now();
+    std::vector<SeedHit> hits;
+    GZ_STEP(dev->SeedHits(ci, &hits, st));
+    st->seed_hits += hits.size();
+    st->seconds_seed += Now() - t;
+
+    // 2. per-query enumeration order and chain filter
+    t = Now();
+    std::vector<uint32_t> qstart(nq + 1, 0);
+    for (const SeedHit &h : hits) ++qstart[h.qid + 1];
+    for (size_t k = 0; k < nq; ++k) qstart[k + 1] += qstart[k];
+    std::vector<uint32_t> byq(hits.size());
+    {
+      std::vector<uint32_t> fill(qstart.begin(), qstart.end() - 1);
+      for (uint32_t k = 0; k < hits.size(); ++k) byq[fill[hits[k].qid]++] = k;
+    }
+    std::vector<std::vector<Hit> > chained(nq);
+    ParallelBlocks(nq, threads, 64, [&](size_t b, size_t e) {
+      std::vector<uint64_t> scratch;
+      KeyOrder order{hits.data()};
+      for (size_t qi = b; qi < e; ++qi) {
+        const uint32_t s0 = qstart[qi], s1 = qstart[qi + 1];
+        if (s0 == s1) continue;
+        std::sort(byq.begin() + s0, byq.begin() + s1, order);
+        std::vector<Hit> &hv = chained[qi];
+        hv.resize(s1 - s0);
+        for (uint32_t k = s0; k < s1; ++k) {
+          const SeedHit &h = hits[byq[k]];
+          hv[k - s0].q = h.qpos - (uint32_t)coff[qi];
+          hv[k - s0].d = h.dpos;
+        }
+        ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl,
+                    hv, scratch);
+      }
+    });
+    std::vector<SeedHit>().swap(hits);
+    std::vector<uint32_t> gstart(nq + 1, 0);
+    for (size_t k = 0; k < nq; ++k) gstart[k + 1] = gstart[k] + (uint32_t)chained[k].size();
+    const uint32_t G = gstart[nq];
+    st->chained_hits += G;
+    st->seconds_chain += Now() - t;
+
+    // 3. gapped extension of both sides on the device
+    t = Now();
+    std::vector<uint32_t> rq(G), rd(G), fq(G), fd(G);
+    for (size_t qi = 0; qi < nq; ++qi)
+      for (uint32_t k = 0; k < chained[qi].size(); ++k) {
+        const uint32_t g = gstart[qi] + k;
+        const Hit &h = chained[qi][k];
+        rq[g] = (uint32_t)coff[qi] + h.q - 1;
+        rd[g] = h.d - 1;
+        fq[g] = (uint32_t)coff[qi] + h.q;
+        fd[g] = h.d;
+      }
+    std::vector<int> rs(G), fs(G);
+    if (G > 0) {
+      // schedule: the order extensions are handed to the lanes
+      std::vector<uint32_t> perm_r(G), perm_f(G);
+      std::iota(perm_r.begin(), perm_r.end(), 0);
+      std::iota(perm_f.begin(), perm_f.end(), 0);
+      if (opt.schedule != Schedule::kPlain) {
+        std::vector<float> cr(G), cf(G);
+        if (opt.schedule == Schedule::kLearned && opt.cost_model) {
+          opt.cost_model->Predict(concat.data(), chunk.seq.data(), rq.data(), rd.data(), G,
+                                  true, cr.data());
+          opt.cost_model->Predict(concat.data(), chunk.seq.data(), fq.data(), fd.data(), G,
+                                  false, cf.data());
+        } else {
+          for (size_t qi = 0; qi < nq; ++qi)
+            for (uint32_t k = gstart[qi]; k < gstart[qi + 1]; ++k) {
+              cr[k] = (float)q.DistanceFromStart(qi, rq[k] - (uint32_t)coff[qi]);
+              cf[k] = (float)q.DistanceFromEnd(qi, fq[k] - (uint32_t)coff[qi]);
+            }
+        }
+        std::stable_sort(perm_r.begin(), perm_r.end(),
+                         [&](uint32_t a, uint32_t b) { return cr[a] > cr[b]; });
+        std::stable_sort(perm_f.begin(), perm_f.end(),
+                         [&](uint32_t a, uint32_t b) { return cf[a] > cf[b]; });
+      }
+      std::vector<uint32_t> aq(2 * (size_t)G), ad(2 * (size_t)G);
+      for (uint32_t k = 0; k < G; ++k) {
+        aq[k] = rq[perm_r[k]];
+        ad[k] = rd[perm_r[k]];
+        aq[G + k] = fq[perm_f[k]];
+        ad[G + k] = fd[perm_f[k]];
+      }
+      std::vector<int> sc(2 * (size_t)G);
+      std::vector<uint32_t> cells(opt.record_cells ? 2 * (size_t)G : 0);
+      GZ_STEP(dev->GappedExtend(ci, G, G, aq.data(), ad.data(), sc.data(),
+                                cells.empty() ? nullptr : cells.data(), st));
+      for (uint32_t k = 0; k < G; ++k) {
+        rq[perm_r[k]] = aq[k];
+        rd[perm_r[k]] = ad[k];
+        rs[perm_r[k]] = sc[k];
+        fq[perm_f[k]] = aq[G + k];
+        fd[perm_f[k]] = ad[G + k];
+        fs[perm_f[k]] = sc[G + k];
+      }
+      if (!cells.empty()) {
+        // seed positions before extension, per job in launch order
+        std::vector<uint32_t> owner(G);
+        for (size_t qi = 0; qi < nq; ++qi)
+          for (uint32_t g = gstart[qi]; g < gstart[qi + 1]; ++g) owner[g] = (uint32_t)qi;
+        for (int side = 0; side < 2; ++side) {
+          const std::vector<uint32_t> &perm = side == 0 ? perm_r : perm_f;
+          for (uint32_t k = 0; k < G; ++k) {
+            const uint32_t g = perm[k];
+            const uint32_t qi = owner[g];
+            const Hit &h = chained[qi][g - gstart[qi]];
+            GappedTrace tr;
+            tr.chunk = (uint32_t)ci;
+            tr.reverse = side == 0 ? 1 : 0;
+            tr.qpos = (uint32_t)coff[qi] + h.q - (side == 0 ? 1 : 0);
+            tr.dpos = h.d - (side == 0 ? 1 : 0);
+            tr.cells = cells[(size_t)side * G + k];
+            if (traces) traces->push_back(tr);
+            st->gapped_cells += tr.cells;
+          }
+        }
+      }
+    }
+    st->gapped_extensions += 2 * (uint64_t)G;
+    st->seconds_gapped += Now() - t;
+
+    // 4. presearched results and AddResults per query
+    t = Now();
+    ParallelBlocks(nq, threads, 64, [&](size_t b, size_t e) {
+      std::vector<std::vector<AlignmentPosition> > starts;
+      std::vector<Presearched> tmp;
+      for (size_t qi = b; qi < e; ++qi) {
+        tmp.clear();
+        for (uint32_t g = gstart[qi]; g < gstart[qi + 1]; ++g) {
+          const Hit &h = chained[qi][g - gstart[qi]];
+          Presearched r;
+          r.chunk = -1;
+          r.subject = 0;
+          r.hit.q = h.q;
+          r.hit.d = h.d;
+          r.start.q = rq[g] - (uint32_t)coff[qi];
+          r.start.d = rd[g];
+          r.end.q = fq[g] - (uint32_t)coff[qi];
+          r.end.d = fd[g];
+          r.score = rs[g] + fs[g];
+          r.hit_count = 1;
+          tmp.push_back(r);
+        }
+        if (!tmp.empty() || !pre[qi].empty())
+          AddResults(chunk, (int)ci, P, starts, tmp, pre[qi]);
+      }
+    });
+    st->seconds_add += Now() - t;
+  }
+  dev->EndQueryChunk();
+
+  // BuildResults and WriteOutput
+  t = Now();
+  std::vector<std::vector<AlignmentResult> > results;
+  BuildResults(q, db, P, result_cutoff, pre, results, threads, &st->sequential_build_chunks);
+  for (auto &r : results) st->results += r.size();
+  st->seconds_build += Now() - t;
+  t = Now();
+  std::ostringstream os;
+  WriteOutput(os, q, db, results);
+  output->append(os.str());
+  st->seconds_output += Now() - t;
+  return 0;
+}
+
+}  // namespace
+
+int AlignQueries(const QuerySet &queries, const Database &db, const AlignerOptions &opt,
+                 PresearchDevice *dev, std::string *output, AlignStats *stats,
+                 std::vector<GappedTrace> *traces, std::vector<uint8_t> *concat) {
+  AlignStats dummy;
+  AlignStats *st = stats ? stats : &dummy;
+  std::vector<uint8_t> local_concat;
+  std::vector<uint8_t> &cc = concat ? *concat : local_concat;
+  if (traces) traces->clear();
+  const double t0 = Now();
+  // query chunks (Queries::SetQueries, src/queries.cpp:183-224)
+  const size_t n = queries.size();
+  size_t i = 0;
+  while (i < n) {
+    uint64_t sum = 0;
+    size_t j = i;
+    while (j < n) {
+      const uint64_t next = sum + queries.Length(j);
+      if (next >= opt.queries_chunk_size) break;
+      sum = next;
+      ++j;
+    }
+    if (j == i) j = i + 1;
+    QuerySet part;
+    part.translated = queries.translated;
+    part.max_length = 0;
+    part.offsets.push_back(0);
+    for (size_t k = i; k < j; ++k) {
+      part.names.push_back(queries.names[k]);
+      part.codes.insert(part.codes.end(), queries.Sequence(k),
+                        queries.Sequence(k) + queries.Length(k));
+      part.offsets.push_back(part.codes.size());
+      part.lambda.push_back(queries.lambda[k]);
+      part.K.push_back(queries.K[k]);
+      if (queries.translated) part.dna_lengths.push_back(queries.dna_lengths[k]);
+      part.max_length = std::max(part.max_length, queries.Length(k));
+    }
+    GZ_STEP(AlignQueryChunk(part, db, opt, dev, output, st, traces, cc));
+    ++st->query_chunks;
+    i = j;
+  }
+  st->seconds_total = Now() - t0;
+  return 0;
+}
+
+}  // namespace gz

@@ -1,0 +1,112 @@
+// presearch.h -- the host side of AlignerGpu's search (src/aligner_gpu.cpp,
+// src/aligner_gpu_presearch_thread.cpp:140-329): query chunking, hash
+// lists, per-query seed order, chain filter, gapped-extension schedule,
+// AddResults, then BuildResults and WriteOutput.  The device stages (seed
+// search and one-sided gapped extension) are reached through
+// PresearchDevice; the product implements it with the HIP kernels
+// (aligner.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "database.h"
+#include "queries.h"
+#include "query_lists.h"
+#include "results.h"
+
+namespace gz {
+
+// One passing seed of the device seed stage with the key (list rank,
+// representative or value index, member index + 1 or 0, entry index) that
+// orders it exactly as the one-thread CPU search enumerates it.
+struct SeedHit {
+  uint32_t qid;
+  uint32_t qpos;   // concatenated query position
+  uint32_t dpos;   // database chunk position
+  uint32_t list;   // list rank
+  uint32_t rv;     // representative index, or number of reps + value index
+  uint32_t member; // 0 for a representative / value, 1 + member index
+  uint32_t entry;  // entry index within the list
+};
+
+struct AlignStats {
+  double seconds_total = 0;       // presearch start .. output written
+  double seconds_query_lists = 0;
+  double seconds_upload = 0;
+  double seconds_seed = 0;        // device seed stage incl. hit download
+  double seconds_chain = 0;
+  double seconds_gapped = 0;      // device gapped extension incl. transfers
+  double seconds_add = 0;
+  double seconds_build = 0;
+  double seconds_output = 0;
+  uint64_t query_chunks = 0;
+  uint64_t lists = 0;
+  uint64_t units_rep = 0, units_value = 0;
+  uint64_t seed_hits = 0;
+  uint64_t chained_hits = 0;
+  uint64_t gapped_extensions = 0;
+  uint64_t gapped_cells = 0;
+  uint64_t results = 0;
+  uint64_t sequential_build_chunks = 0;
+  double gapped_kernel_ms = 0;    // measured with HIP events on the stream
+  double seed_kernel_ms[2] = {0, 0};  // seed kernels: representatives, values
+  uint64_t seed_launches[2] = {0, 0};
+  uint64_t gapped_launches = 0;
+};
+
+enum class Schedule { kPlain = 0, kLengthSorted = 1, kLearned = 2 };
+
+class CostModel;  // learned gapped-extension cost (cost_model.h)
+
+struct AlignerOptions {
+  SearchParameters search;
+  uint64_t queries_chunk_size = 1ull << 27;  // -l (framed codes)
+  int threads = 0;
+  Schedule schedule = Schedule::kPlain;
+  const CostModel *cost_model = nullptr;     // for Schedule::kLearned
+  bool record_cells = false;                 // keep per-extension DP cells
+};
+
+// One gapped extension job as traced for the learned scheduler.
+struct GappedTrace {
+  uint32_t qpos;   // concatenated seed position
+  uint32_t dpos;
+  uint32_t chunk;
+  uint8_t reverse;
+  uint32_t cells;
+};
+
+// Device stages of one query chunk.  Every call returns 0 on success.
+class PresearchDevice {
+ public:
+  virtual ~PresearchDevice() {}
+  // Makes the query chunk resident: concatenated codes (shared delimiters),
+  // hash lists, per-query ungapped cutoffs / gapped triggers.
+  virtual int BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &concat,
+                              const QueryLists &lists, const std::vector<int> &cutoffs,
+                              const std::vector<int> &triggers, int presearch_cutoff,
+                              AlignStats *st) = 0;
+  // Passing seeds of all lists against database chunk `chunk` (any order).
+  virtual int SeedHits(size_t chunk, std::vector<SeedHit> *hits, AlignStats *st) = 0;
+  // One-sided gapped X-drop extensions of database chunk `chunk`: the first
+  // n_reverse jobs extend backwards, the next n_forward forwards; qpos/dpos
+  // (concatenated query / chunk positions) are replaced by the extension
+  // ends and scores written; cells (optional) receives the DP cells of each.
+  virtual int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward,
+                           uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
+                           AlignStats *st) = 0;
+  virtual void EndQueryChunk() = 0;
+};
+
+// Aligns all queries (split into chunks by queries_chunk_size) against all
+// database chunks and appends the tabular output.  traces (optional) gets
+// the gapped extensions with their DP cells when options.record_cells is
+// set; concat (optional) keeps the last chunk's concatenated codes.
+int AlignQueries(const QuerySet &queries, const Database &db, const AlignerOptions &options,
+                 PresearchDevice *device, std::string *output, AlignStats *st,
+                 std::vector<GappedTrace> *traces, std::vector<uint8_t> *concat);
+
+}  // namespace gz

@@ -19,17 +19,11 @@
 
 #include <hip/hip_runtime.h>
 
-namespace gz {
+#include <vector>
 
-struct SeedHit {
-  uint32_t qid;
-  uint32_t qpos;   // concatenated query position
-  uint32_t dpos;   // database chunk position
-  uint32_t list;   // list rank
-  uint32_t rv;     // representative index, or number of reps + value index
-  uint32_t member; // 0 for a representative / value, 1 + member index
-  uint32_t entry;  // entry index within the list
-};
+#include "presearch.h"
+
+namespace gz {
 
 struct DeviceDbChunk {
   const uint64_t *packed = nullptr;
@@ -65,6 +59,9 @@ struct SeedStageStats {
   uint64_t units_value = 0;    // (list, value, entry) units
   uint64_t distance_pairs = 0;
   uint64_t hits = 0;
+  // seed kernel time (HIP events on the stream) and launches, per kind
+  double kernel_ms[2] = {0, 0};  // [0] representatives, [1] values
+  uint64_t launches[2] = {0, 0};
 };
 
 // Workspace owned by the caller (grown on demand).
@@ -85,6 +82,7 @@ class SeedStage {
   void *active_ = nullptr; size_t active_cap_ = 0;
   void *hits_ = nullptr; size_t hits_cap_ = 0;
   uint32_t *counters_ = nullptr;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
   void *pinned_ = nullptr; size_t pinned_cap_ = 0;
 };
 

@@ -272,6 +272,8 @@ SeedStage::~SeedStage() {
     if (p) (void)hipFree(p);
   if (counters_) (void)hipFree(counters_);
   if (pinned_) (void)hipHostFree(pinned_);
+  for (hipEvent_t e : ev_)
+    if (e) (void)hipEventDestroy(e);
 }
 
 hipError_t SeedStage::Reserve(void **p, size_t *cap, size_t bytes) {
@@ -321,6 +323,10 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
   GZ_TRY(Reserve(&active_, &active_cap_, 2 * (AlignUp((size_t)(n + 1) * 4, 256) +
                                               AlignUp((size_t)(n + 1) * 8, 256))));
   if (!hits_) GZ_TRY(Reserve(&hits_, &hits_cap_, kInitialHitCapacity * sizeof(SeedHit)));
+  if (!ev_[0]) {
+    GZ_TRY(hipEventCreate(&ev_[0]));
+    GZ_TRY(hipEventCreate(&ev_[1]));
+  }
 
   const unsigned grid_n = (n + 255) / 256;
   hipLaunchKernelGGL(LookupKernel, dim3(grid_n), dim3(256), 0, s, q, db,
@@ -373,6 +379,7 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
       GZ_TRY(hipMemsetAsync(counters_, 0, kCounters * sizeof(uint32_t), s));
       const uint64_t tiles = (u1 - u0 + kSeedTile - 1) / kSeedTile;
       const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)cus * 8);
+      GZ_TRY(hipEventRecord(ev_[0], s));
       if (k == 0) {
         hipLaunchKernelGGL((SeedKernel<false>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
                            info, act[0], act_off[0], nact[0], u0, u1, (SeedHit *)hits_, cap,
@@ -383,10 +390,17 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
                            counters_);
       }
       GZ_TRY(hipGetLastError());
+      GZ_TRY(hipEventRecord(ev_[1], s));
       uint32_t *hc = (uint32_t *)pinned_;
       GZ_TRY(hipMemcpyAsync(hc, counters_, kCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
       GZ_TRY(hipStreamSynchronize(s));
       const uint32_t count = hc[0];
+      if (stats) {
+        float ms = 0;
+        GZ_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        stats->kernel_ms[k] += ms;
+        stats->launches[k] += 1;
+      }
       if (count > cap) {
         // too many passing seeds for the buffer: grow it and redo the range
         size_t want = (size_t)count * 2 * sizeof(SeedHit);

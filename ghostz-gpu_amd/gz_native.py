@@ -205,3 +205,361 @@ def pack_on_device(codes, stream=None):
     check(lib().gz_pack_sequence_async(d_codes.ptr, codes.size, d_packed.ptr, stream))
     check(lib().gz_stream_synchronize(stream))
     return d_codes, d_packed
+
+
+# ---- pipeline (queries, database, aligner) ----------------------------------
+from ctypes import Structure, c_char, c_double, c_float  # noqa: E402
+
+
+class QueryOptions(Structure):
+    _fields_ = [("translated", c_int), ("filter", c_int), ("threads", c_int)]
+
+
+class QueriesView(Structure):
+    _fields_ = [("number_queries", c_uint64), ("codes", c_void_p), ("offsets", c_void_p),
+                ("names", c_void_p), ("lambda_", c_void_p), ("K", c_void_p),
+                ("translated", c_int), ("dna_lengths", c_void_p)]
+
+
+class DatabaseOptions(Structure):
+    _fields_ = [("clustering", c_int), ("subsequence_length", c_uint32),
+                ("seed_threshold", c_int), ("chunk_size", c_uint64), ("threads", c_int)]
+
+
+class DatabaseInfo(Structure):
+    _fields_ = [("number_chunks", c_uint32), ("total_length", c_uint64),
+                ("total_sequences", c_uint64), ("max_length", c_uint32),
+                ("subsequence_length", c_uint32), ("reduced_code_map", c_uint8 * 25),
+                ("hash_code_map", c_uint8 * 25), ("hash_code_scores", c_int * 16),
+                ("hash_max_code", c_uint32), ("hash_score_threshold", c_int),
+                ("hash_max_length", c_uint32), ("hash_shift", c_uint32),
+                ("max_mismatches", c_int)]
+
+
+class ChunkView(Structure):
+    _fields_ = [("sequence", c_void_p), ("sequence_length", c_uint32),
+                ("number_sequences", c_uint32), ("offsets", c_void_p), ("names", c_void_p),
+                ("number_hashes", c_uint32), ("hashes", c_void_p), ("rep_offsets", c_void_p),
+                ("rep_positions", c_void_p), ("rep_member_offsets", c_void_p),
+                ("member_positions", c_void_p), ("value_offsets", c_void_p),
+                ("value_positions", c_void_p), ("number_reps", c_uint64),
+                ("number_members", c_uint64), ("number_values", c_uint64),
+                ("clustering", c_int)]
+
+
+class AlignerParameters(Structure):
+    _fields_ = [("gap_open", c_int), ("gap_extension", c_int),
+                ("normalized_ungapped_cutoff", c_float), ("normalized_gapped_trigger", c_float),
+                ("normalized_presearch_cutoff", c_float), ("normalized_result_cutoff", c_float),
+                ("max_results", c_uint32), ("max_results_per_subject", c_uint32),
+                ("queries_chunk_size", c_uint64), ("threads", c_int), ("schedule", c_int),
+                ("record_traces", c_int)]
+
+
+class AlignStats(Structure):
+    _fields_ = [(n, c_double) for n in (
+        "seconds_total", "seconds_query_lists", "seconds_upload", "seconds_seed",
+        "seconds_chain", "seconds_gapped", "seconds_add", "seconds_build",
+        "seconds_output")] + [(n, c_uint64) for n in (
+            "query_chunks", "lists", "units_rep", "units_value", "seed_hits", "chained_hits",
+            "gapped_extensions", "gapped_cells", "results", "sequential_build_chunks")] + [
+        ("gapped_kernel_ms", c_double), ("seed_rep_kernel_ms", c_double),
+        ("seed_value_kernel_ms", c_double), ("seed_rep_launches", c_uint64),
+        ("seed_value_launches", c_uint64), ("gapped_launches", c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class GappedTrace(Structure):
+    _fields_ = [("query_position", c_uint32), ("database_position", c_uint32),
+                ("chunk", c_uint32), ("reverse", c_uint32), ("cells", c_uint32)]
+
+
+SCHEDULE_PLAIN, SCHEDULE_LENGTH_SORTED, SCHEDULE_LEARNED = 0, 1, 2
+
+_SIGNATURES.update({
+    "gz_queries_create": (c_int, [c_void_p, c_char_p, c_void_p, c_uint64,
+                                  POINTER(QueryOptions), POINTER(c_void_p)]),
+    "gz_queries_read_fasta": (c_int, [c_char_p, POINTER(QueryOptions), POINTER(c_void_p)]),
+    "gz_queries_destroy": (c_int, [c_void_p]),
+    "gz_queries_view": (c_int, [c_void_p, POINTER(QueriesView)]),
+    "gz_database_create": (c_int, [c_void_p, c_char_p, c_void_p, c_uint64,
+                                   POINTER(DatabaseOptions), POINTER(c_void_p)]),
+    "gz_database_read_fasta": (c_int, [c_char_p, POINTER(DatabaseOptions), POINTER(c_void_p)]),
+    "gz_database_destroy": (c_int, [c_void_p]),
+    "gz_database_info": (c_int, [c_void_p, POINTER(DatabaseInfo)]),
+    "gz_database_chunk": (c_int, [c_void_p, c_uint32, POINTER(ChunkView)]),
+    "gz_aligner_default_parameters": (None, [POINTER(AlignerParameters)]),
+    "gz_aligner_create": (c_int, [POINTER(AlignerParameters), POINTER(c_void_p)]),
+    "gz_aligner_destroy": (c_int, [c_void_p]),
+    "gz_aligner_set_database": (c_int, [c_void_p, c_void_p]),
+    "gz_aligner_set_cost_model": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "gz_aligner_align": (c_int, [c_void_p, c_void_p, POINTER(c_void_p), POINTER(c_size_t),
+                                 POINTER(AlignStats)]),
+    "gz_aligner_traces": (c_uint64, [c_void_p, c_void_p, c_uint64]),
+    "gz_aligner_last_concatenated_queries": (c_int, [c_void_p, POINTER(c_void_p),
+                                                     POINTER(c_uint64)]),
+    "gz_align_fasta": (c_int, [c_char_p, c_char_p, c_char_p, c_int,
+                               POINTER(AlignerParameters), POINTER(DatabaseOptions)]),
+    "gz_free": (None, [c_void_p]),
+    "gz_mlp_weight_count": (c_size_t, []),
+    "gz_mlp_init_weights": (c_int, [c_uint64, c_void_p]),
+    "gz_mlp_create": (c_int, [c_void_p, c_size_t, POINTER(c_void_p)]),
+    "gz_mlp_destroy": (c_int, [c_void_p]),
+    "gz_mlp_get_weights": (c_int, [c_void_p, c_void_p]),
+    "gz_mlp_predict": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, POINTER(c_float)]),
+    "gz_mlp_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_float,
+                                  POINTER(c_float), c_void_p]),
+    "gz_cost_features": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int,
+                                c_void_p]),
+    "gz_seg_mask": (c_int, [c_char_p, c_void_p]),
+    "gz_seg_ln_factorial": (c_double, [c_int]),
+    "gz_translate": (c_int, [c_char_p, c_void_p, c_void_p]),
+    "gz_karlin_block": (None, [c_void_p, c_int, c_int, POINTER(c_float), POINTER(c_float),
+                               POINTER(c_float)]),
+    "gz_query_karlin": (c_int, [c_void_p, c_uint32, POINTER(c_float), POINTER(c_float),
+                                POINTER(c_float)]),
+    "gz_search_space": (c_uint64, [c_uint64, c_uint64, c_uint64]),
+})
+
+
+def _names_array(names):
+    arr = (c_char_p * len(names))(*[n.encode() if isinstance(n, str) else n for n in names])
+    return arr
+
+
+def _read_array(addr, dtype, count):
+    if count == 0 or not addr:
+        return np.zeros(0, dtype)
+    n = int(count) * np.dtype(dtype).itemsize
+    return np.frombuffer(ctypes.string_at(addr, n), dtype).copy()
+
+
+class Queries:
+    """Host-side query set (Queries): encoding, SEG, translation, Karlin."""
+
+    def __init__(self, names, residues, offsets, translated=False, filter=True, threads=0):
+        offs = np.ascontiguousarray(offsets, np.uint64)
+        o = QueryOptions(int(translated), int(filter), int(threads))
+        h = c_void_p()
+        nm = _names_array(names)
+        check(lib().gz_queries_create(nm, residues, ptr(offs), len(names), ctypes.byref(o),
+                                      ctypes.byref(h)))
+        self.handle = h
+        self.translated = bool(translated)
+
+    @classmethod
+    def from_fasta(cls, path, translated=False, filter=True, threads=0):
+        self = cls.__new__(cls)
+        o = QueryOptions(int(translated), int(filter), int(threads))
+        h = c_void_p()
+        check(lib().gz_queries_read_fasta(path.encode(), ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+        self.translated = bool(translated)
+        return self
+
+    def view(self):
+        v = QueriesView()
+        check(lib().gz_queries_view(self.handle, ctypes.byref(v)))
+        n = v.number_queries
+        offsets = _read_array(v.offsets, np.uint64, n + 1)
+        out = {
+            "number_queries": n,
+            "codes": _read_array(v.codes, np.uint8, int(offsets[-1]) if n else 0),
+            "offsets": offsets,
+            "lambda": _read_array(v.lambda_, np.float32, n),
+            "K": _read_array(v.K, np.float32, n),
+            "translated": v.translated,
+            "dna_lengths": _read_array(v.dna_lengths, np.uint32, n) if v.translated else None,
+        }
+        names_p = ctypes.cast(v.names, POINTER(c_char_p))
+        out["names"] = [names_p[i] for i in range(n)]
+        return out
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().gz_queries_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Database:
+    """In-memory database (Database + SeedSearcherDatabaseParameters index)."""
+
+    def __init__(self, names, residues, offsets, clustering=True, subsequence_length=10,
+                 seed_threshold=39, chunk_size=0, threads=0):
+        offs = np.ascontiguousarray(offsets, np.uint64)
+        o = DatabaseOptions(int(clustering), subsequence_length, seed_threshold, chunk_size,
+                            threads)
+        h = c_void_p()
+        check(lib().gz_database_create(_names_array(names), residues, ptr(offs), len(names),
+                                       ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def from_fasta(cls, path, clustering=True, subsequence_length=10, seed_threshold=39,
+                   chunk_size=0, threads=0):
+        self = cls.__new__(cls)
+        o = DatabaseOptions(int(clustering), subsequence_length, seed_threshold, chunk_size,
+                            threads)
+        h = c_void_p()
+        check(lib().gz_database_read_fasta(path.encode(), ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+        return self
+
+    def info(self):
+        i = DatabaseInfo()
+        check(lib().gz_database_info(self.handle, ctypes.byref(i)))
+        return i
+
+    def chunk(self, c):
+        v = ChunkView()
+        check(lib().gz_database_chunk(self.handle, c, ctypes.byref(v)))
+        nh = v.number_hashes
+        out = {
+            "sequence": _read_array(v.sequence, np.uint8, v.sequence_length),
+            "number_sequences": v.number_sequences,
+            "offsets": _read_array(v.offsets, np.uint32, v.number_sequences + 1),
+            "hashes": _read_array(v.hashes, np.uint32, nh),
+            "rep_offsets": _read_array(v.rep_offsets, np.uint32, nh + 1),
+            "rep_positions": _read_array(v.rep_positions, np.uint32, v.number_reps),
+            "rep_member_offsets": _read_array(v.rep_member_offsets, np.uint32,
+                                              v.number_reps + 1),
+            "member_positions": _read_array(v.member_positions, np.uint32, v.number_members),
+            "value_offsets": _read_array(v.value_offsets, np.uint32, nh + 1),
+            "value_positions": _read_array(v.value_positions, np.uint32, v.number_values),
+            "clustering": v.clustering,
+        }
+        names_p = ctypes.cast(v.names, POINTER(c_char_p))
+        out["names"] = [names_p[i] for i in range(v.number_sequences)]
+        return out
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().gz_database_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def default_parameters(**kw):
+    p = AlignerParameters()
+    lib().gz_aligner_default_parameters(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Aligner:
+    """AlignerGpu on one device: presearch on the GPU, results on the host."""
+
+    def __init__(self, database, **params):
+        self.params = default_parameters(**params)
+        h = c_void_p()
+        check(lib().gz_aligner_create(ctypes.byref(self.params), ctypes.byref(h)))
+        self.handle = h
+        self.database = database
+        check(lib().gz_aligner_set_database(h, database.handle))
+
+    def set_cost_model(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        check(lib().gz_aligner_set_cost_model(self.handle, ptr(flat), flat.size))
+
+    def align(self, queries):
+        out = c_void_p()
+        n = c_size_t()
+        st = AlignStats()
+        check(lib().gz_aligner_align(self.handle, queries.handle, ctypes.byref(out),
+                                     ctypes.byref(n), ctypes.byref(st)))
+        text = ctypes.string_at(out, n.value) if n.value else b""
+        lib().gz_free(out)
+        return text, st.as_dict()
+
+    def traces(self):
+        n = lib().gz_aligner_traces(self.handle, None, 0)
+        arr = (GappedTrace * max(int(n), 1))()
+        n = lib().gz_aligner_traces(self.handle, arr, n)
+        a = np.frombuffer(bytes(arr), np.uint32).reshape(-1, 5)[:n]
+        return a.copy()
+
+    def last_concatenated_queries(self):
+        p = c_void_p()
+        n = c_uint64()
+        check(lib().gz_aligner_last_concatenated_queries(self.handle, ctypes.byref(p),
+                                                         ctypes.byref(n)))
+        return _read_array(p.value, np.uint8, n.value)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().gz_aligner_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Mlp:
+    """The learned extension-cost MLP (bf16 MFMA kernels, fp32 master weights)."""
+
+    def __init__(self, flat=None, seed=1):
+        if flat is None:
+            flat = self.init_weights(seed)
+        flat = np.ascontiguousarray(flat, np.float32)
+        h = c_void_p()
+        check(lib().gz_mlp_create(ptr(flat), flat.size, ctypes.byref(h)))
+        self.handle = h
+
+    @staticmethod
+    def weight_count():
+        return int(lib().gz_mlp_weight_count())
+
+    @staticmethod
+    def init_weights(seed):
+        w = np.zeros(Mlp.weight_count(), np.float32)
+        check(lib().gz_mlp_init_weights(seed, ptr(w)))
+        return w
+
+    def weights(self):
+        w = np.zeros(self.weight_count(), np.float32)
+        check(lib().gz_mlp_get_weights(self.handle, ptr(w)))
+        return w
+
+    def predict(self, codes):
+        codes = np.ascontiguousarray(codes, np.uint8).reshape(-1, 128)
+        out = np.zeros(len(codes), np.float32)
+        ms = c_float()
+        check(lib().gz_mlp_predict(self.handle, ptr(codes), len(codes), ptr(out),
+                                   ctypes.byref(ms)))
+        self.last_kernel_ms = ms.value
+        return out
+
+    def train_step(self, codes, targets, lr, want_grad=False):
+        codes = np.ascontiguousarray(codes, np.uint8).reshape(-1, 128)
+        t = np.ascontiguousarray(targets, np.float32)
+        loss = c_float()
+        g = np.zeros(self.weight_count(), np.float32) if want_grad else None
+        check(lib().gz_mlp_train_step(self.handle, ptr(codes), ptr(t), len(codes), lr,
+                                      ctypes.byref(loss), ptr(g)))
+        return (loss.value, g) if want_grad else loss.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().gz_mlp_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def cost_features(queries, database, qpos, dpos, reverse):
+    q = np.ascontiguousarray(queries, np.uint8)
+    d = np.ascontiguousarray(database, np.uint8)
+    qp = np.ascontiguousarray(qpos, np.uint32)
+    dp = np.ascontiguousarray(dpos, np.uint32)
+    out = np.zeros((len(qp), 128), np.uint8)
+    lib().gz_cost_features(ptr(q), ptr(d), ptr(qp), ptr(dp), len(qp), int(reverse), ptr(out))
+    return out

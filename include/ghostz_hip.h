@@ -282,7 +282,10 @@ typedef struct {
       seconds_chain, seconds_gapped, seconds_add, seconds_build, seconds_output;
   uint64_t query_chunks, lists, units_rep, units_value, seed_hits, chained_hits,
       gapped_extensions, gapped_cells, results, sequential_build_chunks;
-  double gapped_kernel_ms;
+  double gapped_kernel_ms;     /* HIP events around the gapped launches */
+  double seed_rep_kernel_ms;   /* HIP events around the seed kernels */
+  double seed_value_kernel_ms;
+  uint64_t seed_rep_launches, seed_value_launches, gapped_launches;
 } gz_align_stats;
 typedef struct {
   uint32_t query_position; /* concatenated seed position */
@@ -310,6 +313,27 @@ gz_status gz_align_fasta(const char *queries_path, const char *database_path,
                          const gz_aligner_parameters *p,
                          const gz_database_options *database_options);
 void gz_free(void *p);
+
+/* ---- learned extension-cost model (no counterpart in the reference: it
+ *      only orders gapped extensions for load balance) ----
+ * Flat fp32 layout: embed[21][8], w1[256][1024], b1[256], w2[64][256],
+ * b2[64], w3[64], b3[1].  Inputs are 128 residue codes per seed in [0, 21). */
+typedef struct gz_mlp_s *gz_mlp_t;
+size_t gz_mlp_weight_count(void);
+gz_status gz_mlp_init_weights(uint64_t seed, float *flat);
+gz_status gz_mlp_create(const float *flat, size_t count, gz_mlp_t *out);
+gz_status gz_mlp_destroy(gz_mlp_t m);
+gz_status gz_mlp_get_weights(gz_mlp_t m, float *flat);
+/* Predicted log2 DP cells; kernel_ms (optional) = device forward time. */
+gz_status gz_mlp_predict(gz_mlp_t m, const uint8_t *codes, size_t n, float *out,
+                         float *kernel_ms);
+/* One MSE step: gradients (flat layout, optional out) and Adam (lr > 0). */
+gz_status gz_mlp_train_step(gz_mlp_t m, const uint8_t *codes, const float *targets,
+                            size_t n, float lr, float *loss, float *flat_grad);
+/* 128 feature codes per seed from unpacked concatenated query / chunk codes. */
+void gz_cost_features(const uint8_t *queries, const uint8_t *database,
+                      const uint32_t *query_positions, const uint32_t *database_positions,
+                      size_t n, int reverse, uint8_t *out);
 
 /* ---- host preparation pieces (tests) ---- */
 int gz_seg_mask(const char *sequence, char *masked);    /* SequenceSegFilter */

@@ -1,0 +1,212 @@
+/*
+ * device_emulation.cpp -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Runs the product's host flow (AlignQueries: query chunking, hash lists,
+ * per-query seed order, chain filter, schedule, AddResults, BuildResults,
+ * WriteOutput) with its two device stages emulated on the CPU by the
+ * oracle's leaf functions:
+ *   SeedHits     -- every (list, representative/value, member, entry) unit
+ *                   the device seed kernel evaluates, in any order, with the
+ *                   same distance / ungapped-trigger tests (gzo_distance,
+ *                   gzo_ungapped_with_trigger);
+ *   GappedExtend -- gzo_gapped_one_side per job.
+ * This lets CPU tests (and sanitizer builds) exercise the product's host
+ * code without a GPU.  The product library never contains this file.
+ */
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "alphabet.h"
+#include "database.h"
+#include "ghostz_hip.h"
+#include "gz_oracle.h"
+#include "presearch.h"
+#include "queries.h"
+
+using namespace gz;
+
+namespace {
+
+class CpuDevice : public PresearchDevice {
+ public:
+  CpuDevice(const Database &db, const SearchParameters &p) : db_(db), p_(p) {}
+
+  int BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &concat,
+                      const QueryLists &lists, const std::vector<int> &cut,
+                      const std::vector<int> &trig, int presearch_cutoff,
+                      AlignStats *) override {
+    (void)q;
+    concat_ = &concat;
+    lists_ = &lists;
+    cut_ = cut;
+    trig_ = trig;
+    presearch_cutoff_ = presearch_cutoff;
+    return 0;
+  }
+
+  int SeedHits(size_t ci, std::vector<SeedHit> *hits, AlignStats *st) override {
+    const DbChunk &c = db_.chunks[ci];
+    const uint8_t *qs = concat_->data();
+    const uint8_t *ds = c.seq.data();
+    const uint32_t L = db_.subsequence_length;
+    const float rate = 0.8f;
+    const char threshold = L - rate * L;
+    const QueryLists &ql = *lists_;
+    hits->clear();
+    for (uint32_t l = 0; l < ql.NumberLists(); ++l) {
+      const uint32_t h = ql.hash[l];
+      const auto it = std::lower_bound(c.hashes.begin(), c.hashes.end(), h);
+      if (it == c.hashes.end() || *it != h) continue;
+      const uint32_t hi = (uint32_t)(it - c.hashes.begin());
+      const uint32_t eb = ql.entry_off[l], ee = ql.entry_off[l + 1];
+      const uint32_t nrep = c.clustering ? c.rep_off[hi + 1] - c.rep_off[hi] : 0;
+      for (uint32_t r = 0; r < nrep; ++r) {
+        const uint32_t ri = c.rep_off[hi] + r;
+        const uint32_t dpos = c.rep_pos[ri];
+        for (uint32_t e = eb; e < ee; ++e) {
+          st->units_rep += 1;
+          const uint32_t qpos = ql.entry_pos[e], qid = ql.entry_qid[e];
+          const bool check = ql.entry_check[e] != 0;
+          const int d = check ? gzo_distance(qs, qpos, ds, dpos, L, db_.reduced_map) : 0;
+          SeedHit sh = {qid, qpos, dpos, l, r, 0, e - eb};
+          if (d <= threshold && Passes(qs, qpos, ds, dpos, qid)) hits->push_back(sh);
+          if (!check || std::max(d - db_.max_mismatches, 0) <= threshold) {
+            for (uint32_t m = c.mem_off[ri]; m < c.mem_off[ri + 1]; ++m) {
+              SeedHit mh = sh;
+              mh.dpos = c.mem_pos[m];
+              mh.member = 1 + (m - c.mem_off[ri]);
+              if (Passes(qs, qpos, ds, mh.dpos, qid)) hits->push_back(mh);
+            }
+          }
+        }
+      }
+      for (uint32_t v = c.val_off[hi]; v < c.val_off[hi + 1]; ++v) {
+        const uint32_t dpos = c.val_pos[v];
+        for (uint32_t e = eb; e < ee; ++e) {
+          st->units_value += 1;
+          const uint32_t qpos = ql.entry_pos[e], qid = ql.entry_qid[e];
+          SeedHit sh = {qid, qpos, dpos, l, nrep + (v - c.val_off[hi]), 0, e - eb};
+          if (Passes(qs, qpos, ds, dpos, qid)) hits->push_back(sh);
+        }
+      }
+    }
+    // the device returns seeds in no particular order
+    std::reverse(hits->begin(), hits->end());
+    return 0;
+  }
+
+  int GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, uint32_t *dpos,
+                   int *scores, uint32_t *cells, AlignStats *) override {
+    const uint8_t *qs = concat_->data();
+    const uint8_t *ds = db_.chunks[ci].seq.data();
+    for (uint32_t k = 0; k < nr + nf; ++k) {
+      const int reverse = k < nr ? 1 : 0;
+      int s = 0, p0 = 0, p1 = 0;
+      uint32_t ne = 0;
+      uint64_t c = 0;
+      gzo_gapped_one_side(qs + qpos[k], (uint32_t)concat_->size(), ds + dpos[k], kDelimiter,
+                          reverse, p_.matrix, p_.nl, p_.gap_open, p_.gap_extension,
+                          presearch_cutoff_, &s, &p0, &p1, nullptr, &ne, &c);
+      qpos[k] += p0;
+      dpos[k] += p1;
+      scores[k] = s;
+      if (cells) cells[k] = (uint32_t)c;
+    }
+    return 0;
+  }
+
+  void EndQueryChunk() override {}
+
+ private:
+  bool Passes(const uint8_t *qs, uint32_t qpos, const uint8_t *ds, uint32_t dpos,
+              uint32_t qid) const {
+    return gzo_ungapped_with_trigger(qs, qpos, ds, dpos, kDelimiter, p_.matrix, p_.nl,
+                                     cut_[qid], trig_[qid]) > trig_[qid];
+  }
+
+  const Database &db_;
+  const SearchParameters &p_;
+  const std::vector<uint8_t> *concat_ = nullptr;
+  const QueryLists *lists_ = nullptr;
+  std::vector<int> cut_, trig_;
+  int presearch_cutoff_ = 0;
+};
+
+std::vector<FastaRecord> Records(const char *const *names, const char *residues,
+                                 const uint64_t *offsets, uint64_t n) {
+  std::vector<FastaRecord> r(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    r[i].name = names[i];
+    r[i].sequence.assign(residues + offsets[i], residues + offsets[i + 1]);
+  }
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* The product's host flow with CPU-emulated device stages.  stats[0..5] =
+ * seed hits, chained hits, gapped extensions, gapped cells, results,
+ * sequential build chunks.  Returns 0 on success; *output is malloc'd. */
+int gzh_align(const char *const *qnames, const char *qres, const uint64_t *qoffs, uint64_t nq,
+              int translated, int filter, const char *const *dnames, const char *dres,
+              const uint64_t *doffs, uint64_t nd, int clustering, uint64_t db_chunk_size,
+              const gz_aligner_parameters *prm, char **output, size_t *output_length,
+              uint64_t *stats) {
+  std::vector<int> m;
+  if (!ParseScoreMatrix(Blosum62Text(), m)) return 1;
+  QueryOptions qo;
+  qo.translated = translated != 0;
+  qo.filter = filter != 0;
+  QuerySet q;
+  BuildQueries(Records(qnames, qres, qoffs, nq), qo, m.data(), kNumberLetters, &q);
+  DatabaseOptions dopt;
+  dopt.clustering = clustering != 0;
+  if (db_chunk_size) dopt.chunk_size = db_chunk_size;
+  Database db;
+  BuildDatabase(Records(dnames, dres, doffs, nd), dopt, m.data(), kNumberLetters, &db);
+  AlignerOptions opt;
+  opt.search.matrix = m.data();
+  opt.search.nl = kNumberLetters;
+  opt.search.gap_open = prm->gap_open;
+  opt.search.gap_extension = prm->gap_extension;
+  opt.search.normalized_ungapped_cutoff = prm->normalized_ungapped_cutoff;
+  opt.search.normalized_gapped_trigger = prm->normalized_gapped_trigger;
+  opt.search.normalized_presearch_cutoff = prm->normalized_presearch_cutoff;
+  opt.search.normalized_result_cutoff = prm->normalized_result_cutoff;
+  opt.search.max_results = prm->max_results;
+  opt.search.max_results_per_subject = prm->max_results_per_subject;
+  opt.queries_chunk_size = prm->queries_chunk_size ? prm->queries_chunk_size : (1ull << 27);
+  opt.threads = prm->threads;
+  opt.schedule = prm->schedule == GZ_SCHEDULE_LENGTH_SORTED ? Schedule::kLengthSorted
+                                                            : Schedule::kPlain;
+  opt.record_cells = prm->record_traces != 0;
+  CpuDevice dev(db, opt.search);
+  std::string out;
+  AlignStats st;
+  std::vector<GappedTrace> traces;
+  const int r = AlignQueries(q, db, opt, &dev, &out, &st, &traces, nullptr);
+  if (r != 0) return r;
+  char *buf = (char *)malloc(out.size() + 1);
+  memcpy(buf, out.data(), out.size());
+  buf[out.size()] = 0;
+  *output = buf;
+  *output_length = out.size();
+  if (stats) {
+    stats[0] = st.seed_hits;
+    stats[1] = st.chained_hits;
+    stats[2] = st.gapped_extensions;
+    stats[3] = st.gapped_cells;
+    stats[4] = st.results;
+    stats[5] = st.sequential_build_chunks;
+  }
+  return 0;
+}
+
+void gzh_free(void *p) { free(p); }
+
+}  // extern "C"

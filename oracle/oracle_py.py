@@ -150,3 +150,148 @@ def gapped_one_side(lib_kind, seq, start, seq0_len, db, dstart, delim, reversed_
         extra = None
     ed = edits[:n.value].tolist() if traceback else None
     return s.value, p0.value, p1.value, ed, extra
+
+
+# ---- whole-search oracle (gzo_align) ------------------------------------------
+from ctypes import Structure, POINTER as _P  # noqa: E402
+
+
+class _Chunk(Structure):
+    _fields_ = [("sequence", c_void_p), ("sequence_length", c_uint32),
+                ("number_sequences", c_uint32), ("offsets", c_void_p), ("names", c_void_p),
+                ("number_hashes", c_uint32), ("hashes", c_void_p), ("rep_offsets", c_void_p),
+                ("rep_positions", c_void_p), ("rep_member_offsets", c_void_p),
+                ("member_positions", c_void_p), ("value_offsets", c_void_p),
+                ("value_positions", c_void_p), ("clustering", c_int)]
+
+
+class _Database(Structure):
+    _fields_ = [("number_chunks", c_uint32), ("chunks", c_void_p), ("total_length", c_uint64),
+                ("total_sequences", c_uint64), ("subsequence_length", c_uint32),
+                ("reduced_code_map", c_void_p), ("hash_code_map", c_void_p),
+                ("hash_code_scores", c_void_p), ("hash_max_code", c_uint32),
+                ("hash_score_threshold", c_int), ("hash_max_length", c_uint32),
+                ("hash_shift_size", c_uint32), ("max_mismatches", c_int)]
+
+
+class _Queries(Structure):
+    _fields_ = [("number_queries", c_uint32), ("sequences", c_void_p), ("offsets", c_void_p),
+                ("names", c_void_p), ("lambda_", c_void_p), ("K", c_void_p),
+                ("translated", c_int), ("dna_lengths", c_void_p)]
+
+
+class _Parameters(Structure):
+    _fields_ = [("matrix", c_void_p), ("number_letters", c_uint32), ("delimiter", c_uint8),
+                ("gap_open", c_int), ("gap_extension", c_int),
+                ("normalized_ungapped_cutoff", c_float), ("normalized_gapped_trigger", c_float),
+                ("normalized_presearch_cutoff", c_float), ("normalized_result_cutoff", c_float),
+                ("max_results", c_uint32), ("max_results_per_subject", c_uint32)]
+
+
+class _Stats(Structure):
+    _fields_ = [(n, c_uint64) for n in ("hash_lists", "distance_calculations",
+                                         "ungapped_extensions", "hits",
+                                         "hits_after_chain_filter", "gapped_extensions",
+                                         "stale_traceback_reads")]
+
+
+def align(queries_view, chunks, info, matrix, number_letters, gap_open=11, gap_extension=1,
+          cutoffs=(7.0, 22.0, 15.0, 25.0), max_results=10, max_results_per_subject=1):
+    """Runs the oracle search on already-prepared inputs: queries_view as
+    returned by gz_native.Queries.view(), chunks as gz_native.Database.chunk()
+    dicts and info a gz_native.DatabaseInfo.  Returns (text bytes, stats dict)."""
+    keep = []
+
+    def arr(a, dtype):
+        a = np.ascontiguousarray(a, dtype)
+        keep.append(a)
+        return a.ctypes.data if a.size else None
+
+    def names(lst):
+        a = (c_char_p * max(len(lst), 1))(*lst)
+        keep.append(a)
+        return ctypes.cast(a, c_void_p)
+
+    cs = (_Chunk * max(len(chunks), 1))()
+    for i, ch in enumerate(chunks):
+        cs[i] = _Chunk(arr(ch["sequence"], np.uint8), len(ch["sequence"]),
+                       ch["number_sequences"], arr(ch["offsets"], np.uint32), names(ch["names"]),
+                       len(ch["hashes"]), arr(ch["hashes"], np.uint32),
+                       arr(ch["rep_offsets"], np.uint32), arr(ch["rep_positions"], np.uint32),
+                       arr(ch["rep_member_offsets"], np.uint32),
+                       arr(ch["member_positions"], np.uint32),
+                       arr(ch["value_offsets"], np.uint32),
+                       arr(ch["value_positions"], np.uint32), int(ch["clustering"]))
+    rmap = np.array(list(info.reduced_code_map), np.uint8)
+    hmap = np.array(list(info.hash_code_map), np.uint8)
+    hsc = np.array(list(info.hash_code_scores), np.int32)
+    db = _Database(len(chunks), ctypes.cast(cs, c_void_p), info.total_length,
+                   info.total_sequences, info.subsequence_length, arr(rmap, np.uint8),
+                   arr(hmap, np.uint8), arr(hsc, np.int32), info.hash_max_code,
+                   info.hash_score_threshold, info.hash_max_length, info.hash_shift,
+                   info.max_mismatches)
+    qv = queries_view
+    q = _Queries(int(qv["number_queries"]), arr(qv["codes"], np.uint8),
+                 arr(qv["offsets"], np.uint64), names(qv["names"]),
+                 arr(qv["lambda"], np.float32), arr(qv["K"], np.float32),
+                 int(qv["translated"]),
+                 arr(qv["dna_lengths"], np.uint32) if qv["translated"] else None)
+    p = _Parameters(arr(matrix, np.int32), number_letters, 25, gap_open, gap_extension,
+                    *cutoffs, max_results, max_results_per_subject)
+    out = c_void_p()
+    n = c_size_t()
+    rc = oracle().gzo_align(ctypes.byref(q), ctypes.byref(db), ctypes.byref(p),
+                            ctypes.byref(out), ctypes.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"gzo_align failed ({rc})")
+    text = ctypes.string_at(out, n.value) if n.value else b""
+    oracle().gzo_free(out)
+    st = _Stats()
+    oracle().gzo_last_stats(ctypes.byref(st))
+    return text, {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+
+# ---- the product's host flow with emulated device stages (libhostpipe.so) ----
+HOSTPIPE_PATH = os.path.join(HERE, "libhostpipe.so")
+_hostpipe = None
+
+
+def hostpipe(path=None):
+    global _hostpipe
+    if _hostpipe is None or path:
+        L = ctypes.CDLL(path or HOSTPIPE_PATH)
+        L.gzh_align.restype = c_int
+        L.gzh_align.argtypes = [c_void_p, c_char_p, c_void_p, c_uint64, c_int, c_int,
+                                c_void_p, c_char_p, c_void_p, c_uint64, c_int, c_uint64,
+                                c_void_p, _P(c_void_p), _P(c_size_t), c_void_p]
+        L.gzh_free.argtypes = [c_void_p]
+        if path:
+            return L
+        _hostpipe = L
+    return _hostpipe
+
+
+def host_align(queries, database, params, translated=False, filter=True, clustering=True,
+               db_chunk_size=0, lib=None):
+    """queries/database = (names, residues bytes, offsets); params a
+    gz_native.AlignerParameters.  Returns (text, stats dict)."""
+    L = lib or hostpipe()
+    (qn, qr, qo), (dn, dr, do) = queries, database
+    qna = (c_char_p * len(qn))(*[n.encode() for n in qn])
+    dna = (c_char_p * len(dn))(*[n.encode() for n in dn])
+    qo = np.ascontiguousarray(qo, np.uint64)
+    do = np.ascontiguousarray(do, np.uint64)
+    out = c_void_p()
+    n = c_size_t()
+    st = np.zeros(6, np.uint64)
+    rc = L.gzh_align(ctypes.cast(qna, c_void_p), qr, ptr(qo), len(qn), int(translated),
+                     int(filter), ctypes.cast(dna, c_void_p), dr, ptr(do), len(dn),
+                     int(clustering), db_chunk_size, ctypes.byref(params), ctypes.byref(out),
+                     ctypes.byref(n), ptr(st))
+    if rc != 0:
+        raise RuntimeError(f"gzh_align failed ({rc})")
+    text = ctypes.string_at(out, n.value) if n.value else b""
+    L.gzh_free(out)
+    keys = ["seed_hits", "chained_hits", "gapped_extensions", "gapped_cells", "results",
+            "sequential_build_chunks"]
+    return text, dict(zip(keys, (int(x) for x in st)))

@@ -1,0 +1,108 @@
+"""Device MLP (bf16 MFMA, fp32 accumulation) against the torch CPU reference.
+
+Tolerances: bf16 operands with fp32 accumulation in a different order than
+torch's; a rounding boundary crossed by an activation moves it by one bf16
+ulp (2^-8 relative).  Forward outputs agree within 1e-2 of the output scale,
+gradients within 3e-2 relative (L2, per tensor), the Adam update within
+1e-6 absolute.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gz_native as gz
+import mlp_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def device():
+    gz.check(gz.lib().gz_set_device(0))
+
+
+def random_codes(rng, n):
+    c = rng.integers(0, 21, (n, 128)).astype(np.uint8)
+    c[rng.random((n, 128)) < 0.05] = 20
+    return c
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 1000, 70001])
+def test_forward_matches_reference(device, n):
+    say("forward n", n)
+    rng = np.random.default_rng(100 + n)
+    flat = gz.Mlp.init_weights(3)
+    flat[-1] = 0.5
+    flat[21 * 8 + 256 * 1024:21 * 8 + 256 * 1024 + 256] = rng.normal(0, 0.1, 256)
+    mlp = gz.Mlp(flat)
+    codes = random_codes(rng, n)
+    got = mlp.predict(codes)
+    want, _, _ = mlp_oracle.forward_backward(flat, codes)
+    scale = np.abs(want).max() + 1e-3
+    err = np.abs(got - want).max()
+    say("  max err", err, "scale", scale)
+    assert err <= 1e-2 * scale
+
+
+def test_gradients_match_reference(device):
+    rng = np.random.default_rng(7)
+    flat = gz.Mlp.init_weights(5)
+    mlp = gz.Mlp(flat)
+    n = 300
+    codes = random_codes(rng, n)
+    targets = rng.normal(8, 2, n).astype(np.float32)
+    loss, grad = mlp.train_step(codes, targets, 0.0, want_grad=True)
+    _, rloss, rgrad = mlp_oracle.forward_backward(flat, codes, targets)
+    say("loss", loss, rloss)
+    assert abs(loss - rloss) <= 1e-3 * abs(rloss)
+    o = 0
+    for name, shape in mlp_oracle.SIZES:
+        k = int(np.prod(shape))
+        g, r = grad[o:o + k], rgrad[o:o + k]
+        rel = np.linalg.norm(g - r) / (np.linalg.norm(r) + 1e-12)
+        say("  grad", name, "rel", rel)
+        assert rel <= 3e-2, name
+        o += k
+    # lr == 0 leaves the weights alone
+    assert np.array_equal(mlp.weights(), flat)
+
+
+def test_adam_step_matches_reference(device):
+    rng = np.random.default_rng(8)
+    flat = gz.Mlp.init_weights(9)
+    mlp = gz.Mlp(flat)
+    codes = random_codes(rng, 128)
+    targets = rng.normal(6, 1, 128).astype(np.float32)
+    m = np.zeros_like(flat)
+    v = np.zeros_like(flat)
+    w = flat.copy()
+    for step in (1, 2, 3):
+        _, g = mlp.train_step(codes, targets, 1e-3, want_grad=True)
+        w, m, v = mlp_oracle.adam_step(w, g, m, v, step, 1e-3)
+        err = np.abs(mlp.weights() - w).max()
+        say("adam step", step, "max err", err)
+        assert err <= 1e-6
+
+
+def test_training_fits_a_learnable_target(device):
+    rng = np.random.default_rng(10)
+    mlp = gz.Mlp(gz.Mlp.init_weights(11))
+    codes = random_codes(rng, 4096)
+    # a smooth function of the first window's composition
+    t = ((codes[:, :32] < 5).sum(axis=1) * 0.25 + 3).astype(np.float32)
+    first = None
+    for it in range(300):
+        idx = rng.integers(0, len(codes), 256)
+        loss = mlp.train_step(codes[idx], t[idx], 1e-3)
+        first = loss if first is None else first
+    pred = mlp.predict(codes)
+    mae = np.abs(pred - t).mean()
+    base = np.abs(t - t.mean()).mean()
+    say("loss", first, "->", loss, "mae", mae, "constant-predictor mae", base)
+    assert mae < 0.5 * base
