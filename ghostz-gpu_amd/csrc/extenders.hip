@@ -744,7 +744,10 @@ static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
                            uint32_t *d_cells, hipStream_t s) {
   GZ_REQUIRE(h->queries && h->database && h->matrix,
              "gapped extender: queries, database and score parameters must be set");
-  if (!h->counters) GZ_HIP_CHECK(hipMalloc(&h->counters, 2 * sizeof(uint32_t)));
+  if (!h->counters) {
+    GZ_HIP_CHECK(hipMalloc(&h->counters, 2 * sizeof(uint32_t)));
+    GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, 2 * sizeof(uint32_t), s));
+  }
   if (h->overflow_capacity < size) {
     if (h->overflow_list) GZ_HIP_CHECK(hipFree(h->overflow_list));
     size_t cap = std::max<size_t>(size, 1 << 16);
@@ -758,7 +761,9 @@ static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
     GZ_HIP_CHECK(hipMalloc(&h->wide_codes, (size_t)kWideSlots * h->wide));
     h->wide_allocated = h->wide;
   }
-  GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, 2 * sizeof(uint32_t), s));
+  // counters[0]: overflow list length of this run; counters[1]: failures,
+  // kept until gz_gapped_extender_failures reads them
+  GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, sizeof(uint32_t), s));
   GappedParams P;
   P.queries = h->queries;
   P.database = h->database;
@@ -831,6 +836,7 @@ gz_status gz_gapped_extender_failures(gz_gapped_extender_t h, uint32_t *failures
     uint32_t c[2];
     GZ_HIP_CHECK(hipMemcpy(c, h->counters, sizeof(c), hipMemcpyDeviceToHost));
     *failures = c[1];
+    if (c[1] != 0) GZ_HIP_CHECK(hipMemset(h->counters + 1, 0, sizeof(uint32_t)));
   }
   return GZ_OK;
 }

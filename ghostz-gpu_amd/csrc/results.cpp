@@ -147,7 +147,10 @@ void AddResults(const DbChunk &chunk, int chunk_id, const SearchParameters &p,
     }
   }
   results.swap(merged);
-  for (size_t i = 0; i < results.size(); ++i) starts[results[i].subject].clear();
+  // only this chunk's subjects were entered in `starts` (subject numbers of
+  // results kept from earlier chunks index those chunks)
+  for (size_t i = 0; i < results.size(); ++i)
+    if (results[i].chunk == chunk_id) starts[results[i].subject].clear();
 }
 
 // ---------------------------------------------------------------------------

@@ -15,6 +15,15 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "ref: needs oracle/_ref built from the reference sources")
 
 
+@pytest.fixture(autouse=True)
+def _no_torch_in_gpu_tests(request):
+    # torch bundles its own HIP runtime; loaded first, it would serve the
+    # product library's HIP calls instead of the ROCm the library was built for
+    if request.node.get_closest_marker("gpu") is not None:
+        assert "torch" not in sys.modules, "a GPU test process imported torch"
+    yield
+
+
 def pytest_collection_modifyitems(config, items):
     import oracle_py
     skip_ref = pytest.mark.skip(reason="oracle/_ref not built (reference tree absent)")

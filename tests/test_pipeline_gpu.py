@@ -77,6 +77,9 @@ def run_case(name, queries, db, m, nl, schedules=(gz.SCHEDULE_PLAIN,), chunk_siz
     if chunk_size:
         params["queries_chunk_size"] = chunk_size
         kw["chunk_size"] = chunk_size
+    for k in ("max_results", "max_results_per_subject"):
+        if k in params:
+            kw[k] = params[k]
     want, ost = oracle_text(qv, db, m, nl, **kw)
     say("  oracle lines", want.count(b"\n"), ost)
     assert want.count(b"\n") > 0 or name.startswith("edge")
