@@ -44,7 +44,7 @@ def query_residues(view):
     return int(np.count_nonzero(c != 25))
 
 
-def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed):
+def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_000_000):
     """Logs extension traces with the plain schedule on training reads, then
     trains the MLP on the GPU.  Returns (flat weights, report dict)."""
     import synthetic
@@ -53,6 +53,8 @@ def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed):
     al = gz.Aligner(db, record_traces=1)
     al.align(q)
     tr = al.traces()
+    if len(tr) > max_traces:  # a uniform sample keeps training to seconds
+        tr = tr[np.sort(rng.choice(len(tr), max_traces, replace=False))]
     concat = al.last_concatenated_queries()
     chunk_seq = {c: db.chunk(c)["sequence"] for c in np.unique(tr[:, 2])} if len(tr) else {}
     feats = np.zeros((len(tr), 128), np.uint8)
@@ -120,6 +122,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="plain schedule only (for profiling runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also launch a 1 GiB streaming copy kernel (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the search "
@@ -130,6 +136,8 @@ def main():
     import synthetic
     gz.check(gz.lib().gz_set_device(0))
     m, nl = gz.blosum62()
+    log(f"host: os.cpu_count {os.cpu_count()}, affinity {len(os.sched_getaffinity(0))}, "
+        f"OMP_NUM_THREADS {os.environ.get('OMP_NUM_THREADS')}")
 
     n_db = max(1000, int(1_000_000 * args.scale))
     n_reads = max(1000, int(200_000 * args.scale))
@@ -148,15 +156,28 @@ def main():
     q_aa = query_residues(qv)
     log(f"host preparation {time.perf_counter() - t0:.1f} s: {q_aa} query aa")
 
+    if args.calibrate_copy:
+        nbytes = 1 << 30
+        src, dst = gz.DeviceBuffer(nbytes), gz.DeviceBuffer(nbytes)
+        for _ in range(3):
+            gz.check(gz.lib().gz_copy_kernel_async(dst.ptr, src.ptr, nbytes, None))
+        gz.check(gz.lib().gz_stream_synchronize(None))
+        src.free()
+        dst.free()
+        log("copy kernel: 3 x 1 GiB")
+
     t0 = time.perf_counter()
-    weights, learn = train_cost_model(gz, db, db_codes, np.random.default_rng(args.seed + 1),
+    weights, learn = (None, None) if args.no_learned else train_cost_model(gz, db, db_codes, np.random.default_rng(args.seed + 1),
                                       max(1000, int(args.train_reads * min(1.0, args.scale * 10))),
                                       args.epochs, args.seed)
     log(f"cost model trained in {time.perf_counter() - t0:.1f} s: {learn}")
 
     plain = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN)
-    learned = gz.Aligner(db, schedule=gz.SCHEDULE_LEARNED)
-    learned.set_cost_model(weights)
+    if args.no_learned:
+        learned = plain
+    else:
+        learned = gz.Aligner(db, schedule=gz.SCHEDULE_LEARNED)
+        learned.set_cost_model(weights)
 
     def run(al, k, label):
         outs, stats, times = [], [], []
@@ -173,9 +194,12 @@ def main():
         return outs, stats, times
 
     run(plain, args.warmup, "plain warmup")
-    run(learned, args.warmup, "learned warmup")
     p_out, p_st, p_t = run(plain, args.steps, "plain")
-    l_out, l_st, l_t = run(learned, args.steps, "learned")
+    if args.no_learned:
+        l_out, l_st, l_t = p_out, p_st, p_t
+    else:
+        run(learned, args.warmup, "learned warmup")
+        l_out, l_st, l_t = run(learned, args.steps, "learned")
     identical = all(o == p_out[0] for o in p_out + l_out)
     if not identical:
         log("ERROR: learned-scheduled output differs from the plain schedule")
@@ -235,7 +259,11 @@ def main():
                    "query_aa": q_aa, "schedule": "learned"},
         "plain_value": plain_value, "plain_ms_per_step": 1e3 * p_time / args.steps,
         "learned_over_plain": value / plain_value, "outputs_identical": identical,
-        "kernel_ms_per_step": kern, "stage_seconds": {k: round(v, 4) for k, v in st.items()
+        "kernel_ms_per_step": kern,
+        "search_stats": {k: st[k] for k in ("lists", "units_rep", "units_value", "seed_hits",
+                                             "chained_hits", "gapped_extensions", "results",
+                                             "sequential_build_chunks")},
+        "stage_seconds": {k: round(v, 4) for k, v in st.items()
                                                        if k.startswith("seconds_")},
         "cost_model": learn,
         "roofline": roof,

@@ -45,9 +45,10 @@ class Aligner : public PresearchDevice {
                       const QueryLists &lists, const std::vector<int> &cutoffs,
                       const std::vector<int> &triggers, int presearch_cutoff,
                       AlignStats *st) override;
-  int SeedHits(size_t chunk, std::vector<SeedHit> *hits, AlignStats *st) override;
+  int SeedHits(size_t chunk, uint32_t nq, SeedHitView *hits, AlignStats *st) override;
   int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward, uint32_t *qpos,
-                   uint32_t *dpos, int *scores, uint32_t *cells, AlignStats *st) override;
+                   uint32_t *dpos, int *scores, uint32_t *cells, const uint16_t *keys,
+                   bool learned, AlignStats *st) override;
   void EndQueryChunk() override;
 
  private:
@@ -66,7 +67,10 @@ class Aligner : public PresearchDevice {
   DeviceQueryLists dq_;
   SeedStageParams sp_;
   std::vector<void *> query_buffers_;
-  hipEvent_t ev_[2] = {nullptr, nullptr};
+  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  // gapped-stage workspace (grown on demand)
+  void *gapped_ws_ = nullptr;
+  size_t gapped_ws_cap_ = 0;
 };
 
 }  // namespace gz

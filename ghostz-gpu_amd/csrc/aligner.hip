@@ -5,6 +5,9 @@
 #include <cstring>
 
 #include "aligner.h"
+#include "cost_model_gpu.h"
+#include "gz_scan.h"
+#include "schedule.h"
 #include "cost_model.h"
 #include "gz_runtime.h"
 #include "gz_common.h"
@@ -63,6 +66,7 @@ Aligner::Aligner(const AlignerOptions &opt) : opt_(opt) {}
 Aligner::~Aligner() {
   EndQueryChunk();
   chunks_.clear();
+  if (gapped_ws_) (void)hipFree(gapped_ws_);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
   if (gapped_) gz_gapped_extender_destroy(gapped_);
@@ -176,7 +180,7 @@ void Aligner::EndQueryChunk() {
   dq_ = DeviceQueryLists();
 }
 
-int Aligner::SeedHits(size_t ci, std::vector<SeedHit> *hits, AlignStats *st) {
+int Aligner::SeedHits(size_t ci, uint32_t nq, SeedHitView *hits, AlignStats *st) {
   const DeviceChunk &dc = *chunks_[ci];
   DeviceDbChunk ddb;
   ddb.packed = dc.packed;
@@ -190,7 +194,7 @@ int Aligner::SeedHits(size_t ci, std::vector<SeedHit> *hits, AlignStats *st) {
   ddb.val_pos = dc.val_pos;
   sp_.clustering = db_->chunks[ci].clustering;
   SeedStageStats ss;
-  const hipError_t e = seed_.Run(dq_, ddb, sp_, stream_, hits, &ss);
+  const hipError_t e = seed_.Run(dq_, ddb, sp_, nq, stream_, hits, &ss);
   if (e != hipSuccess) return (int)e;
   st->units_rep += ss.units_rep;
   st->units_value += ss.units_value;
@@ -202,36 +206,99 @@ int Aligner::SeedHits(size_t ci, std::vector<SeedHit> *hits, AlignStats *st) {
 }
 
 int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, uint32_t *dpos,
-                          int *scores, uint32_t *cells, AlignStats *st) {
+                          int *scores, uint32_t *cells, const uint16_t *keys, bool learned,
+                          AlignStats *st) {
   const size_t n = (size_t)nr + nf;
   if (n == 0) return 0;
   hipStream_t s = stream_;
-  if (!ev_[0]) {
-    GZ_TRY(hipEventCreate(&ev_[0]));
-    GZ_TRY(hipEventCreate(&ev_[1]));
+  for (hipEvent_t &e : ev_)
+    if (!e) GZ_TRY(hipEventCreate(&e));
+  const DeviceChunk &dc = *chunks_[ci];
+  if (gz_gapped_extender_set_database(gapped_, dc.packed) != GZ_OK) return (int)hipErrorUnknown;
+  const GpuCostModel *model =
+      learned ? dynamic_cast<const GpuCostModel *>(opt_.cost_model) : nullptr;
+  const bool order = keys != nullptr || model != nullptr;
+  // workspace: positions, scores, cells; and for an ordered run the keys,
+  // the permutation, permuted copies, predicted costs and the sort scratch
+  const size_t a4 = AlignUp(n * 4, 256), a2 = AlignUp(n * 2, 256);
+  size_t bytes = 4 * a4;
+  if (order) bytes += a2 + 5 * a4 + a4 + AlignUp(ScheduleTempBytes(n), 256);
+  if (gapped_ws_cap_ < bytes) {
+    if (gapped_ws_) (void)hipFree(gapped_ws_);
+    gapped_ws_ = nullptr;
+    gapped_ws_cap_ = 0;
+    GZ_TRY(hipMalloc(&gapped_ws_, bytes));
+    gapped_ws_cap_ = bytes;
   }
-  if (gz_gapped_extender_set_database(gapped_, chunks_[ci]->packed) != GZ_OK)
-    return (int)hipErrorUnknown;
-  uint32_t *d_pq = nullptr, *d_pd = nullptr, *d_cells = nullptr;
-  int *d_sc = nullptr;
-  hipError_t e = hipMalloc((void **)&d_pq, n * 4);
-  if (e == hipSuccess) e = hipMalloc((void **)&d_pd, n * 4);
-  if (e == hipSuccess) e = hipMalloc((void **)&d_sc, n * 4);
-  if (e == hipSuccess && cells) e = hipMalloc((void **)&d_cells, n * 4);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_pq, qpos, n * 4, hipMemcpyHostToDevice, s);
+  char *w = (char *)gapped_ws_;
+  auto take = [&](size_t b) {
+    char *p = w;
+    w += b;
+    return p;
+  };
+  uint32_t *d_pq = (uint32_t *)take(a4), *d_pd = (uint32_t *)take(a4);
+  int *d_sc = (int *)take(a4);
+  uint32_t *d_cells = (uint32_t *)take(a4);
+  uint16_t *d_keys = nullptr;
+  uint32_t *d_perm = nullptr, *d_jq = nullptr, *d_jd = nullptr, *d_jc = nullptr;
+  int *d_js = nullptr;
+  float *d_pred = nullptr;
+  void *d_tmp = nullptr;
+  if (order) {
+    d_keys = (uint16_t *)take(a2);
+    d_perm = (uint32_t *)take(a4);
+    d_jq = (uint32_t *)take(a4);
+    d_jd = (uint32_t *)take(a4);
+    d_js = (int *)take(a4);
+    d_jc = (uint32_t *)take(a4);
+    d_pred = (float *)take(a4);
+    d_tmp = take(AlignUp(ScheduleTempBytes(n), 256));
+  }
+  hipError_t e = hipMemcpyAsync(d_pq, qpos, n * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_pd, dpos, n * 4, hipMemcpyHostToDevice, s);
+  uint32_t *rq = d_pq, *rd = d_pd, *rc = cells ? d_cells : nullptr;
+  int *rs = d_sc;
+  if (e == hipSuccess && order) {
+    // order the jobs by descending key inside each side, on the device
+    e = hipEventRecord(ev_[2], s);
+    if (e == hipSuccess && model) {
+      e = model->PredictOnDevice(dq_.packed, dc.packed, d_pq, d_pd, nr, true, d_pred, s);
+      if (e == hipSuccess)
+        e = model->PredictOnDevice(dq_.packed, dc.packed, d_pq + nr, d_pd + nr, nf, false,
+                                   d_pred + nr, s);
+      // log2 cells in steps of 1/32
+      if (e == hipSuccess) e = CostToKey(d_pred, (uint32_t)n, 32.f, d_keys, s);
+    } else if (e == hipSuccess) {
+      e = hipMemcpyAsync(d_keys, keys, n * 2, hipMemcpyHostToDevice, s);
+    }
+    if (e == hipSuccess) e = DeviceOrderByKey(d_keys, nr, d_perm, d_tmp, s);
+    if (e == hipSuccess) e = DeviceOrderByKey(d_keys + nr, nf, d_perm + nr, d_tmp, s);
+    if (e == hipSuccess) e = GatherPair(d_perm, nr, d_pq, d_pd, d_jq, d_jd, s);
+    if (e == hipSuccess)
+      e = GatherPair(d_perm + nr, nf, d_pq + nr, d_pd + nr, d_jq + nr, d_jd + nr, s);
+    if (e == hipSuccess) e = hipEventRecord(ev_[3], s);
+    rq = d_jq;
+    rd = d_jd;
+    rs = d_js;
+    rc = cells ? d_jc : nullptr;
+  }
   gz_stream_s wrap{s};
   if (e == hipSuccess) e = hipEventRecord(ev_[0], s);
   if (e == hipSuccess && nr > 0 &&
-      gz_gapped_extender_extend_one_side_score_only_cells_async(gapped_, nr, 1, d_pq, d_pd, d_sc,
-                                                                d_cells, &wrap) != GZ_OK)
+      gz_gapped_extender_extend_one_side_score_only_cells_async(gapped_, nr, 1, rq, rd, rs, rc,
+                                                                &wrap) != GZ_OK)
     e = hipErrorUnknown;
   if (e == hipSuccess && nf > 0 &&
       gz_gapped_extender_extend_one_side_score_only_cells_async(
-          gapped_, nf, 0, d_pq + nr, d_pd + nr, d_sc + nr, d_cells ? d_cells + nr : nullptr,
-          &wrap) != GZ_OK)
+          gapped_, nf, 0, rq + nr, rd + nr, rs + nr, rc ? rc + nr : nullptr, &wrap) != GZ_OK)
     e = hipErrorUnknown;
   if (e == hipSuccess) e = hipEventRecord(ev_[1], s);
+  if (e == hipSuccess && order) {
+    e = ScatterBack(d_perm, nr, d_jq, d_jd, d_js, rc, d_pq, d_pd, d_sc, rc ? d_cells : nullptr, s);
+    if (e == hipSuccess)
+      e = ScatterBack(d_perm + nr, nf, d_jq + nr, d_jd + nr, d_js + nr, rc ? rc + nr : nullptr,
+                      d_pq + nr, d_pd + nr, d_sc + nr, rc ? d_cells + nr : nullptr, s);
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(qpos, d_pq, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(dpos, d_pd, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(scores, d_sc, n * 4, hipMemcpyDeviceToHost, s);
@@ -240,14 +307,12 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
   uint32_t fails = 0;
   if (e == hipSuccess && (gz_gapped_extender_failures(gapped_, &fails) != GZ_OK || fails != 0))
     e = hipErrorUnknown;
-  float ms = 0;
+  float ms = 0, ms2 = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[0], ev_[1]);
-  (void)hipFree(d_pq);
-  (void)hipFree(d_pd);
-  (void)hipFree(d_sc);
-  if (d_cells) (void)hipFree(d_cells);
+  if (e == hipSuccess && order) e = hipEventElapsedTime(&ms2, ev_[2], ev_[3]);
   if (e != hipSuccess) return (int)e;
   st->gapped_kernel_ms += ms;
+  st->schedule_kernel_ms += ms2;
   st->gapped_launches += (nr > 0) + (nf > 0);
   return 0;
 }

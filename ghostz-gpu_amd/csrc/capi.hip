@@ -337,6 +337,7 @@ gz_status gz_aligner_align(gz_aligner_t h, gz_queries_t q, char **output,
     stats->seed_rep_launches = st.seed_launches[0];
     stats->seed_value_launches = st.seed_launches[1];
     stats->gapped_launches = st.gapped_launches;
+    stats->schedule_kernel_ms = st.schedule_kernel_ms;
   }
   return GZ_OK;
 }

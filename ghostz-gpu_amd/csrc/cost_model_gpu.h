@@ -4,6 +4,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <hip/hip_runtime.h>
+
 #include "cost_model.h"
 
 namespace gz {
@@ -11,12 +13,23 @@ namespace gz {
 bool UnflattenWeights(const float *flat, size_t count, CostModelWeights *w);
 size_t FlatWeightCount();
 
+// Feature codes of n one-sided jobs straight from the packed sequences (the
+// same codes CostFeatures produces from unpacked ones): out is n x 128.
+hipError_t CostFeaturesOnDevice(const uint64_t *packed_queries, const uint64_t *packed_database,
+                                const uint32_t *qpos, const uint32_t *dpos, uint32_t n,
+                                bool reverse, uint8_t *out, hipStream_t stream);
+
 class GpuCostModel : public CostModel {
  public:
   explicit GpuCostModel(const CostModelWeights &w);
   ~GpuCostModel() override;
   void Predict(const uint8_t *queries, const uint8_t *database, const uint32_t *qpos,
                const uint32_t *dpos, size_t n, bool reverse, float *out) const override;
+  // Predicted log2 cells of n jobs whose positions are on the device; all
+  // work is queued on `stream` (features, fused MLP); out is a device array.
+  hipError_t PredictOnDevice(const uint64_t *packed_queries, const uint64_t *packed_database,
+                             const uint32_t *d_qpos, const uint32_t *d_dpos, uint32_t n,
+                             bool reverse, float *d_out, hipStream_t stream) const;
 
  private:
   struct Impl;

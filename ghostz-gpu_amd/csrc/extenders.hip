@@ -474,6 +474,25 @@ gz_status gz_memset_async(void *dst, int value, size_t bytes,
 
 size_t gz_packed_words(size_t codes) { return PackedWords(codes); }
 
+// Streaming copy in 16-byte vectors: the reference point for HBM counters
+// (bytes fetched / written by a kernel of known traffic).
+__global__ void CopyKernel(const uint4 *src, uint4 *dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+gz_status gz_copy_kernel_async(void *dst, const void *src, size_t bytes, gz_stream_t stream) {
+  GZ_REQUIRE(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
+             "gz_copy_kernel_async: 16-byte aligned sizes and pointers only");
+  if (bytes == 0) return GZ_OK;
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(CopyKernel, dim3(GridFor(n, 256, 8192)), dim3(256), 0, S(stream),
+                     (const uint4 *)src, (uint4 *)dst, n);
+  GZ_HIP_CHECK(hipGetLastError());
+  return GZ_OK;
+}
+
 gz_status gz_pack_sequence_async(const uint8_t *d_codes, size_t n,
                                  uint64_t *d_packed, gz_stream_t stream) {
   GZ_REQUIRE(d_packed != nullptr, "gz_pack_sequence_async: null output");

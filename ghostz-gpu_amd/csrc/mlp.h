@@ -27,6 +27,9 @@ class MlpDevice {
   // returns the flat gradient (same layout as the flat weights).
   hipError_t TrainStep(const uint8_t *codes, const float *targets, size_t n, float lr,
                        float *loss, float *flat_grad);
+  // Forward on device-resident codes (n x 128) into device outputs, on the
+  // caller's stream (fused persistent inference kernels).
+  hipError_t InferDevice(const uint8_t *d_codes, size_t n, float *d_out, hipStream_t stream);
   // Kernel-only forward time of the last Forward call (HIP events).
   float last_forward_ms = 0;
 
@@ -56,6 +59,8 @@ class MlpDevice {
   uint16_t *dz1h_ = nullptr;
   float *dh1_ = nullptr;        // [B][256]
   float *dx_ = nullptr;         // [B][1024]
+  uint16_t *infer_h1_ = nullptr;  // inference layer-1 output, chunk x 256 bf16
+  size_t infer_h1_cap_ = 0;
 };
 
 }  // namespace gz

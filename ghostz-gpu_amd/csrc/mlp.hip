@@ -13,6 +13,8 @@
 #include <vector>
 
 #include "cost_model.h"
+#include "cost_model_gpu.h"
+#include "gz_runtime.h"
 #include "gz_common.h"
 #include "mlp.h"
 
@@ -137,31 +139,40 @@ __global__ void LossKernel(int B, int n, const float *y, const float *t, float *
   if (threadIdx.x == 0) loss[0] = part[0] / n;
 }
 
-// dW3 / dB3 and dZ2 = dY w3 * gelu'(Z2)
-__global__ void OutputBackwardKernel(int B, const float *dy, const uint16_t *h2,
-                                     const float *z2, const float *w3, float *gw3,
-                                     float *gb3, float *dz2, uint16_t *dz2h) {
-  const int k = threadIdx.x;  // 64 threads
-  float s = 0.f, sb = 0.f;
-  const float w = Bf2F(F2Bf(w3[k]));
-  for (int b = 0; b < B; ++b) {
-    s += dy[b] * Bf2F(h2[b * kCostH2 + k]);
-    sb += dy[b];
-    const float d = dy[b] * w * GeluGrad(z2[b * kCostH2 + k]);
-    dz2[b * kCostH2 + k] = d;
-    dz2h[b * kCostH2 + k] = F2Bf(d);
-  }
-  gw3[k] = s;
-  if (k == 0) gb3[0] = sb;
+// dZ2 = dY w3 * gelu'(Z2), elementwise over B x 64
+__global__ void OutputBackwardKernel(int B, const float *dy, const float *z2, const float *w3,
+                                     float *dz2, uint16_t *dz2h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * kCostH2) return;
+  const int b = i / kCostH2, k = i % kCostH2;
+  const float d = dy[b] * Bf2F(F2Bf(w3[k])) * GeluGrad(z2[i]);
+  dz2[i] = d;
+  dz2h[i] = F2Bf(d);
 }
 
-// column sums of a [B][N] matrix into grad[N]
-__global__ void ColumnSumKernel(int B, int N, const float *m, float *out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
+// out[c] += sum_b wt[b] * M[b][c] (wt optional); M is fp32 or bf16.  Blocks
+// own 64 columns and a slice of rows; partial sums are added atomically.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    ColumnSumKernel(int B, int N, const T *m, const float *wt, float *out) {
+  __shared__ float part[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int lane_row = threadIdx.x >> 6;
+  const int rows = (B + gridDim.y - 1) / gridDim.y;
+  const int r0 = blockIdx.y * rows, r1 = min(B, r0 + rows);
   float s = 0.f;
-  for (int b = 0; b < B; ++b) s += m[(long)b * N + c];
-  out[c] = s;
+  if (c < N)
+    for (int b = r0 + lane_row; b < r1; b += 4) {
+      float v;
+      if constexpr (sizeof(T) == 2) v = Bf2F((uint16_t)m[(long)b * N + c]);
+      else v = (float)m[(long)b * N + c];
+      s += wt ? wt[b] * v : v;
+    }
+  part[lane_row][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (threadIdx.x < 64 && c < N)
+    atomicAdd(&out[c], part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                           part[3][threadIdx.x]);
 }
 
 // d_embed[a][e] = sum over (b, r) with code a of dX[b][r*8 + e]
@@ -201,6 +212,157 @@ __global__ void ToBf16Kernel(size_t n, const float *src, uint16_t *dst) {
 
 inline unsigned Blocks(size_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// ---------------------------------------------------------------------------
+// Inference: two persistent kernels.
+//
+// Layer 1 keeps a 64-output slice of W1 (64 x 1024 bf16 = 128 KB) resident in
+// LDS and streams tiles of 128 seeds through it; the embedding lookup is done
+// while building the MFMA A operand (4 bf16 of one residue's 8-dim vector per
+// lane).  Each wave owns 32 seeds x 64 outputs (two 32x32 accumulators that
+// share the A operand).  The head kernel keeps W2 (32 KB) in LDS, stages a
+// 128 x 256 bf16 H1 tile, runs layer 2 on MFMA and reduces layer 3 across
+// lanes.  Rounding points are those of the training forward.
+// ---------------------------------------------------------------------------
+constexpr int kInferTile = 128;
+constexpr int kInferSlice = 64;
+
+typedef short short4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short4v Load4(const uint16_t *p) {
+  const uint2 v = *reinterpret_cast<const uint2 *>(p);
+  short4v r;
+  r[0] = (short)(v.x & 0xffff);
+  r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xffff);
+  r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+__global__ void __launch_bounds__(256)
+    InferLayer1Kernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h,
+                      const uint16_t *embh, const float *b1, uint16_t *h1) {
+  // W1 slice as [k/4][64][4]: the B operand of a lane is one 8-byte read
+  __shared__ __align__(16) uint16_t w[kCostIn * kInferSlice];
+  // codes transposed [position][seed]
+  __shared__ __align__(16) uint8_t ct[kCostResidues * kInferTile];
+  __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
+  const int slice = blockIdx.x & 3;
+  const int group = blockIdx.x >> 2, groups = gridDim.x >> 2;
+  for (int idx = threadIdx.x; idx < kInferSlice * (kCostIn / 4); idx += 256) {
+    const int nn = idx / (kCostIn / 4), k4 = idx % (kCostIn / 4);
+    const uint2 v = *reinterpret_cast<const uint2 *>(
+        w1h + (size_t)(slice * kInferSlice + nn) * kCostIn + k4 * 4);
+    *reinterpret_cast<uint2 *>(&w[(k4 * kInferSlice + nn) * 4]) = v;
+  }
+  for (int i = threadIdx.x; i < kCostAlphabet * kCostEmbed; i += 256) emb[i] = embh[i];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int kq = lane >> 5, row_in = lane & 31;
+  const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
+  for (uint32_t t = group; t < tiles; t += groups) {
+    __syncthreads();  // previous tile's codes fully consumed (and W1 loaded)
+    const uint32_t s0 = t * kInferTile;
+    for (int idx = threadIdx.x; idx < kInferTile * (kCostResidues / 4); idx += 256) {
+      const int i = idx / (kCostResidues / 4), r4 = idx % (kCostResidues / 4);
+      uint32_t v = 0x14141414u;  // code 20 for seeds past the end
+      if (s0 + i < n)
+        v = *reinterpret_cast<const uint32_t *>(codes + (size_t)(s0 + i) * kCostResidues + r4 * 4);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t c = (v >> (8 * j)) & 0xffu;
+        ct[(r4 * 4 + j) * kInferTile + i] = (uint8_t)(c < 20 ? c : 20);
+      }
+    }
+    __syncthreads();
+    float16_t acc0, acc1;
+    for (int r = 0; r < 16; ++r) {
+      acc0[r] = 0.f;
+      acc1[r] = 0.f;
+    }
+    const int mrow = wave * 32 + row_in;
+    for (int k0 = 0; k0 < kCostIn; k0 += 8) {
+      const int code = ct[(k0 >> 3) * kInferTile + mrow];
+      const short4v a = Load4(&emb[code * kCostEmbed + kq * 4]);
+      const int kb = (k0 >> 2) + kq;
+      const short4v b0 = Load4(&w[(kb * kInferSlice + row_in) * 4]);
+      const short4v b1v = Load4(&w[(kb * kInferSlice + 32 + row_in) * 4]);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1v, acc1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const uint32_t seed = s0 + wave * 32 + 8 * (r >> 2) + 4 * kq + (r & 3);
+      if (seed < n) {
+        const int c0 = slice * kInferSlice + row_in;
+        h1[(size_t)seed * kCostH1 + c0] = F2Bf(Gelu(acc0[r] + b1[c0]));
+        h1[(size_t)seed * kCostH1 + c0 + 32] = F2Bf(Gelu(acc1[r] + b1[c0 + 32]));
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    InferHeadKernel(const uint16_t *h1, uint32_t n, const uint16_t *w2h, const float *b2,
+                    const float *w3, const float *b3, float *y) {
+  __shared__ __align__(16) uint16_t w[kCostH1 * kCostH2];         // [k/4][64][4]
+  __shared__ __align__(16) uint16_t ht[kCostH1 * kInferTile];     // [k/4][128][4]
+  __shared__ float w3b[kCostH2], bb[kCostH2];
+  for (int idx = threadIdx.x; idx < kCostH2 * (kCostH1 / 4); idx += 256) {
+    const int nn = idx / (kCostH1 / 4), k4 = idx % (kCostH1 / 4);
+    *reinterpret_cast<uint2 *>(&w[(k4 * kCostH2 + nn) * 4]) =
+        *reinterpret_cast<const uint2 *>(w2h + (size_t)nn * kCostH1 + k4 * 4);
+  }
+  if (threadIdx.x < kCostH2) {
+    w3b[threadIdx.x] = Bf2F(F2Bf(w3[threadIdx.x]));
+    bb[threadIdx.x] = b2[threadIdx.x];
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int kq = lane >> 5, row_in = lane & 31;
+  const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    __syncthreads();
+    const uint32_t s0 = t * kInferTile;
+    for (int idx = threadIdx.x; idx < kInferTile * (kCostH1 / 4); idx += 256) {
+      const int i = idx / (kCostH1 / 4), k4 = idx % (kCostH1 / 4);
+      uint2 v = make_uint2(0, 0);
+      if (s0 + i < n)
+        v = *reinterpret_cast<const uint2 *>(h1 + (size_t)(s0 + i) * kCostH1 + k4 * 4);
+      *reinterpret_cast<uint2 *>(&ht[(k4 * kInferTile + i) * 4]) = v;
+    }
+    __syncthreads();
+    float16_t acc0, acc1;
+    for (int r = 0; r < 16; ++r) {
+      acc0[r] = 0.f;
+      acc1[r] = 0.f;
+    }
+    const int mrow = wave * 32 + row_in;
+    for (int k0 = 0; k0 < kCostH1; k0 += 8) {
+      const int kb = (k0 >> 2) + kq;
+      const short4v a = Load4(&ht[(kb * kInferTile + mrow) * 4]);
+      const short4v b0 = Load4(&w[(kb * kCostH2 + row_in) * 4]);
+      const short4v b1v = Load4(&w[(kb * kCostH2 + 32 + row_in) * 4]);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1v, acc1, 0, 0, 0);
+    }
+    // layer 3: per row, sum over the 64 columns (32 lanes x 2 accumulators)
+    float part[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int c0 = row_in;
+      part[r] = Bf2F(F2Bf(Gelu(acc0[r] + bb[c0]))) * w3b[c0] +
+                Bf2F(F2Bf(Gelu(acc1[r] + bb[c0 + 32]))) * w3b[c0 + 32];
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) part[r] += __shfl_xor(part[r], off, 64);
+    }
+    if (row_in == 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const uint32_t seed = s0 + wave * 32 + 8 * (r >> 2) + 4 * kq + (r & 3);
+        if (seed < n) y[seed] = part[r] + b3[0];
+      }
+    }
+  }
+}
+
 template <int EPI>
 void Gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, long sam, long sak,
           const uint16_t *B, long sbn, long sbk, float *C, int ldc, const float *bias = nullptr,
@@ -229,7 +391,7 @@ MlpDevice::MlpDevice() {
 MlpDevice::~MlpDevice() {
   void *ps[] = {master_, grad_, m_,   v_,    w1h_,  w2h_,  embh_, codes_, x_,
                 z1_,     h1_,   z2_,  h2_,   y_,    t_,    loss_, dz2_,   dz2h_,
-                dz1_,    dz1h_, dh1_, dx_};
+                dz1_,    dz1h_, dh1_, dx_,  infer_h1_};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -327,34 +489,62 @@ hipError_t MlpDevice::ForwardDevice(size_t n, size_t B) {
   return hipGetLastError();
 }
 
-hipError_t MlpDevice::Forward(const uint8_t *codes, size_t n, float *out) {
-  const size_t chunk = 1 << 16;
-  hipEvent_t e0, e1;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  last_forward_ms = 0;
-  for (size_t b0 = 0; b0 < n; b0 += chunk) {
-    const size_t m = std::min(chunk, n - b0);
-    const size_t B = (m + 63) / 64 * 64;
-    hipError_t e = Reserve(B);
+hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out,
+                                  hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t chunk = (size_t)1 << 20;
+  const size_t want = std::min(n, chunk) * kCostH1 * 2;
+  if (infer_h1_cap_ < want) {
+    if (infer_h1_) (void)hipFree(infer_h1_);
+    infer_h1_ = nullptr;
+    infer_h1_cap_ = 0;
+    hipError_t e = hipMalloc((void **)&infer_h1_, want);
     if (e != hipSuccess) return e;
-    (void)hipMemsetAsync(codes_, 20, B * kCostResidues, stream_);
-    (void)hipMemcpyAsync(codes_, codes + b0 * kCostResidues, m * kCostResidues,
-                         hipMemcpyHostToDevice, stream_);
-    (void)hipEventRecord(e0, stream_);
-    e = ForwardDevice(m, B);
-    if (e != hipSuccess) return e;
-    (void)hipEventRecord(e1, stream_);
-    (void)hipMemcpyAsync(out + b0, y_, m * 4, hipMemcpyDeviceToHost, stream_);
-    e = hipStreamSynchronize(stream_);
-    if (e != hipSuccess) return e;
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    last_forward_ms += ms;
+    infer_h1_cap_ = want;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  for (size_t b0 = 0; b0 < n; b0 += chunk) {
+    const uint32_t m = (uint32_t)std::min(chunk, n - b0);
+    const uint32_t tiles = (m + kInferTile - 1) / kInferTile;
+    const unsigned groups = std::max(1u, std::min<unsigned>(tiles, (unsigned)cus / 4 * 2));
+    hipLaunchKernelGGL(InferLayer1Kernel, dim3(4 * groups), dim3(256), 0, s,
+                       d_codes + b0 * kCostResidues, m, w1h_, embh_, master_ + kOffB1,
+                       infer_h1_);
+    hipLaunchKernelGGL(InferHeadKernel, dim3(std::min<unsigned>(tiles, (unsigned)cus * 2)),
+                       dim3(256), 0, s, infer_h1_, m, w2h_, master_ + kOffB2, master_ + kOffW3,
+                       master_ + kOffB3, d_out + b0);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   return hipSuccess;
+}
+
+hipError_t MlpDevice::Forward(const uint8_t *codes, size_t n, float *out) {
+  last_forward_ms = 0;
+  if (n == 0) return hipSuccess;
+  uint8_t *d_codes = nullptr;
+  float *d_out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void **)&d_codes, n * kCostResidues);
+  if (e == hipSuccess) e = hipMalloc((void **)&d_out, n * 4);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_codes, codes, n * kCostResidues, hipMemcpyHostToDevice, stream_);
+  if (e == hipSuccess) e = hipEventRecord(e0, stream_);
+  if (e == hipSuccess) e = InferDevice(d_codes, n, d_out, stream_);
+  if (e == hipSuccess) e = hipEventRecord(e1, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * 4, hipMemcpyDeviceToHost, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  float ms = 0;
+  if (e == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) last_forward_ms = ms;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (d_codes) (void)hipFree(d_codes);
+  if (d_out) (void)hipFree(d_out);
+  return e;
 }
 
 hipError_t MlpDevice::TrainStep(const uint8_t *codes, const float *targets, size_t n,
@@ -372,21 +562,27 @@ hipError_t MlpDevice::TrainStep(const uint8_t *codes, const float *targets, size
   hipLaunchKernelGGL(LossKernel, dim3(1), dim3(256), 0, s, (int)B, (int)n, y_, t_, y_, loss_);
   // y_ now holds dY
   (void)hipMemsetAsync(grad_, 0, kFlat * 4, s);
-  hipLaunchKernelGGL(OutputBackwardKernel, dim3(1), dim3(kCostH2), 0, s, (int)B, y_, h2_, z2_,
-                     master_ + kOffW3, grad_ + kOffW3, grad_ + kOffB3, dz2_, dz2h_);
+  const unsigned row_split = std::max(1u, std::min(64u, (unsigned)(B / 64)));
+  // dW3 = H2^T dY, dB3 = sum dY, then dZ2
+  hipLaunchKernelGGL((ColumnSumKernel<uint16_t>), dim3(1, row_split), dim3(256), 0, s, (int)B,
+                     kCostH2, h2_, y_, grad_ + kOffW3);
+  hipLaunchKernelGGL((ColumnSumKernel<float>), dim3(1, row_split), dim3(256), 0, s, (int)B, 1, y_,
+                     (const float *)nullptr, grad_ + kOffB3);
+  hipLaunchKernelGGL(OutputBackwardKernel, dim3(Blocks(B * kCostH2)), dim3(256), 0, s, (int)B,
+                     y_, z2_, master_ + kOffW3, dz2_, dz2h_);
   // dW2 = dZ2^T H1
   Gemm<kEpiStore>(s, kCostH2, kCostH1, (int)B, dz2h_, 1, kCostH2, h1_, 1, kCostH1,
                   grad_ + kOffW2, kCostH1);
-  hipLaunchKernelGGL(ColumnSumKernel, dim3(1), dim3(kCostH2), 0, s, (int)B, kCostH2, dz2_,
-                     grad_ + kOffB2);
+  hipLaunchKernelGGL((ColumnSumKernel<float>), dim3(1, row_split), dim3(256), 0, s, (int)B,
+                     kCostH2, dz2_, (const float *)nullptr, grad_ + kOffB2);
   // dZ1 = (dZ2 W2) * gelu'(Z1)
   Gemm<kEpiGeluGrad>(s, (int)B, kCostH1, kCostH2, dz2h_, kCostH2, 1, w2h_, 1, kCostH1, dz1_,
                      kCostH1, nullptr, dz1h_, z1_);
   // dW1 = dZ1^T X
   Gemm<kEpiStore>(s, kCostH1, kCostIn, (int)B, dz1h_, 1, kCostH1, x_, 1, kCostIn,
                   grad_ + kOffW1, kCostIn);
-  hipLaunchKernelGGL(ColumnSumKernel, dim3(1), dim3(kCostH1), 0, s, (int)B, kCostH1, dz1_,
-                     grad_ + kOffB1);
+  hipLaunchKernelGGL((ColumnSumKernel<float>), dim3(kCostH1 / 64, row_split), dim3(256), 0, s,
+                     (int)B, kCostH1, dz1_, (const float *)nullptr, grad_ + kOffB1);
   // dX = dZ1 W1, then the embedding gradient
   Gemm<kEpiStore>(s, (int)B, kCostIn, kCostH1, dz1h_, kCostH1, 1, w1h_, 1, kCostIn, dx_,
                   kCostIn);
@@ -483,6 +679,15 @@ gz_status gz_mlp_get_weights(gz_mlp_t h, float *flat) {
     memcpy(f, v->data(), v->size() * 4);
     f += v->size();
   }
+  return GZ_OK;
+}
+
+gz_status gz_cost_features_packed_async(const uint64_t *pq, const uint64_t *pd,
+                                        const uint32_t *qpos, const uint32_t *dpos, size_t n,
+                                        int reverse, uint8_t *out, gz_stream_t stream) {
+  GZ_REQUIRE(n < (1ull << 32), "gz_cost_features_packed_async: too many seeds");
+  GZ_HIP_CHECK(CostFeaturesOnDevice(pq, pd, qpos, dpos, (uint32_t)n, reverse != 0, out,
+                                    stream ? stream->s : nullptr));
   return GZ_OK;
 }
 

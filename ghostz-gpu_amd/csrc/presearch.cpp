@@ -8,7 +8,6 @@
 #include <sstream>
 #include <thread>
 
-#include "cost_model.h"
 #include "gz_common.h"
 
 namespace gz {
@@ -89,41 +88,35 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     const DbChunk &chunk = db.chunks[ci];
     // 1. seed stage
     t = Now();
-    std::vector<SeedHit> hits;
-    GZ_STEP(dev->SeedHits(ci, &hits, st));
-    st->seed_hits += hits.size();
+    SeedHitView hits;
+    GZ_STEP(dev->SeedHits(ci, (uint32_t)nq, &hits, st));
+    st->seed_hits += hits.count;
     st->seconds_seed += Now() - t;
 
     // 2. per-query enumeration order and chain filter
     t = Now();
-    std::vector<uint32_t> qstart(nq + 1, 0);
-    for (const SeedHit &h : hits) ++qstart[h.qid + 1];
-    for (size_t k = 0; k < nq; ++k) qstart[k + 1] += qstart[k];
-    std::vector<uint32_t> byq(hits.size());
-    {
-      std::vector<uint32_t> fill(qstart.begin(), qstart.end() - 1);
-      for (uint32_t k = 0; k < hits.size(); ++k) byq[fill[hits[k].qid]++] = k;
-    }
     std::vector<std::vector<Hit> > chained(nq);
     ParallelBlocks(nq, threads, 64, [&](size_t b, size_t e) {
       std::vector<uint64_t> scratch;
-      KeyOrder order{hits.data()};
+      std::vector<uint32_t> byq;
+      KeyOrder order{hits.hits};
       for (size_t qi = b; qi < e; ++qi) {
-        const uint32_t s0 = qstart[qi], s1 = qstart[qi + 1];
+        const uint32_t s0 = hits.qstart[qi], s1 = hits.qstart[qi + 1];
         if (s0 == s1) continue;
-        std::sort(byq.begin() + s0, byq.begin() + s1, order);
+        byq.resize(s1 - s0);
+        std::iota(byq.begin(), byq.end(), s0);
+        std::sort(byq.begin(), byq.end(), order);
         std::vector<Hit> &hv = chained[qi];
         hv.resize(s1 - s0);
-        for (uint32_t k = s0; k < s1; ++k) {
-          const SeedHit &h = hits[byq[k]];
-          hv[k - s0].q = h.qpos - (uint32_t)coff[qi];
-          hv[k - s0].d = h.dpos;
+        for (uint32_t k = 0; k < s1 - s0; ++k) {
+          const SeedHit &h = hits.hits[byq[k]];
+          hv[k].q = h.qpos - (uint32_t)coff[qi];
+          hv[k].d = h.dpos;
         }
         ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl,
                     hv, scratch);
       }
     });
-    std::vector<SeedHit>().swap(hits);
     std::vector<uint32_t> gstart(nq + 1, 0);
     for (size_t k = 0; k < nq; ++k) gstart[k + 1] = gstart[k] + (uint32_t)chained[k].size();
     const uint32_t G = gstart[nq];
@@ -133,80 +126,67 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     // 3. gapped extension of both sides on the device
     t = Now();
     std::vector<uint32_t> rq(G), rd(G), fq(G), fd(G);
-    for (size_t qi = 0; qi < nq; ++qi)
-      for (uint32_t k = 0; k < chained[qi].size(); ++k) {
-        const uint32_t g = gstart[qi] + k;
-        const Hit &h = chained[qi][k];
-        rq[g] = (uint32_t)coff[qi] + h.q - 1;
-        rd[g] = h.d - 1;
-        fq[g] = (uint32_t)coff[qi] + h.q;
-        fd[g] = h.d;
-      }
+    ParallelBlocks(nq, threads, 256, [&](size_t b, size_t e) {
+      for (size_t qi = b; qi < e; ++qi)
+        for (uint32_t k = 0; k < chained[qi].size(); ++k) {
+          const uint32_t g = gstart[qi] + k;
+          const Hit &h = chained[qi][k];
+          rq[g] = (uint32_t)coff[qi] + h.q - 1;
+          rd[g] = h.d - 1;
+          fq[g] = (uint32_t)coff[qi] + h.q;
+          fd[g] = h.d;
+        }
+    });
     std::vector<int> rs(G), fs(G);
     if (G > 0) {
-      // schedule: the order extensions are handed to the lanes
-      std::vector<uint32_t> perm_r(G), perm_f(G);
-      std::iota(perm_r.begin(), perm_r.end(), 0);
-      std::iota(perm_f.begin(), perm_f.end(), 0);
-      if (opt.schedule != Schedule::kPlain) {
-        std::vector<float> cr(G), cf(G);
-        if (opt.schedule == Schedule::kLearned && opt.cost_model) {
-          opt.cost_model->Predict(concat.data(), chunk.seq.data(), rq.data(), rd.data(), G,
-                                  true, cr.data());
-          opt.cost_model->Predict(concat.data(), chunk.seq.data(), fq.data(), fd.data(), G,
-                                  false, cf.data());
-        } else {
-          for (size_t qi = 0; qi < nq; ++qi)
-            for (uint32_t k = gstart[qi]; k < gstart[qi + 1]; ++k) {
-              cr[k] = (float)q.DistanceFromStart(qi, rq[k] - (uint32_t)coff[qi]);
-              cf[k] = (float)q.DistanceFromEnd(qi, fq[k] - (uint32_t)coff[qi]);
-            }
-        }
-        std::stable_sort(perm_r.begin(), perm_r.end(),
-                         [&](uint32_t a, uint32_t b) { return cr[a] > cr[b]; });
-        std::stable_sort(perm_f.begin(), perm_f.end(),
-                         [&](uint32_t a, uint32_t b) { return cf[a] > cf[b]; });
-      }
+      // jobs: [0, G) reverse sides, [G, 2G) forward sides; the order the
+      // device runs them in never changes their results
       std::vector<uint32_t> aq(2 * (size_t)G), ad(2 * (size_t)G);
-      for (uint32_t k = 0; k < G; ++k) {
-        aq[k] = rq[perm_r[k]];
-        ad[k] = rd[perm_r[k]];
-        aq[G + k] = fq[perm_f[k]];
-        ad[G + k] = fd[perm_f[k]];
+      std::copy(rq.begin(), rq.end(), aq.begin());
+      std::copy(fq.begin(), fq.end(), aq.begin() + G);
+      std::copy(rd.begin(), rd.end(), ad.begin());
+      std::copy(fd.begin(), fd.end(), ad.begin() + G);
+      std::vector<uint16_t> keys;
+      if (opt.schedule == Schedule::kLengthSorted) {
+        // ExtensionSeedList order: by the query distance the side can extend
+        keys.resize(2 * (size_t)G);
+        ParallelBlocks(nq, threads, 256, [&](size_t b, size_t e) {
+          for (size_t qi = b; qi < e; ++qi)
+            for (uint32_t k = gstart[qi]; k < gstart[qi + 1]; ++k) {
+              const uint32_t r = q.DistanceFromStart(qi, rq[k] - (uint32_t)coff[qi]);
+              const uint32_t f = q.DistanceFromEnd(qi, fq[k] - (uint32_t)coff[qi]);
+              keys[k] = (uint16_t)std::min<uint32_t>(r, kOrderKeyLimit - 1);
+              keys[G + k] = (uint16_t)std::min<uint32_t>(f, kOrderKeyLimit - 1);
+            }
+        });
       }
+      const bool learned = opt.schedule == Schedule::kLearned && opt.cost_model != nullptr;
       std::vector<int> sc(2 * (size_t)G);
       std::vector<uint32_t> cells(opt.record_cells ? 2 * (size_t)G : 0);
       GZ_STEP(dev->GappedExtend(ci, G, G, aq.data(), ad.data(), sc.data(),
-                                cells.empty() ? nullptr : cells.data(), st));
-      for (uint32_t k = 0; k < G; ++k) {
-        rq[perm_r[k]] = aq[k];
-        rd[perm_r[k]] = ad[k];
-        rs[perm_r[k]] = sc[k];
-        fq[perm_f[k]] = aq[G + k];
-        fd[perm_f[k]] = ad[G + k];
-        fs[perm_f[k]] = sc[G + k];
-      }
+                                cells.empty() ? nullptr : cells.data(),
+                                keys.empty() ? nullptr : keys.data(), learned, st));
+      std::copy(aq.begin(), aq.begin() + G, rq.begin());
+      std::copy(aq.begin() + G, aq.end(), fq.begin());
+      std::copy(ad.begin(), ad.begin() + G, rd.begin());
+      std::copy(ad.begin() + G, ad.end(), fd.begin());
+      std::copy(sc.begin(), sc.begin() + G, rs.begin());
+      std::copy(sc.begin() + G, sc.end(), fs.begin());
       if (!cells.empty()) {
-        // seed positions before extension, per job in launch order
-        std::vector<uint32_t> owner(G);
-        for (size_t qi = 0; qi < nq; ++qi)
-          for (uint32_t g = gstart[qi]; g < gstart[qi + 1]; ++g) owner[g] = (uint32_t)qi;
-        for (int side = 0; side < 2; ++side) {
-          const std::vector<uint32_t> &perm = side == 0 ? perm_r : perm_f;
-          for (uint32_t k = 0; k < G; ++k) {
-            const uint32_t g = perm[k];
-            const uint32_t qi = owner[g];
-            const Hit &h = chained[qi][g - gstart[qi]];
-            GappedTrace tr;
-            tr.chunk = (uint32_t)ci;
-            tr.reverse = side == 0 ? 1 : 0;
-            tr.qpos = (uint32_t)coff[qi] + h.q - (side == 0 ? 1 : 0);
-            tr.dpos = h.d - (side == 0 ? 1 : 0);
-            tr.cells = cells[(size_t)side * G + k];
-            if (traces) traces->push_back(tr);
-            st->gapped_cells += tr.cells;
-          }
-        }
+        // seed positions before extension
+        for (int side = 0; side < 2; ++side)
+          for (size_t qi = 0; qi < nq; ++qi)
+            for (uint32_t g = gstart[qi]; g < gstart[qi + 1]; ++g) {
+              const Hit &h = chained[qi][g - gstart[qi]];
+              GappedTrace tr;
+              tr.chunk = (uint32_t)ci;
+              tr.reverse = side == 0 ? 1 : 0;
+              tr.qpos = (uint32_t)coff[qi] + h.q - (side == 0 ? 1 : 0);
+              tr.dpos = h.d - (side == 0 ? 1 : 0);
+              tr.cells = cells[(size_t)side * G + g];
+              if (traces) traces->push_back(tr);
+              st->gapped_cells += tr.cells;
+            }
       }
     }
     st->gapped_extensions += 2 * (uint64_t)G;

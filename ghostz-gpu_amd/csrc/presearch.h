@@ -32,6 +32,15 @@ struct SeedHit {
   uint32_t entry;  // entry index within the list
 };
 
+// Passing seeds of one database chunk grouped by query: the hits of query i
+// are hits[qstart[i] .. qstart[i+1]) in no particular order.  The arrays
+// belong to the device stage and stay valid until its next SeedHits call.
+struct SeedHitView {
+  const SeedHit *hits = nullptr;
+  size_t count = 0;
+  const uint32_t *qstart = nullptr;  // number of queries + 1
+};
+
 struct AlignStats {
   double seconds_total = 0;       // presearch start .. output written
   double seconds_query_lists = 0;
@@ -52,6 +61,7 @@ struct AlignStats {
   uint64_t results = 0;
   uint64_t sequential_build_chunks = 0;
   double gapped_kernel_ms = 0;    // measured with HIP events on the stream
+  double schedule_kernel_ms = 0;  // ordering of the gapped jobs on the device
   double seed_kernel_ms[2] = {0, 0};  // seed kernels: representatives, values
   uint64_t seed_launches[2] = {0, 0};
   uint64_t gapped_launches = 0;
@@ -69,6 +79,9 @@ struct AlignerOptions {
   const CostModel *cost_model = nullptr;     // for Schedule::kLearned
   bool record_cells = false;                 // keep per-extension DP cells
 };
+
+// Order keys of gapped jobs are below this bound (larger key = more work).
+constexpr uint32_t kOrderKeyLimit = 4096;
 
 // One gapped extension job as traced for the learned scheduler.
 struct GappedTrace {
@@ -89,15 +102,19 @@ class PresearchDevice {
                               const QueryLists &lists, const std::vector<int> &cutoffs,
                               const std::vector<int> &triggers, int presearch_cutoff,
                               AlignStats *st) = 0;
-  // Passing seeds of all lists against database chunk `chunk` (any order).
-  virtual int SeedHits(size_t chunk, std::vector<SeedHit> *hits, AlignStats *st) = 0;
+  // Passing seeds of all lists against database chunk `chunk`, grouped by
+  // query (nq = queries of the chunk).
+  virtual int SeedHits(size_t chunk, uint32_t nq, SeedHitView *hits, AlignStats *st) = 0;
   // One-sided gapped X-drop extensions of database chunk `chunk`: the first
   // n_reverse jobs extend backwards, the next n_forward forwards; qpos/dpos
   // (concatenated query / chunk positions) are replaced by the extension
   // ends and scores written; cells (optional) receives the DP cells of each.
+  // The jobs are independent; the device may run them in any order:
+  // keys (optional, < kOrderKeyLimit) ask for descending-key order, learned
+  // asks the device to predict the keys with its cost model.
   virtual int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward,
                            uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
-                           AlignStats *st) = 0;
+                           const uint16_t *keys, bool learned, AlignStats *st) = 0;
   virtual void EndQueryChunk() = 0;
 };
 

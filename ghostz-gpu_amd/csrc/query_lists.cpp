@@ -28,6 +28,25 @@ void ConcatenatedOffsets(const QuerySet &q, std::vector<uint64_t> *offsets,
 
 namespace {
 
+// f(i) for i in [0, n) on up to `threads` threads, in blocks of `grain`
+template <class F>
+void ParallelFor(size_t n, int threads, size_t grain, F f) {
+  const int T = std::max(1, std::min<int>(threads, (int)((n + grain - 1) / std::max<size_t>(grain, 1))));
+  std::atomic<size_t> next(0);
+  auto work = [&]() {
+    for (;;) {
+      const size_t b = next.fetch_add(grain);
+      if (b >= n) break;
+      const size_t e = std::min(n, b + grain);
+      for (size_t i = b; i < e; ++i) f(i);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+}
+
 struct Entry {
   uint32_t hash;
   uint32_t pos;   // concatenated
@@ -126,35 +145,55 @@ void BuildQueryLists(const QuerySet &q, const std::vector<uint64_t> &coff,
     work();
     for (auto &t : pool) t.join();
   }
-  size_t total = 0;
-  for (size_t i = 0; i < n; ++i) total += per[i].size();
-  std::vector<Entry> all;
-  all.reserve(total);
-  for (size_t k = 0; k < n; ++k) {
+  // appearance order: queries in `order`, entries of a query in position order
+  std::vector<size_t> qoff(n + 1, 0);
+  for (size_t k = 0; k < n; ++k) qoff[k + 1] = qoff[k] + per[order[k]].size();
+  const size_t total = qoff[n];
+  std::vector<Entry> all(total);
+  // the appearance index rides through the stable sort in the check word's
+  // upper bits
+  ParallelFor(n, threads, 64, [&](size_t k) {
     std::vector<Entry> &v = per[order[k]];
-    all.insert(all.end(), v.begin(), v.end());
-    std::vector<Entry>().swap(v);
-  }
-  // carry each entry's appearance index through the stable sort in the
-  // check word's upper bits
-  for (size_t i = 0; i < total; ++i) all[i].check |= (uint32_t)(i << 1);
-  StableSortByHash(all, threads);
-  // groups; a list's id is the appearance index of its first entry
-  std::vector<std::pair<uint32_t, uint32_t> > groups;  // (first appearance, start)
-  std::vector<uint32_t> gstart;
-  for (size_t i = 0; i < total; ++i)
-    if (i == 0 || all[i].hash != all[i - 1].hash) {
-      gstart.push_back((uint32_t)i);
-      groups.push_back(std::make_pair(all[i].check >> 1, (uint32_t)groups.size()));
+    Entry *dst = all.data() + qoff[k];
+    for (size_t j = 0; j < v.size(); ++j) {
+      dst[j] = v[j];
+      dst[j].check |= (uint32_t)((qoff[k] + j) << 1);
     }
+    std::vector<Entry>().swap(v);
+  });
+  StableSortByHash(all, threads);
+  // groups of equal hashes (sorted by hash, appearance order inside)
+  std::vector<uint32_t> gstart;
+  {
+    const size_t T = (size_t)std::max(1, threads);
+    const size_t span = (total + T - 1) / std::max<size_t>(T, 1);
+    std::vector<std::vector<uint32_t> > part(T);
+    ParallelFor(T, threads, 1, [&](size_t t) {
+      const size_t b = t * span, e = std::min(total, b + span);
+      for (size_t i = b; i < e; ++i)
+        if (i == 0 || all[i].hash != all[i - 1].hash) part[t].push_back((uint32_t)i);
+    });
+    for (auto &v : part) gstart.insert(gstart.end(), v.begin(), v.end());
+  }
+  const size_t nl = gstart.size();
   gstart.push_back((uint32_t)total);
-  std::sort(groups.begin(), groups.end());
-  const size_t nl = groups.size();
+  // list ids in first-appearance order without a sort: the first entry of a
+  // group is its earliest appearance; walking appearance indexes in order
+  // meets the groups in that order
+  std::vector<uint32_t> group_at(total, 0xffffffffu);
+  ParallelFor(nl, threads, 4096, [&](size_t g) {
+    group_at[all[gstart[g]].check >> 1] = (uint32_t)g;
+  });
+  std::vector<uint32_t> by_appearance;
+  by_appearance.reserve(nl);
+  for (size_t i = 0; i < total; ++i)
+    if (group_at[i] != 0xffffffffu) by_appearance.push_back(group_at[i]);
+  std::vector<uint32_t>().swap(group_at);
   // list id (first-appearance order) -> rank (presearch visiting order)
-  std::vector<uint32_t> id_of_rank(nl);
+  std::vector<uint32_t> group_of_rank(nl);
   for (size_t b = 0; b < nl; b += 256) {
     const size_t e = std::min(nl, b + 256);
-    for (size_t k = 0; k < e - b; ++k) id_of_rank[b + k] = (uint32_t)(e - 1 - k);
+    for (size_t k = 0; k < e - b; ++k) group_of_rank[b + k] = by_appearance[e - 1 - k];
   }
   out->hash.resize(nl);
   out->entry_off.resize(nl + 1);
@@ -163,16 +202,21 @@ void BuildQueryLists(const QuerySet &q, const std::vector<uint64_t> &coff,
   out->entry_check.resize(total);
   size_t o = 0;
   for (size_t r = 0; r < nl; ++r) {
-    const uint32_t g = groups[id_of_rank[r]].second;
+    const uint32_t g = group_of_rank[r];
     out->hash[r] = all[gstart[g]].hash;
     out->entry_off[r] = (uint32_t)o;
-    for (uint32_t i = gstart[g]; i < gstart[g + 1]; ++i, ++o) {
-      out->entry_pos[o] = all[i].pos;
-      out->entry_qid[o] = all[i].qid;
-      out->entry_check[o] = (uint8_t)(all[i].check & 1);
-    }
+    o += gstart[g + 1] - gstart[g];
   }
   out->entry_off[nl] = (uint32_t)o;
+  ParallelFor(nl, threads, 1024, [&](size_t r) {
+    const uint32_t g = group_of_rank[r];
+    size_t oo = out->entry_off[r];
+    for (uint32_t i = gstart[g]; i < gstart[g + 1]; ++i, ++oo) {
+      out->entry_pos[oo] = all[i].pos;
+      out->entry_qid[oo] = all[i].qid;
+      out->entry_check[oo] = (uint8_t)(all[i].check & 1);
+    }
+  });
 }
 
 }  // namespace gz

@@ -1,0 +1,147 @@
+// schedule.hip -- see schedule.h.
+#include <hip/hip_runtime.h>
+
+#include "gz_device.h"
+#include "schedule.h"
+
+namespace gz {
+
+namespace {
+
+constexpr int kOrderThreads = 256;
+constexpr uint32_t kOrderChunk = 8192;  // jobs per block
+
+__global__ void __launch_bounds__(kOrderThreads)
+    BinHistogramKernel(const uint16_t *keys, uint32_t n, uint32_t *counts) {
+  __shared__ uint32_t hist[kScheduleBins];
+  for (uint32_t b = threadIdx.x; b < kScheduleBins; b += kOrderThreads) hist[b] = 0;
+  __syncthreads();
+  const uint32_t begin = blockIdx.x * kOrderChunk;
+  const uint32_t end = min(n, begin + kOrderChunk);
+  for (uint32_t i = begin + threadIdx.x; i < end; i += kOrderThreads)
+    atomicAdd(&hist[keys[i]], 1u);
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < kScheduleBins; b += kOrderThreads)
+    if (hist[b]) atomicAdd(&counts[b], hist[b]);
+}
+
+// cursor[b] = number of jobs in bins above b (descending layout); one block
+__global__ void __launch_bounds__(kOrderThreads)
+    BinScanKernel(const uint32_t *counts, uint32_t *cursor) {
+  __shared__ uint32_t lds[kOrderThreads / kWave];
+  constexpr uint32_t kPer = kScheduleBins / kOrderThreads;
+  // thread t owns bins in descending order: kScheduleBins-1-(t*kPer+j)
+  uint32_t v[kPer];
+  uint32_t sum = 0;
+  for (uint32_t j = 0; j < kPer; ++j) {
+    v[j] = counts[kScheduleBins - 1 - (threadIdx.x * kPer + j)];
+    sum += v[j];
+  }
+  uint32_t inc, exc;
+  BlockScan<uint32_t, false>(sum, lds, &inc, &exc, nullptr);
+  uint32_t run = exc;
+  for (uint32_t j = 0; j < kPer; ++j) {
+    cursor[kScheduleBins - 1 - (threadIdx.x * kPer + j)] = run;
+    run += v[j];
+  }
+}
+
+__global__ void __launch_bounds__(kOrderThreads)
+    BinScatterKernel(const uint16_t *keys, uint32_t n, uint32_t *cursor, uint32_t *perm) {
+  __shared__ uint32_t hist[kScheduleBins];
+  for (uint32_t b = threadIdx.x; b < kScheduleBins; b += kOrderThreads) hist[b] = 0;
+  __syncthreads();
+  const uint32_t begin = blockIdx.x * kOrderChunk;
+  const uint32_t end = min(n, begin + kOrderChunk);
+  for (uint32_t i = begin + threadIdx.x; i < end; i += kOrderThreads)
+    atomicAdd(&hist[keys[i]], 1u);
+  __syncthreads();
+  // reserve this block's range of every bin it uses
+  for (uint32_t b = threadIdx.x; b < kScheduleBins; b += kOrderThreads)
+    if (hist[b]) hist[b] = atomicAdd(&cursor[b], hist[b]);
+  __syncthreads();
+  for (uint32_t i = begin + threadIdx.x; i < end; i += kOrderThreads) {
+    const uint32_t pos = atomicAdd(&hist[keys[i]], 1u);
+    perm[pos] = i;
+  }
+}
+
+__global__ void ZeroKernel(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void CostToKeyKernel(const float *cost, uint32_t n, float scale, uint16_t *keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = cost[i] * scale;
+  int k = (x == x) ? (int)lrintf(fminf(fmaxf(x, 0.f), (float)(kScheduleBins - 1))) : 0;
+  keys[i] = (uint16_t)k;
+}
+
+__global__ void GatherPairKernel(const uint32_t *perm, uint32_t n, const uint32_t *a,
+                                 const uint32_t *b, uint32_t *oa, uint32_t *ob) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = perm[k];
+  oa[k] = a[i];
+  ob[k] = b[i];
+}
+
+__global__ void ScatterBackKernel(const uint32_t *perm, uint32_t n, const uint32_t *a,
+                                  const uint32_t *b, const int *c, const uint32_t *d,
+                                  uint32_t *oa, uint32_t *ob, int *oc, uint32_t *od) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = perm[k];
+  oa[i] = a[k];
+  ob[i] = b[k];
+  oc[i] = c[k];
+  if (d) od[i] = d[k];
+}
+
+inline unsigned Blocks(uint32_t n, unsigned t = 256) { return (n + t - 1) / t; }
+
+}  // namespace
+
+size_t ScheduleTempBytes(size_t) { return 2 * kScheduleBins * sizeof(uint32_t); }
+
+hipError_t DeviceOrderByKey(const uint16_t *keys, uint32_t n, uint32_t *perm, void *temp,
+                            hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint32_t *counts = (uint32_t *)temp;
+  uint32_t *cursor = counts + kScheduleBins;
+  const unsigned blocks = (n + kOrderChunk - 1) / kOrderChunk;
+  hipLaunchKernelGGL(ZeroKernel, dim3(Blocks(kScheduleBins)), dim3(256), 0, s, counts,
+                     kScheduleBins);
+  hipLaunchKernelGGL(BinHistogramKernel, dim3(blocks), dim3(kOrderThreads), 0, s, keys, n,
+                     counts);
+  hipLaunchKernelGGL(BinScanKernel, dim3(1), dim3(kOrderThreads), 0, s, counts, cursor);
+  hipLaunchKernelGGL(BinScatterKernel, dim3(blocks), dim3(kOrderThreads), 0, s, keys, n, cursor,
+                     perm);
+  return hipGetLastError();
+}
+
+hipError_t CostToKey(const float *cost, uint32_t n, float scale, uint16_t *keys, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(CostToKeyKernel, dim3(Blocks(n)), dim3(256), 0, s, cost, n, scale, keys);
+  return hipGetLastError();
+}
+
+hipError_t GatherPair(const uint32_t *perm, uint32_t n, const uint32_t *a, const uint32_t *b,
+                      uint32_t *oa, uint32_t *ob, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(GatherPairKernel, dim3(Blocks(n)), dim3(256), 0, s, perm, n, a, b, oa, ob);
+  return hipGetLastError();
+}
+
+hipError_t ScatterBack(const uint32_t *perm, uint32_t n, const uint32_t *a, const uint32_t *b,
+                       const int *c, const uint32_t *d, uint32_t *oa, uint32_t *ob, int *oc,
+                       uint32_t *od, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(ScatterBackKernel, dim3(Blocks(n)), dim3(256), 0, s, perm, n, a, b, c, d,
+                     oa, ob, oc, od);
+  return hipGetLastError();
+}
+
+}  // namespace gz

@@ -68,11 +68,12 @@ struct SeedStageStats {
 class SeedStage {
  public:
   ~SeedStage();
-  // Runs the seed stage of one query chunk against one database chunk and
-  // returns the passing seeds (host vector, unordered).
+  // Runs the seed stage of one query chunk (nq queries) against one database
+  // chunk and returns the passing seeds grouped by query, in pinned host
+  // memory owned by the stage (valid until the next Run).
   hipError_t Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
-                 const SeedStageParams &p, hipStream_t stream,
-                 std::vector<SeedHit> *hits, SeedStageStats *stats);
+                 const SeedStageParams &p, uint32_t nq, hipStream_t stream,
+                 SeedHitView *hits, SeedStageStats *stats);
 
  private:
   hipError_t Reserve(void **p, size_t *cap, size_t bytes);
@@ -81,6 +82,10 @@ class SeedStage {
   void *scan_tmp_ = nullptr; size_t scan_tmp_cap_ = 0;
   void *active_ = nullptr; size_t active_cap_ = 0;
   void *hits_ = nullptr; size_t hits_cap_ = 0;
+  void *grouped_ = nullptr; size_t grouped_cap_ = 0;   // hits grouped by query
+  void *qcount_ = nullptr; size_t qcount_cap_ = 0;     // per-query counts / cursors
+  void *host_hits_ = nullptr; size_t host_hits_cap_ = 0;  // pinned
+  void *host_qstart_ = nullptr; size_t host_qstart_cap_ = 0;  // pinned
   uint32_t *counters_ = nullptr;
   hipEvent_t ev_[2] = {nullptr, nullptr};
   void *pinned_ = nullptr; size_t pinned_cap_ = 0;

@@ -264,12 +264,33 @@ __global__ void __launch_bounds__(kSeedThreads)
   }
 }
 
+__global__ void FillU32Kernel(uint32_t *p, size_t n, uint32_t v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void QueryCountKernel(const SeedHit *hits, size_t n, uint32_t *count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&count[hits[i].qid], 1u);
+}
+
+__global__ void QueryScatterKernel(const SeedHit *hits, size_t n, uint32_t *cursor,
+                                   SeedHit *out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const SeedHit h = hits[i];
+    out[atomicAdd(&cursor[h.qid], 1u)] = h;
+  }
+}
+
 }  // namespace
 
 SeedStage::~SeedStage() {
-  void *ps[] = {lookup_, units_, scan_tmp_, active_, hits_};
+  void *ps[] = {lookup_, units_, scan_tmp_, active_, hits_, grouped_, qcount_};
   for (void *p : ps)
     if (p) (void)hipFree(p);
+  if (host_hits_) (void)hipHostFree(host_hits_);
+  if (host_qstart_) (void)hipHostFree(host_qstart_);
   if (counters_) (void)hipFree(counters_);
   if (pinned_) (void)hipHostFree(pinned_);
   for (hipEvent_t e : ev_)
@@ -296,127 +317,184 @@ hipError_t SeedStage::Reserve(void **p, size_t *cap, size_t bytes) {
   } while (0)
 
 hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
-                          const SeedStageParams &p, hipStream_t s,
-                          std::vector<SeedHit> *out, SeedStageStats *stats) {
-  out->clear();
-  const uint32_t n = q.nlists;
-  if (n == 0) return hipSuccess;
+                          const SeedStageParams &p, uint32_t nq, hipStream_t s,
+                          SeedHitView *out, SeedStageStats *stats) {
+  *out = SeedHitView();
   if (!counters_) GZ_TRY(hipMalloc(&counters_, kCounters * sizeof(uint32_t)));
   if (!pinned_) {
     GZ_TRY(hipHostMalloc(&pinned_, 64, hipHostMallocDefault));
     pinned_cap_ = 64;
   }
-  // layout: info | units_rep | units_val | off_rep | off_val | flags | slots
-  const size_t sz_info = AlignUp(n * sizeof(ListInfo), 256);
-  const size_t sz_u64 = AlignUp((size_t)(n + 1) * 8, 256);
-  const size_t sz_u32 = AlignUp((size_t)(n + 1) * 4, 256);
-  GZ_TRY(Reserve(&lookup_, &lookup_cap_, sz_info + 4 * sz_u64 + 2 * sz_u32));
-  char *base = (char *)lookup_;
-  ListInfo *info = (ListInfo *)base;
-  uint64_t *units[2] = {(uint64_t *)(base + sz_info), (uint64_t *)(base + sz_info + sz_u64)};
-  uint64_t *offs[2] = {(uint64_t *)(base + sz_info + 2 * sz_u64),
-                       (uint64_t *)(base + sz_info + 3 * sz_u64)};
-  uint32_t *flags = (uint32_t *)(base + sz_info + 4 * sz_u64);
-  uint32_t *slots = (uint32_t *)(base + sz_info + 4 * sz_u64 + sz_u32);
-  const size_t tmp = std::max(DeviceScanTempBytes<uint64_t>(n), DeviceScanTempBytes<uint32_t>(n));
-  GZ_TRY(Reserve(&scan_tmp_, &scan_tmp_cap_, tmp));
-  GZ_TRY(Reserve(&active_, &active_cap_, 2 * (AlignUp((size_t)(n + 1) * 4, 256) +
-                                              AlignUp((size_t)(n + 1) * 8, 256))));
-  if (!hits_) GZ_TRY(Reserve(&hits_, &hits_cap_, kInitialHitCapacity * sizeof(SeedHit)));
   if (!ev_[0]) {
     GZ_TRY(hipEventCreate(&ev_[0]));
     GZ_TRY(hipEventCreate(&ev_[1]));
   }
+  // per-query counts, also the qstart array handed back
+  GZ_TRY(Reserve(&qcount_, &qcount_cap_, 2 * AlignUp((size_t)(nq + 1) * 4, 256) +
+                                             DeviceScanTempBytes<uint32_t>(nq + 1)));
+  if (host_qstart_cap_ < (size_t)(nq + 1) * 4) {
+    if (host_qstart_) GZ_TRY(hipHostFree(host_qstart_));
+    host_qstart_ = nullptr;
+    host_qstart_cap_ = 0;
+    GZ_TRY(hipHostMalloc(&host_qstart_, (size_t)(nq + 1) * 4, hipHostMallocDefault));
+    host_qstart_cap_ = (size_t)(nq + 1) * 4;
+  }
+  uint32_t *h_qstart = (uint32_t *)host_qstart_;
+  const uint32_t n = q.nlists;
+  uint64_t total_hits = 0;
+  if (n > 0) {
+    // layout: info | units_rep | units_val | off_rep | off_val | flags | slots
+    const size_t sz_info = AlignUp(n * sizeof(ListInfo), 256);
+    const size_t sz_u64 = AlignUp((size_t)(n + 1) * 8, 256);
+    const size_t sz_u32 = AlignUp((size_t)(n + 1) * 4, 256);
+    GZ_TRY(Reserve(&lookup_, &lookup_cap_, sz_info + 4 * sz_u64 + 2 * sz_u32));
+    char *base = (char *)lookup_;
+    ListInfo *info = (ListInfo *)base;
+    uint64_t *units[2] = {(uint64_t *)(base + sz_info), (uint64_t *)(base + sz_info + sz_u64)};
+    uint64_t *offs[2] = {(uint64_t *)(base + sz_info + 2 * sz_u64),
+                         (uint64_t *)(base + sz_info + 3 * sz_u64)};
+    uint32_t *flags = (uint32_t *)(base + sz_info + 4 * sz_u64);
+    uint32_t *slots = (uint32_t *)(base + sz_info + 4 * sz_u64 + sz_u32);
+    const size_t tmp =
+        std::max(DeviceScanTempBytes<uint64_t>(n), DeviceScanTempBytes<uint32_t>(n));
+    GZ_TRY(Reserve(&scan_tmp_, &scan_tmp_cap_, tmp));
+    GZ_TRY(Reserve(&active_, &active_cap_, 2 * (AlignUp((size_t)(n + 1) * 4, 256) +
+                                                AlignUp((size_t)(n + 1) * 8, 256))));
+    if (!hits_) GZ_TRY(Reserve(&hits_, &hits_cap_, kInitialHitCapacity * sizeof(SeedHit)));
 
-  const unsigned grid_n = (n + 255) / 256;
-  hipLaunchKernelGGL(LookupKernel, dim3(grid_n), dim3(256), 0, s, q, db,
-                     p.clustering ? 1 : 0, info, units[0], units[1]);
-  GZ_TRY(hipGetLastError());
+    const unsigned grid_n = (n + 255) / 256;
+    hipLaunchKernelGGL(LookupKernel, dim3(grid_n), dim3(256), 0, s, q, db,
+                       p.clustering ? 1 : 0, info, units[0], units[1]);
+    GZ_TRY(hipGetLastError());
 
-  uint32_t *act[2];
-  uint64_t *act_off[2];
-  {
-    char *a = (char *)active_;
-    const size_t s32 = AlignUp((size_t)(n + 1) * 4, 256), s64 = AlignUp((size_t)(n + 1) * 8, 256);
-    act[0] = (uint32_t *)a;
-    act_off[0] = (uint64_t *)(a + s32);
-    act[1] = (uint32_t *)(a + s32 + s64);
-    act_off[1] = (uint64_t *)(a + 2 * s32 + s64);
-  }
-  uint64_t totals[2];
-  uint32_t nact[2];
-  for (int k = 0; k < 2; ++k) {
-    GZ_TRY((DeviceScan<uint64_t, false>(units[k], offs[k], n, false, scan_tmp_, s)));
-    hipLaunchKernelGGL(ActiveFlagsKernel, dim3(grid_n), dim3(256), 0, s, n, units[k], flags);
-    GZ_TRY(hipGetLastError());
-    GZ_TRY((DeviceScan<uint32_t, false>(flags, slots, n, false, scan_tmp_, s)));
-    hipLaunchKernelGGL(ActiveScatterKernel, dim3(grid_n), dim3(256), 0, s, n, units[k],
-                       offs[k], slots, act[k], act_off[k]);
-    GZ_TRY(hipGetLastError());
-    // totals: last offset + last units, last slot + last flag
-    uint64_t *hp = (uint64_t *)pinned_;
-    GZ_TRY(hipMemcpyAsync(hp, offs[k] + (n - 1), 8, hipMemcpyDeviceToHost, s));
-    GZ_TRY(hipMemcpyAsync(hp + 1, units[k] + (n - 1), 8, hipMemcpyDeviceToHost, s));
-    GZ_TRY(hipMemcpyAsync((uint32_t *)(hp + 2), slots + (n - 1), 4, hipMemcpyDeviceToHost, s));
-    GZ_TRY(hipMemcpyAsync((uint32_t *)(hp + 2) + 1, flags + (n - 1), 4, hipMemcpyDeviceToHost, s));
-    GZ_TRY(hipStreamSynchronize(s));
-    totals[k] = hp[0] + hp[1];
-    nact[k] = ((uint32_t *)(hp + 2))[0] + ((uint32_t *)(hp + 2))[1];
-  }
-  if (stats) {
-    stats->units_rep += totals[0];
-    stats->units_value += totals[1];
-  }
-  int dev = 0;
-  GZ_TRY(hipGetDevice(&dev));
-  int cus = 256;
-  GZ_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  for (int k = 0; k < 2; ++k) {
-    if (totals[k] == 0) continue;
-    for (uint64_t u0 = 0; u0 < totals[k];) {
-      const uint64_t u1 = std::min(totals[k], u0 + kUnitsPerLaunch);
-      const uint32_t cap = (uint32_t)(hits_cap_ / sizeof(SeedHit));
-      GZ_TRY(hipMemsetAsync(counters_, 0, kCounters * sizeof(uint32_t), s));
-      const uint64_t tiles = (u1 - u0 + kSeedTile - 1) / kSeedTile;
-      const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)cus * 8);
-      GZ_TRY(hipEventRecord(ev_[0], s));
-      if (k == 0) {
-        hipLaunchKernelGGL((SeedKernel<false>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
-                           info, act[0], act_off[0], nact[0], u0, u1, (SeedHit *)hits_, cap,
-                           counters_);
-      } else {
-        hipLaunchKernelGGL((SeedKernel<true>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
-                           info, act[1], act_off[1], nact[1], u0, u1, (SeedHit *)hits_, cap,
-                           counters_);
-      }
+    uint32_t *act[2];
+    uint64_t *act_off[2];
+    {
+      char *a = (char *)active_;
+      const size_t s32 = AlignUp((size_t)(n + 1) * 4, 256), s64 = AlignUp((size_t)(n + 1) * 8, 256);
+      act[0] = (uint32_t *)a;
+      act_off[0] = (uint64_t *)(a + s32);
+      act[1] = (uint32_t *)(a + s32 + s64);
+      act_off[1] = (uint64_t *)(a + 2 * s32 + s64);
+    }
+    uint64_t totals[2];
+    uint32_t nact[2];
+    for (int k = 0; k < 2; ++k) {
+      GZ_TRY((DeviceScan<uint64_t, false>(units[k], offs[k], n, false, scan_tmp_, s)));
+      hipLaunchKernelGGL(ActiveFlagsKernel, dim3(grid_n), dim3(256), 0, s, n, units[k], flags);
       GZ_TRY(hipGetLastError());
-      GZ_TRY(hipEventRecord(ev_[1], s));
-      uint32_t *hc = (uint32_t *)pinned_;
-      GZ_TRY(hipMemcpyAsync(hc, counters_, kCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      GZ_TRY((DeviceScan<uint32_t, false>(flags, slots, n, false, scan_tmp_, s)));
+      hipLaunchKernelGGL(ActiveScatterKernel, dim3(grid_n), dim3(256), 0, s, n, units[k],
+                         offs[k], slots, act[k], act_off[k]);
+      GZ_TRY(hipGetLastError());
+      // totals: last offset + last units, last slot + last flag
+      uint64_t *hp = (uint64_t *)pinned_;
+      GZ_TRY(hipMemcpyAsync(hp, offs[k] + (n - 1), 8, hipMemcpyDeviceToHost, s));
+      GZ_TRY(hipMemcpyAsync(hp + 1, units[k] + (n - 1), 8, hipMemcpyDeviceToHost, s));
+      GZ_TRY(hipMemcpyAsync((uint32_t *)(hp + 2), slots + (n - 1), 4, hipMemcpyDeviceToHost, s));
+      GZ_TRY(hipMemcpyAsync((uint32_t *)(hp + 2) + 1, flags + (n - 1), 4, hipMemcpyDeviceToHost,
+                            s));
       GZ_TRY(hipStreamSynchronize(s));
-      const uint32_t count = hc[0];
-      if (stats) {
-        float ms = 0;
-        GZ_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-        stats->kernel_ms[k] += ms;
-        stats->launches[k] += 1;
+      totals[k] = hp[0] + hp[1];
+      nact[k] = ((uint32_t *)(hp + 2))[0] + ((uint32_t *)(hp + 2))[1];
+    }
+    if (stats) {
+      stats->units_rep += totals[0];
+      stats->units_value += totals[1];
+    }
+    int dev = 0;
+    GZ_TRY(hipGetDevice(&dev));
+    int cus = 256;
+    GZ_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // passing seeds accumulate in hits_; counters_[0] is their running count
+    GZ_TRY(hipMemsetAsync(counters_, 0, kCounters * sizeof(uint32_t), s));
+    for (int k = 0; k < 2; ++k) {
+      if (totals[k] == 0) continue;
+      for (uint64_t u0 = 0; u0 < totals[k];) {
+        const uint64_t u1 = std::min(totals[k], u0 + kUnitsPerLaunch);
+        const uint32_t cap = (uint32_t)std::min<size_t>(hits_cap_ / sizeof(SeedHit), 0xffffffffu);
+        const uint64_t tiles = (u1 - u0 + kSeedTile - 1) / kSeedTile;
+        const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)cus * 8);
+        GZ_TRY(hipEventRecord(ev_[0], s));
+        if (k == 0) {
+          hipLaunchKernelGGL((SeedKernel<false>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
+                             info, act[0], act_off[0], nact[0], u0, u1, (SeedHit *)hits_, cap,
+                             counters_);
+        } else {
+          hipLaunchKernelGGL((SeedKernel<true>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
+                             info, act[1], act_off[1], nact[1], u0, u1, (SeedHit *)hits_, cap,
+                             counters_);
+        }
+        GZ_TRY(hipGetLastError());
+        GZ_TRY(hipEventRecord(ev_[1], s));
+        uint32_t *hc = (uint32_t *)pinned_;
+        GZ_TRY(hipMemcpyAsync(hc, counters_, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        GZ_TRY(hipStreamSynchronize(s));
+        const uint32_t count = hc[0];
+        if (stats) {
+          float ms = 0;
+          GZ_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+          stats->kernel_ms[k] += ms;
+          stats->launches[k] += 1;
+        }
+        if (count > cap) {
+          // the buffer overflowed during this range: grow it (keeping the
+          // hits of earlier ranges), rewind the count and redo the range
+          const size_t keep = (size_t)total_hits * sizeof(SeedHit);
+          void *bigger = nullptr;
+          GZ_TRY(hipMalloc(&bigger, std::max<size_t>((size_t)count * 2, 1024) * sizeof(SeedHit)));
+          if (keep) GZ_TRY(hipMemcpyAsync(bigger, hits_, keep, hipMemcpyDeviceToDevice, s));
+          GZ_TRY(hipStreamSynchronize(s));
+          GZ_TRY(hipFree(hits_));
+          hits_ = bigger;
+          hits_cap_ = std::max<size_t>((size_t)count * 2, 1024) * sizeof(SeedHit);
+          hc[0] = (uint32_t)total_hits;
+          GZ_TRY(hipMemcpyAsync(counters_, hc, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+          GZ_TRY(hipStreamSynchronize(s));
+          continue;
+        }
+        total_hits = count;
+        u0 = u1;
       }
-      if (count > cap) {
-        // too many passing seeds for the buffer: grow it and redo the range
-        size_t want = (size_t)count * 2 * sizeof(SeedHit);
-        GZ_TRY(Reserve(&hits_, &hits_cap_, want));
-        continue;
-      }
-      const size_t old = out->size();
-      out->resize(old + count);
-      if (count)
-        GZ_TRY(hipMemcpyAsync(out->data() + old, hits_, count * sizeof(SeedHit),
-                              hipMemcpyDeviceToHost, s));
-      GZ_TRY(hipStreamSynchronize(s));
-      if (stats) stats->hits += count;
-      u0 = u1;
     }
   }
+  if (stats) stats->hits += total_hits;
+  // group by query: counts, exclusive scan, scatter
+  uint32_t *qcount = (uint32_t *)qcount_;
+  uint32_t *qstart = (uint32_t *)((char *)qcount_ + AlignUp((size_t)(nq + 1) * 4, 256));
+  void *qtemp = (char *)qcount_ + 2 * AlignUp((size_t)(nq + 1) * 4, 256);
+  hipLaunchKernelGGL(FillU32Kernel, dim3((nq + 1 + 255) / 256), dim3(256), 0, s, qcount,
+                     (size_t)nq + 1, 0u);
+  if (total_hits) {
+    GZ_TRY(Reserve(&grouped_, &grouped_cap_, total_hits * sizeof(SeedHit)));
+    const unsigned g = (unsigned)((total_hits + 255) / 256);
+    hipLaunchKernelGGL(QueryCountKernel, dim3(g), dim3(256), 0, s, (const SeedHit *)hits_,
+                       (size_t)total_hits, qcount);
+  }
+  GZ_TRY((DeviceScan<uint32_t, false>(qcount, qstart, nq + 1, false, qtemp, s)));
+  GZ_TRY(hipMemcpyAsync(h_qstart, qstart, (size_t)(nq + 1) * 4, hipMemcpyDeviceToHost, s));
+  if (total_hits) {
+    // qcount becomes the scatter cursor
+    GZ_TRY(hipMemcpyAsync(qcount, qstart, (size_t)nq * 4, hipMemcpyDeviceToDevice, s));
+    const unsigned g = (unsigned)((total_hits + 255) / 256);
+    hipLaunchKernelGGL(QueryScatterKernel, dim3(g), dim3(256), 0, s, (const SeedHit *)hits_,
+                       (size_t)total_hits, qcount, (SeedHit *)grouped_);
+    GZ_TRY(hipGetLastError());
+    const size_t bytes = total_hits * sizeof(SeedHit);
+    if (host_hits_cap_ < bytes) {
+      if (host_hits_) GZ_TRY(hipHostFree(host_hits_));
+      host_hits_ = nullptr;
+      host_hits_cap_ = 0;
+      const size_t want = bytes + bytes / 4;
+      GZ_TRY(hipHostMalloc(&host_hits_, want, hipHostMallocDefault));
+      host_hits_cap_ = want;
+    }
+    GZ_TRY(hipMemcpyAsync(host_hits_, grouped_, bytes, hipMemcpyDeviceToHost, s));
+  }
+  GZ_TRY(hipStreamSynchronize(s));
+  out->hits = (const SeedHit *)host_hits_;
+  out->count = total_hits;
+  out->qstart = h_qstart;
   return hipSuccess;
 }
 

@@ -32,6 +32,7 @@ _SIGNATURES = {
     "gz_stream_synchronize": (c_int, [c_void_p]),
     "gz_memcpy_async": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "gz_memset_async": (c_int, [c_void_p, c_int, c_size_t, c_void_p]),
+    "gz_copy_kernel_async": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "gz_protein_encode_table": (None, [c_void_p]),
     "gz_score_matrix_blosum62": (c_int, [c_void_p, u32p]),
     "gz_score_matrix_blosum62_text": (c_char_p, []),
@@ -265,7 +266,8 @@ class AlignStats(Structure):
             "gapped_extensions", "gapped_cells", "results", "sequential_build_chunks")] + [
         ("gapped_kernel_ms", c_double), ("seed_rep_kernel_ms", c_double),
         ("seed_value_kernel_ms", c_double), ("seed_rep_launches", c_uint64),
-        ("seed_value_launches", c_uint64), ("gapped_launches", c_uint64)]
+        ("seed_value_launches", c_uint64), ("gapped_launches", c_uint64),
+        ("schedule_kernel_ms", c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -313,6 +315,8 @@ _SIGNATURES.update({
                                   POINTER(c_float), c_void_p]),
     "gz_cost_features": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int,
                                 c_void_p]),
+    "gz_cost_features_packed_async": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
+                                              c_int, c_void_p, c_void_p]),
     "gz_seg_mask": (c_int, [c_char_p, c_void_p]),
     "gz_seg_ln_factorial": (c_double, [c_int]),
     "gz_translate": (c_int, [c_char_p, c_void_p, c_void_p]),

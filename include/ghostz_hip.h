@@ -49,6 +49,10 @@ gz_status gz_memcpy_async(void *dst, const void *src, size_t bytes,
                           gz_stream_t stream);        /* cudaMemcpyDefault */
 gz_status gz_memset_async(void *dst, int value, size_t bytes,
                           gz_stream_t stream);
+/* Streaming device-to-device copy kernel (16-byte aligned): a kernel of known
+ * HBM traffic, used to calibrate hardware byte counters. */
+gz_status gz_copy_kernel_async(void *dst, const void *src, size_t bytes,
+                               gz_stream_t stream);
 
 /* ---- alphabet and score matrix (src/alphabet_coder.cpp:25-50,
  *      src/score_matrix_reader.cpp:57-96, src/reduced_alphabet_coder.cpp:32-98,
@@ -286,6 +290,8 @@ typedef struct {
   double seed_rep_kernel_ms;   /* HIP events around the seed kernels */
   double seed_value_kernel_ms;
   uint64_t seed_rep_launches, seed_value_launches, gapped_launches;
+  double schedule_kernel_ms;   /* device ordering of gapped jobs (features,
+                                  cost model, counting sort, gathers) */
 } gz_align_stats;
 typedef struct {
   uint32_t query_position; /* concatenated seed position */
@@ -334,6 +340,13 @@ gz_status gz_mlp_train_step(gz_mlp_t m, const uint8_t *codes, const float *targe
 void gz_cost_features(const uint8_t *queries, const uint8_t *database,
                       const uint32_t *query_positions, const uint32_t *database_positions,
                       size_t n, int reverse, uint8_t *out);
+/* The same codes computed on the device from packed sequences (all pointers
+ * are device pointers; asynchronous on `stream`). */
+gz_status gz_cost_features_packed_async(const uint64_t *d_packed_queries,
+                                        const uint64_t *d_packed_database,
+                                        const uint32_t *d_query_positions,
+                                        const uint32_t *d_database_positions, size_t n,
+                                        int reverse, uint8_t *d_out, gz_stream_t stream);
 
 /* ---- host preparation pieces (tests) ---- */
 int gz_seg_mask(const char *sequence, char *masked);    /* SequenceSegFilter */

@@ -47,7 +47,8 @@ class CpuDevice : public PresearchDevice {
     return 0;
   }
 
-  int SeedHits(size_t ci, std::vector<SeedHit> *hits, AlignStats *st) override {
+  int SeedHits(size_t ci, uint32_t nq, SeedHitView *view, AlignStats *st) override {
+    std::vector<SeedHit> *hits = &raw_;
     const DbChunk &c = db_.chunks[ci];
     const uint8_t *qs = concat_->data();
     const uint8_t *ds = c.seq.data();
@@ -93,13 +94,23 @@ class CpuDevice : public PresearchDevice {
         }
       }
     }
-    // the device returns seeds in no particular order
+    // the device returns seeds grouped by query, in no particular order
     std::reverse(hits->begin(), hits->end());
+    qstart_.assign(nq + 1, 0);
+    for (const SeedHit &h : *hits) ++qstart_[h.qid + 1];
+    for (uint32_t k = 0; k < nq; ++k) qstart_[k + 1] += qstart_[k];
+    grouped_.resize(hits->size());
+    std::vector<uint32_t> fill(qstart_.begin(), qstart_.end() - 1);
+    for (const SeedHit &h : *hits) grouped_[fill[h.qid]++] = h;
+    view->hits = grouped_.data();
+    view->count = grouped_.size();
+    view->qstart = qstart_.data();
     return 0;
   }
 
   int GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, uint32_t *dpos,
-                   int *scores, uint32_t *cells, AlignStats *) override {
+                   int *scores, uint32_t *cells, const uint16_t *, bool,
+                   AlignStats *) override {
     const uint8_t *qs = concat_->data();
     const uint8_t *ds = db_.chunks[ci].seq.data();
     for (uint32_t k = 0; k < nr + nf; ++k) {
@@ -133,6 +144,8 @@ class CpuDevice : public PresearchDevice {
   const QueryLists *lists_ = nullptr;
   std::vector<int> cut_, trig_;
   int presearch_cutoff_ = 0;
+  std::vector<SeedHit> raw_, grouped_;
+  std::vector<uint32_t> qstart_;
 };
 
 std::vector<FastaRecord> Records(const char *const *names, const char *residues,

@@ -250,3 +250,21 @@ def test_gapped_kernel_reference_gpu_test_answers(dev):
     hs, hq, hd = _gapped_gpu(dev, enc("AGTCAC"), s1, np.array([1], np.uint32),
                              np.array([1], np.uint32), False, INT_MAX, 0, 1, mat, 4, 5)
     assert (hs[0], hq[0], hd[0]) == (7, 5, 4)
+
+
+def test_gapped_kernel_vs_golden_fixture(dev):
+    """GappedKernel against the reference's own GappedExtender outputs
+    (tests/golden/gapped_extender.npz)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "gapped_extender.npz"))
+    q, d = g["query"], g["database"]
+    for reverse in (0, 1):
+        for cutoff in np.unique(g["cutoff"]):
+            sel = np.nonzero((g["reverse"] == reverse) & (g["cutoff"] == cutoff))[0]
+            say("golden gapped", reverse, int(cutoff), sel.size)
+            hs, hq, hd = _gapped_gpu(dev, q, d, g["qstart"][sel].astype(np.uint32),
+                                     g["dstart"][sel].astype(np.uint32), reverse, int(cutoff))
+            assert np.array_equal(hs, g["score"][sel])
+            assert np.array_equal(hq.astype(np.int64), g["qstart"][sel] + g["p0"][sel])
+            assert np.array_equal(hd.astype(np.int64), g["dstart"][sel] + g["p1"][sel])

@@ -106,3 +106,25 @@ def test_training_fits_a_learnable_target(device):
     base = np.abs(t - t.mean()).mean()
     say("loss", first, "->", loss, "mae", mae, "constant-predictor mae", base)
     assert mae < 0.5 * base
+
+
+def test_device_features_match_host_features(device):
+    """Feature codes gathered on the device from packed sequences equal the
+    host's (the codes the network is trained on)."""
+    from helpers import related_pair_set
+    rng = np.random.default_rng(12)
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 3000, min_len=5, max_len=90)
+    _, pq = gz.pack_on_device(q)
+    _, pd = gz.pack_on_device(d)
+    for reverse in (0, 1):
+        a, b = (qpos - 1, dpos - 1) if reverse else (qpos, dpos)
+        a = np.ascontiguousarray(a, np.uint32)
+        b = np.ascontiguousarray(b, np.uint32)
+        want = gz.cost_features(q, d, a, b, reverse)
+        da, db_ = gz.DeviceBuffer.from_array(a), gz.DeviceBuffer.from_array(b)
+        out = gz.DeviceBuffer(a.size * 128)
+        gz.check(gz.lib().gz_cost_features_packed_async(pq.ptr, pd.ptr, da.ptr, db_.ptr, a.size,
+                                                         reverse, out.ptr, None))
+        got = out.to_array(np.uint8, a.size * 128).reshape(-1, 128)
+        say("features reverse", reverse, "seeds", a.size)
+        assert np.array_equal(got, want)
