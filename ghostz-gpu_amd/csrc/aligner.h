@@ -49,6 +49,8 @@ class Aligner : public PresearchDevice {
   int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward, uint32_t *qpos,
                    uint32_t *dpos, int *scores, uint32_t *cells, const uint16_t *keys,
                    bool learned, AlignStats *st) override;
+  int Tracebacks(size_t chunk, const std::vector<TracebackJob> &jobs, int cutoff,
+                 std::vector<TracebackOut> *out, AlignStats *st) override;
   void EndQueryChunk() override;
 
  private:
@@ -71,6 +73,10 @@ class Aligner : public PresearchDevice {
   // gapped-stage workspace (grown on demand)
   void *gapped_ws_ = nullptr;
   size_t gapped_ws_cap_ = 0;
+  // result-build workspace
+  void *tb_ws_ = nullptr;
+  size_t tb_ws_cap_ = 0;
+  uint32_t max_query_length_ = 0;
 };
 
 }  // namespace gz

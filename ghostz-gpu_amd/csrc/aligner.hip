@@ -8,6 +8,7 @@
 #include "cost_model_gpu.h"
 #include "gz_scan.h"
 #include "schedule.h"
+#include "traceback.h"
 #include "cost_model.h"
 #include "gz_runtime.h"
 #include "gz_common.h"
@@ -67,6 +68,7 @@ Aligner::~Aligner() {
   EndQueryChunk();
   chunks_.clear();
   if (gapped_ws_) (void)hipFree(gapped_ws_);
+  if (tb_ws_) (void)hipFree(tb_ws_);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
   if (gapped_) gz_gapped_extender_destroy(gapped_);
@@ -115,6 +117,7 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
                              AlignStats *st) {
   (void)st;
   EndQueryChunk();
+  max_query_length_ = q.max_length;
   hipStream_t s = stream_;
   uint64_t *d_q = nullptr;
   int *d_cut = nullptr, *d_trig = nullptr;
@@ -314,6 +317,50 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
   st->gapped_kernel_ms += ms;
   st->schedule_kernel_ms += ms2;
   st->gapped_launches += (nr > 0) + (nf > 0);
+  return 0;
+}
+
+int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cutoff,
+                        std::vector<TracebackOut> *out, AlignStats *st) {
+  const size_t n = jobs.size();
+  out->resize(n);
+  if (n == 0) return 0;
+  hipStream_t s = stream_;
+  for (hipEvent_t &e : ev_)
+    if (!e) GZ_TRY(hipEventCreate(&e));
+  const LaneCaps caps = TracebackCaps(max_query_length_, db_->max_length);
+  // lanes resident at once, bounded by an 8 GB scratch budget
+  const size_t budget = (size_t)8 << 30;
+  size_t lanes = std::min<size_t>((n + 63) / 64 * 64, (size_t)1 << 16);
+  lanes = std::max<size_t>(64, std::min(lanes, budget / caps.bytes_per_lane / 64 * 64));
+  const size_t a_jobs = AlignUp(n * sizeof(TracebackJob), 256);
+  const size_t a_outs = AlignUp(n * sizeof(TracebackOut), 256);
+  const size_t bytes = a_jobs + a_outs + lanes * caps.bytes_per_lane;
+  if (tb_ws_cap_ < bytes) {
+    if (tb_ws_) (void)hipFree(tb_ws_);
+    tb_ws_ = nullptr;
+    tb_ws_cap_ = 0;
+    GZ_TRY(hipMalloc(&tb_ws_, bytes));
+    tb_ws_cap_ = bytes;
+  }
+  TracebackJob *d_jobs = (TracebackJob *)tb_ws_;
+  TracebackOut *d_outs = (TracebackOut *)((char *)tb_ws_ + a_jobs);
+  uint8_t *scratch = (uint8_t *)tb_ws_ + a_jobs + a_outs;
+  hipError_t e = hipMemcpyAsync(d_jobs, jobs.data(), n * sizeof(TracebackJob),
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipEventRecord(ev_[2], s);
+  if (e == hipSuccess)
+    e = TracebackLaunch(dq_.packed, chunks_[ci]->packed, d_jobs, (uint32_t)n, d_matrix_,
+                        opt_.search.nl, opt_.search.gap_open, opt_.search.gap_extension, cutoff,
+                        scratch, caps, (uint32_t)lanes, d_outs, s);
+  if (e == hipSuccess) e = hipEventRecord(ev_[3], s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out->data(), d_outs, n * sizeof(TracebackOut), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[2], ev_[3]);
+  if (e != hipSuccess) return (int)e;
+  st->traceback_kernel_ms += ms;
   return 0;
 }
 

@@ -14,8 +14,11 @@
 #include "cost_model_gpu.h"
 #include "database.h"
 #include "gz_common.h"
+#include "gz_runtime.h"
+#include "gz_scan.h"
 #include "queries.h"
 #include "seg.h"
+#include "traceback.h"
 
 using namespace gz;
 
@@ -226,6 +229,53 @@ gz_status gz_database_chunk(gz_database_t h, uint32_t i, gz_database_chunk_view_
   v->number_members = c.mem_pos.size();
   v->number_values = c.val_pos.size();
   v->clustering = c.clustering ? 1 : 0;
+  return GZ_OK;
+}
+
+// ---- result build ----
+static_assert(sizeof(gz_traceback_job) == sizeof(TracebackJob), "job layout");
+static_assert(sizeof(gz_traceback_result) == sizeof(TracebackOut), "result layout");
+
+gz_status gz_build_results(const uint64_t *pq, const uint64_t *pd, const gz_traceback_job *jobs,
+                           size_t n, uint32_t max_query_length, uint32_t max_subject_length,
+                           const int *d_matrix, uint32_t nl, int gap_open, int gap_ext,
+                           int cutoff, gz_traceback_result *results, gz_stream_t stream) {
+  GZ_REQUIRE(pq && pd && d_matrix && (n == 0 || (jobs && results)) && nl <= 32,
+             "gz_build_results: bad argument");
+  GZ_REQUIRE(n < (1ull << 31), "gz_build_results: too many jobs");
+  if (n == 0) return GZ_OK;
+  hipStream_t s = stream ? stream->s : nullptr;
+  const LaneCaps caps = TracebackCaps(max_query_length, max_subject_length);
+  const size_t lanes = std::min<size_t>((n + 63) / 64 * 64, (size_t)1 << 14);
+  void *ws = nullptr;
+  const size_t a_jobs = AlignUp(n * sizeof(TracebackJob), 256);
+  const size_t a_outs = AlignUp(n * sizeof(TracebackOut), 256);
+  GZ_HIP_CHECK(hipMalloc(&ws, a_jobs + a_outs + lanes * caps.bytes_per_lane));
+  hipError_t e = hipMemcpyAsync(ws, jobs, n * sizeof(TracebackJob), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = TracebackLaunch(pq, pd, (const TracebackJob *)ws, (uint32_t)n, d_matrix, nl, gap_open,
+                        gap_ext, cutoff, (uint8_t *)ws + a_jobs + a_outs, caps, (uint32_t)lanes,
+                        (TracebackOut *)((char *)ws + a_jobs), s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(results, (char *)ws + a_jobs, n * sizeof(TracebackOut),
+                       hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(ws);
+  GZ_HIP_CHECK(e);
+  return GZ_OK;
+}
+
+gz_status gz_build_result_host(const uint8_t *query, uint32_t query_length,
+                               const uint8_t *database, uint32_t hit_q, uint32_t hit_d,
+                               int cutoff, gz_traceback_result *result) {
+  GZ_REQUIRE(query && database && result && hit_q >= 1 && hit_q < query_length,
+             "gz_build_result_host: bad argument");
+  std::vector<int> m = Blosum();
+  SearchParameters p;
+  p.matrix = m.data();
+  p.nl = kNumberLetters;
+  const TracebackOut o = TracebackOneHost(query, query_length, database, hit_q, hit_d, p, cutoff);
+  memcpy(result, &o, sizeof(o));
   return GZ_OK;
 }
 

@@ -220,12 +220,74 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     });
     st->seconds_add += Now() - t;
   }
-  dev->EndQueryChunk();
-
-  // BuildResults and WriteOutput
+  // BuildResults (on the device where it builds results) and WriteOutput
   t = Now();
-  std::vector<std::vector<AlignmentResult> > results;
-  BuildResults(q, db, P, result_cutoff, pre, results, threads, &st->sequential_build_chunks);
+  std::vector<std::vector<AlignmentResult> > results(nq);
+  for (size_t i = 0; i < nq; ++i) results[i].resize(pre[i].size());
+  for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
+    const DbChunk &chunk = db.chunks[ci];
+    const std::vector<std::pair<uint32_t, uint32_t> > ids = ChunkResultIds(pre, ci);
+    if (ids.empty()) continue;
+    std::vector<TracebackJob> jobs(ids.size());
+    for (size_t k = 0; k < ids.size(); ++k) {
+      const uint32_t qi = ids[k].first;
+      const Presearched &pr = pre[qi][ids[k].second];
+      jobs[k].query_offset = (uint32_t)coff[qi];
+      jobs[k].query_length = q.Length(qi);
+      jobs[k].hit_q = pr.hit.q;
+      jobs[k].hit_d = pr.hit.d;
+    }
+    std::vector<TracebackOut> outs;
+    const int rc = dev->Tracebacks(ci, jobs, result_cutoff, &outs, st);
+    if (rc == PresearchDevice::kTracebackUnsupported) {
+      BuildChunkHost(q, chunk, P, result_cutoff, pre, results, ids, threads,
+                     &st->sequential_build_chunks);
+      continue;
+    }
+    GZ_STEP(rc);
+    bool stale = false;
+    std::vector<size_t> host_jobs;
+    for (size_t k = 0; k < ids.size(); ++k) {
+      if (outs[k].flags & kTracebackStale) stale = true;
+      if (outs[k].flags & kTracebackOverflow) host_jobs.push_back(k);
+    }
+    st->traceback_host_jobs += host_jobs.size();
+    if (!stale) {
+      ParallelBlocks(ids.size(), threads, 4096, [&](size_t b, size_t e) {
+        for (size_t k = b; k < e; ++k) {
+          if (outs[k].flags) continue;
+          const Presearched &pr = pre[ids[k].first][ids[k].second];
+          const TracebackOut &o = outs[k];
+          AlignmentResult &r = results[ids[k].first][ids[k].second];
+          r.chunk = pr.chunk;
+          r.subject = pr.subject;
+          r.score = o.score;
+          r.length = o.length;
+          r.mismatches = o.mismatches;
+          r.gap_openings = o.gap_openings;
+          r.gap_extensions = o.gap_extensions;
+          r.hit = pr.hit;
+          r.start.q = o.start_q;
+          r.start.d = o.start_d;
+          r.end.q = o.end_q;
+          r.end.d = o.end_d;
+        }
+      });
+      for (size_t k : host_jobs) {
+        const uint32_t qi = ids[k].first, r = ids[k].second;
+        if (!BuildOneFresh(q, qi, chunk, pre[qi][r], P, result_cutoff, &results[qi][r])) {
+          stale = true;
+          break;
+        }
+      }
+    }
+    if (stale) {
+      ++st->sequential_build_chunks;
+      ReplayChunkHost(q, chunk, P, result_cutoff, pre, results, ids);
+    }
+  }
+  SortResults(results);
+  dev->EndQueryChunk();
   for (auto &r : results) st->results += r.size();
   st->seconds_build += Now() - t;
   t = Now();

@@ -62,6 +62,8 @@ struct AlignStats {
   uint64_t sequential_build_chunks = 0;
   double gapped_kernel_ms = 0;    // measured with HIP events on the stream
   double schedule_kernel_ms = 0;  // ordering of the gapped jobs on the device
+  double traceback_kernel_ms = 0; // result build on the device
+  uint64_t traceback_host_jobs = 0;  // results the device left to the host
   double seed_kernel_ms[2] = {0, 0};  // seed kernels: representatives, values
   uint64_t seed_launches[2] = {0, 0};
   uint64_t gapped_launches = 0;
@@ -115,7 +117,19 @@ class PresearchDevice {
   virtual int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward,
                            uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
                            const uint16_t *keys, bool learned, AlignStats *st) = 0;
+  // Result build of `jobs` of database chunk `chunk` with the result cutoff.
+  // Returns kTracebackUnsupported when the device does not build results.
+  virtual int Tracebacks(size_t chunk, const std::vector<TracebackJob> &jobs, int cutoff,
+                         std::vector<TracebackOut> *out, AlignStats *st) {
+    (void)chunk;
+    (void)jobs;
+    (void)cutoff;
+    (void)out;
+    (void)st;
+    return kTracebackUnsupported;
+  }
   virtual void EndQueryChunk() = 0;
+  static constexpr int kTracebackUnsupported = -1;
 };
 
 // Aligns all queries (split into chunks by queries_chunk_size) against all

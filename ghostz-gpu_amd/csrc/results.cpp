@@ -384,15 +384,11 @@ class Traceback {
 
 // BuildResults for one presearched result (aligner_build_results_thread.h
 // :97-150 + AlignerCommon::BuildResult, src/aligner_common.h:276-343).
-void BuildOne(Traceback &tb, const QuerySet &q, uint32_t qi, const DbChunk &c,
-              const Presearched &pr, const SearchParameters &p, int cutoff,
-              AlignmentResult *out) {
-  const uint8_t *qs = q.Sequence(qi);
-  const uint32_t qlen = q.Length(qi);
-  const uint8_t *ds = c.seq.data();
+void BuildOneRaw(Traceback &tb, const uint8_t *qs, uint32_t qlen, const uint8_t *ds,
+                 const Coordinate hit, const SearchParameters &p, int cutoff,
+                 AlignmentResult *out) {
   Blocks rev, fwd;
   int s1, s2, p0, p1;
-  const Coordinate hit = pr.hit;
   tb.Extend(qs + hit.q - 1, hit.q - 1, ds + hit.d - 1, true, p.matrix, p.nl, p.gap_open,
             p.gap_extension, cutoff, &s1, &p0, &p1, &rev);
   Coordinate start = {hit.q - 1 + p0, hit.d - 1 + p1};
@@ -423,8 +419,6 @@ void BuildOne(Traceback &tb, const QuerySet &q, uint32_t qi, const DbChunk &c,
       ++len;
     }
   }
-  out->chunk = pr.chunk;
-  out->subject = pr.subject;
   out->score = s1 + s2;
   out->length = len;
   out->mismatches = m;
@@ -435,7 +429,107 @@ void BuildOne(Traceback &tb, const QuerySet &q, uint32_t qi, const DbChunk &c,
   out->end = end;
 }
 
+void BuildOne(Traceback &tb, const QuerySet &q, uint32_t qi, const DbChunk &c,
+              const Presearched &pr, const SearchParameters &p, int cutoff,
+              AlignmentResult *out) {
+  BuildOneRaw(tb, q.Sequence(qi), q.Length(qi), c.seq.data(), pr.hit, p, cutoff, out);
+  out->chunk = pr.chunk;
+  out->subject = pr.subject;
+}
+
 }  // namespace
+
+TracebackOut TracebackOneHost(const uint8_t *query, uint32_t query_length,
+                              const uint8_t *database, uint32_t hit_q, uint32_t hit_d,
+                              const SearchParameters &p, int cutoff) {
+  Traceback tb;
+  AlignmentResult r;
+  const Coordinate hit = {hit_q, hit_d};
+  BuildOneRaw(tb, query, query_length, database, hit, p, cutoff, &r);
+  TracebackOut o;
+  o.score = r.score;
+  o.start_q = r.start.q;
+  o.start_d = r.start.d;
+  o.end_q = r.end.q;
+  o.end_d = r.end.d;
+  o.length = r.length;
+  o.mismatches = r.mismatches;
+  o.gap_openings = r.gap_openings;
+  o.gap_extensions = r.gap_extensions;
+  o.flags = tb.stale ? kTracebackStale : 0;
+  return o;
+}
+
+bool BuildOneFresh(const QuerySet &q, uint32_t qi, const DbChunk &c, const Presearched &pr,
+                   const SearchParameters &p, int cutoff, AlignmentResult *out) {
+  Traceback tb;
+  BuildOne(tb, q, qi, c, pr, p, cutoff, out);
+  return tb.stale == 0;
+}
+
+void ReplayChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters &p, int cutoff,
+                     std::vector<std::vector<Presearched> > &pre,
+                     std::vector<std::vector<AlignmentResult> > &results,
+                     const std::vector<std::pair<uint32_t, uint32_t> > &ids) {
+  // one extender object, pairs handed out in blocks of 32 and taken from the
+  // back of each block (aligner_build_results_thread.h:97-150, one thread)
+  Traceback tb;
+  for (size_t b = 0; b < ids.size(); b += 32) {
+    const size_t e = std::min(ids.size(), b + 32);
+    for (size_t k = e; k-- > b;) {
+      const uint32_t qi = ids[k].first, r = ids[k].second;
+      BuildOne(tb, q, qi, c, pre[qi][r], p, cutoff, &results[qi][r]);
+    }
+  }
+}
+
+void BuildChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters &p, int cutoff,
+                    std::vector<std::vector<Presearched> > &pre,
+                    std::vector<std::vector<AlignmentResult> > &results,
+                    const std::vector<std::pair<uint32_t, uint32_t> > &ids, int threads,
+                    uint64_t *sequential_chunks) {
+  if (ids.empty()) return;
+  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  threads = std::max(1, threads);
+  std::atomic<size_t> next(0);
+  std::atomic<uint64_t> stale(0);
+  const int T = std::max(1, std::min<int>(threads, (int)(ids.size() / 64) + 1));
+  auto work = [&]() {
+    Traceback tb;
+    for (;;) {
+      const size_t b = next.fetch_add(256);
+      if (b >= ids.size()) break;
+      for (size_t k = b; k < std::min(ids.size(), b + 256); ++k) {
+        const uint32_t qi = ids[k].first, r = ids[k].second;
+        BuildOne(tb, q, qi, c, pre[qi][r], p, cutoff, &results[qi][r]);
+      }
+    }
+    stale += tb.stale;
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  if (stale.load() != 0) {
+    // history-dependent bytes were read: replay the chunk exactly as the
+    // reference's one thread does
+    if (sequential_chunks) ++*sequential_chunks;
+    ReplayChunkHost(q, c, p, cutoff, pre, results, ids);
+  }
+}
+
+std::vector<std::pair<uint32_t, uint32_t> > ChunkResultIds(
+    const std::vector<std::vector<Presearched> > &pre, size_t ci) {
+  std::vector<std::pair<uint32_t, uint32_t> > ids;
+  for (uint32_t qi = 0; qi < pre.size(); ++qi)
+    for (uint32_t r = 0; r < pre[qi].size(); ++r)
+      if (pre[qi][r].chunk == (int)ci) ids.push_back(std::make_pair(qi, r));
+  return ids;
+}
+
+void SortResults(std::vector<std::vector<AlignmentResult> > &results) {
+  for (auto &r : results) std::sort(r.begin(), r.end(), ResultByScore());
+}
 
 void BuildResults(const QuerySet &q, const Database &db, const SearchParameters &p,
                   int cutoff, std::vector<std::vector<Presearched> > &pre,
@@ -444,51 +538,10 @@ void BuildResults(const QuerySet &q, const Database &db, const SearchParameters 
   const size_t nq = q.size();
   results.assign(nq, std::vector<AlignmentResult>());
   for (size_t i = 0; i < nq; ++i) results[i].resize(pre[i].size());
-  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
-  threads = std::max(1, threads);
-  for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
-    std::vector<std::pair<uint32_t, uint32_t> > ids;
-    for (uint32_t qi = 0; qi < nq; ++qi)
-      for (uint32_t r = 0; r < pre[qi].size(); ++r)
-        if (pre[qi][r].chunk == (int)ci) ids.push_back(std::make_pair(qi, r));
-    if (ids.empty()) continue;
-    const DbChunk &c = db.chunks[ci];
-    std::atomic<size_t> next(0);
-    std::atomic<uint64_t> stale(0);
-    const int T = std::max(1, std::min<int>(threads, (int)(ids.size() / 64) + 1));
-    auto work = [&]() {
-      Traceback tb;
-      for (;;) {
-        const size_t b = next.fetch_add(256);
-        if (b >= ids.size()) break;
-        for (size_t k = b; k < std::min(ids.size(), b + 256); ++k) {
-          const uint32_t qi = ids[k].first, r = ids[k].second;
-          BuildOne(tb, q, qi, c, pre[qi][r], p, cutoff, &results[qi][r]);
-        }
-      }
-      stale += tb.stale;
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < T; ++t) pool.emplace_back(work);
-    work();
-    for (auto &t : pool) t.join();
-    if (stale.load() != 0) {
-      // history-dependent bytes were read: replay the chunk exactly as the
-      // reference's one thread does -- one extender object, pairs handed out
-      // in blocks of 32 and taken from the back of each block
-      if (sequential_chunks) ++*sequential_chunks;
-      Traceback tb;
-      for (size_t b = 0; b < ids.size(); b += 32) {
-        const size_t e = std::min(ids.size(), b + 32);
-        for (size_t k = e; k-- > b;) {
-          const uint32_t qi = ids[k].first, r = ids[k].second;
-          BuildOne(tb, q, qi, c, pre[qi][r], p, cutoff, &results[qi][r]);
-        }
-      }
-    }
-  }
-  for (size_t i = 0; i < nq; ++i)
-    std::sort(results[i].begin(), results[i].end(), ResultByScore());
+  for (size_t ci = 0; ci < db.chunks.size(); ++ci)
+    BuildChunkHost(q, db.chunks[ci], p, cutoff, pre, results, ChunkResultIds(pre, ci), threads,
+                   sequential_chunks);
+  SortResults(results);
 }
 
 // ---------------------------------------------------------------------------

@@ -6,6 +6,7 @@
 
 #include <ostream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "database.h"
@@ -40,6 +41,22 @@ struct AlignmentResult {
   int score;
   uint32_t length, mismatches, gap_openings, gap_extensions;
   Coordinate hit, start, end;
+};
+
+// One result build (traceback re-extension of both sides of a presearched
+// result) and its outcome.
+struct TracebackJob {
+  uint32_t query_offset;  // concatenated position of the query's leading delimiter
+  uint32_t query_length;  // framed length
+  uint32_t hit_q, hit_d;  // query-local / chunk positions of the hit
+};
+constexpr uint32_t kTracebackStale = 1;     // read a byte the extension did not write
+constexpr uint32_t kTracebackOverflow = 2;  // beyond the device scratch: redo on the host
+struct TracebackOut {
+  int score;
+  uint32_t start_q, start_d, end_q, end_d;
+  uint32_t length, mismatches, gap_openings, gap_extensions;
+  uint32_t flags;
 };
 
 struct SearchParameters {
@@ -79,6 +96,32 @@ void BuildResults(const QuerySet &q, const Database &db, const SearchParameters 
                   std::vector<std::vector<Presearched> > &presearched,
                   std::vector<std::vector<AlignmentResult> > &results, int threads,
                   uint64_t *sequential_chunks);
+
+// Pieces of BuildResults for callers that build some results elsewhere:
+// (query, result) pairs of chunk ci in the reference order; the host build of
+// a chunk (parallel, replayed in the reference order when a traceback read a
+// byte left by an earlier extension); the sequential replay itself; one
+// result with a fresh extender (false when it read such a byte); the final
+// per-query order.
+std::vector<std::pair<uint32_t, uint32_t> > ChunkResultIds(
+    const std::vector<std::vector<Presearched> > &pre, size_t ci);
+void BuildChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters &p, int cutoff,
+                    std::vector<std::vector<Presearched> > &pre,
+                    std::vector<std::vector<AlignmentResult> > &results,
+                    const std::vector<std::pair<uint32_t, uint32_t> > &ids, int threads,
+                    uint64_t *sequential_chunks);
+void ReplayChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters &p, int cutoff,
+                     std::vector<std::vector<Presearched> > &pre,
+                     std::vector<std::vector<AlignmentResult> > &results,
+                     const std::vector<std::pair<uint32_t, uint32_t> > &ids);
+bool BuildOneFresh(const QuerySet &q, uint32_t qi, const DbChunk &c, const Presearched &pr,
+                   const SearchParameters &p, int cutoff, AlignmentResult *out);
+void SortResults(std::vector<std::vector<AlignmentResult> > &results);
+// The result build of one hit with a fresh extender on the framed query
+// (query[0] is its leading delimiter) and the chunk codes.
+TracebackOut TracebackOneHost(const uint8_t *query, uint32_t query_length,
+                              const uint8_t *database, uint32_t hit_q, uint32_t hit_d,
+                              const SearchParameters &p, int cutoff);
 
 // AlignerCommon::WriteOutput (src/aligner_common.h:345-397).
 void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,

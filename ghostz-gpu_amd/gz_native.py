@@ -307,6 +307,10 @@ _SIGNATURES.update({
     "gz_align_fasta": (c_int, [c_char_p, c_char_p, c_char_p, c_int,
                                POINTER(AlignerParameters), POINTER(DatabaseOptions)]),
     "gz_free": (None, [c_void_p]),
+    "gz_build_results": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_uint32, c_uint32,
+                                 c_void_p, c_uint32, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "gz_build_result_host": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_uint32, c_int,
+                                     c_void_p]),
     "gz_mlp_weight_count": (c_size_t, []),
     "gz_mlp_init_weights": (c_int, [c_uint64, c_void_p]),
     "gz_mlp_create": (c_int, [c_void_p, c_size_t, POINTER(c_void_p)]),

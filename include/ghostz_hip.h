@@ -320,6 +320,34 @@ gz_status gz_align_fasta(const char *queries_path, const char *database_path,
                          const gz_database_options *database_options);
 void gz_free(void *p);
 
+/* ---- result build (AlignerBuildResultsThread, aligner_build_results_thread.h
+ *      :64-190 with GappedExtender's traceback, gapped_extender.cpp:70-304,473-531,
+ *      and AlignerCommon::BuildResult, aligner_common.h:276-343) ---- */
+typedef struct {
+  uint32_t query_offset; /* concatenated position of the query's leading delimiter */
+  uint32_t query_length; /* framed length */
+  uint32_t hit_q, hit_d; /* query-local / chunk positions of the hit */
+} gz_traceback_job;
+#define GZ_TRACEBACK_STALE 1    /* read a byte the extension did not write */
+#define GZ_TRACEBACK_OVERFLOW 2 /* beyond the device scratch: build on the host */
+typedef struct {
+  int score;
+  uint32_t start_q, start_d, end_q, end_d;
+  uint32_t length, mismatches, gap_openings, gap_extensions;
+  uint32_t flags;
+} gz_traceback_result;
+/* Re-extends both sides of every job with traceback at `cutoff` on the
+ * device and counts the alignment; synchronous. */
+gz_status gz_build_results(const uint64_t *d_packed_queries, const uint64_t *d_packed_database,
+                           const gz_traceback_job *jobs, size_t n, uint32_t max_query_length,
+                           uint32_t max_subject_length, const int *d_matrix,
+                           uint32_t number_letters, int gap_open, int gap_extension, int cutoff,
+                           gz_traceback_result *results, gz_stream_t stream);
+/* The same for one job on the host (BLOSUM62 11/1), from unpacked codes. */
+gz_status gz_build_result_host(const uint8_t *query, uint32_t query_length,
+                               const uint8_t *database, uint32_t hit_q, uint32_t hit_d,
+                               int cutoff, gz_traceback_result *result);
+
 /* ---- learned extension-cost model (no counterpart in the reference: it
  *      only orders gapped extensions for load balance) ----
  * Flat fp32 layout: embed[21][8], w1[256][1024], b1[256], w2[64][256],

@@ -25,6 +25,7 @@
 #include "gz_oracle.h"
 #include "presearch.h"
 #include "queries.h"
+#include "results.h"
 
 using namespace gz;
 
@@ -125,6 +126,19 @@ class CpuDevice : public PresearchDevice {
       dpos[k] += p1;
       scores[k] = s;
       if (cells) cells[k] = (uint32_t)c;
+    }
+    return 0;
+  }
+
+  int Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cutoff,
+                 std::vector<TracebackOut> *out, AlignStats *) override {
+    out->resize(jobs.size());
+    for (size_t k = 0; k < jobs.size(); ++k) {
+      const TracebackJob &j = jobs[k];
+      (*out)[k] = TracebackOneHost(concat_->data() + j.query_offset, j.query_length,
+                                   db_.chunks[ci].seq.data(), j.hit_q, j.hit_d, p_, cutoff);
+      // exercise the host fall-back path too
+      if (k % 97 == 5) (*out)[k].flags |= kTracebackOverflow;
     }
     return 0;
   }

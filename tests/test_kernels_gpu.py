@@ -268,3 +268,42 @@ def test_gapped_kernel_vs_golden_fixture(dev):
             assert np.array_equal(hs, g["score"][sel])
             assert np.array_equal(hq.astype(np.int64), g["qstart"][sel] + g["p0"][sel])
             assert np.array_equal(hd.astype(np.int64), g["dstart"][sel] + g["p1"][sel])
+
+
+def _tb_jobs(rng, n, **kw):
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, n, **kw)
+    jobs = np.zeros((n, 4), np.uint32)
+    for i in range(n):
+        lead = int(qo[i]) - 1
+        jobs[i] = (lead, int(qo[i + 1] - qo[i]) + 1, int(qpos[i]) - lead, int(dpos[i]))
+    return q, d, jobs
+
+
+@pytest.mark.parametrize("cutoff", [52, 26, 8])
+def test_result_build_kernel_vs_host(dev, cutoff):
+    """gz_build_results (one lane per result, the reference's gap_states_
+    layout) equals the host result build, field by field and in the stale
+    flag, on homologous pairs with indels."""
+    rng = np.random.default_rng(40 + cutoff)
+    q, d, jobs = _tb_jobs(rng, 1200, min_len=5, max_len=400, identity=(0.25, 0.95), indel=0.05)
+    say("result build jobs", len(jobs), "cutoff", cutoff)
+    _, pq = gz.pack_on_device(q)
+    _, pd = gz.pack_on_device(d)
+    out = np.zeros((len(jobs), 10), np.uint32)
+    qlens = jobs[:, 1]
+    gz.check(gz.lib().gz_build_results(pq.ptr, pd.ptr, gz.ptr(jobs), len(jobs), int(qlens.max()),
+                                       400, dev["d_matrix"].ptr, dev["nl"], 11, 1, cutoff,
+                                       gz.ptr(out), None))
+    host = np.zeros(10, np.uint32)
+    overflow = stale = 0
+    for k, (lead, qlen, hq, hd) in enumerate(jobs):
+        qs = np.ascontiguousarray(q[lead:lead + qlen])
+        gz.check(gz.lib().gz_build_result_host(gz.ptr(qs), int(qlen), gz.ptr(d), int(hq), int(hd),
+                                               cutoff, gz.ptr(host)))
+        if out[k, 9] & 2:
+            overflow += 1
+            continue
+        stale += int(host[9] != 0)
+        assert np.array_equal(out[k], host), (k, out[k], host)
+    say("  overflow", overflow, "stale", stale)
+    assert overflow < len(jobs) // 20
