@@ -276,7 +276,7 @@ def _tb_jobs(rng, n, **kw):
     for i in range(n):
         lead = int(qo[i]) - 1
         jobs[i] = (lead, int(qo[i + 1] - qo[i]) + 1, int(qpos[i]) - lead, int(dpos[i]))
-    return q, d, jobs
+    return q, d, jobs, int(np.diff(do.astype(np.int64)).max())
 
 
 @pytest.mark.parametrize("cutoff", [52, 26, 8])
@@ -285,14 +285,15 @@ def test_result_build_kernel_vs_host(dev, cutoff):
     layout) equals the host result build, field by field and in the stale
     flag, on homologous pairs with indels."""
     rng = np.random.default_rng(40 + cutoff)
-    q, d, jobs = _tb_jobs(rng, 1200, min_len=5, max_len=400, identity=(0.25, 0.95), indel=0.05)
+    q, d, jobs, dmax = _tb_jobs(rng, 1200, min_len=5, max_len=400, identity=(0.25, 0.95),
+                                indel=0.05)
     say("result build jobs", len(jobs), "cutoff", cutoff)
     _, pq = gz.pack_on_device(q)
     _, pd = gz.pack_on_device(d)
     out = np.zeros((len(jobs), 10), np.uint32)
     qlens = jobs[:, 1]
     gz.check(gz.lib().gz_build_results(pq.ptr, pd.ptr, gz.ptr(jobs), len(jobs), int(qlens.max()),
-                                       400, dev["d_matrix"].ptr, dev["nl"], 11, 1, cutoff,
+                                       dmax, dev["d_matrix"].ptr, dev["nl"], 11, 1, cutoff,
                                        gz.ptr(out), None))
     host = np.zeros(10, np.uint32)
     overflow = stale = 0
