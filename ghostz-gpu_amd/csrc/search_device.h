@@ -83,6 +83,7 @@ class SeedStage {
   void *active_ = nullptr; size_t active_cap_ = 0;
   void *hits_ = nullptr; size_t hits_cap_ = 0;
   void *grouped_ = nullptr; size_t grouped_cap_ = 0;   // hits grouped by query
+  void *tile_range_ = nullptr; size_t tile_range_cap_ = 0;  // per-tile list ranges
   void *qcount_ = nullptr; size_t qcount_cap_ = 0;     // per-query counts / cursors
   void *host_hits_ = nullptr; size_t host_hits_cap_ = 0;  // pinned
   void *host_qstart_ = nullptr; size_t host_qstart_cap_ = 0;  // pinned

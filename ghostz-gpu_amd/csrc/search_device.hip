@@ -15,7 +15,7 @@ namespace gz {
 namespace {
 
 constexpr int kSeedThreads = 256;
-constexpr int kSeedItems = 4;
+constexpr int kSeedItems = 2;
 constexpr uint32_t kSeedTile = kSeedThreads * kSeedItems;
 constexpr uint64_t kUnitsPerLaunch = 1ull << 28;
 constexpr size_t kInitialHitCapacity = 1u << 22;
@@ -152,46 +152,66 @@ __device__ __forceinline__ int Distance(const uint64_t *queries, uint32_t qc,
   return count;
 }
 
+// first and last active list of every tile of kSeedTile units starting at
+// unit_begin (upper bounds over the active lists' unit offsets)
+__global__ void TileRangeKernel(const uint64_t *active_off, uint32_t nactive,
+                                uint64_t unit_begin, uint64_t unit_end, uint32_t tiles,
+                                uint2 *range) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tiles) return;
+  const uint64_t tile = unit_begin + (uint64_t)t * kSeedTile;
+  const uint64_t tile_last = min(tile + (uint64_t)kSeedTile, unit_end) - 1;
+  uint32_t lo = 0, hi = nactive + 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (active_off[mid] <= tile) lo = mid + 1; else hi = mid;
+  }
+  const uint32_t a = lo - 1;
+  hi = nactive + 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (active_off[mid] <= tile_last) lo = mid + 1; else hi = mid;
+  }
+  range[t] = make_uint2(a, lo - a);  // lists a .. lo-1
+}
+
 template <bool kValues>
 __global__ void __launch_bounds__(kSeedThreads)
     SeedKernel(DeviceQueryLists q, DeviceDbChunk db, SeedStageParams p,
                const ListInfo *info, const uint32_t *active,
-               const uint64_t *active_off, uint32_t nactive, uint64_t unit_begin,
-               uint64_t unit_end, SeedHit *hits, uint32_t capacity,
-               uint32_t *counters) {
+               const uint64_t *active_off, const uint2 *tile_range, uint64_t unit_begin,
+               uint64_t unit_end, SeedHit *hits, uint32_t capacity, uint32_t *counters) {
   __shared__ int8_t sm[kTableDim * kTableDim];
   __shared__ uint8_t rmap[kTableDim];
+  // the tile's lists: unit offset, list rank, first entry, entries, first
+  // representative / value, representatives
   __shared__ uint64_t loff[kSeedTile + 1];
-  __shared__ uint32_t lfirst, lcount;
+  __shared__ uint32_t lrank[kSeedTile], leb[kSeedTile], lp[kSeedTile], lbeg[kSeedTile],
+      lnrep[kSeedTile];
   StageScoreTable(sm, p.matrix, p.nl);
   if (threadIdx.x < kTableDim)
     rmap[threadIdx.x] = threadIdx.x < 25 ? p.reduced_map[threadIdx.x]
                                          : (uint8_t)(100 + threadIdx.x);
-  __syncthreads();
   const uint32_t delim = p.delimiter;
   for (uint64_t tile = unit_begin + (uint64_t)blockIdx.x * kSeedTile; tile < unit_end;
        tile += (uint64_t)gridDim.x * kSeedTile) {
     const uint64_t tile_end = min(tile + (uint64_t)kSeedTile, unit_end);
-    if (threadIdx.x == 0) {
-      // active lists covering [tile, tile_end): upper_bound - 1
-      uint32_t lo = 0, hi = nactive + 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (active_off[mid] <= tile) lo = mid + 1; else hi = mid;
+    const uint2 tr = tile_range[(tile - unit_begin) / kSeedTile];
+    const uint32_t first = tr.x, count = tr.y;
+    __syncthreads();  // the previous tile's table is no longer read
+    for (uint32_t i = threadIdx.x; i <= count; i += kSeedThreads) {
+      loff[i] = active_off[first + i];
+      if (i < count) {
+        const uint32_t l = active[first + i];
+        const ListInfo li = info[l];
+        const uint32_t eb = q.entry_off[l];
+        lrank[i] = l;
+        leb[i] = eb;
+        lp[i] = q.entry_off[l + 1] - eb;
+        lbeg[i] = kValues ? li.val_begin : li.rep_begin;
+        lnrep[i] = li.nrep;
       }
-      const uint32_t a = lo - 1;
-      lo = a;
-      hi = nactive + 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (active_off[mid] <= tile_end - 1) lo = mid + 1; else hi = mid;
-      }
-      lfirst = a;
-      lcount = lo - a;  // lists a .. lo-1
     }
-    __syncthreads();
-    const uint32_t first = lfirst, count = lcount;
-    for (uint32_t i = threadIdx.x; i <= count; i += kSeedThreads) loff[i] = active_off[first + i];
     __syncthreads();
 #pragma unroll 1
     for (int j = 0; j < kSeedItems; ++j) {
@@ -208,29 +228,31 @@ __global__ void __launch_bounds__(kSeedThreads)
           const uint32_t mid = (lo + hi) >> 1;
           if (loff[mid] <= u) lo = mid; else hi = mid;
         }
-        const uint32_t l = active[first + lo];
         const uint64_t local = u - loff[lo];
-        const uint32_t eb = q.entry_off[l];
-        const uint32_t P = q.entry_off[l + 1] - eb;
-        const uint32_t idx = (uint32_t)(local / P);
-        const uint32_t e = (uint32_t)(local - (uint64_t)idx * P);
-        const uint32_t ent = eb + e;
+        const uint32_t P = lp[lo];
+        uint32_t idx, e;
+        if (local < 0xffffffffull) {
+          idx = (uint32_t)local / P;
+          e = (uint32_t)local - idx * P;
+        } else {
+          idx = (uint32_t)(local / P);
+          e = (uint32_t)(local - (uint64_t)idx * P);
+        }
+        const uint32_t ent = leb[lo] + e;
         const uint32_t qpos = q.entry_pos[ent];
         const uint32_t qid = q.entry_qid[ent];
-        const ListInfo li = info[l];
+        const uint32_t ri = lbeg[lo] + idx;
+        h.dpos = kValues ? db.val_pos[ri] : db.rep_pos[ri];
         cut = q.cutoffs[qid];
         trig = q.triggers[qid];
         h.qid = qid;
         h.qpos = qpos;
-        h.list = l;
+        h.list = lrank[lo];
         h.entry = e;
         if (kValues) {
-          h.dpos = db.val_pos[li.val_begin + idx];
-          h.rv = li.nrep + idx;
+          h.rv = lnrep[lo] + idx;
           pass = Ungapped(q.packed, qpos, db.packed, h.dpos, sm, delim, cut, trig) > trig;
         } else {
-          const uint32_t ri = li.rep_begin + idx;
-          h.dpos = db.rep_pos[ri];
           h.rv = idx;
           const bool check = q.entry_check[ent] != 0;
           int d = 0;
@@ -260,7 +282,6 @@ __global__ void __launch_bounds__(kSeedThreads)
         }
       }
     }
-    __syncthreads();
   }
 }
 
@@ -286,7 +307,7 @@ __global__ void QueryScatterKernel(const SeedHit *hits, size_t n, uint32_t *curs
 }  // namespace
 
 SeedStage::~SeedStage() {
-  void *ps[] = {lookup_, units_, scan_tmp_, active_, hits_, grouped_, qcount_};
+  void *ps[] = {lookup_, units_, scan_tmp_, active_, hits_, grouped_, qcount_, tile_range_};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (host_hits_) (void)hipHostFree(host_hits_);
@@ -415,15 +436,20 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
         const uint32_t cap = (uint32_t)std::min<size_t>(hits_cap_ / sizeof(SeedHit), 0xffffffffu);
         const uint64_t tiles = (u1 - u0 + kSeedTile - 1) / kSeedTile;
         const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)cus * 8);
+        GZ_TRY(Reserve(&tile_range_, &tile_range_cap_, tiles * sizeof(uint2)));
+        hipLaunchKernelGGL(TileRangeKernel, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0,
+                           s, act_off[k], nact[k], u0, u1, (uint32_t)tiles,
+                           (uint2 *)tile_range_);
+        GZ_TRY(hipGetLastError());
         GZ_TRY(hipEventRecord(ev_[0], s));
         if (k == 0) {
           hipLaunchKernelGGL((SeedKernel<false>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
-                             info, act[0], act_off[0], nact[0], u0, u1, (SeedHit *)hits_, cap,
-                             counters_);
+                             info, act[0], act_off[0], (const uint2 *)tile_range_, u0, u1,
+                             (SeedHit *)hits_, cap, counters_);
         } else {
           hipLaunchKernelGGL((SeedKernel<true>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
-                             info, act[1], act_off[1], nact[1], u0, u1, (SeedHit *)hits_, cap,
-                             counters_);
+                             info, act[1], act_off[1], (const uint2 *)tile_range_, u0, u1,
+                             (SeedHit *)hits_, cap, counters_);
         }
         GZ_TRY(hipGetLastError());
         GZ_TRY(hipEventRecord(ev_[1], s));
