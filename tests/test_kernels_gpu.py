@@ -282,8 +282,8 @@ def _tb_jobs(rng, n, **kw):
 @pytest.mark.parametrize("cutoff", [52, 26, 8])
 def test_result_build_kernel_vs_host(dev, cutoff):
     """gz_build_results (one lane per result, the reference's gap_states_
-    layout) equals the host result build, field by field and in the stale
-    flag, on homologous pairs with indels."""
+    layout) equals the host result build on homologous pairs with indels:
+    the same stale flag always, every field when no stale byte was read."""
     rng = np.random.default_rng(40 + cutoff)
     q, d, jobs, dmax = _tb_jobs(rng, 1200, min_len=5, max_len=400, identity=(0.25, 0.95),
                                 indel=0.05)
@@ -304,7 +304,13 @@ def test_result_build_kernel_vs_host(dev, cutoff):
         if out[k, 9] & 2:
             overflow += 1
             continue
-        stale += int(host[9] != 0)
+        # a stale read makes the result history-dependent: both builds must flag
+        # it (the caller then replays the chunk on the host); otherwise every
+        # field is equal
+        assert out[k, 9] == host[9], (k, out[k], host)
+        if host[9]:
+            stale += 1
+            continue
         assert np.array_equal(out[k], host), (k, out[k], host)
     say("  overflow", overflow, "stale", stale)
     assert overflow < len(jobs) // 20
