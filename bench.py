@@ -7,12 +7,15 @@ search of all reads: query lists, seed search / distance / ungapped
 extension on the device, chain filter, gapped extension on the device,
 AddResults, BuildResults and tabular output.
 
-Two forms run on the same inputs: the plain GPU schedule and the
-learned-scheduled form (gapped extensions ordered by the MLP's predicted
-cost; the MLP is trained on the GPU from the plain run's extension traces
-during set-up).  Their outputs must be byte-identical.  `value` is the
-learned form's query residues per second (query residues = amino acids of
-the six translated frames); the plain form's rate is reported beside it.
+Three forms run on the same inputs: the plain GPU form (stages one after the
+other, jobs in hit order), the batched form (query blocks pipelined so that
+the host stages of one block overlap the device stages of the next) and the
+batched learned-scheduled form (the same, with gapped extensions ordered on
+the device by the MLP's predicted cost; the MLP is trained on the GPU from
+the plain run's extension traces during set-up).  Their outputs must be
+byte-identical.  `value` is the batched learned-scheduled form's query
+residues per second (query residues = amino acids of the six translated
+frames); the other forms' rates are reported beside it.
 
 The CPU baseline is the oracle (a single-threaded restatement of the
 reference's CPU aligner) timed on a bounded sample of the same reads.
@@ -167,17 +170,20 @@ def main():
         log("copy kernel: 3 x 1 GiB")
 
     t0 = time.perf_counter()
-    weights, learn = (None, None) if args.no_learned else train_cost_model(gz, db, db_codes, np.random.default_rng(args.seed + 1),
-                                      max(1000, int(args.train_reads * min(1.0, args.scale * 10))),
-                                      args.epochs, args.seed)
+    weights, learn = None, None
+    if not args.no_learned:
+        train_reads = max(1000, int(args.train_reads * min(1.0, args.scale * 10)))
+        weights, learn = train_cost_model(gz, db, db_codes, np.random.default_rng(args.seed + 1),
+                                          train_reads, args.epochs, args.seed)
     log(f"cost model trained in {time.perf_counter() - t0:.1f} s: {learn}")
 
     plain = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN)
-    if args.no_learned:
-        learned = plain
-    else:
-        learned = gz.Aligner(db, schedule=gz.SCHEDULE_LEARNED)
+    forms = [("plain", plain)]
+    if not args.no_learned:
+        batched = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN, batched=1)
+        learned = gz.Aligner(db, schedule=gz.SCHEDULE_LEARNED, batched=1)
         learned.set_cost_model(weights)
+        forms += [("batched", batched), ("learned", learned)]
 
     def run(al, k, label):
         outs, stats, times = [], [], []
@@ -193,21 +199,21 @@ def main():
                 f"(kernel {st['gapped_kernel_ms']:.1f} ms)")
         return outs, stats, times
 
-    run(plain, args.warmup, "plain warmup")
-    p_out, p_st, p_t = run(plain, args.steps, "plain")
-    if args.no_learned:
-        l_out, l_st, l_t = p_out, p_st, p_t
-    else:
-        run(learned, args.warmup, "learned warmup")
-        l_out, l_st, l_t = run(learned, args.steps, "learned")
-    identical = all(o == p_out[0] for o in p_out + l_out)
+    results = {}
+    for name, al in forms:
+        run(al, args.warmup, name + " warmup")
+        results[name] = run(al, args.steps, name)
+    p_out, p_st, p_t = results["plain"]
+    l_out, l_st, l_t = results["learned" if "learned" in results else "plain"]
+    identical = all(o == p_out[0] for r in results.values() for o in r[0])
     if not identical:
-        log("ERROR: learned-scheduled output differs from the plain schedule")
+        log("ERROR: the forms' outputs differ")
 
     p_time = sum(p_t)
     l_time = sum(l_t)
     value = q_aa * args.steps / l_time
     plain_value = q_aa * args.steps / p_time
+    form_values = {name: q_aa * args.steps / sum(r[2]) for name, r in results.items()}
 
     # roofline of the dominant kernel, from the HIP-event times of the timed steps
     st = l_st[-1]
@@ -258,6 +264,7 @@ def main():
                    "reads": n_reads, "db_proteins": n_db, "db_residues": int(db.info().total_length),
                    "query_aa": q_aa, "schedule": "learned"},
         "plain_value": plain_value, "plain_ms_per_step": 1e3 * p_time / args.steps,
+        "form_values": form_values,
         "learned_over_plain": value / plain_value, "outputs_identical": identical,
         "kernel_ms_per_step": kern,
         "search_stats": {k: st[k] for k in ("lists", "units_rep", "units_value", "seed_hits",

@@ -294,6 +294,8 @@ void gz_aligner_default_parameters(gz_aligner_parameters *p) {
   p->threads = 0;
   p->schedule = GZ_SCHEDULE_PLAIN;
   p->record_traces = 0;
+  p->batched = 0;
+  p->pipeline_blocks = 8;
 }
 
 gz_status gz_aligner_create(const gz_aligner_parameters *p, gz_aligner_t *out) {
@@ -318,6 +320,8 @@ gz_status gz_aligner_create(const gz_aligner_parameters *p, gz_aligner_t *out) {
   o.threads = p->threads;
   o.schedule = (Schedule)p->schedule;
   o.record_cells = p->record_traces != 0;
+  o.batched = p->batched != 0;
+  o.pipeline_blocks = p->pipeline_blocks ? p->pipeline_blocks : 8;
   h->aligner.reset(new Aligner(o));
   *out = h;
   return GZ_OK;

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
 #include <numeric>
 #include <sstream>
 #include <thread>
@@ -93,133 +95,220 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     st->seed_hits += hits.count;
     st->seconds_seed += Now() - t;
 
-    // 2. per-query enumeration order and chain filter
-    t = Now();
     std::vector<std::vector<Hit> > chained(nq);
-    ParallelBlocks(nq, threads, 64, [&](size_t b, size_t e) {
-      std::vector<uint64_t> scratch;
-      std::vector<uint32_t> byq;
-      KeyOrder order{hits.hits};
-      for (size_t qi = b; qi < e; ++qi) {
-        const uint32_t s0 = hits.qstart[qi], s1 = hits.qstart[qi + 1];
-        if (s0 == s1) continue;
-        byq.resize(s1 - s0);
-        std::iota(byq.begin(), byq.end(), s0);
-        std::sort(byq.begin(), byq.end(), order);
-        std::vector<Hit> &hv = chained[qi];
-        hv.resize(s1 - s0);
-        for (uint32_t k = 0; k < s1 - s0; ++k) {
-          const SeedHit &h = hits.hits[byq[k]];
-          hv[k].q = h.qpos - (uint32_t)coff[qi];
-          hv[k].d = h.dpos;
+    // 2. per-query enumeration order and chain filter of queries [qb, qe)
+    auto chain = [&](size_t qb, size_t qe) {
+      ParallelBlocks(qe - qb, threads, 64, [&](size_t b, size_t e) {
+        std::vector<uint64_t> scratch;
+        std::vector<uint32_t> byq;
+        KeyOrder order{hits.hits};
+        for (size_t qi = qb + b; qi < qb + e; ++qi) {
+          const uint32_t s0 = hits.qstart[qi], s1 = hits.qstart[qi + 1];
+          if (s0 == s1) continue;
+          byq.resize(s1 - s0);
+          std::iota(byq.begin(), byq.end(), s0);
+          std::sort(byq.begin(), byq.end(), order);
+          std::vector<Hit> &hv = chained[qi];
+          hv.resize(s1 - s0);
+          for (uint32_t k = 0; k < s1 - s0; ++k) {
+            const SeedHit &h = hits.hits[byq[k]];
+            hv[k].q = h.qpos - (uint32_t)coff[qi];
+            hv[k].d = h.dpos;
+          }
+          ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl,
+                      hv, scratch);
         }
-        ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl,
-                    hv, scratch);
-      }
-    });
-    std::vector<uint32_t> gstart(nq + 1, 0);
-    for (size_t k = 0; k < nq; ++k) gstart[k + 1] = gstart[k] + (uint32_t)chained[k].size();
-    const uint32_t G = gstart[nq];
-    st->chained_hits += G;
-    st->seconds_chain += Now() - t;
-
-    // 3. gapped extension of both sides on the device
-    t = Now();
-    std::vector<uint32_t> rq(G), rd(G), fq(G), fd(G);
-    ParallelBlocks(nq, threads, 256, [&](size_t b, size_t e) {
-      for (size_t qi = b; qi < e; ++qi)
-        for (uint32_t k = 0; k < chained[qi].size(); ++k) {
-          const uint32_t g = gstart[qi] + k;
-          const Hit &h = chained[qi][k];
-          rq[g] = (uint32_t)coff[qi] + h.q - 1;
-          rd[g] = h.d - 1;
-          fq[g] = (uint32_t)coff[qi] + h.q;
-          fd[g] = h.d;
-        }
-    });
-    std::vector<int> rs(G), fs(G);
-    if (G > 0) {
-      // jobs: [0, G) reverse sides, [G, 2G) forward sides; the order the
-      // device runs them in never changes their results
+      });
+    };
+    // 3. gapped extension of both sides of the chained hits of [qb, qe)
+    struct Block {
+      size_t qb = 0, qe = 0;
+      std::vector<uint32_t> gstart, rq, rd, fq, fd;
+      std::vector<int> rs, fs;
+      int status = 0;
+    };
+    auto gapped = [&](Block &B) -> int {
+      const size_t n = B.qe - B.qb;
+      B.gstart.assign(n + 1, 0);
+      for (size_t k = 0; k < n; ++k)
+        B.gstart[k + 1] = B.gstart[k] + (uint32_t)chained[B.qb + k].size();
+      const uint32_t G = B.gstart[n];
+      B.rq.resize(G);
+      B.rd.resize(G);
+      B.fq.resize(G);
+      B.fd.resize(G);
+      B.rs.resize(G);
+      B.fs.resize(G);
+      if (G == 0) return 0;
       std::vector<uint32_t> aq(2 * (size_t)G), ad(2 * (size_t)G);
-      std::copy(rq.begin(), rq.end(), aq.begin());
-      std::copy(fq.begin(), fq.end(), aq.begin() + G);
-      std::copy(rd.begin(), rd.end(), ad.begin());
-      std::copy(fd.begin(), fd.end(), ad.begin() + G);
       std::vector<uint16_t> keys;
-      if (opt.schedule == Schedule::kLengthSorted) {
-        // ExtensionSeedList order: by the query distance the side can extend
-        keys.resize(2 * (size_t)G);
-        ParallelBlocks(nq, threads, 256, [&](size_t b, size_t e) {
-          for (size_t qi = b; qi < e; ++qi)
-            for (uint32_t k = gstart[qi]; k < gstart[qi + 1]; ++k) {
-              const uint32_t r = q.DistanceFromStart(qi, rq[k] - (uint32_t)coff[qi]);
-              const uint32_t f = q.DistanceFromEnd(qi, fq[k] - (uint32_t)coff[qi]);
-              keys[k] = (uint16_t)std::min<uint32_t>(r, kOrderKeyLimit - 1);
-              keys[G + k] = (uint16_t)std::min<uint32_t>(f, kOrderKeyLimit - 1);
-            }
-        });
+      if (opt.schedule == Schedule::kLengthSorted) keys.resize(2 * (size_t)G);
+      for (size_t k = 0; k < n; ++k) {
+        const size_t qi = B.qb + k;
+        for (uint32_t j = 0; j < chained[qi].size(); ++j) {
+          const uint32_t g = B.gstart[k] + j;
+          const Hit &h = chained[qi][j];
+          aq[g] = (uint32_t)coff[qi] + h.q - 1;  // reverse side starts before the hit
+          ad[g] = h.d - 1;
+          aq[G + g] = (uint32_t)coff[qi] + h.q;
+          ad[G + g] = h.d;
+          if (!keys.empty()) {
+            // ExtensionSeedList order: by the query distance the side can extend
+            const uint32_t r = q.DistanceFromStart(qi, h.q - 1);
+            const uint32_t f = q.DistanceFromEnd(qi, h.q);
+            keys[g] = (uint16_t)std::min<uint32_t>(r, kOrderKeyLimit - 1);
+            keys[G + g] = (uint16_t)std::min<uint32_t>(f, kOrderKeyLimit - 1);
+          }
+        }
       }
+      // the order the device runs the jobs in never changes their results
       const bool learned = opt.schedule == Schedule::kLearned && opt.cost_model != nullptr;
       std::vector<int> sc(2 * (size_t)G);
       std::vector<uint32_t> cells(opt.record_cells ? 2 * (size_t)G : 0);
-      GZ_STEP(dev->GappedExtend(ci, G, G, aq.data(), ad.data(), sc.data(),
-                                cells.empty() ? nullptr : cells.data(),
-                                keys.empty() ? nullptr : keys.data(), learned, st));
-      std::copy(aq.begin(), aq.begin() + G, rq.begin());
-      std::copy(aq.begin() + G, aq.end(), fq.begin());
-      std::copy(ad.begin(), ad.begin() + G, rd.begin());
-      std::copy(ad.begin() + G, ad.end(), fd.begin());
-      std::copy(sc.begin(), sc.begin() + G, rs.begin());
-      std::copy(sc.begin() + G, sc.end(), fs.begin());
+      const int r = dev->GappedExtend(ci, G, G, aq.data(), ad.data(), sc.data(),
+                                      cells.empty() ? nullptr : cells.data(),
+                                      keys.empty() ? nullptr : keys.data(), learned, st);
+      if (r != 0) return r;
+      std::copy(aq.begin(), aq.begin() + G, B.rq.begin());
+      std::copy(aq.begin() + G, aq.end(), B.fq.begin());
+      std::copy(ad.begin(), ad.begin() + G, B.rd.begin());
+      std::copy(ad.begin() + G, ad.end(), B.fd.begin());
+      std::copy(sc.begin(), sc.begin() + G, B.rs.begin());
+      std::copy(sc.begin() + G, sc.end(), B.fs.begin());
       if (!cells.empty()) {
-        // seed positions before extension
         for (int side = 0; side < 2; ++side)
-          for (size_t qi = 0; qi < nq; ++qi)
-            for (uint32_t g = gstart[qi]; g < gstart[qi + 1]; ++g) {
-              const Hit &h = chained[qi][g - gstart[qi]];
+          for (size_t k = 0; k < n; ++k) {
+            const size_t qi = B.qb + k;
+            for (uint32_t j = 0; j < chained[qi].size(); ++j) {
+              const Hit &h = chained[qi][j];
               GappedTrace tr;
               tr.chunk = (uint32_t)ci;
               tr.reverse = side == 0 ? 1 : 0;
               tr.qpos = (uint32_t)coff[qi] + h.q - (side == 0 ? 1 : 0);
               tr.dpos = h.d - (side == 0 ? 1 : 0);
-              tr.cells = cells[(size_t)side * G + g];
+              tr.cells = cells[(size_t)side * G + B.gstart[k] + j];
               if (traces) traces->push_back(tr);
               st->gapped_cells += tr.cells;
             }
+          }
+      }
+      st->gapped_extensions += 2 * (uint64_t)G;
+      st->chained_hits += G;
+      return 0;
+    };
+    // 4. presearched results and AddResults for the queries of a block
+    auto add = [&](const Block &B) {
+      ParallelBlocks(B.qe - B.qb, threads, 64, [&](size_t b, size_t e) {
+        std::vector<std::vector<AlignmentPosition> > starts;
+        std::vector<Presearched> tmp;
+        for (size_t k = b; k < e; ++k) {
+          const size_t qi = B.qb + k;
+          tmp.clear();
+          for (uint32_t g = B.gstart[k]; g < B.gstart[k + 1]; ++g) {
+            const Hit &h = chained[qi][g - B.gstart[k]];
+            Presearched r;
+            r.chunk = -1;
+            r.subject = 0;
+            r.hit.q = h.q;
+            r.hit.d = h.d;
+            r.start.q = B.rq[g] - (uint32_t)coff[qi];
+            r.start.d = B.rd[g];
+            r.end.q = B.fq[g] - (uint32_t)coff[qi];
+            r.end.d = B.fd[g];
+            r.score = B.rs[g] + B.fs[g];
+            r.hit_count = 1;
+            tmp.push_back(r);
+          }
+          if (!tmp.empty() || !pre[qi].empty())
+            AddResults(chunk, (int)ci, P, starts, tmp, pre[qi]);
+          std::vector<Hit>().swap(chained[qi]);
+        }
+      });
+    };
+
+    if (!opt.batched) {
+      // plain: each stage over all queries, one after the other
+      t = Now();
+      chain(0, nq);
+      st->seconds_chain += Now() - t;
+      t = Now();
+      Block all;
+      all.qb = 0;
+      all.qe = nq;
+      GZ_STEP(gapped(all));
+      st->seconds_gapped += Now() - t;
+      t = Now();
+      add(all);
+      st->seconds_add += Now() - t;
+      continue;
+    }
+    // batched: blocks of queries flow through chain (host) -> gapped (device,
+    // on its own host thread) -> AddResults (host), so that the host stages
+    // of one block overlap the device stage of the next
+    const size_t nblocks = std::max<size_t>(1, std::min<size_t>(opt.pipeline_blocks, nq));
+    std::vector<Block> blocks(nblocks);
+    {
+      // equal shares of seed hits per block
+      const uint64_t total = hits.count;
+      size_t qb = 0;
+      for (size_t b = 0; b < nblocks; ++b) {
+        const uint64_t target = total * (b + 1) / nblocks;
+        size_t qe = qb;
+        while (qe < nq && (b + 1 == nblocks || hits.qstart[qe] < target)) ++qe;
+        if (b + 1 == nblocks) qe = nq;
+        blocks[b].qb = qb;
+        blocks[b].qe = qe;
+        qb = qe;
       }
     }
-    st->gapped_extensions += 2 * (uint64_t)G;
-    st->seconds_gapped += Now() - t;
-
-    // 4. presearched results and AddResults per query
-    t = Now();
-    ParallelBlocks(nq, threads, 64, [&](size_t b, size_t e) {
-      std::vector<std::vector<AlignmentPosition> > starts;
-      std::vector<Presearched> tmp;
-      for (size_t qi = b; qi < e; ++qi) {
-        tmp.clear();
-        for (uint32_t g = gstart[qi]; g < gstart[qi + 1]; ++g) {
-          const Hit &h = chained[qi][g - gstart[qi]];
-          Presearched r;
-          r.chunk = -1;
-          r.subject = 0;
-          r.hit.q = h.q;
-          r.hit.d = h.d;
-          r.start.q = rq[g] - (uint32_t)coff[qi];
-          r.start.d = rd[g];
-          r.end.q = fq[g] - (uint32_t)coff[qi];
-          r.end.d = fd[g];
-          r.score = rs[g] + fs[g];
-          r.hit_count = 1;
-          tmp.push_back(r);
+    std::mutex mu;
+    std::condition_variable cv;
+    size_t chained_upto = 0, gapped_upto = 0;
+    int device_status = 0;
+    std::thread device([&]() {
+      for (size_t b = 0; b < nblocks; ++b) {
+        {
+          std::unique_lock<std::mutex> lock(mu);
+          cv.wait(lock, [&] { return chained_upto > b; });
         }
-        if (!tmp.empty() || !pre[qi].empty())
-          AddResults(chunk, (int)ci, P, starts, tmp, pre[qi]);
+        const double tg = Now();
+        const int r = device_status == 0 ? gapped(blocks[b]) : device_status;
+        {
+          std::lock_guard<std::mutex> lock(mu);
+          st->seconds_gapped += Now() - tg;
+          if (r != 0) device_status = r;
+          gapped_upto = b + 1;
+        }
+        cv.notify_all();
       }
     });
-    st->seconds_add += Now() - t;
+    for (size_t b = 0; b <= nblocks; ++b) {
+      if (b < nblocks) {
+        const double tc = Now();
+        chain(blocks[b].qb, blocks[b].qe);
+        st->seconds_chain += Now() - tc;
+        {
+          std::lock_guard<std::mutex> lock(mu);
+          chained_upto = b + 1;
+        }
+        cv.notify_all();
+      }
+      if (b >= 1) {
+        {
+          std::unique_lock<std::mutex> lock(mu);
+          cv.wait(lock, [&] { return gapped_upto >= b; });
+        }
+        if (device_status == 0) {
+          const double ta = Now();
+          add(blocks[b - 1]);
+          st->seconds_add += Now() - ta;
+        }
+      }
+    }
+    device.join();
+    GZ_STEP(device_status);
   }
+
   // BuildResults (on the device where it builds results) and WriteOutput
   t = Now();
   std::vector<std::vector<AlignmentResult> > results(nq);

@@ -80,6 +80,10 @@ struct AlignerOptions {
   Schedule schedule = Schedule::kPlain;
   const CostModel *cost_model = nullptr;     // for Schedule::kLearned
   bool record_cells = false;                 // keep per-extension DP cells
+  // batched form: query blocks pipelined through chain filter (host), gapped
+  // extension (device) and AddResults (host)
+  bool batched = false;
+  uint32_t pipeline_blocks = 8;
 };
 
 // Order keys of gapped jobs are below this bound (larger key = more work).

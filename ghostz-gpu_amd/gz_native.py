@@ -256,7 +256,7 @@ class AlignerParameters(Structure):
                 ("normalized_presearch_cutoff", c_float), ("normalized_result_cutoff", c_float),
                 ("max_results", c_uint32), ("max_results_per_subject", c_uint32),
                 ("queries_chunk_size", c_uint64), ("threads", c_int), ("schedule", c_int),
-                ("record_traces", c_int)]
+                ("record_traces", c_int), ("batched", c_int), ("pipeline_blocks", c_uint32)]
 
 
 class AlignStats(Structure):

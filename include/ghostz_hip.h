@@ -280,6 +280,9 @@ typedef struct {
   int threads;                               /* host threads, 0 = all */
   int schedule;                              /* GZ_SCHEDULE_* */
   int record_traces;                         /* keep per-extension DP cells */
+  int batched;              /* pipeline query blocks: host stages of one block
+                               overlap the device stages of the next */
+  uint32_t pipeline_blocks; /* query blocks per chunk when batched (0 = 8) */
 } gz_aligner_parameters;
 typedef struct {
   double seconds_total, seconds_query_lists, seconds_upload, seconds_seed,

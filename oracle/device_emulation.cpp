@@ -212,6 +212,8 @@ int gzh_align(const char *const *qnames, const char *qres, const uint64_t *qoffs
   opt.schedule = prm->schedule == GZ_SCHEDULE_LENGTH_SORTED ? Schedule::kLengthSorted
                                                             : Schedule::kPlain;
   opt.record_cells = prm->record_traces != 0;
+  opt.batched = prm->batched != 0;
+  opt.pipeline_blocks = prm->pipeline_blocks ? prm->pipeline_blocks : 8;
   CpuDevice dev(db, opt.search);
   std::string out;
   AlignStats st;

@@ -40,7 +40,7 @@ def test_struct_sizes_match_c():
     assert ctypes.sizeof(gz.DatabaseOptions) == 32
     assert ctypes.sizeof(gz.DatabaseInfo) == 168
     assert ctypes.sizeof(gz.ChunkView) == 128
-    assert ctypes.sizeof(gz.AlignerParameters) == 56
+    assert ctypes.sizeof(gz.AlignerParameters) == 64
     assert ctypes.sizeof(gz.GappedTrace) == 20
     assert ctypes.sizeof(gz.AlignStats) == 208
 

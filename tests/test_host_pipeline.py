@@ -15,7 +15,7 @@ from test_pipeline_gpu import oracle_text
 
 
 def check(name, qd, dd, translated=False, filter=True, clustering=True, db_chunk_size=0,
-          schedules=(0, 1), **params):
+          schedules=(0, 1), batched=(0, 1), **params):
     m, nl = gz.blosum62()
     q = gz.Queries(*qd, translated=translated, filter=filter)
     db = gz.Database(*dd, clustering=clustering, chunk_size=db_chunk_size)
@@ -25,13 +25,15 @@ def check(name, qd, dd, translated=False, filter=True, clustering=True, db_chunk
     ok = {k: params[k] for k in ("max_results", "max_results_per_subject") if k in params}
     want, ost = oracle_text(q.view(), db, m, nl, **kw, **ok)
     for sch in schedules:
-        p = gz.default_parameters(schedule=sch, record_traces=1, **params)
-        got, st = oracle_py.host_align(qd, dd, p, translated=translated, filter=filter,
-                                       clustering=clustering, db_chunk_size=db_chunk_size)
-        assert got == want, (name, sch)
-        assert st["seed_hits"] == ost["hits"]
-        assert st["chained_hits"] == ost["hits_after_chain_filter"]
-        assert st["gapped_extensions"] == ost["gapped_extensions"]
+        for bat in batched:
+            p = gz.default_parameters(schedule=sch, record_traces=1, batched=bat,
+                                      pipeline_blocks=5, **params)
+            got, st = oracle_py.host_align(qd, dd, p, translated=translated, filter=filter,
+                                           clustering=clustering, db_chunk_size=db_chunk_size)
+            assert got == want, (name, sch, bat)
+            assert st["seed_hits"] == ost["hits"]
+            assert st["chained_hits"] == ost["hits_after_chain_filter"]
+            assert st["gapped_extensions"] == ost["gapped_extensions"]
     return want
 
 

@@ -83,12 +83,14 @@ def run_case(name, queries, db, m, nl, schedules=(gz.SCHEDULE_PLAIN,), chunk_siz
     want, ost = oracle_text(qv, db, m, nl, **kw)
     say("  oracle lines", want.count(b"\n"), ost)
     assert want.count(b"\n") > 0 or name.startswith("edge")
-    for sch in schedules:
-        al = gz.Aligner(db, schedule=sch, **params)
+    forms = [(sch, 0) for sch in schedules] + [(schedules[-1], 1)]
+    for sch, batched in forms:
+        al = gz.Aligner(db, schedule=sch, batched=batched, pipeline_blocks=6, **params)
         if sch == gz.SCHEDULE_LEARNED:
             al.set_cost_model(gz.Mlp.init_weights(7))
         got, st = al.align(queries)
-        say("  schedule", sch, "lines", got.count(b"\n"), "hits", st["seed_hits"],
+        say("  schedule", sch, "batched", batched, "lines", got.count(b"\n"), "hits",
+            st["seed_hits"],
             "chained", st["chained_hits"], "gapped", st["gapped_extensions"])
         if got != want:
             gl, wl = got.split(b"\n"), want.split(b"\n")
