@@ -219,6 +219,7 @@ def main():
     st = l_st[-1]
     kern = {"seed_rep": st["seed_rep_kernel_ms"], "seed_value": st["seed_value_kernel_ms"],
             "gapped": st["gapped_kernel_ms"]}
+    side_kernels = {"schedule": st["schedule_kernel_ms"], "traceback": st["traceback_kernel_ms"]}
     dom = max(kern, key=kern.get)
     if dom == "gapped":
         # one DP cell = 10 int32 VALU ops (3 max, 2 subtract, 1 add, 3 compares / selects,
@@ -266,7 +267,7 @@ def main():
         "plain_value": plain_value, "plain_ms_per_step": 1e3 * p_time / args.steps,
         "form_values": form_values,
         "learned_over_plain": value / plain_value, "outputs_identical": identical,
-        "kernel_ms_per_step": kern,
+        "kernel_ms_per_step": dict(kern, **side_kernels),
         "search_stats": {k: st[k] for k in ("lists", "units_rep", "units_value", "seed_hits",
                                              "chained_hits", "gapped_extensions", "results",
                                              "sequential_build_chunks")},

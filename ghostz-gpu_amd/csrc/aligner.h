@@ -52,6 +52,7 @@ class Aligner : public PresearchDevice {
   int Tracebacks(size_t chunk, const std::vector<TracebackJob> &jobs, int cutoff,
                  std::vector<TracebackOut> *out, AlignStats *st) override;
   void EndQueryChunk() override;
+  uint8_t *JobBuffer(size_t slot, size_t bytes) override;
 
  private:
   struct DeviceChunk;
@@ -77,6 +78,8 @@ class Aligner : public PresearchDevice {
   void *tb_ws_ = nullptr;
   size_t tb_ws_cap_ = 0;
   uint32_t max_query_length_ = 0;
+  // pinned job buffers per pipeline slot
+  std::vector<std::pair<void *, size_t> > pinned_jobs_;
 };
 
 }  // namespace gz

@@ -69,6 +69,8 @@ Aligner::~Aligner() {
   chunks_.clear();
   if (gapped_ws_) (void)hipFree(gapped_ws_);
   if (tb_ws_) (void)hipFree(tb_ws_);
+  for (auto &b : pinned_jobs_)
+    if (b.first) (void)hipHostFree(b.first);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
   if (gapped_) gz_gapped_extender_destroy(gapped_);
@@ -362,6 +364,23 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   if (e != hipSuccess) return (int)e;
   st->traceback_kernel_ms += ms;
   return 0;
+}
+
+uint8_t *Aligner::JobBuffer(size_t slot, size_t bytes) {
+  if (pinned_jobs_.size() <= slot) pinned_jobs_.resize(slot + 1, std::make_pair(nullptr, 0));
+  std::pair<void *, size_t> &b = pinned_jobs_[slot];
+  if (b.second < bytes) {
+    if (b.first) (void)hipHostFree(b.first);
+    b.first = nullptr;
+    b.second = 0;
+    const size_t want = bytes + bytes / 4;
+    if (hipHostMalloc(&b.first, want, hipHostMallocDefault) != hipSuccess) {
+      b.first = nullptr;
+      return PresearchDevice::JobBuffer(slot, bytes);  // pageable fallback
+    }
+    b.second = want;
+  }
+  return (uint8_t *)b.first;
 }
 
 }  // namespace gz

@@ -392,6 +392,8 @@ gz_status gz_aligner_align(gz_aligner_t h, gz_queries_t q, char **output,
     stats->seed_value_launches = st.seed_launches[1];
     stats->gapped_launches = st.gapped_launches;
     stats->schedule_kernel_ms = st.schedule_kernel_ms;
+    stats->traceback_kernel_ms = st.traceback_kernel_ms;
+    stats->traceback_host_jobs = st.traceback_host_jobs;
   }
   return GZ_OK;
 }

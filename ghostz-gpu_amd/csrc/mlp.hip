@@ -279,6 +279,7 @@ __global__ void __launch_bounds__(256)
       acc1[r] = 0.f;
     }
     const int mrow = wave * 32 + row_in;
+#pragma unroll 8
     for (int k0 = 0; k0 < kCostIn; k0 += 8) {
       const int code = ct[(k0 >> 3) * kInferTile + mrow];
       const short4v a = Load4(&emb[code * kCostEmbed + kq * 4]);
@@ -335,6 +336,7 @@ __global__ void __launch_bounds__(256)
       acc1[r] = 0.f;
     }
     const int mrow = wave * 32 + row_in;
+#pragma unroll 8
     for (int k0 = 0; k0 < kCostH1; k0 += 8) {
       const int kb = (k0 >> 2) + kq;
       const short4v a = Load4(&ht[(kb * kInferTile + mrow) * 4]);

@@ -122,10 +122,13 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     };
     // 3. gapped extension of both sides of the chained hits of [qb, qe)
     struct Block {
-      size_t qb = 0, qe = 0;
-      std::vector<uint32_t> gstart, rq, rd, fq, fd;
-      std::vector<int> rs, fs;
-      int status = 0;
+      size_t qb = 0, qe = 0, slot = 0;
+      std::vector<uint32_t> gstart;
+      // jobs [0, G) reverse sides, [G, 2G) forward sides: positions in,
+      // extension ends and scores out
+      uint32_t *aq = nullptr, *ad = nullptr;
+      int *sc = nullptr;
+      uint32_t G = 0;
     };
     auto gapped = [&](Block &B) -> int {
       const size_t n = B.qe - B.qb;
@@ -133,48 +136,43 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       for (size_t k = 0; k < n; ++k)
         B.gstart[k + 1] = B.gstart[k] + (uint32_t)chained[B.qb + k].size();
       const uint32_t G = B.gstart[n];
-      B.rq.resize(G);
-      B.rd.resize(G);
-      B.fq.resize(G);
-      B.fd.resize(G);
-      B.rs.resize(G);
-      B.fs.resize(G);
+      B.G = G;
       if (G == 0) return 0;
-      std::vector<uint32_t> aq(2 * (size_t)G), ad(2 * (size_t)G);
+      const size_t jobs = 2 * (size_t)G;
+      uint8_t *buf = dev->JobBuffer(B.slot, jobs * 12);
+      B.aq = reinterpret_cast<uint32_t *>(buf);
+      B.ad = B.aq + jobs;
+      B.sc = reinterpret_cast<int *>(B.ad + jobs);
       std::vector<uint16_t> keys;
-      if (opt.schedule == Schedule::kLengthSorted) keys.resize(2 * (size_t)G);
-      for (size_t k = 0; k < n; ++k) {
-        const size_t qi = B.qb + k;
-        for (uint32_t j = 0; j < chained[qi].size(); ++j) {
-          const uint32_t g = B.gstart[k] + j;
-          const Hit &h = chained[qi][j];
-          aq[g] = (uint32_t)coff[qi] + h.q - 1;  // reverse side starts before the hit
-          ad[g] = h.d - 1;
-          aq[G + g] = (uint32_t)coff[qi] + h.q;
-          ad[G + g] = h.d;
-          if (!keys.empty()) {
-            // ExtensionSeedList order: by the query distance the side can extend
-            const uint32_t r = q.DistanceFromStart(qi, h.q - 1);
-            const uint32_t f = q.DistanceFromEnd(qi, h.q);
-            keys[g] = (uint16_t)std::min<uint32_t>(r, kOrderKeyLimit - 1);
-            keys[G + g] = (uint16_t)std::min<uint32_t>(f, kOrderKeyLimit - 1);
+      if (opt.schedule == Schedule::kLengthSorted) keys.resize(jobs);
+      ParallelBlocks(n, opt.batched ? std::max(1, threads / 4) : threads, 256,
+                     [&](size_t b, size_t e) {
+        for (size_t k = b; k < e; ++k) {
+          const size_t qi = B.qb + k;
+          for (uint32_t j = 0; j < chained[qi].size(); ++j) {
+            const uint32_t g = B.gstart[k] + j;
+            const Hit &h = chained[qi][j];
+            B.aq[g] = (uint32_t)coff[qi] + h.q - 1;  // reverse side starts before the hit
+            B.ad[g] = h.d - 1;
+            B.aq[G + g] = (uint32_t)coff[qi] + h.q;
+            B.ad[G + g] = h.d;
+            if (!keys.empty()) {
+              // ExtensionSeedList order: by the query distance the side can extend
+              const uint32_t r = q.DistanceFromStart(qi, h.q - 1);
+              const uint32_t f = q.DistanceFromEnd(qi, h.q);
+              keys[g] = (uint16_t)std::min<uint32_t>(r, kOrderKeyLimit - 1);
+              keys[G + g] = (uint16_t)std::min<uint32_t>(f, kOrderKeyLimit - 1);
+            }
           }
         }
-      }
+      });
       // the order the device runs the jobs in never changes their results
       const bool learned = opt.schedule == Schedule::kLearned && opt.cost_model != nullptr;
-      std::vector<int> sc(2 * (size_t)G);
-      std::vector<uint32_t> cells(opt.record_cells ? 2 * (size_t)G : 0);
-      const int r = dev->GappedExtend(ci, G, G, aq.data(), ad.data(), sc.data(),
+      std::vector<uint32_t> cells(opt.record_cells ? jobs : 0);
+      const int r = dev->GappedExtend(ci, G, G, B.aq, B.ad, B.sc,
                                       cells.empty() ? nullptr : cells.data(),
                                       keys.empty() ? nullptr : keys.data(), learned, st);
       if (r != 0) return r;
-      std::copy(aq.begin(), aq.begin() + G, B.rq.begin());
-      std::copy(aq.begin() + G, aq.end(), B.fq.begin());
-      std::copy(ad.begin(), ad.begin() + G, B.rd.begin());
-      std::copy(ad.begin() + G, ad.end(), B.fd.begin());
-      std::copy(sc.begin(), sc.begin() + G, B.rs.begin());
-      std::copy(sc.begin() + G, sc.end(), B.fs.begin());
       if (!cells.empty()) {
         for (int side = 0; side < 2; ++side)
           for (size_t k = 0; k < n; ++k) {
@@ -192,7 +190,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
             }
           }
       }
-      st->gapped_extensions += 2 * (uint64_t)G;
+      st->gapped_extensions += jobs;
       st->chained_hits += G;
       return 0;
     };
@@ -211,11 +209,11 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
             r.subject = 0;
             r.hit.q = h.q;
             r.hit.d = h.d;
-            r.start.q = B.rq[g] - (uint32_t)coff[qi];
-            r.start.d = B.rd[g];
-            r.end.q = B.fq[g] - (uint32_t)coff[qi];
-            r.end.d = B.fd[g];
-            r.score = B.rs[g] + B.fs[g];
+            r.start.q = B.aq[g] - (uint32_t)coff[qi];
+            r.start.d = B.ad[g];
+            r.end.q = B.aq[B.G + g] - (uint32_t)coff[qi];
+            r.end.d = B.ad[B.G + g];
+            r.score = B.sc[g] + B.sc[B.G + g];
             r.hit_count = 1;
             tmp.push_back(r);
           }
@@ -258,6 +256,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
         if (b + 1 == nblocks) qe = nq;
         blocks[b].qb = qb;
         blocks[b].qe = qe;
+        blocks[b].slot = b;
         qb = qe;
       }
     }

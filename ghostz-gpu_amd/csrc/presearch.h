@@ -133,7 +133,19 @@ class PresearchDevice {
     return kTracebackUnsupported;
   }
   virtual void EndQueryChunk() = 0;
+  // Host memory for the job arrays of pipeline slot `slot` (at least `bytes`,
+  // 16-byte aligned, valid until the next call for the same slot); a device
+  // may return pinned memory so that its copies run at full speed.
+  virtual uint8_t *JobBuffer(size_t slot, size_t bytes) {
+    if (job_buffers_.size() <= slot) job_buffers_.resize(slot + 1);
+    std::vector<uint64_t> &b = job_buffers_[slot];
+    if (b.size() * 8 < bytes) b.resize((bytes + 7) / 8);
+    return reinterpret_cast<uint8_t *>(b.data());
+  }
   static constexpr int kTracebackUnsupported = -1;
+
+ private:
+  std::vector<std::vector<uint64_t> > job_buffers_;
 };
 
 // Aligns all queries (split into chunks by queries_chunk_size) against all

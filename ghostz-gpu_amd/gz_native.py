@@ -269,7 +269,8 @@ class AlignStats(Structure):
         ("gapped_kernel_ms", c_double), ("seed_rep_kernel_ms", c_double),
         ("seed_value_kernel_ms", c_double), ("seed_rep_launches", c_uint64),
         ("seed_value_launches", c_uint64), ("gapped_launches", c_uint64),
-        ("schedule_kernel_ms", c_double)]
+        ("schedule_kernel_ms", c_double), ("traceback_kernel_ms", c_double),
+        ("traceback_host_jobs", c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

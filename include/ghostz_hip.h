@@ -295,6 +295,8 @@ typedef struct {
   uint64_t seed_rep_launches, seed_value_launches, gapped_launches;
   double schedule_kernel_ms;   /* device ordering of gapped jobs (features,
                                   cost model, counting sort, gathers) */
+  double traceback_kernel_ms;  /* device result build */
+  uint64_t traceback_host_jobs; /* results the device left to the host */
 } gz_align_stats;
 typedef struct {
   uint32_t query_position; /* concatenated seed position */
