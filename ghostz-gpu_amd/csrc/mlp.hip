@@ -243,8 +243,6 @@ __global__ void __launch_bounds__(256)
                       const uint16_t *embh, const float *b1, uint16_t *h1) {
   // W1 slice as [k/4][64][4]: the B operand of a lane is one 8-byte read
   __shared__ __align__(16) uint16_t w[kCostIn * kInferSlice];
-  // codes transposed [position][seed]
-  __shared__ __align__(16) uint8_t ct[kCostResidues * kInferTile];
   __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
   const int slice = blockIdx.x & 3;
   const int group = blockIdx.x >> 2, groups = gridDim.x >> 2;
@@ -255,39 +253,52 @@ __global__ void __launch_bounds__(256)
     *reinterpret_cast<uint2 *>(&w[(k4 * kInferSlice + nn) * 4]) = v;
   }
   for (int i = threadIdx.x; i < kCostAlphabet * kCostEmbed; i += 256) emb[i] = embh[i];
+  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int kq = lane >> 5, row_in = lane & 31;
   const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
   for (uint32_t t = group; t < tiles; t += groups) {
-    __syncthreads();  // previous tile's codes fully consumed (and W1 loaded)
     const uint32_t s0 = t * kInferTile;
-    for (int idx = threadIdx.x; idx < kInferTile * (kCostResidues / 4); idx += 256) {
-      const int i = idx / (kCostResidues / 4), r4 = idx % (kCostResidues / 4);
-      uint32_t v = 0x14141414u;  // code 20 for seeds past the end
-      if (s0 + i < n)
-        v = *reinterpret_cast<const uint32_t *>(codes + (size_t)(s0 + i) * kCostResidues + r4 * 4);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t c = (v >> (8 * j)) & 0xffu;
-        ct[(r4 * 4 + j) * kInferTile + i] = (uint8_t)(c < 20 ? c : 20);
-      }
-    }
-    __syncthreads();
+    const uint32_t seed_row = s0 + wave * 32 + row_in;
+    // the lane's 128 residue codes (its A-operand row) stream through
+    // registers 16 at a time, the next 16 loading while these are used
+    const bool live = seed_row < n;
+    const uint4 *src = reinterpret_cast<const uint4 *>(codes + (size_t)(live ? seed_row : 0) *
+                                                                   kCostResidues);
+    const uint32_t fill = live ? 0u : 0x14141414u;  // code 20 for rows past the end
+    uint4 cur = src[0];
+    cur.x |= fill;
+    cur.y |= fill;
+    cur.z |= fill;
+    cur.w |= fill;
     float16_t acc0, acc1;
     for (int r = 0; r < 16; ++r) {
       acc0[r] = 0.f;
       acc1[r] = 0.f;
     }
-    const int mrow = wave * 32 + row_in;
-#pragma unroll 8
-    for (int k0 = 0; k0 < kCostIn; k0 += 8) {
-      const int code = ct[(k0 >> 3) * kInferTile + mrow];
-      const short4v a = Load4(&emb[code * kCostEmbed + kq * 4]);
-      const int kb = (k0 >> 2) + kq;
-      const short4v b0 = Load4(&w[(kb * kInferSlice + row_in) * 4]);
-      const short4v b1v = Load4(&w[(kb * kInferSlice + 32 + row_in) * 4]);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1v, acc1, 0, 0, 0);
+#pragma unroll 1
+    for (int v = 0; v < kCostResidues / 16; ++v) {
+      uint4 nxt = src[v + 1 < kCostResidues / 16 ? v + 1 : v];
+      nxt.x |= fill;
+      nxt.y |= fill;
+      nxt.z |= fill;
+      nxt.w |= fill;
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const uint32_t word = q4 == 0 ? cur.x : q4 == 1 ? cur.y : q4 == 2 ? cur.z : cur.w;
+#pragma unroll
+        for (int bt = 0; bt < 4; ++bt) {
+          uint32_t code = (word >> (8 * bt)) & 0xffu;
+          code = code < 20 ? code : 20;
+          const short4v a = Load4(&emb[code * kCostEmbed + kq * 4]);
+          const int kb = 2 * (v * 16 + q4 * 4 + bt) + kq;
+          const short4v b0 = Load4(&w[(kb * kInferSlice + row_in) * 4]);
+          const short4v b1v = Load4(&w[(kb * kInferSlice + 32 + row_in) * 4]);
+          acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1v, acc1, 0, 0, 0);
+        }
+      }
+      cur = nxt;
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
