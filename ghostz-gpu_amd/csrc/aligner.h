@@ -45,7 +45,8 @@ class Aligner : public PresearchDevice {
                       const QueryLists &lists, const std::vector<int> &cutoffs,
                       const std::vector<int> &triggers, int presearch_cutoff,
                       AlignStats *st) override;
-  int SeedHits(size_t chunk, uint32_t nq, SeedHitView *hits, AlignStats *st) override;
+  int SeedHits(size_t chunk, uint32_t qb, uint32_t qe, SeedHitView *hits,
+               AlignStats *st) override;
   int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward, uint32_t *qpos,
                    uint32_t *dpos, int *scores, uint32_t *cells, const uint16_t *keys,
                    bool learned, AlignStats *st) override;
@@ -69,6 +70,8 @@ class Aligner : public PresearchDevice {
   // resident query chunk
   DeviceQueryLists dq_;
   SeedStageParams sp_;
+  static constexpr size_t kNoChunk = ~(size_t)0;
+  size_t lookup_chunk_ = kNoChunk;  // database chunk of the seed stage's lookup
   std::vector<void *> query_buffers_;
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
   // gapped-stage workspace (grown on demand)

@@ -148,7 +148,9 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
   dq_.packed = d_q;
   dq_.cutoffs = d_cut;
   dq_.triggers = d_trig;
+  dq_.nqueries = (uint32_t)q.size();
   dq_.nlists = (uint32_t)lists.NumberLists();
+  lookup_chunk_ = kNoChunk;
   dq_.hash = d_lhash;
   dq_.entry_off = d_eoff;
   dq_.entry_pos = d_epos;
@@ -185,7 +187,8 @@ void Aligner::EndQueryChunk() {
   dq_ = DeviceQueryLists();
 }
 
-int Aligner::SeedHits(size_t ci, uint32_t nq, SeedHitView *hits, AlignStats *st) {
+int Aligner::SeedHits(size_t ci, uint32_t qb, uint32_t qe, SeedHitView *hits,
+                      AlignStats *st) {
   const DeviceChunk &dc = *chunks_[ci];
   DeviceDbChunk ddb;
   ddb.packed = dc.packed;
@@ -198,8 +201,14 @@ int Aligner::SeedHits(size_t ci, uint32_t nq, SeedHitView *hits, AlignStats *st)
   ddb.val_off = dc.val_off;
   ddb.val_pos = dc.val_pos;
   sp_.clustering = db_->chunks[ci].clustering;
+  if (lookup_chunk_ != ci) {
+    lookup_chunk_ = kNoChunk;
+    const hipError_t e = seed_.Lookup(dq_, ddb, sp_.clustering, stream_);
+    if (e != hipSuccess) return (int)e;
+    lookup_chunk_ = ci;
+  }
   SeedStageStats ss;
-  const hipError_t e = seed_.Run(dq_, ddb, sp_, nq, stream_, hits, &ss);
+  const hipError_t e = seed_.Run(dq_, ddb, sp_, qb, qe, stream_, hits, &ss);
   if (e != hipSuccess) return (int)e;
   st->units_rep += ss.units_rep;
   st->units_value += ss.units_value;

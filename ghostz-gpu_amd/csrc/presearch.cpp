@@ -86,303 +86,446 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   st->seconds_upload += Now() - t;
 
   std::vector<std::vector<Presearched> > pre(nq);
-  for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
-    const DbChunk &chunk = db.chunks[ci];
-    // 1. seed stage
-    t = Now();
-    SeedHitView hits;
-    GZ_STEP(dev->SeedHits(ci, (uint32_t)nq, &hits, st));
-    st->seed_hits += hits.count;
-    st->seconds_seed += Now() - t;
+  std::vector<std::vector<Hit> > chained(nq);
+  std::vector<std::vector<AlignmentResult> > results(nq);
 
-    std::vector<std::vector<Hit> > chained(nq);
-    // 2. per-query enumeration order and chain filter of queries [qb, qe)
-    auto chain = [&](size_t qb, size_t qe) {
-      ParallelBlocks(qe - qb, threads, 64, [&](size_t b, size_t e) {
-        std::vector<uint64_t> scratch;
-        std::vector<uint32_t> byq;
-        KeyOrder order{hits.hits};
-        for (size_t qi = qb + b; qi < qb + e; ++qi) {
-          const uint32_t s0 = hits.qstart[qi], s1 = hits.qstart[qi + 1];
-          if (s0 == s1) continue;
-          byq.resize(s1 - s0);
-          std::iota(byq.begin(), byq.end(), s0);
-          std::sort(byq.begin(), byq.end(), order);
-          std::vector<Hit> &hv = chained[qi];
-          hv.resize(s1 - s0);
-          for (uint32_t k = 0; k < s1 - s0; ++k) {
-            const SeedHit &h = hits.hits[byq[k]];
-            hv[k].q = h.qpos - (uint32_t)coff[qi];
-            hv[k].d = h.dpos;
-          }
-          ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl,
-                      hv, scratch);
+  // 2. per-query enumeration order and chain filter of queries [qb, qe)
+  //    (hits: the seed view of a run that started at query hb)
+  auto chain = [&](const DbChunk &chunk, const SeedHitView &hits, size_t hb, size_t qb,
+                   size_t qe) {
+    ParallelBlocks(qe - qb, threads, 64, [&](size_t b, size_t e) {
+      std::vector<uint64_t> scratch;
+      std::vector<uint32_t> byq;
+      KeyOrder order{hits.hits};
+      for (size_t qi = qb + b; qi < qb + e; ++qi) {
+        const uint32_t s0 = hits.qstart[qi - hb], s1 = hits.qstart[qi - hb + 1];
+        if (s0 == s1) continue;
+        byq.resize(s1 - s0);
+        std::iota(byq.begin(), byq.end(), s0);
+        std::sort(byq.begin(), byq.end(), order);
+        std::vector<Hit> &hv = chained[qi];
+        hv.resize(s1 - s0);
+        for (uint32_t k = 0; k < s1 - s0; ++k) {
+          const SeedHit &h = hits.hits[byq[k]];
+          hv[k].q = h.qpos - (uint32_t)coff[qi];
+          hv[k].d = h.dpos;
         }
-      });
-    };
-    // 3. gapped extension of both sides of the chained hits of [qb, qe)
-    struct Block {
-      size_t qb = 0, qe = 0, slot = 0;
-      std::vector<uint32_t> gstart;
-      // jobs [0, G) reverse sides, [G, 2G) forward sides: positions in,
-      // extension ends and scores out
-      uint32_t *aq = nullptr, *ad = nullptr;
-      int *sc = nullptr;
-      uint32_t G = 0;
-    };
-    auto gapped = [&](Block &B) -> int {
-      const size_t n = B.qe - B.qb;
-      B.gstart.assign(n + 1, 0);
-      for (size_t k = 0; k < n; ++k)
-        B.gstart[k + 1] = B.gstart[k] + (uint32_t)chained[B.qb + k].size();
-      const uint32_t G = B.gstart[n];
-      B.G = G;
-      if (G == 0) return 0;
-      const size_t jobs = 2 * (size_t)G;
-      uint8_t *buf = dev->JobBuffer(B.slot, jobs * 12);
-      B.aq = reinterpret_cast<uint32_t *>(buf);
-      B.ad = B.aq + jobs;
-      B.sc = reinterpret_cast<int *>(B.ad + jobs);
-      std::vector<uint16_t> keys;
-      if (opt.schedule == Schedule::kLengthSorted) keys.resize(jobs);
-      ParallelBlocks(n, opt.batched ? std::max(1, threads / 4) : threads, 256,
-                     [&](size_t b, size_t e) {
-        for (size_t k = b; k < e; ++k) {
+        ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl,
+                    hv, scratch);
+      }
+    });
+  };
+  // 3. gapped extension of both sides of the chained hits of [qb, qe)
+  struct Block {
+    size_t qb = 0, qe = 0, slot = 0;
+    std::vector<uint32_t> gstart;
+    // jobs [0, G) reverse sides, [G, 2G) forward sides: positions in,
+    // extension ends and scores out
+    uint32_t *aq = nullptr, *ad = nullptr;
+    int *sc = nullptr;
+    uint32_t G = 0;
+  };
+  auto gapped = [&](size_t ci, Block &B) -> int {
+    const size_t n = B.qe - B.qb;
+    B.gstart.assign(n + 1, 0);
+    for (size_t k = 0; k < n; ++k)
+      B.gstart[k + 1] = B.gstart[k] + (uint32_t)chained[B.qb + k].size();
+    const uint32_t G = B.gstart[n];
+    B.G = G;
+    if (G == 0) return 0;
+    const size_t jobs = 2 * (size_t)G;
+    uint8_t *buf = dev->JobBuffer(B.slot, jobs * 12);
+    B.aq = reinterpret_cast<uint32_t *>(buf);
+    B.ad = B.aq + jobs;
+    B.sc = reinterpret_cast<int *>(B.ad + jobs);
+    std::vector<uint16_t> keys;
+    if (opt.schedule == Schedule::kLengthSorted) keys.resize(jobs);
+    ParallelBlocks(n, opt.batched ? std::max(1, threads / 4) : threads, 256,
+                   [&](size_t b, size_t e) {
+      for (size_t k = b; k < e; ++k) {
+        const size_t qi = B.qb + k;
+        for (uint32_t j = 0; j < chained[qi].size(); ++j) {
+          const uint32_t g = B.gstart[k] + j;
+          const Hit &h = chained[qi][j];
+          B.aq[g] = (uint32_t)coff[qi] + h.q - 1;  // reverse side starts before the hit
+          B.ad[g] = h.d - 1;
+          B.aq[G + g] = (uint32_t)coff[qi] + h.q;
+          B.ad[G + g] = h.d;
+          if (!keys.empty()) {
+            // ExtensionSeedList order: by the query distance the side can extend
+            const uint32_t r = q.DistanceFromStart(qi, h.q - 1);
+            const uint32_t f = q.DistanceFromEnd(qi, h.q);
+            keys[g] = (uint16_t)std::min<uint32_t>(r, kOrderKeyLimit - 1);
+            keys[G + g] = (uint16_t)std::min<uint32_t>(f, kOrderKeyLimit - 1);
+          }
+        }
+      }
+    });
+    // the order the device runs the jobs in never changes their results
+    const bool learned = opt.schedule == Schedule::kLearned && opt.cost_model != nullptr;
+    std::vector<uint32_t> cells(opt.record_cells ? jobs : 0);
+    const int r = dev->GappedExtend(ci, G, G, B.aq, B.ad, B.sc,
+                                    cells.empty() ? nullptr : cells.data(),
+                                    keys.empty() ? nullptr : keys.data(), learned, st);
+    if (r != 0) return r;
+    if (!cells.empty()) {
+      for (int side = 0; side < 2; ++side)
+        for (size_t k = 0; k < n; ++k) {
           const size_t qi = B.qb + k;
           for (uint32_t j = 0; j < chained[qi].size(); ++j) {
-            const uint32_t g = B.gstart[k] + j;
             const Hit &h = chained[qi][j];
-            B.aq[g] = (uint32_t)coff[qi] + h.q - 1;  // reverse side starts before the hit
-            B.ad[g] = h.d - 1;
-            B.aq[G + g] = (uint32_t)coff[qi] + h.q;
-            B.ad[G + g] = h.d;
-            if (!keys.empty()) {
-              // ExtensionSeedList order: by the query distance the side can extend
-              const uint32_t r = q.DistanceFromStart(qi, h.q - 1);
-              const uint32_t f = q.DistanceFromEnd(qi, h.q);
-              keys[g] = (uint16_t)std::min<uint32_t>(r, kOrderKeyLimit - 1);
-              keys[G + g] = (uint16_t)std::min<uint32_t>(f, kOrderKeyLimit - 1);
-            }
+            GappedTrace tr;
+            tr.chunk = (uint32_t)ci;
+            tr.reverse = side == 0 ? 1 : 0;
+            tr.qpos = (uint32_t)coff[qi] + h.q - (side == 0 ? 1 : 0);
+            tr.dpos = h.d - (side == 0 ? 1 : 0);
+            tr.cells = cells[(size_t)side * G + B.gstart[k] + j];
+            if (traces) traces->push_back(tr);
+            st->gapped_cells += tr.cells;
           }
         }
-      });
-      // the order the device runs the jobs in never changes their results
-      const bool learned = opt.schedule == Schedule::kLearned && opt.cost_model != nullptr;
-      std::vector<uint32_t> cells(opt.record_cells ? jobs : 0);
-      const int r = dev->GappedExtend(ci, G, G, B.aq, B.ad, B.sc,
-                                      cells.empty() ? nullptr : cells.data(),
-                                      keys.empty() ? nullptr : keys.data(), learned, st);
-      if (r != 0) return r;
-      if (!cells.empty()) {
-        for (int side = 0; side < 2; ++side)
-          for (size_t k = 0; k < n; ++k) {
-            const size_t qi = B.qb + k;
-            for (uint32_t j = 0; j < chained[qi].size(); ++j) {
-              const Hit &h = chained[qi][j];
-              GappedTrace tr;
-              tr.chunk = (uint32_t)ci;
-              tr.reverse = side == 0 ? 1 : 0;
-              tr.qpos = (uint32_t)coff[qi] + h.q - (side == 0 ? 1 : 0);
-              tr.dpos = h.d - (side == 0 ? 1 : 0);
-              tr.cells = cells[(size_t)side * G + B.gstart[k] + j];
-              if (traces) traces->push_back(tr);
-              st->gapped_cells += tr.cells;
-            }
-          }
+    }
+    st->gapped_extensions += jobs;
+    st->chained_hits += G;
+    return 0;
+  };
+  // 4. presearched results and AddResults for the queries of a block
+  auto add = [&](size_t ci, const Block &B) {
+    const DbChunk &chunk = db.chunks[ci];
+    ParallelBlocks(B.qe - B.qb, threads, 64, [&](size_t b, size_t e) {
+      std::vector<std::vector<AlignmentPosition> > starts;
+      std::vector<Presearched> tmp;
+      for (size_t k = b; k < e; ++k) {
+        const size_t qi = B.qb + k;
+        tmp.clear();
+        for (uint32_t g = B.gstart[k]; g < B.gstart[k + 1]; ++g) {
+          const Hit &h = chained[qi][g - B.gstart[k]];
+          Presearched r;
+          r.chunk = -1;
+          r.subject = 0;
+          r.hit.q = h.q;
+          r.hit.d = h.d;
+          r.start.q = B.aq[g] - (uint32_t)coff[qi];
+          r.start.d = B.ad[g];
+          r.end.q = B.aq[B.G + g] - (uint32_t)coff[qi];
+          r.end.d = B.ad[B.G + g];
+          r.score = B.sc[g] + B.sc[B.G + g];
+          r.hit_count = 1;
+          tmp.push_back(r);
+        }
+        if (!tmp.empty() || !pre[qi].empty())
+          AddResults(chunk, (int)ci, P, starts, tmp, pre[qi]);
+        std::vector<Hit>().swap(chained[qi]);
       }
-      st->gapped_extensions += jobs;
-      st->chained_hits += G;
-      return 0;
-    };
-    // 4. presearched results and AddResults for the queries of a block
-    auto add = [&](const Block &B) {
-      ParallelBlocks(B.qe - B.qb, threads, 64, [&](size_t b, size_t e) {
-        std::vector<std::vector<AlignmentPosition> > starts;
-        std::vector<Presearched> tmp;
-        for (size_t k = b; k < e; ++k) {
-          const size_t qi = B.qb + k;
-          tmp.clear();
-          for (uint32_t g = B.gstart[k]; g < B.gstart[k + 1]; ++g) {
-            const Hit &h = chained[qi][g - B.gstart[k]];
-            Presearched r;
-            r.chunk = -1;
-            r.subject = 0;
-            r.hit.q = h.q;
-            r.hit.d = h.d;
-            r.start.q = B.aq[g] - (uint32_t)coff[qi];
-            r.start.d = B.ad[g];
-            r.end.q = B.aq[B.G + g] - (uint32_t)coff[qi];
-            r.end.d = B.ad[B.G + g];
-            r.score = B.sc[g] + B.sc[B.G + g];
-            r.hit_count = 1;
-            tmp.push_back(r);
+    });
+  };
+  // 5. result build of queries [qb, qe), device part: traceback jobs of every
+  //    database chunk (kTracebackUnsupported when the device has none)
+  struct BuildPart {
+    size_t ci;
+    std::vector<std::pair<uint32_t, uint32_t> > ids;
+    std::vector<TracebackOut> outs;
+  };
+  auto build_device = [&](size_t qb, size_t qe, std::vector<BuildPart> *parts) -> int {
+    parts->clear();
+    for (size_t i = qb; i < qe; ++i) results[i].assign(pre[i].size(), AlignmentResult());
+    for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
+      BuildPart part;
+      part.ci = ci;
+      part.ids = ChunkResultIds(pre, ci, qb, qe);
+      if (part.ids.empty()) continue;
+      std::vector<TracebackJob> jobs(part.ids.size());
+      for (size_t k = 0; k < part.ids.size(); ++k) {
+        const uint32_t qi = part.ids[k].first;
+        const Presearched &pr = pre[qi][part.ids[k].second];
+        jobs[k].query_offset = (uint32_t)coff[qi];
+        jobs[k].query_length = q.Length(qi);
+        jobs[k].hit_q = pr.hit.q;
+        jobs[k].hit_d = pr.hit.d;
+      }
+      const int rc = dev->Tracebacks(ci, jobs, result_cutoff, &part.outs, st);
+      if (rc != 0) return rc;
+      parts->push_back(std::move(part));
+    }
+    return 0;
+  };
+  //    host part: copy the device's results; false when a traceback read a
+  //    byte an earlier extension left (the chunk must then be replayed in
+  //    the reference's order); host_jobs counts results redone on the host
+  auto build_host = [&](std::vector<BuildPart> &parts, uint64_t *host_jobs,
+                        std::vector<size_t> *stale_chunks) {
+    for (BuildPart &part : parts) {
+      const DbChunk &chunk = db.chunks[part.ci];
+      const auto &ids = part.ids;
+      const auto &outs = part.outs;
+      bool stale = false;
+      std::vector<size_t> redo;
+      for (size_t k = 0; k < ids.size(); ++k) {
+        if (outs[k].flags & kTracebackStale) stale = true;
+        if (outs[k].flags & kTracebackOverflow) redo.push_back(k);
+      }
+      *host_jobs += redo.size();
+      if (!stale) {
+        ParallelBlocks(ids.size(), threads, 4096, [&](size_t b, size_t e) {
+          for (size_t k = b; k < e; ++k) {
+            if (outs[k].flags) continue;
+            const Presearched &pr = pre[ids[k].first][ids[k].second];
+            const TracebackOut &o = outs[k];
+            AlignmentResult &r = results[ids[k].first][ids[k].second];
+            r.chunk = pr.chunk;
+            r.subject = pr.subject;
+            r.score = o.score;
+            r.length = o.length;
+            r.mismatches = o.mismatches;
+            r.gap_openings = o.gap_openings;
+            r.gap_extensions = o.gap_extensions;
+            r.hit = pr.hit;
+            r.start.q = o.start_q;
+            r.start.d = o.start_d;
+            r.end.q = o.end_q;
+            r.end.d = o.end_d;
           }
-          if (!tmp.empty() || !pre[qi].empty())
-            AddResults(chunk, (int)ci, P, starts, tmp, pre[qi]);
-          std::vector<Hit>().swap(chained[qi]);
+        });
+        for (size_t k : redo) {
+          const uint32_t qi = ids[k].first, r = ids[k].second;
+          if (!BuildOneFresh(q, qi, chunk, pre[qi][r], P, result_cutoff, &results[qi][r])) {
+            stale = true;
+            break;
+          }
         }
-      });
-    };
+      }
+      if (stale) stale_chunks->push_back(part.ci);
+    }
+  };
+  //    the whole query chunk, with the reference-order replay of every
+  //    database chunk where a traceback read an earlier extension's byte
+  auto build_all = [&]() -> int {
+    std::vector<BuildPart> parts;
+    const int rc = build_device(0, nq, &parts);
+    if (rc == PresearchDevice::kTracebackUnsupported) {
+      for (size_t i = 0; i < nq; ++i) results[i].assign(pre[i].size(), AlignmentResult());
+      for (size_t ci = 0; ci < db.chunks.size(); ++ci)
+        BuildChunkHost(q, db.chunks[ci], P, result_cutoff, pre, results,
+                       ChunkResultIds(pre, ci), threads, &st->sequential_build_chunks);
+    } else {
+      GZ_STEP(rc);
+      std::vector<size_t> stale;
+      build_host(parts, &st->traceback_host_jobs, &stale);
+      for (size_t ci : stale) {
+        ++st->sequential_build_chunks;
+        ReplayChunkHost(q, db.chunks[ci], P, result_cutoff, pre, results,
+                        ChunkResultIds(pre, ci));
+      }
+    }
+    SortResults(results, 0, nq);
+    return 0;
+  };
 
-    if (!opt.batched) {
-      // plain: each stage over all queries, one after the other
+  if (!opt.batched) {
+    // plain: each stage over all queries, one after the other
+    for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
+      const DbChunk &chunk = db.chunks[ci];
       t = Now();
-      chain(0, nq);
+      SeedHitView hits;
+      GZ_STEP(dev->SeedHits(ci, 0, (uint32_t)nq, &hits, st));
+      st->seed_hits += hits.count;
+      st->seconds_seed += Now() - t;
+      t = Now();
+      chain(chunk, hits, 0, 0, nq);
       st->seconds_chain += Now() - t;
       t = Now();
       Block all;
       all.qb = 0;
       all.qe = nq;
-      GZ_STEP(gapped(all));
+      GZ_STEP(gapped(ci, all));
       st->seconds_gapped += Now() - t;
       t = Now();
-      add(all);
+      add(ci, all);
       st->seconds_add += Now() - t;
-      continue;
     }
-    // batched: blocks of queries flow through chain (host) -> gapped (device,
-    // on its own host thread) -> AddResults (host), so that the host stages
-    // of one block overlap the device stage of the next
-    const size_t nblocks = std::max<size_t>(1, std::min<size_t>(opt.pipeline_blocks, nq));
-    std::vector<Block> blocks(nblocks);
-    {
-      // equal shares of seed hits per block
-      const uint64_t total = hits.count;
-      size_t qb = 0;
-      for (size_t b = 0; b < nblocks; ++b) {
-        const uint64_t target = total * (b + 1) / nblocks;
-        size_t qe = qb;
-        while (qe < nq && (b + 1 == nblocks || hits.qstart[qe] < target)) ++qe;
-        if (b + 1 == nblocks) qe = nq;
-        blocks[b].qb = qb;
-        blocks[b].qe = qe;
-        blocks[b].slot = b;
-        qb = qe;
-      }
-    }
-    std::mutex mu;
-    std::condition_variable cv;
-    size_t chained_upto = 0, gapped_upto = 0;
-    int device_status = 0;
-    std::thread device([&]() {
-      for (size_t b = 0; b < nblocks; ++b) {
-        {
-          std::unique_lock<std::mutex> lock(mu);
-          cv.wait(lock, [&] { return chained_upto > b; });
-        }
-        const double tg = Now();
-        const int r = device_status == 0 ? gapped(blocks[b]) : device_status;
-        {
-          std::lock_guard<std::mutex> lock(mu);
-          st->seconds_gapped += Now() - tg;
-          if (r != 0) device_status = r;
-          gapped_upto = b + 1;
-        }
-        cv.notify_all();
-      }
-    });
-    for (size_t b = 0; b <= nblocks; ++b) {
-      if (b < nblocks) {
-        const double tc = Now();
-        chain(blocks[b].qb, blocks[b].qe);
-        st->seconds_chain += Now() - tc;
-        {
-          std::lock_guard<std::mutex> lock(mu);
-          chained_upto = b + 1;
-        }
-        cv.notify_all();
-      }
-      if (b >= 1) {
-        {
-          std::unique_lock<std::mutex> lock(mu);
-          cv.wait(lock, [&] { return gapped_upto >= b; });
-        }
-        if (device_status == 0) {
-          const double ta = Now();
-          add(blocks[b - 1]);
-          st->seconds_add += Now() - ta;
-        }
-      }
-    }
-    device.join();
-    GZ_STEP(device_status);
+    t = Now();
+    GZ_STEP(build_all());
+    dev->EndQueryChunk();
+    for (auto &r : results) st->results += r.size();
+    st->seconds_build += Now() - t;
+    t = Now();
+    std::ostringstream os;
+    WriteOutput(os, q, db, results);
+    output->append(os.str());
+    st->seconds_output += Now() - t;
+    return 0;
   }
 
-  // BuildResults (on the device where it builds results) and WriteOutput
-  t = Now();
-  std::vector<std::vector<AlignmentResult> > results(nq);
-  for (size_t i = 0; i < nq; ++i) results[i].resize(pre[i].size());
-  for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
-    const DbChunk &chunk = db.chunks[ci];
-    const std::vector<std::pair<uint32_t, uint32_t> > ids = ChunkResultIds(pre, ci);
-    if (ids.empty()) continue;
-    std::vector<TracebackJob> jobs(ids.size());
-    for (size_t k = 0; k < ids.size(); ++k) {
-      const uint32_t qi = ids[k].first;
-      const Presearched &pr = pre[qi][ids[k].second];
-      jobs[k].query_offset = (uint32_t)coff[qi];
-      jobs[k].query_length = q.Length(qi);
-      jobs[k].hit_q = pr.hit.q;
-      jobs[k].hit_d = pr.hit.d;
-    }
-    std::vector<TracebackOut> outs;
-    const int rc = dev->Tracebacks(ci, jobs, result_cutoff, &outs, st);
-    if (rc == PresearchDevice::kTracebackUnsupported) {
-      BuildChunkHost(q, chunk, P, result_cutoff, pre, results, ids, threads,
-                     &st->sequential_build_chunks);
-      continue;
-    }
-    GZ_STEP(rc);
-    bool stale = false;
-    std::vector<size_t> host_jobs;
-    for (size_t k = 0; k < ids.size(); ++k) {
-      if (outs[k].flags & kTracebackStale) stale = true;
-      if (outs[k].flags & kTracebackOverflow) host_jobs.push_back(k);
-    }
-    st->traceback_host_jobs += host_jobs.size();
-    if (!stale) {
-      ParallelBlocks(ids.size(), threads, 4096, [&](size_t b, size_t e) {
-        for (size_t k = b; k < e; ++k) {
-          if (outs[k].flags) continue;
-          const Presearched &pr = pre[ids[k].first][ids[k].second];
-          const TracebackOut &o = outs[k];
-          AlignmentResult &r = results[ids[k].first][ids[k].second];
-          r.chunk = pr.chunk;
-          r.subject = pr.subject;
-          r.score = o.score;
-          r.length = o.length;
-          r.mismatches = o.mismatches;
-          r.gap_openings = o.gap_openings;
-          r.gap_extensions = o.gap_extensions;
-          r.hit = pr.hit;
-          r.start.q = o.start_q;
-          r.start.d = o.start_d;
-          r.end.q = o.end_q;
-          r.end.d = o.end_d;
-        }
-      });
-      for (size_t k : host_jobs) {
-        const uint32_t qi = ids[k].first, r = ids[k].second;
-        if (!BuildOneFresh(q, qi, chunk, pre[qi][r], P, result_cutoff, &results[qi][r])) {
-          stale = true;
-          break;
+  // batched: blocks of queries (multiples of 64, equal residues) flow through
+  // seed (device) -> chain filter (host) -> gapped extension (device) ->
+  // AddResults (host), and after the last database chunk through the result
+  // build (device, then host) and the output text; one host thread drives
+  // the device, so the host stages of one block overlap the device stages
+  // of the next
+  std::vector<Block> blocks;
+  {
+    const size_t want = std::max<size_t>(1, std::min<size_t>(opt.pipeline_blocks,
+                                                             (nq + 63) / 64));
+    uint64_t total = 0;
+    for (size_t i = 0; i < nq; ++i) total += q.Length(i);
+    size_t qb = 0;
+    uint64_t sum = 0;
+    for (size_t b = 0; b < want && qb < nq; ++b) {
+      size_t qe = nq;
+      if (b + 1 < want) {
+        const uint64_t target = total * (b + 1) / want;
+        qe = qb;
+        while (qe < nq) {
+          sum += q.Length(qe);
+          ++qe;
+          if (qe % 64 == 0 && sum >= target) break;
         }
       }
-    }
-    if (stale) {
-      ++st->sequential_build_chunks;
-      ReplayChunkHost(q, chunk, P, result_cutoff, pre, results, ids);
+      Block B;
+      B.qb = qb;
+      B.qe = qe;
+      B.slot = blocks.size();
+      blocks.push_back(B);
+      qb = qe;
     }
   }
-  SortResults(results);
+  const size_t nb = blocks.size();
+  std::vector<std::string> block_text(nb);
+  std::vector<std::vector<BuildPart> > block_parts(nb);
+  std::vector<size_t> stale_chunks;
+  bool rebuild = false;  // a block could not be built on its own
+  double sec_seed = 0, sec_gapped = 0, sec_chain = 0, sec_add = 0, sec_build = 0,
+         sec_build_host = 0, sec_output = 0;
+  uint64_t seed_hits = 0, host_jobs = 0;
+  for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
+    const DbChunk &chunk = db.chunks[ci];
+    const bool last = ci + 1 == db.chunks.size();
+    std::vector<SeedHitView> views(nb);
+    std::mutex mu;
+    std::condition_variable cv;
+    size_t seeded = 0, chained_upto = 0, gapped_upto = 0, added = 0, traced = 0;
+    int status = 0;
+    bool unsupported = false;
+    auto wait_for = [&](const size_t &counter, size_t value) {
+      std::unique_lock<std::mutex> lock(mu);
+      cv.wait(lock, [&] { return counter >= value || status != 0; });
+      return status == 0;
+    };
+    auto publish = [&](size_t &counter, size_t value, int r) {
+      {
+        std::lock_guard<std::mutex> lock(mu);
+        counter = value;
+        if (r != 0 && status == 0) status = r;
+      }
+      cv.notify_all();
+    };
+    std::thread device([&]() {
+      for (size_t k = 0; k < nb + 2; ++k) {
+        if (k < nb) {
+          // the view of block k reuses the buffer of block k - 2
+          if (k >= 2 && !wait_for(chained_upto, k - 1)) return;
+          const double t0 = Now();
+          const int r = dev->SeedHits(ci, (uint32_t)blocks[k].qb, (uint32_t)blocks[k].qe,
+                                      &views[k], st);
+          seed_hits += views[k].count;
+          sec_seed += Now() - t0;
+          publish(seeded, k + 1, r);
+          if (r != 0) return;
+        }
+        if (k >= 1 && k <= nb) {
+          if (!wait_for(chained_upto, k)) return;
+          const double t0 = Now();
+          const int r = gapped(ci, blocks[k - 1]);
+          sec_gapped += Now() - t0;
+          publish(gapped_upto, k, r);
+          if (r != 0) return;
+        }
+        if (last && k >= 2) {
+          if (!wait_for(added, k - 1)) return;
+          const double t0 = Now();
+          int r = 0;
+          if (!unsupported) {
+            r = build_device(blocks[k - 2].qb, blocks[k - 2].qe, &block_parts[k - 2]);
+            if (r == PresearchDevice::kTracebackUnsupported) {
+              unsupported = true;
+              r = 0;
+            }
+          }
+          sec_build += Now() - t0;
+          publish(traced, k - 1, r);
+          if (r != 0) return;
+        }
+      }
+    });
+    std::thread adder([&]() {
+      for (size_t b = 0; b < nb; ++b) {
+        if (!wait_for(gapped_upto, b + 1)) return;
+        const double t0 = Now();
+        add(ci, blocks[b]);
+        sec_add += Now() - t0;
+        publish(added, b + 1, 0);
+      }
+    });
+    std::thread writer;
+    if (last) {
+      writer = std::thread([&]() {
+        for (size_t b = 0; b < nb; ++b) {
+          if (!wait_for(traced, b + 1)) return;
+          if (unsupported || rebuild) continue;
+          const double t0 = Now();
+          const size_t before = stale_chunks.size();
+          build_host(block_parts[b], &host_jobs, &stale_chunks);
+          std::vector<BuildPart>().swap(block_parts[b]);
+          if (stale_chunks.size() != before) {
+            rebuild = true;
+            continue;
+          }
+          SortResults(results, blocks[b].qb, blocks[b].qe);
+          const double t1 = Now();
+          sec_build_host += t1 - t0;
+          std::ostringstream os;
+          WriteOutputRange(os, q, db, results, blocks[b].qb, blocks[b].qe,
+                           std::max(1, threads / 2));
+          block_text[b] = os.str();
+          sec_output += Now() - t1;
+        }
+      });
+    }
+    for (size_t b = 0; b < nb; ++b) {
+      if (!wait_for(seeded, b + 1)) break;
+      const double t0 = Now();
+      chain(chunk, views[b], blocks[b].qb, blocks[b].qb, blocks[b].qe);
+      sec_chain += Now() - t0;
+      publish(chained_upto, b + 1, 0);
+    }
+    device.join();
+    adder.join();
+    if (writer.joinable()) writer.join();
+    GZ_STEP(status);
+    if (last && (unsupported || rebuild)) {
+      // rebuild every result in the reference's order and write all queries
+      t = Now();
+      GZ_STEP(build_all());
+      sec_build += Now() - t;
+      t = Now();
+      std::fill(block_text.begin(), block_text.end(), std::string());
+      std::ostringstream os;
+      WriteOutput(os, q, db, results);
+      block_text[0] = os.str();
+      sec_output += Now() - t;
+    }
+  }
   dev->EndQueryChunk();
+  st->seed_hits += seed_hits;
+  st->seconds_seed += sec_seed;
+  st->seconds_chain += sec_chain;
+  st->seconds_gapped += sec_gapped;
+  st->seconds_add += sec_add;
+  st->seconds_build += sec_build + sec_build_host;
+  st->seconds_output += sec_output;
+  st->traceback_host_jobs += host_jobs;
   for (auto &r : results) st->results += r.size();
-  st->seconds_build += Now() - t;
-  t = Now();
-  std::ostringstream os;
-  WriteOutput(os, q, db, results);
-  output->append(os.str());
-  st->seconds_output += Now() - t;
+  for (const std::string &text : block_text) output->append(text);
   return 0;
 }
 

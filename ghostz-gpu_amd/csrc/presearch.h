@@ -32,9 +32,10 @@ struct SeedHit {
   uint32_t entry;  // entry index within the list
 };
 
-// Passing seeds of one database chunk grouped by query: the hits of query i
-// are hits[qstart[i] .. qstart[i+1]) in no particular order.  The arrays
-// belong to the device stage and stay valid until its next SeedHits call.
+// Passing seeds of one database chunk for queries [qb, qe) grouped by query:
+// the hits of query qb + i are hits[qstart[i] .. qstart[i+1]) in no
+// particular order.  The arrays belong to the device stage and stay valid
+// until its second SeedHits call after this one.
 struct SeedHitView {
   const SeedHit *hits = nullptr;
   size_t count = 0;
@@ -108,9 +109,11 @@ class PresearchDevice {
                               const QueryLists &lists, const std::vector<int> &cutoffs,
                               const std::vector<int> &triggers, int presearch_cutoff,
                               AlignStats *st) = 0;
-  // Passing seeds of all lists against database chunk `chunk`, grouped by
-  // query (nq = queries of the chunk).
-  virtual int SeedHits(size_t chunk, uint32_t nq, SeedHitView *hits, AlignStats *st) = 0;
+  // Passing seeds of queries [qb, qe) of the chunk against database chunk
+  // `chunk`, grouped by query; qb is a multiple of 64 and so is qe unless it
+  // is the number of queries.
+  virtual int SeedHits(size_t chunk, uint32_t qb, uint32_t qe, SeedHitView *hits,
+                       AlignStats *st) = 0;
   // One-sided gapped X-drop extensions of database chunk `chunk`: the first
   // n_reverse jobs extend backwards, the next n_forward forwards; qpos/dpos
   // (concatenated query / chunk positions) are replaced by the extension

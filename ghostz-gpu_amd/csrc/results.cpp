@@ -520,15 +520,24 @@ void BuildChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters 
 
 std::vector<std::pair<uint32_t, uint32_t> > ChunkResultIds(
     const std::vector<std::vector<Presearched> > &pre, size_t ci) {
+  return ChunkResultIds(pre, ci, 0, pre.size());
+}
+
+std::vector<std::pair<uint32_t, uint32_t> > ChunkResultIds(
+    const std::vector<std::vector<Presearched> > &pre, size_t ci, size_t qb, size_t qe) {
   std::vector<std::pair<uint32_t, uint32_t> > ids;
-  for (uint32_t qi = 0; qi < pre.size(); ++qi)
+  for (uint32_t qi = (uint32_t)qb; qi < qe; ++qi)
     for (uint32_t r = 0; r < pre[qi].size(); ++r)
       if (pre[qi][r].chunk == (int)ci) ids.push_back(std::make_pair(qi, r));
   return ids;
 }
 
 void SortResults(std::vector<std::vector<AlignmentResult> > &results) {
-  for (auto &r : results) std::sort(r.begin(), r.end(), ResultByScore());
+  SortResults(results, 0, results.size());
+}
+
+void SortResults(std::vector<std::vector<AlignmentResult> > &results, size_t qb, size_t qe) {
+  for (size_t i = qb; i < qe; ++i) std::sort(results[i].begin(), results[i].end(), ResultByScore());
 }
 
 void BuildResults(const QuerySet &q, const Database &db, const SearchParameters &p,
@@ -549,11 +558,17 @@ void BuildResults(const QuerySet &q, const Database &db, const SearchParameters 
 // ---------------------------------------------------------------------------
 void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,
                  const std::vector<std::vector<AlignmentResult> > &results) {
+  WriteOutputRange(os, q, db, results, 0, q.size(), 0);
+}
+
+void WriteOutputRange(std::ostream &os, const QuerySet &q, const Database &db,
+                      const std::vector<std::vector<AlignmentResult> > &results, size_t qb,
+                      size_t qe, int threads) {
   const KarlinParameters gk = Statistics::GappedBlosum62();
   const double alpha = 1.9, beta = -30;
-  const size_t nq = q.size();
+  const size_t nq = qe;
   const size_t block = 2048;
-  const size_t nblocks = (nq + block - 1) / block;
+  const size_t nblocks = (qe - qb + block - 1) / block;
   std::vector<std::string> text(nblocks);
   std::atomic<size_t> next(0);
   auto work = [&]() {
@@ -561,7 +576,7 @@ void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,
       const size_t b = next.fetch_add(1);
       if (b >= nblocks) break;
       std::ostringstream s;
-      for (size_t qi = b * block; qi < std::min(nq, (b + 1) * block); ++qi) {
+      for (size_t qi = qb + b * block; qi < std::min(nq, qb + (b + 1) * block); ++qi) {
         if (results[qi].empty()) continue;
         const uint64_t space = Statistics::SearchSpace(q.RealLength(qi), db.total_length,
                                                        db.total_sequences, gk, alpha, beta);
@@ -583,8 +598,8 @@ void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,
       text[b] = s.str();
     }
   };
-  const int T = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(),
-                                          (int)nblocks));
+  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  const int T = std::max(1, std::min<int>(threads, (int)nblocks));
   std::vector<std::thread> pool;
   for (int t = 1; t < T; ++t) pool.emplace_back(work);
   work();

@@ -105,6 +105,9 @@ void BuildResults(const QuerySet &q, const Database &db, const SearchParameters 
 // per-query order.
 std::vector<std::pair<uint32_t, uint32_t> > ChunkResultIds(
     const std::vector<std::vector<Presearched> > &pre, size_t ci);
+// the same for queries [qb, qe) only (a contiguous part of the above)
+std::vector<std::pair<uint32_t, uint32_t> > ChunkResultIds(
+    const std::vector<std::vector<Presearched> > &pre, size_t ci, size_t qb, size_t qe);
 void BuildChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters &p, int cutoff,
                     std::vector<std::vector<Presearched> > &pre,
                     std::vector<std::vector<AlignmentResult> > &results,
@@ -117,14 +120,19 @@ void ReplayChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters
 bool BuildOneFresh(const QuerySet &q, uint32_t qi, const DbChunk &c, const Presearched &pr,
                    const SearchParameters &p, int cutoff, AlignmentResult *out);
 void SortResults(std::vector<std::vector<AlignmentResult> > &results);
+void SortResults(std::vector<std::vector<AlignmentResult> > &results, size_t qb, size_t qe);
 // The result build of one hit with a fresh extender on the framed query
 // (query[0] is its leading delimiter) and the chunk codes.
 TracebackOut TracebackOneHost(const uint8_t *query, uint32_t query_length,
                               const uint8_t *database, uint32_t hit_q, uint32_t hit_d,
                               const SearchParameters &p, int cutoff);
 
-// AlignerCommon::WriteOutput (src/aligner_common.h:345-397).
+// AlignerCommon::WriteOutput (src/aligner_common.h:345-397); the range form
+// writes queries [qb, qe) on up to `threads` threads (0: all cores).
 void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,
                  const std::vector<std::vector<AlignmentResult> > &results);
+void WriteOutputRange(std::ostream &os, const QuerySet &q, const Database &db,
+                      const std::vector<std::vector<AlignmentResult> > &results, size_t qb,
+                      size_t qe, int threads);
 
 }  // namespace gz

@@ -37,6 +37,7 @@ struct DeviceQueryLists {
   const uint64_t *packed = nullptr;  // concatenated queries
   const int *cutoffs = nullptr;
   const int *triggers = nullptr;
+  uint32_t nqueries = 0;
   uint32_t nlists = 0;
   const uint32_t *hash = nullptr, *entry_off = nullptr, *entry_pos = nullptr,
                  *entry_qid = nullptr;
@@ -68,11 +69,18 @@ struct SeedStageStats {
 class SeedStage {
  public:
   ~SeedStage();
-  // Runs the seed stage of one query chunk (nq queries) against one database
-  // chunk and returns the passing seeds grouped by query, in pinned host
-  // memory owned by the stage (valid until the next Run).
+  // Finds the database representatives and values of every list of a query
+  // chunk in one database chunk; Run uses them until the next Lookup.
+  hipError_t Lookup(const DeviceQueryLists &q, const DeviceDbChunk &db, bool clustering,
+                    hipStream_t stream);
+  // Runs the seed stage of queries [qb, qe) of the chunk (qb a multiple of
+  // 64, qe too or the number of queries) against the looked-up database
+  // chunk and returns their passing seeds grouped by query (qstart indexed
+  // by qid - qb), in pinned host memory owned by the stage.  The stage keeps
+  // two such buffers and alternates: a view stays valid until the second
+  // Run after it.
   hipError_t Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
-                 const SeedStageParams &p, uint32_t nq, hipStream_t stream,
+                 const SeedStageParams &p, uint32_t qb, uint32_t qe, hipStream_t stream,
                  SeedHitView *hits, SeedStageStats *stats);
 
  private:
@@ -85,8 +93,13 @@ class SeedStage {
   void *grouped_ = nullptr; size_t grouped_cap_ = 0;   // hits grouped by query
   void *tile_range_ = nullptr; size_t tile_range_cap_ = 0;  // per-tile list ranges
   void *qcount_ = nullptr; size_t qcount_cap_ = 0;     // per-query counts / cursors
-  void *host_hits_ = nullptr; size_t host_hits_cap_ = 0;  // pinned
-  void *host_qstart_ = nullptr; size_t host_qstart_cap_ = 0;  // pinned
+  void *base_info_ = nullptr; size_t base_info_cap_ = 0;  // per-list database ranges
+  bool looked_up_ = false;
+  int flip_ = 0;  // host buffer of the next Run
+  void *host_hits_[2] = {nullptr, nullptr};  // pinned
+  size_t host_hits_cap_[2] = {0, 0};
+  void *host_qstart_[2] = {nullptr, nullptr};  // pinned
+  size_t host_qstart_cap_[2] = {0, 0};
   uint32_t *counters_ = nullptr;
   hipEvent_t ev_[2] = {nullptr, nullptr};
   void *pinned_ = nullptr; size_t pinned_cap_ = 0;

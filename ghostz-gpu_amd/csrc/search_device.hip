@@ -20,26 +20,26 @@ constexpr uint32_t kSeedTile = kSeedThreads * kSeedItems;
 constexpr uint64_t kUnitsPerLaunch = 1ull << 28;
 constexpr size_t kInitialHitCapacity = 1u << 22;
 
+// per list: database representatives and values with the list's hash, and
+// the list's entries that take part in this run
 struct ListInfo {
-  uint32_t rep_begin, nrep, val_begin, nval;
+  uint32_t rep_begin, nrep, val_begin, nval, ent_begin, nent;
 };
 
 // counters: [0] hits, [1] distance pairs, [2] ungapped extensions
 constexpr int kCounters = 4;
 
 __global__ void LookupKernel(DeviceQueryLists q, DeviceDbChunk db, int clustering,
-                             ListInfo *info, uint64_t *units_rep,
-                             uint64_t *units_val) {
+                             ListInfo *info) {
   const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= q.nlists) return;
   const uint32_t h = q.hash[l];
-  const uint64_t P = q.entry_off[l + 1] - q.entry_off[l];
   uint32_t lo = 0, hi = db.nhash;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
     if (db.hashes[mid] < h) lo = mid + 1; else hi = mid;
   }
-  ListInfo li = {0, 0, 0, 0};
+  ListInfo li = {0, 0, 0, 0, 0, 0};
   if (lo < db.nhash && db.hashes[lo] == h) {
     if (clustering) {
       li.rep_begin = db.rep_off[lo];
@@ -49,8 +49,39 @@ __global__ void LookupKernel(DeviceQueryLists q, DeviceDbChunk db, int clusterin
     li.nval = db.val_off[lo + 1] - li.val_begin;
   }
   info[l] = li;
-  units_rep[l] = (uint64_t)li.nrep * P;
-  units_val[l] = (uint64_t)li.nval * P;
+}
+
+// Entries of queries [qb, qe) in every list and the list's units.  Entries
+// are in appearance order, which visits queries in blocks of 64, so with qb
+// and qe multiples of 64 (or qe past the last query) they are one run.
+__global__ void BlockUnitsKernel(DeviceQueryLists q, const ListInfo *base, uint32_t qb,
+                                 uint32_t qe, ListInfo *info, uint64_t *units_rep,
+                                 uint64_t *units_val) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= q.nlists) return;
+  ListInfo li = base[l];
+  const uint32_t eb = q.entry_off[l], ee = q.entry_off[l + 1];
+  uint32_t lo = eb, hi = ee;
+  if (qb > 0 || qe < 0xffffffffu) {
+    const uint32_t gb = qb >> 6, ge = qe == 0xffffffffu ? 0xffffffffu : qe >> 6;
+    uint32_t a = eb, b = ee;  // first entry with query block >= gb
+    while (a < b) {
+      const uint32_t mid = (a + b) >> 1;
+      if ((q.entry_qid[mid] >> 6) < gb) a = mid + 1; else b = mid;
+    }
+    lo = a;
+    b = ee;  // first entry with query block >= ge
+    while (a < b) {
+      const uint32_t mid = (a + b) >> 1;
+      if ((q.entry_qid[mid] >> 6) < ge) a = mid + 1; else b = mid;
+    }
+    hi = a;
+  }
+  li.ent_begin = lo;
+  li.nent = hi - lo;
+  info[l] = li;
+  units_rep[l] = (uint64_t)li.nrep * li.nent;
+  units_val[l] = (uint64_t)li.nval * li.nent;
 }
 
 __global__ void ActiveFlagsKernel(uint32_t n, const uint64_t *units, uint32_t *flags) {
@@ -204,10 +235,9 @@ __global__ void __launch_bounds__(kSeedThreads)
       if (i < count) {
         const uint32_t l = active[first + i];
         const ListInfo li = info[l];
-        const uint32_t eb = q.entry_off[l];
         lrank[i] = l;
-        leb[i] = eb;
-        lp[i] = q.entry_off[l + 1] - eb;
+        leb[i] = li.ent_begin;
+        lp[i] = li.nent;
         lbeg[i] = kValues ? li.val_begin : li.rep_begin;
         lnrep[i] = li.nrep;
       }
@@ -290,28 +320,31 @@ __global__ void FillU32Kernel(uint32_t *p, size_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void QueryCountKernel(const SeedHit *hits, size_t n, uint32_t *count) {
+__global__ void QueryCountKernel(const SeedHit *hits, size_t n, uint32_t qb, uint32_t *count) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&count[hits[i].qid], 1u);
+  if (i < n) atomicAdd(&count[hits[i].qid - qb], 1u);
 }
 
-__global__ void QueryScatterKernel(const SeedHit *hits, size_t n, uint32_t *cursor,
-                                   SeedHit *out) {
+__global__ void QueryScatterKernel(const SeedHit *hits, size_t n, uint32_t qb,
+                                   uint32_t *cursor, SeedHit *out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const SeedHit h = hits[i];
-    out[atomicAdd(&cursor[h.qid], 1u)] = h;
+    out[atomicAdd(&cursor[h.qid - qb], 1u)] = h;
   }
 }
 
 }  // namespace
 
 SeedStage::~SeedStage() {
-  void *ps[] = {lookup_, units_, scan_tmp_, active_, hits_, grouped_, qcount_, tile_range_};
+  void *ps[] = {lookup_,  units_,  scan_tmp_,   active_,   hits_,
+                grouped_, qcount_, tile_range_, base_info_};
   for (void *p : ps)
     if (p) (void)hipFree(p);
-  if (host_hits_) (void)hipHostFree(host_hits_);
-  if (host_qstart_) (void)hipHostFree(host_qstart_);
+  for (int k = 0; k < 2; ++k) {
+    if (host_hits_[k]) (void)hipHostFree(host_hits_[k]);
+    if (host_qstart_[k]) (void)hipHostFree(host_qstart_[k]);
+  }
   if (counters_) (void)hipFree(counters_);
   if (pinned_) (void)hipHostFree(pinned_);
   for (hipEvent_t e : ev_)
@@ -337,10 +370,30 @@ hipError_t SeedStage::Reserve(void **p, size_t *cap, size_t bytes) {
     if (_e != hipSuccess) return _e;   \
   } while (0)
 
+hipError_t SeedStage::Lookup(const DeviceQueryLists &q, const DeviceDbChunk &db,
+                             bool clustering, hipStream_t s) {
+  looked_up_ = false;
+  const uint32_t n = q.nlists;
+  if (n > 0) {
+    GZ_TRY(Reserve(&base_info_, &base_info_cap_, (size_t)n * sizeof(ListInfo)));
+    hipLaunchKernelGGL(LookupKernel, dim3((n + 255) / 256), dim3(256), 0, s, q, db,
+                       clustering ? 1 : 0, (ListInfo *)base_info_);
+    GZ_TRY(hipGetLastError());
+  }
+  looked_up_ = true;
+  return hipSuccess;
+}
+
 hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
-                          const SeedStageParams &p, uint32_t nq, hipStream_t s,
+                          const SeedStageParams &p, uint32_t qb, uint32_t qe, hipStream_t s,
                           SeedHitView *out, SeedStageStats *stats) {
   *out = SeedHitView();
+  if (!looked_up_ || qb > qe || qe > q.nqueries || (qb & 63) != 0 ||
+      ((qe & 63) != 0 && qe != q.nqueries))
+    return hipErrorInvalidValue;
+  const uint32_t nq = qe - qb;
+  const int slot = flip_;
+  flip_ ^= 1;
   if (!counters_) GZ_TRY(hipMalloc(&counters_, kCounters * sizeof(uint32_t)));
   if (!pinned_) {
     GZ_TRY(hipHostMalloc(&pinned_, 64, hipHostMallocDefault));
@@ -353,17 +406,17 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
   // per-query counts, also the qstart array handed back
   GZ_TRY(Reserve(&qcount_, &qcount_cap_, 2 * AlignUp((size_t)(nq + 1) * 4, 256) +
                                              DeviceScanTempBytes<uint32_t>(nq + 1)));
-  if (host_qstart_cap_ < (size_t)(nq + 1) * 4) {
-    if (host_qstart_) GZ_TRY(hipHostFree(host_qstart_));
-    host_qstart_ = nullptr;
-    host_qstart_cap_ = 0;
-    GZ_TRY(hipHostMalloc(&host_qstart_, (size_t)(nq + 1) * 4, hipHostMallocDefault));
-    host_qstart_cap_ = (size_t)(nq + 1) * 4;
+  if (host_qstart_cap_[slot] < (size_t)(nq + 1) * 4) {
+    if (host_qstart_[slot]) GZ_TRY(hipHostFree(host_qstart_[slot]));
+    host_qstart_[slot] = nullptr;
+    host_qstart_cap_[slot] = 0;
+    GZ_TRY(hipHostMalloc(&host_qstart_[slot], (size_t)(nq + 1) * 4, hipHostMallocDefault));
+    host_qstart_cap_[slot] = (size_t)(nq + 1) * 4;
   }
-  uint32_t *h_qstart = (uint32_t *)host_qstart_;
+  uint32_t *h_qstart = (uint32_t *)host_qstart_[slot];
   const uint32_t n = q.nlists;
   uint64_t total_hits = 0;
-  if (n > 0) {
+  if (n > 0 && nq > 0) {
     // layout: info | units_rep | units_val | off_rep | off_val | flags | slots
     const size_t sz_info = AlignUp(n * sizeof(ListInfo), 256);
     const size_t sz_u64 = AlignUp((size_t)(n + 1) * 8, 256);
@@ -384,8 +437,9 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
     if (!hits_) GZ_TRY(Reserve(&hits_, &hits_cap_, kInitialHitCapacity * sizeof(SeedHit)));
 
     const unsigned grid_n = (n + 255) / 256;
-    hipLaunchKernelGGL(LookupKernel, dim3(grid_n), dim3(256), 0, s, q, db,
-                       p.clustering ? 1 : 0, info, units[0], units[1]);
+    hipLaunchKernelGGL(BlockUnitsKernel, dim3(grid_n), dim3(256), 0, s, q,
+                       (const ListInfo *)base_info_, qb,
+                       qe == q.nqueries ? 0xffffffffu : qe, info, units[0], units[1]);
     GZ_TRY(hipGetLastError());
 
     uint32_t *act[2];
@@ -495,7 +549,7 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
     GZ_TRY(Reserve(&grouped_, &grouped_cap_, total_hits * sizeof(SeedHit)));
     const unsigned g = (unsigned)((total_hits + 255) / 256);
     hipLaunchKernelGGL(QueryCountKernel, dim3(g), dim3(256), 0, s, (const SeedHit *)hits_,
-                       (size_t)total_hits, qcount);
+                       (size_t)total_hits, qb, qcount);
   }
   GZ_TRY((DeviceScan<uint32_t, false>(qcount, qstart, nq + 1, false, qtemp, s)));
   GZ_TRY(hipMemcpyAsync(h_qstart, qstart, (size_t)(nq + 1) * 4, hipMemcpyDeviceToHost, s));
@@ -504,21 +558,21 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
     GZ_TRY(hipMemcpyAsync(qcount, qstart, (size_t)nq * 4, hipMemcpyDeviceToDevice, s));
     const unsigned g = (unsigned)((total_hits + 255) / 256);
     hipLaunchKernelGGL(QueryScatterKernel, dim3(g), dim3(256), 0, s, (const SeedHit *)hits_,
-                       (size_t)total_hits, qcount, (SeedHit *)grouped_);
+                       (size_t)total_hits, qb, qcount, (SeedHit *)grouped_);
     GZ_TRY(hipGetLastError());
     const size_t bytes = total_hits * sizeof(SeedHit);
-    if (host_hits_cap_ < bytes) {
-      if (host_hits_) GZ_TRY(hipHostFree(host_hits_));
-      host_hits_ = nullptr;
-      host_hits_cap_ = 0;
+    if (host_hits_cap_[slot] < bytes) {
+      if (host_hits_[slot]) GZ_TRY(hipHostFree(host_hits_[slot]));
+      host_hits_[slot] = nullptr;
+      host_hits_cap_[slot] = 0;
       const size_t want = bytes + bytes / 4;
-      GZ_TRY(hipHostMalloc(&host_hits_, want, hipHostMallocDefault));
-      host_hits_cap_ = want;
+      GZ_TRY(hipHostMalloc(&host_hits_[slot], want, hipHostMallocDefault));
+      host_hits_cap_[slot] = want;
     }
-    GZ_TRY(hipMemcpyAsync(host_hits_, grouped_, bytes, hipMemcpyDeviceToHost, s));
+    GZ_TRY(hipMemcpyAsync(host_hits_[slot], grouped_, bytes, hipMemcpyDeviceToHost, s));
   }
   GZ_TRY(hipStreamSynchronize(s));
-  out->hits = (const SeedHit *)host_hits_;
+  out->hits = (const SeedHit *)host_hits_[slot];
   out->count = total_hits;
   out->qstart = h_qstart;
   return hipSuccess;

@@ -31,6 +31,10 @@ using namespace gz;
 
 namespace {
 
+// every n-th traceback (n > 0) reports a stale read: the host must then
+// rebuild the chunk in the reference's order
+int g_forced_stale = 0;
+
 class CpuDevice : public PresearchDevice {
  public:
   CpuDevice(const Database &db, const SearchParameters &p) : db_(db), p_(p) {}
@@ -48,7 +52,12 @@ class CpuDevice : public PresearchDevice {
     return 0;
   }
 
-  int SeedHits(size_t ci, uint32_t nq, SeedHitView *view, AlignStats *st) override {
+  int SeedHits(size_t ci, uint32_t qb, uint32_t qe, SeedHitView *view,
+               AlignStats *st) override {
+    if (qb > qe || (qb & 63) != 0 || ((qe & 63) != 0 && qe != cut_.size())) return -2;
+    const uint32_t nq = qe - qb;
+    const int slot = flip_;
+    flip_ ^= 1;
     std::vector<SeedHit> *hits = &raw_;
     const DbChunk &c = db_.chunks[ci];
     const uint8_t *qs = concat_->data();
@@ -69,6 +78,7 @@ class CpuDevice : public PresearchDevice {
         const uint32_t ri = c.rep_off[hi] + r;
         const uint32_t dpos = c.rep_pos[ri];
         for (uint32_t e = eb; e < ee; ++e) {
+          if (ql.entry_qid[e] < qb || ql.entry_qid[e] >= qe) continue;
           st->units_rep += 1;
           const uint32_t qpos = ql.entry_pos[e], qid = ql.entry_qid[e];
           const bool check = ql.entry_check[e] != 0;
@@ -88,6 +98,7 @@ class CpuDevice : public PresearchDevice {
       for (uint32_t v = c.val_off[hi]; v < c.val_off[hi + 1]; ++v) {
         const uint32_t dpos = c.val_pos[v];
         for (uint32_t e = eb; e < ee; ++e) {
+          if (ql.entry_qid[e] < qb || ql.entry_qid[e] >= qe) continue;
           st->units_value += 1;
           const uint32_t qpos = ql.entry_pos[e], qid = ql.entry_qid[e];
           SeedHit sh = {qid, qpos, dpos, l, nrep + (v - c.val_off[hi]), 0, e - eb};
@@ -97,15 +108,17 @@ class CpuDevice : public PresearchDevice {
     }
     // the device returns seeds grouped by query, in no particular order
     std::reverse(hits->begin(), hits->end());
-    qstart_.assign(nq + 1, 0);
-    for (const SeedHit &h : *hits) ++qstart_[h.qid + 1];
-    for (uint32_t k = 0; k < nq; ++k) qstart_[k + 1] += qstart_[k];
-    grouped_.resize(hits->size());
-    std::vector<uint32_t> fill(qstart_.begin(), qstart_.end() - 1);
-    for (const SeedHit &h : *hits) grouped_[fill[h.qid]++] = h;
-    view->hits = grouped_.data();
-    view->count = grouped_.size();
-    view->qstart = qstart_.data();
+    std::vector<uint32_t> &qstart = qstart_[slot];
+    std::vector<SeedHit> &grouped = grouped_[slot];
+    qstart.assign(nq + 1, 0);
+    for (const SeedHit &h : *hits) ++qstart[h.qid - qb + 1];
+    for (uint32_t k = 0; k < nq; ++k) qstart[k + 1] += qstart[k];
+    grouped.resize(hits->size());
+    std::vector<uint32_t> fill(qstart.begin(), qstart.end() - 1);
+    for (const SeedHit &h : *hits) grouped[fill[h.qid - qb]++] = h;
+    view->hits = grouped.data();
+    view->count = grouped.size();
+    view->qstart = qstart.data();
     return 0;
   }
 
@@ -137,8 +150,9 @@ class CpuDevice : public PresearchDevice {
       const TracebackJob &j = jobs[k];
       (*out)[k] = TracebackOneHost(concat_->data() + j.query_offset, j.query_length,
                                    db_.chunks[ci].seq.data(), j.hit_q, j.hit_d, p_, cutoff);
-      // exercise the host fall-back path too
+      // exercise the host fall-back paths too
       if (k % 97 == 5) (*out)[k].flags |= kTracebackOverflow;
+      if (g_forced_stale > 0 && k % g_forced_stale == 3) (*out)[k].flags |= kTracebackStale;
     }
     return 0;
   }
@@ -158,8 +172,9 @@ class CpuDevice : public PresearchDevice {
   const QueryLists *lists_ = nullptr;
   std::vector<int> cut_, trig_;
   int presearch_cutoff_ = 0;
-  std::vector<SeedHit> raw_, grouped_;
-  std::vector<uint32_t> qstart_;
+  std::vector<SeedHit> raw_, grouped_[2];
+  std::vector<uint32_t> qstart_[2];
+  int flip_ = 0;
 };
 
 std::vector<FastaRecord> Records(const char *const *names, const char *residues,
@@ -175,6 +190,8 @@ std::vector<FastaRecord> Records(const char *const *names, const char *residues,
 }  // namespace
 
 extern "C" {
+
+void gzh_force_stale(int every) { g_forced_stale = every; }
 
 /* The product's host flow with CPU-emulated device stages.  stats[0..5] =
  * seed hits, chained hits, gapped extensions, gapped cells, results,

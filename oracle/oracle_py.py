@@ -265,6 +265,8 @@ def hostpipe(path=None):
                                 c_void_p, c_char_p, c_void_p, c_uint64, c_int, c_uint64,
                                 c_void_p, _P(c_void_p), _P(c_size_t), c_void_p]
         L.gzh_free.argtypes = [c_void_p]
+        L.gzh_force_stale.restype = None
+        L.gzh_force_stale.argtypes = [c_int]
         if path:
             return L
         _hostpipe = L
@@ -272,10 +274,12 @@ def hostpipe(path=None):
 
 
 def host_align(queries, database, params, translated=False, filter=True, clustering=True,
-               db_chunk_size=0, lib=None):
+               db_chunk_size=0, lib=None, force_stale=0):
     """queries/database = (names, residues bytes, offsets); params a
-    gz_native.AlignerParameters.  Returns (text, stats dict)."""
+    gz_native.AlignerParameters.  Returns (text, stats dict).  force_stale=n
+    makes every n-th emulated device traceback report a stale read."""
     L = lib or hostpipe()
+    L.gzh_force_stale(int(force_stale))
     (qn, qr, qo), (dn, dr, do) = queries, database
     qna = (c_char_p * len(qn))(*[n.encode() for n in qn])
     dna = (c_char_p * len(dn))(*[n.encode() for n in dn])

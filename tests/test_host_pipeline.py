@@ -15,7 +15,7 @@ from test_pipeline_gpu import oracle_text
 
 
 def check(name, qd, dd, translated=False, filter=True, clustering=True, db_chunk_size=0,
-          schedules=(0, 1), batched=(0, 1), **params):
+          schedules=(0, 1), batched=(0, 1), force_stale=0, **params):
     m, nl = gz.blosum62()
     q = gz.Queries(*qd, translated=translated, filter=filter)
     db = gz.Database(*dd, clustering=clustering, chunk_size=db_chunk_size)
@@ -29,7 +29,8 @@ def check(name, qd, dd, translated=False, filter=True, clustering=True, db_chunk
             p = gz.default_parameters(schedule=sch, record_traces=1, batched=bat,
                                       pipeline_blocks=5, **params)
             got, st = oracle_py.host_align(qd, dd, p, translated=translated, filter=filter,
-                                           clustering=clustering, db_chunk_size=db_chunk_size)
+                                           clustering=clustering, db_chunk_size=db_chunk_size,
+                                           force_stale=force_stale)
             assert got == want, (name, sch, bat)
             assert st["seed_hits"] == ost["hits"]
             assert st["chained_hits"] == ost["hits_after_chain_filter"]
@@ -75,3 +76,16 @@ def test_edge_queries():
     offs = np.zeros(len(texts) + 1, np.uint64)
     np.cumsum([len(t) for t in texts], out=offs[1:])
     check("edge", ([f"e{i}" for i in range(len(texts))], b"".join(texts), offs), dd)
+
+
+def test_many_blocks_and_database_chunks():
+    # several 64-query pipeline blocks, each seeded, extended, built and
+    # written on its own, over three database chunks
+    qd, dd = synthetic.protein_workload(38, 1500, 330)
+    check("blocks", qd, dd, db_chunk_size=150_000, schedules=(0,))
+
+
+def test_stale_traceback_rebuilds_in_reference_order():
+    # a stale device traceback makes the host rebuild and rewrite everything
+    qd, dd = synthetic.protein_workload(39, 1200, 200)
+    check("stale", qd, dd, db_chunk_size=200_000, schedules=(0,), force_stale=11)
