@@ -60,7 +60,8 @@ class Aligner : public PresearchDevice {
   AlignerOptions opt_;
   const Database *db_ = nullptr;
   std::vector<std::unique_ptr<DeviceChunk> > chunks_;
-  hipStream_t stream_ = nullptr;
+  hipStream_t stream_ = nullptr;         // uploads and the seed stage
+  hipStream_t extend_stream_ = nullptr;  // gapped extension and result build
   int *d_matrix_ = nullptr;
   uint8_t *d_reduced_ = nullptr;
   SeedStage seed_;

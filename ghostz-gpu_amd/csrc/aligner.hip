@@ -9,6 +9,7 @@
 #include "gz_scan.h"
 #include "schedule.h"
 #include "traceback.h"
+#include "ungapped.h"
 #include "cost_model.h"
 #include "gz_runtime.h"
 #include "gz_common.h"
@@ -77,11 +78,13 @@ Aligner::~Aligner() {
   if (d_matrix_) (void)hipFree(d_matrix_);
   if (d_reduced_) (void)hipFree(d_reduced_);
   if (stream_) (void)hipStreamDestroy(stream_);
+  if (extend_stream_) (void)hipStreamDestroy(extend_stream_);
 }
 
 hipError_t Aligner::SetDatabase(const Database *db) {
   db_ = db;
   if (!stream_) GZ_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (!extend_stream_) GZ_TRY(hipStreamCreateWithFlags(&extend_stream_, hipStreamNonBlocking));
   chunks_.clear();
   for (const DbChunk &c : db->chunks) {
     std::unique_ptr<DeviceChunk> d(new DeviceChunk());
@@ -119,6 +122,8 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
                              AlignStats *st) {
   (void)st;
   EndQueryChunk();
+  for (int c : cut)
+    if (c > kMaxUngappedCutoff) return (int)hipErrorInvalidValue;  // beyond the stop score
   max_query_length_ = q.max_length;
   hipStream_t s = stream_;
   uint64_t *d_q = nullptr;
@@ -224,7 +229,7 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
                           AlignStats *st) {
   const size_t n = (size_t)nr + nf;
   if (n == 0) return 0;
-  hipStream_t s = stream_;
+  hipStream_t s = extend_stream_;
   for (hipEvent_t &e : ev_)
     if (!e) GZ_TRY(hipEventCreate(&e));
   const DeviceChunk &dc = *chunks_[ci];
@@ -336,7 +341,7 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   const size_t n = jobs.size();
   out->resize(n);
   if (n == 0) return 0;
-  hipStream_t s = stream_;
+  hipStream_t s = extend_stream_;
   for (hipEvent_t &e : ev_)
     if (!e) GZ_TRY(hipEventCreate(&e));
   const LaneCaps caps = TracebackCaps(max_query_length_, db_->max_length);

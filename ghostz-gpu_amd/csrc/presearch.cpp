@@ -419,40 +419,43 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       }
       cv.notify_all();
     };
-    std::thread device([&]() {
-      for (size_t k = 0; k < nb + 2; ++k) {
+    // the seeder and the extender drive the device's two streams
+    std::thread seeder([&]() {
+      for (size_t k = 0; k < nb; ++k) {
+        // the view of block k reuses the buffer of block k - 2
+        if (k >= 2 && !wait_for(chained_upto, k - 1)) return;
+        const double t0 = Now();
+        const int r = dev->SeedHits(ci, (uint32_t)blocks[k].qb, (uint32_t)blocks[k].qe,
+                                    &views[k], st);
+        seed_hits += views[k].count;
+        sec_seed += Now() - t0;
+        publish(seeded, k + 1, r);
+        if (r != 0) return;
+      }
+    });
+    std::thread extender([&]() {
+      for (size_t k = 0; k <= nb; ++k) {
         if (k < nb) {
-          // the view of block k reuses the buffer of block k - 2
-          if (k >= 2 && !wait_for(chained_upto, k - 1)) return;
+          if (!wait_for(chained_upto, k + 1)) return;
           const double t0 = Now();
-          const int r = dev->SeedHits(ci, (uint32_t)blocks[k].qb, (uint32_t)blocks[k].qe,
-                                      &views[k], st);
-          seed_hits += views[k].count;
-          sec_seed += Now() - t0;
-          publish(seeded, k + 1, r);
-          if (r != 0) return;
-        }
-        if (k >= 1 && k <= nb) {
-          if (!wait_for(chained_upto, k)) return;
-          const double t0 = Now();
-          const int r = gapped(ci, blocks[k - 1]);
+          const int r = gapped(ci, blocks[k]);
           sec_gapped += Now() - t0;
-          publish(gapped_upto, k, r);
+          publish(gapped_upto, k + 1, r);
           if (r != 0) return;
         }
-        if (last && k >= 2) {
-          if (!wait_for(added, k - 1)) return;
+        if (last && k >= 1) {
+          if (!wait_for(added, k)) return;
           const double t0 = Now();
           int r = 0;
           if (!unsupported) {
-            r = build_device(blocks[k - 2].qb, blocks[k - 2].qe, &block_parts[k - 2]);
+            r = build_device(blocks[k - 1].qb, blocks[k - 1].qe, &block_parts[k - 1]);
             if (r == PresearchDevice::kTracebackUnsupported) {
               unsupported = true;
               r = 0;
             }
           }
           sec_build += Now() - t0;
-          publish(traced, k - 1, r);
+          publish(traced, k, r);
           if (r != 0) return;
         }
       }
@@ -498,7 +501,8 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       sec_chain += Now() - t0;
       publish(chained_upto, b + 1, 0);
     }
-    device.join();
+    seeder.join();
+    extender.join();
     adder.join();
     if (writer.joinable()) writer.join();
     GZ_STEP(status);

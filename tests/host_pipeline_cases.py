@@ -1,6 +1,7 @@
-"""Runs the host-flow cases of test_host_pipeline.py against a given
-libhostpipe build (a child process of test_host_sanitizer.py, which loads
-the AddressSanitizer build).  Exits non-zero on a mismatch."""
+"""Runs the host-flow cases of test_host_pipeline.py (all, or those named
+after the library path) against a given libhostpipe build (a child process
+of test_host_sanitizer.py, which loads a sanitizer build).  Exits non-zero
+on a mismatch."""
 import os
 import sys
 
@@ -13,8 +14,9 @@ import oracle_py  # noqa: E402
 oracle_py.HOSTPIPE_PATH = sys.argv[1]
 import test_host_pipeline as T  # noqa: E402
 
+only = set(sys.argv[2:])
 for name in sorted(dir(T)):
-    if name.startswith("test_"):
+    if name.startswith("test_") and (not only or name in only):
         fn = getattr(T, name)
         if name == "test_thread_count_does_not_change_output":
             for t in (1, 3):

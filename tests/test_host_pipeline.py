@@ -89,3 +89,12 @@ def test_stale_traceback_rebuilds_in_reference_order():
     # a stale device traceback makes the host rebuild and rewrite everything
     qd, dd = synthetic.protein_workload(39, 1200, 200)
     check("stale", qd, dd, db_chunk_size=200_000, schedules=(0,), force_stale=11)
+
+
+def test_small_many_blocks():
+    # five 64-query blocks over small database chunks, with and without
+    # stale tracebacks (small enough for the ThreadSanitizer run)
+    qd, dd = synthetic.protein_workload(40, 150, 300)
+    check("small blocks", qd, dd, db_chunk_size=20_000, schedules=(0,), batched=(1,))
+    check("small blocks stale", qd, dd, db_chunk_size=20_000, schedules=(0,), batched=(1,),
+          force_stale=5)
