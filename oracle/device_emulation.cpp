@@ -26,6 +26,7 @@
 #include "presearch.h"
 #include "queries.h"
 #include "results.h"
+#include "ungapped.h"
 
 using namespace gz;
 
@@ -254,5 +255,25 @@ int gzh_align(const char *const *qnames, const char *qres, const uint64_t *qoffs
 }
 
 void gzh_free(void *p) { free(p); }
+
+/* The product's word-at-a-time seed tests (ungapped.h) run on the host over
+ * padded packed arrays, for checks against the oracle. */
+void gzh_ungapped_words(const uint64_t *qw, const uint64_t *dw, uint64_t n,
+                        const uint32_t *qpos, const uint32_t *dpos, const int *matrix,
+                        uint32_t nl, const int *cutoff, const int *trigger, int *out) {
+  int16_t tab[kTableDim * kTableDim];
+  for (uint32_t i = 0; i < kTableDim * kTableDim; ++i) tab[i] = UngappedTableEntry(matrix, nl, i);
+  for (uint64_t k = 0; k < n; ++k)
+    out[k] = UngappedWithTrigger(qw, qpos[k], dw, dpos[k], tab, cutoff[k], trigger[k]);
+}
+
+void gzh_distance_words(const uint64_t *qw, const uint64_t *dw, uint64_t n,
+                        const uint32_t *qc, const uint32_t *dc, uint32_t length,
+                        const uint8_t *reduced_map, int *out) {
+  uint8_t differs[kTableDim * kTableDim];
+  for (uint32_t i = 0; i < kTableDim * kTableDim; ++i) differs[i] = DiffersEntry(reduced_map, i);
+  for (uint64_t k = 0; k < n; ++k)
+    out[k] = DistanceWindow(qw, qc[k], dw, dc[k], length, differs);
+}
 
 }  // extern "C"

@@ -267,6 +267,12 @@ def hostpipe(path=None):
         L.gzh_free.argtypes = [c_void_p]
         L.gzh_force_stale.restype = None
         L.gzh_force_stale.argtypes = [c_int]
+        L.gzh_ungapped_words.restype = None
+        L.gzh_ungapped_words.argtypes = [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p,
+                                         c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]
+        L.gzh_distance_words.restype = None
+        L.gzh_distance_words.argtypes = [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p,
+                                         c_uint32, c_void_p, c_void_p]
         if path:
             return L
         _hostpipe = L
