@@ -9,6 +9,7 @@
 #include "gz_device.h"
 #include "gz_scan.h"
 #include "search_device.h"
+#include "ungapped.h"
 
 namespace gz {
 
@@ -127,62 +128,6 @@ __device__ __forceinline__ void WaveCount(bool pred, uint32_t *counter) {
     atomicAdd(counter, (uint32_t)__popcll(mask));
 }
 
-template <class Cursor>
-__device__ __forceinline__ int Side(Cursor &q, Cursor &d, const int8_t *sm,
-                                    uint32_t delim, int cutoff) {
-  int best = 0;
-  uint32_t qc = q.Next(), dc = d.Next();
-  if (dc != delim && qc != delim) {
-    int score = 0;
-    do {
-      if (dc == delim || qc == delim) break;
-      score += sm[dc * kTableDim + qc];
-      qc = q.Next();
-      dc = d.Next();
-      if (score > 0) {
-        do {
-          best += score;
-          if (dc == delim || qc == delim) break;
-          score = sm[dc * kTableDim + qc];
-          qc = q.Next();
-          dc = d.Next();
-        } while (score > 0);
-      }
-    } while (score > -cutoff);
-  }
-  return best;
-}
-
-// SeedSearcher::UngappedExtend (src/seed_searcher.h:133-158)
-__device__ __forceinline__ int Ungapped(const uint64_t *queries, uint32_t qpos,
-                                        const uint64_t *database, uint32_t dpos,
-                                        const int8_t *sm, uint32_t delim,
-                                        int cutoff, int trigger) {
-  ReverseCursor rq, rd;
-  rq.Init(queries, qpos - 1);
-  rd.Init(database, dpos - 1);
-  int sum = Side(rq, rd, sm, delim, cutoff);
-  if (sum <= trigger) {
-    ForwardCursor fq, fd;
-    fq.Init(queries, qpos);
-    fd.Init(database, dpos);
-    sum += Side(fq, fd, sm, delim, cutoff);
-  }
-  return sum;
-}
-
-__device__ __forceinline__ int Distance(const uint64_t *queries, uint32_t qc,
-                                        const uint64_t *database, uint32_t dc,
-                                        uint32_t length, const uint8_t *map) {
-  ForwardCursor a, b;
-  const uint32_t half = length / 2;
-  a.Init(queries, qc - half);
-  b.Init(database, dc - half);
-  int count = 0;
-  for (uint32_t k = 0; k < length; ++k) count += (map[a.Next()] != map[b.Next()]) ? 1 : 0;
-  return count;
-}
-
 // first and last active list of every tile of kSeedTile units starting at
 // unit_begin (upper bounds over the active lists' unit offsets)
 __global__ void TileRangeKernel(const uint64_t *active_off, uint32_t nactive,
@@ -212,18 +157,18 @@ __global__ void __launch_bounds__(kSeedThreads)
                const ListInfo *info, const uint32_t *active,
                const uint64_t *active_off, const uint2 *tile_range, uint64_t unit_begin,
                uint64_t unit_end, SeedHit *hits, uint32_t capacity, uint32_t *counters) {
-  __shared__ int8_t sm[kTableDim * kTableDim];
-  __shared__ uint8_t rmap[kTableDim];
+  __shared__ int16_t utab[kTableDim * kTableDim];
+  __shared__ uint8_t differs[kTableDim * kTableDim];
   // the tile's lists: unit offset, list rank, first entry, entries, first
   // representative / value, representatives
   __shared__ uint64_t loff[kSeedTile + 1];
   __shared__ uint32_t lrank[kSeedTile], leb[kSeedTile], lp[kSeedTile], lbeg[kSeedTile],
       lnrep[kSeedTile];
-  StageScoreTable(sm, p.matrix, p.nl);
-  if (threadIdx.x < kTableDim)
-    rmap[threadIdx.x] = threadIdx.x < 25 ? p.reduced_map[threadIdx.x]
-                                         : (uint8_t)(100 + threadIdx.x);
-  const uint32_t delim = p.delimiter;
+  for (uint32_t i = threadIdx.x; i < kTableDim * kTableDim; i += blockDim.x) {
+    utab[i] = UngappedTableEntry(p.matrix, p.nl, i);
+    if (!kValues) differs[i] = DiffersEntry(p.reduced_map, i);
+  }
+  const int lane = threadIdx.x & (kWave - 1);
   for (uint64_t tile = unit_begin + (uint64_t)blockIdx.x * kSeedTile; tile < unit_end;
        tile += (uint64_t)gridDim.x * kSeedTile) {
     const uint64_t tile_end = min(tile + (uint64_t)kSeedTile, unit_end);
@@ -281,14 +226,15 @@ __global__ void __launch_bounds__(kSeedThreads)
         h.entry = e;
         if (kValues) {
           h.rv = lnrep[lo] + idx;
-          pass = Ungapped(q.packed, qpos, db.packed, h.dpos, sm, delim, cut, trig) > trig;
+          pass = UngappedWithTrigger(q.packed, qpos, db.packed, h.dpos, utab, cut, trig) > trig;
         } else {
           h.rv = idx;
           const bool check = q.entry_check[ent] != 0;
           int d = 0;
-          if (check) d = Distance(q.packed, qpos, db.packed, h.dpos, p.subsequence_length, rmap);
+          if (check)
+            d = DistanceWindow(q.packed, qpos, db.packed, h.dpos, p.subsequence_length, differs);
           if (d <= p.distance_threshold)
-            pass = Ungapped(q.packed, qpos, db.packed, h.dpos, sm, delim, cut, trig) > trig;
+            pass = UngappedWithTrigger(q.packed, qpos, db.packed, h.dpos, utab, cut, trig) > trig;
           member_ok = !check || max(d - p.max_mismatches, 0) <= p.distance_threshold;
           if (member_ok) {
             mb = db.mem_off[ri];
@@ -298,15 +244,37 @@ __global__ void __launch_bounds__(kSeedThreads)
       }
       WaveAppend(pass, h, hits, counters, capacity);
       if (!kValues) {
-        for (uint32_t it = 0;; ++it) {
-          const bool act = member_ok && (mb + it < me);
-          if (!__any(act)) break;
+        // the members of the wave's representatives, spread evenly over its
+        // lanes: item t belongs to the lane whose [first, first + count)
+        // holds it
+        const uint32_t count = member_ok ? me - mb : 0u;
+        const uint32_t incl = WaveInclusiveSum(count);
+        const uint32_t total = __shfl(incl, kWave - 1, kWave);
+        const uint32_t first = incl - count;
+        for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+          const uint32_t t = t0 + (uint32_t)lane;
+          int owner = 0;  // first lane with incl > t
+#pragma unroll
+          for (int step = kWave / 2; step >= 1; step >>= 1) {
+            const uint32_t v = __shfl(incl, owner + step - 1, kWave);
+            if (v <= t) owner += step;
+          }
+          owner = min(owner, kWave - 1);
+          SeedHit mh;
+          mh.qid = __shfl(h.qid, owner, kWave);
+          mh.qpos = __shfl(h.qpos, owner, kWave);
+          mh.list = __shfl(h.list, owner, kWave);
+          mh.rv = __shfl(h.rv, owner, kWave);
+          mh.entry = __shfl(h.entry, owner, kWave);
+          const uint32_t k = t - __shfl(first, owner, kWave);
+          const uint32_t m = __shfl(mb, owner, kWave) + k;
+          const int mcut = __shfl(cut, owner, kWave), mtrig = __shfl(trig, owner, kWave);
           bool mp = false;
-          SeedHit mh = h;
-          if (act) {
-            mh.dpos = db.mem_pos[mb + it];
-            mh.member = 1 + it;
-            mp = Ungapped(q.packed, mh.qpos, db.packed, mh.dpos, sm, delim, cut, trig) > trig;
+          if (t < total) {
+            mh.dpos = db.mem_pos[m];
+            mh.member = 1 + k;
+            mp = UngappedWithTrigger(q.packed, mh.qpos, db.packed, mh.dpos, utab, mcut, mtrig) >
+                 mtrig;
           }
           WaveAppend(mp, mh, hits, counters, capacity);
         }
