@@ -213,15 +213,14 @@ __global__ void ToBf16Kernel(size_t n, const float *src, uint16_t *dst) {
 inline unsigned Blocks(size_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // ---------------------------------------------------------------------------
-// Inference: two persistent kernels.
+// Inference: two persistent kernels on v_mfma_f32_32x32x16_bf16.
 //
 // Layer 1 keeps a 64-output slice of W1 (64 x 1024 bf16 = 128 KB) resident in
-// LDS and streams tiles of 128 seeds through it; the embedding lookup is done
-// while building the MFMA A operand (4 bf16 of one residue's 8-dim vector per
-// lane).  Each wave owns 32 seeds x 64 outputs (two 32x32 accumulators that
-// share the A operand).  The head kernel keeps W2 (32 KB) in LDS, stages a
-// 128 x 256 bf16 H1 tile, runs layer 2 on MFMA and reduces layer 3 across
-// lanes.  Rounding points are those of the training forward.
+// LDS and streams tiles of 512 seeds through it; the embedding lookup is done
+// while building the MFMA A operand (a lane's 8 bf16 are one residue's 8-dim
+// vector).  The head kernel keeps W2 (32 KB) in LDS, reads H1 rows straight
+// into the A operand, runs layer 2 on MFMA and reduces layer 3 across lanes.
+// Rounding points are those of the training forward.
 // ---------------------------------------------------------------------------
 constexpr int kInferTile = 128;
 constexpr int kInferSlice = 64;
@@ -238,138 +237,155 @@ __device__ __forceinline__ short4v Load4(const uint16_t *p) {
   return r;
 }
 
-__global__ void __launch_bounds__(256)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 Load8(const uint16_t *p) {
+  return *reinterpret_cast<const bf16x8 *>(p);
+}
+
+// Layer 1 on v_mfma_f32_32x32x16_bf16.  A block of 8 waves holds one 64-column
+// slice of W1 as [k/8][64][8] (128 KB, the B operand of a lane is one 16-byte
+// read) and streams tiles of 512 seeds through it; each wave owns 64 seeds x
+// 64 columns (four 32x32 accumulators).  Operand layout: lane l holds row
+// (A) or column (B) l % 32 and k = 8 * (l / 32) + j, j < 8, so one K step of
+// 16 covers residues 2i (lanes 0-31) and 2i + 1 (lanes 32-63), each lane's A
+// operand being the 8-dim embedding of its residue.
+constexpr int kL1Waves = 8;
+constexpr int kL1Tile = kL1Waves * 64;
+
+__global__ void __launch_bounds__(kL1Waves * 64)
     InferLayer1Kernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h,
                       const uint16_t *embh, const float *b1, uint16_t *h1) {
-  // W1 slice as [k/4][64][4]: the B operand of a lane is one 8-byte read
   __shared__ __align__(16) uint16_t w[kCostIn * kInferSlice];
   __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
   const int slice = blockIdx.x & 3;
   const int group = blockIdx.x >> 2, groups = gridDim.x >> 2;
-  for (int idx = threadIdx.x; idx < kInferSlice * (kCostIn / 4); idx += 256) {
-    const int nn = idx / (kCostIn / 4), k4 = idx % (kCostIn / 4);
-    const uint2 v = *reinterpret_cast<const uint2 *>(
-        w1h + (size_t)(slice * kInferSlice + nn) * kCostIn + k4 * 4);
-    *reinterpret_cast<uint2 *>(&w[(k4 * kInferSlice + nn) * 4]) = v;
+  for (int idx = threadIdx.x; idx < kInferSlice * (kCostIn / 8); idx += blockDim.x) {
+    const int nn = idx / (kCostIn / 8), k8 = idx % (kCostIn / 8);
+    const uint4 v = *reinterpret_cast<const uint4 *>(
+        w1h + (size_t)(slice * kInferSlice + nn) * kCostIn + k8 * 8);
+    *reinterpret_cast<uint4 *>(&w[(k8 * kInferSlice + nn) * 8]) = v;
   }
-  for (int i = threadIdx.x; i < kCostAlphabet * kCostEmbed; i += 256) emb[i] = embh[i];
+  for (int i = threadIdx.x; i < kCostAlphabet * kCostEmbed; i += blockDim.x) emb[i] = embh[i];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int kq = lane >> 5, row_in = lane & 31;
-  const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
+  const int kq = lane >> 5, r32 = lane & 31;
+  const uint32_t tiles = (n + kL1Tile - 1) / kL1Tile;
   for (uint32_t t = group; t < tiles; t += groups) {
-    const uint32_t s0 = t * kInferTile;
-    const uint32_t seed_row = s0 + wave * 32 + row_in;
-    // the lane's 128 residue codes (its A-operand row) stream through
-    // registers 16 at a time, the next 16 loading while these are used
-    const bool live = seed_row < n;
-    const uint4 *src = reinterpret_cast<const uint4 *>(codes + (size_t)(live ? seed_row : 0) *
-                                                                   kCostResidues);
-    const uint32_t fill = live ? 0u : 0x14141414u;  // code 20 for rows past the end
-    uint4 cur = src[0];
-    cur.x |= fill;
-    cur.y |= fill;
-    cur.z |= fill;
-    cur.w |= fill;
-    float16_t acc0, acc1;
+    const uint32_t base = t * kL1Tile + wave * 64;
+    // rows base + r32 and base + 32 + r32: their 128 codes stream through
+    // registers 16 at a time (rows past the end read row 0 as code 20)
+    const uint32_t ra = base + r32, rb = ra + 32;
+    const uint4 *sa = reinterpret_cast<const uint4 *>(codes + (size_t)(ra < n ? ra : 0) *
+                                                                  kCostResidues);
+    const uint4 *sb = reinterpret_cast<const uint4 *>(codes + (size_t)(rb < n ? rb : 0) *
+                                                                  kCostResidues);
+    const uint32_t fa = ra < n ? 0u : 0x14141414u, fb = rb < n ? 0u : 0x14141414u;
+    float16_t acc00, acc01, acc10, acc11;
+#pragma unroll
     for (int r = 0; r < 16; ++r) {
-      acc0[r] = 0.f;
-      acc1[r] = 0.f;
+      acc00[r] = 0.f;
+      acc01[r] = 0.f;
+      acc10[r] = 0.f;
+      acc11[r] = 0.f;
     }
+    const uint32_t kshift = 8u * (uint32_t)kq;
 #pragma unroll 1
     for (int v = 0; v < kCostResidues / 16; ++v) {
-      uint4 nxt = src[v + 1 < kCostResidues / 16 ? v + 1 : v];
-      nxt.x |= fill;
-      nxt.y |= fill;
-      nxt.z |= fill;
-      nxt.w |= fill;
+      uint4 ca = sa[v], cb = sb[v];
+      ca.x |= fa; ca.y |= fa; ca.z |= fa; ca.w |= fa;
+      cb.x |= fb; cb.y |= fb; cb.z |= fb; cb.w |= fb;
 #pragma unroll
-      for (int q4 = 0; q4 < 4; ++q4) {
-        const uint32_t word = q4 == 0 ? cur.x : q4 == 1 ? cur.y : q4 == 2 ? cur.z : cur.w;
-#pragma unroll
-        for (int bt = 0; bt < 4; ++bt) {
-          uint32_t code = (word >> (8 * bt)) & 0xffu;
-          code = code < 20 ? code : 20;
-          const short4v a = Load4(&emb[code * kCostEmbed + kq * 4]);
-          const int kb = 2 * (v * 16 + q4 * 4 + bt) + kq;
-          const short4v b0 = Load4(&w[(kb * kInferSlice + row_in) * 4]);
-          const short4v b1v = Load4(&w[(kb * kInferSlice + 32 + row_in) * 4]);
-          acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1v, acc1, 0, 0, 0);
-        }
+      for (int q = 0; q < 8; ++q) {
+        // residue v * 16 + 2q + kq: byte 2 (q & 1) + kq of word q / 2
+        const uint32_t wa = q < 2 ? ca.x : q < 4 ? ca.y : q < 6 ? ca.z : ca.w;
+        const uint32_t wb = q < 2 ? cb.x : q < 4 ? cb.y : q < 6 ? cb.z : cb.w;
+        uint32_t code_a = (wa >> (16 * (q & 1) + kshift)) & 0xffu;
+        uint32_t code_b = (wb >> (16 * (q & 1) + kshift)) & 0xffu;
+        code_a = code_a < 20 ? code_a : 20;
+        code_b = code_b < 20 ? code_b : 20;
+        const bf16x8 a0 = Load8(&emb[code_a * kCostEmbed]);
+        const bf16x8 a1 = Load8(&emb[code_b * kCostEmbed]);
+        const int kb = v * 16 + 2 * q + kq;
+        const bf16x8 b0 = Load8(&w[(kb * kInferSlice + r32) * 8]);
+        const bf16x8 b1v = Load8(&w[(kb * kInferSlice + 32 + r32) * 8]);
+        acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc00, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1v, acc01, 0, 0, 0);
+        acc10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc10, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1v, acc11, 0, 0, 0);
       }
-      cur = nxt;
     }
+    const int c0 = slice * kInferSlice + r32;
+    const float bias0 = b1[c0], bias1 = b1[c0 + 32];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const uint32_t seed = s0 + wave * 32 + 8 * (r >> 2) + 4 * kq + (r & 3);
-      if (seed < n) {
-        const int c0 = slice * kInferSlice + row_in;
-        h1[(size_t)seed * kCostH1 + c0] = F2Bf(Gelu(acc0[r] + b1[c0]));
-        h1[(size_t)seed * kCostH1 + c0 + 32] = F2Bf(Gelu(acc1[r] + b1[c0 + 32]));
+      const uint32_t row = 8 * (r >> 2) + 4 * kq + (r & 3);
+      const uint32_t sa_row = base + row, sb_row = base + 32 + row;
+      if (sa_row < n) {
+        h1[(size_t)sa_row * kCostH1 + c0] = F2Bf(Gelu(acc00[r] + bias0));
+        h1[(size_t)sa_row * kCostH1 + c0 + 32] = F2Bf(Gelu(acc01[r] + bias1));
+      }
+      if (sb_row < n) {
+        h1[(size_t)sb_row * kCostH1 + c0] = F2Bf(Gelu(acc10[r] + bias0));
+        h1[(size_t)sb_row * kCostH1 + c0 + 32] = F2Bf(Gelu(acc11[r] + bias1));
       }
     }
   }
 }
 
+// Layers 2 and 3: W2 as [k/8][64][8] in LDS (32 KB), each wave owns 32
+// seeds and reads their H1 rows straight into its A operand (16 bytes per
+// lane and K step of 16), layer 3 reduces across lanes.
 __global__ void __launch_bounds__(256)
     InferHeadKernel(const uint16_t *h1, uint32_t n, const uint16_t *w2h, const float *b2,
                     const float *w3, const float *b3, float *y) {
-  __shared__ __align__(16) uint16_t w[kCostH1 * kCostH2];         // [k/4][64][4]
-  __shared__ __align__(16) uint16_t ht[kCostH1 * kInferTile];     // [k/4][128][4]
+  __shared__ __align__(16) uint16_t w[kCostH1 * kCostH2];
   __shared__ float w3b[kCostH2], bb[kCostH2];
-  for (int idx = threadIdx.x; idx < kCostH2 * (kCostH1 / 4); idx += 256) {
-    const int nn = idx / (kCostH1 / 4), k4 = idx % (kCostH1 / 4);
-    *reinterpret_cast<uint2 *>(&w[(k4 * kCostH2 + nn) * 4]) =
-        *reinterpret_cast<const uint2 *>(w2h + (size_t)nn * kCostH1 + k4 * 4);
+  for (int idx = threadIdx.x; idx < kCostH2 * (kCostH1 / 8); idx += 256) {
+    const int nn = idx / (kCostH1 / 8), k8 = idx % (kCostH1 / 8);
+    *reinterpret_cast<uint4 *>(&w[(k8 * kCostH2 + nn) * 8]) =
+        *reinterpret_cast<const uint4 *>(w2h + (size_t)nn * kCostH1 + k8 * 8);
   }
   if (threadIdx.x < kCostH2) {
     w3b[threadIdx.x] = Bf2F(F2Bf(w3[threadIdx.x]));
     bb[threadIdx.x] = b2[threadIdx.x];
   }
+  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int kq = lane >> 5, row_in = lane & 31;
+  const int kq = lane >> 5, r32 = lane & 31;
   const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
   for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-    __syncthreads();
-    const uint32_t s0 = t * kInferTile;
-    for (int idx = threadIdx.x; idx < kInferTile * (kCostH1 / 4); idx += 256) {
-      const int i = idx / (kCostH1 / 4), k4 = idx % (kCostH1 / 4);
-      uint2 v = make_uint2(0, 0);
-      if (s0 + i < n)
-        v = *reinterpret_cast<const uint2 *>(h1 + (size_t)(s0 + i) * kCostH1 + k4 * 4);
-      *reinterpret_cast<uint2 *>(&ht[(k4 * kInferTile + i) * 4]) = v;
-    }
-    __syncthreads();
+    const uint32_t base = t * kInferTile + wave * 32;
+    const uint32_t row = base + r32;
+    const uint16_t *src = h1 + (size_t)(row < n ? row : 0) * kCostH1 + 8 * kq;
     float16_t acc0, acc1;
+#pragma unroll
     for (int r = 0; r < 16; ++r) {
       acc0[r] = 0.f;
       acc1[r] = 0.f;
     }
-    const int mrow = wave * 32 + row_in;
-#pragma unroll 8
-    for (int k0 = 0; k0 < kCostH1; k0 += 8) {
-      const int kb = (k0 >> 2) + kq;
-      const short4v a = Load4(&ht[(kb * kInferTile + mrow) * 4]);
-      const short4v b0 = Load4(&w[(kb * kCostH2 + row_in) * 4]);
-      const short4v b1v = Load4(&w[(kb * kCostH2 + 32 + row_in) * 4]);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1v, acc1, 0, 0, 0);
+#pragma unroll 4
+    for (int k16 = 0; k16 < kCostH1 / 16; ++k16) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(src + 16 * k16);
+      const int kb = 2 * k16 + kq;
+      const bf16x8 b0 = Load8(&w[(kb * kCostH2 + r32) * 8]);
+      const bf16x8 b1v = Load8(&w[(kb * kCostH2 + 32 + r32) * 8]);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1v, acc1, 0, 0, 0);
     }
     // layer 3: per row, sum over the 64 columns (32 lanes x 2 accumulators)
     float part[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int c0 = row_in;
-      part[r] = Bf2F(F2Bf(Gelu(acc0[r] + bb[c0]))) * w3b[c0] +
-                Bf2F(F2Bf(Gelu(acc1[r] + bb[c0 + 32]))) * w3b[c0 + 32];
+      part[r] = Bf2F(F2Bf(Gelu(acc0[r] + bb[r32]))) * w3b[r32] +
+                Bf2F(F2Bf(Gelu(acc1[r] + bb[r32 + 32]))) * w3b[r32 + 32];
 #pragma unroll
       for (int off = 16; off > 0; off >>= 1) part[r] += __shfl_xor(part[r], off, 64);
     }
-    if (row_in == 0) {
+    if (r32 == 0) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const uint32_t seed = s0 + wave * 32 + 8 * (r >> 2) + 4 * kq + (r & 3);
+        const uint32_t seed = base + 8 * (r >> 2) + 4 * kq + (r & 3);
         if (seed < n) y[seed] = part[r] + b3[0];
       }
     }
@@ -521,11 +537,12 @@ hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out
   for (size_t b0 = 0; b0 < n; b0 += chunk) {
     const uint32_t m = (uint32_t)std::min(chunk, n - b0);
     const uint32_t tiles = (m + kInferTile - 1) / kInferTile;
-    const unsigned groups = std::max(1u, std::min<unsigned>(tiles, (unsigned)cus / 4 * 2));
-    hipLaunchKernelGGL(InferLayer1Kernel, dim3(4 * groups), dim3(256), 0, s,
+    const uint32_t l1_tiles = (m + kL1Tile - 1) / kL1Tile;
+    const unsigned groups = std::max(1u, std::min<unsigned>(l1_tiles, (unsigned)cus / 4));
+    hipLaunchKernelGGL(InferLayer1Kernel, dim3(4 * groups), dim3(kL1Waves * 64), 0, s,
                        d_codes + b0 * kCostResidues, m, w1h_, embh_, master_ + kOffB1,
                        infer_h1_);
-    hipLaunchKernelGGL(InferHeadKernel, dim3(std::min<unsigned>(tiles, (unsigned)cus * 2)),
+    hipLaunchKernelGGL(InferHeadKernel, dim3(std::min<unsigned>(tiles, (unsigned)cus * 8)),
                        dim3(256), 0, s, infer_h1_, m, w2h_, master_ + kOffB2, master_ + kOffW3,
                        master_ + kOffB3, d_out + b0);
     const hipError_t e = hipGetLastError();

@@ -128,3 +128,62 @@ def test_device_features_match_host_features(device):
         got = out.to_array(np.uint8, a.size * 128).reshape(-1, 128)
         say("features reverse", reverse, "seeds", a.size)
         assert np.array_equal(got, want)
+
+
+def wave_cost(cells, order, lanes=64):
+    """Lane-steps a wave-per-64-jobs launch spends: each wave runs as long as
+    its most expensive job."""
+    c = cells[order]
+    pad = (-len(c)) % lanes
+    c = np.concatenate([c, np.zeros(pad, c.dtype)]).reshape(-1, lanes)
+    return int(c.max(axis=1).sum()) * lanes
+
+
+def test_trained_network_gain_over_plain_order(device):
+    """What the learned part buys: trained on the plain run's extension traces
+    of one read set, the network predicts the one-sided extension costs of
+    another read set's seeds far better than a constant, and ordering each
+    side's jobs by it packs the waves of a launch with fewer idle lane-steps
+    than the plain order."""
+    import synthetic
+    rng = np.random.default_rng(21)
+    (names, res, offs), (dnames, dres, doffs) = synthetic.translated_workload(22, 20000, 4000)
+    db = gz.Database(dnames, dres, doffs)
+    half = len(names) // 2
+
+    def traced(lo, hi):
+        q = gz.Queries(names[lo:hi], res[int(offs[lo]):int(offs[hi])],
+                       offs[lo:hi + 1] - offs[lo], translated=True)
+        al = gz.Aligner(db, record_traces=1)
+        al.align(q)
+        tr = al.traces()
+        concat = al.last_concatenated_queries()
+        seq = db.chunk(0)["sequence"]
+        feats = np.zeros((len(tr), 128), np.uint8)
+        for rev in (0, 1):
+            sel = np.nonzero(tr[:, 3] == rev)[0]
+            feats[sel] = gz.cost_features(concat, seq, tr[sel, 0], tr[sel, 1], rev)
+        return tr, feats, np.log2(np.maximum(tr[:, 4], 1)).astype(np.float32)
+
+    tr_a, f_a, t_a = traced(0, half)
+    tr_b, f_b, t_b = traced(half, len(names))
+    say("training extensions", len(t_a), "evaluation extensions", len(t_b))
+    assert len(t_a) > 10000 and len(t_b) > 10000
+    mu = float(t_a.mean())
+    mlp = gz.Mlp(gz.Mlp.init_weights(23))
+    for ep in range(12):
+        order = rng.permutation(len(t_a))
+        for b in range(0, len(order) - 255, 256):
+            mlp.train_step(f_a[order[b:b + 256]], t_a[order[b:b + 256]] - mu, 1e-3)
+    pred = mlp.predict(f_b) + mu
+    mae = float(np.abs(pred - t_b).mean())
+    base = float(np.abs(t_b - mu).mean())
+    gains = []
+    for rev in (1, 0):
+        sel = np.nonzero(tr_b[:, 3] == rev)[0]  # this side's jobs in the plain order
+        cells = tr_b[sel, 4].astype(np.int64)
+        learned = np.argsort(-pred[sel], kind="stable")
+        gains.append(wave_cost(cells, np.arange(len(sel))) / wave_cost(cells, learned))
+    say("mae", mae, "constant-predictor mae", base, "wave-cost gain (reverse, forward)", gains)
+    assert mae < 0.5 * base
+    assert min(gains) > 1.1
