@@ -20,6 +20,7 @@
 #include "presearch.h"
 #include "queries.h"
 #include "results.h"
+#include "query_lists_device.h"
 #include "search_device.h"
 
 struct gz_gapped_extender_s;
@@ -42,7 +43,7 @@ class Aligner : public PresearchDevice {
 
   // PresearchDevice
   int BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &concat,
-                      const QueryLists &lists, const std::vector<int> &cutoffs,
+                      const std::vector<uint64_t> &coff, const std::vector<int> &cutoffs,
                       const std::vector<int> &triggers, int presearch_cutoff,
                       AlignStats *st) override;
   int SeedHits(size_t chunk, uint32_t qb, uint32_t qe, SeedHitView *hits,
@@ -65,6 +66,7 @@ class Aligner : public PresearchDevice {
   int *d_matrix_ = nullptr;
   uint8_t *d_reduced_ = nullptr;
   SeedStage seed_;
+  DeviceListBuilder lists_;  // query hash lists of the resident chunk
   gz_gapped_extender_s *gapped_ = nullptr;
   std::vector<GappedTrace> traces_;
   std::vector<uint8_t> concat_;

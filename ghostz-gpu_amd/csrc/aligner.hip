@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include "aligner.h"
@@ -117,10 +118,9 @@ hipError_t Aligner::Align(const QuerySet &queries, std::string *output, AlignSta
 }
 
 int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &concat,
-                             const QueryLists &lists, const std::vector<int> &cut,
+                             const std::vector<uint64_t> &coff, const std::vector<int> &cut,
                              const std::vector<int> &trig, int presearch_cutoff,
                              AlignStats *st) {
-  (void)st;
   EndQueryChunk();
   for (int c : cut)
     if (c > kMaxUngappedCutoff) return (int)hipErrorInvalidValue;  // beyond the stop score
@@ -128,39 +128,28 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
   hipStream_t s = stream_;
   uint64_t *d_q = nullptr;
   int *d_cut = nullptr, *d_trig = nullptr;
-  uint32_t *d_lhash = nullptr, *d_eoff = nullptr, *d_epos = nullptr, *d_eqid = nullptr;
-  uint8_t *d_echeck = nullptr;
   hipError_t e = UploadPacked(&d_q, concat, s);
   query_buffers_.push_back(d_q);
   if (e == hipSuccess) e = Upload(&d_cut, cut, s);
   query_buffers_.push_back(d_cut);
   if (e == hipSuccess) e = Upload(&d_trig, trig, s);
   query_buffers_.push_back(d_trig);
-  if (e == hipSuccess) e = Upload(&d_lhash, lists.hash, s);
-  query_buffers_.push_back(d_lhash);
-  if (e == hipSuccess) e = Upload(&d_eoff, lists.entry_off, s);
-  query_buffers_.push_back(d_eoff);
-  if (e == hipSuccess) e = Upload(&d_epos, lists.entry_pos, s);
-  query_buffers_.push_back(d_epos);
-  if (e == hipSuccess) e = Upload(&d_eqid, lists.entry_qid, s);
-  query_buffers_.push_back(d_eqid);
-  if (e == hipSuccess) e = Upload(&d_echeck, lists.entry_check, s);
-  query_buffers_.push_back(d_echeck);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return (int)e;
-
+  // the hash lists, built on the device from the packed concatenation
   dq_ = DeviceQueryLists();
+  const auto t0 = std::chrono::steady_clock::now();
+  uint64_t entries = 0;
+  e = lists_.Build(d_q, concat.size(), q, coff, db_->hash, db_->subsequence_length, s, &dq_,
+                   &entries);
+  if (e != hipSuccess) return (int)e;
+  st->seconds_query_lists +=
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  st->lists += dq_.nlists;
   dq_.packed = d_q;
   dq_.cutoffs = d_cut;
   dq_.triggers = d_trig;
   dq_.nqueries = (uint32_t)q.size();
-  dq_.nlists = (uint32_t)lists.NumberLists();
   lookup_chunk_ = kNoChunk;
-  dq_.hash = d_lhash;
-  dq_.entry_off = d_eoff;
-  dq_.entry_pos = d_epos;
-  dq_.entry_qid = d_eqid;
-  dq_.entry_check = d_echeck;
   sp_ = SeedStageParams();
   sp_.matrix = d_matrix_;
   sp_.nl = opt_.search.nl;

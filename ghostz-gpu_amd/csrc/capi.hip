@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -17,6 +18,8 @@
 #include "gz_runtime.h"
 #include "gz_scan.h"
 #include "queries.h"
+#include "query_lists.h"
+#include "query_lists_device.h"
 #include "seg.h"
 #include "traceback.h"
 
@@ -140,6 +143,110 @@ gz_status gz_queries_view(gz_queries_t h, gz_queries_view_t *v) {
   v->translated = h->q.translated ? 1 : 0;
   v->dna_lengths = h->q.translated ? h->q.dna_lengths.data() : nullptr;
   return GZ_OK;
+}
+
+// ---- query hash lists ----
+gz_status gz_query_lists_build(gz_queries_t h, gz_database_t db, int on_device,
+                               gz_query_lists *out) {
+  GZ_REQUIRE(h && db && out, "gz_query_lists_build: null argument");
+  memset(out, 0, sizeof(*out));
+  std::vector<uint64_t> coff;
+  std::vector<uint8_t> concat;
+  ConcatenatedOffsets(h->q, &coff, &concat);
+  std::vector<uint32_t> hash, off, pos, qid;
+  std::vector<uint8_t> check;
+  if (!on_device) {
+    const auto t0 = std::chrono::steady_clock::now();
+    QueryLists l;
+    BuildQueryLists(h->q, coff, db->db, 0, &l);
+    out->seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    hash.swap(l.hash);
+    off.swap(l.entry_off);
+    pos.swap(l.entry_pos);
+    qid.swap(l.entry_qid);
+    check.swap(l.entry_check);
+  } else {
+    hipStream_t s = nullptr;
+    uint8_t *d_codes = nullptr;
+    uint64_t *d_packed = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    DeviceListBuilder builder;
+    DeviceQueryLists dl;
+    uint64_t entries = 0;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_codes, std::max<size_t>(concat.size(), 16));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_packed, PackedWords(concat.size()) * 8);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_codes, concat.data(), concat.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+      gz_stream_s wrap{s};
+      if (gz_pack_sequence_async(d_codes, concat.size(), d_packed, &wrap) != GZ_OK)
+        e = hipErrorUnknown;
+    }
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+    if (e == hipSuccess)
+      e = builder.Build(d_packed, concat.size(), h->q, coff, db->db.hash,
+                        db->db.subsequence_length, s, &dl, &entries);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+    if (e == hipSuccess) {
+      hash.resize(dl.nlists);
+      off.resize((size_t)dl.nlists + 1, 0);
+      pos.resize(entries);
+      qid.resize(entries);
+      check.resize(entries);
+      if (dl.nlists) {
+        e = hipMemcpyAsync(hash.data(), dl.hash, hash.size() * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(off.data(), dl.entry_off, off.size() * 4, hipMemcpyDeviceToHost, s);
+      }
+      if (e == hipSuccess && entries) {
+        e = hipMemcpyAsync(pos.data(), dl.entry_pos, entries * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(qid.data(), dl.entry_qid, entries * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(check.data(), dl.entry_check, entries, hipMemcpyDeviceToHost, s);
+      }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    out->seconds = ms / 1e3;
+    for (hipEvent_t x : ev)
+      if (x) (void)hipEventDestroy(x);
+    if (d_codes) (void)hipFree(d_codes);
+    if (d_packed) (void)hipFree(d_packed);
+    if (s) (void)hipStreamDestroy(s);
+    if (e != hipSuccess) {
+      SetError(std::string("gz_query_lists_build: ") + hipGetErrorString(e));
+      return GZ_ERROR_HIP;
+    }
+  }
+  auto give = [](const void *src, size_t bytes) {
+    void *p = malloc(std::max<size_t>(bytes, 1));
+    if (p && bytes) memcpy(p, src, bytes);
+    return p;
+  };
+  out->number_lists = hash.size();
+  out->number_entries = pos.size();
+  out->hash = (uint32_t *)give(hash.data(), hash.size() * 4);
+  out->entry_offsets = (uint32_t *)give(off.data(), off.size() * 4);
+  out->entry_positions = (uint32_t *)give(pos.data(), pos.size() * 4);
+  out->entry_queries = (uint32_t *)give(qid.data(), qid.size() * 4);
+  out->entry_checks = (uint8_t *)give(check.data(), check.size());
+  return GZ_OK;
+}
+
+void gz_query_lists_free(gz_query_lists *l) {
+  if (!l) return;
+  free(l->hash);
+  free(l->entry_offsets);
+  free(l->entry_positions);
+  free(l->entry_queries);
+  free(l->entry_checks);
+  memset(l, 0, sizeof(*l));
 }
 
 // ---- database ----

@@ -74,16 +74,11 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     trig[k] = Statistics::Normalized2Nominal(P.normalized_gapped_trigger, kp);
   }
   double t = Now();
+  const double lists_before = st->seconds_query_lists;
   std::vector<uint64_t> coff;
   ConcatenatedOffsets(q, &coff, &concat);
-  QueryLists lists;
-  BuildQueryLists(q, coff, db, threads, &lists);
-  st->seconds_query_lists += Now() - t;
-  st->lists += lists.NumberLists();
-
-  t = Now();
-  GZ_STEP(dev->BeginQueryChunk(q, concat, lists, cut, trig, presearch_cutoff, st));
-  st->seconds_upload += Now() - t;
+  GZ_STEP(dev->BeginQueryChunk(q, concat, coff, cut, trig, presearch_cutoff, st));
+  st->seconds_upload += Now() - t - (st->seconds_query_lists - lists_before);
 
   std::vector<std::vector<Presearched> > pre(nq);
   std::vector<std::vector<Hit> > chained(nq);

@@ -106,10 +106,13 @@ struct GappedTrace {
 class PresearchDevice {
  public:
   virtual ~PresearchDevice() {}
-  // Makes the query chunk resident: concatenated codes (shared delimiters),
-  // hash lists, per-query ungapped cutoffs / gapped triggers.
+  // Makes the query chunk resident: concatenated codes (shared delimiters;
+  // coff = each query's offset), per-query ungapped cutoffs / gapped
+  // triggers, and the chunk's hash lists, which the device builds itself
+  // (it adds their count to st->lists and the time to
+  // st->seconds_query_lists).
   virtual int BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &concat,
-                              const QueryLists &lists, const std::vector<int> &cutoffs,
+                              const std::vector<uint64_t> &coff, const std::vector<int> &cutoffs,
                               const std::vector<int> &triggers, int presearch_cutoff,
                               AlignStats *st) = 0;
   // Passing seeds of queries [qb, qe) of the chunk against database chunk

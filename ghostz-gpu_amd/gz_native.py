@@ -276,6 +276,12 @@ class AlignStats(Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class QueryListsC(Structure):
+    _fields_ = [("number_lists", c_uint64), ("number_entries", c_uint64), ("hash", c_void_p),
+                ("entry_offsets", c_void_p), ("entry_positions", c_void_p),
+                ("entry_queries", c_void_p), ("entry_checks", c_void_p), ("seconds", c_double)]
+
+
 class GappedTrace(Structure):
     _fields_ = [("query_position", c_uint32), ("database_position", c_uint32),
                 ("chunk", c_uint32), ("reverse", c_uint32), ("cells", c_uint32)]
@@ -289,6 +295,8 @@ _SIGNATURES.update({
     "gz_queries_read_fasta": (c_int, [c_char_p, POINTER(QueryOptions), POINTER(c_void_p)]),
     "gz_queries_destroy": (c_int, [c_void_p]),
     "gz_queries_view": (c_int, [c_void_p, POINTER(QueriesView)]),
+    "gz_query_lists_build": (c_int, [c_void_p, c_void_p, c_int, POINTER(QueryListsC)]),
+    "gz_query_lists_free": (None, [POINTER(QueryListsC)]),
     "gz_database_create": (c_int, [c_void_p, c_char_p, c_void_p, c_uint64,
                                    POINTER(DatabaseOptions), POINTER(c_void_p)]),
     "gz_database_read_fasta": (c_int, [c_char_p, POINTER(DatabaseOptions), POINTER(c_void_p)]),
@@ -564,6 +572,31 @@ class Mlp:
                 self.handle = None
         except Exception:
             pass
+
+
+def query_lists(queries, database, on_device):
+    """Hash lists of the query set taken as one chunk (device or host build):
+    dict of numpy arrays hash, entry_offsets, entry_positions (in the
+    shared-delimiter concatenation), entry_queries, entry_checks, plus the
+    build time in seconds."""
+    ql = QueryListsC()
+    check(lib().gz_query_lists_build(queries.handle, database.handle, int(on_device),
+                                     ctypes.byref(ql)))
+    try:
+        nl, ne = int(ql.number_lists), int(ql.number_entries)
+
+        def take(p, n, ty):
+            return np.frombuffer(ctypes.string_at(p, n * np.dtype(ty).itemsize), ty).copy() \
+                if n else np.zeros(0, ty)
+        out = {"hash": take(ql.hash, nl, np.uint32),
+               "entry_offsets": take(ql.entry_offsets, nl + 1, np.uint32),
+               "entry_positions": take(ql.entry_positions, ne, np.uint32),
+               "entry_queries": take(ql.entry_queries, ne, np.uint32),
+               "entry_checks": take(ql.entry_checks, ne, np.uint8),
+               "seconds": float(ql.seconds)}
+    finally:
+        lib().gz_query_lists_free(ctypes.byref(ql))
+    return out
 
 
 def cost_features(queries, database, qpos, dpos, reverse):

@@ -214,6 +214,7 @@ gz_status gz_queries_read_fasta(const char *path, const gz_query_options *option
 gz_status gz_queries_destroy(gz_queries_t q);
 gz_status gz_queries_view(gz_queries_t q, gz_queries_view_t *view);
 
+
 /* ---- database (Database / DatabaseChunk / SeedSearcherDatabaseParameters,
  *      src/database.h:440-514, src/database_chunk.h:188-210,
  *      src/seed_searcher_database_parameters.cpp:41-319) ---- */
@@ -261,6 +262,28 @@ gz_status gz_database_destroy(gz_database_t db);
 gz_status gz_database_info(gz_database_t db, gz_database_info_t *info);
 gz_status gz_database_chunk(gz_database_t db, uint32_t chunk,
                             gz_database_chunk_view_t *view);
+
+/* Query hash lists of a query set taken as one query chunk, as
+ * SeedSearcherGpuQueryParameters::Build forms them
+ * (src/seed_searcher_gpu_query_parameters.cpp:17-272 over
+ * SeedSearcherQueryParameters::Build, src/seed_searcher_query_parameters.cpp
+ * :17-219): lists in the presearch's visiting order, entries with their
+ * query, position in the shared-delimiter concatenation of the queries and
+ * the distance-check flag.  Built on the current device (on_device != 0) or
+ * on the host; the arrays are malloc'd (gz_query_lists_free). */
+typedef struct {
+  uint64_t number_lists;
+  uint64_t number_entries;
+  uint32_t *hash;          /* number_lists */
+  uint32_t *entry_offsets; /* number_lists + 1 */
+  uint32_t *entry_positions;
+  uint32_t *entry_queries;
+  uint8_t *entry_checks;
+  double seconds;          /* build time (device: HIP-event time of the kernels) */
+} gz_query_lists;
+gz_status gz_query_lists_build(gz_queries_t q, gz_database_t db, int on_device,
+                               gz_query_lists *out);
+void gz_query_lists_free(gz_query_lists *lists);
 
 /* ---- aligner (AlignerGpu, src/aligner_gpu.h:22-62; one device, one
  *      GpuStreamController-style pipeline) ---- */

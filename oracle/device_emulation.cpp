@@ -41,12 +41,12 @@ class CpuDevice : public PresearchDevice {
   CpuDevice(const Database &db, const SearchParameters &p) : db_(db), p_(p) {}
 
   int BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &concat,
-                      const QueryLists &lists, const std::vector<int> &cut,
+                      const std::vector<uint64_t> &coff, const std::vector<int> &cut,
                       const std::vector<int> &trig, int presearch_cutoff,
-                      AlignStats *) override {
-    (void)q;
+                      AlignStats *st) override {
     concat_ = &concat;
-    lists_ = &lists;
+    BuildQueryLists(q, coff, db_, 0, &lists_);
+    st->lists += lists_.NumberLists();
     cut_ = cut;
     trig_ = trig;
     presearch_cutoff_ = presearch_cutoff;
@@ -66,7 +66,7 @@ class CpuDevice : public PresearchDevice {
     const uint32_t L = db_.subsequence_length;
     const float rate = 0.8f;
     const char threshold = L - rate * L;
-    const QueryLists &ql = *lists_;
+    const QueryLists &ql = lists_;
     hits->clear();
     for (uint32_t l = 0; l < ql.NumberLists(); ++l) {
       const uint32_t h = ql.hash[l];
@@ -170,7 +170,7 @@ class CpuDevice : public PresearchDevice {
   const Database &db_;
   const SearchParameters &p_;
   const std::vector<uint8_t> *concat_ = nullptr;
-  const QueryLists *lists_ = nullptr;
+  QueryLists lists_;
   std::vector<int> cut_, trig_;
   int presearch_cutoff_ = 0;
   std::vector<SeedHit> raw_, grouped_[2];

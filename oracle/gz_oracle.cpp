@@ -898,6 +898,45 @@ int gzo_hash(const uint8_t *sequence, const uint8_t *code_map,
 
 void gzo_last_stats(gzo_stats *s) { *s = g_stats; }
 
+int gzo_query_lists(const gzo_queries *Qp, const gzo_database *Dp, uint64_t *number_lists,
+                    uint32_t **hash, uint32_t **entry_offsets, uint64_t *number_entries,
+                    uint32_t **entry_queries, uint32_t **entry_positions,
+                    uint8_t **entry_checks) {
+  std::vector<HashList> lists;
+  BuildHashLists(*Qp, *Dp, 25, lists);
+  // the presearch visits blocks of 256 lists, each from its back
+  // (AlignerPresearchThread::Run, src/aligner_presearch_thread.cpp:140-152)
+  const uint32_t nl = (uint32_t)lists.size();
+  std::vector<uint32_t> visit;
+  for (uint32_t block = 0; block < nl; block += 256) {
+    const uint32_t bend = std::min(nl, block + 256);
+    for (uint32_t li = bend; li-- > block;) visit.push_back(li);
+  }
+  uint64_t ne = 0;
+  for (const HashList &l : lists) ne += l.data.size();
+  *number_lists = nl;
+  *number_entries = ne;
+  *hash = (uint32_t *)malloc(sizeof(uint32_t) * (nl + 1));
+  *entry_offsets = (uint32_t *)malloc(sizeof(uint32_t) * (nl + 1));
+  *entry_queries = (uint32_t *)malloc(sizeof(uint32_t) * (ne + 1));
+  *entry_positions = (uint32_t *)malloc(sizeof(uint32_t) * (ne + 1));
+  *entry_checks = (uint8_t *)malloc(ne + 1);
+  uint64_t o = 0;
+  for (uint32_t r = 0; r < nl; ++r) {
+    const HashList &l = lists[visit[r]];
+    (*hash)[r] = l.hash;
+    (*entry_offsets)[r] = (uint32_t)o;
+    for (const ListEntry &e : l.data) {
+      (*entry_queries)[o] = e.query_id;
+      (*entry_positions)[o] = e.position;
+      (*entry_checks)[o] = e.check ? 1 : 0;
+      ++o;
+    }
+  }
+  (*entry_offsets)[nl] = (uint32_t)o;
+  return 0;
+}
+
 void gzo_free(void *p) { free(p); }
 
 int gzo_align(const gzo_queries *Qp, const gzo_database *Dp,

@@ -154,6 +154,17 @@ int gzo_align(const gzo_queries *queries, const gzo_database *database,
               const gzo_parameters *parameters, char **output,
               size_t *output_length);
 
+/* SeedSearcherQueryParameters::Build with one thread
+ * (src/seed_searcher_query_parameters.cpp:17-219): the hash lists of the
+ * queries taken as one chunk, in the order the presearch visits them
+ * (src/aligner_presearch_thread.cpp:140-152); each entry's query id,
+ * query-local position and check flag.  The five arrays are malloc'd
+ * (gzo_free each).  Returns 0. */
+int gzo_query_lists(const gzo_queries *queries, const gzo_database *database,
+                    uint64_t *number_lists, uint32_t **hash, uint32_t **entry_offsets,
+                    uint64_t *number_entries, uint32_t **entry_queries,
+                    uint32_t **entry_positions, uint8_t **entry_checks);
+
 /* Presearch-only statistics of the last gzo_align call on this thread. */
 typedef struct {
   uint64_t hash_lists;

@@ -58,6 +58,10 @@ def oracle():
                                 POINTER(c_size_t)]
         L.gzo_free.restype = None
         L.gzo_free.argtypes = [c_void_p]
+        L.gzo_query_lists.restype = c_int
+        L.gzo_query_lists.argtypes = [c_void_p, c_void_p, POINTER(c_uint64), POINTER(c_void_p),
+                                      POINTER(c_void_p), POINTER(c_uint64), POINTER(c_void_p),
+                                      POINTER(c_void_p), POINTER(c_void_p)]
         L.gzo_last_stats.restype = None
         L.gzo_last_stats.argtypes = [c_void_p]
         _oracle = L
@@ -195,13 +199,9 @@ class _Stats(Structure):
                                          "stale_traceback_reads")]
 
 
-def align(queries_view, chunks, info, matrix, number_letters, gap_open=11, gap_extension=1,
-          cutoffs=(7.0, 22.0, 15.0, 25.0), max_results=10, max_results_per_subject=1):
-    """Runs the oracle search on already-prepared inputs: queries_view as
-    returned by gz_native.Queries.view(), chunks as gz_native.Database.chunk()
-    dicts and info a gz_native.DatabaseInfo.  Returns (text bytes, stats dict)."""
-    keep = []
-
+def _structs(queries_view, chunks, info, keep):
+    """gzo_queries / gzo_database for prepared inputs (arrays kept alive in
+    `keep`)."""
     def arr(a, dtype):
         a = np.ascontiguousarray(a, dtype)
         keep.append(a)
@@ -213,6 +213,7 @@ def align(queries_view, chunks, info, matrix, number_letters, gap_open=11, gap_e
         return ctypes.cast(a, c_void_p)
 
     cs = (_Chunk * max(len(chunks), 1))()
+    keep.append(cs)
     for i, ch in enumerate(chunks):
         cs[i] = _Chunk(arr(ch["sequence"], np.uint8), len(ch["sequence"]),
                        ch["number_sequences"], arr(ch["offsets"], np.uint32), names(ch["names"]),
@@ -236,6 +237,16 @@ def align(queries_view, chunks, info, matrix, number_letters, gap_open=11, gap_e
                  arr(qv["lambda"], np.float32), arr(qv["K"], np.float32),
                  int(qv["translated"]),
                  arr(qv["dna_lengths"], np.uint32) if qv["translated"] else None)
+    return q, db, arr
+
+
+def align(queries_view, chunks, info, matrix, number_letters, gap_open=11, gap_extension=1,
+          cutoffs=(7.0, 22.0, 15.0, 25.0), max_results=10, max_results_per_subject=1):
+    """Runs the oracle search on already-prepared inputs: queries_view as
+    returned by gz_native.Queries.view(), chunks as gz_native.Database.chunk()
+    dicts and info a gz_native.DatabaseInfo.  Returns (text bytes, stats dict)."""
+    keep = []
+    q, db, arr = _structs(queries_view, chunks, info, keep)
     p = _Parameters(arr(matrix, np.int32), number_letters, 25, gap_open, gap_extension,
                     *cutoffs, max_results, max_results_per_subject)
     out = c_void_p()
@@ -249,6 +260,28 @@ def align(queries_view, chunks, info, matrix, number_letters, gap_open=11, gap_e
     st = _Stats()
     oracle().gzo_last_stats(ctypes.byref(st))
     return text, {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+
+def query_lists(queries_view, chunks, info):
+    """The oracle's hash lists of the queries taken as one chunk, in visiting
+    order: dict of hash, entry_offsets, entry_queries, entry_positions
+    (query-local) and entry_checks arrays."""
+    keep = []
+    q, db, _ = _structs(queries_view, chunks, info, keep)
+    nl, ne = c_uint64(), c_uint64()
+    ptrs = [c_void_p() for _ in range(5)]
+    oracle().gzo_query_lists(ctypes.byref(q), ctypes.byref(db), ctypes.byref(nl),
+                             ctypes.byref(ptrs[0]), ctypes.byref(ptrs[1]), ctypes.byref(ne),
+                             ctypes.byref(ptrs[2]), ctypes.byref(ptrs[3]), ctypes.byref(ptrs[4]))
+    sizes = [nl.value, nl.value + 1, ne.value, ne.value, ne.value]
+    types = [np.uint32, np.uint32, np.uint32, np.uint32, np.uint8]
+    keys = ["hash", "entry_offsets", "entry_queries", "entry_positions", "entry_checks"]
+    out = {}
+    for k, pp, sz, ty in zip(keys, ptrs, sizes, types):
+        buf = ctypes.string_at(pp, sz * np.dtype(ty).itemsize) if sz else b""
+        out[k] = np.frombuffer(buf, ty).copy()
+        oracle().gzo_free(pp)
+    return out
 
 
 # ---- the product's host flow with emulated device stages (libhostpipe.so) ----
