@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--pipeline-blocks", type=int, default=0,
+                    help="query blocks per chunk in the batched forms (0: library default)")
     ap.add_argument("--no-learned", action="store_true",
                     help="plain schedule only (for profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -180,8 +182,9 @@ def main():
     plain = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN)
     forms = [("plain", plain)]
     if not args.no_learned:
-        batched = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN, batched=1)
-        learned = gz.Aligner(db, schedule=gz.SCHEDULE_LEARNED, batched=1)
+        pb = {"pipeline_blocks": args.pipeline_blocks} if args.pipeline_blocks else {}
+        batched = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN, batched=1, **pb)
+        learned = gz.Aligner(db, schedule=gz.SCHEDULE_LEARNED, batched=1, **pb)
         learned.set_cost_model(weights)
         forms += [("batched", batched), ("learned", learned)]
 

@@ -39,16 +39,12 @@ void ParallelBlocks(size_t n, int threads, size_t grain, F f) {
   for (auto &t : pool) t.join();
 }
 
-// the one-thread CPU enumeration order of passing seeds within one query
-struct KeyOrder {
-  const SeedHit *h;
-  bool operator()(uint32_t a, uint32_t b) const {
-    const SeedHit &x = h[a], &y = h[b];
-    if (x.list != y.list) return x.list < y.list;
-    if (x.rv != y.rv) return x.rv < y.rv;
-    if (x.member != y.member) return x.member < y.member;
-    return x.entry < y.entry;
-  }
+// A passing seed with its one-thread CPU enumeration key (list rank,
+// representative or value index, member index + 1 or 0, entry index).
+struct KeyedHit {
+  uint64_t hi, lo;  // (list << 32 | rv), (member << 32 | entry)
+  uint32_t q, d;
+  bool operator<(const KeyedHit &o) const { return hi != o.hi ? hi < o.hi : lo < o.lo; }
 };
 
 #define GZ_STEP(x)            \
@@ -90,20 +86,26 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
                    size_t qe) {
     ParallelBlocks(qe - qb, threads, 64, [&](size_t b, size_t e) {
       std::vector<uint64_t> scratch;
-      std::vector<uint32_t> byq;
-      KeyOrder order{hits.hits};
+      std::vector<KeyedHit> keyed;
       for (size_t qi = qb + b; qi < qb + e; ++qi) {
         const uint32_t s0 = hits.qstart[qi - hb], s1 = hits.qstart[qi - hb + 1];
         if (s0 == s1) continue;
-        byq.resize(s1 - s0);
-        std::iota(byq.begin(), byq.end(), s0);
-        std::sort(byq.begin(), byq.end(), order);
+        // the (list, rv, member, entry) keys are distinct, so sorting them
+        // gives exactly the one-thread enumeration order
+        keyed.resize(s1 - s0);
+        for (uint32_t k = 0; k < s1 - s0; ++k) {
+          const SeedHit &h = hits.hits[s0 + k];
+          keyed[k].hi = ((uint64_t)h.list << 32) | h.rv;
+          keyed[k].lo = ((uint64_t)h.member << 32) | h.entry;
+          keyed[k].q = h.qpos - (uint32_t)coff[qi];
+          keyed[k].d = h.dpos;
+        }
+        std::sort(keyed.begin(), keyed.end());
         std::vector<Hit> &hv = chained[qi];
         hv.resize(s1 - s0);
         for (uint32_t k = 0; k < s1 - s0; ++k) {
-          const SeedHit &h = hits.hits[byq[k]];
-          hv[k].q = h.qpos - (uint32_t)coff[qi];
-          hv[k].d = h.dpos;
+          hv[k].q = keyed[k].q;
+          hv[k].d = keyed[k].d;
         }
         ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl,
                     hv, scratch);
@@ -188,7 +190,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   auto add = [&](size_t ci, const Block &B) {
     const DbChunk &chunk = db.chunks[ci];
     ParallelBlocks(B.qe - B.qb, threads, 64, [&](size_t b, size_t e) {
-      std::vector<std::vector<AlignmentPosition> > starts;
+      AddResultsScratch scratch;
       std::vector<Presearched> tmp;
       for (size_t k = b; k < e; ++k) {
         const size_t qi = B.qb + k;
@@ -209,7 +211,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
           tmp.push_back(r);
         }
         if (!tmp.empty() || !pre[qi].empty())
-          AddResults(chunk, (int)ci, P, starts, tmp, pre[qi]);
+          AddResults(chunk, (int)ci, P, scratch, tmp, pre[qi]);
         std::vector<Hit>().swap(chained[qi]);
       }
     });
@@ -363,12 +365,17 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
                                                              (nq + 63) / 64));
     uint64_t total = 0;
     for (size_t i = 0; i < nq; ++i) total += q.Length(i);
+    // block weights ramp up over the first blocks and down over the last
+    // ones, so that the pipeline fills and drains on small blocks
+    std::vector<double> cum(want + 1, 0.0);
+    for (size_t b = 0; b < want; ++b)
+      cum[b + 1] = cum[b] + (double)std::min<size_t>(std::min(b + 1, want - b), 4);
     size_t qb = 0;
     uint64_t sum = 0;
     for (size_t b = 0; b < want && qb < nq; ++b) {
       size_t qe = nq;
       if (b + 1 < want) {
-        const uint64_t target = total * (b + 1) / want;
+        const uint64_t target = (uint64_t)((double)total * cum[b + 1] / cum[want]);
         qe = qb;
         while (qe < nq) {
           sum += q.Length(qe);

@@ -2,6 +2,7 @@
 #include "results.h"
 
 #include <algorithm>
+#include <charconv>
 #include <atomic>
 #include <climits>
 #include <cstring>
@@ -100,11 +101,41 @@ struct PositionLess {
 };
 }  // namespace
 
+std::vector<AlignmentPosition> &SubjectStarts::Of(uint32_t subject) {
+  if (used_ > 16) {
+    const auto f = index_.find(subject);
+    if (f != index_.end()) return lists_[f->second];
+  } else {
+    for (size_t i = 0; i < used_; ++i)
+      if (subjects_[i] == subject) return lists_[i];
+  }
+  if (used_ == subjects_.size()) {
+    subjects_.push_back(0);
+    lists_.emplace_back();
+  }
+  const size_t slot = used_++;
+  subjects_[slot] = subject;
+  lists_[slot].clear();
+  if (used_ > 16) {
+    if (index_.empty())
+      for (size_t i = 0; i < used_; ++i) index_[subjects_[i]] = (uint32_t)i;
+    else
+      index_[subject] = (uint32_t)slot;
+  }
+  return lists_[slot];
+}
+
+void SubjectStarts::Clear() {
+  used_ = 0;
+  index_.clear();
+}
+
 void AddResults(const DbChunk &chunk, int chunk_id, const SearchParameters &p,
-                std::vector<std::vector<AlignmentPosition> > &starts,
-                std::vector<Presearched> &added, std::vector<Presearched> &results) {
-  std::vector<Presearched> merged;
-  if (starts.size() < chunk.NumberSequences()) starts.resize(chunk.NumberSequences());
+                AddResultsScratch &scratch, std::vector<Presearched> &added,
+                std::vector<Presearched> &results) {
+  std::vector<Presearched> &merged = scratch.merged;
+  merged.clear();
+  scratch.starts.Clear();
   std::stable_sort(added.begin(), added.end(), ByScore());
   size_t ri = 0, ai = 0;
   while (merged.size() < p.max_results && (ai < added.size() || ri < results.size())) {
@@ -121,7 +152,9 @@ void AddResults(const DbChunk &chunk, int chunk_id, const SearchParameters &p,
       a = results[ri++];
     }
     if (a.chunk == chunk_id) {
-      std::vector<AlignmentPosition> &sp = starts[a.subject];
+      // only this chunk's subjects are entered (subject numbers of results
+      // kept from earlier chunks index those chunks)
+      std::vector<AlignmentPosition> &sp = scratch.starts.Of(a.subject);
       if (sp.size() < p.max_results_per_subject) {
         AlignmentPosition key = {0, a.start.d};
         auto it = std::lower_bound(sp.begin(), sp.end(), key, PositionLess());
@@ -147,10 +180,6 @@ void AddResults(const DbChunk &chunk, int chunk_id, const SearchParameters &p,
     }
   }
   results.swap(merged);
-  // only this chunk's subjects were entered in `starts` (subject numbers of
-  // results kept from earlier chunks index those chunks)
-  for (size_t i = 0; i < results.size(); ++i)
-    if (results[i].chunk == chunk_id) starts[results[i].subject].clear();
 }
 
 // ---------------------------------------------------------------------------
@@ -575,7 +604,18 @@ void WriteOutputRange(std::ostream &os, const QuerySet &q, const Database &db,
     for (;;) {
       const size_t b = next.fetch_add(1);
       if (b >= nblocks) break;
-      std::ostringstream s;
+      std::string &s = text[b];
+      char num[64];
+      // the fields as std::ostream writes them: integers in decimal,
+      // floating-point values as printf's %g (6 significant digits)
+      auto put_u = [&](uint64_t v) {
+        const auto r = std::to_chars(num, num + sizeof(num), v);
+        s.append(num, r.ptr);
+      };
+      auto put_g = [&](double v) {
+        const auto r = std::to_chars(num, num + sizeof(num), v, std::chars_format::general, 6);
+        s.append(num, r.ptr);
+      };
       for (size_t qi = qb + b * block; qi < std::min(nq, qb + (b + 1) * block); ++qi) {
         if (results[qi].empty()) continue;
         const uint64_t space = Statistics::SearchSpace(q.RealLength(qi), db.total_length,
@@ -583,19 +623,34 @@ void WriteOutputRange(std::ostream &os, const QuerySet &q, const Database &db,
         for (const AlignmentResult &r : results[qi]) {
           const DbChunk &c = db.chunks[r.chunk];
           const uint32_t base = c.offsets[r.subject] - 1;
-          s << q.names[qi] << "\t" << c.names[r.subject] << "\t";
-          s << (1.0 - (static_cast<float>(r.mismatches + r.gap_openings + r.gap_extensions) /
-                       static_cast<float>(r.length))) *
-                   100
-            << "\t";
-          s << r.length << "\t" << r.mismatches << "\t" << r.gap_openings << "\t";
-          s << q.RealStart(qi, r.start.q) << "\t" << q.RealEnd(qi, r.end.q) << "\t";
-          s << r.start.d - base << "\t" << r.end.d - base << "\t";
-          s << Statistics::Nominal2EValue(r.score, space, gk) << "\t";
-          s << Statistics::Nominal2Normalized(r.score, gk) << '\n';
+          s += q.names[qi];
+          s += '\t';
+          s += c.names[r.subject];
+          s += '\t';
+          put_g((1.0 - (static_cast<float>(r.mismatches + r.gap_openings + r.gap_extensions) /
+                        static_cast<float>(r.length))) *
+                100);
+          s += '\t';
+          put_u(r.length);
+          s += '\t';
+          put_u(r.mismatches);
+          s += '\t';
+          put_u(r.gap_openings);
+          s += '\t';
+          put_u(q.RealStart(qi, r.start.q));
+          s += '\t';
+          put_u(q.RealEnd(qi, r.end.q));
+          s += '\t';
+          put_u((uint32_t)(r.start.d - base));
+          s += '\t';
+          put_u((uint32_t)(r.end.d - base));
+          s += '\t';
+          put_g(Statistics::Nominal2EValue(r.score, space, gk));
+          s += '\t';
+          put_g((double)Statistics::Nominal2Normalized(r.score, gk));
+          s += '\n';
         }
       }
-      text[b] = s.str();
     }
   };
   if (threads <= 0) threads = (int)std::thread::hardware_concurrency();

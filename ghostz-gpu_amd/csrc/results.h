@@ -6,6 +6,7 @@
 
 #include <ostream>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -83,9 +84,30 @@ struct AlignmentPosition {
   uint32_t result_id;
   uint32_t position;
 };
+// The sorted start positions of the kept results per database subject.  The
+// reference keeps a vector per subject of the chunk; one query holds at most
+// max_results results, so a short list of the subjects in use (with a hash
+// index once it grows) holds the same lists without touching a
+// chunk-sized array per query.
+class SubjectStarts {
+ public:
+  std::vector<AlignmentPosition> &Of(uint32_t subject);
+  void Clear();
+
+ private:
+  std::vector<uint32_t> subjects_;
+  std::vector<std::vector<AlignmentPosition> > lists_;
+  size_t used_ = 0;
+  std::unordered_map<uint32_t, uint32_t> index_;  // subject -> slot, when used_ > 16
+};
+// Per-thread scratch of AddResults, reused across calls.
+struct AddResultsScratch {
+  SubjectStarts starts;
+  std::vector<Presearched> merged;
+};
 void AddResults(const DbChunk &chunk, int chunk_id, const SearchParameters &p,
-                std::vector<std::vector<AlignmentPosition> > &starts,
-                std::vector<Presearched> &added, std::vector<Presearched> &results);
+                AddResultsScratch &scratch, std::vector<Presearched> &added,
+                std::vector<Presearched> &results);
 
 // Traceback re-extension of every presearched result of one chunk and the
 // final per-query sort (AlignerBuildResultsThread, aligner_build_results_thread.h
