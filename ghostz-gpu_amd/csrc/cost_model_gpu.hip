@@ -12,6 +12,7 @@
 #include "gz_common.h"
 #include "gz_device.h"
 #include "mlp.h"
+#include "ungapped.h"
 
 namespace gz {
 
@@ -38,55 +39,58 @@ bool UnflattenWeights(const float *f, size_t count, CostModelWeights *w) {
 
 namespace {
 
-template <class Cursor>
-__device__ __forceinline__ void Window(Cursor &c, uint8_t *out) {
+// One feature window: 32 codes from position `start` stepping forward or
+// backward, codes >= 20 as 20, the delimiter and everything after it as 20
+// (CostFeatures, cost_model.cpp).  Twelve codes come per packed word pair;
+// a pair is read only while no delimiter has been met, so reads stay inside
+// the padded array.
+template <bool kBackward>
+__device__ __forceinline__ void FeatureWindow(const uint64_t *packed, uint32_t start,
+                                              uint32_t out[8]) {
   bool stopped = false;
-  for (int k = 0; k < kCostWindow; ++k) {
-    uint32_t x = 20;
-    if (!stopped) {
-      const uint32_t v = c.Next();
-      if (v == kDelimiter) stopped = true;
-      else x = v < 20 ? v : 20;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) out[i] = 0x14141414u;
+#pragma unroll
+  for (int part = 0; part < 3; ++part) {
+    if (stopped) break;
+    // codes start + 12 part + j (forward) or start - 12 part - j (backward)
+    const uint32_t first = kBackward ? start - 12u * part - 11u : start + 12u * part;
+    const uint32_t c = first + kPackPadCodes, w = c / kCodesPerWord;
+    const uint64_t win = Window12(packed[w] & kCodeBitsMask, packed[w + 1] & kCodeBitsMask,
+                                  c - w * kCodesPerWord);
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+      const int k = 12 * part + j;  // index in the window
+      if (k >= kCostWindow) break;
+      const uint32_t code = (uint32_t)(win >> (kCodeBits * (kBackward ? 11 - j : j))) & 31u;
+      stopped = stopped || code == kDelimiter;
+      const uint32_t x = stopped ? 20u : (code < 20u ? code : 20u);
+      out[k >> 2] = (out[k >> 2] & ~(0xffu << (8 * (k & 3)))) | (x << (8 * (k & 3)));
     }
-    out[k] = (uint8_t)x;
   }
 }
 
-// CostFeatures (cost_model.cpp) on packed codes: windows along the extension
-// (query, subject) then away from it (query, subject).
+// CostFeatures (cost_model.cpp) on packed codes, one thread per (seed,
+// window): windows along the extension (query, subject) then away from it
+// (query, subject).
 __global__ void CostFeatureKernel(const uint64_t *pq, const uint64_t *pd, const uint32_t *qpos,
                                   const uint32_t *dpos, uint32_t n, int reverse, uint8_t *out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint8_t w[kCostResidues];
-  const uint32_t q = qpos[i], d = dpos[i];
-  if (reverse) {
-    ReverseCursor a, b;
-    a.Init(pq, q);
-    b.Init(pd, d);
-    Window(a, w);
-    Window(b, w + kCostWindow);
-    ForwardCursor c, e;
-    c.Init(pq, q + 1);
-    e.Init(pd, d + 1);
-    Window(c, w + 2 * kCostWindow);
-    Window(e, w + 3 * kCostWindow);
-  } else {
-    ForwardCursor a, b;
-    a.Init(pq, q);
-    b.Init(pd, d);
-    Window(a, w);
-    Window(b, w + kCostWindow);
-    ReverseCursor c, e;
-    c.Init(pq, q - 1);
-    e.Init(pd, d - 1);
-    Window(c, w + 2 * kCostWindow);
-    Window(e, w + 3 * kCostWindow);
-  }
-  uint4 *dst = reinterpret_cast<uint4 *>(out + (size_t)i * kCostResidues);
-  const uint4 *src = reinterpret_cast<const uint4 *>(w);
-#pragma unroll
-  for (int k = 0; k < kCostResidues / 16; ++k) dst[k] = src[k];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4u * n) return;
+  const uint32_t i = t >> 2, wi = t & 3u;
+  const uint64_t *seq = (wi & 1u) ? pd : pq;
+  const uint32_t pos = (wi & 1u) ? dpos[i] : qpos[i];
+  // along: from pos in the extension's direction; away: from the next
+  // position the other way
+  const bool along = wi < 2u;
+  const bool backward = along == (reverse != 0);
+  const uint32_t start = along ? pos : (reverse ? pos + 1 : pos - 1);
+  uint32_t codes[8];
+  if (backward) FeatureWindow<true>(seq, start, codes);
+  else FeatureWindow<false>(seq, start, codes);
+  uint4 *dst = reinterpret_cast<uint4 *>(out + (size_t)i * kCostResidues + wi * kCostWindow);
+  dst[0] = make_uint4(codes[0], codes[1], codes[2], codes[3]);
+  dst[1] = make_uint4(codes[4], codes[5], codes[6], codes[7]);
 }
 
 }  // namespace
@@ -95,7 +99,7 @@ hipError_t CostFeaturesOnDevice(const uint64_t *pq, const uint64_t *pd, const ui
                                 const uint32_t *dpos, uint32_t n, bool reverse, uint8_t *out,
                                 hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(CostFeatureKernel, dim3((n + 255) / 256), dim3(256), 0, s, pq, pd, qpos,
+  hipLaunchKernelGGL(CostFeatureKernel, dim3((4 * (size_t)n + 255) / 256), dim3(256), 0, s, pq, pd, qpos,
                      dpos, n, reverse ? 1 : 0, out);
   return hipGetLastError();
 }

@@ -310,11 +310,15 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
 
 constexpr int kGappedBlock = 64;
 
+// Jobs [begin, end), or the `*list_count` jobs of `list` when list is set;
+// a job whose band outgrows the W-column ring is appended to the overflow
+// list for the next, wider ring.
 template <bool kReverse, int W>
 __global__ void __launch_bounds__(kGappedBlock)
-    GappedKernel(size_t size, GappedParams P, const int *matrix, uint32_t nl,
-                 uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
-                 uint32_t *overflow_count, uint32_t *overflow_list) {
+    GappedKernel(size_t begin, size_t end, const uint32_t *list, const uint32_t *list_count,
+                 GappedParams P, const int *matrix, uint32_t nl, uint32_t *qpos,
+                 uint32_t *dpos, int *scores, uint32_t *cells, uint32_t *overflow_count,
+                 uint32_t *overflow_list) {
   __shared__ int8_t sm[kTableDim * kTableDim];
   __shared__ int2 ring_cells[W * kWave];
   __shared__ uint8_t ring_codes[W * kWave];
@@ -325,8 +329,10 @@ __global__ void __launch_bounds__(kGappedBlock)
   R.codes = ring_codes;
   R.mask = W - 1;
   R.lane = threadIdx.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < size;
-       i += (size_t)gridDim.x * blockDim.x) {
+  const size_t n = list ? (size_t)*list_count : end - begin;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = list ? (size_t)list[k] : begin + k;
     const uint32_t q = qpos[i], d = dpos[i];
     int s, p0, p1;
     uint32_t nc;
@@ -677,6 +683,8 @@ gz_status gz_ungapped_extender_extend_with_trigger_async(
 }
 
 // ---- GappedExtenderGpu ----
+// LDS rings: most bands fit the narrow one (more waves per CU); the rest
+// move to the 64-column ring, then to the global-memory ring
 constexpr int kGappedRing = 64;
 constexpr uint32_t kWideSlots = 256;
 
@@ -688,9 +696,13 @@ struct gz_gapped_extender_s {
   uint32_t number_letters = 0;
   int gap_open = 0, gap_ext = 0, cutoff = 0;
   uint32_t wide = 1u << 16;  // wide-ring columns (>= longest query + 2)
+  uint32_t narrow = 32;      // columns of the first LDS ring (16 or 32)
   // workspace
-  uint32_t *counters = nullptr;        // [0] overflow count, [1] failures
-  uint32_t *overflow_list = nullptr;
+  // [0] jobs out of the narrow ring, [1] failures, [2] jobs out of the
+  // 64-column ring
+  uint32_t *counters = nullptr;
+  uint32_t *overflow_list = nullptr;   // out of the narrow ring
+  uint32_t *overflow_list2 = nullptr;  // out of the 64-column ring
   size_t overflow_capacity = 0;
   int2 *wide_cells = nullptr;
   uint8_t *wide_codes = nullptr;
@@ -698,6 +710,7 @@ struct gz_gapped_extender_s {
   ~gz_gapped_extender_s() {
     if (counters) (void)hipFree(counters);
     if (overflow_list) (void)hipFree(overflow_list);
+    if (overflow_list2) (void)hipFree(overflow_list2);
     if (wide_cells) (void)hipFree(wide_cells);
     if (wide_codes) (void)hipFree(wide_codes);
   }
@@ -758,19 +771,62 @@ gz_status gz_gapped_extender_convert_to_gapped_extension_seeds_async(
   return GZ_OK;
 }
 
+extern "C++" {
+template <bool kReverse>
+static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t size,
+                         size_t wide_first, uint32_t *d_q, uint32_t *d_d, int *d_scores,
+                         uint32_t *d_cells, hipStream_t s) {
+  uint32_t *narrow_out = h->counters, *failures = h->counters + 1, *wide_out = h->counters + 2;
+  // jobs [0, wide_first) start on the 64-column ring, the rest on the
+  // narrow one; the 64-column ring then takes the narrow ring's overflow
+  if (wide_first > 0)
+    hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing>),
+                       dim3(GridFor(wide_first, kGappedBlock, 256u * 32)), dim3(kGappedBlock),
+                       0, s, (size_t)0, wide_first, (const uint32_t *)nullptr,
+                       (const uint32_t *)nullptr, P, h->matrix, h->number_letters, d_q, d_d,
+                       d_scores, d_cells, wide_out, h->overflow_list2);
+  if (size > wide_first) {
+    const dim3 grid(GridFor(size - wide_first, kGappedBlock, 256u * 64));
+    if (h->narrow == 16)
+      hipLaunchKernelGGL((GappedKernel<kReverse, 16>), grid, dim3(kGappedBlock), 0, s,
+                         wide_first, size, (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                         P, h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells,
+                         narrow_out, h->overflow_list);
+    else
+      hipLaunchKernelGGL((GappedKernel<kReverse, 32>), grid, dim3(kGappedBlock), 0, s,
+                         wide_first, size, (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                         P, h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells,
+                         narrow_out, h->overflow_list);
+  }
+  hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing>), dim3(256u * 4), dim3(kGappedBlock),
+                     0, s, (size_t)0, (size_t)0, (const uint32_t *)h->overflow_list,
+                     (const uint32_t *)narrow_out, P, h->matrix, h->number_letters, d_q, d_d,
+                     d_scores, d_cells, wide_out, h->overflow_list2);
+  hipLaunchKernelGGL((GappedWideKernel<kReverse>), dim3(kWideSlots / kGappedBlock),
+                     dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q, d_d,
+                     d_scores, d_cells, wide_out, h->overflow_list2, h->wide_cells,
+                     h->wide_codes, h->wide, failures);
+}
+}  // extern "C++"
+
 static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
                            uint32_t *d_q, uint32_t *d_d, int *d_scores,
-                           uint32_t *d_cells, hipStream_t s) {
+                           uint32_t *d_cells, hipStream_t s, size_t wide_first = 0) {
   GZ_REQUIRE(h->queries && h->database && h->matrix,
              "gapped extender: queries, database and score parameters must be set");
+  GZ_REQUIRE(wide_first <= size, "gapped extender: wide_first beyond the jobs");
   if (!h->counters) {
-    GZ_HIP_CHECK(hipMalloc(&h->counters, 2 * sizeof(uint32_t)));
-    GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, 2 * sizeof(uint32_t), s));
+    GZ_HIP_CHECK(hipMalloc(&h->counters, 3 * sizeof(uint32_t)));
+    GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, 3 * sizeof(uint32_t), s));
   }
   if (h->overflow_capacity < size) {
     if (h->overflow_list) GZ_HIP_CHECK(hipFree(h->overflow_list));
+    if (h->overflow_list2) GZ_HIP_CHECK(hipFree(h->overflow_list2));
+    h->overflow_list = h->overflow_list2 = nullptr;
+    h->overflow_capacity = 0;
     size_t cap = std::max<size_t>(size, 1 << 16);
     GZ_HIP_CHECK(hipMalloc(&h->overflow_list, cap * sizeof(uint32_t)));
+    GZ_HIP_CHECK(hipMalloc(&h->overflow_list2, cap * sizeof(uint32_t)));
     h->overflow_capacity = cap;
   }
   if (h->wide_allocated < h->wide) {
@@ -780,9 +836,10 @@ static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
     GZ_HIP_CHECK(hipMalloc(&h->wide_codes, (size_t)kWideSlots * h->wide));
     h->wide_allocated = h->wide;
   }
-  // counters[0]: overflow list length of this run; counters[1]: failures,
-  // kept until gz_gapped_extender_failures reads them
+  // the overflow counts of this run start at zero; failures (counters[1])
+  // are kept until gz_gapped_extender_failures reads them
   GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, sizeof(uint32_t), s));
+  GZ_HIP_CHECK(hipMemsetAsync(h->counters + 2, 0, sizeof(uint32_t), s));
   GappedParams P;
   P.queries = h->queries;
   P.database = h->database;
@@ -790,26 +847,8 @@ static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
   P.gap_open = h->gap_open;
   P.gap_ext = h->gap_ext;
   P.cutoff = h->cutoff;
-  const unsigned grid = GridFor(size, kGappedBlock, 256u * 32);
-  if (reverse) {
-    hipLaunchKernelGGL((GappedKernel<true, kGappedRing>), dim3(grid), dim3(kGappedBlock), 0, s,
-                       size, P, h->matrix, h->number_letters, d_q, d_d, d_scores,
-                       d_cells, h->counters, h->overflow_list);
-    GZ_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL((GappedWideKernel<true>), dim3(kWideSlots / kGappedBlock),
-                       dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q,
-                       d_d, d_scores, d_cells, h->counters, h->overflow_list,
-                       h->wide_cells, h->wide_codes, h->wide, h->counters + 1);
-  } else {
-    hipLaunchKernelGGL((GappedKernel<false, kGappedRing>), dim3(grid), dim3(kGappedBlock), 0, s,
-                       size, P, h->matrix, h->number_letters, d_q, d_d, d_scores,
-                       d_cells, h->counters, h->overflow_list);
-    GZ_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL((GappedWideKernel<false>), dim3(kWideSlots / kGappedBlock),
-                       dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q,
-                       d_d, d_scores, d_cells, h->counters, h->overflow_list,
-                       h->wide_cells, h->wide_codes, h->wide, h->counters + 1);
-  }
+  if (reverse) GappedLaunch<true>(h, P, size, wide_first, d_q, d_d, d_scores, d_cells, s);
+  else GappedLaunch<false>(h, P, size, wide_first, d_q, d_d, d_scores, d_cells, s);
   GZ_HIP_CHECK(hipGetLastError());
   return GZ_OK;
 }
@@ -839,6 +878,14 @@ gz_status gz_gapped_extender_extend_one_side_score_only_cells_async(
   return GappedRun(h, size, reverse, d_q, d_d, d_scores, d_cells, S(stream));
 }
 
+gz_status gz_gapped_extender_extend_one_side_score_only_binned_async(
+    gz_gapped_extender_t h, size_t size, size_t wide_first, int reverse, uint32_t *d_q,
+    uint32_t *d_d, int *d_scores, uint32_t *d_cells, gz_stream_t stream) {
+  GZ_REQUIRE(h, "null handle");
+  if (size == 0) return GZ_OK;
+  return GappedRun(h, size, reverse, d_q, d_d, d_scores, d_cells, S(stream), wide_first);
+}
+
 gz_status gz_gapped_extender_set_max_sequence_length(gz_gapped_extender_t h,
                                                      uint32_t length) {
   GZ_REQUIRE(h, "null handle");
@@ -848,11 +895,31 @@ gz_status gz_gapped_extender_set_max_sequence_length(gz_gapped_extender_t h,
   return GZ_OK;
 }
 
+gz_status gz_gapped_extender_set_narrow_ring(gz_gapped_extender_t h, uint32_t columns) {
+  GZ_REQUIRE(h, "null handle");
+  GZ_REQUIRE(columns == 16 || columns == 32, "narrow ring: 16 or 32 columns");
+  h->narrow = columns;
+  return GZ_OK;
+}
+
+gz_status gz_gapped_extender_overflows(gz_gapped_extender_t h, uint32_t *narrow,
+                                       uint32_t *ring64) {
+  GZ_REQUIRE(h && narrow && ring64, "null argument");
+  *narrow = *ring64 = 0;
+  if (h->counters) {
+    uint32_t c[3];
+    GZ_HIP_CHECK(hipMemcpy(c, h->counters, sizeof(c), hipMemcpyDeviceToHost));
+    *narrow = c[0];
+    *ring64 = c[2];
+  }
+  return GZ_OK;
+}
+
 gz_status gz_gapped_extender_failures(gz_gapped_extender_t h, uint32_t *failures) {
   GZ_REQUIRE(h && failures, "null argument");
   *failures = 0;
   if (h->counters) {
-    uint32_t c[2];
+    uint32_t c[3];
     GZ_HIP_CHECK(hipMemcpy(c, h->counters, sizeof(c), hipMemcpyDeviceToHost));
     *failures = c[1];
     if (c[1] != 0) GZ_HIP_CHECK(hipMemset(h->counters + 1, 0, sizeof(uint32_t)));

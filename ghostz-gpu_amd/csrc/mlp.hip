@@ -243,14 +243,14 @@ __device__ __forceinline__ bf16x8 Load8(const uint16_t *p) {
   return *reinterpret_cast<const bf16x8 *>(p);
 }
 
-// Layer 1 on v_mfma_f32_32x32x16_bf16.  A block of 8 waves holds one 64-column
+// Layer 1 on v_mfma_f32_32x32x16_bf16.  A block of 16 waves holds one 64-column
 // slice of W1 as [k/8][64][8] (128 KB, the B operand of a lane is one 16-byte
-// read) and streams tiles of 512 seeds through it; each wave owns 64 seeds x
+// read) and streams tiles of 1024 seeds through it; each wave owns 64 seeds x
 // 64 columns (four 32x32 accumulators).  Operand layout: lane l holds row
 // (A) or column (B) l % 32 and k = 8 * (l / 32) + j, j < 8, so one K step of
 // 16 covers residues 2i (lanes 0-31) and 2i + 1 (lanes 32-63), each lane's A
 // operand being the 8-dim embedding of its residue.
-constexpr int kL1Waves = 8;
+constexpr int kL1Waves = 16;
 constexpr int kL1Tile = kL1Waves * 64;
 
 __global__ void __launch_bounds__(kL1Waves * 64)

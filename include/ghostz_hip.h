@@ -176,6 +176,21 @@ gz_status gz_gapped_extender_extend_one_side_score_only_cells_async(
     gz_gapped_extender_t h, size_t size, int reverse,
     uint32_t *d_query_concatenated_positions, uint32_t *d_database_positions,
     int *d_scores, uint32_t *d_cells, gz_stream_t stream);
+/* Same, with the first wide_first seeds (predicted long) started on the
+ * 64-column ring instead of the narrow one; no reference counterpart (the
+ * binned batches of the learned schedule).  Results do not depend on
+ * wide_first. */
+/* Columns of the first LDS ring (16 or 32; default 32): bands that outgrow
+ * it move to a 64-column ring, then to a global-memory ring.  No reference
+ * counterpart (the reference's single ring caps sequences at 128). */
+gz_status gz_gapped_extender_set_narrow_ring(gz_gapped_extender_t h, uint32_t columns);
+/* Seeds of the last run that outgrew the narrow ring / the 64-column ring. */
+gz_status gz_gapped_extender_overflows(gz_gapped_extender_t h, uint32_t *narrow,
+                                       uint32_t *ring64);
+gz_status gz_gapped_extender_extend_one_side_score_only_binned_async(
+    gz_gapped_extender_t h, size_t size, size_t wide_first, int reverse,
+    uint32_t *d_query_concatenated_positions, uint32_t *d_database_positions,
+    int *d_scores, uint32_t *d_cells, gz_stream_t stream);
 /* Longest query the extender must handle (sizes the wide-band fallback
  * ring; default 65534 residues).  No counterpart in the reference, whose
  * GPU kernel caps the query side at 128 residues
