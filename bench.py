@@ -42,6 +42,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# HBM bytes per work unit of the seed kernels at this workload, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and calibrated against a copy kernel
+TRAFFIC_PROFILE = "profiles/hbm_traffic_translated_reads_200k_vs_db_1m.txt"
+
+
+def traffic_per_unit(kernel):
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+    try:
+        for line in open(path):
+            f = line.split()
+            if f and f[0] == "gz::" + kernel and len(f) >= 8:
+                return float(f[7])
+    except OSError:
+        pass
+    return None
+
+
 def query_residues(view):
     c = view["codes"]
     return int(np.count_nonzero(c != 25))
@@ -218,11 +235,18 @@ def main():
     plain_value = q_aa * args.steps / p_time
     form_values = {name: q_aa * args.steps / sum(r[2]) for name, r in results.items()}
 
-    # roofline of the dominant kernel, from the HIP-event times of the timed steps
-    st = l_st[-1]
-    kern = {"seed_rep": st["seed_rep_kernel_ms"], "seed_value": st["seed_value_kernel_ms"],
-            "gapped": st["gapped_kernel_ms"]}
-    side_kernels = {"schedule": st["schedule_kernel_ms"], "traceback": st["traceback_kernel_ms"]}
+    # kernel times of the learned form (its two streams overlap, so these
+    # include sharing the GPU) and of the plain form (one kernel at a time)
+    def kernel_ms(st):
+        return {"seed_rep": st["seed_rep_kernel_ms"], "seed_value": st["seed_value_kernel_ms"],
+                "gapped": st["gapped_kernel_ms"], "schedule": st["schedule_kernel_ms"],
+                "traceback": st["traceback_kernel_ms"]}
+    learned_kernels = kernel_ms(l_st[-1])
+    plain_kernels = kernel_ms(p_st[-1])
+    # roofline of the dominant kernel, from the HIP-event times of the plain
+    # form's timed steps, where each kernel has the GPU to itself
+    st = p_st[-1]
+    kern = {k: plain_kernels[k] for k in ("seed_rep", "seed_value", "gapped")}
     dom = max(kern, key=kern.get)
     if dom == "gapped":
         # one DP cell = 10 int32 VALU ops (3 max, 2 subtract, 1 add, 3 compares / selects,
@@ -247,10 +271,15 @@ def main():
         # + 2 x 8 (two packed database words around the seed)
         per_unit = 24.0
         achieved = per_unit * units / launches / (avg_ms / 1e3) / 1e9
+        name = "SeedKernel<%s>" % ("false" if k == 0 else "true")
+        measured = traffic_per_unit(name) if args.scale == 1.0 else None
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "SeedKernel<%s>" % ("false" if k == 0 else "true"),
-                "avg_launch_ms": avg_ms}
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured * units / launches if measured else None,
+                "kernel": name, "avg_launch_ms": avg_ms, "units_per_launch": units / launches,
+                "algorithmic_bytes_per_unit": per_unit,
+                "traffic_bytes_per_unit": measured,
+                "traffic_source": TRAFFIC_PROFILE if measured else None}
 
     cpu = None
     if not args.no_cpu:
@@ -270,7 +299,8 @@ def main():
         "plain_value": plain_value, "plain_ms_per_step": 1e3 * p_time / args.steps,
         "form_values": form_values,
         "learned_over_plain": value / plain_value, "outputs_identical": identical,
-        "kernel_ms_per_step": dict(kern, **side_kernels),
+        "kernel_ms_per_step": learned_kernels,
+        "plain_kernel_ms_per_step": plain_kernels,
         "search_stats": {k: st[k] for k in ("lists", "units_rep", "units_value", "seed_hits",
                                              "chained_hits", "gapped_extensions", "results",
                                              "sequential_build_chunks")},

@@ -170,7 +170,7 @@ def test_set_query_seeds_expansion(dev):
 
 
 def _gapped_gpu(dev, q, d, qpos, dpos, reverse, cutoff, gap_open=11, gap_ext=1, matrix=None,
-                nl=None, delim=DELIM):
+                nl=None, delim=DELIM, narrow=None, wide_first=None):
     L = gz.lib()
     n = qpos.size
     _, qp = gz.pack_on_device(q)
@@ -182,14 +182,24 @@ def _gapped_gpu(dev, q, d, qpos, dpos, reverse, cutoff, gap_open=11, gap_ext=1, 
     gz.check(L.gz_gapped_extender_set_database(h, dp.ptr))
     gz.check(L.gz_gapped_extender_set_score_parameters(h, d_m.ptr, nl or dev["nl"], gap_open,
                                                        gap_ext, cutoff))
+    if narrow is not None:
+        gz.check(L.gz_gapped_extender_set_narrow_ring(h, narrow))
     hq, hd = qpos.copy(), dpos.copy()
     hs = np.zeros(n, np.int32)
     d_q, d_d, d_s = gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n * 4)
     s = gz.Stream()
-    gz.check(L.gz_gapped_extender_extend_one_side_score_only_async(
-        h, n, int(reverse), gz.ptr(hq), gz.ptr(hd), gz.ptr(hs), d_q.ptr, d_d.ptr, d_s.ptr,
-        s.handle))
-    s.sync()
+    if wide_first is None:
+        gz.check(L.gz_gapped_extender_extend_one_side_score_only_async(
+            h, n, int(reverse), gz.ptr(hq), gz.ptr(hd), gz.ptr(hs), d_q.ptr, d_d.ptr, d_s.ptr,
+            s.handle))
+        s.sync()
+    else:
+        d_q, d_d = gz.DeviceBuffer.from_array(hq), gz.DeviceBuffer.from_array(hd)
+        gz.check(L.gz_gapped_extender_extend_one_side_score_only_binned_async(
+            h, n, wide_first, int(reverse), d_q.ptr, d_d.ptr, d_s.ptr, None, s.handle))
+        s.sync()
+        hq, hd, hs = (d_q.to_array(np.uint32, n), d_d.to_array(np.uint32, n),
+                      d_s.to_array(np.int32, n))
     L.gz_gapped_extender_destroy(h)
     return hs, hq, hd
 
@@ -219,6 +229,27 @@ def test_gapped_kernel_vs_oracle(dev, reverse, cutoff):
     got = np.stack([hs, hq.astype(np.int64), hd.astype(np.int64)], 1)
     bad = np.nonzero((got != want).any(1))[0]
     assert bad.size == 0, (bad[:5], got[bad[:5]], want[bad[:5]])
+
+
+@pytest.mark.parametrize("narrow,wide_first", [(16, None), (32, None), (32, 500)])
+def test_gapped_kernel_ring_tiers(dev, narrow, wide_first):
+    """The first LDS ring (16 or 32 columns, or the 64-column ring for the
+    first wide_first seeds) only decides where a band runs: every tier gives
+    the oracle's answers.  Long homologous pairs make many bands outgrow the
+    narrow ring."""
+    rng = np.random.default_rng(30 + narrow)
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 1500, min_len=40, max_len=900,
+                                                identity=(0.4, 0.98), indel=0.05)
+    say("ring tiers: narrow", narrow, "wide_first", wide_first, "seeds", qpos.size)
+    L = gz.lib()
+    for reverse in (False, True):
+        a, b = (qpos - 1, dpos - 1) if reverse else (qpos, dpos)
+        hs, hq, hd = _gapped_gpu(dev, q, d, a, b, reverse, 52, narrow=narrow,
+                                 wide_first=wide_first)
+        want = _gapped_oracle(q, d, a, b, reverse, 52, dev["matrix"], dev["nl"])
+        got = np.stack([hs, hq.astype(np.int64), hd.astype(np.int64)], 1)
+        bad = np.nonzero((got != want).any(1))[0]
+        assert bad.size == 0, (bad[:5], got[bad[:5]], want[bad[:5]])
 
 
 def test_gapped_kernel_wide_band(dev):
