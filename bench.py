@@ -52,8 +52,8 @@ def traffic_per_unit(kernel):
     try:
         for line in open(path):
             f = line.split()
-            if f and f[0] == "gz::" + kernel and len(f) >= 8:
-                return float(f[7])
+            if f and f[0] == "gz::" + kernel and len(f) >= 7:
+                return float(f[6])
     except OSError:
         pass
     return None

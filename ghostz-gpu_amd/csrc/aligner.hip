@@ -149,6 +149,9 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
   dq_.cutoffs = d_cut;
   dq_.triggers = d_trig;
   dq_.nqueries = (uint32_t)q.size();
+  e = seed_.PrepareEntries(&dq_, entries, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return (int)e;
   lookup_chunk_ = kNoChunk;
   sp_ = SeedStageParams();
   sp_.matrix = d_matrix_;

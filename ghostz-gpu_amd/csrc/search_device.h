@@ -42,6 +42,7 @@ struct DeviceQueryLists {
   const uint32_t *hash = nullptr, *entry_off = nullptr, *entry_pos = nullptr,
                  *entry_qid = nullptr;
   const uint8_t *entry_check = nullptr;
+  const uint4 *entry_rec = nullptr;  // per entry: position, query | check << 31, cutoff, trigger
 };
 
 struct SeedStageParams {
@@ -69,6 +70,9 @@ struct SeedStageStats {
 class SeedStage {
  public:
   ~SeedStage();
+  // Builds the per-entry records the seed kernels read (one 16-byte load
+  // per unit) and sets q->entry_rec; once per query chunk.
+  hipError_t PrepareEntries(DeviceQueryLists *q, uint64_t entries, hipStream_t stream);
   // Finds the database representatives and values of every list of a query
   // chunk in one database chunk; Run uses them until the next Lookup.
   hipError_t Lookup(const DeviceQueryLists &q, const DeviceDbChunk &db, bool clustering,
@@ -94,6 +98,7 @@ class SeedStage {
   void *tile_range_ = nullptr; size_t tile_range_cap_ = 0;  // per-tile list ranges
   void *qcount_ = nullptr; size_t qcount_cap_ = 0;     // per-query counts / cursors
   void *base_info_ = nullptr; size_t base_info_cap_ = 0;  // per-list database ranges
+  void *entry_rec_ = nullptr; size_t entry_rec_cap_ = 0;  // per-entry records
   bool looked_up_ = false;
   int flip_ = 0;  // host buffer of the next Run
   void *host_hits_[2] = {nullptr, nullptr};  // pinned

@@ -188,55 +188,87 @@ __global__ void __launch_bounds__(kSeedThreads)
       }
     }
     __syncthreads();
-#pragma unroll 1
+    // all items' indices and first loads are issued before any is evaluated:
+    // the entry record and database position, then the packed words around
+    // both seed positions
+    bool valid[kSeedItems];
+    uint32_t lidx[kSeedItems], ridx[kSeedItems], eidx[kSeedItems], dposv[kSeedItems];
+    uint4 rec[kSeedItems];
+#pragma unroll
     for (int j = 0; j < kSeedItems; ++j) {
       const uint64_t u = tile + (uint64_t)j * kSeedThreads + threadIdx.x;
-      const bool valid = u < tile_end;
+      valid[j] = u < tile_end;
+      uint32_t lo = 0, hi = count;  // largest i < count with loff[i] <= u
+      while (lo + 1 < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (loff[mid] <= u) lo = mid; else hi = mid;
+      }
+      const uint64_t local = valid[j] ? u - loff[lo] : 0;
+      const uint32_t P = lp[lo] ? lp[lo] : 1u;
+      uint32_t idx, e;
+      if (local < 0xffffffffull) {
+        idx = (uint32_t)local / P;
+        e = (uint32_t)local - idx * P;
+      } else {
+        idx = (uint32_t)(local / P);
+        e = (uint32_t)(local - (uint64_t)idx * P);
+      }
+      lidx[j] = lo;
+      ridx[j] = idx;
+      eidx[j] = e;
+      rec[j] = make_uint4(0, 0, 0, 0);
+      dposv[j] = 0;
+      if (valid[j]) {
+        rec[j] = q.entry_rec[leb[lo] + e];
+        const uint32_t ri = lbeg[lo] + idx;
+        dposv[j] = kValues ? db.val_pos[ri] : db.rep_pos[ri];
+      }
+    }
+    SeedWindow wq[kSeedItems], wd[kSeedItems];
+#pragma unroll
+    for (int j = 0; j < kSeedItems; ++j) {
+      if (valid[j]) {
+        wq[j] = LoadSeedWindow(q.packed, rec[j].x);
+        wd[j] = LoadSeedWindow(db.packed, dposv[j]);
+      }
+    }
+    // items are evaluated in turn from slot 0; the next item's data moves
+    // down afterwards, so every index is static and stays in registers
+#pragma unroll 1
+    for (int j = 0; j < kSeedItems; ++j) {
       SeedHit h = {0, 0, 0, 0, 0, 0, 0};
       bool pass = false;
       bool member_ok = false;
       uint32_t mb = 0, me = 0;
       int trig = 0, cut = 0;
-      if (valid) {
-        uint32_t lo = 0, hi = count;  // largest i < count with loff[i] <= u
-        while (lo + 1 < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (loff[mid] <= u) lo = mid; else hi = mid;
-        }
-        const uint64_t local = u - loff[lo];
-        const uint32_t P = lp[lo];
-        uint32_t idx, e;
-        if (local < 0xffffffffull) {
-          idx = (uint32_t)local / P;
-          e = (uint32_t)local - idx * P;
-        } else {
-          idx = (uint32_t)(local / P);
-          e = (uint32_t)(local - (uint64_t)idx * P);
-        }
-        const uint32_t ent = leb[lo] + e;
-        const uint32_t qpos = q.entry_pos[ent];
-        const uint32_t qid = q.entry_qid[ent];
-        const uint32_t ri = lbeg[lo] + idx;
-        h.dpos = kValues ? db.val_pos[ri] : db.rep_pos[ri];
-        cut = q.cutoffs[qid];
-        trig = q.triggers[qid];
+      if (valid[0]) {
+        const uint32_t lo = lidx[0], idx = ridx[0];
+        const uint32_t qpos = rec[0].x;
+        const uint32_t qid = rec[0].y & 0x7fffffffu;
+        cut = (int)rec[0].z;
+        trig = (int)rec[0].w;
+        h.dpos = dposv[0];
         h.qid = qid;
         h.qpos = qpos;
         h.list = lrank[lo];
-        h.entry = e;
+        h.entry = eidx[0];
         if (kValues) {
           h.rv = lnrep[lo] + idx;
-          pass = UngappedWithTrigger(q.packed, qpos, db.packed, h.dpos, utab, cut, trig) > trig;
+          pass = UngappedWithTrigger(q.packed, wq[0], db.packed, wd[0], utab, cut, trig) > trig;
         } else {
           h.rv = idx;
-          const bool check = q.entry_check[ent] != 0;
+          const bool check = (rec[0].y >> 31) != 0;
           int d = 0;
           if (check)
-            d = DistanceWindow(q.packed, qpos, db.packed, h.dpos, p.subsequence_length, differs);
+            d = p.subsequence_length <= kCodesPerWord
+                    ? DistanceFrom(wq[0], wd[0], p.subsequence_length, differs)
+                    : DistanceWindow(q.packed, qpos, db.packed, h.dpos, p.subsequence_length,
+                                     differs);
           if (d <= p.distance_threshold)
-            pass = UngappedWithTrigger(q.packed, qpos, db.packed, h.dpos, utab, cut, trig) > trig;
+            pass = UngappedWithTrigger(q.packed, wq[0], db.packed, wd[0], utab, cut, trig) > trig;
           member_ok = !check || max(d - p.max_mismatches, 0) <= p.distance_threshold;
           if (member_ok) {
+            const uint32_t ri = lbeg[lo] + idx;
             mb = db.mem_off[ri];
             me = db.mem_off[ri + 1];
           }
@@ -279,8 +311,29 @@ __global__ void __launch_bounds__(kSeedThreads)
           WaveAppend(mp, mh, hits, counters, capacity);
         }
       }
+#pragma unroll
+      for (int k = 0; k + 1 < kSeedItems; ++k) {
+        valid[k] = valid[k + 1];
+        lidx[k] = lidx[k + 1];
+        ridx[k] = ridx[k + 1];
+        eidx[k] = eidx[k + 1];
+        dposv[k] = dposv[k + 1];
+        rec[k] = rec[k + 1];
+        wq[k] = wq[k + 1];
+        wd[k] = wd[k + 1];
+      }
     }
   }
+}
+
+// One 16-byte record per list entry: query position, query id (bit 31: the
+// distance check), ungapped cutoff and trigger of the query.
+__global__ void EntryRecordKernel(DeviceQueryLists q, uint32_t n, uint4 *rec) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t qid = q.entry_qid[i];
+  rec[i] = make_uint4(q.entry_pos[i], qid | (q.entry_check[i] ? 0x80000000u : 0u),
+                      (uint32_t)q.cutoffs[qid], (uint32_t)q.triggers[qid]);
 }
 
 __global__ void FillU32Kernel(uint32_t *p, size_t n, uint32_t v) {
@@ -306,7 +359,7 @@ __global__ void QueryScatterKernel(const SeedHit *hits, size_t n, uint32_t qb,
 
 SeedStage::~SeedStage() {
   void *ps[] = {lookup_,  units_,  scan_tmp_,   active_,   hits_,
-                grouped_, qcount_, tile_range_, base_info_};
+                grouped_, qcount_, tile_range_, base_info_, entry_rec_};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; ++k) {
@@ -337,6 +390,19 @@ hipError_t SeedStage::Reserve(void **p, size_t *cap, size_t bytes) {
     hipError_t _e = (x);               \
     if (_e != hipSuccess) return _e;   \
   } while (0)
+
+hipError_t SeedStage::PrepareEntries(DeviceQueryLists *q, uint64_t entries, hipStream_t s) {
+  q->entry_rec = nullptr;
+  if (entries >= 0xffffffffull) return hipErrorInvalidValue;
+  GZ_TRY(Reserve(&entry_rec_, &entry_rec_cap_, std::max<uint64_t>(entries, 1) * sizeof(uint4)));
+  if (entries > 0) {
+    hipLaunchKernelGGL(EntryRecordKernel, dim3((unsigned)((entries + 255) / 256)), dim3(256), 0,
+                       s, *q, (uint32_t)entries, (uint4 *)entry_rec_);
+    GZ_TRY(hipGetLastError());
+  }
+  q->entry_rec = (const uint4 *)entry_rec_;
+  return hipSuccess;
+}
 
 hipError_t SeedStage::Lookup(const DeviceQueryLists &q, const DeviceDbChunk &db,
                              bool clustering, hipStream_t s) {

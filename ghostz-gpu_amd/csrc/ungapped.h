@@ -1,7 +1,7 @@
 // ungapped.h -- the seed tests of the presearch on packed codes, twelve
 // residues (one 64-bit word) per step:
 //
-//  * UngappedSide / UngappedWithTrigger: SeedSearcher::UngappedExtend
+//  * UngappedSideFrom / UngappedWithTrigger: SeedSearcher::UngappedExtend
 //    (src/seed_searcher.h:133-158).  The reference walks one residue at a
 //    time, committing the running sum whenever it rises above the best
 //    score and stopping at a delimiter or when the sum falls cutoff below
@@ -10,7 +10,7 @@
 //    prefix sum at or below best - cutoff.  Here a delimiter scores
 //    kStopScore, which is a stop for every cutoff below -kStopScore, so a
 //    step is branch-free: add, max, compare per residue.
-//  * DistanceWindow: DistanceCalculator::CalculateDistance
+//  * DistanceWindow / DistanceFrom: DistanceCalculator::CalculateDistance
 //    (src/distance_calculator.cpp:35-54), the reduced-alphabet mismatches of
 //    a window of up to 12 residues, through a 32 x 32 "differs" table.
 //
@@ -47,22 +47,39 @@ GZ_HD uint64_t Window12(uint64_t lo, uint64_t hi, uint32_t k) {
   return (lo >> (kCodeBits * k)) | (hi << (kCodeBits * (kCodesPerWord - k)));
 }
 
-// One side of the ungapped extension from (qpos, dpos): codes at qpos,
-// qpos+1, ... (kReverse: qpos, qpos-1, ...) of the padded packed arrays.
-// tab: 32 x 32 ungapped table.  Returns the best prefix sum (>= 0).
+// The 36 codes around a seed position `pos`: words w0..w2 hold codes
+// [pos - 12, pos + 12) at offset k, so the reverse side's first twelve codes
+// (pos - 12 .. pos - 1) are Window12(w0, w1, k) and the forward side's
+// (pos .. pos + 11) Window12(w1, w2, k).  Loading both sides' words up front
+// lets a kernel issue them together.
+struct SeedWindow {
+  uint64_t w0, w1, w2;
+  uint32_t wi;  // word index of w0
+  uint32_t k;
+};
+
+GZ_HD SeedWindow LoadSeedWindow(const uint64_t *packed, uint32_t pos) {
+  SeedWindow s;
+  const uint32_t c = pos - kCodesPerWord + kPackPadCodes;
+  s.wi = c / kCodesPerWord;
+  s.k = c - s.wi * kCodesPerWord;
+  s.w0 = packed[s.wi] & kCodeBitsMask;
+  s.w1 = packed[s.wi + 1] & kCodeBitsMask;
+  s.w2 = packed[s.wi + 2] & kCodeBitsMask;
+  return s;
+}
+
+// UngappedSide from preloaded words: the reverse side from pos - 1
+// backwards, the forward side from pos onwards.
 template <bool kReverse>
-GZ_HD int UngappedSide(const uint64_t *qw, uint32_t qpos, const uint64_t *dw, uint32_t dpos,
-                       const int16_t *tab, int cutoff) {
-  // first window: [pos, pos + 12), or [pos - 11, pos] read backwards
-  const uint32_t qs = (kReverse ? qpos - (kCodesPerWord - 1) : qpos) + kPackPadCodes;
-  const uint32_t ds = (kReverse ? dpos - (kCodesPerWord - 1) : dpos) + kPackPadCodes;
-  uint32_t qi = qs / kCodesPerWord, di = ds / kCodesPerWord;
-  const uint32_t qk = qs - qi * kCodesPerWord, dk = ds - di * kCodesPerWord;
-  uint64_t q0 = qw[qi] & kCodeBitsMask, q1 = qw[qi + 1] & kCodeBitsMask;
-  uint64_t d0 = dw[di] & kCodeBitsMask, d1 = dw[di + 1] & kCodeBitsMask;
+GZ_HD int UngappedSideFrom(const uint64_t *qw, const SeedWindow &sq, const uint64_t *dw,
+                           const SeedWindow &sd, const int16_t *tab, int cutoff) {
+  uint64_t q0 = kReverse ? sq.w0 : sq.w1, q1 = kReverse ? sq.w1 : sq.w2;
+  uint64_t d0 = kReverse ? sd.w0 : sd.w1, d1 = kReverse ? sd.w1 : sd.w2;
+  uint32_t qi = kReverse ? sq.wi : sq.wi + 1, di = kReverse ? sd.wi : sd.wi + 1;
   int c = 0, best = 0;
   for (;;) {
-    const uint64_t Q = Window12(q0, q1, qk), D = Window12(d0, d1, dk);
+    const uint64_t Q = Window12(q0, q1, sq.k), D = Window12(d0, d1, sd.k);
     bool alive = true;
 #pragma unroll
     for (int j = 0; j < (int)kCodesPerWord; ++j) {
@@ -92,11 +109,18 @@ GZ_HD int UngappedSide(const uint64_t *qw, uint32_t qpos, const uint64_t *dw, ui
 
 // SeedSearcher::UngappedExtend: reverse from (qpos-1, dpos-1), forward from
 // (qpos, dpos) only while the sum is <= trigger.
+GZ_HD int UngappedWithTrigger(const uint64_t *qw, const SeedWindow &sq, const uint64_t *dw,
+                              const SeedWindow &sd, const int16_t *tab, int cutoff,
+                              int trigger) {
+  int sum = UngappedSideFrom<true>(qw, sq, dw, sd, tab, cutoff);
+  if (sum <= trigger) sum += UngappedSideFrom<false>(qw, sq, dw, sd, tab, cutoff);
+  return sum;
+}
+
 GZ_HD int UngappedWithTrigger(const uint64_t *qw, uint32_t qpos, const uint64_t *dw,
                               uint32_t dpos, const int16_t *tab, int cutoff, int trigger) {
-  int sum = UngappedSide<true>(qw, qpos - 1, dw, dpos - 1, tab, cutoff);
-  if (sum <= trigger) sum += UngappedSide<false>(qw, qpos, dw, dpos, tab, cutoff);
-  return sum;
+  const SeedWindow sq = LoadSeedWindow(qw, qpos), sd = LoadSeedWindow(dw, dpos);
+  return UngappedWithTrigger(qw, sq, dw, sd, tab, cutoff, trigger);
 }
 
 // Mismatches of the reduced codes over [qc - length/2, qc - length/2 +
@@ -120,6 +144,25 @@ GZ_HD int DistanceWindow(const uint64_t *qw, uint32_t qc, const uint64_t *dw, ui
     }
     qs += kCodesPerWord;
     ds += kCodesPerWord;
+  }
+  return count;
+}
+
+// DistanceWindow from preloaded words (length <= 12: the window lies inside
+// the 36 preloaded codes).
+GZ_HD int DistanceFrom(const SeedWindow &sq, const SeedWindow &sd, uint32_t length,
+                       const uint8_t *differs) {
+  const uint32_t oq = sq.k + kCodesPerWord - length / 2, od = sd.k + kCodesPerWord - length / 2;
+  const uint64_t Q = oq >= kCodesPerWord ? Window12(sq.w1, sq.w2, oq - kCodesPerWord)
+                                         : Window12(sq.w0, sq.w1, oq);
+  const uint64_t D = od >= kCodesPerWord ? Window12(sd.w1, sd.w2, od - kCodesPerWord)
+                                         : Window12(sd.w0, sd.w1, od);
+  int count = 0;
+#pragma unroll
+  for (int j = 0; j < (int)kCodesPerWord; ++j) {
+    const uint32_t a = (uint32_t)(Q >> (kCodeBits * j)) & 31u;
+    const uint32_t b = (uint32_t)(D >> (kCodeBits * j)) & 31u;
+    if ((uint32_t)j < length) count += differs[(b << 5) | a];
   }
   return count;
 }

@@ -272,8 +272,14 @@ void gzh_distance_words(const uint64_t *qw, const uint64_t *dw, uint64_t n,
                         const uint8_t *reduced_map, int *out) {
   uint8_t differs[kTableDim * kTableDim];
   for (uint32_t i = 0; i < kTableDim * kTableDim; ++i) differs[i] = DiffersEntry(reduced_map, i);
-  for (uint64_t k = 0; k < n; ++k)
+  for (uint64_t k = 0; k < n; ++k) {
     out[k] = DistanceWindow(qw, qc[k], dw, dc[k], length, differs);
+    // the preloaded-window form the seed kernel uses for length <= 12
+    if (length <= kCodesPerWord &&
+        DistanceFrom(LoadSeedWindow(qw, qc[k]), LoadSeedWindow(dw, dc[k]), length, differs) !=
+            out[k])
+      out[k] = -1000;
+  }
 }
 
 }  // extern "C"
