@@ -164,6 +164,9 @@ __global__ void __launch_bounds__(kSeedThreads)
   __shared__ uint64_t loff[kSeedTile + 1];
   __shared__ uint32_t lrank[kSeedTile], leb[kSeedTile], lp[kSeedTile], lbeg[kSeedTile],
       lnrep[kSeedTile];
+  __shared__ double linv[kSeedTile];        // 1 / entries, for the unit division
+  __shared__ uint32_t owner[kSeedTile];     // the tile list of each unit of the tile
+  __shared__ uint32_t scan_lds[kSeedThreads / kWave];
   for (uint32_t i = threadIdx.x; i < kTableDim * kTableDim; i += blockDim.x) {
     utab[i] = UngappedTableEntry(p.matrix, p.nl, i);
     if (!kValues) differs[i] = DiffersEntry(p.reduced_map, i);
@@ -175,6 +178,7 @@ __global__ void __launch_bounds__(kSeedThreads)
     const uint2 tr = tile_range[(tile - unit_begin) / kSeedTile];
     const uint32_t first = tr.x, count = tr.y;
     __syncthreads();  // the previous tile's table is no longer read
+    for (uint32_t i = threadIdx.x; i < kSeedTile; i += kSeedThreads) owner[i] = 0;
     for (uint32_t i = threadIdx.x; i <= count; i += kSeedThreads) {
       loff[i] = active_off[first + i];
       if (i < count) {
@@ -183,9 +187,25 @@ __global__ void __launch_bounds__(kSeedThreads)
         lrank[i] = l;
         leb[i] = li.ent_begin;
         lp[i] = li.nent;
+        linv[i] = 1.0 / (double)(li.nent ? li.nent : 1u);
         lbeg[i] = kValues ? li.val_begin : li.rep_begin;
         lnrep[i] = li.nrep;
       }
+    }
+    __syncthreads();
+    // owner[x] = the list holding unit tile + x: mark where each list starts
+    // inside the tile, then an inclusive max-scan over the tile
+    for (uint32_t i = threadIdx.x + 1; i < count; i += kSeedThreads)
+      owner[loff[i] - tile] = i;  // lists after the first start inside the tile
+    __syncthreads();
+    {
+      static_assert(kSeedItems == 2, "owner scan covers two units per thread");
+      const uint32_t a = owner[2 * threadIdx.x], b = owner[2 * threadIdx.x + 1];
+      const uint32_t mine = a > b ? a : b;
+      uint32_t incl, excl;
+      BlockScan<uint32_t, true>(mine, scan_lds, &incl, &excl, nullptr);
+      owner[2 * threadIdx.x] = a > excl ? a : excl;
+      owner[2 * threadIdx.x + 1] = incl;
     }
     __syncthreads();
     // all items' indices and first loads are issued before any is evaluated:
@@ -198,17 +218,24 @@ __global__ void __launch_bounds__(kSeedThreads)
     for (int j = 0; j < kSeedItems; ++j) {
       const uint64_t u = tile + (uint64_t)j * kSeedThreads + threadIdx.x;
       valid[j] = u < tile_end;
-      uint32_t lo = 0, hi = count;  // largest i < count with loff[i] <= u
-      while (lo + 1 < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (loff[mid] <= u) lo = mid; else hi = mid;
-      }
+      const uint32_t lo = valid[j] ? owner[u - tile] : 0u;
       const uint64_t local = valid[j] ? u - loff[lo] : 0;
       const uint32_t P = lp[lo] ? lp[lo] : 1u;
       uint32_t idx, e;
       if (local < 0xffffffffull) {
-        idx = (uint32_t)local / P;
-        e = (uint32_t)local - idx * P;
+        // quotient from the double reciprocal (53 bits: off by at most one)
+        const uint32_t n = (uint32_t)local;
+        uint32_t qt = (uint32_t)((double)n * linv[lo]);
+        int64_t r = (int64_t)n - (int64_t)qt * P;
+        if (r < 0) {
+          --qt;
+          r += P;
+        } else if (r >= (int64_t)P) {
+          ++qt;
+          r -= P;
+        }
+        idx = qt;
+        e = (uint32_t)r;
       } else {
         idx = (uint32_t)(local / P);
         e = (uint32_t)(local - (uint64_t)idx * P);
