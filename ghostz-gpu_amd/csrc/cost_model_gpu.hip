@@ -41,23 +41,27 @@ namespace {
 
 // One feature window: 32 codes from position `start` stepping forward or
 // backward, codes >= 20 as 20, the delimiter and everything after it as 20
-// (CostFeatures, cost_model.cpp).  Twelve codes come per packed word pair;
-// a pair is read only while no delimiter has been met, so reads stay inside
-// the padded array.
+// (CostFeatures, cost_model.cpp).  Twelve codes come per packed word pair and
+// the three pairs are consecutive words at one offset, so the four words are
+// read at once: for any start inside the sequences they lie between the
+// 48-code front pad and the four tail words.
 template <bool kBackward>
 __device__ __forceinline__ void FeatureWindow(const uint64_t *packed, uint32_t start,
                                               uint32_t out[8]) {
+  // part p covers codes start + 12 p + j (forward) or start - 12 p - j (backward)
+  const uint32_t c0 = (kBackward ? start - 11u : start) + kPackPadCodes;
+  const uint32_t q = c0 / kCodesPerWord, off = c0 - q * kCodesPerWord;
+  const uint32_t wb = kBackward ? q - 2u : q;
+  uint64_t wd[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) wd[i] = packed[wb + i] & kCodeBitsMask;
   bool stopped = false;
 #pragma unroll
   for (int i = 0; i < 8; ++i) out[i] = 0x14141414u;
 #pragma unroll
   for (int part = 0; part < 3; ++part) {
-    if (stopped) break;
-    // codes start + 12 part + j (forward) or start - 12 part - j (backward)
-    const uint32_t first = kBackward ? start - 12u * part - 11u : start + 12u * part;
-    const uint32_t c = first + kPackPadCodes, w = c / kCodesPerWord;
-    const uint64_t win = Window12(packed[w] & kCodeBitsMask, packed[w + 1] & kCodeBitsMask,
-                                  c - w * kCodesPerWord);
+    const uint64_t win = kBackward ? Window12(wd[2 - part], wd[3 - part], off)
+                                   : Window12(wd[part], wd[part + 1], off);
 #pragma unroll
     for (int j = 0; j < 12; ++j) {
       const int k = 12 * part + j;  // index in the window

@@ -1,5 +1,5 @@
 // mlp.h -- the cost-model MLP on the device: embedding 21x8 per residue,
-// 1024 -> 256 -> 64 -> 1 with GELU, bf16 operands, fp32 accumulation (MFMA),
+// 1024 -> 64 -> 64 -> 1 with GELU, bf16 operands, fp32 accumulation (MFMA),
 // fp32 master weights and Adam for training.
 #pragma once
 
@@ -28,7 +28,7 @@ class MlpDevice {
   hipError_t TrainStep(const uint8_t *codes, const float *targets, size_t n, float lr,
                        float *loss, float *flat_grad);
   // Forward on device-resident codes (n x 128) into device outputs, on the
-  // caller's stream (fused persistent inference kernels).
+  // caller's stream (one persistent kernel for all three layers).
   hipError_t InferDevice(const uint8_t *d_codes, size_t n, float *d_out, hipStream_t stream);
   // Kernel-only forward time of the last Forward call (HIP events).
   float last_forward_ms = 0;
@@ -48,19 +48,17 @@ class MlpDevice {
   // activations
   uint8_t *codes_ = nullptr;
   uint16_t *x_ = nullptr;       // [B][1024] bf16
-  float *z1_ = nullptr;         // [B][256]
-  uint16_t *h1_ = nullptr;      // [B][256] bf16
+  float *z1_ = nullptr;         // [B][64]
+  uint16_t *h1_ = nullptr;      // [B][64] bf16
   float *z2_ = nullptr;         // [B][64]
   uint16_t *h2_ = nullptr;      // [B][64] bf16
   float *y_ = nullptr, *t_ = nullptr, *loss_ = nullptr;
   float *dz2_ = nullptr;        // [B][64]
   uint16_t *dz2h_ = nullptr;
-  float *dz1_ = nullptr;        // [B][256]
+  float *dz1_ = nullptr;        // [B][64]
   uint16_t *dz1h_ = nullptr;
-  float *dh1_ = nullptr;        // [B][256]
+  float *dh1_ = nullptr;        // [B][64]
   float *dx_ = nullptr;         // [B][1024]
-  uint16_t *infer_h1_ = nullptr;  // inference layer-1 output, chunk x 256 bf16
-  size_t infer_h1_cap_ = 0;
 };
 
 }  // namespace gz

@@ -213,29 +213,28 @@ __global__ void ToBf16Kernel(size_t n, const float *src, uint16_t *dst) {
 inline unsigned Blocks(size_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // ---------------------------------------------------------------------------
-// Inference: two persistent kernels on v_mfma_f32_32x32x16_bf16.
+// Inference: one persistent kernel on v_mfma_f32_32x32x16_bf16 runs all three
+// layers per tile of seeds, so no activation leaves the CU.
 //
-// Layer 1 keeps a 64-output slice of W1 (64 x 1024 bf16 = 128 KB) resident in
-// LDS and streams tiles of 512 seeds through it; the embedding lookup is done
-// while building the MFMA A operand (a lane's 8 bf16 are one residue's 8-dim
-// vector).  The head kernel keeps W2 (32 KB) in LDS, reads H1 rows straight
-// into the A operand, runs layer 2 on MFMA and reduces layer 3 across lanes.
-// Rounding points are those of the training forward.
+// Operand layout of the 32x32x16 MFMA: lane l supplies A[row l % 32][k] and
+// B[k][column l % 32] for k = 8 * (l / 32) + j, j < 8; accumulator register r
+// of lane l holds C[8 * (r / 4) + 4 * (l / 32) + (r % 4)][l % 32].
+//
+// Layers are computed transposed (hidden unit = row, seed = column):
+//   layer 1: C1[h][seed] = W1[h][:] . x[seed][:], A = W1 (LDS, [k/8][64][8]),
+//            B = the residue embeddings of 32 seeds (one 16-byte LDS read);
+//   layer 2: C2[n][seed] = W2[n][:] . h1[:][seed], whose B operand is the
+//            lane's own layer-1 accumulators (a lane holds one seed's column):
+//            K step t takes registers 8 (t % 2) .. 8 (t % 2) + 7 of row block
+//            t / 2, i.e. hidden units 16 t + 8 (j / 4) + 4 (l / 32) + j % 4,
+//            and W2's columns are staged in LDS in that order;
+//   layer 3: each lane sums its 32 rows of C2 and adds its partner lane's.
+// Rounding points are those of the training forward (bf16 h1, h2, w3).
 // ---------------------------------------------------------------------------
-constexpr int kInferTile = 128;
-constexpr int kInferSlice = 64;
-
-typedef short short4v __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ short4v Load4(const uint16_t *p) {
-  const uint2 v = *reinterpret_cast<const uint2 *>(p);
-  short4v r;
-  r[0] = (short)(v.x & 0xffff);
-  r[1] = (short)(v.x >> 16);
-  r[2] = (short)(v.y & 0xffff);
-  r[3] = (short)(v.y >> 16);
-  return r;
-}
+static_assert(kCostH1 == 64 && kCostH2 == 64, "the fused inference kernel is 64 x 64");
+constexpr int kInferWaves = 8;
+constexpr int kInferBlocks = 4;                            // 32-seed columns per wave
+constexpr int kInferTile = kInferWaves * 32 * kInferBlocks;  // seeds per block pass
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -243,151 +242,125 @@ __device__ __forceinline__ bf16x8 Load8(const uint16_t *p) {
   return *reinterpret_cast<const bf16x8 *>(p);
 }
 
-// Layer 1 on v_mfma_f32_32x32x16_bf16.  A block of 16 waves holds one 64-column
-// slice of W1 as [k/8][64][8] (128 KB, the B operand of a lane is one 16-byte
-// read) and streams tiles of 1024 seeds through it; each wave owns 64 seeds x
-// 64 columns (four 32x32 accumulators).  Operand layout: lane l holds row
-// (A) or column (B) l % 32 and k = 8 * (l / 32) + j, j < 8, so one K step of
-// 16 covers residues 2i (lanes 0-31) and 2i + 1 (lanes 32-63), each lane's A
-// operand being the 8-dim embedding of its residue.
-constexpr int kL1Waves = 16;
-constexpr int kL1Tile = kL1Waves * 64;
-
-__global__ void __launch_bounds__(kL1Waves * 64)
-    InferLayer1Kernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h,
-                      const uint16_t *embh, const float *b1, uint16_t *h1) {
-  __shared__ __align__(16) uint16_t w[kCostIn * kInferSlice];
-  __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
-  const int slice = blockIdx.x & 3;
-  const int group = blockIdx.x >> 2, groups = gridDim.x >> 2;
-  for (int idx = threadIdx.x; idx < kInferSlice * (kCostIn / 8); idx += blockDim.x) {
-    const int nn = idx / (kCostIn / 8), k8 = idx % (kCostIn / 8);
-    const uint4 v = *reinterpret_cast<const uint4 *>(
-        w1h + (size_t)(slice * kInferSlice + nn) * kCostIn + k8 * 8);
-    *reinterpret_cast<uint4 *>(&w[(k8 * kInferSlice + nn) * 8]) = v;
-  }
-  for (int i = threadIdx.x; i < kCostAlphabet * kCostEmbed; i += blockDim.x) emb[i] = embh[i];
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int kq = lane >> 5, r32 = lane & 31;
-  const uint32_t tiles = (n + kL1Tile - 1) / kL1Tile;
-  for (uint32_t t = group; t < tiles; t += groups) {
-    const uint32_t base = t * kL1Tile + wave * 64;
-    // rows base + r32 and base + 32 + r32: their 128 codes stream through
-    // registers 16 at a time (rows past the end read row 0 as code 20)
-    const uint32_t ra = base + r32, rb = ra + 32;
-    const uint4 *sa = reinterpret_cast<const uint4 *>(codes + (size_t)(ra < n ? ra : 0) *
-                                                                  kCostResidues);
-    const uint4 *sb = reinterpret_cast<const uint4 *>(codes + (size_t)(rb < n ? rb : 0) *
-                                                                  kCostResidues);
-    const uint32_t fa = ra < n ? 0u : 0x14141414u, fb = rb < n ? 0u : 0x14141414u;
-    float16_t acc00, acc01, acc10, acc11;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      acc00[r] = 0.f;
-      acc01[r] = 0.f;
-      acc10[r] = 0.f;
-      acc11[r] = 0.f;
-    }
-    const uint32_t kshift = 8u * (uint32_t)kq;
-#pragma unroll 1
-    for (int v = 0; v < kCostResidues / 16; ++v) {
-      uint4 ca = sa[v], cb = sb[v];
-      ca.x |= fa; ca.y |= fa; ca.z |= fa; ca.w |= fa;
-      cb.x |= fb; cb.y |= fb; cb.z |= fb; cb.w |= fb;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        // residue v * 16 + 2q + kq: byte 2 (q & 1) + kq of word q / 2
-        const uint32_t wa = q < 2 ? ca.x : q < 4 ? ca.y : q < 6 ? ca.z : ca.w;
-        const uint32_t wb = q < 2 ? cb.x : q < 4 ? cb.y : q < 6 ? cb.z : cb.w;
-        uint32_t code_a = (wa >> (16 * (q & 1) + kshift)) & 0xffu;
-        uint32_t code_b = (wb >> (16 * (q & 1) + kshift)) & 0xffu;
-        code_a = code_a < 20 ? code_a : 20;
-        code_b = code_b < 20 ? code_b : 20;
-        const bf16x8 a0 = Load8(&emb[code_a * kCostEmbed]);
-        const bf16x8 a1 = Load8(&emb[code_b * kCostEmbed]);
-        const int kb = v * 16 + 2 * q + kq;
-        const bf16x8 b0 = Load8(&w[(kb * kInferSlice + r32) * 8]);
-        const bf16x8 b1v = Load8(&w[(kb * kInferSlice + 32 + r32) * 8]);
-        acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc00, 0, 0, 0);
-        acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1v, acc01, 0, 0, 0);
-        acc10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc10, 0, 0, 0);
-        acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1v, acc11, 0, 0, 0);
-      }
-    }
-    const int c0 = slice * kInferSlice + r32;
-    const float bias0 = b1[c0], bias1 = b1[c0 + 32];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const uint32_t row = 8 * (r >> 2) + 4 * kq + (r & 3);
-      const uint32_t sa_row = base + row, sb_row = base + 32 + row;
-      if (sa_row < n) {
-        h1[(size_t)sa_row * kCostH1 + c0] = F2Bf(Gelu(acc00[r] + bias0));
-        h1[(size_t)sa_row * kCostH1 + c0 + 32] = F2Bf(Gelu(acc01[r] + bias1));
-      }
-      if (sb_row < n) {
-        h1[(size_t)sb_row * kCostH1 + c0] = F2Bf(Gelu(acc10[r] + bias0));
-        h1[(size_t)sb_row * kCostH1 + c0 + 32] = F2Bf(Gelu(acc11[r] + bias1));
-      }
-    }
-  }
+__device__ __forceinline__ __bf16 AsBf16(uint16_t h) {
+  return __builtin_bit_cast(__bf16, h);
 }
 
-// Layers 2 and 3: W2 as [k/8][64][8] in LDS (32 KB), each wave owns 32
-// seeds and reads their H1 rows straight into its A operand (16 bytes per
-// lane and K step of 16), layer 3 reduces across lanes.
-__global__ void __launch_bounds__(256)
-    InferHeadKernel(const uint16_t *h1, uint32_t n, const uint16_t *w2h, const float *b2,
-                    const float *w3, const float *b3, float *y) {
-  __shared__ __align__(16) uint16_t w[kCostH1 * kCostH2];
-  __shared__ float w3b[kCostH2], bb[kCostH2];
-  for (int idx = threadIdx.x; idx < kCostH2 * (kCostH1 / 8); idx += 256) {
-    const int nn = idx / (kCostH1 / 8), k8 = idx % (kCostH1 / 8);
-    *reinterpret_cast<uint4 *>(&w[(k8 * kCostH2 + nn) * 8]) =
-        *reinterpret_cast<const uint4 *>(w2h + (size_t)nn * kCostH1 + k8 * 8);
+__global__ void __launch_bounds__(kInferWaves * 64)
+    InferKernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h, const uint16_t *embh,
+                const float *b1, const uint16_t *w2h, const float *b2, const float *w3,
+                const float *b3, float *y) {
+  __shared__ __align__(16) uint16_t w1[kCostIn * kCostH1];   // [k/8][64][8], 128 KB
+  __shared__ __align__(16) uint16_t w2[4 * 2 * 64 * 8];      // [t][row block][lane][8]
+  __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
+  __shared__ float b1s[kCostH1], b2s[kCostH2], w3s[kCostH2];
+  for (int idx = threadIdx.x; idx < kCostH1 * (kCostIn / 8); idx += blockDim.x) {
+    const int h = idx / (kCostIn / 8), k8 = idx % (kCostIn / 8);
+    *reinterpret_cast<uint4 *>(&w1[(k8 * kCostH1 + h) * 8]) =
+        *reinterpret_cast<const uint4 *>(w1h + (size_t)h * kCostIn + k8 * 8);
   }
+  for (int idx = threadIdx.x; idx < 4 * 2 * 64 * 8; idx += blockDim.x) {
+    const int j = idx & 7, l = (idx >> 3) & 63, nb = (idx >> 9) & 1, t = idx >> 10;
+    const int row = 32 * nb + (l & 31);
+    const int h = 16 * t + 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
+    w2[idx] = w2h[row * kCostH1 + h];
+  }
+  for (int i = threadIdx.x; i < kCostAlphabet * kCostEmbed; i += blockDim.x) emb[i] = embh[i];
+  if (threadIdx.x < kCostH1) b1s[threadIdx.x] = b1[threadIdx.x];
   if (threadIdx.x < kCostH2) {
-    w3b[threadIdx.x] = Bf2F(F2Bf(w3[threadIdx.x]));
-    bb[threadIdx.x] = b2[threadIdx.x];
+    b2s[threadIdx.x] = b2[threadIdx.x];
+    w3s[threadIdx.x] = Bf2F(F2Bf(w3[threadIdx.x]));
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int kq = lane >> 5, r32 = lane & 31;
+  const uint32_t kshift = 8u * (uint32_t)kq;
+  const float bias3 = b3[0];
   const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
-  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-    const uint32_t base = t * kInferTile + wave * 32;
-    const uint32_t row = base + r32;
-    const uint16_t *src = h1 + (size_t)(row < n ? row : 0) * kCostH1 + 8 * kq;
-    float16_t acc0, acc1;
+  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const uint32_t base = tile * kInferTile + wave * (32 * kInferBlocks);
+    // the codes of seed base + 32 c + r32 stream through registers 16 at a
+    // time; seeds past the end read seed 0's row as code 20 throughout
+    const uint4 *src[kInferBlocks];
+    uint32_t fill[kInferBlocks];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      acc0[r] = 0.f;
-      acc1[r] = 0.f;
+    for (int c = 0; c < kInferBlocks; ++c) {
+      const uint32_t seed = base + 32 * c + r32;
+      src[c] = reinterpret_cast<const uint4 *>(codes + (size_t)(seed < n ? seed : 0) *
+                                                           kCostResidues);
+      fill[c] = seed < n ? 0u : 0x14141414u;
     }
-#pragma unroll 4
-    for (int k16 = 0; k16 < kCostH1 / 16; ++k16) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(src + 16 * k16);
-      const int kb = 2 * k16 + kq;
-      const bf16x8 b0 = Load8(&w[(kb * kCostH2 + r32) * 8]);
-      const bf16x8 b1v = Load8(&w[(kb * kCostH2 + 32 + r32) * 8]);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1v, acc1, 0, 0, 0);
-    }
-    // layer 3: per row, sum over the 64 columns (32 lanes x 2 accumulators)
-    float part[16];
+    float16_t acc[2][kInferBlocks];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      part[r] = Bf2F(F2Bf(Gelu(acc0[r] + bb[r32]))) * w3b[r32] +
-                Bf2F(F2Bf(Gelu(acc1[r] + bb[r32 + 32]))) * w3b[r32 + 32];
+    for (int hb = 0; hb < 2; ++hb)
 #pragma unroll
-      for (int off = 16; off > 0; off >>= 1) part[r] += __shfl_xor(part[r], off, 64);
+      for (int c = 0; c < kInferBlocks; ++c)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[hb][c][r] = 0.f;
+#pragma unroll 1
+    for (int v = 0; v < kCostResidues / 16; ++v) {
+      uint4 cw[kInferBlocks];
+#pragma unroll
+      for (int c = 0; c < kInferBlocks; ++c) {
+        cw[c] = src[c][v];
+        cw[c].x |= fill[c];
+        cw[c].y |= fill[c];
+        cw[c].z |= fill[c];
+        cw[c].w |= fill[c];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        // residue 16 v + 2 q + kq: byte 2 (q & 1) + kq of word q / 2
+        const int kb = v * 16 + 2 * q + kq;
+        const bf16x8 a0 = Load8(&w1[(kb * kCostH1 + r32) * 8]);
+        const bf16x8 a1 = Load8(&w1[(kb * kCostH1 + 32 + r32) * 8]);
+#pragma unroll
+        for (int c = 0; c < kInferBlocks; ++c) {
+          const uint32_t word = q < 2 ? cw[c].x : q < 4 ? cw[c].y : q < 6 ? cw[c].z : cw[c].w;
+          uint32_t code = (word >> (16 * (q & 1) + kshift)) & 0xffu;
+          code = code < 20 ? code : 20;
+          const bf16x8 bx = Load8(&emb[code * kCostEmbed]);
+          acc[0][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bx, acc[0][c], 0, 0, 0);
+          acc[1][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bx, acc[1][c], 0, 0, 0);
+        }
+      }
     }
-    if (r32 == 0) {
+#pragma unroll
+    for (int c = 0; c < kInferBlocks; ++c) {
+      // keeps the epilogue's LDS reads (biases, W2, w3) inside the tile loop
+      // instead of holding them in registers across it
+      __asm__ volatile("" ::: "memory");
+      float16_t acc2[2];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const uint32_t seed = base + 8 * (r >> 2) + 4 * kq + (r & 3);
-        if (seed < n) y[seed] = part[r] + b3[0];
+        acc2[0][r] = 0.f;
+        acc2[1][r] = 0.f;
       }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 bh;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int h = 16 * t + 8 * (j >> 2) + 4 * kq + (j & 3);
+          bh[j] = AsBf16(F2Bf(Gelu(acc[t >> 1][c][8 * (t & 1) + j] + b1s[h])));
+        }
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+          const bf16x8 a = Load8(&w2[((t * 2 + nb) * 64 + lane) * 8]);
+          acc2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bh, acc2[nb], 0, 0, 0);
+        }
+      }
+      float part = 0.f;
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = 32 * nb + 8 * (r >> 2) + 4 * kq + (r & 3);
+          part += Bf2F(F2Bf(Gelu(acc2[nb][r] + b2s[row]))) * w3s[row];
+        }
+      part += __shfl_xor(part, 32, 64);
+      const uint32_t seed = base + 32 * c + r32;
+      if (kq == 0 && seed < n) y[seed] = part + bias3;
     }
   }
 }
@@ -420,7 +393,7 @@ MlpDevice::MlpDevice() {
 MlpDevice::~MlpDevice() {
   void *ps[] = {master_, grad_, m_,   v_,    w1h_,  w2h_,  embh_, codes_, x_,
                 z1_,     h1_,   z2_,  h2_,   y_,    t_,    loss_, dz2_,   dz2h_,
-                dz1_,    dz1h_, dh1_, dx_,  infer_h1_};
+                dz1_,    dz1h_, dh1_, dx_};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -521,34 +494,16 @@ hipError_t MlpDevice::ForwardDevice(size_t n, size_t B) {
 hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out,
                                   hipStream_t s) {
   if (n == 0) return hipSuccess;
-  const size_t chunk = (size_t)1 << 20;
-  const size_t want = std::min(n, chunk) * kCostH1 * 2;
-  if (infer_h1_cap_ < want) {
-    if (infer_h1_) (void)hipFree(infer_h1_);
-    infer_h1_ = nullptr;
-    infer_h1_cap_ = 0;
-    hipError_t e = hipMalloc((void **)&infer_h1_, want);
-    if (e != hipSuccess) return e;
-    infer_h1_cap_ = want;
-  }
+  if (n > 0xFFFFFFFFull - kInferTile) return hipErrorInvalidValue;
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  for (size_t b0 = 0; b0 < n; b0 += chunk) {
-    const uint32_t m = (uint32_t)std::min(chunk, n - b0);
-    const uint32_t tiles = (m + kInferTile - 1) / kInferTile;
-    const uint32_t l1_tiles = (m + kL1Tile - 1) / kL1Tile;
-    const unsigned groups = std::max(1u, std::min<unsigned>(l1_tiles, (unsigned)cus / 4));
-    hipLaunchKernelGGL(InferLayer1Kernel, dim3(4 * groups), dim3(kL1Waves * 64), 0, s,
-                       d_codes + b0 * kCostResidues, m, w1h_, embh_, master_ + kOffB1,
-                       infer_h1_);
-    hipLaunchKernelGGL(InferHeadKernel, dim3(std::min<unsigned>(tiles, (unsigned)cus * 8)),
-                       dim3(256), 0, s, infer_h1_, m, w2h_, master_ + kOffB2, master_ + kOffW3,
-                       master_ + kOffB3, d_out + b0);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  const uint32_t tiles = (uint32_t)((n + kInferTile - 1) / kInferTile);
+  hipLaunchKernelGGL(InferKernel, dim3(std::min<unsigned>(tiles, (unsigned)cus)),
+                     dim3(kInferWaves * 64), 0, s, d_codes, (uint32_t)n, w1h_, embh_,
+                     master_ + kOffB1, w2h_, master_ + kOffB2, master_ + kOffW3,
+                     master_ + kOffB3, d_out);
+  return hipGetLastError();
 }
 
 hipError_t MlpDevice::Forward(const uint8_t *codes, size_t n, float *out) {

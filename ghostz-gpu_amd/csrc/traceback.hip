@@ -21,6 +21,7 @@
 #include "gz_common.h"
 #include "gz_device.h"
 #include "traceback.h"
+#include "ungapped.h"
 
 namespace gz {
 
@@ -30,6 +31,24 @@ constexpr uint8_t kSub = 0, kGap0 = 1, kGap1 = 2, kExt0 = 4, kExt1 = 8;
 constexpr uint8_t kEditGap0 = 0, kEditSub = 1, kEditGap1 = 2;
 constexpr uint8_t kUnwritten = 0xFF;
 constexpr int kTbThreads = 64;
+// Query sides of up to kTbLdsCols - 1 residues keep their DP columns in LDS,
+// column-major over the block's lanes ([column][lane], 32 KB per block).
+constexpr uint32_t kTbLdsCols = 64;
+
+__shared__ int2 tb_dp[kTbLdsCols * kTbThreads];
+// The query side's codes by DP column: column a (a >= 1) holds the code at
+// q0 + (a - 1) inc, the delimiter closing the side included.
+__shared__ uint8_t tb_qc[kTbLdsCols * kTbThreads];
+
+// DP column access: the lane's LDS column set, or its global scratch.
+template <bool kLds>
+struct DpCells {
+  int2 *global;
+  __device__ __forceinline__ int2 &operator[](int i) const {
+    if constexpr (kLds) return tb_dp[i * kTbThreads + (int)threadIdx.x];
+    else return global[i];
+  }
+};
 
 struct Lane {
   int2 *dp;
@@ -48,21 +67,21 @@ __device__ __forceinline__ uint32_t CodeAt(const uint64_t *packed, int64_t pos) 
   return PackedCodeAt(packed, (uint32_t)pos);
 }
 
-// GappedExtender::ExtendOneSide with traceback; ops are generated from the
-// best cell back to the origin.  Returns flags (kTracebackStale/Overflow).
-__device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const uint64_t *pd,
-                           int64_t d0, bool reversed, const int8_t *sm, int gap_open,
-                           int gap_ext, int cutoff, Lane &L, uint32_t op_base, Side *out) {
-  const int inc = reversed ? -1 : 1;
-  out->score = 0;
-  out->p0 = -inc;
-  out->p1 = -inc;
-  out->nops = 0;
-  if (CodeAt(pq, q0) == kDelimiter || CodeAt(pd, d0) == kDelimiter) return 0;
-  uint32_t n0 = 0;
-  while (CodeAt(pq, q0 + (int64_t)n0 * inc) != kDelimiter) ++n0;
-  const uint32_t need = max(s0len + 1, n0 + 2);
-  if (need > L.dp_cap) return kTracebackOverflow;
+// GappedExtender::ExtendOneSide with traceback (the DP and band bookkeeping
+// after the delimiter checks); ops are generated from the best cell back to
+// the origin.  Returns flags (kTracebackStale/Overflow).
+template <bool kLds>
+__device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, uint32_t s0len,
+                                               const uint64_t *pd, int64_t d0, int inc,
+                                               const int8_t *sm, int gap_open, int gap_ext,
+                                               int cutoff, Lane &L, uint32_t op_base,
+                                               Side *out) {
+  const DpCells<kLds> dp{L.dp};
+  // the query code at q0 + p, p = (a - 1) inc for DP column a
+  auto qcode = [&](int p) -> uint32_t {
+    if constexpr (kLds) return tb_qc[(p * inc + 1) * kTbThreads + (int)threadIdx.x];
+    else return CodeAt(pq, q0 + p);
+  };
   const uint32_t extra =
       (gap_ext > 0 && cutoff != INT_MAX) ? (uint32_t)(cutoff / gap_ext + 3) : s0len + 3;
   uint32_t used = 0;
@@ -75,24 +94,27 @@ __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const
   const int gi = gap_open + gap_ext;
   int as = 0, ae = 0, ai = 0, last = 0;
   int score = -gi;
-  L.dp[0] = make_int2(0, -gi);
+  dp[0] = make_int2(0, -gi);
   int p0 = 0;
-  for (ai = 1; CodeAt(pq, q0 + p0) != kDelimiter; ++ai, p0 += inc) {
+  for (ai = 1; qcode(p0) != kDelimiter; ++ai, p0 += inc) {
     if (score < -cutoff) break;
-    L.dp[ai] = make_int2(score, score - gi);
+    dp[ai] = make_int2(score, score - gi);
     score -= gap_ext;
   }
-  if (CodeAt(pq, q0 + p0) != kDelimiter) {
-    L.dp[ai] = make_int2(kInitScore, kInitScore);
+  if (qcode(p0) != kDelimiter) {
+    dp[ai] = make_int2(kInitScore, kInitScore);
     ae = ai + 1;
   } else {
     ae = ai;
   }
   int max_score = 0, mp0 = -inc, mp1 = -inc;
   uint32_t nrows = 0;
+  uint32_t c1_next = CodeAt(pd, d0);
   for (int p1 = 0;; p1 += inc) {
-    const uint32_t c1 = CodeAt(pd, d0 + p1);
+    const uint32_t c1 = c1_next;
     if (c1 == kDelimiter) break;
+    // the next row's code, read ahead (in bounds: c1 is a residue)
+    c1_next = CodeAt(pd, d0 + p1 + inc);
     const uint32_t length = gap_ext > 0 ? (uint32_t)(ae - as) + extra : extra - (uint32_t)as;
     const uint32_t l = length + length / 3;
     if ((uint64_t)used + l >= L.arena_cap || nrows >= L.row_cap) return kTracebackOverflow;
@@ -102,19 +124,19 @@ __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const
     const int zero = base - off;
     const int8_t *row = sm + c1 * kTableDim;
     p0 = as * inc;
-    int prev = L.dp[as].x;
+    int prev = dp[as].x;
     {
-      int2 c = L.dp[as];
+      int2 c = dp[as];
       c.x = (max_score - c.y > cutoff) ? kInitScore : c.y;
-      L.dp[as] = make_int2(c.x, c.y - gap_ext);
+      dp[as] = make_int2(c.x, c.y - gap_ext);
     }
-    int gap_row = L.dp[as].x - gi;
+    int gap_row = dp[as].x - gi;
     L.arena[zero + as] = kGap0 + kExt0;
     written = max(written, (int64_t)(zero + as));
     last = as;
     for (ai = as + 1; ai < ae; ++ai, p0 += inc) {
-      int s = prev + row[CodeAt(pq, q0 + p0)];
-      const int2 cell = L.dp[ai];
+      int s = prev + row[qcode(p0)];
+      const int2 cell = dp[ai];
       prev = cell.x;
       int gap_col = cell.y;
       uint8_t op = kSub;
@@ -128,7 +150,7 @@ __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const
       }
       if (max_score - s > cutoff) {
         if (as + 1 == ai) ++as;
-        L.dp[ai] = make_int2(kInitScore, kInitScore);
+        dp[ai] = make_int2(kInitScore, kInitScore);
         gap_row = kInitScore;
         L.arena[zero + ai] = kUnwritten;
       } else {
@@ -149,7 +171,7 @@ __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const
         }
         if (gap_row < s - gi) gap_row = s - gi; else op += kExt1;
         L.arena[zero + ai] = op;
-        L.dp[ai] = make_int2(s, gap_keep);
+        dp[ai] = make_int2(s, gap_keep);
       }
       written = max(written, (int64_t)(zero + ai));
     }
@@ -157,8 +179,8 @@ __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const
     if (last < ae - 1) {
       ae = last + 1;
     } else {
-      while (gap_row >= (max_score - cutoff) && CodeAt(pq, q0 + p0) != kDelimiter) {
-        L.dp[ae] = make_int2(gap_row, gap_row - gi);
+      while (gap_row >= (max_score - cutoff) && qcode(p0) != kDelimiter) {
+        dp[ae] = make_int2(gap_row, gap_row - gi);
         L.arena[zero + ae] = kGap0 + kExt0;
         written = max(written, (int64_t)(zero + ae));
         gap_row -= gap_ext;
@@ -166,8 +188,8 @@ __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const
         p0 += inc;
       }
     }
-    if (CodeAt(pq, q0 + p0) != kDelimiter) {
-      L.dp[ae] = make_int2(kInitScore, kInitScore);
+    if (qcode(p0) != kDelimiter) {
+      dp[ae] = make_int2(kInitScore, kInitScore);
       L.arena[zero + ae] = kUnwritten;
       written = max(written, (int64_t)(zero + ae));
       ++ae;
@@ -231,6 +253,52 @@ __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const
   else if (q * inc >= 0) emit(kEditGap1, q * inc + 1);
   out->nops = n - op_base;
   return flags;
+}
+
+__device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const uint64_t *pd,
+                           int64_t d0, bool reversed, const int8_t *sm, int gap_open,
+                           int gap_ext, int cutoff, Lane &L, uint32_t op_base, Side *out) {
+  const int inc = reversed ? -1 : 1;
+  out->score = 0;
+  out->p0 = -inc;
+  out->p1 = -inc;
+  out->nops = 0;
+  if (CodeAt(pq, q0) == kDelimiter || CodeAt(pd, d0) == kDelimiter) return 0;
+  // n0 = residues before the delimiter, twelve codes per packed word pair;
+  // the codes go to the side's LDS columns on the way.  A window starts at a
+  // residue, so its words lie inside the padded array.
+  uint32_t n0 = 0;
+  bool found = false;
+  for (uint32_t base = 0; !found; base += kCodesPerWord) {
+    const uint32_t first =
+        (uint32_t)(reversed ? q0 - (int64_t)base - (int64_t)(kCodesPerWord - 1)
+                            : q0 + (int64_t)base) + kPackPadCodes;
+    const uint32_t w = first / kCodesPerWord;
+    const uint64_t win = Window12(pq[w] & kCodeBitsMask, pq[w + 1] & kCodeBitsMask,
+                                  first - w * kCodesPerWord);
+#pragma unroll
+    for (int j = 0; j < (int)kCodesPerWord; ++j) {
+      const uint32_t code =
+          (uint32_t)(win >> (kCodeBits * (reversed ? (int)kCodesPerWord - 1 - j : j))) & 31u;
+      if (!found) {
+        const uint32_t a = base + (uint32_t)j + 1;
+        if (a < kTbLdsCols) tb_qc[a * kTbThreads + threadIdx.x] = (uint8_t)code;
+        if (code == kDelimiter) {
+          found = true;
+          n0 = base + (uint32_t)j;
+        }
+      }
+    }
+  }
+  // columns 0 .. n0 + 1 are touched (the band may step one past the last
+  // residue); s0len, the distance to the query's far end, can exceed n0 when
+  // the query holds several frames
+  if (n0 + 2 <= kTbLdsCols)
+    return ExtendBody<true>(pq, q0, s0len, pd, d0, inc, sm, gap_open, gap_ext, cutoff, L,
+                            op_base, out);
+  if (max(s0len + 1, n0 + 2) > L.dp_cap) return kTracebackOverflow;
+  return ExtendBody<false>(pq, q0, s0len, pd, d0, inc, sm, gap_open, gap_ext, cutoff, L,
+                           op_base, out);
 }
 
 __global__ void __launch_bounds__(kTbThreads)
