@@ -106,20 +106,39 @@ __global__ void ActiveScatterKernel(uint32_t n, const uint64_t *units,
   }
 }
 
-__device__ __forceinline__ void WaveAppend(bool pred, const SeedHit &h,
-                                           SeedHit *out, uint32_t *counter,
+// Hits staged per wave in LDS and written in batches: one atomic on the
+// shared hit counter per batch of up to kWaveStage hits instead of one per
+// append.  `count` is wave-uniform; appends and flushes are wave-wide calls.
+constexpr uint32_t kWaveStage = 64;
+
+struct HitStage {
+  SeedHit *buf;  // this wave's kWaveStage slots
+  uint32_t count;
+};
+
+__device__ __forceinline__ void StageFlush(HitStage &st, SeedHit *out, uint32_t *counter,
                                            uint32_t capacity) {
+  if (st.count == 0) return;
+  const uint32_t lane = (uint32_t)__lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(counter, st.count);
+  base = __shfl(base, 0, kWave);
+  if (lane < st.count && base + lane < capacity) out[base + lane] = st.buf[lane];
+  st.count = 0;
+}
+
+__device__ __forceinline__ void StageAppend(bool pred, const SeedHit &h, HitStage &st,
+                                            SeedHit *out, uint32_t *counter,
+                                            uint32_t capacity) {
   const uint64_t mask = __ballot(pred);
   if (mask == 0) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader, kWave);
+  const uint32_t n = (uint32_t)__popcll(mask);
+  if (st.count + n > kWaveStage) StageFlush(st, out, counter, capacity);
   if (pred) {
-    const uint32_t off = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    if (off < capacity) out[off] = h;
+    const int lane = __lane_id();
+    st.buf[st.count + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = h;
   }
+  st.count += n;
 }
 
 __device__ __forceinline__ void WaveCount(bool pred, uint32_t *counter) {
@@ -151,8 +170,11 @@ __global__ void TileRangeKernel(const uint64_t *active_off, uint32_t nactive,
   range[t] = make_uint2(a, lo - a);  // lists a .. lo-1
 }
 
+// Register budgets: five waves per SIMD for the value kernel, four for the
+// representative kernel (whose member loop needs more).
 template <bool kValues>
 __global__ void __launch_bounds__(kSeedThreads)
+    __attribute__((amdgpu_waves_per_eu(kValues ? 5 : 4)))
     SeedKernel(DeviceQueryLists q, DeviceDbChunk db, SeedStageParams p,
                const ListInfo *info, const uint32_t *active,
                const uint64_t *active_off, const uint2 *tile_range, uint64_t unit_begin,
@@ -167,6 +189,8 @@ __global__ void __launch_bounds__(kSeedThreads)
   __shared__ double linv[kSeedTile];        // 1 / entries, for the unit division
   __shared__ uint32_t owner[kSeedTile];     // the tile list of each unit of the tile
   __shared__ uint32_t scan_lds[kSeedThreads / kWave];
+  __shared__ SeedHit stage_lds[kSeedThreads / kWave][kWaveStage];
+  HitStage stage{stage_lds[threadIdx.x / kWave], 0u};
   for (uint32_t i = threadIdx.x; i < kTableDim * kTableDim; i += blockDim.x) {
     utab[i] = UngappedTableEntry(p.matrix, p.nl, i);
     if (!kValues) differs[i] = DiffersEntry(p.reduced_map, i);
@@ -301,7 +325,7 @@ __global__ void __launch_bounds__(kSeedThreads)
           }
         }
       }
-      WaveAppend(pass, h, hits, counters, capacity);
+      StageAppend(pass, h, stage, hits, counters, capacity);
       if (!kValues) {
         // the members of the wave's representatives, spread evenly over its
         // lanes: item t belongs to the lane whose [first, first + count)
@@ -335,7 +359,7 @@ __global__ void __launch_bounds__(kSeedThreads)
             mp = UngappedWithTrigger(q.packed, mh.qpos, db.packed, mh.dpos, utab, mcut, mtrig) >
                  mtrig;
           }
-          WaveAppend(mp, mh, hits, counters, capacity);
+          StageAppend(mp, mh, stage, hits, counters, capacity);
         }
       }
 #pragma unroll
@@ -351,6 +375,7 @@ __global__ void __launch_bounds__(kSeedThreads)
       }
     }
   }
+  StageFlush(stage, hits, counters, capacity);
 }
 
 // One 16-byte record per list entry: query position, query id (bit 31: the

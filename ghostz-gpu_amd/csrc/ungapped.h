@@ -70,7 +70,11 @@ GZ_HD SeedWindow LoadSeedWindow(const uint64_t *packed, uint32_t pos) {
 }
 
 // UngappedSide from preloaded words: the reverse side from pos - 1
-// backwards, the forward side from pos onwards.
+// backwards, the forward side from pos onwards.  The word after the current
+// pair is read before the pair is scored, so its latency overlaps the
+// scoring; it lies inside the padded array because the pair holds no
+// delimiter before the side's last window (a side stops at the first one),
+// and the front pad and the tail hold four words each.
 template <bool kReverse>
 GZ_HD int UngappedSideFrom(const uint64_t *qw, const SeedWindow &sq, const uint64_t *dw,
                            const SeedWindow &sd, const int16_t *tab, int cutoff) {
@@ -79,6 +83,8 @@ GZ_HD int UngappedSideFrom(const uint64_t *qw, const SeedWindow &sq, const uint6
   uint32_t qi = kReverse ? sq.wi : sq.wi + 1, di = kReverse ? sd.wi : sd.wi + 1;
   int c = 0, best = 0;
   for (;;) {
+    const uint64_t qn = kReverse ? qw[qi - 1] : qw[qi + 2];
+    const uint64_t dn = kReverse ? dw[di - 1] : dw[di + 2];
     const uint64_t Q = Window12(q0, q1, sq.k), D = Window12(d0, d1, sd.k);
     bool alive = true;
 #pragma unroll
@@ -89,20 +95,24 @@ GZ_HD int UngappedSideFrom(const uint64_t *qw, const SeedWindow &sq, const uint6
       best = alive && c > best ? c : best;
       alive = alive && c - best > -cutoff;
     }
-    if (!alive) break;
+    // the pair moves on before the exit test, so the read-ahead is used on
+    // every path and stays at the top of the window
     if (kReverse) {
       q1 = q0;
       d1 = d0;
-      q0 = qw[--qi] & kCodeBitsMask;
-      d0 = dw[--di] & kCodeBitsMask;
+      q0 = qn & kCodeBitsMask;
+      d0 = dn & kCodeBitsMask;
+      --qi;
+      --di;
     } else {
       q0 = q1;
       d0 = d1;
+      q1 = qn & kCodeBitsMask;
+      d1 = dn & kCodeBitsMask;
       ++qi;
       ++di;
-      q1 = qw[qi + 1] & kCodeBitsMask;
-      d1 = dw[di + 1] & kCodeBitsMask;
     }
+    if (!alive) break;
   }
   return best;
 }
