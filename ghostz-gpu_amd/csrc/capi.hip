@@ -7,6 +7,8 @@
 #include <fstream>
 #include <memory>
 #include <string>
+#include <unordered_map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -83,6 +85,11 @@ void FinishDatabase(gz_database_s *h) {
 }
 
 }  // namespace
+
+// Output texts handed to the caller by gz_aligner_align, by data pointer,
+// until gz_free.
+static std::mutex g_texts_mu;
+static std::unordered_map<void *, std::string *> g_texts;
 
 extern "C" {
 
@@ -463,15 +470,16 @@ gz_status gz_aligner_align(gz_aligner_t h, gz_queries_t q, char **output,
                            size_t *output_length, gz_align_stats *stats) {
   GZ_REQUIRE(h && q && output && output_length, "gz_aligner_align: null argument");
   GZ_REQUIRE(h->aligner->HasDatabase(), "gz_aligner_align: no database set");
-  std::string out;
+  std::unique_ptr<std::string> out(new std::string);
   AlignStats st;
-  GZ_HIP_CHECK(h->aligner->Align(q->q, &out, &st));
-  char *buf = (char *)malloc(out.size() + 1);
-  if (!buf) return GZ_ERROR_OUT_OF_MEMORY;
-  memcpy(buf, out.data(), out.size());
-  buf[out.size()] = 0;
-  *output = buf;
-  *output_length = out.size();
+  GZ_HIP_CHECK(h->aligner->Align(q->q, out.get(), &st));
+  // the text is handed over without a copy; gz_free releases it
+  *output_length = out->size();
+  *output = &(*out)[0];
+  {
+    std::lock_guard<std::mutex> lock(g_texts_mu);
+    g_texts[*output] = out.release();
+  }
   if (stats) {
     stats->seconds_total = st.seconds_total;
     stats->seconds_query_lists = st.seconds_query_lists;
@@ -560,7 +568,19 @@ gz_status gz_align_fasta(const char *queries_path, const char *database_path,
   return st;
 }
 
-void gz_free(void *p) { free(p); }
+void gz_free(void *p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lock(g_texts_mu);
+    auto it = g_texts.find(p);
+    if (it != g_texts.end()) {
+      delete it->second;
+      g_texts.erase(it);
+      return;
+    }
+  }
+  free(p);
+}
 
 // ---- host-side preparation pieces exposed for tests ----
 int gz_seg_mask(const char *seq, char *out) {

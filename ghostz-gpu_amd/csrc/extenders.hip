@@ -336,14 +336,23 @@ __global__ void __launch_bounds__(kGappedBlock)
     const uint32_t q = qpos[i], d = dpos[i];
     int s, p0, p1;
     uint32_t nc;
-    if (GappedOneSide<kReverse>(R, P, sm, q, d, &s, &p0, &p1, &nc)) {
+    const bool ok = GappedOneSide<kReverse>(R, P, sm, q, d, &s, &p0, &p1, &nc);
+    if (ok) {
       scores[i] = s;
       qpos[i] = (uint32_t)((int)q + p0);
       dpos[i] = (uint32_t)((int)d + p1);
       if (cells) cells[i] = nc;
-    } else {
-      const uint32_t slot = atomicAdd(overflow_count, 1u);
-      overflow_list[slot] = (uint32_t)i;
+    }
+    // the lanes whose band outgrew the ring go to the overflow list, one
+    // atomic per wave
+    const uint64_t m = __ballot(!ok);
+    if (m != 0) {
+      const int lane = __lane_id();
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(overflow_count, (uint32_t)__popcll(m));
+      base = __shfl(base, leader, kWave);
+      if (!ok) overflow_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
     }
   }
 }

@@ -5,6 +5,8 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <numeric>
 #include <sstream>
@@ -20,6 +22,44 @@ double Now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
 }
+
+// Optional per-block stage trace of the batched pipeline: with the
+// environment variable GZ_PIPELINE_TRACE naming a file, one line per stage
+// run ("stage block start_ms end_ms", from the trace's creation) is appended
+// to it when the trace ends.
+class StageTrace {
+ public:
+  StageTrace() {
+    const char *p = getenv("GZ_PIPELINE_TRACE");
+    if (p && *p) path_ = p;
+    t0_ = Now();
+  }
+  ~StageTrace() {
+    if (path_.empty()) return;
+    FILE *f = fopen(path_.c_str(), "a");
+    if (!f) return;
+    fprintf(f, "# chunk\n");
+    for (const Event &e : events_)
+      fprintf(f, "%s %zu %.3f %.3f\n", e.stage, e.block, (e.t0 - t0_) * 1e3, (e.t1 - t0_) * 1e3);
+    fclose(f);
+  }
+  void Add(const char *stage, size_t block, double t0, double t1) {
+    if (path_.empty()) return;
+    std::lock_guard<std::mutex> lock(mu_);
+    events_.push_back({stage, block, t0, t1});
+  }
+
+ private:
+  struct Event {
+    const char *stage;
+    size_t block;
+    double t0, t1;
+  };
+  std::string path_;
+  double t0_ = 0;
+  std::mutex mu_;
+  std::vector<Event> events_;
+};
 
 template <class F>
 void ParallelBlocks(size_t n, int threads, size_t grain, F f) {
@@ -46,6 +86,40 @@ struct KeyedHit {
   uint32_t q, d;
   bool operator<(const KeyedHit &o) const { return hi != o.hi ? hi < o.hi : lo < o.lo; }
 };
+
+// (dpos << 32 | index) of n hits sorted by dpos into *keys (LSD radix over
+// the bytes of dpos that differ); false when two hits share a dpos.
+bool SortByPosition(const SeedHit *hits, uint32_t n, std::vector<uint64_t> *keys,
+                    std::vector<uint64_t> *tmp) {
+  keys->resize(n);
+  tmp->resize(n);
+  uint64_t *a = keys->data(), *b = tmp->data();
+  uint32_t count[4][256] = {};
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t d = hits[k].dpos;
+    a[k] = ((uint64_t)d << 32) | k;
+    ++count[0][d & 0xff];
+    ++count[1][(d >> 8) & 0xff];
+    ++count[2][(d >> 16) & 0xff];
+    ++count[3][d >> 24];
+  }
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 32 + 8 * pass;
+    if (count[pass][(a[0] >> shift) & 0xff] == n) continue;  // one digit value
+    uint32_t sum = 0;
+    for (int v = 0; v < 256; ++v) {
+      const uint32_t c = count[pass][v];
+      count[pass][v] = sum;
+      sum += c;
+    }
+    for (uint32_t k = 0; k < n; ++k) b[count[pass][(a[k] >> shift) & 0xff]++] = a[k];
+    std::swap(a, b);
+  }
+  if (a != keys->data()) std::copy(a, a + n, keys->data());
+  for (uint32_t k = 1; k < n; ++k)
+    if (((*keys)[k] >> 32) == ((*keys)[k - 1] >> 32)) return false;
+  return true;
+}
 
 #define GZ_STEP(x)            \
   do {                        \
@@ -85,11 +159,26 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   auto chain = [&](const DbChunk &chunk, const SeedHitView &hits, size_t hb, size_t qb,
                    size_t qe) {
     ParallelBlocks(qe - qb, threads, 64, [&](size_t b, size_t e) {
-      std::vector<uint64_t> scratch;
+      std::vector<uint64_t> scratch, by_d, by_d_tmp;
       std::vector<KeyedHit> keyed;
       for (size_t qi = qb + b; qi < qb + e; ++qi) {
         const uint32_t s0 = hits.qstart[qi - hb], s1 = hits.qstart[qi - hb + 1];
         if (s0 == s1) continue;
+        // ChainFilter sorts by database position with an unstable sort.
+        // When no two hits share a position that order is unique, and the
+        // enumeration order it starts from does not matter.
+        if (SortByPosition(hits.hits + s0, s1 - s0, &by_d, &by_d_tmp)) {
+          std::vector<Hit> &hv = chained[qi];
+          hv.resize(s1 - s0);
+          for (uint32_t k = 0; k < s1 - s0; ++k) {
+            const SeedHit &h = hits.hits[s0 + (uint32_t)by_d[k]];
+            hv[k].q = h.qpos - (uint32_t)coff[qi];
+            hv[k].d = h.dpos;
+          }
+          ChainFilterSorted(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix,
+                            P.nl, hv, scratch);
+          continue;
+        }
         // the (list, rv, member, entry) keys are distinct, so sorting them
         // gives exactly the one-thread enumeration order
         keyed.resize(s1 - s0);
@@ -392,6 +481,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     }
   }
   const size_t nb = blocks.size();
+  StageTrace trace;
   std::vector<std::string> block_text(nb);
   std::vector<std::vector<BuildPart> > block_parts(nb);
   std::vector<size_t> stale_chunks;
@@ -431,6 +521,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
                                     &views[k], st);
         seed_hits += views[k].count;
         sec_seed += Now() - t0;
+        trace.Add("seed", k, t0, Now());
         publish(seeded, k + 1, r);
         if (r != 0) return;
       }
@@ -442,6 +533,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
           const double t0 = Now();
           const int r = gapped(ci, blocks[k]);
           sec_gapped += Now() - t0;
+          trace.Add("gapped", k, t0, Now());
           publish(gapped_upto, k + 1, r);
           if (r != 0) return;
         }
@@ -457,6 +549,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
             }
           }
           sec_build += Now() - t0;
+          trace.Add("build_device", k - 1, t0, Now());
           publish(traced, k, r);
           if (r != 0) return;
         }
@@ -468,6 +561,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
         const double t0 = Now();
         add(ci, blocks[b]);
         sec_add += Now() - t0;
+        trace.Add("add", b, t0, Now());
         publish(added, b + 1, 0);
       }
     });
@@ -488,11 +582,13 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
           SortResults(results, blocks[b].qb, blocks[b].qe);
           const double t1 = Now();
           sec_build_host += t1 - t0;
+          trace.Add("build_host", b, t0, t1);
           std::ostringstream os;
           WriteOutputRange(os, q, db, results, blocks[b].qb, blocks[b].qe,
                            std::max(1, threads / 2));
           block_text[b] = os.str();
           sec_output += Now() - t1;
+          trace.Add("output", b, t1, Now());
         }
       });
     }
@@ -501,6 +597,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       const double t0 = Now();
       chain(chunk, views[b], blocks[b].qb, blocks[b].qb, blocks[b].qe);
       sec_chain += Now() - t0;
+      trace.Add("chain", b, t0, Now());
       publish(chained_upto, b + 1, 0);
     }
     seeder.join();
@@ -531,6 +628,9 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   st->seconds_output += sec_output;
   st->traceback_host_jobs += host_jobs;
   for (auto &r : results) st->results += r.size();
+  size_t text_bytes = output->size();
+  for (const std::string &text : block_text) text_bytes += text.size();
+  output->reserve(text_bytes);
   for (const std::string &text : block_text) output->append(text);
   return 0;
 }

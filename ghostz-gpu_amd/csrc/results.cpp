@@ -45,9 +45,15 @@ bool Connected(const uint8_t *q, const uint8_t *d, const int *sm, uint32_t nl,
 void ChainFilter(const uint8_t *q, uint32_t qlen, int cutoff, const uint8_t *d,
                  const int *sm, uint32_t nl, std::vector<Hit> &hits,
                  std::vector<uint64_t> &chains) {
+  std::sort(hits.begin(), hits.end(), DbOrder());
+  ChainFilterSorted(q, qlen, cutoff, d, sm, nl, hits, chains);
+}
+
+void ChainFilterSorted(const uint8_t *q, uint32_t qlen, int cutoff, const uint8_t *d,
+                       const int *sm, uint32_t nl, std::vector<Hit> &hits,
+                       std::vector<uint64_t> &chains) {
   // per diagonal slot: (query_end << 32) | diagonal
   chains.assign(qlen, 0);
-  std::sort(hits.begin(), hits.end(), DbOrder());
   size_t n = 0;
   for (size_t i = 0; i < hits.size(); ++i) {
     const Hit h = hits[i];

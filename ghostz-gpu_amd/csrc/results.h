@@ -79,6 +79,13 @@ void ChainFilter(const uint8_t *query, uint32_t query_length, int cutoff,
                  const uint8_t *database, const int *matrix, uint32_t nl,
                  std::vector<Hit> &hits, std::vector<uint64_t> &chain_scratch);
 
+// The filter pass of ChainFilter on hits already in database-position order
+// (the order std::sort gives; it is unique when no two hits share a
+// database position).
+void ChainFilterSorted(const uint8_t *query, uint32_t query_length, int cutoff,
+                       const uint8_t *database, const int *matrix, uint32_t nl,
+                       std::vector<Hit> &hits, std::vector<uint64_t> &chain_scratch);
+
 // AlignerCommon::AddResults (src/aligner_common.h:147-274).
 struct AlignmentPosition {
   uint32_t result_id;

@@ -5,6 +5,7 @@ Loading fails loudly when it has not been built -- there is no fallback.
 """
 import ctypes
 import os
+import time
 from ctypes import POINTER, c_char_p, c_int, c_size_t, c_uint8, c_uint32, c_uint64, c_void_p
 
 import numpy as np
@@ -494,11 +495,15 @@ class Aligner:
         check(lib().gz_aligner_set_cost_model(self.handle, ptr(flat), flat.size))
 
     def align(self, queries):
+        """Output text and stats; `last_call_seconds` is the wall time of the
+        C call alone (the copy into a Python bytes object excluded)."""
         out = c_void_p()
         n = c_size_t()
         st = AlignStats()
+        t0 = time.perf_counter()
         check(lib().gz_aligner_align(self.handle, queries.handle, ctypes.byref(out),
                                      ctypes.byref(n), ctypes.byref(st)))
+        self.last_call_seconds = time.perf_counter() - t0
         text = ctypes.string_at(out, n.value) if n.value else b""
         lib().gz_free(out)
         return text, st.as_dict()

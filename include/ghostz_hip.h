@@ -350,7 +350,8 @@ gz_status gz_aligner_set_database(gz_aligner_t a, gz_database_t db);
 /* Flat fp32 weights of the cost model (embed, w1, b1, w2, b2, w3, b3). */
 gz_status gz_aligner_set_cost_model(gz_aligner_t a, const float *weights, size_t count);
 /* AlignerGpu::Align for in-memory queries: presearch, BuildResults and
- * WriteOutput; *output is malloc'd (gz_free). */
+ * WriteOutput; *output (NUL-terminated, *output_length bytes before the NUL)
+ * belongs to the caller until gz_free. */
 gz_status gz_aligner_align(gz_aligner_t a, gz_queries_t q, char **output,
                            size_t *output_length, gz_align_stats *stats);
 uint64_t gz_aligner_traces(gz_aligner_t a, gz_gapped_trace *out, uint64_t capacity);
