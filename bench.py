@@ -35,7 +35,6 @@ for sub in ("ghostz-gpu_amd", "oracle"):
     sys.path.insert(0, os.path.join(ROOT, sub))
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 VALU ops: 256 CUs x 64 lanes x 2.4 GHz
 
 
 def log(*a):
@@ -247,40 +246,28 @@ def main():
     # roofline of the dominant kernel, from the HIP-event times of the plain
     # form's timed steps, where each kernel has the GPU to itself
     st = p_st[-1]
-    kern = {k: plain_kernels[k] for k in ("seed_rep", "seed_value", "gapped")}
+    # the seed kernels are the largest single kernels of the step (the gapped
+    # time is spread over four kernels: two sides, two ring widths)
+    kern = {k: plain_kernels[k] for k in ("seed_rep", "seed_value")}
     dom = max(kern, key=kern.get)
-    if dom == "gapped":
-        # one DP cell = 10 int32 VALU ops (3 max, 2 subtract, 1 add, 3 compares / selects,
-        # 1 X-drop test); cells counted by the plain run's traces are not available
-        # here, so the cell count comes from a counting run of the same inputs
-        cnt = gz.Aligner(db, record_traces=1)
-        _, cst = cnt.align(queries)
-        ops = 10.0 * cst["gapped_cells"]
-        launches = max(1, st["gapped_launches"])
-        avg_ms = st["gapped_kernel_ms"] / launches
-        achieved = ops / launches / (avg_ms / 1e3) / 1e12
-        roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS,
-                "unit": "Tops/s", "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
-                "kernel": "GappedKernel", "avg_launch_ms": avg_ms}
-    else:
-        k = 0 if dom == "seed_rep" else 1
-        units = st["units_rep"] if k == 0 else st["units_value"]
-        launches = max(1, st["seed_rep_launches"] if k == 0 else st["seed_value_launches"])
-        ms = st["seed_rep_kernel_ms"] if k == 0 else st["seed_value_kernel_ms"]
-        avg_ms = ms / launches
-        # algorithmic bytes per unit: 4 (entry position) + 4 (database position)
-        # + 2 x 8 (two packed database words around the seed)
-        per_unit = 24.0
-        achieved = per_unit * units / launches / (avg_ms / 1e3) / 1e9
-        name = "SeedKernel<%s>" % ("false" if k == 0 else "true")
-        measured = traffic_per_unit(name) if args.scale == 1.0 else None
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured * units / launches if measured else None,
-                "kernel": name, "avg_launch_ms": avg_ms, "units_per_launch": units / launches,
-                "algorithmic_bytes_per_unit": per_unit,
-                "traffic_bytes_per_unit": measured,
-                "traffic_source": TRAFFIC_PROFILE if measured else None}
+    k = 0 if dom == "seed_rep" else 1
+    units = st["units_rep"] if k == 0 else st["units_value"]
+    launches = max(1, st["seed_rep_launches"] if k == 0 else st["seed_value_launches"])
+    ms = st["seed_rep_kernel_ms"] if k == 0 else st["seed_value_kernel_ms"]
+    avg_ms = ms / launches
+    # algorithmic bytes per unit: 4 (entry position) + 4 (database position)
+    # + 2 x 8 (two packed database words around the seed)
+    per_unit = 24.0
+    achieved = per_unit * units / launches / (avg_ms / 1e3) / 1e9
+    name = "SeedKernel<%s>" % ("false" if k == 0 else "true")
+    measured = traffic_per_unit(name) if args.scale == 1.0 else None
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured * units / launches if measured else None,
+            "kernel": name, "avg_launch_ms": avg_ms, "units_per_launch": units / launches,
+            "algorithmic_bytes_per_unit": per_unit,
+            "traffic_bytes_per_unit": measured,
+            "traffic_source": TRAFFIC_PROFILE if measured else None}
 
     cpu = None
     if not args.no_cpu:

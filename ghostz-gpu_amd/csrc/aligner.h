@@ -76,7 +76,9 @@ class Aligner : public PresearchDevice {
   static constexpr size_t kNoChunk = ~(size_t)0;
   size_t lookup_chunk_ = kNoChunk;  // database chunk of the seed stage's lookup
   std::vector<void *> query_buffers_;
-  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  // timing events: [0, 4) the gapped stage's, [4, 6) the result build's (the
+  // two stages may run on different host threads)
+  hipEvent_t ev_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // gapped-stage workspace (grown on demand)
   void *gapped_ws_ = nullptr;
   size_t gapped_ws_cap_ = 0;

@@ -222,8 +222,8 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
   const size_t n = (size_t)nr + nf;
   if (n == 0) return 0;
   hipStream_t s = extend_stream_;
-  for (hipEvent_t &e : ev_)
-    if (!e) GZ_TRY(hipEventCreate(&e));
+  for (int i = 0; i < 4; ++i)
+    if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
   const DeviceChunk &dc = *chunks_[ci];
   if (gz_gapped_extender_set_database(gapped_, dc.packed) != GZ_OK) return (int)hipErrorUnknown;
   const GpuCostModel *model =
@@ -334,8 +334,8 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   out->resize(n);
   if (n == 0) return 0;
   hipStream_t s = extend_stream_;
-  for (hipEvent_t &e : ev_)
-    if (!e) GZ_TRY(hipEventCreate(&e));
+  for (int i = 4; i < 6; ++i)
+    if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
   const LaneCaps caps = TracebackCaps(max_query_length_, db_->max_length);
   // lanes resident at once, bounded by an 8 GB scratch budget
   const size_t budget = (size_t)8 << 30;
@@ -356,17 +356,17 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   uint8_t *scratch = (uint8_t *)tb_ws_ + a_jobs + a_outs;
   hipError_t e = hipMemcpyAsync(d_jobs, jobs.data(), n * sizeof(TracebackJob),
                                 hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipEventRecord(ev_[2], s);
+  if (e == hipSuccess) e = hipEventRecord(ev_[4], s);
   if (e == hipSuccess)
     e = TracebackLaunch(dq_.packed, chunks_[ci]->packed, d_jobs, (uint32_t)n, d_matrix_,
                         opt_.search.nl, opt_.search.gap_open, opt_.search.gap_extension, cutoff,
                         scratch, caps, (uint32_t)lanes, d_outs, s);
-  if (e == hipSuccess) e = hipEventRecord(ev_[3], s);
+  if (e == hipSuccess) e = hipEventRecord(ev_[5], s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(out->data(), d_outs, n * sizeof(TracebackOut), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   float ms = 0;
-  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[2], ev_[3]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[4], ev_[5]);
   if (e != hipSuccess) return (int)e;
   st->traceback_kernel_ms += ms;
   return 0;

@@ -435,9 +435,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     for (auto &r : results) st->results += r.size();
     st->seconds_build += Now() - t;
     t = Now();
-    std::ostringstream os;
-    WriteOutput(os, q, db, results);
-    output->append(os.str());
+    WriteOutputText(output, q, db, results, 0, nq, threads);
     st->seconds_output += Now() - t;
     return 0;
   }
@@ -482,7 +480,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   }
   const size_t nb = blocks.size();
   StageTrace trace;
-  std::vector<std::string> block_text(nb);
+  const size_t output_before = output->size();
   std::vector<std::vector<BuildPart> > block_parts(nb);
   std::vector<size_t> stale_chunks;
   bool rebuild = false;  // a block could not be built on its own
@@ -526,35 +524,40 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
         if (r != 0) return;
       }
     });
+    // the extender and the builder share the device's extension stream;
+    // their host work overlaps
     std::thread extender([&]() {
-      for (size_t k = 0; k <= nb; ++k) {
-        if (k < nb) {
-          if (!wait_for(chained_upto, k + 1)) return;
-          const double t0 = Now();
-          const int r = gapped(ci, blocks[k]);
-          sec_gapped += Now() - t0;
-          trace.Add("gapped", k, t0, Now());
-          publish(gapped_upto, k + 1, r);
-          if (r != 0) return;
-        }
-        if (last && k >= 1) {
-          if (!wait_for(added, k)) return;
+      for (size_t k = 0; k < nb; ++k) {
+        if (!wait_for(chained_upto, k + 1)) return;
+        const double t0 = Now();
+        const int r = gapped(ci, blocks[k]);
+        sec_gapped += Now() - t0;
+        trace.Add("gapped", k, t0, Now());
+        publish(gapped_upto, k + 1, r);
+        if (r != 0) return;
+      }
+    });
+    std::thread builder;
+    if (last) {
+      builder = std::thread([&]() {
+        for (size_t k = 0; k < nb; ++k) {
+          if (!wait_for(added, k + 1)) return;
           const double t0 = Now();
           int r = 0;
           if (!unsupported) {
-            r = build_device(blocks[k - 1].qb, blocks[k - 1].qe, &block_parts[k - 1]);
+            r = build_device(blocks[k].qb, blocks[k].qe, &block_parts[k]);
             if (r == PresearchDevice::kTracebackUnsupported) {
               unsupported = true;
               r = 0;
             }
           }
           sec_build += Now() - t0;
-          trace.Add("build_device", k - 1, t0, Now());
-          publish(traced, k, r);
+          trace.Add("build_device", k, t0, Now());
+          publish(traced, k + 1, r);
           if (r != 0) return;
         }
-      }
-    });
+      });
+    }
     std::thread adder([&]() {
       for (size_t b = 0; b < nb; ++b) {
         if (!wait_for(gapped_upto, b + 1)) return;
@@ -583,10 +586,8 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
           const double t1 = Now();
           sec_build_host += t1 - t0;
           trace.Add("build_host", b, t0, t1);
-          std::ostringstream os;
-          WriteOutputRange(os, q, db, results, blocks[b].qb, blocks[b].qe,
-                           std::max(1, threads / 2));
-          block_text[b] = os.str();
+          // blocks are written in query order, straight into the output
+          WriteOutputText(output, q, db, results, blocks[b].qb, blocks[b].qe, threads);
           sec_output += Now() - t1;
           trace.Add("output", b, t1, Now());
         }
@@ -602,6 +603,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     }
     seeder.join();
     extender.join();
+    if (builder.joinable()) builder.join();
     adder.join();
     if (writer.joinable()) writer.join();
     GZ_STEP(status);
@@ -611,10 +613,8 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       GZ_STEP(build_all());
       sec_build += Now() - t;
       t = Now();
-      std::fill(block_text.begin(), block_text.end(), std::string());
-      std::ostringstream os;
-      WriteOutput(os, q, db, results);
-      block_text[0] = os.str();
+      output->resize(output_before);
+      WriteOutputText(output, q, db, results, 0, nq, threads);
       sec_output += Now() - t;
     }
   }
@@ -628,10 +628,6 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   st->seconds_output += sec_output;
   st->traceback_host_jobs += host_jobs;
   for (auto &r : results) st->results += r.size();
-  size_t text_bytes = output->size();
-  for (const std::string &text : block_text) text_bytes += text.size();
-  output->reserve(text_bytes);
-  for (const std::string &text : block_text) output->append(text);
   return 0;
 }
 
@@ -659,6 +655,12 @@ int AlignQueries(const QuerySet &queries, const Database &db, const AlignerOptio
       ++j;
     }
     if (j == i) j = i + 1;
+    if (i == 0 && j == n) {
+      // one chunk holds every query: no copy
+      GZ_STEP(AlignQueryChunk(queries, db, opt, dev, output, st, traces, cc));
+      ++st->query_chunks;
+      break;
+    }
     QuerySet part;
     part.translated = queries.translated;
     part.max_length = 0;

@@ -100,9 +100,9 @@ struct GappedTrace {
 };
 
 // Device stages of one query chunk.  Every call returns 0 on success.
-// SeedHits may run on one host thread while GappedExtend, Tracebacks and
-// JobBuffer run on another; each of the two groups is called from one
-// thread at a time.
+// SeedHits, GappedExtend (with JobBuffer) and Tracebacks may run on three
+// host threads at once; each of the three is called from one thread at a
+// time.
 class PresearchDevice {
  public:
   virtual ~PresearchDevice() {}

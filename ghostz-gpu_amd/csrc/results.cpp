@@ -593,16 +593,18 @@ void BuildResults(const QuerySet &q, const Database &db, const SearchParameters 
 // ---------------------------------------------------------------------------
 void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,
                  const std::vector<std::vector<AlignmentResult> > &results) {
-  WriteOutputRange(os, q, db, results, 0, q.size(), 0);
+  std::string text;
+  WriteOutputText(&text, q, db, results, 0, q.size(), 0);
+  os << text;
 }
 
-void WriteOutputRange(std::ostream &os, const QuerySet &q, const Database &db,
-                      const std::vector<std::vector<AlignmentResult> > &results, size_t qb,
-                      size_t qe, int threads) {
+void WriteOutputText(std::string *out, const QuerySet &q, const Database &db,
+                     const std::vector<std::vector<AlignmentResult> > &results, size_t qb,
+                     size_t qe, int threads) {
   const KarlinParameters gk = Statistics::GappedBlosum62();
   const double alpha = 1.9, beta = -30;
   const size_t nq = qe;
-  const size_t block = 2048;
+  const size_t block = 256;
   const size_t nblocks = (qe - qb + block - 1) / block;
   std::vector<std::string> text(nblocks);
   std::atomic<size_t> next(0);
@@ -665,7 +667,10 @@ void WriteOutputRange(std::ostream &os, const QuerySet &q, const Database &db,
   for (int t = 1; t < T; ++t) pool.emplace_back(work);
   work();
   for (auto &t : pool) t.join();
-  for (auto &t : text) os << t;
+  size_t bytes = out->size();
+  for (const auto &t : text) bytes += t.size();
+  if (out->capacity() < bytes) out->reserve(std::max(bytes, 2 * out->capacity()));
+  for (const auto &t : text) out->append(t);
 }
 
 }  // namespace gz

@@ -156,12 +156,13 @@ TracebackOut TracebackOneHost(const uint8_t *query, uint32_t query_length,
                               const uint8_t *database, uint32_t hit_q, uint32_t hit_d,
                               const SearchParameters &p, int cutoff);
 
-// AlignerCommon::WriteOutput (src/aligner_common.h:345-397); the range form
-// writes queries [qb, qe) on up to `threads` threads (0: all cores).
+// AlignerCommon::WriteOutput (src/aligner_common.h:345-397); the text form
+// appends the lines of queries [qb, qe) to *out, formatted on up to
+// `threads` threads (0: all cores).
 void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,
                  const std::vector<std::vector<AlignmentResult> > &results);
-void WriteOutputRange(std::ostream &os, const QuerySet &q, const Database &db,
-                      const std::vector<std::vector<AlignmentResult> > &results, size_t qb,
-                      size_t qe, int threads);
+void WriteOutputText(std::string *out, const QuerySet &q, const Database &db,
+                     const std::vector<std::vector<AlignmentResult> > &results, size_t qb,
+                     size_t qe, int threads);
 
 }  // namespace gz
