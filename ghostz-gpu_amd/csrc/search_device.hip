@@ -170,6 +170,42 @@ __global__ void TileRangeKernel(const uint64_t *active_off, uint32_t nactive,
   range[t] = make_uint2(a, lo - a);  // lists a .. lo-1
 }
 
+// The `kReverse` sides of the lane's items whose flag is set, one twelve-code
+// window per round: the lane starts its next item as soon as the current
+// walk stops.  Rounds repeat while any lane of the wave has a walk left.
+template <bool kReverse>
+__device__ __forceinline__ void WalkItems(const uint64_t *qw, const uint64_t *dw,
+                                          const int16_t *tab,
+                                          const SeedWindow (&sq)[kSeedItems],
+                                          const SeedWindow (&sd)[kSeedItems],
+                                          const bool (&todo)[kSeedItems],
+                                          const int (&cutoff)[kSeedItems],
+                                          int (&score)[kSeedItems]) {
+  static_assert(kSeedItems == 2, "the walk switches between two items");
+  score[0] = 0;
+  score[1] = 0;
+  int cur = todo[0] ? 0 : (todo[1] ? 1 : 2);
+  SideWalk w = SideStart<kReverse>(cur == 1 ? sq[1] : sq[0], cur == 1 ? sd[1] : sd[0]);
+  int cut = cur == 1 ? cutoff[1] : cutoff[0];
+  for (;;) {
+    const bool active = cur < 2;
+    if (!__any(active)) break;
+    if (active && !SideStep<kReverse>(qw, dw, tab, cut, w)) {
+      if (cur == 0) {
+        score[0] = w.best;
+        cur = todo[1] ? 1 : 2;
+        if (cur == 1) {
+          w = SideStart<kReverse>(sq[1], sd[1]);
+          cut = cutoff[1];
+        }
+      } else {
+        score[1] = w.best;
+        cur = 2;
+      }
+    }
+  }
+}
+
 // Register budgets: five waves per SIMD for the value kernel, four for the
 // representative kernel (whose member loop needs more).
 template <bool kValues>
@@ -278,46 +314,64 @@ __global__ void __launch_bounds__(kSeedThreads)
     SeedWindow wq[kSeedItems], wd[kSeedItems];
 #pragma unroll
     for (int j = 0; j < kSeedItems; ++j) {
+      wq[j] = SeedWindow{0, 0, 0, 0, 0};
+      wd[j] = SeedWindow{0, 0, 0, 0, 0};
       if (valid[j]) {
         wq[j] = LoadSeedWindow(q.packed, rec[j].x);
         wd[j] = LoadSeedWindow(db.packed, dposv[j]);
       }
     }
-    // items are evaluated in turn from slot 0; the next item's data moves
-    // down afterwards, so every index is static and stays in registers
-#pragma unroll 1
+    // which items extend: every valid value unit; a representative unit
+    // when its reduced-alphabet distance is within the threshold
+    bool need[kSeedItems];
+    int cutv[kSeedItems], trigv[kSeedItems], dist[kSeedItems];
+#pragma unroll
     for (int j = 0; j < kSeedItems; ++j) {
+      cutv[j] = (int)rec[j].z;
+      trigv[j] = (int)rec[j].w;
+      dist[j] = 0;
+      need[j] = valid[j];
+      if (!kValues && valid[j] && (rec[j].y >> 31) != 0) {
+        dist[j] = p.subsequence_length <= kCodesPerWord
+                      ? DistanceFrom(wq[j], wd[j], p.subsequence_length, differs)
+                      : DistanceWindow(q.packed, rec[j].x, db.packed, dposv[j],
+                                       p.subsequence_length, differs);
+        need[j] = dist[j] <= p.distance_threshold;
+      }
+    }
+    // the reverse sides of the lane's items, then the forward sides of those
+    // whose reverse score is within the trigger: a lane moves on to its next
+    // item as soon as the current walk stops, so a wave runs as long as its
+    // longest sum of walks rather than the sum of its longest walks
+    int sum[kSeedItems];
+    WalkItems<true>(q.packed, db.packed, utab, wq, wd, need, cutv, sum);
+    bool fwd[kSeedItems];
+    int fsum[kSeedItems];
+#pragma unroll
+    for (int j = 0; j < kSeedItems; ++j) fwd[j] = need[j] && sum[j] <= trigv[j];
+    WalkItems<false>(q.packed, db.packed, utab, wq, wd, fwd, cutv, fsum);
+#pragma unroll
+    for (int j = 0; j < kSeedItems; ++j) {
+      if (fwd[j]) sum[j] += fsum[j];
       SeedHit h = {0, 0, 0, 0, 0, 0, 0};
       bool pass = false;
       bool member_ok = false;
       uint32_t mb = 0, me = 0;
       int trig = 0, cut = 0;
-      if (valid[0]) {
-        const uint32_t lo = lidx[0], idx = ridx[0];
-        const uint32_t qpos = rec[0].x;
-        const uint32_t qid = rec[0].y & 0x7fffffffu;
-        cut = (int)rec[0].z;
-        trig = (int)rec[0].w;
-        h.dpos = dposv[0];
-        h.qid = qid;
-        h.qpos = qpos;
+      if (valid[j]) {
+        const uint32_t lo = lidx[j], idx = ridx[j];
+        cut = cutv[j];
+        trig = trigv[j];
+        h.dpos = dposv[j];
+        h.qid = rec[j].y & 0x7fffffffu;
+        h.qpos = rec[j].x;
         h.list = lrank[lo];
-        h.entry = eidx[0];
-        if (kValues) {
-          h.rv = lnrep[lo] + idx;
-          pass = UngappedWithTrigger(q.packed, wq[0], db.packed, wd[0], utab, cut, trig) > trig;
-        } else {
-          h.rv = idx;
-          const bool check = (rec[0].y >> 31) != 0;
-          int d = 0;
-          if (check)
-            d = p.subsequence_length <= kCodesPerWord
-                    ? DistanceFrom(wq[0], wd[0], p.subsequence_length, differs)
-                    : DistanceWindow(q.packed, qpos, db.packed, h.dpos, p.subsequence_length,
-                                     differs);
-          if (d <= p.distance_threshold)
-            pass = UngappedWithTrigger(q.packed, wq[0], db.packed, wd[0], utab, cut, trig) > trig;
-          member_ok = !check || max(d - p.max_mismatches, 0) <= p.distance_threshold;
+        h.entry = eidx[j];
+        h.rv = kValues ? lnrep[lo] + idx : idx;
+        pass = need[j] && sum[j] > trig;
+        if (!kValues) {
+          const bool check = (rec[j].y >> 31) != 0;
+          member_ok = !check || max(dist[j] - p.max_mismatches, 0) <= p.distance_threshold;
           if (member_ok) {
             const uint32_t ri = lbeg[lo] + idx;
             mb = db.mem_off[ri];
@@ -361,17 +415,6 @@ __global__ void __launch_bounds__(kSeedThreads)
           }
           StageAppend(mp, mh, stage, hits, counters, capacity);
         }
-      }
-#pragma unroll
-      for (int k = 0; k + 1 < kSeedItems; ++k) {
-        valid[k] = valid[k + 1];
-        lidx[k] = lidx[k + 1];
-        ridx[k] = ridx[k + 1];
-        eidx[k] = eidx[k + 1];
-        dposv[k] = dposv[k + 1];
-        rec[k] = rec[k + 1];
-        wq[k] = wq[k + 1];
-        wd[k] = wd[k + 1];
       }
     }
   }

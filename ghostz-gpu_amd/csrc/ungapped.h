@@ -69,52 +69,83 @@ GZ_HD SeedWindow LoadSeedWindow(const uint64_t *packed, uint32_t pos) {
   return s;
 }
 
-// UngappedSide from preloaded words: the reverse side from pos - 1
-// backwards, the forward side from pos onwards.  The word after the current
-// pair is read before the pair is scored, so its latency overlaps the
-// scoring; it lies inside the padded array because the pair holds no
-// delimiter before the side's last window (a side stops at the first one),
-// and the front pad and the tail hold four words each.
+// One side of an ungapped X-drop extension walked a window (twelve
+// residues) at a time: the reverse side from pos - 1 backwards, the forward
+// side from pos onwards.  The word after the current pair is read before the
+// pair is scored, so its latency overlaps the scoring; it lies inside the
+// padded array because the pair holds no delimiter before the side's last
+// window (a side stops at the first one), and the front pad and the tail hold
+// four words each.
+struct SideWalk {
+  uint64_t q0, q1, d0, d1;  // the current word pair of each sequence
+  uint32_t qi, di;          // word index of q0 / d0
+  uint32_t qk, dk;          // code offset of the window in the pair
+  int c, best;
+};
+
+template <bool kReverse>
+GZ_HD SideWalk SideStart(const SeedWindow &sq, const SeedWindow &sd) {
+  SideWalk w;
+  w.q0 = kReverse ? sq.w0 : sq.w1;
+  w.q1 = kReverse ? sq.w1 : sq.w2;
+  w.d0 = kReverse ? sd.w0 : sd.w1;
+  w.d1 = kReverse ? sd.w1 : sd.w2;
+  w.qi = kReverse ? sq.wi : sq.wi + 1;
+  w.di = kReverse ? sd.wi : sd.wi + 1;
+  w.qk = sq.k;
+  w.dk = sd.k;
+  w.c = 0;
+  w.best = 0;
+  return w;
+}
+
+// Scores one window; false once the side has stopped (w.best is then its
+// score).
+template <bool kReverse>
+GZ_HD bool SideStep(const uint64_t *qw, const uint64_t *dw, const int16_t *tab, int cutoff,
+                    SideWalk &w) {
+  const uint64_t qn = kReverse ? qw[w.qi - 1] : qw[w.qi + 2];
+  const uint64_t dn = kReverse ? dw[w.di - 1] : dw[w.di + 2];
+  const uint64_t Q = Window12(w.q0, w.q1, w.qk), D = Window12(w.d0, w.d1, w.dk);
+  int c = w.c, best = w.best;
+  bool alive = true;
+#pragma unroll
+  for (int j = 0; j < (int)kCodesPerWord; ++j) {
+    const int sh = (int)kCodeBits * (kReverse ? (int)kCodesPerWord - 1 - j : j);
+    const uint32_t a = (uint32_t)(Q >> sh) & 31u, b = (uint32_t)(D >> sh) & 31u;
+    c += tab[(b << 5) | a];  // c is not used once the side has stopped
+    best = alive && c > best ? c : best;
+    alive = alive && c - best > -cutoff;
+  }
+  w.c = c;
+  w.best = best;
+  // the pair moves on before the caller's exit test, so the read-ahead is
+  // used on every path and stays at the top of the window
+  if (kReverse) {
+    w.q1 = w.q0;
+    w.d1 = w.d0;
+    w.q0 = qn & kCodeBitsMask;
+    w.d0 = dn & kCodeBitsMask;
+    --w.qi;
+    --w.di;
+  } else {
+    w.q0 = w.q1;
+    w.d0 = w.d1;
+    w.q1 = qn & kCodeBitsMask;
+    w.d1 = dn & kCodeBitsMask;
+    ++w.qi;
+    ++w.di;
+  }
+  return alive;
+}
+
 template <bool kReverse>
 GZ_HD int UngappedSideFrom(const uint64_t *qw, const SeedWindow &sq, const uint64_t *dw,
                            const SeedWindow &sd, const int16_t *tab, int cutoff) {
-  uint64_t q0 = kReverse ? sq.w0 : sq.w1, q1 = kReverse ? sq.w1 : sq.w2;
-  uint64_t d0 = kReverse ? sd.w0 : sd.w1, d1 = kReverse ? sd.w1 : sd.w2;
-  uint32_t qi = kReverse ? sq.wi : sq.wi + 1, di = kReverse ? sd.wi : sd.wi + 1;
-  int c = 0, best = 0;
-  for (;;) {
-    const uint64_t qn = kReverse ? qw[qi - 1] : qw[qi + 2];
-    const uint64_t dn = kReverse ? dw[di - 1] : dw[di + 2];
-    const uint64_t Q = Window12(q0, q1, sq.k), D = Window12(d0, d1, sd.k);
-    bool alive = true;
-#pragma unroll
-    for (int j = 0; j < (int)kCodesPerWord; ++j) {
-      const int sh = (int)kCodeBits * (kReverse ? (int)kCodesPerWord - 1 - j : j);
-      const uint32_t a = (uint32_t)(Q >> sh) & 31u, b = (uint32_t)(D >> sh) & 31u;
-      c += tab[(b << 5) | a];  // c is not used once the side has stopped
-      best = alive && c > best ? c : best;
-      alive = alive && c - best > -cutoff;
-    }
-    // the pair moves on before the exit test, so the read-ahead is used on
-    // every path and stays at the top of the window
-    if (kReverse) {
-      q1 = q0;
-      d1 = d0;
-      q0 = qn & kCodeBitsMask;
-      d0 = dn & kCodeBitsMask;
-      --qi;
-      --di;
-    } else {
-      q0 = q1;
-      d0 = d1;
-      q1 = qn & kCodeBitsMask;
-      d1 = dn & kCodeBitsMask;
-      ++qi;
-      ++di;
-    }
-    if (!alive) break;
+  SideWalk w = SideStart<kReverse>(sq, sd);
+  while (SideStep<kReverse>(qw, dw, tab, cutoff, w)) {
   }
-  return best;
+  return w.best;
 }
 
 // SeedSearcher::UngappedExtend: reverse from (qpos-1, dpos-1), forward from
