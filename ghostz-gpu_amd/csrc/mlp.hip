@@ -221,7 +221,8 @@ inline unsigned Blocks(size_t n, unsigned t = 256) { return (unsigned)((n + t - 
 // of lane l holds C[8 * (r / 4) + 4 * (l / 32) + (r % 4)][l % 32].
 //
 // Layers are computed transposed (hidden unit = row, seed = column):
-//   layer 1: C1[h][seed] = W1[h][:] . x[seed][:], A = W1 (LDS, [k/8][64][8]),
+//   layer 1: C1[h][seed] = W1[h][:] . x[seed][:], A = W1 (LDS, [k/8][64][8],
+//            staged a quarter of the inputs at a time),
 //            B = the residue embeddings of 32 seeds (one 16-byte LDS read);
 //   layer 2: C2[n][seed] = W2[n][:] . h1[:][seed], whose B operand is the
 //            lane's own layer-1 accumulators (a lane holds one seed's column):
@@ -235,6 +236,8 @@ static_assert(kCostH1 == 64 && kCostH2 == 64, "the fused inference kernel is 64 
 constexpr int kInferWaves = 8;
 constexpr int kInferBlocks = 4;                            // 32-seed columns per wave
 constexpr int kInferTile = kInferWaves * 32 * kInferBlocks;  // seeds per block pass
+constexpr int kW1Part = kCostIn / 4;                  // W1 inputs staged at a time
+constexpr int kVPerPart = kW1Part / kCostEmbed / 16;  // 16-residue steps per quarter
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -250,15 +253,12 @@ __global__ void __launch_bounds__(kInferWaves * 64)
     InferKernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h, const uint16_t *embh,
                 const float *b1, const uint16_t *w2h, const float *b2, const float *w3,
                 const float *b3, float *y) {
-  __shared__ __align__(16) uint16_t w1[kCostIn * kCostH1];   // [k/8][64][8], 128 KB
+  // one quarter of W1's inputs at a time ([k/8][64][8], 32 KB), so a block
+  // fits on a CU beside other kernels' blocks
+  __shared__ __align__(16) uint16_t w1[kW1Part * kCostH1];
   __shared__ __align__(16) uint16_t w2[4 * 2 * 64 * 8];      // [t][row block][lane][8]
   __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
   __shared__ float b1s[kCostH1], b2s[kCostH2], w3s[kCostH2];
-  for (int idx = threadIdx.x; idx < kCostH1 * (kCostIn / 8); idx += blockDim.x) {
-    const int h = idx / (kCostIn / 8), k8 = idx % (kCostIn / 8);
-    *reinterpret_cast<uint4 *>(&w1[(k8 * kCostH1 + h) * 8]) =
-        *reinterpret_cast<const uint4 *>(w1h + (size_t)h * kCostIn + k8 * 8);
-  }
   for (int idx = threadIdx.x; idx < 4 * 2 * 64 * 8; idx += blockDim.x) {
     const int j = idx & 7, l = (idx >> 3) & 63, nb = (idx >> 9) & 1, t = idx >> 10;
     const int row = 32 * nb + (l & 31);
@@ -299,6 +299,17 @@ __global__ void __launch_bounds__(kInferWaves * 64)
         for (int r = 0; r < 16; ++r) acc[hb][c][r] = 0.f;
 #pragma unroll 1
     for (int v = 0; v < kCostResidues / 16; ++v) {
+      if (v % kVPerPart == 0) {
+        // the next quarter of W1 (every wave has finished with the last)
+        const int part = v / kVPerPart;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kCostH1 * (kW1Part / 8); idx += blockDim.x) {
+          const int h = idx / (kW1Part / 8), k8 = idx % (kW1Part / 8);
+          *reinterpret_cast<uint4 *>(&w1[(k8 * kCostH1 + h) * 8]) = *reinterpret_cast<const uint4 *>(
+              w1h + (size_t)h * kCostIn + (size_t)part * kW1Part + k8 * 8);
+        }
+        __syncthreads();
+      }
       uint4 cw[kInferBlocks];
 #pragma unroll
       for (int c = 0; c < kInferBlocks; ++c) {
@@ -311,7 +322,7 @@ __global__ void __launch_bounds__(kInferWaves * 64)
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         // residue 16 v + 2 q + kq: byte 2 (q & 1) + kq of word q / 2
-        const int kb = v * 16 + 2 * q + kq;
+        const int kb = (v % kVPerPart) * 16 + 2 * q + kq;  // in the staged quarter
         const bf16x8 a0 = Load8(&w1[(kb * kCostH1 + r32) * 8]);
         const bf16x8 a1 = Load8(&w1[(kb * kCostH1 + 32 + r32) * 8]);
 #pragma unroll
