@@ -15,6 +15,8 @@
 //                            whose band outgrows W go to a wide-ring pass.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -167,12 +169,40 @@ __global__ void SeedGatherKernel(size_t size, const uint32_t *run_index,
 // gapped X-drop, one seed per lane
 // ---------------------------------------------------------------------------
 struct LaneRingLds {
+  static constexpr bool kShort = false;
   int2 *cells;     // [W][64]
   uint8_t *codes;  // [W][64]
   uint32_t mask;
   int lane;
-  __device__ __forceinline__ int2 &Cell(int i) {
-    return cells[(i & mask) * kWave + lane];
+  __device__ __forceinline__ int2 Get(int i) const { return cells[(i & mask) * kWave + lane]; }
+  __device__ __forceinline__ void Set(int i, int2 v) { cells[(i & mask) * kWave + lane] = v; }
+  __device__ __forceinline__ uint8_t &Code(int i) {
+    return codes[(i & mask) * kWave + lane];
+  }
+};
+
+// 16-bit cells (half the LDS of LaneRingLds, so more waves per CU).  Exact
+// under two conditions.  The host uses it only when the cutoff is at most
+// 8192 and gap open + extension at most 1024; then every cell value that is
+// compared is above -16384 + the gap extension, and values stored below
+// -16384 (the band's first gap column keeps decreasing row after row) can
+// be clamped there without changing a comparison.  And the extension leaves
+// the ring (as when its band outgrows it) once its best score passes 16384,
+// so stored values stay below 16384 + 127, the largest int8 matrix entry.
+struct LaneRingLds16 {
+  static constexpr bool kShort = true;
+  static constexpr int kLow = -16384;
+  short2 *cells;   // [W][64]
+  uint8_t *codes;  // [W][64]
+  uint32_t mask;
+  int lane;
+  __device__ __forceinline__ int2 Get(int i) const {
+    const short2 v = cells[(i & mask) * kWave + lane];
+    return make_int2(v.x, v.y);
+  }
+  __device__ __forceinline__ void Set(int i, int2 v) {
+    cells[(i & mask) * kWave + lane] =
+        make_short2((short)(v.x > kLow ? v.x : kLow), (short)(v.y > kLow ? v.y : kLow));
   }
   __device__ __forceinline__ uint8_t &Code(int i) {
     return codes[(i & mask) * kWave + lane];
@@ -180,10 +210,12 @@ struct LaneRingLds {
 };
 
 struct LaneRingGlobal {
+  static constexpr bool kShort = false;
   int2 *cells;     // [W]
   uint8_t *codes;  // [W]
   uint32_t mask;
-  __device__ __forceinline__ int2 &Cell(int i) { return cells[i & mask]; }
+  __device__ __forceinline__ int2 Get(int i) const { return cells[i & mask]; }
+  __device__ __forceinline__ void Set(int i, int2 v) { cells[i & mask] = v; }
   __device__ __forceinline__ uint8_t &Code(int i) { return codes[i & mask]; }
 };
 
@@ -221,19 +253,19 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
   const int cutoff = P.cutoff;
   int array_start = 0, array_end = 0, ai;
   int score = -gap_init;
-  R.Cell(0) = make_int2(0, -gap_init);
+  R.Set(0, make_int2(0, -gap_init));
   int col = 1;
   for (ai = 1; c != P.delim; ++ai) {
     if (score < -cutoff) break;
     if (ai >= W) return false;
-    R.Cell(ai) = make_int2(score, score - gap_init);
+    R.Set(ai, make_int2(score, score - gap_init));
     R.Code(ai) = (uint8_t)c;
     score -= P.gap_ext;
     c = load_code(++col);
   }
   if (c != P.delim) {
     if (ai >= W) return false;
-    R.Cell(ai) = make_int2(kInitScore, kInitScore);
+    R.Set(ai, make_int2(kInitScore, kInitScore));
     R.Code(ai) = (uint8_t)c;
     array_end = ai + 1;
   } else {
@@ -242,26 +274,28 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
   int max_score = 0, max_p0 = -inc, max_p1 = -inc;
   uint32_t cells = 0;
   for (int row = 0; d != P.delim; ++row, d = dc.Next()) {
+    // a row's values exceed its starting best by at most one matrix entry
+    if (Ring::kShort && max_score > 16384) return false;
     const int8_t *mrow = sm + d * kTableDim;
-    int2 first = R.Cell(array_start);
+    int2 first = R.Get(array_start);
     int prev = first.x;
     first.x = (max_score - first.y > cutoff) ? kInitScore : first.y;
     int gap_row = first.x - gap_init;
     first.y -= P.gap_ext;
-    R.Cell(array_start) = first;
+    R.Set(array_start, first);
     int last = array_start;
     const int as0 = array_start;
     for (ai = as0 + 1; ai < array_end; ++ai) {
       ++cells;
       int s = prev + mrow[R.Code(ai)];
-      int2 cell = R.Cell(ai);
+      int2 cell = R.Get(ai);
       prev = cell.x;
       const int gap_col = cell.y;
       s = (s < gap_col) ? gap_col : s;
       s = (s < gap_row) ? gap_row : s;
       if (max_score - s > cutoff) {
         if (array_start + 1 == ai) ++array_start;
-        R.Cell(ai) = make_int2(kInitScore, kInitScore);
+        R.Set(ai, make_int2(kInitScore, kInitScore));
         gap_row = kInitScore;
       } else {
         last = ai;
@@ -271,7 +305,7 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
           max_p1 = row * inc;
         }
         const int ng = s - gap_init;
-        R.Cell(ai) = make_int2(s, (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext);
+        R.Set(ai, make_int2(s, (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext));
         gap_row = (ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext;
       }
     }
@@ -285,7 +319,7 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
       c = load_code(p0col);
       while (gap_row >= (max_score - cutoff) && c != P.delim) {
         if (array_end - array_start >= W) return false;
-        R.Cell(array_end) = make_int2(gap_row, gap_row - gap_init);
+        R.Set(array_end, make_int2(gap_row, gap_row - gap_init));
         R.Code(array_end) = (uint8_t)c;
         gap_row -= P.gap_ext;
         ++array_end;
@@ -297,7 +331,7 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
       // the appended column's code: cached if it was inside the band,
       // otherwise it is the column just inspected
       if (array_end == p0col) R.Code(array_end) = (uint8_t)c;
-      R.Cell(array_end) = make_int2(kInitScore, kInitScore);
+      R.Set(array_end, make_int2(kInitScore, kInitScore));
       ++array_end;
     }
   }
@@ -313,18 +347,20 @@ constexpr int kGappedBlock = 64;
 // Jobs [begin, end), or the `*list_count` jobs of `list` when list is set;
 // a job whose band outgrows the W-column ring is appended to the overflow
 // list for the next, wider ring.
-template <bool kReverse, int W>
+template <bool kReverse, int W, bool kShort>
 __global__ void __launch_bounds__(kGappedBlock)
     GappedKernel(size_t begin, size_t end, const uint32_t *list, const uint32_t *list_count,
                  GappedParams P, const int *matrix, uint32_t nl, uint32_t *qpos,
                  uint32_t *dpos, int *scores, uint32_t *cells, uint32_t *overflow_count,
                  uint32_t *overflow_list) {
+  using Ring = typename std::conditional<kShort, LaneRingLds16, LaneRingLds>::type;
+  using Cell = typename std::conditional<kShort, short2, int2>::type;
   __shared__ int8_t sm[kTableDim * kTableDim];
-  __shared__ int2 ring_cells[W * kWave];
+  __shared__ Cell ring_cells[W * kWave];
   __shared__ uint8_t ring_codes[W * kWave];
   StageScoreTable(sm, matrix, nl);
   __syncthreads();
-  LaneRingLds R;
+  Ring R;
   R.cells = ring_cells;
   R.codes = ring_codes;
   R.mask = W - 1;
@@ -789,25 +825,33 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
   // jobs [0, wide_first) start on the 64-column ring, the rest on the
   // narrow one; the 64-column ring then takes the narrow ring's overflow
   if (wide_first > 0)
-    hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing>),
+    hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing, false>),
                        dim3(GridFor(wide_first, kGappedBlock, 256u * 32)), dim3(kGappedBlock),
                        0, s, (size_t)0, wide_first, (const uint32_t *)nullptr,
                        (const uint32_t *)nullptr, P, h->matrix, h->number_letters, d_q, d_d,
                        d_scores, d_cells, wide_out, h->overflow_list2);
   if (size > wide_first) {
     const dim3 grid(GridFor(size - wide_first, kGappedBlock, 256u * 64));
-    if (h->narrow == 16)
-      hipLaunchKernelGGL((GappedKernel<kReverse, 16>), grid, dim3(kGappedBlock), 0, s,
-                         wide_first, size, (const uint32_t *)nullptr, (const uint32_t *)nullptr,
-                         P, h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells,
-                         narrow_out, h->overflow_list);
-    else
-      hipLaunchKernelGGL((GappedKernel<kReverse, 32>), grid, dim3(kGappedBlock), 0, s,
-                         wide_first, size, (const uint32_t *)nullptr, (const uint32_t *)nullptr,
-                         P, h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells,
-                         narrow_out, h->overflow_list);
+    // 16-bit cells in the first ring when the cutoff and gap costs allow them
+    // (LaneRingLds16); bands whose scores grow past them move on like bands
+    // that outgrow the ring
+    const bool short_cells = P.cutoff <= 8192 && P.gap_ext >= 0 &&
+                             P.gap_open + P.gap_ext <= 1024;
+#define GZ_NARROW(WIDTH, SHORT)                                                              \
+  hipLaunchKernelGGL((GappedKernel<kReverse, WIDTH, SHORT>), grid, dim3(kGappedBlock), 0, s, \
+                     wide_first, size, (const uint32_t *)nullptr, (const uint32_t *)nullptr, \
+                     P, h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells,           \
+                     narrow_out, h->overflow_list)
+    if (h->narrow == 16) {
+      if (short_cells) GZ_NARROW(16, true);
+      else GZ_NARROW(16, false);
+    } else {
+      if (short_cells) GZ_NARROW(32, true);
+      else GZ_NARROW(32, false);
+    }
+#undef GZ_NARROW
   }
-  hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing>), dim3(256u * 4), dim3(kGappedBlock),
+  hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing, false>), dim3(256u * 4), dim3(kGappedBlock),
                      0, s, (size_t)0, (size_t)0, (const uint32_t *)h->overflow_list,
                      (const uint32_t *)narrow_out, P, h->matrix, h->number_letters, d_q, d_d,
                      d_scores, d_cells, wide_out, h->overflow_list2);
