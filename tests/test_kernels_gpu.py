@@ -266,6 +266,28 @@ def test_gapped_kernel_wide_band(dev):
         assert np.array_equal(got, want)
 
 
+def test_gapped_kernel_scores_past_16_bits(dev):
+    """Identical 7,000-residue sequences seeded in the middle: each side
+    scores past 16384, where the 16-bit first ring hands the band on to the
+    32-bit ring; the answers stay the oracle's."""
+    rng = np.random.default_rng(44)
+    L = 7000
+    seq = rng.integers(0, 20, L).astype(np.uint8)
+    q = np.concatenate([[DELIM], seq, [DELIM]]).astype(np.uint8)
+    d = np.concatenate([[DELIM], seq, [DELIM], rng.integers(0, 20, 300).astype(np.uint8),
+                        [DELIM]]).astype(np.uint8)
+    qpos = np.array([3500, 3400, 6990, 20], np.uint32)
+    dpos = np.array([3500, 3400, 6990, 20], np.uint32)
+    for reverse in (False, True):
+        a, b = (qpos - 1, dpos - 1) if reverse else (qpos, dpos)
+        hs, hq, hd = _gapped_gpu(dev, q, d, a, b, reverse, 52)
+        want = _gapped_oracle(q, d, a, b, reverse, 52, dev["matrix"], dev["nl"])
+        got = np.stack([hs, hq.astype(np.int64), hd.astype(np.int64)], 1)
+        say("past 16 bits: reverse", reverse, "scores", list(hs))
+        assert got[:2, 0].min() > 16384
+        assert np.array_equal(got, want)
+
+
 def test_gapped_kernel_reference_gpu_test_answers(dev):
     """test/gapped_extender_gpu_test.cu:40-253 on the DNA test alphabet."""
     D = {"A": 0, "C": 1, "G": 2, "T": 3}
