@@ -337,9 +337,10 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   for (int i = 4; i < 6; ++i)
     if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
   const LaneCaps caps = TracebackCaps(max_query_length_, db_->max_length);
-  // lanes resident at once, bounded by an 8 GB scratch budget
-  const size_t budget = (size_t)8 << 30;
-  size_t lanes = std::min<size_t>((n + 63) / 64 * 64, (size_t)1 << 16);
+  // lanes resident at once (up to seven 64-lane blocks per CU), bounded by a
+  // 12 GB scratch budget
+  const size_t budget = (size_t)12 << 30;
+  size_t lanes = std::min<size_t>((n + 63) / 64 * 64, (size_t)7 * 64 * 256);
   lanes = std::max<size_t>(64, std::min(lanes, budget / caps.bytes_per_lane / 64 * 64));
   const size_t a_jobs = AlignUp(n * sizeof(TracebackJob), 256);
   const size_t a_outs = AlignUp(n * sizeof(TracebackOut), 256);

@@ -31,22 +31,48 @@ constexpr uint8_t kSub = 0, kGap0 = 1, kGap1 = 2, kExt0 = 4, kExt1 = 8;
 constexpr uint8_t kEditGap0 = 0, kEditSub = 1, kEditGap1 = 2;
 constexpr uint8_t kUnwritten = 0xFF;
 constexpr int kTbThreads = 64;
-// Query sides of up to kTbLdsCols - 1 residues keep their DP columns in LDS,
-// column-major over the block's lanes ([column][lane], 32 KB per block).
+// Query sides of up to kTbLdsCols - 2 residues keep their DP columns in LDS,
+// column-major over the block's lanes ([column][lane]): 16-bit cells (16 KB
+// per block) when the cutoff and gap costs allow them, else 32-bit ones.
 constexpr uint32_t kTbLdsCols = 64;
+constexpr int kTbLow = -16384;
 
 __shared__ int2 tb_dp[kTbLdsCols * kTbThreads];
+// 16-bit columns.  Exact when the cutoff is at most 8192 and gap open +
+// extension at most 1024 (the launch checks): a side of at most 62 residues
+// scores at most 62 x 127, and values stored below -16384 (the band's first
+// gap column keeps decreasing) are clamped there, which changes no comparison
+// and no recorded op, since every value compared with them is above
+// -16384 + the gap extension.
+__shared__ short2 tb_dp16[kTbLdsCols * kTbThreads];
 // The query side's codes by DP column: column a (a >= 1) holds the code at
 // q0 + (a - 1) inc, the delimiter closing the side included.
 __shared__ uint8_t tb_qc[kTbLdsCols * kTbThreads];
 
-// DP column access: the lane's LDS column set, or its global scratch.
-template <bool kLds>
+// DP column access: the lane's LDS column set (32- or 16-bit cells), or its
+// global scratch.
+template <bool kLds, bool kShort>
 struct DpCells {
   int2 *global;
-  __device__ __forceinline__ int2 &operator[](int i) const {
-    if constexpr (kLds) return tb_dp[i * kTbThreads + (int)threadIdx.x];
-    else return global[i];
+  __device__ __forceinline__ int2 Get(int i) const {
+    if constexpr (kLds && kShort) {
+      const short2 v = tb_dp16[i * kTbThreads + (int)threadIdx.x];
+      return make_int2(v.x, v.y);
+    } else if constexpr (kLds) {
+      return tb_dp[i * kTbThreads + (int)threadIdx.x];
+    } else {
+      return global[i];
+    }
+  }
+  __device__ __forceinline__ void Set(int i, int2 v) const {
+    if constexpr (kLds && kShort) {
+      tb_dp16[i * kTbThreads + (int)threadIdx.x] =
+          make_short2((short)(v.x > kTbLow ? v.x : kTbLow), (short)(v.y > kTbLow ? v.y : kTbLow));
+    } else if constexpr (kLds) {
+      tb_dp[i * kTbThreads + (int)threadIdx.x] = v;
+    } else {
+      global[i] = v;
+    }
   }
 };
 
@@ -70,13 +96,13 @@ __device__ __forceinline__ uint32_t CodeAt(const uint64_t *packed, int64_t pos) 
 // GappedExtender::ExtendOneSide with traceback (the DP and band bookkeeping
 // after the delimiter checks); ops are generated from the best cell back to
 // the origin.  Returns flags (kTracebackStale/Overflow).
-template <bool kLds>
+template <bool kLds, bool kShort>
 __device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, uint32_t s0len,
                                                const uint64_t *pd, int64_t d0, int inc,
                                                const int8_t *sm, int gap_open, int gap_ext,
                                                int cutoff, Lane &L, uint32_t op_base,
                                                Side *out) {
-  const DpCells<kLds> dp{L.dp};
+  const DpCells<kLds, kShort> dp{L.dp};
   // the query code at q0 + p, p = (a - 1) inc for DP column a
   auto qcode = [&](int p) -> uint32_t {
     if constexpr (kLds) return tb_qc[(p * inc + 1) * kTbThreads + (int)threadIdx.x];
@@ -94,15 +120,15 @@ __device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, u
   const int gi = gap_open + gap_ext;
   int as = 0, ae = 0, ai = 0, last = 0;
   int score = -gi;
-  dp[0] = make_int2(0, -gi);
+  dp.Set(0, make_int2(0, -gi));
   int p0 = 0;
   for (ai = 1; qcode(p0) != kDelimiter; ++ai, p0 += inc) {
     if (score < -cutoff) break;
-    dp[ai] = make_int2(score, score - gi);
+    dp.Set(ai, make_int2(score, score - gi));
     score -= gap_ext;
   }
   if (qcode(p0) != kDelimiter) {
-    dp[ai] = make_int2(kInitScore, kInitScore);
+    dp.Set(ai, make_int2(kInitScore, kInitScore));
     ae = ai + 1;
   } else {
     ae = ai;
@@ -124,19 +150,19 @@ __device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, u
     const int zero = base - off;
     const int8_t *row = sm + c1 * kTableDim;
     p0 = as * inc;
-    int prev = dp[as].x;
+    int prev = dp.Get(as).x;
     {
-      int2 c = dp[as];
+      int2 c = dp.Get(as);
       c.x = (max_score - c.y > cutoff) ? kInitScore : c.y;
-      dp[as] = make_int2(c.x, c.y - gap_ext);
+      dp.Set(as, make_int2(c.x, c.y - gap_ext));
     }
-    int gap_row = dp[as].x - gi;
+    int gap_row = dp.Get(as).x - gi;
     L.arena[zero + as] = kGap0 + kExt0;
     written = max(written, (int64_t)(zero + as));
     last = as;
     for (ai = as + 1; ai < ae; ++ai, p0 += inc) {
       int s = prev + row[qcode(p0)];
-      const int2 cell = dp[ai];
+      const int2 cell = dp.Get(ai);
       prev = cell.x;
       int gap_col = cell.y;
       uint8_t op = kSub;
@@ -150,7 +176,7 @@ __device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, u
       }
       if (max_score - s > cutoff) {
         if (as + 1 == ai) ++as;
-        dp[ai] = make_int2(kInitScore, kInitScore);
+        dp.Set(ai, make_int2(kInitScore, kInitScore));
         gap_row = kInitScore;
         L.arena[zero + ai] = kUnwritten;
       } else {
@@ -171,7 +197,7 @@ __device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, u
         }
         if (gap_row < s - gi) gap_row = s - gi; else op += kExt1;
         L.arena[zero + ai] = op;
-        dp[ai] = make_int2(s, gap_keep);
+        dp.Set(ai, make_int2(s, gap_keep));
       }
       written = max(written, (int64_t)(zero + ai));
     }
@@ -180,7 +206,7 @@ __device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, u
       ae = last + 1;
     } else {
       while (gap_row >= (max_score - cutoff) && qcode(p0) != kDelimiter) {
-        dp[ae] = make_int2(gap_row, gap_row - gi);
+        dp.Set(ae, make_int2(gap_row, gap_row - gi));
         L.arena[zero + ae] = kGap0 + kExt0;
         written = max(written, (int64_t)(zero + ae));
         gap_row -= gap_ext;
@@ -189,7 +215,7 @@ __device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, u
       }
     }
     if (qcode(p0) != kDelimiter) {
-      dp[ae] = make_int2(kInitScore, kInitScore);
+      dp.Set(ae, make_int2(kInitScore, kInitScore));
       L.arena[zero + ae] = kUnwritten;
       written = max(written, (int64_t)(zero + ae));
       ++ae;
@@ -255,6 +281,7 @@ __device__ __forceinline__ uint32_t ExtendBody(const uint64_t *pq, int64_t q0, u
   return flags;
 }
 
+template <bool kShort>
 __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const uint64_t *pd,
                            int64_t d0, bool reversed, const int8_t *sm, int gap_open,
                            int gap_ext, int cutoff, Lane &L, uint32_t op_base, Side *out) {
@@ -294,13 +321,14 @@ __device__ uint32_t Extend(const uint64_t *pq, int64_t q0, uint32_t s0len, const
   // residue); s0len, the distance to the query's far end, can exceed n0 when
   // the query holds several frames
   if (n0 + 2 <= kTbLdsCols)
-    return ExtendBody<true>(pq, q0, s0len, pd, d0, inc, sm, gap_open, gap_ext, cutoff, L,
-                            op_base, out);
+    return ExtendBody<true, kShort>(pq, q0, s0len, pd, d0, inc, sm, gap_open, gap_ext, cutoff,
+                                    L, op_base, out);
   if (max(s0len + 1, n0 + 2) > L.dp_cap) return kTracebackOverflow;
-  return ExtendBody<false>(pq, q0, s0len, pd, d0, inc, sm, gap_open, gap_ext, cutoff, L,
-                           op_base, out);
+  return ExtendBody<false, false>(pq, q0, s0len, pd, d0, inc, sm, gap_open, gap_ext, cutoff,
+                                  L, op_base, out);
 }
 
+template <bool kShort>
 __global__ void __launch_bounds__(kTbThreads)
     TracebackKernel(const uint64_t *pq, const uint64_t *pd, const TracebackJob *jobs,
                     uint32_t n, const int *matrix, uint32_t nl, int gap_open, int gap_ext,
@@ -325,11 +353,11 @@ __global__ void __launch_bounds__(kTbThreads)
     TracebackOut o = {};
     Side rev, fwd;
     // reverse side from (hit - 1), forward side from the hit
-    uint32_t flags = Extend(pq, (int64_t)job.query_offset + job.hit_q - 1, job.hit_q - 1, pd,
+    uint32_t flags = Extend<kShort>(pq, (int64_t)job.query_offset + job.hit_q - 1, job.hit_q - 1, pd,
                             (int64_t)job.hit_d - 1, true, sm, gap_open, gap_ext, cutoff, L, 0,
                             &rev);
     if (!(flags & kTracebackOverflow))
-      flags |= Extend(pq, (int64_t)job.query_offset + job.hit_q, job.query_length - job.hit_q,
+      flags |= Extend<kShort>(pq, (int64_t)job.query_offset + job.hit_q, job.query_length - job.hit_q,
                       pd, (int64_t)job.hit_d, false, sm, gap_open, gap_ext, cutoff, L, rev.nops,
                       &fwd);
     o.flags = flags;
@@ -390,8 +418,13 @@ hipError_t TracebackLaunch(const uint64_t *pq, const uint64_t *pd, const Traceba
                            uint32_t lanes, TracebackOut *outs, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t blocks = (lanes + kTbThreads - 1) / kTbThreads;
-  hipLaunchKernelGGL(TracebackKernel, dim3(blocks), dim3(kTbThreads), 0, s, pq, pd, jobs, n,
-                     matrix, nl, gap_open, gap_ext, cutoff, scratch, caps, outs);
+  const bool short_cells = cutoff <= 8192 && gap_ext >= 0 && gap_open + gap_ext <= 1024;
+  if (short_cells)
+    hipLaunchKernelGGL(TracebackKernel<true>, dim3(blocks), dim3(kTbThreads), 0, s, pq, pd, jobs,
+                       n, matrix, nl, gap_open, gap_ext, cutoff, scratch, caps, outs);
+  else
+    hipLaunchKernelGGL(TracebackKernel<false>, dim3(blocks), dim3(kTbThreads), 0, s, pq, pd,
+                       jobs, n, matrix, nl, gap_open, gap_ext, cutoff, scratch, caps, outs);
   return hipGetLastError();
 }
 
