@@ -1,7 +1,7 @@
 // cost_model.h -- learned cost of a gapped X-drop extension.
 //
 // A small MLP (embedding 21 x 8 per residue, 128 residues = the 32-residue
-// windows of query and subject on either side of the seed, 1024 -> 64 ->
+// windows of query and subject on either side of the seed, 1024 -> 256 ->
 // 64 -> 1 with GELU) predicts log2 of the DP cells the one-sided extension
 // will evaluate.  The prediction only orders and bins extensions for load
 // balance; it never changes which extensions run or what they return.
@@ -19,14 +19,14 @@ constexpr int kCostResidues = 4 * kCostWindow;     // query/subject x left/right
 constexpr int kCostEmbed = 8;
 constexpr int kCostAlphabet = 21;                  // 20 amino acids + other
 constexpr int kCostIn = kCostResidues * kCostEmbed;  // 1024
-constexpr int kCostH1 = 64;
+constexpr int kCostH1 = 256;
 constexpr int kCostH2 = 64;
 
 // Parameters in fp32 master form; the device kernels use bf16 copies.
 struct CostModelWeights {
   std::vector<float> embed;  // [21][8]
-  std::vector<float> w1, b1; // [64][1024], [64]
-  std::vector<float> w2, b2; // [64][64], [64]
+  std::vector<float> w1, b1; // [256][1024], [256]
+  std::vector<float> w2, b2; // [64][256], [64]
   std::vector<float> w3, b3; // [64], [1]
   void Init(uint64_t seed);
   size_t Count() const;

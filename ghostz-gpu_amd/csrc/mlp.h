@@ -1,5 +1,5 @@
 // mlp.h -- the cost-model MLP on the device: embedding 21x8 per residue,
-// 1024 -> 64 -> 64 -> 1 with GELU, bf16 operands, fp32 accumulation (MFMA),
+// 1024 -> 256 -> 64 -> 1 with GELU, bf16 operands, fp32 accumulation (MFMA),
 // fp32 master weights and Adam for training.
 #pragma once
 
@@ -48,16 +48,16 @@ class MlpDevice {
   // activations
   uint8_t *codes_ = nullptr;
   uint16_t *x_ = nullptr;       // [B][1024] bf16
-  float *z1_ = nullptr;         // [B][64]
-  uint16_t *h1_ = nullptr;      // [B][64] bf16
+  float *z1_ = nullptr;         // [B][256]
+  uint16_t *h1_ = nullptr;      // [B][256] bf16
   float *z2_ = nullptr;         // [B][64]
   uint16_t *h2_ = nullptr;      // [B][64] bf16
   float *y_ = nullptr, *t_ = nullptr, *loss_ = nullptr;
   float *dz2_ = nullptr;        // [B][64]
   uint16_t *dz2h_ = nullptr;
-  float *dz1_ = nullptr;        // [B][64]
+  float *dz1_ = nullptr;        // [B][256]
   uint16_t *dz1h_ = nullptr;
-  float *dh1_ = nullptr;        // [B][64]
+  float *dh1_ = nullptr;        // [B][256]
   float *dx_ = nullptr;         // [B][1024]
 };
 

@@ -220,10 +220,14 @@ inline unsigned Blocks(size_t n, unsigned t = 256) { return (unsigned)((n + t - 
 // B[k][column l % 32] for k = 8 * (l / 32) + j, j < 8; accumulator register r
 // of lane l holds C[8 * (r / 4) + 4 * (l / 32) + (r % 4)][l % 32].
 //
-// Layers are computed transposed (hidden unit = row, seed = column):
-//   layer 1: C1[h][seed] = W1[h][:] . x[seed][:], A = W1 (LDS, [k/8][64][8],
-//            staged a quarter of the inputs at a time),
-//            B = the residue embeddings of 32 seeds (one 16-byte LDS read);
+// Layers are computed transposed (hidden unit = row, seed = column); each
+// wave owns 32 seeds:
+//   layer 1: C1[h][seed] = W1[h][:] . x[seed][:] in kHB row blocks of 32
+//            units; A = W1 from LDS ([k/8][unit][8]), a sixteenth of the
+//            inputs (32 KB) per stage in two alternating buffers, the next
+//            stage read into registers during this one's MFMAs and stored
+//            into the other buffer; B = the residue embeddings (one 16-byte
+//            LDS read);
 //   layer 2: C2[n][seed] = W2[n][:] . h1[:][seed], whose B operand is the
 //            lane's own layer-1 accumulators (a lane holds one seed's column):
 //            K step t takes registers 8 (t % 2) .. 8 (t % 2) + 7 of row block
@@ -232,12 +236,17 @@ inline unsigned Blocks(size_t n, unsigned t = 256) { return (unsigned)((n + t - 
 //   layer 3: each lane sums its 32 rows of C2 and adds its partner lane's.
 // Rounding points are those of the training forward (bf16 h1, h2, w3).
 // ---------------------------------------------------------------------------
-static_assert(kCostH1 == 64 && kCostH2 == 64, "the fused inference kernel is 64 x 64");
+static_assert(kCostH1 % 32 == 0 && kCostH2 == 64, "inference tiles: H1 in 32s, H2 = 64");
+constexpr int kHB = kCostH1 / 32;                  // layer-1 row blocks
 constexpr int kInferWaves = 8;
-constexpr int kInferBlocks = 4;                            // 32-seed columns per wave
-constexpr int kInferTile = kInferWaves * 32 * kInferBlocks;  // seeds per block pass
-constexpr int kW1Part = kCostIn / 4;                  // W1 inputs staged at a time
-constexpr int kVPerPart = kW1Part / kCostEmbed / 16;  // 16-residue steps per quarter
+constexpr int kInferTile = kInferWaves * 32;       // seeds per block pass
+constexpr int kStageIn = 64;                       // W1 inputs per stage (8 residues)
+constexpr int kStages = kCostIn / kStageIn;        // 16
+constexpr int kStageVec = kCostH1 * kStageIn / 8;  // 16-byte pieces per stage
+constexpr int kPrefetch = kStageVec / (kInferWaves * 64);
+static_assert(kStageVec % (kInferWaves * 64) == 0, "a stage splits evenly over the block");
+static_assert(kStages % 2 == 0, "stages alternate between two buffers across tiles");
+constexpr int kL2Steps = kCostH1 / 16;             // layer-2 K steps
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -249,27 +258,41 @@ __device__ __forceinline__ __bf16 AsBf16(uint16_t h) {
   return __builtin_bit_cast(__bf16, h);
 }
 
+// Piece i of stage s: unit h = i / 8, inputs s * 64 + (i % 8) * 8 .. + 8.
+__device__ __forceinline__ uint4 StagePiece(const uint16_t *w1h, int s, int i) {
+  const int h = i / (kStageIn / 8), k8 = i % (kStageIn / 8);
+  return *reinterpret_cast<const uint4 *>(w1h + (size_t)h * kCostIn + s * kStageIn + k8 * 8);
+}
+__device__ __forceinline__ void PutPiece(uint16_t *w1s, int i, uint4 v) {
+  const int h = i / (kStageIn / 8), k8 = i % (kStageIn / 8);
+  *reinterpret_cast<uint4 *>(&w1s[(k8 * kCostH1 + h) * 8]) = v;
+}
+
 __global__ void __launch_bounds__(kInferWaves * 64)
     InferKernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h, const uint16_t *embh,
                 const float *b1, const uint16_t *w2h, const float *b2, const float *w3,
                 const float *b3, float *y) {
-  // one quarter of W1's inputs at a time ([k/8][64][8], 32 KB), so a block
-  // fits on a CU beside other kernels' blocks
-  __shared__ __align__(16) uint16_t w1[kW1Part * kCostH1];
-  __shared__ __align__(16) uint16_t w2[4 * 2 * 64 * 8];      // [t][row block][lane][8]
+  __shared__ __align__(16) uint16_t w1s[2][kStageIn * kCostH1];     // two stages, 64 KB
+  __shared__ __align__(16) uint16_t w2[kL2Steps * 2 * 64 * 8];      // [t][row block][lane][8]
   __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
   __shared__ float b1s[kCostH1], b2s[kCostH2], w3s[kCostH2];
-  for (int idx = threadIdx.x; idx < 4 * 2 * 64 * 8; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < kL2Steps * 2 * 64 * 8; idx += blockDim.x) {
     const int j = idx & 7, l = (idx >> 3) & 63, nb = (idx >> 9) & 1, t = idx >> 10;
     const int row = 32 * nb + (l & 31);
     const int h = 16 * t + 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
     w2[idx] = w2h[row * kCostH1 + h];
   }
   for (int i = threadIdx.x; i < kCostAlphabet * kCostEmbed; i += blockDim.x) emb[i] = embh[i];
-  if (threadIdx.x < kCostH1) b1s[threadIdx.x] = b1[threadIdx.x];
+  for (int i = threadIdx.x; i < kCostH1; i += blockDim.x) b1s[i] = b1[i];
   if (threadIdx.x < kCostH2) {
     b2s[threadIdx.x] = b2[threadIdx.x];
     w3s[threadIdx.x] = Bf2F(F2Bf(w3[threadIdx.x]));
+  }
+  // stage 0 of W1; every later stage is read one stage ahead
+#pragma unroll
+  for (int p = 0; p < kPrefetch; ++p) {
+    const int i = (int)threadIdx.x + p * kInferWaves * 64;
+    PutPiece(w1s[0], i, StagePiece(w1h, 0, i));
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -278,101 +301,92 @@ __global__ void __launch_bounds__(kInferWaves * 64)
   const float bias3 = b3[0];
   const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
   for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const uint32_t base = tile * kInferTile + wave * (32 * kInferBlocks);
-    // the codes of seed base + 32 c + r32 stream through registers 16 at a
-    // time; seeds past the end read seed 0's row as code 20 throughout
-    const uint4 *src[kInferBlocks];
-    uint32_t fill[kInferBlocks];
+    // the codes of this lane's seed stream through registers 8 at a time;
+    // seeds past the end read seed 0's row as code 20 throughout
+    const uint32_t seed = tile * kInferTile + wave * 32 + r32;
+    const uint4 *src =
+        reinterpret_cast<const uint4 *>(codes + (size_t)(seed < n ? seed : 0) * kCostResidues);
+    const uint32_t fill = seed < n ? 0u : 0x14141414u;
+    float16_t acc[kHB];
 #pragma unroll
-    for (int c = 0; c < kInferBlocks; ++c) {
-      const uint32_t seed = base + 32 * c + r32;
-      src[c] = reinterpret_cast<const uint4 *>(codes + (size_t)(seed < n ? seed : 0) *
-                                                           kCostResidues);
-      fill[c] = seed < n ? 0u : 0x14141414u;
-    }
-    float16_t acc[2][kInferBlocks];
+    for (int hb = 0; hb < kHB; ++hb)
 #pragma unroll
-    for (int hb = 0; hb < 2; ++hb)
-#pragma unroll
-      for (int c = 0; c < kInferBlocks; ++c)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[hb][c][r] = 0.f;
+      for (int r = 0; r < 16; ++r) acc[hb][r] = 0.f;
 #pragma unroll 1
-    for (int v = 0; v < kCostResidues / 16; ++v) {
-      if (v % kVPerPart == 0) {
-        // the next quarter of W1 (every wave has finished with the last)
-        const int part = v / kVPerPart;
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < kCostH1 * (kW1Part / 8); idx += blockDim.x) {
-          const int h = idx / (kW1Part / 8), k8 = idx % (kW1Part / 8);
-          *reinterpret_cast<uint4 *>(&w1[(k8 * kCostH1 + h) * 8]) = *reinterpret_cast<const uint4 *>(
-              w1h + (size_t)h * kCostIn + (size_t)part * kW1Part + k8 * 8);
-        }
-        __syncthreads();
-      }
-      uint4 cw[kInferBlocks];
+    for (int s = 0; s < kStages; ++s) {
+      // stage s is in buffer s & 1; the next one (the first again after the
+      // last; kStages is even) goes to the other buffer, which every wave
+      // finished reading before the barrier that ended stage s - 1
+      uint4 pf[kPrefetch];
+      const int ns = (s + 1) % kStages;
 #pragma unroll
-      for (int c = 0; c < kInferBlocks; ++c) {
-        cw[c] = src[c][v];
-        cw[c].x |= fill[c];
-        cw[c].y |= fill[c];
-        cw[c].z |= fill[c];
-        cw[c].w |= fill[c];
+      for (int p = 0; p < kPrefetch; ++p)
+        pf[p] = StagePiece(w1h, ns, (int)threadIdx.x + p * kInferWaves * 64);
+      const uint16_t *cur = w1s[s & 1];
+      // residues 8 s .. 8 s + 7 of this lane's seed: half of a 16-code group
+      uint2 cw = reinterpret_cast<const uint2 *>(src)[s];
+      cw.x |= fill;
+      cw.y |= fill;
+      // a K step's operands are read as one batch, a step ahead of its
+      // MFMAs (the scheduling barriers keep that order), so the LDS latency
+      // overlaps the previous step's MFMAs
+      bf16x8 a[2][kHB], bx[2];
+      auto load_step = [&](int q, bf16x8 (&av)[kHB], bf16x8 &b) {
+        // residue 8 s + 2 q + kq: byte 2 (q & 1) + kq of word q / 2
+        const int k8 = 2 * q + kq;
+        const uint32_t word = q < 2 ? cw.x : cw.y;
+        uint32_t code = (word >> (16 * (q & 1) + kshift)) & 0xffu;
+        code = code < 20 ? code : 20;
+        b = Load8(&emb[code * kCostEmbed]);
+#pragma unroll
+        for (int hb = 0; hb < kHB; ++hb) av[hb] = Load8(&cur[(k8 * kCostH1 + 32 * hb + r32) * 8]);
+      };
+      load_step(0, a[0], bx[0]);
+#pragma unroll
+      for (int q = 0; q < kStageIn / 16; ++q) {
+        if (q + 1 < kStageIn / 16) load_step(q + 1, a[(q + 1) & 1], bx[(q + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int hb = 0; hb < kHB; ++hb)
+          acc[hb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q & 1][hb], bx[q & 1], acc[hb],
+                                                            0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        // residue 16 v + 2 q + kq: byte 2 (q & 1) + kq of word q / 2
-        const int kb = (v % kVPerPart) * 16 + 2 * q + kq;  // in the staged quarter
-        const bf16x8 a0 = Load8(&w1[(kb * kCostH1 + r32) * 8]);
-        const bf16x8 a1 = Load8(&w1[(kb * kCostH1 + 32 + r32) * 8]);
+      for (int p = 0; p < kPrefetch; ++p)
+        PutPiece(w1s[ns & 1], (int)threadIdx.x + p * kInferWaves * 64, pf[p]);
+      __syncthreads();
+    }
+    float16_t acc2[2];
 #pragma unroll
-        for (int c = 0; c < kInferBlocks; ++c) {
-          const uint32_t word = q < 2 ? cw[c].x : q < 4 ? cw[c].y : q < 6 ? cw[c].z : cw[c].w;
-          uint32_t code = (word >> (16 * (q & 1) + kshift)) & 0xffu;
-          code = code < 20 ? code : 20;
-          const bf16x8 bx = Load8(&emb[code * kCostEmbed]);
-          acc[0][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bx, acc[0][c], 0, 0, 0);
-          acc[1][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bx, acc[1][c], 0, 0, 0);
-        }
-      }
+    for (int r = 0; r < 16; ++r) {
+      acc2[0][r] = 0.f;
+      acc2[1][r] = 0.f;
     }
 #pragma unroll
-    for (int c = 0; c < kInferBlocks; ++c) {
-      // keeps the epilogue's LDS reads (biases, W2, w3) inside the tile loop
-      // instead of holding them in registers across it
-      __asm__ volatile("" ::: "memory");
-      float16_t acc2[2];
+    for (int t = 0; t < kL2Steps; ++t) {
+      bf16x8 bh;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int h = 16 * t + 8 * (j >> 2) + 4 * kq + (j & 3);
+        bh[j] = AsBf16(F2Bf(Gelu(acc[t >> 1][8 * (t & 1) + j] + b1s[h])));
+      }
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) {
+        const bf16x8 a = Load8(&w2[((t * 2 + nb) * 64 + lane) * 8]);
+        acc2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bh, acc2[nb], 0, 0, 0);
+      }
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        acc2[0][r] = 0.f;
-        acc2[1][r] = 0.f;
+        const int row = 32 * nb + 8 * (r >> 2) + 4 * kq + (r & 3);
+        part += Bf2F(F2Bf(Gelu(acc2[nb][r] + b2s[row]))) * w3s[row];
       }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        bf16x8 bh;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int h = 16 * t + 8 * (j >> 2) + 4 * kq + (j & 3);
-          bh[j] = AsBf16(F2Bf(Gelu(acc[t >> 1][c][8 * (t & 1) + j] + b1s[h])));
-        }
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-          const bf16x8 a = Load8(&w2[((t * 2 + nb) * 64 + lane) * 8]);
-          acc2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bh, acc2[nb], 0, 0, 0);
-        }
-      }
-      float part = 0.f;
-#pragma unroll
-      for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int row = 32 * nb + 8 * (r >> 2) + 4 * kq + (r & 3);
-          part += Bf2F(F2Bf(Gelu(acc2[nb][r] + b2s[row]))) * w3s[row];
-        }
-      part += __shfl_xor(part, 32, 64);
-      const uint32_t seed = base + 32 * c + r32;
-      if (kq == 0 && seed < n) y[seed] = part + bias3;
-    }
+    part += __shfl_xor(part, 32, 64);
+    if (kq == 0 && seed < n) y[seed] = part + bias3;
   }
 }
 

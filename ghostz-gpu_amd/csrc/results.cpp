@@ -142,16 +142,42 @@ void AddResults(const DbChunk &chunk, int chunk_id, const SearchParameters &p,
   std::vector<Presearched> &merged = scratch.merged;
   merged.clear();
   scratch.starts.Clear();
-  std::stable_sort(added.begin(), added.end(), ByScore());
+  // The new results are taken in stable score order (the reference's
+  // stable_sort), but the merge below stops after max_results kept ones, so
+  // only a prefix of that order is produced: a partial sort of indices by
+  // (score order, index), whose first k equal the stable sort's first k,
+  // extended when the merge runs past them.
+  std::vector<uint32_t> &order = scratch.order;
+  const size_t na = added.size();
+  order.resize(na);
+  for (size_t k = 0; k < na; ++k) order[k] = (uint32_t)k;
+  auto before = [&](uint32_t x, uint32_t y) {
+    // ByScore never calls two results equal; stable_sort keeps the input
+    // order of results neither of which comes before the other
+    const Presearched &a = added[x], &b = added[y];
+    if (Compare(a.score, a.start, a.end, b.score, b.start, b.end) > 0) return true;
+    if (Compare(b.score, b.start, b.end, a.score, a.start, a.end) > 0) return false;
+    return x < y;
+  };
+  size_t sorted = 0;  // order[0, sorted) is final
+  auto extend = [&](size_t want) {
+    want = std::min(na, want);
+    if (want <= sorted) return;
+    std::partial_sort(order.begin() + sorted, order.begin() + want, order.end(), before);
+    sorted = want;
+  };
+  extend(2 * (size_t)p.max_results + 8);
   size_t ri = 0, ai = 0;
-  while (merged.size() < p.max_results && (ai < added.size() || ri < results.size())) {
+  while (merged.size() < p.max_results && (ai < na || ri < results.size())) {
+    if (ai < na && ai == sorted) extend(2 * sorted);
     Presearched a;
-    if (ai == added.size()) {
+    if (ai == na) {
       a = results[ri++];
     } else if (ri == results.size() ||
                Compare(results[ri].score, results[ri].start, results[ri].end,
-                       added[ai].score, added[ai].start, added[ai].end) < 0) {
-      a = added[ai++];
+                       added[order[ai]].score, added[order[ai]].start,
+                       added[order[ai]].end) < 0) {
+      a = added[order[ai++]];
       a.chunk = chunk_id;
       a.subject = chunk.SubjectOf(a.start.d);
     } else {

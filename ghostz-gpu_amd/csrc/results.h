@@ -111,6 +111,7 @@ class SubjectStarts {
 struct AddResultsScratch {
   SubjectStarts starts;
   std::vector<Presearched> merged;
+  std::vector<uint32_t> order;  // the new results by score, a prefix at a time
 };
 void AddResults(const DbChunk &chunk, int chunk_id, const SearchParameters &p,
                 AddResultsScratch &scratch, std::vector<Presearched> &added,

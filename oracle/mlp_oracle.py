@@ -6,8 +6,8 @@ only orders gapped extensions.  This module restates its definition so the
 device kernels (forward, backward, Adam) can be checked:
 
   x[b, 8r + e] = bf16(embed[code[b, r], e])              r < 128, e < 8
-  z1 = x bf16(W1)^T + b1,   h1 = bf16(gelu(z1))          1024 -> 64
-  z2 = h1 bf16(W2)^T + b2,  h2 = bf16(gelu(z2))          64 -> 64
+  z1 = x bf16(W1)^T + b1,   h1 = bf16(gelu(z1))          1024 -> 256
+  z2 = h1 bf16(W2)^T + b2,  h2 = bf16(gelu(z2))          256 -> 64
   y  = h2 . bf16(w3) + b3                                64 -> 1
   loss = mean_b (y_b - t_b)^2
 
@@ -20,7 +20,7 @@ into the process).
 import numpy as np
 from scipy.special import erf
 
-SIZES = [("embed", (21, 8)), ("w1", (64, 1024)), ("b1", (64,)), ("w2", (64, 64)),
+SIZES = [("embed", (21, 8)), ("w1", (256, 1024)), ("b1", (256,)), ("w2", (64, 256)),
          ("b2", (64,)), ("w3", (64,)), ("b3", (1,))]
 
 

@@ -39,7 +39,7 @@ def test_forward_matches_reference(device, n):
     rng = np.random.default_rng(100 + n)
     flat = gz.Mlp.init_weights(3)
     flat[-1] = 0.5
-    flat[21 * 8 + 64 * 1024:21 * 8 + 64 * 1024 + 64] = rng.normal(0, 0.1, 64)
+    flat[21 * 8 + 256 * 1024:21 * 8 + 256 * 1024 + 256] = rng.normal(0, 0.1, 256)
     mlp = gz.Mlp(flat)
     codes = random_codes(rng, n)
     got = mlp.predict(codes)

@@ -9,7 +9,7 @@ import sys
 import numpy as np
 import torch
 
-SIZES = [("embed", (21, 8)), ("w1", (64, 1024)), ("b1", (64,)), ("w2", (64, 64)),
+SIZES = [("embed", (21, 8)), ("w1", (256, 1024)), ("b1", (256,)), ("w2", (64, 256)),
          ("b2", (64,)), ("w3", (64,)), ("b3", (1,))]
 
 
