@@ -223,8 +223,8 @@ inline unsigned Blocks(size_t n, unsigned t = 256) { return (unsigned)((n + t - 
 // Layers are computed transposed (hidden unit = row, seed = column); each
 // wave owns 32 seeds:
 //   layer 1: C1[h][seed] = W1[h][:] . x[seed][:] in kHB row blocks of 32
-//            units; A = W1 from LDS ([k/8][unit][8]), a sixteenth of the
-//            inputs (32 KB) per stage in two alternating buffers, the next
+//            units; A = W1 from LDS ([k/8][unit][8]), a 32nd of the inputs
+//            (16 KB) per stage in two alternating buffers, the next
 //            stage read into registers during this one's MFMAs and stored
 //            into the other buffer; B = the residue embeddings (one 16-byte
 //            LDS read);
@@ -238,10 +238,13 @@ inline unsigned Blocks(size_t n, unsigned t = 256) { return (unsigned)((n + t - 
 // ---------------------------------------------------------------------------
 static_assert(kCostH1 % 32 == 0 && kCostH2 == 64, "inference tiles: H1 in 32s, H2 = 64");
 constexpr int kHB = kCostH1 / 32;                  // layer-1 row blocks
-constexpr int kInferWaves = 8;
+// Blocks of four waves and 66 KB of LDS, two per CU: their stage barriers
+// are independent, so one block's GELU epilogue overlaps the other's MFMAs.
+constexpr int kInferWaves = 4;
 constexpr int kInferTile = kInferWaves * 32;       // seeds per block pass
-constexpr int kStageIn = 64;                       // W1 inputs per stage (8 residues)
-constexpr int kStages = kCostIn / kStageIn;        // 16
+constexpr int kStageIn = 32;                       // W1 inputs per stage (4 residues)
+constexpr int kStages = kCostIn / kStageIn;        // 32
+static_assert(kStageIn == 32, "a stage's residues are one 32-bit word of codes");
 constexpr int kStageVec = kCostH1 * kStageIn / 8;  // 16-byte pieces per stage
 constexpr int kPrefetch = kStageVec / (kInferWaves * 64);
 static_assert(kStageVec % (kInferWaves * 64) == 0, "a stage splits evenly over the block");
@@ -258,7 +261,7 @@ __device__ __forceinline__ __bf16 AsBf16(uint16_t h) {
   return __builtin_bit_cast(__bf16, h);
 }
 
-// Piece i of stage s: unit h = i / 8, inputs s * 64 + (i % 8) * 8 .. + 8.
+// Piece i of stage s: unit h = i / 4, inputs s * 32 + (i % 4) * 8 .. + 8.
 __device__ __forceinline__ uint4 StagePiece(const uint16_t *w1h, int s, int i) {
   const int h = i / (kStageIn / 8), k8 = i % (kStageIn / 8);
   return *reinterpret_cast<const uint4 *>(w1h + (size_t)h * kCostIn + s * kStageIn + k8 * 8);
@@ -268,11 +271,11 @@ __device__ __forceinline__ void PutPiece(uint16_t *w1s, int i, uint4 v) {
   *reinterpret_cast<uint4 *>(&w1s[(k8 * kCostH1 + h) * 8]) = v;
 }
 
-__global__ void __launch_bounds__(kInferWaves * 64)
+__global__ void __launch_bounds__(kInferWaves * 64) __attribute__((amdgpu_waves_per_eu(2)))
     InferKernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h, const uint16_t *embh,
                 const float *b1, const uint16_t *w2h, const float *b2, const float *w3,
                 const float *b3, float *y) {
-  __shared__ __align__(16) uint16_t w1s[2][kStageIn * kCostH1];     // two stages, 64 KB
+  __shared__ __align__(16) uint16_t w1s[2][kStageIn * kCostH1];     // two stages, 32 KB
   __shared__ __align__(16) uint16_t w2[kL2Steps * 2 * 64 * 8];      // [t][row block][lane][8]
   __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
   __shared__ float b1s[kCostH1], b2s[kCostH2], w3s[kCostH2];
@@ -301,7 +304,7 @@ __global__ void __launch_bounds__(kInferWaves * 64)
   const float bias3 = b3[0];
   const uint32_t tiles = (n + kInferTile - 1) / kInferTile;
   for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    // the codes of this lane's seed stream through registers 8 at a time;
+    // the codes of this lane's seed stream through registers 4 at a time;
     // seeds past the end read seed 0's row as code 20 throughout
     const uint32_t seed = tile * kInferTile + wave * 32 + r32;
     const uint4 *src =
@@ -323,19 +326,16 @@ __global__ void __launch_bounds__(kInferWaves * 64)
       for (int p = 0; p < kPrefetch; ++p)
         pf[p] = StagePiece(w1h, ns, (int)threadIdx.x + p * kInferWaves * 64);
       const uint16_t *cur = w1s[s & 1];
-      // residues 8 s .. 8 s + 7 of this lane's seed: half of a 16-code group
-      uint2 cw = reinterpret_cast<const uint2 *>(src)[s];
-      cw.x |= fill;
-      cw.y |= fill;
+      // residues 4 s .. 4 s + 3 of this lane's seed
+      const uint32_t cw = reinterpret_cast<const uint32_t *>(src)[s] | fill;
       // a K step's operands are read as one batch, a step ahead of its
       // MFMAs (the scheduling barriers keep that order), so the LDS latency
       // overlaps the previous step's MFMAs
       bf16x8 a[2][kHB], bx[2];
       auto load_step = [&](int q, bf16x8 (&av)[kHB], bf16x8 &b) {
-        // residue 8 s + 2 q + kq: byte 2 (q & 1) + kq of word q / 2
+        // residue 4 s + 2 q + kq: byte 2 q + kq of the word
         const int k8 = 2 * q + kq;
-        const uint32_t word = q < 2 ? cw.x : cw.y;
-        uint32_t code = (word >> (16 * (q & 1) + kshift)) & 0xffu;
+        uint32_t code = (cw >> (16 * q + kshift)) & 0xffu;
         code = code < 20 ? code : 20;
         b = Load8(&emb[code * kCostEmbed]);
 #pragma unroll
@@ -524,7 +524,7 @@ hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const uint32_t tiles = (uint32_t)((n + kInferTile - 1) / kInferTile);
-  hipLaunchKernelGGL(InferKernel, dim3(std::min<unsigned>(tiles, (unsigned)cus)),
+  hipLaunchKernelGGL(InferKernel, dim3(std::min<unsigned>(tiles, 2u * (unsigned)cus)),
                      dim3(kInferWaves * 64), 0, s, d_codes, (uint32_t)n, w1h_, embh_,
                      master_ + kOffB1, w2h_, master_ + kOffB2, master_ + kOffW3,
                      master_ + kOffB3, d_out);
