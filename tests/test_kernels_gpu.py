@@ -367,3 +367,28 @@ def test_result_build_kernel_vs_host(dev, cutoff):
         assert np.array_equal(out[k], host), (k, out[k], host)
     say("  overflow", overflow, "stale", stale)
     assert overflow < len(jobs) // 20
+
+
+def test_result_build_kernel_large_cutoff(dev):
+    """A cutoff above 8192 keeps the traceback's LDS columns in 32 bits: short
+    query sides (at most 62 residues, the LDS path) with X-drop nearly off
+    still equal the host result build."""
+    rng = np.random.default_rng(47)
+    q, d, jobs, dmax = _tb_jobs(rng, 400, min_len=5, max_len=60, identity=(0.25, 0.95),
+                                indel=0.05)
+    cutoff = 10000
+    _, pq = gz.pack_on_device(q)
+    _, pd = gz.pack_on_device(d)
+    out = np.zeros((len(jobs), 10), np.uint32)
+    gz.check(gz.lib().gz_build_results(pq.ptr, pd.ptr, gz.ptr(jobs), len(jobs),
+                                       int(jobs[:, 1].max()), dmax, dev["d_matrix"].ptr,
+                                       dev["nl"], 11, 1, cutoff, gz.ptr(out), None))
+    host = np.zeros(10, np.uint32)
+    for k, (lead, qlen, hq, hd) in enumerate(jobs):
+        qs = np.ascontiguousarray(q[lead:lead + qlen])
+        gz.check(gz.lib().gz_build_result_host(gz.ptr(qs), int(qlen), gz.ptr(d), int(hq), int(hd),
+                                               cutoff, gz.ptr(host)))
+        assert not out[k, 9] & 2, (k, out[k])
+        assert out[k, 9] == host[9], (k, out[k], host)
+        if not host[9]:
+            assert np.array_equal(out[k], host), (k, out[k], host)
