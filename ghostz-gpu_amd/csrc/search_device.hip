@@ -141,11 +141,6 @@ __device__ __forceinline__ void StageAppend(bool pred, const SeedHit &h, HitStag
   st.count += n;
 }
 
-__device__ __forceinline__ void WaveCount(bool pred, uint32_t *counter) {
-  const uint64_t mask = __ballot(pred);
-  if (mask != 0 && __lane_id() == __ffsll((unsigned long long)mask) - 1)
-    atomicAdd(counter, (uint32_t)__popcll(mask));
-}
 
 // first and last active list of every tile of kSeedTile units starting at
 // unit_begin (upper bounds over the active lists' unit offsets)
