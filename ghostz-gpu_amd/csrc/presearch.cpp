@@ -588,6 +588,15 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
           trace.Add("build_host", b, t0, t1);
           // blocks are written in query order, straight into the output
           WriteOutputText(output, q, db, results, blocks[b].qb, blocks[b].qe, threads);
+          // reserve for the whole query chunk, extrapolated from the text so
+          // far, so that the text moves at most once more rather than at
+          // every doubling
+          if (b + 1 < nb) {
+            const double share = (double)blocks[b].qe / (double)nq;
+            const size_t want =
+                output_before + (size_t)((double)(output->size() - output_before) / share * 1.15);
+            if (output->capacity() < want) output->reserve(want);
+          }
           sec_output += Now() - t1;
           trace.Add("output", b, t1, Now());
         }

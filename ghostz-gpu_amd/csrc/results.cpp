@@ -5,9 +5,13 @@
 #include <charconv>
 #include <atomic>
 #include <climits>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
+#include <string>
 #include <thread>
+#include <vector>
 
 #include "gz_common.h"
 
@@ -624,6 +628,96 @@ void WriteOutput(std::ostream &os, const QuerySet &q, const Database &db,
   os << text;
 }
 
+namespace {
+// 10^k for k in [kPow10Low, kPow10Low + kPow10Count), each the double nearest
+// the decimal value (strtod rounds correctly)
+constexpr int kPow10Low = -296, kPow10Count = 604;
+const double *Pow10Table() {
+  static const std::vector<double> table = [] {
+    std::vector<double> t(kPow10Count);
+    for (int i = 0; i < kPow10Count; ++i) {
+      const std::string text = "1e" + std::to_string(kPow10Low + i);
+      t[i] = std::strtod(text.c_str(), nullptr);
+    }
+    return t;
+  }();
+  return table.data();
+}
+}  // namespace
+
+char *FormatG6(double v, char *p, char *end) {
+  const double a = std::fabs(v);
+  // the fast path covers 1e-300 <= |v| < 1e300; zero, signed zero, the
+  // extremes and non-finite values are left to the library
+  if (!(a >= 1e-300 && a < 1e300) || end - p < 16)
+    return std::to_chars(p, end, v, std::chars_format::general, 6).ptr;
+  // decimal exponent: a in [2^(e-1), 2^e) puts log10(a) within 0.302 above
+  // (e - 1) log10(2), so x is floor((e - 1) log10(2)) or one more
+  // (floor(n log10 2) = (n * 78913) >> 18 for |n| < 1200)
+  uint64_t bits;
+  std::memcpy(&bits, &a, sizeof(bits));
+  const int e = (int)((bits >> 52) & 0x7ff) - 1022;
+  int x = ((e - 1) * 78913) >> 18;
+  const double *pow10 = Pow10Table();
+  uint32_t n = 0;
+  for (int tries = 0;; ++tries) {
+    // a * 10^(5 - x) below 10^7, with a relative error of at most 2^-52
+    // (one rounding of the power, one of the product): within 3e-9 of the
+    // exact product, so a fraction further than 1e-6 from one half rounds
+    // the way the exact product does; nearer ties are left to the library
+    const int k = 5 - x - kPow10Low;
+    if (tries == 2 || k < 0 || k >= kPow10Count)
+      return std::to_chars(p, end, v, std::chars_format::general, 6).ptr;
+    const double scaled = a * pow10[k];
+    const uint64_t ip = (uint64_t)(int64_t)scaled;
+    const double f = scaled - (double)ip;  // exact: scaled < 2^53
+    if (f > 0.5 - 1e-6 && f < 0.5 + 1e-6)
+      return std::to_chars(p, end, v, std::chars_format::general, 6).ptr;
+    const uint64_t r = f < 0.5 ? ip : ip + 1;
+    if (r >= 1000000) {
+      ++x;
+    } else if (r < 100000) {
+      --x;
+    } else {
+      n = (uint32_t)r;
+      break;
+    }
+  }
+  char d[6];
+  for (int i = 5; i >= 0; --i) {
+    d[i] = (char)('0' + n % 10);
+    n /= 10;
+  }
+  int nd = 6;  // significant digits left once trailing zeros go
+  while (nd > 1 && d[nd - 1] == '0') --nd;
+  if (v < 0) *p++ = '-';
+  if (x < -4 || x >= 6) {
+    *p++ = d[0];
+    if (nd > 1) {
+      *p++ = '.';
+      for (int i = 1; i < nd; ++i) *p++ = d[i];
+    }
+    *p++ = 'e';
+    *p++ = x < 0 ? '-' : '+';
+    const int ex = x < 0 ? -x : x;  // at least two digits
+    if (ex >= 100) *p++ = (char)('0' + ex / 100);
+    *p++ = (char)('0' + ex / 10 % 10);
+    *p++ = (char)('0' + ex % 10);
+  } else if (x >= 0) {
+    for (int i = 0; i <= x; ++i) *p++ = d[i];
+    if (nd > x + 1) {
+      *p++ = '.';
+      for (int i = x + 1; i < nd; ++i) *p++ = d[i];
+    }
+  } else {
+    *p++ = '0';
+    *p++ = '.';
+    for (int i = 0; i < -x - 1; ++i) *p++ = '0';
+    for (int i = 0; i < nd; ++i) *p++ = d[i];
+  }
+  return p;
+}
+
 void WriteOutputText(std::string *out, const QuerySet &q, const Database &db,
                      const std::vector<std::vector<AlignmentResult> > &results, size_t qb,
                      size_t qe, int threads) {
@@ -632,13 +726,30 @@ void WriteOutputText(std::string *out, const QuerySet &q, const Database &db,
   const size_t nq = qe;
   const size_t block = 256;
   const size_t nblocks = (qe - qb + block - 1) / block;
-  std::vector<std::string> text(nblocks);
+  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  const int T = std::max(1, std::min<int>(threads, (int)nblocks));
+  // each thread formats its blocks into one buffer of its own, reserved once
+  // (a line holds the query name and about 50 bytes of numbers; 96 more
+  // bytes cover a subject name of 40), and the blocks are then copied out
+  // in query order
+  size_t estimate = 0;
+  for (size_t qi = qb; qi < qe; ++qi)
+    estimate += results[qi].size() * (q.names[qi].size() + 96);
+  struct Piece {
+    int buffer;
+    size_t begin, end;
+  };
+  std::vector<Piece> pieces(nblocks);
+  std::vector<std::string> buffers(T);
   std::atomic<size_t> next(0);
-  auto work = [&]() {
+  auto work = [&](int t) {
+    std::string &s = buffers[t];
+    s.reserve(estimate / T + estimate / (4 * T) + 4096);
     for (;;) {
       const size_t b = next.fetch_add(1);
       if (b >= nblocks) break;
-      std::string &s = text[b];
+      pieces[b].buffer = t;
+      pieces[b].begin = s.size();
       char num[64];
       // the fields as std::ostream writes them: integers in decimal,
       // floating-point values as printf's %g (6 significant digits)
@@ -646,10 +757,7 @@ void WriteOutputText(std::string *out, const QuerySet &q, const Database &db,
         const auto r = std::to_chars(num, num + sizeof(num), v);
         s.append(num, r.ptr);
       };
-      auto put_g = [&](double v) {
-        const auto r = std::to_chars(num, num + sizeof(num), v, std::chars_format::general, 6);
-        s.append(num, r.ptr);
-      };
+      auto put_g = [&](double v) { s.append(num, FormatG6(v, num, num + sizeof(num))); };
       for (size_t qi = qb + b * block; qi < std::min(nq, qb + (b + 1) * block); ++qi) {
         if (results[qi].empty()) continue;
         const uint64_t space = Statistics::SearchSpace(q.RealLength(qi), db.total_length,
@@ -685,18 +793,17 @@ void WriteOutputText(std::string *out, const QuerySet &q, const Database &db,
           s += '\n';
         }
       }
+      pieces[b].end = s.size();
     }
   };
-  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
-  const int T = std::max(1, std::min<int>(threads, (int)nblocks));
   std::vector<std::thread> pool;
-  for (int t = 1; t < T; ++t) pool.emplace_back(work);
-  work();
+  for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+  work(0);
   for (auto &t : pool) t.join();
   size_t bytes = out->size();
-  for (const auto &t : text) bytes += t.size();
+  for (const Piece &pc : pieces) bytes += pc.end - pc.begin;
   if (out->capacity() < bytes) out->reserve(std::max(bytes, 2 * out->capacity()));
-  for (const auto &t : text) out->append(t);
+  for (const Piece &pc : pieces) out->append(buffers[pc.buffer], pc.begin, pc.end - pc.begin);
 }
 
 }  // namespace gz

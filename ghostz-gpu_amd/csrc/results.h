@@ -157,6 +157,13 @@ TracebackOut TracebackOneHost(const uint8_t *query, uint32_t query_length,
                               const uint8_t *database, uint32_t hit_q, uint32_t hit_d,
                               const SearchParameters &p, int cutoff);
 
+// A double as printf's %g writes it (six significant digits, rounded half
+// to even on the exact binary value, trailing zeros dropped), which is how
+// the reference's std::ostream writes the identity, E-value and bit score
+// (src/aligner_common.h:372-389).  Writes at most 13 characters, so a
+// 32-byte buffer always holds the text; returns its end.
+char *FormatG6(double v, char *p, char *end);
+
 // AlignerCommon::WriteOutput (src/aligner_common.h:345-397); the text form
 // appends the lines of queries [qb, qe) to *out, formatted on up to
 // `threads` threads (0: all cores).

@@ -13,9 +13,11 @@
  * This lets CPU tests (and sanitizer builds) exercise the product's host
  * code without a GPU.  The product library never contains this file.
  */
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <charconv>
 #include <string>
 #include <vector>
 
@@ -193,6 +195,27 @@ std::vector<FastaRecord> Records(const char *const *names, const char *residues,
 extern "C" {
 
 void gzh_force_stale(int every) { g_forced_stale = every; }
+
+/* The product's output number formatter (FormatG6) on n doubles, each text
+ * compared with printf's "%g" and with std::to_chars(general, 6).  Returns
+ * the number of values where any of the three differ; *first_bad = the
+ * first such index (n when none).  out (n * 32 bytes, may be NULL) receives
+ * the product's texts, NUL-terminated. */
+uint64_t gzh_format_g6_check(const double *v, uint64_t n, char *out, uint64_t *first_bad) {
+  uint64_t bad = 0;
+  *first_bad = n;
+  for (uint64_t i = 0; i < n; ++i) {
+    char a[32], b[32], c[32];
+    *gz::FormatG6(v[i], a, a + sizeof(a)) = 0;
+    snprintf(b, sizeof(b), "%g", v[i]);
+    *std::to_chars(c, c + sizeof(c), v[i], std::chars_format::general, 6).ptr = 0;
+    if (strcmp(a, b) != 0 || strcmp(a, c) != 0) {
+      if (bad++ == 0) *first_bad = i;
+    }
+    if (out) memcpy(out + i * 32, a, 32);
+  }
+  return bad;
+}
 
 /* The product's host flow with CPU-emulated device stages.  stats[0..5] =
  * seed hits, chained hits, gapped extensions, gapped cells, results,

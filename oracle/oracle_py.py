@@ -306,6 +306,8 @@ def hostpipe(path=None):
         L.gzh_distance_words.restype = None
         L.gzh_distance_words.argtypes = [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p,
                                          c_uint32, c_void_p, c_void_p]
+        L.gzh_format_g6_check.restype = c_uint64
+        L.gzh_format_g6_check.argtypes = [c_void_p, c_uint64, c_void_p, _P(c_uint64)]
         if path:
             return L
         _hostpipe = L
