@@ -230,10 +230,11 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
       learned ? dynamic_cast<const GpuCostModel *>(opt_.cost_model) : nullptr;
   const bool order = keys != nullptr || model != nullptr;
   // workspace: positions, scores, cells; and for an ordered run the keys,
-  // the permutation, permuted copies, predicted costs and the sort scratch
+  // the permutation, predicted costs and the sort scratch (the extension
+  // kernels read the jobs through the permutation)
   const size_t a4 = AlignUp(n * 4, 256), a2 = AlignUp(n * 2, 256);
   size_t bytes = 4 * a4;
-  if (order) bytes += a2 + 5 * a4 + a4 + AlignUp(ScheduleTempBytes(n), 256);
+  if (order) bytes += a2 + 2 * a4 + AlignUp(ScheduleTempBytes(n), 256);
   if (gapped_ws_cap_ < bytes) {
     if (gapped_ws_) (void)hipFree(gapped_ws_);
     gapped_ws_ = nullptr;
@@ -251,24 +252,18 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
   int *d_sc = (int *)take(a4);
   uint32_t *d_cells = (uint32_t *)take(a4);
   uint16_t *d_keys = nullptr;
-  uint32_t *d_perm = nullptr, *d_jq = nullptr, *d_jd = nullptr, *d_jc = nullptr;
-  int *d_js = nullptr;
+  uint32_t *d_perm = nullptr;
   float *d_pred = nullptr;
   void *d_tmp = nullptr;
   if (order) {
     d_keys = (uint16_t *)take(a2);
     d_perm = (uint32_t *)take(a4);
-    d_jq = (uint32_t *)take(a4);
-    d_jd = (uint32_t *)take(a4);
-    d_js = (int *)take(a4);
-    d_jc = (uint32_t *)take(a4);
     d_pred = (float *)take(a4);
     d_tmp = take(AlignUp(ScheduleTempBytes(n), 256));
   }
   hipError_t e = hipMemcpyAsync(d_pq, qpos, n * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_pd, dpos, n * 4, hipMemcpyHostToDevice, s);
-  uint32_t *rq = d_pq, *rd = d_pd, *rc = cells ? d_cells : nullptr;
-  int *rs = d_sc;
+  uint32_t *rc = cells ? d_cells : nullptr;
   if (e == hipSuccess && order) {
     // order the jobs by descending key inside each side, on the device
     e = hipEventRecord(ev_[2], s);
@@ -284,32 +279,25 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
     }
     if (e == hipSuccess) e = DeviceOrderByKey(d_keys, nr, d_perm, d_tmp, s);
     if (e == hipSuccess) e = DeviceOrderByKey(d_keys + nr, nf, d_perm + nr, d_tmp, s);
-    if (e == hipSuccess) e = GatherPair(d_perm, nr, d_pq, d_pd, d_jq, d_jd, s);
-    if (e == hipSuccess)
-      e = GatherPair(d_perm + nr, nf, d_pq + nr, d_pd + nr, d_jq + nr, d_jd + nr, s);
     if (e == hipSuccess) e = hipEventRecord(ev_[3], s);
-    rq = d_jq;
-    rd = d_jd;
-    rs = d_js;
-    rc = cells ? d_jc : nullptr;
   }
+  // each side's jobs run in the permutation's order (when ordered) and
+  // leave their results at their own indices
   gz_stream_s wrap{s};
+  auto side = [&](size_t off, size_t count, int reverse) {
+    uint32_t *c = rc ? rc + off : nullptr;
+    const gz_status r =
+        order ? gz_gapped_extender_extend_one_side_score_only_ordered_async(
+                    gapped_, count, reverse, d_perm + off, d_pq + off, d_pd + off, d_sc + off,
+                    c, &wrap)
+              : gz_gapped_extender_extend_one_side_score_only_cells_async(
+                    gapped_, count, reverse, d_pq + off, d_pd + off, d_sc + off, c, &wrap);
+    return r == GZ_OK;
+  };
   if (e == hipSuccess) e = hipEventRecord(ev_[0], s);
-  if (e == hipSuccess && nr > 0 &&
-      gz_gapped_extender_extend_one_side_score_only_cells_async(gapped_, nr, 1, rq, rd, rs, rc,
-                                                                &wrap) != GZ_OK)
-    e = hipErrorUnknown;
-  if (e == hipSuccess && nf > 0 &&
-      gz_gapped_extender_extend_one_side_score_only_cells_async(
-          gapped_, nf, 0, rq + nr, rd + nr, rs + nr, rc ? rc + nr : nullptr, &wrap) != GZ_OK)
-    e = hipErrorUnknown;
+  if (e == hipSuccess && nr > 0 && !side(0, nr, 1)) e = hipErrorUnknown;
+  if (e == hipSuccess && nf > 0 && !side(nr, nf, 0)) e = hipErrorUnknown;
   if (e == hipSuccess) e = hipEventRecord(ev_[1], s);
-  if (e == hipSuccess && order) {
-    e = ScatterBack(d_perm, nr, d_jq, d_jd, d_js, rc, d_pq, d_pd, d_sc, rc ? d_cells : nullptr, s);
-    if (e == hipSuccess)
-      e = ScatterBack(d_perm + nr, nf, d_jq + nr, d_jd + nr, d_js + nr, rc ? rc + nr : nullptr,
-                      d_pq + nr, d_pd + nr, d_sc + nr, rc ? d_cells + nr : nullptr, s);
-  }
   if (e == hipSuccess) e = hipMemcpyAsync(qpos, d_pq, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(dpos, d_pd, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(scores, d_sc, n * 4, hipMemcpyDeviceToHost, s);

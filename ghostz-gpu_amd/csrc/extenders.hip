@@ -344,15 +344,17 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
 
 constexpr int kGappedBlock = 64;
 
-// Jobs [begin, end), or the `*list_count` jobs of `list` when list is set;
-// a job whose band outgrows the W-column ring is appended to the overflow
-// list for the next, wider ring.
+// Jobs [begin, end) (through `order` when set: slot k runs job order[k],
+// which must be below `limit`), or the `*list_count` jobs of `list` when
+// list is set; a job whose band outgrows the W-column ring is appended to
+// the overflow list for the next, wider ring.  Results land at each job's
+// own index.
 template <bool kReverse, int W, bool kShort>
 __global__ void __launch_bounds__(kGappedBlock)
     GappedKernel(size_t begin, size_t end, const uint32_t *list, const uint32_t *list_count,
-                 GappedParams P, const int *matrix, uint32_t nl, uint32_t *qpos,
-                 uint32_t *dpos, int *scores, uint32_t *cells, uint32_t *overflow_count,
-                 uint32_t *overflow_list) {
+                 const uint32_t *order, size_t limit, GappedParams P, const int *matrix,
+                 uint32_t nl, uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
+                 uint32_t *overflow_count, uint32_t *overflow_list, uint32_t *failed) {
   using Ring = typename std::conditional<kShort, LaneRingLds16, LaneRingLds>::type;
   using Cell = typename std::conditional<kShort, short2, int2>::type;
   __shared__ int8_t sm[kTableDim * kTableDim];
@@ -368,7 +370,11 @@ __global__ void __launch_bounds__(kGappedBlock)
   const size_t n = list ? (size_t)*list_count : end - begin;
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x) {
-    const size_t i = list ? (size_t)list[k] : begin + k;
+    const size_t i = list ? (size_t)list[k] : order ? (size_t)order[begin + k] : begin + k;
+    if (i >= limit) {  // an order entry out of range: counted, never read
+      atomicAdd(failed, 1u);
+      continue;
+    }
     const uint32_t q = qpos[i], d = dpos[i];
     int s, p0, p1;
     uint32_t nc;
@@ -819,8 +825,8 @@ gz_status gz_gapped_extender_convert_to_gapped_extension_seeds_async(
 extern "C++" {
 template <bool kReverse>
 static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t size,
-                         size_t wide_first, uint32_t *d_q, uint32_t *d_d, int *d_scores,
-                         uint32_t *d_cells, hipStream_t s) {
+                         size_t wide_first, const uint32_t *order, uint32_t *d_q,
+                         uint32_t *d_d, int *d_scores, uint32_t *d_cells, hipStream_t s) {
   uint32_t *narrow_out = h->counters, *failures = h->counters + 1, *wide_out = h->counters + 2;
   // jobs [0, wide_first) start on the 64-column ring, the rest on the
   // narrow one; the 64-column ring then takes the narrow ring's overflow
@@ -828,8 +834,9 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
     hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing, false>),
                        dim3(GridFor(wide_first, kGappedBlock, 256u * 32)), dim3(kGappedBlock),
                        0, s, (size_t)0, wide_first, (const uint32_t *)nullptr,
-                       (const uint32_t *)nullptr, P, h->matrix, h->number_letters, d_q, d_d,
-                       d_scores, d_cells, wide_out, h->overflow_list2);
+                       (const uint32_t *)nullptr, order, size, P, h->matrix,
+                       h->number_letters, d_q, d_d, d_scores, d_cells, wide_out,
+                       h->overflow_list2, failures);
   if (size > wide_first) {
     const dim3 grid(GridFor(size - wide_first, kGappedBlock, 256u * 64));
     // 16-bit cells in the first ring when the cutoff and gap costs allow them
@@ -840,8 +847,8 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
 #define GZ_NARROW(WIDTH, SHORT)                                                              \
   hipLaunchKernelGGL((GappedKernel<kReverse, WIDTH, SHORT>), grid, dim3(kGappedBlock), 0, s, \
                      wide_first, size, (const uint32_t *)nullptr, (const uint32_t *)nullptr, \
-                     P, h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells,           \
-                     narrow_out, h->overflow_list)
+                     order, size, P, h->matrix, h->number_letters, d_q, d_d, d_scores,       \
+                     d_cells, narrow_out, h->overflow_list, failures)
     if (h->narrow == 16) {
       if (short_cells) GZ_NARROW(16, true);
       else GZ_NARROW(16, false);
@@ -853,8 +860,9 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
   }
   hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing, false>), dim3(256u * 4), dim3(kGappedBlock),
                      0, s, (size_t)0, (size_t)0, (const uint32_t *)h->overflow_list,
-                     (const uint32_t *)narrow_out, P, h->matrix, h->number_letters, d_q, d_d,
-                     d_scores, d_cells, wide_out, h->overflow_list2);
+                     (const uint32_t *)narrow_out, (const uint32_t *)nullptr, size, P,
+                     h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells, wide_out,
+                     h->overflow_list2, failures);
   hipLaunchKernelGGL((GappedWideKernel<kReverse>), dim3(kWideSlots / kGappedBlock),
                      dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q, d_d,
                      d_scores, d_cells, wide_out, h->overflow_list2, h->wide_cells,
@@ -864,7 +872,8 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
 
 static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
                            uint32_t *d_q, uint32_t *d_d, int *d_scores,
-                           uint32_t *d_cells, hipStream_t s, size_t wide_first = 0) {
+                           uint32_t *d_cells, hipStream_t s, size_t wide_first = 0,
+                           const uint32_t *order = nullptr) {
   GZ_REQUIRE(h->queries && h->database && h->matrix,
              "gapped extender: queries, database and score parameters must be set");
   GZ_REQUIRE(wide_first <= size, "gapped extender: wide_first beyond the jobs");
@@ -900,8 +909,8 @@ static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
   P.gap_open = h->gap_open;
   P.gap_ext = h->gap_ext;
   P.cutoff = h->cutoff;
-  if (reverse) GappedLaunch<true>(h, P, size, wide_first, d_q, d_d, d_scores, d_cells, s);
-  else GappedLaunch<false>(h, P, size, wide_first, d_q, d_d, d_scores, d_cells, s);
+  if (reverse) GappedLaunch<true>(h, P, size, wide_first, order, d_q, d_d, d_scores, d_cells, s);
+  else GappedLaunch<false>(h, P, size, wide_first, order, d_q, d_d, d_scores, d_cells, s);
   GZ_HIP_CHECK(hipGetLastError());
   return GZ_OK;
 }
@@ -937,6 +946,14 @@ gz_status gz_gapped_extender_extend_one_side_score_only_binned_async(
   GZ_REQUIRE(h, "null handle");
   if (size == 0) return GZ_OK;
   return GappedRun(h, size, reverse, d_q, d_d, d_scores, d_cells, S(stream), wide_first);
+}
+
+gz_status gz_gapped_extender_extend_one_side_score_only_ordered_async(
+    gz_gapped_extender_t h, size_t size, int reverse, const uint32_t *d_order, uint32_t *d_q,
+    uint32_t *d_d, int *d_scores, uint32_t *d_cells, gz_stream_t stream) {
+  GZ_REQUIRE(h && d_order, "null argument");
+  if (size == 0) return GZ_OK;
+  return GappedRun(h, size, reverse, d_q, d_d, d_scores, d_cells, S(stream), 0, d_order);
 }
 
 gz_status gz_gapped_extender_set_max_sequence_length(gz_gapped_extender_t h,

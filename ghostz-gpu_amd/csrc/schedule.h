@@ -27,13 +27,4 @@ hipError_t DeviceOrderByKey(const uint16_t *keys, uint32_t n, uint32_t *perm, vo
 hipError_t CostToKey(const float *cost, uint32_t n, float scale, uint16_t *keys,
                      hipStream_t stream);
 
-// out[k] = in[perm[k]] for two u32 arrays.
-hipError_t GatherPair(const uint32_t *perm, uint32_t n, const uint32_t *a, const uint32_t *b,
-                      uint32_t *out_a, uint32_t *out_b, hipStream_t stream);
-
-// out[perm[k]] = in[k] for two u32 arrays and one i32 array (optional third u32).
-hipError_t ScatterBack(const uint32_t *perm, uint32_t n, const uint32_t *a, const uint32_t *b,
-                       const int *c, const uint32_t *d, uint32_t *out_a, uint32_t *out_b,
-                       int *out_c, uint32_t *out_d, hipStream_t stream);
-
 }  // namespace gz

@@ -79,27 +79,6 @@ __global__ void CostToKeyKernel(const float *cost, uint32_t n, float scale, uint
   keys[i] = (uint16_t)k;
 }
 
-__global__ void GatherPairKernel(const uint32_t *perm, uint32_t n, const uint32_t *a,
-                                 const uint32_t *b, uint32_t *oa, uint32_t *ob) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t i = perm[k];
-  oa[k] = a[i];
-  ob[k] = b[i];
-}
-
-__global__ void ScatterBackKernel(const uint32_t *perm, uint32_t n, const uint32_t *a,
-                                  const uint32_t *b, const int *c, const uint32_t *d,
-                                  uint32_t *oa, uint32_t *ob, int *oc, uint32_t *od) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t i = perm[k];
-  oa[i] = a[k];
-  ob[i] = b[k];
-  oc[i] = c[k];
-  if (d) od[i] = d[k];
-}
-
 inline unsigned Blocks(uint32_t n, unsigned t = 256) { return (n + t - 1) / t; }
 
 }  // namespace
@@ -125,22 +104,6 @@ hipError_t DeviceOrderByKey(const uint16_t *keys, uint32_t n, uint32_t *perm, vo
 hipError_t CostToKey(const float *cost, uint32_t n, float scale, uint16_t *keys, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(CostToKeyKernel, dim3(Blocks(n)), dim3(256), 0, s, cost, n, scale, keys);
-  return hipGetLastError();
-}
-
-hipError_t GatherPair(const uint32_t *perm, uint32_t n, const uint32_t *a, const uint32_t *b,
-                      uint32_t *oa, uint32_t *ob, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(GatherPairKernel, dim3(Blocks(n)), dim3(256), 0, s, perm, n, a, b, oa, ob);
-  return hipGetLastError();
-}
-
-hipError_t ScatterBack(const uint32_t *perm, uint32_t n, const uint32_t *a, const uint32_t *b,
-                       const int *c, const uint32_t *d, uint32_t *oa, uint32_t *ob, int *oc,
-                       uint32_t *od, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(ScatterBackKernel, dim3(Blocks(n)), dim3(256), 0, s, perm, n, a, b, c, d,
-                     oa, ob, oc, od);
   return hipGetLastError();
 }
 

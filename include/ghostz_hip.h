@@ -191,6 +191,15 @@ gz_status gz_gapped_extender_extend_one_side_score_only_binned_async(
     gz_gapped_extender_t h, size_t size, size_t wide_first, int reverse,
     uint32_t *d_query_concatenated_positions, uint32_t *d_database_positions,
     int *d_scores, uint32_t *d_cells, gz_stream_t stream);
+/* Same, with the jobs run in the order d_order gives (slot k runs job
+ * d_order[k], a permutation of [0, size)); each job reads and writes its own
+ * index, so a caller that orders jobs needs no gathered copies.  No
+ * reference counterpart (the learned schedule's order).  An entry of
+ * d_order at or past size is skipped and counted as a failure. */
+gz_status gz_gapped_extender_extend_one_side_score_only_ordered_async(
+    gz_gapped_extender_t h, size_t size, int reverse, const uint32_t *d_order,
+    uint32_t *d_query_concatenated_positions, uint32_t *d_database_positions,
+    int *d_scores, uint32_t *d_cells, gz_stream_t stream);
 /* Longest query the extender must handle (sizes the wide-band fallback
  * ring; default 65534 residues).  No counterpart in the reference, whose
  * GPU kernel caps the query side at 128 residues

@@ -252,6 +252,62 @@ def test_gapped_kernel_ring_tiers(dev, narrow, wide_first):
         assert bad.size == 0, (bad[:5], got[bad[:5]], want[bad[:5]])
 
 
+def test_gapped_kernel_ordered_jobs(dev):
+    """Jobs run through an order array (the learned schedule's permutation)
+    give every job the same score, end positions and DP cells as the
+    unordered call, at the job's own index; an order entry past the end is
+    skipped by the kernel and reported as a failure."""
+    rng = np.random.default_rng(41)
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 1500, min_len=40, max_len=900,
+                                                identity=(0.4, 0.98), indel=0.05)
+    n = qpos.size
+    say("ordered gapped jobs", n)
+    L = gz.lib()
+    _, qp = gz.pack_on_device(q)
+    _, dp = gz.pack_on_device(d)
+    h = ctypes.c_void_p()
+    gz.check(L.gz_gapped_extender_create(ctypes.byref(h)))
+    gz.check(L.gz_gapped_extender_set_queries(h, DELIM, qp.ptr))
+    gz.check(L.gz_gapped_extender_set_database(h, dp.ptr))
+    gz.check(L.gz_gapped_extender_set_score_parameters(h, dev["d_matrix"].ptr, dev["nl"], 11, 1,
+                                                       52))
+    s = gz.Stream()
+
+    def run(reverse, order):
+        a, b = (qpos - 1, dpos - 1) if reverse else (qpos, dpos)
+        d_q, d_d = gz.DeviceBuffer.from_array(a.astype(np.uint32)), \
+            gz.DeviceBuffer.from_array(b.astype(np.uint32))
+        d_s, d_c = gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n * 4)
+        if order is None:
+            gz.check(L.gz_gapped_extender_extend_one_side_score_only_cells_async(
+                h, n, int(reverse), d_q.ptr, d_d.ptr, d_s.ptr, d_c.ptr, s.handle))
+        else:
+            d_o = gz.DeviceBuffer.from_array(order.astype(np.uint32))
+            gz.check(L.gz_gapped_extender_extend_one_side_score_only_ordered_async(
+                h, n, int(reverse), d_o.ptr, d_q.ptr, d_d.ptr, d_s.ptr, d_c.ptr, s.handle))
+        s.sync()
+        return [x.to_array(t, n) for x, t in ((d_s, np.int32), (d_q, np.uint32),
+                                               (d_d, np.uint32), (d_c, np.uint32))]
+
+    def failures():
+        f = (ctypes.c_uint32 * 1)()
+        gz.check(L.gz_gapped_extender_failures(h, f))
+        return f[0]
+
+    for reverse in (False, True):
+        want = run(reverse, None)
+        for order in (rng.permutation(n), np.arange(n)[::-1]):
+            got = run(reverse, order)
+            for g, w in zip(got, want):
+                assert np.array_equal(g, w)
+        assert failures() == 0
+    bad = rng.permutation(n)
+    bad[7] = n + 5  # out of range: that slot is skipped and counted
+    got = run(False, bad)
+    assert failures() == 1
+    L.gz_gapped_extender_destroy(h)
+
+
 def test_gapped_kernel_wide_band(dev):
     """cutoff = INT_MAX disables X-drop: bands span whole sequences, so the
     LDS ring overflows and the wide-ring pass must produce the answers."""
