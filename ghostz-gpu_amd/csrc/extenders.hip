@@ -828,22 +828,25 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
                          size_t wide_first, const uint32_t *order, uint32_t *d_q,
                          uint32_t *d_d, int *d_scores, uint32_t *d_cells, hipStream_t s) {
   uint32_t *narrow_out = h->counters, *failures = h->counters + 1, *wide_out = h->counters + 2;
+  // 16-bit cells in the LDS rings when the cutoff and gap costs allow them
+  // (LaneRingLds16); bands whose scores grow past them move on like bands
+  // that outgrow the ring
+  const bool short_cells = P.cutoff <= 8192 && P.gap_ext >= 0 &&
+                           P.gap_open + P.gap_ext <= 1024;
   // jobs [0, wide_first) start on the 64-column ring, the rest on the
   // narrow one; the 64-column ring then takes the narrow ring's overflow
-  if (wide_first > 0)
-    hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing, false>),
-                       dim3(GridFor(wide_first, kGappedBlock, 256u * 32)), dim3(kGappedBlock),
-                       0, s, (size_t)0, wide_first, (const uint32_t *)nullptr,
-                       (const uint32_t *)nullptr, order, size, P, h->matrix,
-                       h->number_letters, d_q, d_d, d_scores, d_cells, wide_out,
-                       h->overflow_list2, failures);
+#define GZ_RING64(SHORT, GRID, BEGIN, END, LIST, COUNT, ORDER)                                 \
+  hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing, SHORT>), GRID, dim3(kGappedBlock), 0, \
+                     s, (size_t)(BEGIN), (size_t)(END), (const uint32_t *)(LIST),              \
+                     (const uint32_t *)(COUNT), ORDER, size, P, h->matrix, h->number_letters,  \
+                     d_q, d_d, d_scores, d_cells, wide_out, h->overflow_list2, failures)
+  if (wide_first > 0) {
+    const dim3 grid(GridFor(wide_first, kGappedBlock, 256u * 32));
+    if (short_cells) GZ_RING64(true, grid, 0, wide_first, nullptr, nullptr, order);
+    else GZ_RING64(false, grid, 0, wide_first, nullptr, nullptr, order);
+  }
   if (size > wide_first) {
     const dim3 grid(GridFor(size - wide_first, kGappedBlock, 256u * 64));
-    // 16-bit cells in the first ring when the cutoff and gap costs allow them
-    // (LaneRingLds16); bands whose scores grow past them move on like bands
-    // that outgrow the ring
-    const bool short_cells = P.cutoff <= 8192 && P.gap_ext >= 0 &&
-                             P.gap_open + P.gap_ext <= 1024;
 #define GZ_NARROW(WIDTH, SHORT)                                                              \
   hipLaunchKernelGGL((GappedKernel<kReverse, WIDTH, SHORT>), grid, dim3(kGappedBlock), 0, s, \
                      wide_first, size, (const uint32_t *)nullptr, (const uint32_t *)nullptr, \
@@ -858,11 +861,13 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
     }
 #undef GZ_NARROW
   }
-  hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing, false>), dim3(256u * 4), dim3(kGappedBlock),
-                     0, s, (size_t)0, (size_t)0, (const uint32_t *)h->overflow_list,
-                     (const uint32_t *)narrow_out, (const uint32_t *)nullptr, size, P,
-                     h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells, wide_out,
-                     h->overflow_list2, failures);
+  {
+    // the narrow ring's overflow; 16-bit cells leave LDS for more blocks
+    const uint32_t *none = nullptr;
+    if (short_cells) GZ_RING64(true, dim3(256u * 6), 0, 0, h->overflow_list, narrow_out, none);
+    else GZ_RING64(false, dim3(256u * 4), 0, 0, h->overflow_list, narrow_out, none);
+  }
+#undef GZ_RING64
   hipLaunchKernelGGL((GappedWideKernel<kReverse>), dim3(kWideSlots / kGappedBlock),
                      dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q, d_d,
                      d_scores, d_cells, wide_out, h->overflow_list2, h->wide_cells,
