@@ -158,6 +158,29 @@ def test_result_limits_and_no_filter(matrix):
     run_case("limits", q, db, m, nl, max_results=3, max_results_per_subject=2)
 
 
+def long_protein_case():
+    """Queries of 2,000-5,000 residues that are near copies of database
+    subjects (scores far past 16 bits), one distant homolog, one unrelated."""
+    rng = np.random.default_rng(20261209)
+    db = synthetic.protein_database(rng, 400)
+    long_subjects = [synthetic.random_codes(rng, int(n)) for n in rng.integers(2000, 5000, 8)]
+    db = db + long_subjects
+    queries = [synthetic.substitute(rng, s, 0.99) for s in long_subjects[:6]]
+    queries += [synthetic.homolog(rng, long_subjects[6], 0.9), synthetic.random_codes(rng, 3000)]
+    return synthetic._pack(queries, "query_"), synthetic._pack(db, "subject_")
+
+
+def test_long_near_identical_proteins_match_oracle(matrix):
+    """Ungapped and gapped scores run past 16 bits: the gapped bands leave
+    the 16-bit rings for the global ring, and the result build walks
+    alignments thousands of residues long.  Every form gives the oracle's
+    text."""
+    m, nl = matrix
+    (qn, qr, qo), (dn, dr, do) = long_protein_case()
+    run_case("long", gz.Queries(qn, qr, qo), gz.Database(dn, dr, do), m, nl,
+             schedules=(gz.SCHEDULE_PLAIN, gz.SCHEDULE_LEARNED))
+
+
 def test_edge_queries_match_oracle(matrix):
     m, nl = matrix
     rng = np.random.default_rng(20261208)
