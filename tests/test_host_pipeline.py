@@ -11,7 +11,7 @@ import pytest
 import gz_native as gz
 import oracle_py
 import synthetic
-from test_pipeline_gpu import oracle_text
+from test_pipeline_gpu import long_protein_case, oracle_text
 
 
 def check(name, qd, dd, translated=False, filter=True, clustering=True, db_chunk_size=0,
@@ -41,6 +41,12 @@ def check(name, qd, dd, translated=False, filter=True, clustering=True, db_chunk
 def test_protein():
     w = check("protein", *synthetic.protein_workload(31, 1500, 40))
     assert w.count(b"\n") > 40
+
+
+def test_long_near_identical_proteins():
+    # alignments thousands of residues long, scores past 16 bits
+    qd, dd = long_protein_case()
+    check("long", qd, dd)
 
 
 def test_translated():
