@@ -118,7 +118,7 @@ def test_translated_search_matches_oracle(matrix):
     (qn, qr, qo), (dn, dr, do) = synthetic.translated_workload(20261203, 2000, 800)
     q = gz.Queries(qn, qr, qo, translated=True)
     db = gz.Database(dn, dr, do)
-    run_case("translated", q, db, m, nl)
+    run_case("translated", q, db, m, nl, schedules=(gz.SCHEDULE_PLAIN, gz.SCHEDULE_LEARNED))
 
 
 def test_unclustered_database_matches_oracle(matrix):
@@ -147,7 +147,7 @@ def test_uneven_queries_match_oracle(matrix):
     q = gz.Queries(qn, qr, qo)
     db = gz.Database(dn, dr, do)
     run_case("uneven", q, db, m, nl,
-             schedules=(gz.SCHEDULE_PLAIN, gz.SCHEDULE_LENGTH_SORTED))
+             schedules=(gz.SCHEDULE_PLAIN, gz.SCHEDULE_LENGTH_SORTED, gz.SCHEDULE_LEARNED))
 
 
 def test_result_limits_and_no_filter(matrix):
