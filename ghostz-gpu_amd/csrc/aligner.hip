@@ -268,8 +268,15 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
     // order the jobs by descending key inside each side, on the device
     e = hipEventRecord(ev_[2], s);
     if (e == hipSuccess && model) {
-      e = model->PredictOnDevice(dq_.packed, dc.packed, d_pq, d_pd, nr, true, d_pred, s);
-      if (e == hipSuccess)
+      // the host lays out job nr + k as the forward side of job k (both
+      // positions one later), so one feature pass serves both sides; were
+      // that not so, only the order (never a result) would differ
+      if (nr == nf)
+        e = model->PredictBothSidesOnDevice(dq_.packed, dc.packed, d_pq, d_pd, nr, d_pred,
+                                            d_pred + nr, s);
+      else
+        e = model->PredictOnDevice(dq_.packed, dc.packed, d_pq, d_pd, nr, true, d_pred, s);
+      if (e == hipSuccess && nr != nf)
         e = model->PredictOnDevice(dq_.packed, dc.packed, d_pq + nr, d_pd + nr, nf, false,
                                    d_pred + nr, s);
       // log2 cells in steps of 1/32

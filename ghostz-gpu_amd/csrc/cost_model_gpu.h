@@ -30,8 +30,20 @@ class GpuCostModel : public CostModel {
   hipError_t PredictOnDevice(const uint64_t *packed_queries, const uint64_t *packed_database,
                              const uint32_t *d_qpos, const uint32_t *d_dpos, uint32_t n,
                              bool reverse, float *d_out, hipStream_t stream) const;
+  // Both sides of n hits: job k extends backwards from (d_qpos[k], d_dpos[k])
+  // and its forward side starts one position later.  The forward side's
+  // windows are the backward side's in the other order (along the
+  // extension, then away), so one feature pass serves both: the forward
+  // predictions read each row with its halves swapped.
+  hipError_t PredictBothSidesOnDevice(const uint64_t *packed_queries,
+                                      const uint64_t *packed_database, const uint32_t *d_qpos,
+                                      const uint32_t *d_dpos, uint32_t n, float *d_out_reverse,
+                                      float *d_out_forward, hipStream_t stream) const;
 
  private:
+  hipError_t Predict2(const uint64_t *packed_queries, const uint64_t *packed_database,
+                      const uint32_t *d_qpos, const uint32_t *d_dpos, uint32_t n, bool reverse,
+                      float *d_out, float *d_out_other, hipStream_t stream) const;
   struct Impl;
   Impl *impl_;
 };

@@ -140,6 +140,23 @@ hipError_t GpuCostModel::PredictOnDevice(const uint64_t *pq, const uint64_t *pd,
                                          const uint32_t *d_qpos, const uint32_t *d_dpos,
                                          uint32_t n, bool reverse, float *d_out,
                                          hipStream_t s) const {
+  return Predict2(pq, pd, d_qpos, d_dpos, n, reverse, d_out, nullptr, s);
+}
+
+hipError_t GpuCostModel::PredictBothSidesOnDevice(const uint64_t *pq, const uint64_t *pd,
+                                                  const uint32_t *d_qpos,
+                                                  const uint32_t *d_dpos, uint32_t n,
+                                                  float *d_out_reverse, float *d_out_forward,
+                                                  hipStream_t s) const {
+  return Predict2(pq, pd, d_qpos, d_dpos, n, true, d_out_reverse, d_out_forward, s);
+}
+
+// features of n jobs on side `reverse` into d_out; with d_out_other, also the
+// other side's predictions from the same rows, halves swapped
+hipError_t GpuCostModel::Predict2(const uint64_t *pq, const uint64_t *pd,
+                                  const uint32_t *d_qpos, const uint32_t *d_dpos, uint32_t n,
+                                  bool reverse, float *d_out, float *d_out_other,
+                                  hipStream_t s) const {
   if (n == 0) return hipSuccess;
   const size_t chunk = (size_t)1 << 22;
   const size_t want = std::min<size_t>(n, chunk) * kCostResidues;
@@ -156,6 +173,8 @@ hipError_t GpuCostModel::PredictOnDevice(const uint64_t *pq, const uint64_t *pd,
     hipError_t e = CostFeaturesOnDevice(pq, pd, d_qpos + b, d_dpos + b, m, reverse,
                                         impl_->codes, s);
     if (e == hipSuccess) e = impl_->mlp.InferDevice(impl_->codes, m, d_out + b, s);
+    if (e == hipSuccess && d_out_other)
+      e = impl_->mlp.InferDevice(impl_->codes, m, d_out_other + b, s, kCostResidues / 8);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

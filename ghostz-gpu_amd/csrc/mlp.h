@@ -21,7 +21,7 @@ class MlpDevice {
   void SetWeights(const CostModelWeights &w);
   void GetWeights(CostModelWeights *w);
   // Predicted log2 cells for n seeds given their 128 feature codes each.
-  hipError_t Forward(const uint8_t *codes, size_t n, float *out);
+  hipError_t Forward(const uint8_t *codes, size_t n, float *out, uint32_t rotate_words = 0);
   // One training step on a batch: mean squared error against targets;
   // computes gradients, applies Adam (lr) unless lr == 0, and optionally
   // returns the flat gradient (same layout as the flat weights).
@@ -29,7 +29,10 @@ class MlpDevice {
                        float *loss, float *flat_grad);
   // Forward on device-resident codes (n x 128) into device outputs, on the
   // caller's stream (one persistent kernel for all three layers).
-  hipError_t InferDevice(const uint8_t *d_codes, size_t n, float *d_out, hipStream_t stream);
+  // rotate_words: each 128-code row is read from its 32-bit word
+  // rotate_words on, wrapping (16 swaps the row's two halves)
+  hipError_t InferDevice(const uint8_t *d_codes, size_t n, float *d_out, hipStream_t stream,
+                         uint32_t rotate_words = 0);
   // Kernel-only forward time of the last Forward call (HIP events).
   float last_forward_ms = 0;
 

@@ -249,6 +249,8 @@ constexpr int kStageVec = kCostH1 * kStageIn / 8;  // 16-byte pieces per stage
 constexpr int kPrefetch = kStageVec / (kInferWaves * 64);
 static_assert(kStageVec % (kInferWaves * 64) == 0, "a stage splits evenly over the block");
 static_assert(kStages % 2 == 0, "stages alternate between two buffers across tiles");
+static_assert((kStages & (kStages - 1)) == 0 && kStages * 4 == kCostResidues,
+              "a row is kStages words; rotations wrap with a mask");
 constexpr int kL2Steps = kCostH1 / 16;             // layer-2 K steps
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -274,7 +276,7 @@ __device__ __forceinline__ void PutPiece(uint16_t *w1s, int i, uint4 v) {
 __global__ void __launch_bounds__(kInferWaves * 64) __attribute__((amdgpu_waves_per_eu(2)))
     InferKernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h, const uint16_t *embh,
                 const float *b1, const uint16_t *w2h, const float *b2, const float *w3,
-                const float *b3, float *y) {
+                const float *b3, uint32_t rotate, float *y) {
   __shared__ __align__(16) uint16_t w1s[2][kStageIn * kCostH1];     // two stages, 32 KB
   __shared__ __align__(16) uint16_t w2[kL2Steps * 2 * 64 * 8];      // [t][row block][lane][8]
   __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
@@ -326,8 +328,10 @@ __global__ void __launch_bounds__(kInferWaves * 64) __attribute__((amdgpu_waves_
       for (int p = 0; p < kPrefetch; ++p)
         pf[p] = StagePiece(w1h, ns, (int)threadIdx.x + p * kInferWaves * 64);
       const uint16_t *cur = w1s[s & 1];
-      // residues 4 s .. 4 s + 3 of this lane's seed
-      const uint32_t cw = reinterpret_cast<const uint32_t *>(src)[s] | fill;
+      // residues 4 s .. 4 s + 3 of this lane's seed, the row read from word
+      // `rotate` on (16 swaps its two halves of windows)
+      const uint32_t cw =
+          reinterpret_cast<const uint32_t *>(src)[(s + (int)rotate) & (kStages - 1)] | fill;
       // a K step's operands are read as one batch, a step ahead of its
       // MFMAs (the scheduling barriers keep that order), so the LDS latency
       // overlaps the previous step's MFMAs
@@ -517,7 +521,7 @@ hipError_t MlpDevice::ForwardDevice(size_t n, size_t B) {
 }
 
 hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out,
-                                  hipStream_t s) {
+                                  hipStream_t s, uint32_t rotate_words) {
   if (n == 0) return hipSuccess;
   if (n > 0xFFFFFFFFull - kInferTile) return hipErrorInvalidValue;
   int dev = 0, cus = 256;
@@ -527,11 +531,11 @@ hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out
   hipLaunchKernelGGL(InferKernel, dim3(std::min<unsigned>(tiles, 2u * (unsigned)cus)),
                      dim3(kInferWaves * 64), 0, s, d_codes, (uint32_t)n, w1h_, embh_,
                      master_ + kOffB1, w2h_, master_ + kOffB2, master_ + kOffW3,
-                     master_ + kOffB3, d_out);
+                     master_ + kOffB3, rotate_words & (kStages - 1), d_out);
   return hipGetLastError();
 }
 
-hipError_t MlpDevice::Forward(const uint8_t *codes, size_t n, float *out) {
+hipError_t MlpDevice::Forward(const uint8_t *codes, size_t n, float *out, uint32_t rotate_words) {
   last_forward_ms = 0;
   if (n == 0) return hipSuccess;
   uint8_t *d_codes = nullptr;
@@ -544,7 +548,7 @@ hipError_t MlpDevice::Forward(const uint8_t *codes, size_t n, float *out) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(d_codes, codes, n * kCostResidues, hipMemcpyHostToDevice, stream_);
   if (e == hipSuccess) e = hipEventRecord(e0, stream_);
-  if (e == hipSuccess) e = InferDevice(d_codes, n, d_out, stream_);
+  if (e == hipSuccess) e = InferDevice(d_codes, n, d_out, stream_, rotate_words);
   if (e == hipSuccess) e = hipEventRecord(e1, stream_);
   if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * 4, hipMemcpyDeviceToHost, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
@@ -670,6 +674,12 @@ gz_status gz_mlp_predict(gz_mlp_t h, const uint8_t *codes, size_t n, float *out,
   GZ_REQUIRE(h && (n == 0 || (codes && out)), "gz_mlp_predict: null argument");
   GZ_HIP_CHECK(h->dev.Forward(codes, n, out));
   if (kernel_ms) *kernel_ms = h->dev.last_forward_ms;
+  return GZ_OK;
+}
+
+gz_status gz_mlp_predict_swapped(gz_mlp_t h, const uint8_t *codes, size_t n, float *out) {
+  GZ_REQUIRE(h && (n == 0 || (codes && out)), "gz_mlp_predict_swapped: null argument");
+  GZ_HIP_CHECK(h->dev.Forward(codes, n, out, kCostResidues / 8));
   return GZ_OK;
 }
 

@@ -335,6 +335,7 @@ _SIGNATURES.update({
     "gz_mlp_destroy": (c_int, [c_void_p]),
     "gz_mlp_get_weights": (c_int, [c_void_p, c_void_p]),
     "gz_mlp_predict": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, POINTER(c_float)]),
+    "gz_mlp_predict_swapped": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "gz_mlp_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_float,
                                   POINTER(c_float), c_void_p]),
     "gz_cost_features": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int,

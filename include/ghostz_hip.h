@@ -414,6 +414,11 @@ gz_status gz_mlp_get_weights(gz_mlp_t m, float *flat);
 /* Predicted log2 DP cells; kernel_ms (optional) = device forward time. */
 gz_status gz_mlp_predict(gz_mlp_t m, const uint8_t *codes, size_t n, float *out,
                          float *kernel_ms);
+/* The same for each row with its two halves of windows swapped (the other
+ * side of the same hit), without moving the codes: the aligner predicts a
+ * hit's forward side from its reverse side's features this way.  No
+ * reference counterpart. */
+gz_status gz_mlp_predict_swapped(gz_mlp_t m, const uint8_t *codes, size_t n, float *out);
 /* One MSE step: gradients (flat layout, optional out) and Adam (lr > 0). */
 gz_status gz_mlp_train_step(gz_mlp_t m, const uint8_t *codes, const float *targets,
                             size_t n, float lr, float *loss, float *flat_grad);

@@ -270,3 +270,19 @@ def test_chunking_follows_cumulative_residue_rule():
         assert ch["names"][0] == dn[seen].encode()
         seen += k
     assert seen == len(dn)
+
+
+def test_forward_features_are_reverse_features_swapped():
+    """The forward side of a hit (positions q, d) sees the reverse side's
+    (q - 1, d - 1) windows in the other order: along the extension, then away
+    from it.  The aligner relies on this to predict both sides from one
+    feature pass (the network reads the forward row with its halves
+    swapped)."""
+    from helpers import related_pair_set
+    rng = np.random.default_rng(13)
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 3000, min_len=5, max_len=90)
+    rev = gz.cost_features(q, d, np.ascontiguousarray(qpos - 1, np.uint32),
+                           np.ascontiguousarray(dpos - 1, np.uint32), 1)
+    fwd = gz.cost_features(q, d, np.ascontiguousarray(qpos, np.uint32),
+                           np.ascontiguousarray(dpos, np.uint32), 0)
+    assert np.array_equal(fwd, np.concatenate([rev[:, 64:], rev[:, :64]], axis=1))

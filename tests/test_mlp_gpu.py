@@ -131,6 +131,20 @@ def test_device_features_match_host_features(device):
         assert np.array_equal(got, want)
 
 
+def test_swapped_rows_predict_like_swapped_codes(device):
+    """Predictions read with each row's halves swapped (how the aligner
+    predicts a hit's forward side from its reverse side's features) equal
+    predictions of rows swapped on the host, bit for bit."""
+    rng = np.random.default_rng(14)
+    mlp = gz.Mlp(gz.Mlp.init_weights(5))
+    codes = rng.integers(0, 21, (5000, 128)).astype(np.uint8)
+    want = mlp.predict(np.concatenate([codes[:, 64:], codes[:, :64]], axis=1))
+    got = np.zeros(len(codes), np.float32)
+    gz.check(gz.lib().gz_mlp_predict_swapped(mlp.handle, gz.ptr(codes), len(codes), gz.ptr(got)))
+    say("swapped rows", len(codes))
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 def wave_cost(cells, order, lanes=64):
     """Lane-steps a wave-per-64-jobs launch spends: each wave runs as long as
     its most expensive job."""
