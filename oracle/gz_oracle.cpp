@@ -118,7 +118,11 @@ int Distance(const uint8_t *q, uint32_t qc, const uint8_t *d, uint32_t dc,
   const uint8_t *a = q + qc - half;
   const uint8_t *b = d + dc - half;
   char count = 0;  // DistanceCalculator::Distance is char
-  for (uint32_t i = 0; i < L; ++i) count += (map[a[i]] != map[b[i]]) ? 1 : 0;
+  // map holds the 25 letter codes; a delimiter (or any code past them)
+  // differs from every letter and equals only itself, and is never read
+  // from map (which has no entry for it)
+  auto reduced = [&](uint8_t c) { return c < 25 ? (int)map[c] : 256 + c; };
+  for (uint32_t i = 0; i < L; ++i) count += (reduced(a[i]) != reduced(b[i])) ? 1 : 0;
   return count;
 }
 
