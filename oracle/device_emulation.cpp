@@ -194,6 +194,52 @@ std::vector<FastaRecord> Records(const char *const *names, const char *residues,
 
 extern "C" {
 
+/* The product's AddResults (results.cpp) on the same arguments as
+ * gzo_add_results (gz_oracle.h); the subject of an added result comes from
+ * DbChunk::SubjectOf over `subject_offsets`. */
+int gzh_add_results(const uint32_t *subject_offsets, uint32_t number_subjects, int chunk_id,
+                    uint32_t max_results, uint32_t max_results_per_subject,
+                    gzo_presearched *added, uint32_t number_added, gzo_presearched *results,
+                    uint32_t *number_results) {
+  DbChunk c;
+  c.offsets.assign(subject_offsets, subject_offsets + number_subjects + 1);
+  c.names.assign(number_subjects, std::string());
+  SearchParameters p;
+  p.max_results = max_results;
+  p.max_results_per_subject = max_results_per_subject;
+  auto to = [](const gzo_presearched &x) {
+    Presearched r;
+    r.chunk = x.chunk;
+    r.subject = x.subject;
+    r.score = x.score;
+    r.hit = {x.hit_q, x.hit_d};
+    r.start = {x.start_q, x.start_d};
+    r.end = {x.end_q, x.end_d};
+    r.hit_count = x.hit_count;
+    return r;
+  };
+  std::vector<Presearched> a, r;
+  for (uint32_t i = 0; i < number_added; ++i) a.push_back(to(added[i]));
+  for (uint32_t i = 0; i < *number_results; ++i) r.push_back(to(results[i]));
+  AddResultsScratch scratch;
+  AddResults(c, chunk_id, p, scratch, a, r);
+  for (size_t i = 0; i < r.size(); ++i) {
+    gzo_presearched &x = results[i];
+    x.chunk = r[i].chunk;
+    x.subject = r[i].subject;
+    x.score = r[i].score;
+    x.hit_q = r[i].hit.q;
+    x.hit_d = r[i].hit.d;
+    x.start_q = r[i].start.q;
+    x.start_d = r[i].start.d;
+    x.end_q = r[i].end.q;
+    x.end_d = r[i].end.d;
+    x.hit_count = r[i].hit_count;
+  }
+  *number_results = (uint32_t)r.size();
+  return 0;
+}
+
 void gzh_force_stale(int every) { g_forced_stale = every; }
 
 /* The product's output number formatter (FormatG6) on n doubles, each text

@@ -755,7 +755,8 @@ void AddResults(const ChunkView &db, const gzo_parameters &p,
   for (size_t i = 0; i < nr.size(); ++i) starts[nr[i].subject].clear();
 }
 
-gzo_stats g_stats;
+// per thread, so that searches on several threads at once keep their own counts
+thread_local gzo_stats g_stats;
 
 struct ListEntry {
   bool check;
@@ -938,6 +939,60 @@ int gzo_query_lists(const gzo_queries *Qp, const gzo_database *Dp, uint64_t *num
     }
   }
   (*entry_offsets)[nl] = (uint32_t)o;
+  return 0;
+}
+
+int gzo_add_results(const uint32_t *subject_offsets, uint32_t number_subjects, int chunk_id,
+                    uint32_t max_results, uint32_t max_results_per_subject,
+                    gzo_presearched *added, uint32_t number_added, gzo_presearched *results,
+                    uint32_t *number_results) {
+  gzo_db_chunk c;
+  memset(&c, 0, sizeof(c));
+  c.offsets = subject_offsets;
+  c.number_sequences = number_subjects;
+  ChunkView db;
+  db.c = &c;
+  db.id = chunk_id;
+  gzo_parameters p;
+  memset(&p, 0, sizeof(p));
+  p.max_results = max_results;
+  p.max_results_per_subject = max_results_per_subject;
+  auto to = [](const gzo_presearched &x) {
+    Presearched r;
+    r.chunk = x.chunk;
+    r.subject = x.subject;
+    r.score = x.score;
+    r.hit.q = x.hit_q;
+    r.hit.d = x.hit_d;
+    r.start.q = x.start_q;
+    r.start.d = x.start_d;
+    r.end.q = x.end_q;
+    r.end.d = x.end_d;
+    r.hit_count = x.hit_count;
+    return r;
+  };
+  auto from = [](const Presearched &r) {
+    gzo_presearched x;
+    x.chunk = r.chunk;
+    x.subject = r.subject;
+    x.score = r.score;
+    x.hit_q = r.hit.q;
+    x.hit_d = r.hit.d;
+    x.start_q = r.start.q;
+    x.start_d = r.start.d;
+    x.end_q = r.end.q;
+    x.end_d = r.end.d;
+    x.hit_count = r.hit_count;
+    return x;
+  };
+  std::vector<Presearched> a, r;
+  for (uint32_t i = 0; i < number_added; ++i) a.push_back(to(added[i]));
+  for (uint32_t i = 0; i < *number_results; ++i) r.push_back(to(results[i]));
+  std::vector<std::vector<AlignmentPosition> > starts;
+  AddResults(db, p, starts, a, r);
+  for (uint32_t i = 0; i < number_added; ++i) added[i] = from(a[i]);
+  for (size_t i = 0; i < r.size(); ++i) results[i] = from(r[i]);
+  *number_results = (uint32_t)r.size();
   return 0;
 }
 

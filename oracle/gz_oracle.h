@@ -165,6 +165,26 @@ int gzo_query_lists(const gzo_queries *queries, const gzo_database *database,
                     uint64_t *number_entries, uint32_t **entry_queries,
                     uint32_t **entry_positions, uint8_t **entry_checks);
 
+/* One presearched result as AddResults sees it (PresearchedResult,
+ * src/aligner_common.h:70-78). */
+typedef struct {
+  int chunk;
+  uint32_t subject;
+  int score;
+  uint32_t hit_q, hit_d, start_q, start_d, end_q, end_d;
+  uint32_t hit_count;
+} gzo_presearched;
+
+/* AlignerCommon::AddResults (src/aligner_common.h:147-274) for one query:
+ * merges the `added` results of database chunk `chunk_id` (subject starts
+ * `subject_offsets`, number_subjects + 1 of them) into `results`
+ * (*number_results in, the merged count out; room for max_results).
+ * `added` is sorted in place as the reference sorts it.  Returns 0. */
+int gzo_add_results(const uint32_t *subject_offsets, uint32_t number_subjects, int chunk_id,
+                    uint32_t max_results, uint32_t max_results_per_subject,
+                    gzo_presearched *added, uint32_t number_added, gzo_presearched *results,
+                    uint32_t *number_results);
+
 /* Presearch-only statistics of the last gzo_align call on this thread. */
 typedef struct {
   uint64_t hash_lists;
