@@ -1,31 +1,44 @@
 """Throughput of the `aln` search path on one MI355X.
 
-Workload (default): 200,000 synthetic 150-nt reads, translated search
-(aln -q d -t p) against 1,000,000 synthetic proteins (~330M aa, one chunk
-resident in HBM), BLOSUM62 11/1, -b 10 -v 1, SEG on.  A step is one full
-search of all reads: query lists, seed search / distance / ungapped
-extension on the device, chain filter, gapped extension on the device,
-AddResults, BuildResults and tabular output.
+Workload (default, `--workload reads1m_db4m`): 1,000,000 synthetic 150-nt
+reads, translated search (aln -q d -t p) against 4,000,000 synthetic proteins
+(~1.32G aa in four database chunks of ~330M aa, all resident in HBM),
+BLOSUM62 11/1, -b 10 -v 1, SEG on.  The reads form three query chunks of
+333,334 / 333,333 / 333,333 reads.  A step is one query chunk searched
+against all four database chunks, exactly as the reference's per-query-chunk
+loop does it: query lists, seed search / distance / ungapped extension on the
+device, chain filter, gapped extension on the device, AddResults over the
+four database chunks, BuildResults and tabular output.  Steps cycle through
+the three query chunks.  `--workload reads200k_db1m` is the smaller
+translated workload (200,000 reads against 1,000,000 proteins, one chunk
+each), where a step is the whole search.
 
 Three forms run on the same inputs: the plain GPU form (stages one after the
 other, jobs in hit order), the batched form (query blocks pipelined so that
 the host stages of one block overlap the device stages of the next) and the
 batched learned-scheduled form (the same, with gapped extensions ordered on
 the device by the MLP's predicted cost; the MLP is trained on the GPU from
-the plain run's extension traces during set-up).  Their outputs must be
-byte-identical.  `value` is the batched learned-scheduled form's query
-residues per second (query residues = amino acids of the six translated
-frames); the other forms' rates are reported beside it.
+the plain form's extension traces during set-up).  Their outputs must be
+byte-identical for every query chunk.  `value` is the batched
+learned-scheduled form's query residues per second over K timed steps
+(query residues = amino acids of the six translated frames); the plain and
+batched forms run one timed step per query chunk (after their own warm-up)
+and their rates are reported beside it.
 
-The CPU baseline is the oracle (a single-threaded restatement of the
-reference's CPU aligner) timed on a bounded sample of the same reads.
+A deterministic subset of the reads (16 sub-chunks of 256 reads, each its
+own query chunk) is searched by the GPU (learned form) and by the oracle, a
+restatement of the reference's CPU aligner, on all host cores (one sub-chunk
+per thread); the outputs must be byte-identical, and the oracle's rate is the
+CPU baseline.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--scale S]
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -35,19 +48,34 @@ for sub in ("ghostz-gpu_amd", "oracle"):
     sys.path.insert(0, os.path.join(ROOT, sub))
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+# int32 VALU: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_TOPS = 39.3
+GAPPED_OPS_PER_CELL = 10
+MLP_FLOP_PER_SEED = 2 * (1024 * 256 + 256 * 64 + 64)
+
+WORKLOADS = {
+    "reads1m_db4m": {"reads": 1_000_000, "db_proteins": 4_000_000, "db_chunks": 4,
+                     "query_chunks": 3, "train_reads": 10_000},
+    "reads200k_db1m": {"reads": 200_000, "db_proteins": 1_000_000, "db_chunks": 1,
+                       "query_chunks": 1, "train_reads": 40_000},
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# HBM bytes per work unit of the seed kernels at this workload, measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and calibrated against a copy kernel
-TRAFFIC_PROFILE = "profiles/hbm_traffic_translated_reads_200k_vs_db_1m.txt"
+# HBM bytes per work unit of the seed kernels, measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE and calibrated against a copy kernel
+TRAFFIC_PROFILES = {
+    "reads1m_db4m": "profiles/hbm_traffic_translated_reads_1m_vs_db_4m.txt",
+    "reads200k_db1m": "profiles/hbm_traffic_translated_reads_200k_vs_db_1m.txt",
+}
 
 
-def traffic_per_unit(kernel):
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+def traffic_per_unit(path, kernel):
+    path = os.path.join(ROOT, path)
     try:
         for line in open(path):
             f = line.split()
@@ -63,6 +91,27 @@ def query_residues(view):
     return int(np.count_nonzero(c != 25))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, min(n, 16))
+
+
 def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_000_000):
     """Logs extension traces with the plain schedule on training reads, then
     trains the MLP on the GPU.  Returns (flat weights, report dict)."""
@@ -70,18 +119,32 @@ def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_
     names, res, offs = synthetic.translated_reads(rng, db_codes, n_reads)
     q = gz.Queries(names, res, offs, translated=True)
     al = gz.Aligner(db, record_traces=1)
-    al.align(q)
+    _, st = al.align(q)
     tr = al.traces()
+    all_traces = len(tr)
+    # the plain-order gapped kernels' VALU roofline (each kernel alone on the
+    # GPU): 10 int32 operations per DP cell
+    cells = int(st["gapped_cells"])
+    gk_ms = float(st["gapped_kernel_ms"])
+    gapped_roof = None
+    if cells and gk_ms > 0:
+        tops = GAPPED_OPS_PER_CELL * cells / (gk_ms / 1e3) / 1e12
+        gapped_roof = {"bound": "valu_int32", "achieved": tops, "peak": VALU_PEAK_TOPS,
+                       "unit": "Tops/s", "frac": tops / VALU_PEAK_TOPS, "cells": cells,
+                       "extensions": int(st["gapped_extensions"]), "kernel_ms": gk_ms,
+                       "launches": int(st["gapped_launches"]),
+                       "workload": f"{n_reads} training reads, plain order"}
     if len(tr) > max_traces:  # a uniform sample keeps training to seconds
         tr = tr[np.sort(rng.choice(len(tr), max_traces, replace=False))]
     concat = al.last_concatenated_queries()
-    chunk_seq = {c: db.chunk(c)["sequence"] for c in np.unique(tr[:, 2])} if len(tr) else {}
+    chunk_seq = {c: db.chunk_sequence(c) for c in np.unique(tr[:, 2])} if len(tr) else {}
     feats = np.zeros((len(tr), 128), np.uint8)
     for c, seqc in chunk_seq.items():
         for rev in (0, 1):
             sel = np.nonzero((tr[:, 2] == c) & (tr[:, 3] == rev))[0]
             if len(sel):
                 feats[sel] = gz.cost_features(concat, seqc, tr[sel, 0], tr[sel, 1], rev)
+    del chunk_seq
     target = np.log2(np.maximum(tr[:, 4], 1)).astype(np.float32)
     perm = rng.permutation(len(tr))
     n_test = max(1, len(tr) // 10)
@@ -100,51 +163,98 @@ def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_
     pred = mlp.predict(feats[test])
     mae = float(np.abs(pred - target[test]).mean())
     base = float(np.abs(target[test] - target[train].mean()).mean())
-    # inference throughput (kernel time only, all traces)
+    # inference throughput (kernel time only, all sampled traces)
     mlp.predict(feats)
-    seeds_per_s = len(feats) / (mlp.last_kernel_ms / 1e3) if mlp.last_kernel_ms > 0 else None
-    report = {"traces": int(len(tr)), "held_out": int(n_test), "epochs": epochs,
-              "train_steps": steps, "train_seconds": round(train_s, 3),
+    infer_s = mlp.last_kernel_ms / 1e3
+    seeds_per_s = len(feats) / infer_s if infer_s > 0 else None
+    infer_roof = None
+    if seeds_per_s:
+        tf = seeds_per_s * MLP_FLOP_PER_SEED / 1e12
+        infer_roof = {"bound": "mfma_bf16", "achieved": tf, "peak": BF16_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "frac": tf / BF16_PEAK_TFLOPS}
+    train_flop = 3 * MLP_FLOP_PER_SEED * steps * batch
+    report = {"traces_logged": int(all_traces), "traces": int(len(tr)), "held_out": int(n_test),
+              "epochs": epochs, "train_steps": steps, "train_seconds": round(train_s, 3),
+              "train_tflops": round(train_flop / train_s / 1e12, 3) if train_s > 0 else None,
               "mae_log2_cells": round(mae, 4), "mae_constant_predictor": round(base, 4),
-              "inference_seeds_per_s": seeds_per_s}
-    return mlp.weights(), report
+              "inference_seeds_per_s": seeds_per_s, "inference_roofline": infer_roof}
+    return mlp.weights(), report, gapped_roof
 
 
-def cpu_baseline(gz, oracle_py, db, names, res, offs, m, nl, budget_s):
-    """Oracle (single thread) on the first reads of the workload."""
+def oracle_subset(gz, oracle_py, db, al, names, res, offs, m, nl, n_sub, sub_reads):
+    """A deterministic subset of the reads, cut into n_sub sub-chunks of
+    sub_reads reads, each searched as its own query chunk by the GPU (al)
+    and by the oracle (one sub-chunk per host thread).  Returns (identical,
+    cpu baseline dict)."""
     info = db.info()
     chunks = [db.chunk(c) for c in range(info.number_chunks)]
-    n = 128
-    while True:
-        qs = gz.Queries(names[:n], res[:int(offs[n])], offs[:n + 1], translated=True)
-        view = qs.view()
-        t0 = time.perf_counter()
-        oracle_py.align(view, chunks, info, m, nl)
-        dt = time.perf_counter() - t0
-        log(f"cpu baseline: {n} reads in {dt:.2f} s")
-        if dt >= budget_s / 3 or n >= len(names):
-            return {"value": query_residues(view) / dt, "unit": "query aa/s", "cores": 1,
-                    "kind": "port", "sample": f"first {n} reads of the workload vs the full "
-                    f"database ({dt:.1f} s, oracle, one thread)"}
-        n = min(len(names), int(n * max(2.0, budget_s / 2 / max(dt, 1e-3))))
+    stride = max(1, len(names) // (n_sub * sub_reads))
+    sub_q, views = [], []
+    for k in range(n_sub):
+        idx = [(k * sub_reads + j) * stride for j in range(sub_reads)]
+        idx = [i for i in idx if i < len(names)]
+        nm = [names[i] for i in idx]
+        rs = b"".join(res[int(offs[i]):int(offs[i + 1])] for i in idx)
+        ln = np.array([int(offs[i + 1] - offs[i]) for i in idx], np.uint64)
+        of = np.zeros(len(idx) + 1, np.uint64)
+        np.cumsum(ln, out=of[1:])
+        q = gz.Queries(nm, rs, of, translated=True)
+        sub_q.append(q)
+        views.append(q.view())
+    gpu = [al.align(q)[0] for q in sub_q]
+    cpu = [None] * n_sub
+    err = []
+
+    def work(k):
+        try:
+            cpu[k] = oracle_py.align(views[k], chunks, info, m, nl)[0]
+        except Exception as e:  # reported below
+            err.append(e)
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(n_sub)]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    dt = time.perf_counter() - t0
+    if err:
+        raise err[0]
+    identical = all(g == c for g, c in zip(gpu, cpu))
+    mism = [k for k in range(n_sub) if gpu[k] != cpu[k]]
+    if mism:
+        log(f"ERROR: GPU output differs from the oracle on sub-chunks {mism}")
+    aa = sum(query_residues(v) for v in views)
+    nreads = sum(int(v["number_queries"]) for v in views)
+    base = {"value": aa / dt, "unit": "query aa/s", "cores": n_sub, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"{nreads} reads (every {stride}th read of the workload, {n_sub} "
+                      f"sub-chunks of {sub_reads} reads, each its own query chunk) against "
+                      f"the full database; oracle, one sub-chunk per thread on {n_sub} "
+                      f"threads, {dt:.1f} s wall",
+            "reads_per_s": nreads / dt, "lines": int(sum(c.count(b"\n") for c in cpu))}
+    return identical, base
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="reads1m_db4m", choices=sorted(WORKLOADS))
     ap.add_argument("--scale", type=float, default=1.0,
                     help="fraction of the workload (reads and database proteins)")
-    ap.add_argument("--train-reads", type=int, default=40000)
+    ap.add_argument("--train-reads", type=int, default=0)
     ap.add_argument("--epochs", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--compare-warmup", type=int, default=2,
+                    help="warm-up steps of the plain and batched forms")
+    ap.add_argument("--subset-reads", type=int, default=256,
+                    help="reads per oracle sub-chunk (16 sub-chunks)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--pipeline-blocks", type=int, default=0,
                     help="query blocks per chunk in the batched forms (0: library default)")
     ap.add_argument("--no-learned", action="store_true",
-                    help="plain schedule only (for profiling runs)")
+                    help="plain form only (for profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch a 1 GiB streaming copy kernel (counter calibration)")
     args = ap.parse_args()
@@ -157,25 +267,43 @@ def main():
     import synthetic
     gz.check(gz.lib().gz_set_device(0))
     m, nl = gz.blosum62()
-    log(f"host: os.cpu_count {os.cpu_count()}, affinity {len(os.sched_getaffinity(0))}, "
-        f"OMP_NUM_THREADS {os.environ.get('OMP_NUM_THREADS')}")
+    log(f"host: os.cpu_count {os.cpu_count()}, threads {host_threads()}, "
+        f"OMP_NUM_THREADS {os.environ.get('OMP_NUM_THREADS')}, cpu {cpu_model()}")
 
-    n_db = max(1000, int(1_000_000 * args.scale))
-    n_reads = max(1000, int(200_000 * args.scale))
+    wl = WORKLOADS[args.workload]
+    n_db = max(1000, int(wl["db_proteins"] * args.scale))
+    n_reads = max(1000, int(wl["reads"] * args.scale))
     rng = np.random.default_rng(args.seed)
     t0 = time.perf_counter()
     db_codes = synthetic.protein_database(rng, n_db)
     names, res, offs = synthetic.translated_reads(rng, db_codes, n_reads)
     dn, dr, do = synthetic._pack(db_codes, "subject_")
+    total_aa = int(do[-1])
     log(f"workload generated in {time.perf_counter() - t0:.1f} s: {n_db} proteins "
-        f"({len(dr)} aa), {n_reads} reads")
+        f"({total_aa} aa), {n_reads} reads")
     t0 = time.perf_counter()
-    db = gz.Database(dn, dr, do)
-    del dr
-    queries = gz.Queries(names, res, offs, translated=True)
-    qv = queries.view()
-    q_aa = query_residues(qv)
-    log(f"host preparation {time.perf_counter() - t0:.1f} s: {q_aa} query aa")
+    # database chunks of equal residue budgets (db -l): subjects are added to
+    # a chunk while its residues stay below the budget
+    chunk_size = total_aa // wl["db_chunks"] + 4096 if wl["db_chunks"] > 1 else 0
+    db = gz.Database(dn, dr, do, chunk_size=chunk_size)
+    del dr, dn
+    n_chunks = int(db.info().number_chunks)
+    t_db = time.perf_counter() - t0
+    # query chunks: equal thirds of the reads, each one query chunk (every
+    # third stays below the default -l of 2^27 framed codes)
+    nq = wl["query_chunks"]
+    bounds = [n_reads * i // nq for i in range(nq + 1)]
+    qchunks, q_aa = [], []
+    t0 = time.perf_counter()
+    for i in range(nq):
+        a, b = bounds[i], bounds[i + 1]
+        base = int(offs[a])
+        q = gz.Queries(names[a:b], res[base:int(offs[b])],
+                       np.asarray(offs[a:b + 1], np.uint64) - np.uint64(base), translated=True)
+        qchunks.append(q)
+        q_aa.append(query_residues(q.view()))
+    log(f"database built in {t_db:.1f} s ({n_chunks} chunks); queries prepared in "
+        f"{time.perf_counter() - t0:.1f} s: {nq} query chunks, {sum(q_aa)} query aa")
 
     if args.calibrate_copy:
         nbytes = 1 << 30
@@ -188,118 +316,149 @@ def main():
         log("copy kernel: 3 x 1 GiB")
 
     t0 = time.perf_counter()
-    weights, learn = None, None
+    weights, learn, gapped_roof = None, None, None
     if not args.no_learned:
-        train_reads = max(1000, int(args.train_reads * min(1.0, args.scale * 10)))
-        weights, learn = train_cost_model(gz, db, db_codes, np.random.default_rng(args.seed + 1),
-                                          train_reads, args.epochs, args.seed)
+        tr_reads = args.train_reads or wl["train_reads"]
+        tr_reads = max(1000, int(tr_reads * min(1.0, args.scale * 10)))
+        weights, learn, gapped_roof = train_cost_model(
+            gz, db, db_codes, np.random.default_rng(args.seed + 1), tr_reads, args.epochs,
+            args.seed)
+    del db_codes
     log(f"cost model trained in {time.perf_counter() - t0:.1f} s: {learn}")
 
+    pb = {"pipeline_blocks": args.pipeline_blocks} if args.pipeline_blocks else {}
     plain = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN)
     forms = [("plain", plain)]
     if not args.no_learned:
-        pb = {"pipeline_blocks": args.pipeline_blocks} if args.pipeline_blocks else {}
         batched = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN, batched=1, **pb)
         learned = gz.Aligner(db, schedule=gz.SCHEDULE_LEARNED, batched=1, **pb)
         learned.set_cost_model(weights)
         forms += [("batched", batched), ("learned", learned)]
 
-    def run(al, k, label):
-        outs, stats, times = [], [], []
+    digests = {}  # query chunk -> set of output digests over all forms and runs
+
+    def run(al, first, k, label):
+        stats, times, aa = [], [], 0
         for i in range(k):
+            c = (first + i) % nq
             # the step is the C call: queries in host memory to the output
             # text (the copy into a Python bytes object is not part of it)
-            text, st = al.align(queries)
+            text, st = al.align(qchunks[c])
             times.append(al.last_call_seconds)
-            outs.append(text)
+            digests.setdefault(c, set()).add(hashlib.sha256(text).hexdigest())
             stats.append(st)
-            nl_count = text.count(b"\n")
-            log(f"{label} step {i}: {times[-1]:.3f} s, {nl_count} lines, "
-                f"seed {st['seconds_seed']:.3f} s, gapped {st['seconds_gapped']:.3f} s "
-                f"(kernel {st['gapped_kernel_ms']:.1f} ms)")
-        return outs, stats, times
+            aa += q_aa[c]
+            lines = text.count(b"\n")
+            log(f"{label} step {i} (query chunk {c}): {times[-1]:.3f} s, "
+                f"{lines} lines, seed {st['seconds_seed']:.3f} s, gapped "
+                f"{st['seconds_gapped']:.3f} s (kernel {st['gapped_kernel_ms']:.1f} ms, "
+                f"schedule {st['schedule_kernel_ms']:.1f} ms)")
+            del text
+        return stats, times, aa
 
     results = {}
     for name, al in forms:
-        run(al, args.warmup, name + " warmup")
-        results[name] = run(al, args.steps, name)
-    p_out, p_st, p_t = results["plain"]
-    l_out, l_st, l_t = results["learned" if "learned" in results else "plain"]
-    identical = all(o == p_out[0] for r in results.values() for o in r[0])
+        flagship = name == forms[-1][0]
+        w = args.warmup if flagship else min(args.warmup, args.compare_warmup)
+        k = args.steps if flagship else nq
+        run(al, 0, w, name + " warmup")
+        results[name] = run(al, w, k, name)
+    flag = forms[-1][0]
+    identical = all(len(v) == 1 for v in digests.values())
     if not identical:
         log("ERROR: the forms' outputs differ")
+    form_values = {name: r[2] / sum(r[1]) for name, r in results.items()}
+    form_ms = {name: 1e3 * sum(r[1]) / len(r[1]) for name, r in results.items()}
+    l_st, l_t, l_aa = results[flag]
+    p_st, p_t, p_aa = results["plain"]
+    value = form_values[flag]
+    plain_value = form_values["plain"]
 
-    p_time = sum(p_t)
-    l_time = sum(l_t)
-    value = q_aa * args.steps / l_time
-    plain_value = q_aa * args.steps / p_time
-    form_values = {name: q_aa * args.steps / sum(r[2]) for name, r in results.items()}
+    def total(sts, key):
+        return sum(s[key] for s in sts)
 
-    # kernel times of the learned form (its two streams overlap, so these
-    # include sharing the GPU) and of the plain form (one kernel at a time)
-    def kernel_ms(st):
-        return {"seed_rep": st["seed_rep_kernel_ms"], "seed_value": st["seed_value_kernel_ms"],
-                "gapped": st["gapped_kernel_ms"], "schedule": st["schedule_kernel_ms"],
-                "traceback": st["traceback_kernel_ms"]}
-    learned_kernels = kernel_ms(l_st[-1])
-    plain_kernels = kernel_ms(p_st[-1])
+    def kernel_ms(sts):
+        n = len(sts)
+        return {"seed_rep": total(sts, "seed_rep_kernel_ms") / n,
+                "seed_value": total(sts, "seed_value_kernel_ms") / n,
+                "gapped": total(sts, "gapped_kernel_ms") / n,
+                "schedule": total(sts, "schedule_kernel_ms") / n,
+                "traceback": total(sts, "traceback_kernel_ms") / n}
+    learned_kernels = kernel_ms(l_st)
+    plain_kernels = kernel_ms(p_st)
     # roofline of the dominant kernel, from the HIP-event times of the plain
-    # form's timed steps, where each kernel has the GPU to itself
-    st = p_st[-1]
-    # the seed kernels are the largest single kernels of the step (the gapped
-    # time is spread over four kernels: two sides, two ring widths)
+    # form's timed steps, where each kernel has the GPU to itself: the seed
+    # kernels are the largest single kernels of the step (the gapped time is
+    # spread over four kernels: two sides, two ring widths)
     kern = {k: plain_kernels[k] for k in ("seed_rep", "seed_value")}
     dom = max(kern, key=kern.get)
     k = 0 if dom == "seed_rep" else 1
-    units = st["units_rep"] if k == 0 else st["units_value"]
-    launches = max(1, st["seed_rep_launches"] if k == 0 else st["seed_value_launches"])
-    ms = st["seed_rep_kernel_ms"] if k == 0 else st["seed_value_kernel_ms"]
+    units = total(p_st, "units_rep" if k == 0 else "units_value")
+    launches = max(1, total(p_st, "seed_rep_launches" if k == 0 else "seed_value_launches"))
+    ms = total(p_st, "seed_rep_kernel_ms" if k == 0 else "seed_value_kernel_ms")
     avg_ms = ms / launches
     # algorithmic bytes per unit: 4 (entry position) + 4 (database position)
     # + 2 x 8 (two packed database words around the seed)
     per_unit = 24.0
     achieved = per_unit * units / launches / (avg_ms / 1e3) / 1e9
     name = "SeedKernel<%s>" % ("false" if k == 0 else "true")
-    measured = traffic_per_unit(name) if args.scale == 1.0 else None
+    tp = TRAFFIC_PROFILES[args.workload]
+    measured = traffic_per_unit(tp, name) if args.scale == 1.0 else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured * units / launches if measured else None,
             "kernel": name, "avg_launch_ms": avg_ms, "units_per_launch": units / launches,
             "algorithmic_bytes_per_unit": per_unit,
             "traffic_bytes_per_unit": measured,
-            "traffic_source": TRAFFIC_PROFILE if measured else None}
+            "traffic_source": tp if measured else None}
+    stage_roofs = {"gapped_plain_order": gapped_roof,
+                   "mlp_inference": learn["inference_roofline"] if learn else None}
 
-    cpu = None
+    cpu, subset_ok = None, None
     if not args.no_cpu:
         import oracle_py
-        cpu = cpu_baseline(gz, oracle_py, db, names, res, offs, m, nl, args.cpu_seconds)
+        subset_ok, cpu = oracle_subset(gz, oracle_py, db, forms[-1][1], names, res, offs, m,
+                                       nl, host_threads(), args.subset_reads)
 
+    sched_ms = learned_kernels["schedule"]
     line = {
         "metric": "query aa/s vs synthetic DB: learned-scheduled vs plain GPU, bit-exact hits",
         "value": value, "unit": "query aa/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * l_time / args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * sum(l_t) / len(l_t),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": "translated 150-nt reads vs synthetic protein database "
-                   "(aln -q d -t p, BLOSUM62 11/1, -b 10 -v 1, SEG)",
-                   "reads": n_reads, "db_proteins": n_db, "db_residues": int(db.info().total_length),
-                   "query_aa": q_aa, "schedule": "learned"},
-        "plain_value": plain_value, "plain_ms_per_step": 1e3 * p_time / args.steps,
-        "form_values": form_values,
-        "learned_over_plain": value / plain_value, "outputs_identical": identical,
+                   "(aln -q d -t p, BLOSUM62 11/1, -b 10 -v 1, SEG); step = one query "
+                   "chunk against every database chunk",
+                   "name": args.workload, "reads": n_reads, "db_proteins": n_db,
+                   "db_residues": int(db.info().total_length), "db_chunks": n_chunks,
+                   "query_chunks": nq, "query_aa": int(sum(q_aa)),
+                   "schedule": flag},
+        "plain_value": plain_value, "plain_ms_per_step": form_ms["plain"],
+        "form_values": form_values, "form_ms_per_step": form_ms,
+        "form_steps": {name: len(r[1]) for name, r in results.items()},
+        "learned_over_plain": value / plain_value,
+        "learned_over_batched": (value / form_values["batched"]) if "batched" in form_values
+        else None,
+        "outputs_identical": identical, "oracle_subset_identical": subset_ok,
         "kernel_ms_per_step": learned_kernels,
         "plain_kernel_ms_per_step": plain_kernels,
-        "search_stats": {k: st[k] for k in ("lists", "units_rep", "units_value", "seed_hits",
-                                             "chained_hits", "gapped_extensions", "results",
-                                             "sequential_build_chunks")},
-        "stage_seconds": {k: round(v, 4) for k, v in st.items()
-                                                       if k.startswith("seconds_")},
+        "schedule_ms_vs_gapped_ms_saved": None if args.no_learned else {
+            "schedule_ms": sched_ms,
+            "gapped_ms_batched": kernel_ms(results["batched"][0])["gapped"],
+            "gapped_ms_learned": learned_kernels["gapped"]},
+        "search_stats": {k: total(p_st, k) // len(p_st) for k in (
+            "lists", "units_rep", "units_value", "seed_hits", "chained_hits",
+            "gapped_extensions", "results", "sequential_build_chunks")},
+        "stage_seconds": {k: round(total(p_st, k) / len(p_st), 4) for k in p_st[0]
+                          if k.startswith("seconds_")},
         "cost_model": learn,
         "roofline": roof,
+        "stage_rooflines": stage_roofs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    if not identical:
+    if not identical or subset_ok is False:
         sys.exit(1)
 
 

@@ -39,6 +39,11 @@ class Aligner : public PresearchDevice {
   // the tabular output.
   hipError_t Align(const QuerySet &queries, std::string *output, AlignStats *stats);
   const std::vector<GappedTrace> &Traces() const { return traces_; }
+  // The resident query chunk's hash lists on the device (after
+  // BeginQueryChunk) and the seed sources SeedHits runs (bit 0:
+  // representatives and members, bit 1: non-clustered values; default both).
+  const DeviceQueryLists &QueryLists() const { return dq_; }
+  void SetSeedKinds(unsigned kinds) { seed_kinds_ = kinds; }
   const std::vector<uint8_t> &LastConcatenatedQueries() const { return concat_; }
 
   // PresearchDevice
@@ -76,6 +81,7 @@ class Aligner : public PresearchDevice {
   static constexpr size_t kNoChunk = ~(size_t)0;
   size_t lookup_chunk_ = kNoChunk;  // database chunk of the seed stage's lookup
   std::vector<void *> query_buffers_;
+  unsigned seed_kinds_ = 3;
   // timing events: [0, 4) the gapped stage's, [4, 6) the result build's (the
   // two stages may run on different host threads)
   hipEvent_t ev_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};

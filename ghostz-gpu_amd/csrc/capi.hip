@@ -14,6 +14,7 @@
 
 #include "aligner.h"
 #include "alphabet.h"
+#include "capi_types.h"
 #include "cost_model_gpu.h"
 #include "database.h"
 #include "gz_common.h"
@@ -26,16 +27,6 @@
 #include "traceback.h"
 
 using namespace gz;
-
-struct gz_queries_s {
-  QuerySet q;
-  std::vector<const char *> name_ptrs;
-};
-
-struct gz_database_s {
-  Database db;
-  std::vector<std::vector<const char *> > name_ptrs;
-};
 
 struct gz_aligner_s {
   std::vector<int> matrix;
@@ -460,6 +451,7 @@ gz_status gz_aligner_set_cost_model(gz_aligner_t h, const float *weights, size_t
     return GZ_ERROR_INVALID_ARGUMENT;
   }
   std::unique_ptr<GpuCostModel> model(new GpuCostModel(w));
+  GZ_HIP_CHECK(model->status());
   h->aligner->SetCostModel(model.get());
   h->opt.cost_model = model.get();
   h->model = std::move(model);

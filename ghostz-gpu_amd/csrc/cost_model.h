@@ -42,9 +42,11 @@ class CostModel {
                        bool reverse, float *out) const = 0;
 };
 
-// Feature codes for one seed: 128 residue indexes in [0, 21), the window
-// order being query-before, query-after, subject-before, subject-after
-// (positions beyond a delimiter map to 20).
+// Feature codes for one seed: 128 residue indexes in [0, 21) in four windows
+// of 32: query along the extension, subject along the extension, query away
+// from it, subject away from it (positions at or beyond a delimiter map to
+// 20).  A hit's forward side (seed one position later) sees the reverse
+// side's windows with the two halves of the row swapped.
 void CostFeatures(const uint8_t *queries, const uint8_t *database, uint32_t qpos,
                   uint32_t dpos, bool reverse, uint8_t *out128);
 

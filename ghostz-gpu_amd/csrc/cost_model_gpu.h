@@ -19,10 +19,17 @@ hipError_t CostFeaturesOnDevice(const uint64_t *packed_queries, const uint64_t *
                                 const uint32_t *qpos, const uint32_t *dpos, uint32_t n,
                                 bool reverse, uint8_t *out, hipStream_t stream);
 
+// The device predictions share one feature scratch buffer: a model serves one
+// stream at a time (each aligner owns its own model and calls it from its
+// extension stream only).  Concurrent calls on two streams would overwrite
+// each other's features, which changes only the predicted order, never a
+// search result.
 class GpuCostModel : public CostModel {
  public:
   explicit GpuCostModel(const CostModelWeights &w);
   ~GpuCostModel() override;
+  // hipSuccess, or the error of the device allocations made at construction
+  hipError_t status() const;
   void Predict(const uint8_t *queries, const uint8_t *database, const uint32_t *qpos,
                const uint32_t *dpos, size_t n, bool reverse, float *out) const override;
   // Predicted log2 cells of n jobs whose positions are on the device; all

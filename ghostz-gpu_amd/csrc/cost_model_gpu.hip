@@ -118,8 +118,10 @@ struct GpuCostModel::Impl {
 };
 
 GpuCostModel::GpuCostModel(const CostModelWeights &w) : impl_(new Impl()) {
-  impl_->mlp.SetWeights(w);
+  (void)impl_->mlp.SetWeights(w);
 }
+
+hipError_t GpuCostModel::status() const { return impl_->mlp.status(); }
 
 GpuCostModel::~GpuCostModel() { delete impl_; }
 

@@ -18,8 +18,11 @@ class MlpDevice {
  public:
   MlpDevice();
   ~MlpDevice();
-  void SetWeights(const CostModelWeights &w);
-  void GetWeights(CostModelWeights *w);
+  // hipSuccess, or the first error of the constructor's stream creation and
+  // allocations (every other call then returns it without doing anything)
+  hipError_t status() const { return init_status_; }
+  hipError_t SetWeights(const CostModelWeights &w);
+  hipError_t GetWeights(CostModelWeights *w);
   // Predicted log2 cells for n seeds given their 128 feature codes each.
   hipError_t Forward(const uint8_t *codes, size_t n, float *out, uint32_t rotate_words = 0);
   // One training step on a batch: mean squared error against targets;
@@ -41,6 +44,7 @@ class MlpDevice {
   hipError_t ForwardDevice(size_t n, size_t padded);
   void SyncBf16();
 
+  hipError_t init_status_ = hipSuccess;
   hipStream_t stream_ = nullptr;
   size_t cap_ = 0;
   // fp32 master weights + gradients + Adam moments, flat

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "cost_model.h"
@@ -406,17 +407,18 @@ void Gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, long sam, long 
 }  // namespace
 
 MlpDevice::MlpDevice() {
-  (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
   count_ = kFlat;
-  (void)hipMalloc((void **)&master_, kFlat * 4);
-  (void)hipMalloc((void **)&grad_, kFlat * 4);
-  (void)hipMalloc((void **)&m_, kFlat * 4);
-  (void)hipMalloc((void **)&v_, kFlat * 4);
-  (void)hipMemset(m_, 0, kFlat * 4);
-  (void)hipMemset(v_, 0, kFlat * 4);
-  (void)hipMalloc((void **)&w1h_, (size_t)kCostH1 * kCostIn * 2);
-  (void)hipMalloc((void **)&w2h_, (size_t)kCostH2 * kCostH1 * 2);
-  (void)hipMalloc((void **)&embh_, (size_t)kCostAlphabet * kCostEmbed * 2);
+  hipError_t e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void **)&master_, kFlat * 4);
+  if (e == hipSuccess) e = hipMalloc((void **)&grad_, kFlat * 4);
+  if (e == hipSuccess) e = hipMalloc((void **)&m_, kFlat * 4);
+  if (e == hipSuccess) e = hipMalloc((void **)&v_, kFlat * 4);
+  if (e == hipSuccess) e = hipMemset(m_, 0, kFlat * 4);
+  if (e == hipSuccess) e = hipMemset(v_, 0, kFlat * 4);
+  if (e == hipSuccess) e = hipMalloc((void **)&w1h_, (size_t)kCostH1 * kCostIn * 2);
+  if (e == hipSuccess) e = hipMalloc((void **)&w2h_, (size_t)kCostH2 * kCostH1 * 2);
+  if (e == hipSuccess) e = hipMalloc((void **)&embh_, (size_t)kCostAlphabet * kCostEmbed * 2);
+  init_status_ = e;
 }
 
 MlpDevice::~MlpDevice() {
@@ -437,7 +439,8 @@ void MlpDevice::SyncBf16() {
                      (size_t)kCostAlphabet * kCostEmbed, master_ + kOffEmbed, embh_);
 }
 
-void MlpDevice::SetWeights(const CostModelWeights &w) {
+hipError_t MlpDevice::SetWeights(const CostModelWeights &w) {
+  if (init_status_ != hipSuccess) return init_status_;
   std::vector<float> flat(kFlat);
   float *f = flat.data();
   auto put = [&](const std::vector<float> &v, size_t off) { memcpy(f + off, v.data(), v.size() * 4); };
@@ -448,17 +451,23 @@ void MlpDevice::SetWeights(const CostModelWeights &w) {
   put(w.b2, kOffB2);
   put(w.w3, kOffW3);
   put(w.b3, kOffB3);
-  (void)hipMemcpyAsync(master_, flat.data(), kFlat * 4, hipMemcpyHostToDevice, stream_);
-  (void)hipMemsetAsync(m_, 0, kFlat * 4, stream_);
-  (void)hipMemsetAsync(v_, 0, kFlat * 4, stream_);
+  hipError_t e = hipMemcpyAsync(master_, flat.data(), kFlat * 4, hipMemcpyHostToDevice, stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(m_, 0, kFlat * 4, stream_);
+  if (e == hipSuccess) e = hipMemsetAsync(v_, 0, kFlat * 4, stream_);
   step_ = 0;
-  SyncBf16();
-  (void)hipStreamSynchronize(stream_);
+  if (e == hipSuccess) {
+    SyncBf16();
+    e = hipGetLastError();
+  }
+  const hipError_t e2 = hipStreamSynchronize(stream_);
+  return e != hipSuccess ? e : e2;
 }
 
-void MlpDevice::GetWeights(CostModelWeights *w) {
+hipError_t MlpDevice::GetWeights(CostModelWeights *w) {
+  if (init_status_ != hipSuccess) return init_status_;
   std::vector<float> flat(kFlat);
-  (void)hipMemcpy(flat.data(), master_, kFlat * 4, hipMemcpyDeviceToHost);
+  const hipError_t e = hipMemcpy(flat.data(), master_, kFlat * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
   auto get = [&](std::vector<float> &v, size_t off, size_t n) {
     v.assign(flat.begin() + off, flat.begin() + off + n);
   };
@@ -469,9 +478,11 @@ void MlpDevice::GetWeights(CostModelWeights *w) {
   get(w->b2, kOffB2, kCostH2);
   get(w->w3, kOffW3, kCostH2);
   get(w->b3, kOffB3, 1);
+  return hipSuccess;
 }
 
 hipError_t MlpDevice::Reserve(size_t batch) {
+  if (init_status_ != hipSuccess) return init_status_;
   if (batch <= cap_) return hipSuccess;
   void **ps[] = {(void **)&codes_, (void **)&x_,   (void **)&z1_,  (void **)&h1_,
                  (void **)&z2_,    (void **)&h2_,  (void **)&y_,   (void **)&t_,
@@ -522,6 +533,7 @@ hipError_t MlpDevice::ForwardDevice(size_t n, size_t B) {
 
 hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out,
                                   hipStream_t s, uint32_t rotate_words) {
+  if (init_status_ != hipSuccess) return init_status_;
   if (n == 0) return hipSuccess;
   if (n > 0xFFFFFFFFull - kInferTile) return hipErrorInvalidValue;
   int dev = 0, cus = 256;
@@ -647,9 +659,9 @@ gz_status gz_mlp_create(const float *flat, size_t count, gz_mlp_t *out) {
   take(w.b2, kCostH2);
   take(w.w3, kCostH2);
   take(w.b3, 1);
-  gz_mlp_s *h = new gz_mlp_s();
-  h->dev.SetWeights(w);
-  *out = h;
+  std::unique_ptr<gz_mlp_s> h(new gz_mlp_s());
+  GZ_HIP_CHECK(h->dev.SetWeights(w));
+  *out = h.release();
   return GZ_OK;
 }
 
@@ -693,7 +705,7 @@ gz_status gz_mlp_train_step(gz_mlp_t h, const uint8_t *codes, const float *targe
 gz_status gz_mlp_get_weights(gz_mlp_t h, float *flat) {
   GZ_REQUIRE(h && flat, "null argument");
   CostModelWeights w;
-  h->dev.GetWeights(&w);
+  GZ_HIP_CHECK(h->dev.GetWeights(&w));
   float *f = flat;
   for (auto *v : {&w.embed, &w.w1, &w.b1, &w.w2, &w.b2, &w.w3, &w.b3}) {
     memcpy(f, v->data(), v->size() * 4);

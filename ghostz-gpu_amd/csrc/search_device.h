@@ -82,10 +82,11 @@ class SeedStage {
   // chunk and returns their passing seeds grouped by query (qstart indexed
   // by qid - qb), in pinned host memory owned by the stage.  The stage keeps
   // two such buffers and alternates: a view stays valid until the second
-  // Run after it.
+  // Run after it.  kinds selects the seed sources: bit 0 the representatives
+  // and their members, bit 1 the non-clustered values.
   hipError_t Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
                  const SeedStageParams &p, uint32_t qb, uint32_t qe, hipStream_t stream,
-                 SeedHitView *hits, SeedStageStats *stats);
+                 SeedHitView *hits, SeedStageStats *stats, unsigned kinds = 3);
 
  private:
   hipError_t Reserve(void **p, size_t *cap, size_t bytes);

@@ -510,7 +510,7 @@ hipError_t SeedStage::Lookup(const DeviceQueryLists &q, const DeviceDbChunk &db,
 
 hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
                           const SeedStageParams &p, uint32_t qb, uint32_t qe, hipStream_t s,
-                          SeedHitView *out, SeedStageStats *stats) {
+                          SeedHitView *out, SeedStageStats *stats, unsigned kinds) {
   *out = SeedHitView();
   if (!looked_up_ || qb > qe || qe > q.nqueries || (qb & 63) != 0 ||
       ((qe & 63) != 0 && qe != q.nqueries))
@@ -608,7 +608,7 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
     // passing seeds accumulate in hits_; counters_[0] is their running count
     GZ_TRY(hipMemsetAsync(counters_, 0, kCounters * sizeof(uint32_t), s));
     for (int k = 0; k < 2; ++k) {
-      if (totals[k] == 0) continue;
+      if (totals[k] == 0 || !((kinds >> k) & 1)) continue;
       for (uint64_t u0 = 0; u0 < totals[k];) {
         const uint64_t u1 = std::min(totals[k], u0 + kUnitsPerLaunch);
         const uint32_t cap = (uint32_t)std::min<size_t>(hits_cap_ / sizeof(SeedHit), 0xffffffffu);

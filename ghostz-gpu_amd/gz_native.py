@@ -353,6 +353,52 @@ _SIGNATURES.update({
 })
 
 
+class SeedHits(Structure):
+    _fields_ = [("number_hits", c_uint64), ("number_queries", c_uint32),
+                ("query_offsets", c_void_p), ("query_positions", c_void_p),
+                ("database_positions", c_void_p)]
+
+
+_SIGNATURES.update({
+    "gz_host_seeds_memory_create": (c_int, [c_size_t, POINTER(c_void_p)]),
+    "gz_host_seeds_memory_destroy": (c_int, [c_void_p]),
+    "gz_host_seeds_memory_size": (c_size_t, [c_void_p]),
+    "gz_host_seeds_memory_query_seed_ids": (c_void_p, [c_void_p]),
+    "gz_host_seeds_memory_query_seed_starts": (c_void_p, [c_void_p]),
+    "gz_host_seeds_memory_query_ids": (c_void_p, [c_void_p]),
+    "gz_host_seeds_memory_query_concatenated_positions": (c_void_p, [c_void_p]),
+    "gz_host_seeds_memory_database_positions": (c_void_p, [c_void_p]),
+    "gz_host_seeds_memory_result_values": (c_void_p, [c_void_p]),
+    "gz_stream_controller_create": (c_int, [POINTER(c_void_p)]),
+    "gz_stream_controller_destroy": (c_int, [c_void_p]),
+    "gz_stream_controller_init": (c_int, [c_void_p, c_size_t]),
+    "gz_stream_controller_set_distance_calculator": (c_int, [c_void_p, c_void_p]),
+    "gz_stream_controller_set_ungapped_extender": (c_int, [c_void_p, c_void_p]),
+    "gz_stream_controller_set_gapped_extender": (c_int, [c_void_p, c_void_p]),
+    "gz_stream_controller_run_distance_calculation": (c_int, [c_void_p, c_size_t, c_void_p,
+                                                              POINTER(c_int)]),
+    "gz_stream_controller_run_ungapped_extension_with_trigger": (
+        c_int, [c_void_p, c_size_t, c_size_t, c_void_p, POINTER(c_int)]),
+    "gz_stream_controller_run_gapped_extension": (
+        c_int, [c_void_p, c_size_t, c_size_t, c_size_t, c_size_t, c_void_p, POINTER(c_int)]),
+    "gz_stream_controller_wait_run": (c_int, [c_void_p, c_int]),
+    "gz_stream_controller_wait_run_all": (c_int, [c_void_p]),
+    "gz_stream_controller_finished_run": (c_int, [c_void_p, c_int]),
+    "gz_stream_controller_release_resource": (c_int, [c_void_p, c_int]),
+    "gz_stream_controller_available_resources": (c_uint32, [c_void_p]),
+    "gz_seed_searcher_create": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "gz_seed_searcher_destroy": (c_int, [c_void_p]),
+    "gz_seed_searcher_reset": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, POINTER(c_uint32)]),
+    "gz_seed_searcher_search_seeds_from_representatives_for_similarity_filtering": (
+        c_int, [c_void_p, c_uint32, c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
+    "gz_seed_searcher_search_seeds_from_clustering_subsequence": (
+        c_int, [c_void_p, c_uint32, c_uint32, POINTER(SeedHits)]),
+    "gz_seed_searcher_search_seeds_from_non_clustering_subsequence": (
+        c_int, [c_void_p, c_uint32, c_uint32, POINTER(SeedHits)]),
+    "gz_seed_searcher_search_seeds": (c_int, [c_void_p, c_uint32, c_uint32, POINTER(SeedHits)]),
+})
+
+
 def _names_array(names):
     arr = (c_char_p * len(names))(*[n.encode() if isinstance(n, str) else n for n in names])
     return arr
@@ -465,6 +511,12 @@ class Database:
         names_p = ctypes.cast(v.names, POINTER(c_char_p))
         out["names"] = [names_p[i] for i in range(v.number_sequences)]
         return out
+
+    def chunk_sequence(self, c):
+        """The codes of chunk c alone (without copying its index arrays)."""
+        v = ChunkView()
+        check(lib().gz_database_chunk(self.handle, c, ctypes.byref(v)))
+        return _read_array(v.sequence, np.uint8, v.sequence_length)
 
     def __del__(self):
         try:

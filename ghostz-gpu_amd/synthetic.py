@@ -1,12 +1,15 @@
 """Synthetic search workloads (numpy, seeded).
 
 Proteins: residues i.i.d. from the BLOSUM62 background frequencies
-(Robinson & Robinson), lengths log-normal (median 300 aa, clipped to
-50..2000) for database entries.  Homologs of a query are planted into random
-database entries: the query is copied with BLOSUM62-weighted substitutions at
-a target identity (P(b | a) proportional to p_b * exp(lambda * s(a, b)),
-b != a, lambda = 0.3176) and indels at rate 0.02 per residue with geometric
-lengths of mean 3, then written over a random window of the entry.
+(A .074 R .052 N .045 D .054 C .025 Q .034 E .054 G .074 H .026 I .068
+L .099 K .058 M .025 F .047 P .039 S .057 T .051 W .013 Y .032 V .073,
+normalised), lengths log-normal (median 300 aa, sigma 0.44, so that one
+million proteins hold about 330M aa; clipped to 50..2000) for database
+entries.  Homologs of a query are planted into random database entries: the
+query is copied with BLOSUM62-weighted substitutions at a target identity
+(P(b | a) proportional to p_b * 2^(s(a, b) / 2), b != a: BLOSUM62 is in
+half-bits) and indels at rate 0.02 per residue with geometric lengths of
+mean 3, then written over a random window of the entry.
 
 Translated reads: 150-nt reads, 70 % reverse-translated from random 50-aa
 windows of database proteins (uniformly random synonymous codons, 1 % i.i.d.
@@ -20,11 +23,11 @@ gz_database_create take.
 import numpy as np
 
 AMINO = b"ARNDCQEGHILKMFPSTWYV"
-BACKGROUND = np.array([.07805, .05129, .04487, .05364, .01925, .04264, .06295, .07377,
-                       .02199, .05142, .09019, .05744, .02243, .03856, .05203, .07120,
-                       .05841, .01330, .03216, .06441])
+BACKGROUND = np.array([.074, .052, .045, .054, .025, .034, .054, .074, .026, .068,
+                       .099, .058, .025, .047, .039, .057, .051, .013, .032, .073])
 BACKGROUND = BACKGROUND / BACKGROUND.sum()
-BLOSUM62_LAMBDA = 0.3176
+_BACKGROUND_CDF = np.cumsum(BACKGROUND)
+_BACKGROUND_CDF[-1] = 1.0
 
 _B62_ROWS = """
  4 -1 -2 -2  0 -1 -1  0 -2 -1 -1 -1 -1 -2 -1  1  0 -3 -2  0
@@ -50,8 +53,8 @@ _B62_ROWS = """
 """
 BLOSUM62_20 = np.array(_B62_ROWS.split(), dtype=np.int64).reshape(20, 20)
 
-# substitution kernel: P(b | a) ~ p_b exp(lambda s(a, b)), b != a
-_SUB = BACKGROUND[None, :] * np.exp(BLOSUM62_LAMBDA * BLOSUM62_20)
+# substitution kernel: P(b | a) ~ p_b 2^(s(a, b) / 2), b != a
+_SUB = BACKGROUND[None, :] * np.exp2(BLOSUM62_20 / 2.0)
 np.fill_diagonal(_SUB, 0.0)
 _SUB /= _SUB.sum(axis=1, keepdims=True)
 _SUB_CDF = np.cumsum(_SUB, axis=1)
@@ -74,12 +77,18 @@ for _x, _y in zip(b"ACGT", b"TGCA"):
     _COMPLEMENT[_x] = _y
 
 
+# residue by a uniform 16-bit draw: each background probability is rounded to
+# a multiple of 2^-16 (an error below 1.6e-5 per letter)
+_CODE_TABLE = np.searchsorted(_BACKGROUND_CDF, (np.arange(1 << 16) + 0.5) / (1 << 16),
+                              side="right").astype(np.uint8)
+
+
 def random_codes(rng, n):
     """n residue indexes in AMINO order from the background frequencies."""
-    return rng.choice(20, size=n, p=BACKGROUND).astype(np.uint8)
+    return _CODE_TABLE[rng.integers(0, 1 << 16, n, dtype=np.uint16)]
 
 
-def lognormal_lengths(rng, n, median=300, sigma=0.6, lo=50, hi=2000):
+def lognormal_lengths(rng, n, median=300, sigma=0.44, lo=50, hi=2000):
     x = rng.lognormal(np.log(median), sigma, n)
     return np.clip(np.rint(x), lo, hi).astype(np.int64)
 

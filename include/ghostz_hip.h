@@ -211,6 +211,74 @@ gz_status gz_gapped_extender_set_max_sequence_length(gz_gapped_extender_t h,
 gz_status gz_gapped_extender_failures(gz_gapped_extender_t h,
                                       uint32_t *failures);
 
+/* ---- HostSeedsMemory (src/host_seeds_memory.h:18-80,
+ *      src/host_seeds_memory.cpp:26-37): six pinned arrays of `size`
+ *      entries that a GpuStreamController run reads and writes ---- */
+typedef struct gz_host_seeds_memory_s *gz_host_seeds_memory_t;
+gz_status gz_host_seeds_memory_create(size_t size, gz_host_seeds_memory_t *out);
+gz_status gz_host_seeds_memory_destroy(gz_host_seeds_memory_t m);
+size_t gz_host_seeds_memory_size(gz_host_seeds_memory_t m);
+uint32_t *gz_host_seeds_memory_query_seed_ids(gz_host_seeds_memory_t m);
+uint32_t *gz_host_seeds_memory_query_seed_starts(gz_host_seeds_memory_t m);
+uint32_t *gz_host_seeds_memory_query_ids(gz_host_seeds_memory_t m);
+uint32_t *gz_host_seeds_memory_query_concatenated_positions(gz_host_seeds_memory_t m);
+uint32_t *gz_host_seeds_memory_database_positions(gz_host_seeds_memory_t m);
+int *gz_host_seeds_memory_result_values(gz_host_seeds_memory_t m);
+
+/* ---- GpuStreamController (src/gpu_stream_controller.h:24-101,
+ *      src/gpu_stream_controller.cpp:20-77) with GpuStreamRunner
+ *      (src/gpu_stream_runner.cpp:14-107): two resources, each two streams
+ *      and a DeviceSeedsMemory (src/device_seeds_memory.h:18-63).  A Run*
+ *      call takes a free resource (GZ_ERROR_INVALID_ARGUMENT when none is
+ *      free, where the reference asserts), enqueues copies and kernels on
+ *      it, and returns its id; results are in the host memory after
+ *      WaitRun(id); ReleaseResource(id) hands the resource back.  Gapped
+ *      runs are serialized on the device (the extender's ring workspace is
+ *      per handle); distance and ungapped runs of the two resources overlap.
+ *      Result values: distance runs write one char per seed, ungapped runs one
+ *      flag char per seed, gapped runs an int score per seed with the
+ *      positions replaced by the best cells. ---- */
+typedef struct gz_stream_controller_s *gz_stream_controller_t;
+gz_status gz_stream_controller_create(gz_stream_controller_t *out);
+gz_status gz_stream_controller_destroy(gz_stream_controller_t c);
+/* Init(device_buffer_size): seed arrays of that many entries per resource. */
+gz_status gz_stream_controller_init(gz_stream_controller_t c, size_t device_buffer_size);
+gz_status gz_stream_controller_set_distance_calculator(gz_stream_controller_t c,
+                                                       gz_distance_calculator_t d);
+gz_status gz_stream_controller_set_ungapped_extender(gz_stream_controller_t c,
+                                                     gz_ungapped_extender_t u);
+gz_status gz_stream_controller_set_gapped_extender(gz_stream_controller_t c,
+                                                   gz_gapped_extender_t g);
+/* Run(DistanceCalculationSeedsParameters{size}, host memory): distances of
+ * (query_concatenated_positions[i], database_positions[i]) into
+ * ((char *)result_values)[i]. */
+gz_status gz_stream_controller_run_distance_calculation(gz_stream_controller_t c, size_t size,
+                                                        gz_host_seeds_memory_t m,
+                                                        int *resource_id);
+/* Run(UngappedExtensionWithTriggerSeedsParameters{query_seed_size, size}):
+ * query_seed_ids / query_seed_starts hold the seed runs, database_positions
+ * the size seeds; ((char *)result_values)[i] = trigger flag. */
+gz_status gz_stream_controller_run_ungapped_extension_with_trigger(gz_stream_controller_t c,
+                                                                   size_t query_seed_size,
+                                                                   size_t size,
+                                                                   gz_host_seeds_memory_t m,
+                                                                   int *resource_id);
+/* Run(GappedExtensionSeesParameters): per side, query_seed_ids[off + i]
+ * indexes the side's positions (ConvertToGappedExtensionSeedsAsync), then
+ * ExtendOneSideScoreOnlyAsync writes result_values[off + i] and the best
+ * cells into the position arrays. */
+gz_status gz_stream_controller_run_gapped_extension(gz_stream_controller_t c,
+                                                    size_t reverse_offset, size_t reverse_length,
+                                                    size_t forward_offset, size_t forward_length,
+                                                    gz_host_seeds_memory_t m, int *resource_id);
+/* WaitRun(id) (id < 0: nothing to wait for) and WaitRun() for all. */
+gz_status gz_stream_controller_wait_run(gz_stream_controller_t c, int resource_id);
+gz_status gz_stream_controller_wait_run_all(gz_stream_controller_t c);
+/* FinishedRun(id): 1 when the resource's streams are idle (or id < 0). */
+int gz_stream_controller_finished_run(gz_stream_controller_t c, int resource_id);
+gz_status gz_stream_controller_release_resource(gz_stream_controller_t c, int resource_id);
+uint32_t gz_stream_controller_available_resources(gz_stream_controller_t c);
+
 /* ---- queries (Queries / ProteinQuery / TranslatedDnaQuery,
  *      src/queries.cpp:183-259, src/protein_query.cpp:22-60,
  *      src/translated_dna_query.cpp:27-85) ---- */
@@ -372,6 +440,63 @@ gz_status gz_align_fasta(const char *queries_path, const char *database_path,
                          const gz_aligner_parameters *p,
                          const gz_database_options *database_options);
 void gz_free(void *p);
+
+/* ---- SeedSearcherGpu (src/seed_searcher_gpu.h:28-115,
+ *      src/seed_searcher_gpu.cpp:48-548) on the device seed stage.
+ *      The reference builds host seed lists that its kernels score (distance,
+ *      ungapped extension with trigger) before the host sorts the passing
+ *      seeds into per-query buckets (src/aligner_gpu_presearch_thread.cpp
+ *      :822-896).  Here the searches run that whole chain on the device for
+ *      the queries [query_begin, query_end) (query_begin a multiple of 64,
+ *      query_end one too or the number of queries) and return its output:
+ *      the passing seeds of each query in the one-thread CPU enumeration
+ *      order (lists in presearch order; per list the representatives, each
+ *      followed by its members, then the non-clustered values), which is the
+ *      order the chain filter expects.  Calls on one searcher are serialized
+ *      by the caller. ---- */
+typedef struct gz_seed_searcher_s *gz_seed_searcher_t;
+typedef struct {
+  uint64_t number_hits;
+  uint32_t number_queries;              /* query_end - query_begin */
+  const uint32_t *query_offsets;        /* number_queries + 1: hits of query
+                                           query_begin + i are [off[i], off[i+1]) */
+  const uint32_t *query_positions;      /* query-local (Hit::query_sequence_position) */
+  const uint32_t *database_positions;   /* chunk positions */
+} gz_seed_hits;  /* owned by the searcher, valid until its next call */
+/* p: the cutoffs (NULL: defaults); the searcher owns its device state. */
+gz_status gz_seed_searcher_create(const gz_aligner_parameters *p, gz_seed_searcher_t *out);
+gz_status gz_seed_searcher_destroy(gz_seed_searcher_t h);
+/* Reset(QueryParameters&, DatabaseParameters&): the query set (one query
+ * chunk) and database chunk `chunk` become the searched pair; the hash lists
+ * are built on the device (again only when the query set changes).  Returns
+ * the number of hash lists. */
+gz_status gz_seed_searcher_reset(gz_seed_searcher_t h, gz_queries_t queries,
+                                 gz_database_t database, uint32_t chunk,
+                                 uint32_t *number_hash_lists);
+/* SearchSeedsFromRepresentativesForSimilarityFiltering(id, seed list): the
+ * distance-calculation pairs of hash list `list` (presearch order):
+ * (checked entry's concatenated query position, representative position),
+ * representatives outer.  *count receives the number of pairs;
+ * GZ_ERROR_CAPACITY when it exceeds `capacity`.  Unclustered chunks have
+ * none. */
+gz_status gz_seed_searcher_search_seeds_from_representatives_for_similarity_filtering(
+    gz_seed_searcher_t h, uint32_t list, uint32_t *query_concatenated_positions,
+    uint32_t *database_positions, size_t capacity, size_t *count);
+/* SearchSeedsFromClusteringSubsequence: passing seeds from the
+ * representatives and their members (distance filter included). */
+gz_status gz_seed_searcher_search_seeds_from_clustering_subsequence(gz_seed_searcher_t h,
+                                                                    uint32_t query_begin,
+                                                                    uint32_t query_end,
+                                                                    gz_seed_hits *out);
+/* SearchSeedsFromNonClusteringSubsequence: passing seeds from the
+ * non-clustered values. */
+gz_status gz_seed_searcher_search_seeds_from_non_clustering_subsequence(gz_seed_searcher_t h,
+                                                                        uint32_t query_begin,
+                                                                        uint32_t query_end,
+                                                                        gz_seed_hits *out);
+/* Both sources, merged in the enumeration order. */
+gz_status gz_seed_searcher_search_seeds(gz_seed_searcher_t h, uint32_t query_begin,
+                                        uint32_t query_end, gz_seed_hits *out);
 
 /* ---- result build (AlignerBuildResultsThread, aligner_build_results_thread.h
  *      :64-190 with GappedExtender's traceback, gapped_extender.cpp:70-304,473-531,

@@ -12,14 +12,22 @@
  *     on randomised inputs, plus the known answers of the reference tests;
  *   - chain filter: reference tests test/chain_filter_test.cpp:11-121 plus a
  *     brute-force check of the X-drop walk;
- *   - seed enumeration order, AddResults, BuildResults, output: they do not
- *     compile standalone in the reference (Boost), so they are pinned by
- *     invariants (tests/test_oracle_aligner.py) and by brute force on tiny
- *     inputs.
+ *   - the whole search (translation, hash lists and their visiting order,
+ *     chain filter, AddResults, BuildResults, output fields): the reference's
+ *     own end-to-end example, README.md:88-97 on data/*.fasta
+ *     (tests/test_readme_example.py);
+ *   - AddResults alone: known answers worked by hand from
+ *     src/aligner_common.h:147-274 for every rule, including its containment
+ *     quirk (tests/test_add_results.py);
+ *   - seed enumeration order and BuildResults' arena order: the reference
+ *     does not compile standalone (Boost), so they stay pinned only by
+ *     reading (parity unpinned beyond the README example).
  * Parity hazard decisions (recorded once):
  *   - presearch `end` (uninitialised in aligner_presearch_thread.cpp:322,
  *     363-364) is defined as hit + forward best offset, as the GPU path does
- *     (aligner_gpu_presearch_thread.cpp:969-973);
+ *     (aligner_gpu_presearch_thread.cpp:969-973); gzo_set_end_reading(1)
+ *     switches to the other reading (zero at the declaration, accumulated
+ *     over the hits of one Extend call) to count what that would change;
  *   - order = `-a 1` order.
  */
 #include "gz_oracle.h"
@@ -757,6 +765,9 @@ void AddResults(const ChunkView &db, const gzo_parameters &p,
 
 // per thread, so that searches on several threads at once keep their own counts
 thread_local gzo_stats g_stats;
+// 0: presearch end = hit + forward best offset; 1: end zero-initialised and
+// accumulated over one query's hits of one chunk
+thread_local int g_end_reading = 0;
 
 struct ListEntry {
   bool check;
@@ -810,6 +821,101 @@ void BuildHashLists(const gzo_queries &Q, const gzo_database &D,
           }
         }
         off = e + 1;
+      }
+    }
+  }
+}
+
+// The seed search of one database chunk for every query: hits appended per
+// query in the one-thread enumeration order (AlignerPresearchThread::Run,
+// src/aligner_presearch_thread.cpp:140-152, with SeedSearcher::Search,
+// src/seed_searcher.cpp:103-283).
+void SearchChunk(const gzo_queries &Q, const gzo_database &D, const ChunkView &db,
+                 const std::vector<HashList> &lists, char distance_threshold,
+                 int max_mismatch, uint8_t delim, const int *sm, uint32_t nl,
+                 const std::vector<int> &ucut, const std::vector<int> &trig,
+                 std::vector<std::vector<Hit> > &hits) {
+  const uint32_t L = D.subsequence_length;
+  const uint8_t *ds = db.c->sequence;
+  // AlignerPresearchThread::Run seed-search loop (:140-152): blocks of
+  // 256 hash lists, popped from the back.
+  const uint32_t nlists = (uint32_t)lists.size();
+  for (uint32_t block = 0; block < nlists; block += 256) {
+    uint32_t bend = std::min(nlists, block + 256);
+    for (uint32_t li = bend; li-- > block;) {
+      const HashList &hl = lists[li];
+      uint32_t hidx;
+      if (!db.Lookup(hl.hash, &hidx)) continue;
+      // SearchFromClusteringSubsequences (src/seed_searcher.cpp:103-223)
+      if (db.c->clustering) {
+        for (uint32_t r = db.c->rep_offsets[hidx];
+             r < db.c->rep_offsets[hidx + 1]; ++r) {
+          const uint32_t rpos = db.c->rep_positions[r];
+          std::vector<uint32_t> hitting;
+          for (size_t k = 0; k < hl.data.size(); ++k) {
+            const ListEntry &e = hl.data[k];
+            const uint8_t *qs = Q.sequences + Q.offsets[e.query_id];
+            int dist = 0;
+            if (e.check) {
+              ++g_stats.distance_calculations;
+              dist = Distance(qs, e.position, ds, rpos, L, D.reduced_code_map);
+              int lower = std::max(dist - max_mismatch, 0);
+              if (distance_threshold >= lower) hitting.push_back((uint32_t)k);
+            } else {
+              hitting.push_back((uint32_t)k);
+            }
+            if (distance_threshold >= dist) {
+              ++g_stats.ungapped_extensions;
+              int s = UngappedWithTrigger(qs, e.position, ds, rpos, delim,
+                                          sm, nl, ucut[e.query_id],
+                                          trig[e.query_id]);
+              if (s > trig[e.query_id]) {
+                Hit h;
+                h.q = e.position;
+                h.d = rpos;
+                hits[e.query_id].push_back(h);
+              }
+            }
+          }
+          if (!hitting.empty()) {
+            for (uint32_t m = db.c->rep_member_offsets[r];
+                 m < db.c->rep_member_offsets[r + 1]; ++m) {
+              const uint32_t mpos = db.c->member_positions[m];
+              for (size_t k = 0; k < hitting.size(); ++k) {
+                const ListEntry &e = hl.data[hitting[k]];
+                const uint8_t *qs = Q.sequences + Q.offsets[e.query_id];
+                ++g_stats.ungapped_extensions;
+                int s = UngappedWithTrigger(qs, e.position, ds, mpos, delim,
+                                            sm, nl, ucut[e.query_id],
+                                            trig[e.query_id]);
+                if (s > trig[e.query_id]) {
+                  Hit h;
+                  h.q = e.position;
+                  h.d = mpos;
+                  hits[e.query_id].push_back(h);
+                }
+              }
+            }
+          }
+        }
+      }
+      // SearchFromNonClusteringSubsequences (src/seed_searcher.cpp:225-283)
+      for (uint32_t v = db.c->value_offsets[hidx];
+           v < db.c->value_offsets[hidx + 1]; ++v) {
+        const uint32_t vpos = db.c->value_positions[v];
+        for (size_t k = 0; k < hl.data.size(); ++k) {
+          const ListEntry &e = hl.data[k];
+          const uint8_t *qs = Q.sequences + Q.offsets[e.query_id];
+          ++g_stats.ungapped_extensions;
+          int s = UngappedWithTrigger(qs, e.position, ds, vpos, delim, sm,
+                                      nl, ucut[e.query_id], trig[e.query_id]);
+          if (s > trig[e.query_id]) {
+            Hit h;
+            h.q = e.position;
+            h.d = vpos;
+            hits[e.query_id].push_back(h);
+          }
+        }
       }
     }
   }
@@ -902,6 +1008,8 @@ int gzo_hash(const uint8_t *sequence, const uint8_t *code_map,
 }
 
 void gzo_last_stats(gzo_stats *s) { *s = g_stats; }
+
+void gzo_set_end_reading(int reading) { g_end_reading = reading; }
 
 int gzo_query_lists(const gzo_queries *Qp, const gzo_database *Dp, uint64_t *number_lists,
                     uint32_t **hash, uint32_t **entry_offsets, uint64_t *number_entries,
@@ -996,6 +1104,50 @@ int gzo_add_results(const uint32_t *subject_offsets, uint32_t number_subjects, i
   return 0;
 }
 
+int gzo_seed_hits(const gzo_queries *Qp, const gzo_database *Dp, const gzo_parameters *Pp,
+                  uint32_t chunk, uint32_t **query_offsets, uint32_t **hits) {
+  const gzo_queries &Q = *Qp;
+  const gzo_database &D = *Dp;
+  const gzo_parameters &P = *Pp;
+  if (chunk >= D.number_chunks) return 1;
+  const uint32_t nq = Q.number_queries;
+  std::vector<int> ucut(nq), trig(nq);
+  for (uint32_t i = 0; i < nq; ++i) {
+    Karlin k;
+    k.lambda = Q.lambda[i];
+    k.K = Q.K[i];
+    k.H = 0;
+    ucut[i] = NormalizedCutoff2NominalCutoff(P.normalized_ungapped_cutoff, k);
+    trig[i] = Normalized2Nominal(P.normalized_gapped_trigger, k);
+  }
+  std::vector<HashList> lists;
+  BuildHashLists(Q, D, P.delimiter, lists);
+  const uint32_t L = D.subsequence_length;
+  const float rate = 0.8f;
+  const char distance_threshold = L - rate * L;
+  ChunkView db;
+  db.c = &D.chunks[chunk];
+  db.id = (int)chunk;
+  std::vector<std::vector<Hit> > hv(nq);
+  SearchChunk(Q, D, db, lists, distance_threshold, D.max_mismatches, P.delimiter, P.matrix,
+              P.number_letters, ucut, trig, hv);
+  size_t total = 0;
+  for (const auto &v : hv) total += v.size();
+  *query_offsets = (uint32_t *)malloc(sizeof(uint32_t) * (nq + 1));
+  *hits = (uint32_t *)malloc(sizeof(uint32_t) * (2 * total + 1));
+  size_t o = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    (*query_offsets)[i] = (uint32_t)o;
+    for (const Hit &h : hv[i]) {
+      (*hits)[2 * o] = h.q;
+      (*hits)[2 * o + 1] = h.d;
+      ++o;
+    }
+  }
+  (*query_offsets)[nq] = (uint32_t)o;
+  return 0;
+}
+
 void gzo_free(void *p) { free(p); }
 
 int gzo_align(const gzo_queries *Qp, const gzo_database *Dp,
@@ -1049,88 +1201,8 @@ int gzo_align(const gzo_queries *Qp, const gzo_database *Dp,
     db.id = (int)ci;
     const uint8_t *ds = db.c->sequence;
 
-    // AlignerPresearchThread::Run seed-search loop (:140-152): blocks of
-    // 256 hash lists, popped from the back.
-    const uint32_t nlists = (uint32_t)lists.size();
-    for (uint32_t block = 0; block < nlists; block += 256) {
-      uint32_t bend = std::min(nlists, block + 256);
-      for (uint32_t li = bend; li-- > block;) {
-        const HashList &hl = lists[li];
-        uint32_t hidx;
-        if (!db.Lookup(hl.hash, &hidx)) continue;
-        // SearchFromClusteringSubsequences (src/seed_searcher.cpp:103-223)
-        if (db.c->clustering) {
-          for (uint32_t r = db.c->rep_offsets[hidx];
-               r < db.c->rep_offsets[hidx + 1]; ++r) {
-            const uint32_t rpos = db.c->rep_positions[r];
-            std::vector<uint32_t> hitting;
-            for (size_t k = 0; k < hl.data.size(); ++k) {
-              const ListEntry &e = hl.data[k];
-              const uint8_t *qs = Q.sequences + Q.offsets[e.query_id];
-              int dist = 0;
-              if (e.check) {
-                ++g_stats.distance_calculations;
-                dist = Distance(qs, e.position, ds, rpos, L, D.reduced_code_map);
-                int lower = std::max(dist - max_mismatch, 0);
-                if (distance_threshold >= lower) hitting.push_back((uint32_t)k);
-              } else {
-                hitting.push_back((uint32_t)k);
-              }
-              if (distance_threshold >= dist) {
-                ++g_stats.ungapped_extensions;
-                int s = UngappedWithTrigger(qs, e.position, ds, rpos, delim,
-                                            sm, nl, ucut[e.query_id],
-                                            trig[e.query_id]);
-                if (s > trig[e.query_id]) {
-                  Hit h;
-                  h.q = e.position;
-                  h.d = rpos;
-                  hits[e.query_id].push_back(h);
-                }
-              }
-            }
-            if (!hitting.empty()) {
-              for (uint32_t m = db.c->rep_member_offsets[r];
-                   m < db.c->rep_member_offsets[r + 1]; ++m) {
-                const uint32_t mpos = db.c->member_positions[m];
-                for (size_t k = 0; k < hitting.size(); ++k) {
-                  const ListEntry &e = hl.data[hitting[k]];
-                  const uint8_t *qs = Q.sequences + Q.offsets[e.query_id];
-                  ++g_stats.ungapped_extensions;
-                  int s = UngappedWithTrigger(qs, e.position, ds, mpos, delim,
-                                              sm, nl, ucut[e.query_id],
-                                              trig[e.query_id]);
-                  if (s > trig[e.query_id]) {
-                    Hit h;
-                    h.q = e.position;
-                    h.d = mpos;
-                    hits[e.query_id].push_back(h);
-                  }
-                }
-              }
-            }
-          }
-        }
-        // SearchFromNonClusteringSubsequences (src/seed_searcher.cpp:225-283)
-        for (uint32_t v = db.c->value_offsets[hidx];
-             v < db.c->value_offsets[hidx + 1]; ++v) {
-          const uint32_t vpos = db.c->value_positions[v];
-          for (size_t k = 0; k < hl.data.size(); ++k) {
-            const ListEntry &e = hl.data[k];
-            const uint8_t *qs = Q.sequences + Q.offsets[e.query_id];
-            ++g_stats.ungapped_extensions;
-            int s = UngappedWithTrigger(qs, e.position, ds, vpos, delim, sm,
-                                        nl, ucut[e.query_id], trig[e.query_id]);
-            if (s > trig[e.query_id]) {
-              Hit h;
-              h.q = e.position;
-              h.d = vpos;
-              hits[e.query_id].push_back(h);
-            }
-          }
-        }
-      }
-    }
+    SearchChunk(Q, D, db, lists, distance_threshold, max_mismatch, delim, sm, nl, ucut, trig,
+                hits);
 
     // chain filter + gapped extension + AddResults per query
     // (src/aligner_presearch_thread.cpp:157-206, Extend :315-383)
@@ -1142,6 +1214,7 @@ int gzo_align(const gzo_queries *Qp, const gzo_database *Dp,
       ChainFilterRun(qs, qlen, ucut[qi], ds, delim, sm, nl, hv);
       g_stats.hits_after_chain_filter += hv.size();
       std::vector<Presearched> tmp;
+      Coordinate acc = {0, 0};
       for (size_t i = 0; i < hv.size(); ++i) {
         Presearched r;
         r.chunk = -1;
@@ -1160,8 +1233,16 @@ int gzo_align(const gzo_queries *Qp, const gzo_database *Dp,
                              false, sm, nl, P.gap_open, P.gap_extension,
                              presearch_cutoff, &score, &p0, &p1, NULL, NULL, NULL);
         r.score += score;
-        r.end.q = r.hit.q + p0;
-        r.end.d = r.hit.d + p1;
+        if (g_end_reading == 0) {
+          r.end.q = r.hit.q + p0;
+          r.end.d = r.hit.d + p1;
+        } else {
+          // AlignerPresearchThread::Extend (:322, 363-364): one tmp_result
+          // per call, end += offset + 1 for each hit
+          acc.q += p0 + 1;
+          acc.d += p1 + 1;
+          r.end = acc;
+        }
         r.hit_count = 1;
         tmp.push_back(r);
         g_stats.gapped_extensions += 2;

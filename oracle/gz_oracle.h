@@ -185,6 +185,21 @@ int gzo_add_results(const uint32_t *subject_offsets, uint32_t number_subjects, i
                     gzo_presearched *added, uint32_t number_added, gzo_presearched *results,
                     uint32_t *number_results);
 
+/* Parity hazard 1 (src/aligner_presearch_thread.cpp:322, 363-364): 0 (the
+ * default) defines the presearch end as hit + forward best offset, as the
+ * GPU path does; 1 reads it as zero-initialised and accumulated over the hits
+ * of one query and chunk.  Applies to later gzo_align calls on this thread. */
+void gzo_set_end_reading(int reading);
+
+/* The seed search of database chunk `chunk` alone: every query's passing
+ * seeds (hits before the chain filter) in the one-thread enumeration order,
+ * as (query-local position, chunk position) pairs; hits of query i are
+ * pairs [query_offsets[i], query_offsets[i+1]).  Both arrays are malloc'd
+ * (gzo_free).  Returns 0. */
+int gzo_seed_hits(const gzo_queries *queries, const gzo_database *database,
+                  const gzo_parameters *parameters, uint32_t chunk,
+                  uint32_t **query_offsets, uint32_t **hits);
+
 /* Presearch-only statistics of the last gzo_align call on this thread. */
 typedef struct {
   uint64_t hash_lists;
