@@ -24,6 +24,7 @@
 #include "query_lists.h"
 #include "query_lists_device.h"
 #include "seg.h"
+#include "statistics.h"
 #include "traceback.h"
 
 using namespace gz;
@@ -140,6 +141,26 @@ gz_status gz_queries_view(gz_queries_t h, gz_queries_view_t *v) {
   v->K = h->q.K.data();
   v->translated = h->q.translated ? 1 : 0;
   v->dna_lengths = h->q.translated ? h->q.dna_lengths.data() : nullptr;
+  return GZ_OK;
+}
+
+gz_status gz_queries_cutoffs(gz_queries_t h, const gz_aligner_parameters *p,
+                             int *ungapped_cutoffs, int *gapped_triggers,
+                             int *gapped_cutoff) {
+  GZ_REQUIRE(h && (h->q.size() == 0 || (ungapped_cutoffs && gapped_triggers)),
+             "gz_queries_cutoffs: null argument");
+  gz_aligner_parameters d;
+  gz_aligner_default_parameters(&d);
+  if (!p) p = &d;
+  for (size_t k = 0; k < h->q.size(); ++k) {
+    KarlinParameters kp = {h->q.lambda[k], h->q.K[k], 0};
+    ungapped_cutoffs[k] =
+        Statistics::NormalizedCutoff2NominalCutoff(p->normalized_ungapped_cutoff, kp);
+    gapped_triggers[k] = Statistics::Normalized2Nominal(p->normalized_gapped_trigger, kp);
+  }
+  if (gapped_cutoff)
+    *gapped_cutoff = Statistics::Normalized2Nominal(p->normalized_presearch_cutoff,
+                                                    Statistics::GappedBlosum62());
   return GZ_OK;
 }
 

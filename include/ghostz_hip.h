@@ -421,6 +421,13 @@ typedef struct {
   uint32_t cells;          /* DP cells the extension evaluated */
 } gz_gapped_trace;
 void gz_aligner_default_parameters(gz_aligner_parameters *p);
+/* Per-query ungapped cutoffs and gapped triggers, and the gapped
+ * extension's cutoff, as AlignerGpu::SetQueriesData computes them
+ * (src/aligner_gpu.cpp:213-242, src/statistics.cpp:28-49,236-249) from the
+ * normalized values in p (NULL: defaults). */
+gz_status gz_queries_cutoffs(gz_queries_t q, const gz_aligner_parameters *p,
+                             int *ungapped_cutoffs, int *gapped_triggers,
+                             int *gapped_cutoff);
 gz_status gz_aligner_create(const gz_aligner_parameters *p, gz_aligner_t *out);
 gz_status gz_aligner_destroy(gz_aligner_t a);
 gz_status gz_aligner_set_database(gz_aligner_t a, gz_database_t db);
