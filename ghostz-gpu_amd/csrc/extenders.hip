@@ -19,7 +19,6 @@
 
 #include <algorithm>
 #include <cstring>
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -758,6 +757,9 @@ struct gz_gapped_extender_s {
   int2 *wide_cells = nullptr;
   uint8_t *wide_codes = nullptr;
   uint32_t wide_allocated = 0;
+  // the failure counter of the last run (its own workspace's, or the
+  // allocation-free entry point's scratch on the device it ran on)
+  uint32_t *failure_counter = nullptr;
   ~gz_gapped_extender_s() {
     if (counters) (void)hipFree(counters);
     if (overflow_list) (void)hipFree(overflow_list);
@@ -823,11 +825,23 @@ gz_status gz_gapped_extender_convert_to_gapped_extension_seeds_async(
 }
 
 extern "C++" {
+// Device memory one gapped run uses beyond its jobs: counters ([0] jobs out
+// of the narrow ring, [1] failures, [2] jobs out of the 64-column ring), the
+// two overflow lists (room for every job) and the global ring (kWideSlots
+// slots of `wide` cells and codes).
+struct GappedWork {
+  uint32_t *counters;
+  uint32_t *overflow_list, *overflow_list2;
+  int2 *wide_cells;
+  uint8_t *wide_codes;
+  uint32_t wide;
+};
+
 template <bool kReverse>
-static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t size,
-                         size_t wide_first, const uint32_t *order, uint32_t *d_q,
+static void GappedLaunch(gz_gapped_extender_t h, const GappedWork &w, const GappedParams &P,
+                         size_t size, size_t wide_first, const uint32_t *order, uint32_t *d_q,
                          uint32_t *d_d, int *d_scores, uint32_t *d_cells, hipStream_t s) {
-  uint32_t *narrow_out = h->counters, *failures = h->counters + 1, *wide_out = h->counters + 2;
+  uint32_t *narrow_out = w.counters, *failures = w.counters + 1, *wide_out = w.counters + 2;
   // 16-bit cells in the LDS rings when the cutoff and gap costs allow them
   // (LaneRingLds16); bands whose scores grow past them move on like bands
   // that outgrow the ring
@@ -839,7 +853,7 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
   hipLaunchKernelGGL((GappedKernel<kReverse, kGappedRing, SHORT>), GRID, dim3(kGappedBlock), 0, \
                      s, (size_t)(BEGIN), (size_t)(END), (const uint32_t *)(LIST),              \
                      (const uint32_t *)(COUNT), ORDER, size, P, h->matrix, h->number_letters,  \
-                     d_q, d_d, d_scores, d_cells, wide_out, h->overflow_list2, failures)
+                     d_q, d_d, d_scores, d_cells, wide_out, w.overflow_list2, failures)
   if (wide_first > 0) {
     const dim3 grid(GridFor(wide_first, kGappedBlock, 256u * 32));
     if (short_cells) GZ_RING64(true, grid, 0, wide_first, nullptr, nullptr, order);
@@ -851,7 +865,7 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
   hipLaunchKernelGGL((GappedKernel<kReverse, WIDTH, SHORT>), grid, dim3(kGappedBlock), 0, s, \
                      wide_first, size, (const uint32_t *)nullptr, (const uint32_t *)nullptr, \
                      order, size, P, h->matrix, h->number_letters, d_q, d_d, d_scores,       \
-                     d_cells, narrow_out, h->overflow_list, failures)
+                     d_cells, narrow_out, w.overflow_list, failures)
     if (h->narrow == 16) {
       if (short_cells) GZ_NARROW(16, true);
       else GZ_NARROW(16, false);
@@ -864,16 +878,33 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedParams &P, size_t s
   {
     // the narrow ring's overflow; 16-bit cells leave LDS for more blocks
     const uint32_t *none = nullptr;
-    if (short_cells) GZ_RING64(true, dim3(256u * 6), 0, 0, h->overflow_list, narrow_out, none);
-    else GZ_RING64(false, dim3(256u * 4), 0, 0, h->overflow_list, narrow_out, none);
+    if (short_cells) GZ_RING64(true, dim3(256u * 6), 0, 0, w.overflow_list, narrow_out, none);
+    else GZ_RING64(false, dim3(256u * 4), 0, 0, w.overflow_list, narrow_out, none);
   }
 #undef GZ_RING64
   hipLaunchKernelGGL((GappedWideKernel<kReverse>), dim3(kWideSlots / kGappedBlock),
                      dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q, d_d,
-                     d_scores, d_cells, wide_out, h->overflow_list2, h->wide_cells,
-                     h->wide_codes, h->wide, failures);
+                     d_scores, d_cells, wide_out, w.overflow_list2, w.wide_cells,
+                     w.wide_codes, w.wide, failures);
 }
 }  // extern "C++"
+
+static gz_status GappedLaunchAll(gz_gapped_extender_t h, const GappedWork &w, size_t size,
+                                 int reverse, uint32_t *d_q, uint32_t *d_d, int *d_scores,
+                                 uint32_t *d_cells, hipStream_t s, size_t wide_first,
+                                 const uint32_t *order) {
+  GappedParams P;
+  P.queries = h->queries;
+  P.database = h->database;
+  P.delim = h->delimiter;
+  P.gap_open = h->gap_open;
+  P.gap_ext = h->gap_ext;
+  P.cutoff = h->cutoff;
+  if (reverse) GappedLaunch<true>(h, w, P, size, wide_first, order, d_q, d_d, d_scores, d_cells, s);
+  else GappedLaunch<false>(h, w, P, size, wide_first, order, d_q, d_d, d_scores, d_cells, s);
+  GZ_HIP_CHECK(hipGetLastError());
+  return GZ_OK;
+}
 
 static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
                            uint32_t *d_q, uint32_t *d_d, int *d_scores,
@@ -907,17 +938,10 @@ static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
   // are kept until gz_gapped_extender_failures reads them
   GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, sizeof(uint32_t), s));
   GZ_HIP_CHECK(hipMemsetAsync(h->counters + 2, 0, sizeof(uint32_t), s));
-  GappedParams P;
-  P.queries = h->queries;
-  P.database = h->database;
-  P.delim = h->delimiter;
-  P.gap_open = h->gap_open;
-  P.gap_ext = h->gap_ext;
-  P.cutoff = h->cutoff;
-  if (reverse) GappedLaunch<true>(h, P, size, wide_first, order, d_q, d_d, d_scores, d_cells, s);
-  else GappedLaunch<false>(h, P, size, wide_first, order, d_q, d_d, d_scores, d_cells, s);
-  GZ_HIP_CHECK(hipGetLastError());
-  return GZ_OK;
+  const GappedWork w = {h->counters, h->overflow_list, h->overflow_list2, h->wide_cells,
+                        h->wide_codes, h->wide};
+  h->failure_counter = h->counters + 1;
+  return GappedLaunchAll(h, w, size, reverse, d_q, d_d, d_scores, d_cells, s, wide_first, order);
 }
 
 gz_status gz_gapped_extender_extend_one_side_score_only_async(
@@ -993,11 +1017,11 @@ gz_status gz_gapped_extender_overflows(gz_gapped_extender_t h, uint32_t *narrow,
 gz_status gz_gapped_extender_failures(gz_gapped_extender_t h, uint32_t *failures) {
   GZ_REQUIRE(h && failures, "null argument");
   *failures = 0;
-  if (h->counters) {
-    uint32_t c[3];
-    GZ_HIP_CHECK(hipMemcpy(c, h->counters, sizeof(c), hipMemcpyDeviceToHost));
-    *failures = c[1];
-    if (c[1] != 0) GZ_HIP_CHECK(hipMemset(h->counters + 1, 0, sizeof(uint32_t)));
+  if (h->failure_counter) {
+    uint32_t c = 0;
+    GZ_HIP_CHECK(hipMemcpy(&c, h->failure_counter, sizeof(c), hipMemcpyDeviceToHost));
+    *failures = c;
+    if (c != 0) GZ_HIP_CHECK(hipMemset(h->failure_counter, 0, sizeof(uint32_t)));
   }
   return GZ_OK;
 }

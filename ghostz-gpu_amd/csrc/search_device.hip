@@ -1,6 +1,8 @@
 // search_device.hip -- see search_device.h.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -20,6 +22,18 @@ constexpr int kSeedItems = 2;
 constexpr uint32_t kSeedTile = kSeedThreads * kSeedItems;
 constexpr uint64_t kUnitsPerLaunch = 1ull << 28;
 constexpr size_t kInitialHitCapacity = 1u << 22;
+
+// The hit buffer's first size: kInitialHitCapacity, or GZ_SEED_HIT_CAPACITY
+// seeds when that variable is set (a small value makes every launch take the
+// grow-and-redo path, which the tests use).
+size_t InitialHitCapacity() {
+  const char *v = getenv("GZ_SEED_HIT_CAPACITY");
+  if (v && *v) {
+    const unsigned long long n = strtoull(v, nullptr, 10);
+    if (n >= 64) return (size_t)n;
+  }
+  return kInitialHitCapacity;
+}
 
 // per list: database representatives and values with the list's hash, and
 // the list's entries that take part in this run
@@ -558,7 +572,7 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
     GZ_TRY(Reserve(&scan_tmp_, &scan_tmp_cap_, tmp));
     GZ_TRY(Reserve(&active_, &active_cap_, 2 * (AlignUp((size_t)(n + 1) * 4, 256) +
                                                 AlignUp((size_t)(n + 1) * 8, 256))));
-    if (!hits_) GZ_TRY(Reserve(&hits_, &hits_cap_, kInitialHitCapacity * sizeof(SeedHit)));
+    if (!hits_) GZ_TRY(Reserve(&hits_, &hits_cap_, InitialHitCapacity() * sizeof(SeedHit)));
 
     const unsigned grid_n = (n + 255) / 256;
     hipLaunchKernelGGL(BlockUnitsKernel, dim3(grid_n), dim3(256), 0, s, q,

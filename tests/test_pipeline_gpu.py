@@ -196,3 +196,30 @@ def test_edge_queries_match_oracle(matrix):
     q = gz.Queries(names, b"".join(texts), offs)
     db = gz.Database(dn, dr, do)
     run_case("edge", q, db, m, nl)
+
+
+def test_protein_search_at_full_small_config_matches_oracle(matrix):
+    """The plain protein workload at its stated size: 500 queries of 100-500
+    aa, each planted three times into a 20,000-protein database."""
+    m, nl = matrix
+    (qn, qr, qo), (dn, dr, do) = synthetic.protein_workload(20261210, 20000, 500)
+    q = gz.Queries(qn, qr, qo)
+    db = gz.Database(dn, dr, do)
+    st, ost = run_case("protein500x20000", q, db, m, nl,
+                       schedules=(gz.SCHEDULE_PLAIN, gz.SCHEDULE_LEARNED))
+    assert st["results"] > 1500
+
+
+def test_seed_hit_buffer_growth_matches_oracle(matrix, monkeypatch):
+    """A 64-seed first hit buffer: every seed launch outgrows it, grows it
+    keeping the hits already found and redoes its range.  The output is the
+    oracle's."""
+    m, nl = matrix
+    monkeypatch.setenv("GZ_SEED_HIT_CAPACITY", "64")
+    (qn, qr, qo), (dn, dr, do) = synthetic.translated_workload(20261211, 1500, 600)
+    q = gz.Queries(qn, qr, qo, translated=True)
+    db = gz.Database(dn, dr, do, chunk_size=150_000)
+    assert db.info().number_chunks >= 2
+    st, _ = run_case("hit-buffer-growth", q, db, m, nl,
+                     schedules=(gz.SCHEDULE_PLAIN, gz.SCHEDULE_LEARNED))
+    assert st["seed_hits"] > 64 * 20
