@@ -304,6 +304,7 @@ _SIGNATURES.update({
     "gz_queries_read_fasta": (c_int, [c_char_p, POINTER(QueryOptions), POINTER(c_void_p)]),
     "gz_queries_destroy": (c_int, [c_void_p]),
     "gz_queries_view": (c_int, [c_void_p, POINTER(QueriesView)]),
+    "gz_queries_cutoffs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_int)]),
     "gz_query_lists_build": (c_int, [c_void_p, c_void_p, c_int, POINTER(QueryListsC)]),
     "gz_query_lists_free": (None, [POINTER(QueryListsC)]),
     "gz_database_create": (c_int, [c_void_p, c_char_p, c_void_p, c_uint64,
