@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -983,6 +984,88 @@ gz_status gz_gapped_extender_extend_one_side_score_only_ordered_async(
   GZ_REQUIRE(h && d_order, "null argument");
   if (size == 0) return GZ_OK;
   return GappedRun(h, size, reverse, d_q, d_d, d_scores, d_cells, S(stream), 0, d_order);
+}
+
+}  // extern "C"
+
+// ---- the allocation-free entry point ----
+// Its workspace is one file-scope device array per GPU: the three counters,
+// the two overflow lists for up to kNoAllocMaxJobs jobs and the global ring
+// for queries of up to kNoAllocMaxWide - 2 residues.
+constexpr size_t kNoAllocMaxJobs = (size_t)1 << 20;
+constexpr uint32_t kNoAllocMaxWide = 4096;
+constexpr size_t kNoAllocCountersBytes = 256;
+constexpr size_t kNoAllocListBytes = kNoAllocMaxJobs * sizeof(uint32_t);
+constexpr size_t kNoAllocCellsBytes = (size_t)kWideSlots * kNoAllocMaxWide * sizeof(int2);
+constexpr size_t kNoAllocCodesBytes = (size_t)kWideSlots * kNoAllocMaxWide;
+constexpr size_t kNoAllocScratchBytes =
+    kNoAllocCountersBytes + 2 * kNoAllocListBytes + kNoAllocCellsBytes + kNoAllocCodesBytes;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch[kNoAllocMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device with
+// that device current (host code must not pass the array's name itself).
+gz_status ScratchOfCurrentDevice(uint8_t **out) {
+  int dev = 0;
+  GZ_HIP_CHECK(hipGetDevice(&dev));
+  GZ_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "no-alloc gapped extension: device id");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    GZ_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    GZ_REQUIRE(p != nullptr, "no-alloc gapped extension: no scratch address");
+    g_scratch[dev] = p;
+  }
+  *out = (uint8_t *)g_scratch[dev];
+  return GZ_OK;
+}
+
+__global__ void ClearCountersKernel(uint32_t *counters) {
+  if (threadIdx.x < 3) counters[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+gz_status gz_gapped_extender_extend_one_side_score_only_noalloc_async(
+    gz_gapped_extender_t h, size_t size, int reverse, uint32_t *d_q, uint32_t *d_d,
+    int *d_scores, uint32_t *d_cells, gz_stream_t stream) {
+  GZ_REQUIRE(h, "null handle");
+  GZ_REQUIRE(h->queries && h->database && h->matrix,
+             "gapped extender: queries, database and score parameters must be set");
+  if (size > kNoAllocMaxJobs) {
+    SetError("no-alloc gapped extension: more than 1048576 jobs in one call");
+    return GZ_ERROR_CAPACITY;
+  }
+  if (h->wide > kNoAllocMaxWide) {
+    SetError("no-alloc gapped extension: the maximum sequence length is above 4094 residues");
+    return GZ_ERROR_CAPACITY;
+  }
+  if (size == 0) return GZ_OK;
+  GZ_REQUIRE(d_q && d_d && d_scores, "null argument");
+  uint8_t *base = nullptr;
+  const gz_status st = ScratchOfCurrentDevice(&base);
+  if (st != GZ_OK) return st;
+  hipStream_t s = S(stream);
+  GappedWork w;
+  w.counters = (uint32_t *)base;
+  w.overflow_list = (uint32_t *)(base + kNoAllocCountersBytes);
+  w.overflow_list2 = (uint32_t *)(base + kNoAllocCountersBytes + kNoAllocListBytes);
+  w.wide_cells = (int2 *)(base + kNoAllocCountersBytes + 2 * kNoAllocListBytes);
+  w.wide_codes = base + kNoAllocCountersBytes + 2 * kNoAllocListBytes + kNoAllocCellsBytes;
+  w.wide = h->wide;
+  // this call's counts and failures start at zero (a fill kernel, no memset
+  // node, so that the call can be captured into a graph)
+  hipLaunchKernelGGL(ClearCountersKernel, dim3(1), dim3(64), 0, s, w.counters);
+  GZ_HIP_CHECK(hipGetLastError());
+  h->failure_counter = w.counters + 1;
+  return GappedLaunchAll(h, w, size, reverse, d_q, d_d, d_scores, d_cells, s, 0, nullptr);
 }
 
 gz_status gz_gapped_extender_set_max_sequence_length(gz_gapped_extender_t h,

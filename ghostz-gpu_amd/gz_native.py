@@ -85,6 +85,8 @@ _SIGNATURES = {
     "gz_gapped_extender_extend_one_side_score_only_ordered_async": (
         c_int, [c_void_p, c_size_t, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                 c_void_p]),
+    "gz_gapped_extender_extend_one_side_score_only_noalloc_async": (
+        c_int, [c_void_p, c_size_t, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gz_gapped_extender_set_max_sequence_length": (c_int, [c_void_p, c_uint32]),
     "gz_gapped_extender_set_narrow_ring": (c_int, [c_void_p, c_uint32]),
     "gz_gapped_extender_overflows": (c_int, [c_void_p, u32p, u32p]),

@@ -200,6 +200,20 @@ gz_status gz_gapped_extender_extend_one_side_score_only_ordered_async(
     gz_gapped_extender_t h, size_t size, int reverse, const uint32_t *d_order,
     uint32_t *d_query_concatenated_positions, uint32_t *d_database_positions,
     int *d_scores, uint32_t *d_cells, gz_stream_t stream);
+/* The same as gz_gapped_extender_extend_one_side_score_only_cells_async
+ * (d_cells may be NULL), allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops).  Its workspace is one static device
+ * array per GPU (about 18 MB, held until the process ends); the call
+ * rejects more than 1,048,576 jobs or a maximum sequence length
+ * (gz_gapped_extender_set_max_sequence_length) above 4094 residues with
+ * GZ_ERROR_CAPACITY.  Every call on a GPU uses that GPU's one array, so
+ * calls that overlap on one GPU (two streams, two threads) race: serialize
+ * them.  gz_gapped_extender_failures reports the last call's failures. */
+gz_status gz_gapped_extender_extend_one_side_score_only_noalloc_async(
+    gz_gapped_extender_t h, size_t size, int reverse,
+    uint32_t *d_query_concatenated_positions, uint32_t *d_database_positions,
+    int *d_scores, uint32_t *d_cells, gz_stream_t stream);
 /* Longest query the extender must handle (sizes the wide-band fallback
  * ring; default 65534 residues).  No counterpart in the reference, whose
  * GPU kernel caps the query side at 128 residues
