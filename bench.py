@@ -55,10 +55,14 @@ GAPPED_OPS_PER_CELL = 10
 MLP_FLOP_PER_SEED = 2 * (1024 * 256 + 256 * 64 + 64)
 
 WORKLOADS = {
-    "reads1m_db4m": {"reads": 1_000_000, "db_proteins": 4_000_000, "db_chunks": 4,
-                     "query_chunks": 3, "train_reads": 10_000},
-    "reads200k_db1m": {"reads": 200_000, "db_proteins": 1_000_000, "db_chunks": 1,
-                       "query_chunks": 1, "train_reads": 40_000},
+    "reads1m_db4m": {"kind": "translated", "reads": 1_000_000, "db_proteins": 4_000_000,
+                     "db_chunks": 4, "query_chunks": 3, "train_reads": 10_000},
+    "reads200k_db1m": {"kind": "translated", "reads": 200_000, "db_proteins": 1_000_000,
+                       "db_chunks": 1, "query_chunks": 1, "train_reads": 40_000},
+    # protein queries of Pareto-tailed lengths (30 to 35,000 aa, median 250,
+    # 20% with three planted homologs) against 1,000,000 proteins
+    "pareto50k_db1m": {"kind": "uneven_protein", "reads": 50_000, "db_proteins": 1_000_000,
+                       "db_chunks": 1, "query_chunks": 2, "train_reads": 4_000},
 }
 
 
@@ -112,12 +116,25 @@ def host_threads():
     return max(1, min(n, 16))
 
 
-def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_000_000):
-    """Logs extension traces with the plain schedule on training reads, then
-    trains the MLP on the GPU.  Returns (flat weights, report dict)."""
+def training_queries(gz, kind, rng, db_codes, n):
+    """Training queries drawn like the workload's: translated reads, or
+    homologs of random database proteins (30-90% identity) for protein
+    queries."""
     import synthetic
-    names, res, offs = synthetic.translated_reads(rng, db_codes, n_reads)
-    q = gz.Queries(names, res, offs, translated=True)
+    if kind == "translated":
+        names, res, offs = synthetic.translated_reads(rng, db_codes, n)
+        return gz.Queries(names, res, offs, translated=True)
+    qs = [synthetic.homolog(rng, db_codes[int(rng.integers(len(db_codes)))],
+                            rng.uniform(0.3, 0.9)) for _ in range(n)]
+    names, res, offs = synthetic._pack(qs, "train_")
+    return gz.Queries(names, res, offs)
+
+
+def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_000_000,
+                     kind="translated"):
+    """Logs extension traces with the plain schedule on training queries, then
+    trains the MLP on the GPU.  Returns (flat weights, report dict)."""
+    q = training_queries(gz, kind, rng, db_codes, n_reads)
     al = gz.Aligner(db, record_traces=1)
     _, st = al.align(q)
     tr = al.traces()
@@ -181,7 +198,8 @@ def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_
     return mlp.weights(), report, gapped_roof
 
 
-def oracle_subset(gz, oracle_py, db, al, names, res, offs, m, nl, n_sub, sub_reads):
+def oracle_subset(gz, oracle_py, db, al, names, res, offs, m, nl, n_sub, sub_reads,
+                  translated=True):
     """A deterministic subset of the reads, cut into n_sub sub-chunks of
     sub_reads reads, each searched as its own query chunk by the GPU (al)
     and by the oracle (one sub-chunk per host thread).  Returns (identical,
@@ -198,7 +216,7 @@ def oracle_subset(gz, oracle_py, db, al, names, res, offs, m, nl, n_sub, sub_rea
         ln = np.array([int(offs[i + 1] - offs[i]) for i in idx], np.uint64)
         of = np.zeros(len(idx) + 1, np.uint64)
         np.cumsum(ln, out=of[1:])
-        q = gz.Queries(nm, rs, of, translated=True)
+        q = gz.Queries(nm, rs, of, translated=translated)
         sub_q.append(q)
         views.append(q.view())
     gpu = [al.align(q)[0] for q in sub_q]
@@ -235,6 +253,21 @@ def oracle_subset(gz, oracle_py, db, al, names, res, offs, m, nl, n_sub, sub_rea
     return identical, base
 
 
+def tuner_report(al):
+    """The batched form's autotuner after the run: per query-length bucket,
+    the configuration in use and the measured kernel ms per job of each."""
+    chosen, ms, trials = al.tuner_state()
+    names = ["ring32/queue", "ring16/queue", "ring32/grid", "ring16/grid"]
+    out = {}
+    for b, label in enumerate(["<=128", "<=512", "<=2048", ">2048"]):
+        if chosen[b] < 0:
+            continue
+        out[label] = {"chosen": names[chosen[b]],
+                      "us_per_job": {names[c]: round(1e3 * ms[b, c], 5)
+                                     for c in range(4) if trials[b, c]}}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,8 +286,14 @@ def main():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--pipeline-blocks", type=int, default=0,
                     help="query blocks per chunk in the batched forms (0: library default)")
+    ap.add_argument("--profile-form", choices=["plain", "batched", "length_sorted", "learned"],
+                    default=None,
+                    help="run only this form (W warm-up and K timed steps) and print its "
+                         "rate; for profiler runs, no result line")
     ap.add_argument("--no-learned", action="store_true",
                     help="plain form only (for profiling runs)")
+    ap.add_argument("--sorted-form", action="store_true",
+                    help="also run the batched form in query-side-length order")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch a 1 GiB streaming copy kernel (counter calibration)")
     args = ap.parse_args()
@@ -275,8 +314,18 @@ def main():
     n_reads = max(1000, int(wl["reads"] * args.scale))
     rng = np.random.default_rng(args.seed)
     t0 = time.perf_counter()
+    translated = wl["kind"] == "translated"
     db_codes = synthetic.protein_database(rng, n_db)
-    names, res, offs = synthetic.translated_reads(rng, db_codes, n_reads)
+    if translated:
+        names, res, offs = synthetic.translated_reads(rng, db_codes, n_reads)
+    else:
+        lens = synthetic.pareto_lengths(rng, n_reads)
+        qcodes = [synthetic.random_codes(rng, int(L)) for L in lens]
+        for qc in qcodes:
+            if rng.random() < 0.2:
+                synthetic.plant(rng, db_codes, qc, 3)
+        names, res, offs = synthetic._pack(qcodes, "query_")
+        del qcodes
     dn, dr, do = synthetic._pack(db_codes, "subject_")
     total_aa = int(do[-1])
     log(f"workload generated in {time.perf_counter() - t0:.1f} s: {n_db} proteins "
@@ -292,14 +341,20 @@ def main():
     # query chunks: equal thirds of the reads, each one query chunk (every
     # third stays below the default -l of 2^27 framed codes)
     nq = wl["query_chunks"]
-    bounds = [n_reads * i // nq for i in range(nq + 1)]
+    if translated:
+        bounds = [n_reads * i // nq for i in range(nq + 1)]
+    else:  # equal residue shares
+        cum = np.asarray(offs, np.int64)
+        bounds = [0] + [int(np.searchsorted(cum, cum[-1] * i // nq)) for i in range(1, nq)] + \
+            [n_reads]
     qchunks, q_aa = [], []
     t0 = time.perf_counter()
     for i in range(nq):
         a, b = bounds[i], bounds[i + 1]
         base = int(offs[a])
         q = gz.Queries(names[a:b], res[base:int(offs[b])],
-                       np.asarray(offs[a:b + 1], np.uint64) - np.uint64(base), translated=True)
+                       np.asarray(offs[a:b + 1], np.uint64) - np.uint64(base),
+                       translated=translated)
         qchunks.append(q)
         q_aa.append(query_residues(q.view()))
     log(f"database built in {t_db:.1f} s ({n_chunks} chunks); queries prepared in "
@@ -322,7 +377,7 @@ def main():
         tr_reads = max(1000, int(tr_reads * min(1.0, args.scale * 10)))
         weights, learn, gapped_roof = train_cost_model(
             gz, db, db_codes, np.random.default_rng(args.seed + 1), tr_reads, args.epochs,
-            args.seed)
+            args.seed, kind=wl["kind"])
     del db_codes
     log(f"cost model trained in {time.perf_counter() - t0:.1f} s: {learn}")
 
@@ -333,7 +388,21 @@ def main():
         batched = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN, batched=1, **pb)
         learned = gz.Aligner(db, schedule=gz.SCHEDULE_LEARNED, batched=1, **pb)
         learned.set_cost_model(weights)
-        forms += [("batched", batched), ("learned", learned)]
+        forms += [("batched", batched)]
+        if args.sorted_form:
+            forms += [("length_sorted", gz.Aligner(db, schedule=gz.SCHEDULE_LENGTH_SORTED,
+                                                   batched=1, **pb))]
+        forms += [("learned", learned)]
+
+    if args.profile_form:
+        al = {"plain": plain}.get(args.profile_form)
+        if al is None:
+            for name, a in forms:
+                if name == args.profile_form:
+                    al = a
+        if al is None and args.profile_form == "length_sorted":
+            al = gz.Aligner(db, schedule=gz.SCHEDULE_LENGTH_SORTED, batched=1, **pb)
+        forms = [(args.profile_form, al)]
 
     digests = {}  # query chunk -> set of output digests over all forms and runs
 
@@ -352,7 +421,8 @@ def main():
             log(f"{label} step {i} (query chunk {c}): {times[-1]:.3f} s, "
                 f"{lines} lines, seed {st['seconds_seed']:.3f} s, gapped "
                 f"{st['seconds_gapped']:.3f} s (kernel {st['gapped_kernel_ms']:.1f} ms, "
-                f"schedule {st['schedule_kernel_ms']:.1f} ms)")
+                f"schedule {st['schedule_kernel_ms']:.1f} ms, predicted jobs "
+                f"{st['gated_jobs']} of {st['gapped_extensions']})")
             del text
         return stats, times, aa
 
@@ -364,6 +434,10 @@ def main():
         run(al, 0, w, name + " warmup")
         results[name] = run(al, w, k, name)
     flag = forms[-1][0]
+    if args.profile_form:
+        r = results[flag]
+        log(f"{flag}: {r[2] / sum(r[1]):.4g} query aa/s over {len(r[1])} steps")
+        return
     identical = all(len(v) == 1 for v in digests.values())
     if not identical:
         log("ERROR: the forms' outputs differ")
@@ -418,7 +492,9 @@ def main():
     if not args.no_cpu:
         import oracle_py
         subset_ok, cpu = oracle_subset(gz, oracle_py, db, forms[-1][1], names, res, offs, m,
-                                       nl, host_threads(), args.subset_reads)
+                                       nl, host_threads(),
+                                       args.subset_reads if translated else 16,
+                                       translated=translated)
 
     sched_ms = learned_kernels["schedule"]
     line = {
@@ -427,9 +503,11 @@ def main():
         "warmup": args.warmup, "ms_per_step": 1e3 * sum(l_t) / len(l_t),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "translated 150-nt reads vs synthetic protein database "
-                   "(aln -q d -t p, BLOSUM62 11/1, -b 10 -v 1, SEG); step = one query "
-                   "chunk against every database chunk",
+        "config": {"workload": ("translated 150-nt reads" if translated else
+                                "protein queries of Pareto-tailed lengths") +
+                   " vs synthetic protein database (aln -q " + ("d" if translated else "p") +
+                   " -t p, BLOSUM62 11/1, -b 10 -v 1, SEG); step = one query chunk against "
+                   "every database chunk",
                    "name": args.workload, "reads": n_reads, "db_proteins": n_db,
                    "db_residues": int(db.info().total_length), "db_chunks": n_chunks,
                    "query_chunks": nq, "query_aa": int(sum(q_aa)),
@@ -446,13 +524,18 @@ def main():
         "schedule_ms_vs_gapped_ms_saved": None if args.no_learned else {
             "schedule_ms": sched_ms,
             "gapped_ms_batched": kernel_ms(results["batched"][0])["gapped"],
-            "gapped_ms_learned": learned_kernels["gapped"]},
+            "gapped_ms_learned": learned_kernels["gapped"],
+            "predicted_jobs": total(l_st, "gated_jobs") // len(l_st),
+            "gapped_jobs": total(l_st, "gapped_extensions") // len(l_st)},
         "search_stats": {k: total(p_st, k) // len(p_st) for k in (
             "lists", "units_rep", "units_value", "seed_hits", "chained_hits",
             "gapped_extensions", "results", "sequential_build_chunks")},
         "stage_seconds": {k: round(total(p_st, k) / len(p_st), 4) for k in p_st[0]
                           if k.startswith("seconds_")},
         "cost_model": learn,
+        "autotuner": {name: tuner_report(al) for name, al in forms if name != "plain"},
+        "graphs": {name: dict(zip(("instantiated", "updated"), al.graph_counts()))
+                   for name, al in forms if name != "plain"},
         "roofline": roof,
         "stage_rooflines": stage_roofs,
         "cpu_baseline": cpu,

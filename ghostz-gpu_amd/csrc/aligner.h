@@ -12,7 +12,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +28,23 @@
 struct gz_gapped_extender_s;
 
 namespace gz {
+
+// The batched form's autotuner: per query-length bucket, every
+// configuration of the first gapped ring (width 32 or 16 columns, persistent
+// work queue or static grid) is timed kTrials times on real blocks, then the
+// one with the fewest kernel milliseconds per job is used.  Results never
+// depend on the configuration.
+struct GappedTuner {
+  static constexpr int kConfigs = 4;
+  static constexpr int kTrials = 2;
+  double ms[kLengthBuckets][kConfigs] = {};
+  double jobs[kLengthBuckets][kConfigs] = {};
+  int trials[kLengthBuckets][kConfigs] = {};
+  static uint32_t Narrow(int config) { return (config & 1) ? 16u : 32u; }
+  static int Persistent(int config) { return (config & 2) ? 0 : 1; }
+  int Pick(int bucket) const;
+  void Record(int bucket, int config, double kernel_ms, double njobs);
+};
 
 class Aligner : public PresearchDevice {
  public:
@@ -55,7 +74,15 @@ class Aligner : public PresearchDevice {
                AlignStats *st) override;
   int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward, uint32_t *qpos,
                    uint32_t *dpos, int *scores, uint32_t *cells, const uint16_t *keys,
-                   bool learned, AlignStats *st) override;
+                   bool learned, AlignStats *st, const uint32_t *bucket_off) override;
+  void GraphCounts(uint64_t *instantiations, uint64_t *updates) const {
+    *instantiations = graph_instantiations_;
+    *updates = graph_updates_;
+  }
+  GappedTuner Tuner() {
+    std::lock_guard<std::mutex> lock(tuner_mutex_);
+    return tuner_;
+  }
   int Tracebacks(size_t chunk, const std::vector<TracebackJob> &jobs, int cutoff,
                  std::vector<TracebackOut> *out, AlignStats *st) override;
   void EndQueryChunk() override;
@@ -67,7 +94,10 @@ class Aligner : public PresearchDevice {
   const Database *db_ = nullptr;
   std::vector<std::unique_ptr<DeviceChunk> > chunks_;
   hipStream_t stream_ = nullptr;         // uploads and the seed stage
-  hipStream_t extend_stream_ = nullptr;  // gapped extension and result build
+  hipStream_t extend_stream_ = nullptr;  // gapped extension
+  // result build (tracebacks): its own stream, since the batched form runs
+  // it on another thread while the extension stream may be capturing a graph
+  hipStream_t traceback_stream_ = nullptr;
   int *d_matrix_ = nullptr;
   uint8_t *d_reduced_ = nullptr;
   SeedStage seed_;
@@ -94,6 +124,20 @@ class Aligner : public PresearchDevice {
   uint32_t max_query_length_ = 0;
   // pinned job buffers per pipeline slot
   std::vector<std::pair<void *, size_t> > pinned_jobs_;
+  // the learned schedule's gate (query side residues) and the pinned counts
+  // of gated jobs read back per call
+  uint32_t learned_gate_ = 44;
+  GappedTuner tuner_;
+  std::mutex tuner_mutex_;
+  hipEvent_t tune_ev_[2 * kLengthBuckets] = {};
+  // per (bucket, configuration): the executable graph of its launch
+  // sequence, updated with each call's arguments
+  bool use_graphs_ = true;
+  hipGraphExec_t graph_exec_[kLengthBuckets * GappedTuner::kConfigs] = {};
+  uint64_t graph_updates_ = 0, graph_instantiations_ = 0;
+  hipError_t LaunchCaptured(int slot, hipStream_t s, const std::function<bool()> &launch);
+  uint32_t *gated_counts_ = nullptr;
+  uint32_t *GatedCountsHost();
 };
 
 }  // namespace gz

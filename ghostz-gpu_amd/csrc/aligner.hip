@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <functional>
+#include <mutex>
 
 #include "aligner.h"
 #include "cost_model_gpu.h"
@@ -22,7 +24,7 @@ namespace {
 
 template <class T>
 hipError_t Upload(T **d, const std::vector<T> &h, hipStream_t s) {
-  hipError_t e = hipMalloc((void **)d, std::max<size_t>(h.size() * sizeof(T), 16));
+  hipError_t e = gz::Malloc((void **)d, std::max<size_t>(h.size() * sizeof(T), 16));
   if (e != hipSuccess) return e;
   if (!h.empty()) e = hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
   return e;
@@ -31,16 +33,16 @@ hipError_t Upload(T **d, const std::vector<T> &h, hipStream_t s) {
 // packs host codes through a temporary device copy
 hipError_t UploadPacked(uint64_t **d, const std::vector<uint8_t> &codes, hipStream_t s) {
   uint8_t *tmp = nullptr;
-  hipError_t e = hipMalloc((void **)&tmp, std::max<size_t>(codes.size(), 16));
+  hipError_t e = gz::Malloc((void **)&tmp, std::max<size_t>(codes.size(), 16));
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(tmp, codes.data(), codes.size(), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMalloc((void **)d, PackedWords(codes.size()) * 8);
+  if (e == hipSuccess) e = gz::Malloc((void **)d, PackedWords(codes.size()) * 8);
   if (e == hipSuccess) {
     gz_stream_s wrap{s};
     if (gz_pack_sequence_async(tmp, codes.size(), *d, &wrap) != GZ_OK) e = hipErrorUnknown;
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(tmp);
+  (void)gz::Free(tmp);
   return e;
 }
 
@@ -60,32 +62,49 @@ struct Aligner::DeviceChunk {
   ~DeviceChunk() {
     void *ps[] = {packed, hashes, rep_off, rep_pos, mem_off, mem_pos, val_off, val_pos};
     for (void *p : ps)
-      if (p) (void)hipFree(p);
+      if (p) (void)gz::Free(p);
   }
 };
 
-Aligner::Aligner(const AlignerOptions &opt) : opt_(opt) {}
+Aligner::Aligner(const AlignerOptions &opt) : opt_(opt) {
+  // jobs whose query side has at least this many residues get a prediction
+  // (GZ_LEARNED_GATE overrides the default, for tuning)
+  const char *g = getenv("GZ_LEARNED_GATE");
+  if (g && *g) learned_gate_ = (uint32_t)strtoul(g, nullptr, 10);
+  // the batched form launches each bucket's gapped sequence as a graph
+  // (GZ_NO_GRAPHS=1 launches the same kernels directly, for comparison)
+  const char *ng = getenv("GZ_NO_GRAPHS");
+  use_graphs_ = !(ng && *ng && *ng != '0');
+}
 
 Aligner::~Aligner() {
   EndQueryChunk();
   chunks_.clear();
-  if (gapped_ws_) (void)hipFree(gapped_ws_);
-  if (tb_ws_) (void)hipFree(tb_ws_);
+  if (gapped_ws_) (void)gz::Free(gapped_ws_);
+  if (tb_ws_) (void)gz::Free(tb_ws_);
+  if (gated_counts_) (void)gz::HostFree(gated_counts_);
   for (auto &b : pinned_jobs_)
-    if (b.first) (void)hipHostFree(b.first);
+    if (b.first) (void)gz::HostFree(b.first);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : tune_ev_)
+    if (e) (void)hipEventDestroy(e);
+  for (hipGraphExec_t x : graph_exec_)
+    if (x) (void)hipGraphExecDestroy(x);
   if (gapped_) gz_gapped_extender_destroy(gapped_);
-  if (d_matrix_) (void)hipFree(d_matrix_);
-  if (d_reduced_) (void)hipFree(d_reduced_);
+  if (d_matrix_) (void)gz::Free(d_matrix_);
+  if (d_reduced_) (void)gz::Free(d_reduced_);
   if (stream_) (void)hipStreamDestroy(stream_);
   if (extend_stream_) (void)hipStreamDestroy(extend_stream_);
+  if (traceback_stream_) (void)hipStreamDestroy(traceback_stream_);
 }
 
 hipError_t Aligner::SetDatabase(const Database *db) {
   db_ = db;
   if (!stream_) GZ_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   if (!extend_stream_) GZ_TRY(hipStreamCreateWithFlags(&extend_stream_, hipStreamNonBlocking));
+  if (!traceback_stream_)
+    GZ_TRY(hipStreamCreateWithFlags(&traceback_stream_, hipStreamNonBlocking));
   chunks_.clear();
   for (const DbChunk &c : db->chunks) {
     std::unique_ptr<DeviceChunk> d(new DeviceChunk());
@@ -103,10 +122,13 @@ hipError_t Aligner::SetDatabase(const Database *db) {
   std::vector<int> m(opt_.search.matrix, opt_.search.matrix + opt_.search.nl * opt_.search.nl);
   if (!d_matrix_) GZ_TRY(Upload(&d_matrix_, m, stream_));
   std::vector<uint8_t> rm(db->reduced_map, db->reduced_map + 25);
-  if (d_reduced_) GZ_TRY(hipFree(d_reduced_));
+  if (d_reduced_) GZ_TRY(gz::Free(d_reduced_));
   GZ_TRY(Upload(&d_reduced_, rm, stream_));
   GZ_TRY(hipStreamSynchronize(stream_));
   if (!gapped_ && gz_gapped_extender_create(&gapped_) != GZ_OK) return hipErrorUnknown;
+  // the batched form runs the first ring as a persistent work-queue launch
+  if (gz_gapped_extender_set_persistent(gapped_, opt_.batched ? 1 : 0) != GZ_OK)
+    return hipErrorUnknown;
   return hipSuccess;
 }
 
@@ -122,8 +144,18 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
                              const std::vector<int> &trig, int presearch_cutoff,
                              AlignStats *st) {
   EndQueryChunk();
-  for (int c : cut)
-    if (c > kMaxUngappedCutoff) return (int)hipErrorInvalidValue;  // beyond the stop score
+  // the seed walk's stop scores serve cutoffs in [1, kMaxUngappedCutoff]
+  // (byte tables: [1, kMaxByteCutoff])
+  bool byte_tables = true;
+  for (int c : cut) {
+    if (c < 1 || c > kMaxUngappedCutoff) return (int)hipErrorInvalidValue;
+    byte_tables = byte_tables && c <= kMaxByteCutoff;
+  }
+  for (uint32_t i = 0; i < opt_.search.nl * opt_.search.nl; ++i) {
+    const int v = opt_.search.matrix[i];
+    if (v < -kMaxUngappedCutoff + 1 || v > kMaxUngappedCutoff - 1) return (int)hipErrorInvalidValue;
+    byte_tables = byte_tables && v >= -127 && v <= 127;
+  }
   max_query_length_ = q.max_length;
   hipStream_t s = stream_;
   uint64_t *d_q = nullptr;
@@ -166,6 +198,7 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
     sp_.distance_threshold = threshold;
   }
   sp_.max_mismatches = db_->max_mismatches;
+  sp_.byte_tables = byte_tables;
 
   if (gz_gapped_extender_set_max_sequence_length(gapped_, q.max_length) != GZ_OK ||
       gz_gapped_extender_set_queries(gapped_, kDelimiter, d_q) != GZ_OK ||
@@ -179,7 +212,7 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
 
 void Aligner::EndQueryChunk() {
   for (void *p : query_buffers_)
-    if (p) (void)hipFree(p);
+    if (p) (void)gz::Free(p);
   query_buffers_.clear();
   dq_ = DeviceQueryLists();
 }
@@ -216,9 +249,24 @@ int Aligner::SeedHits(size_t ci, uint32_t qb, uint32_t qe, SeedHitView *hits,
   return 0;
 }
 
+int GappedTuner::Pick(int bucket) const {
+  for (int c = 0; c < kConfigs; ++c)
+    if (trials[bucket][c] < kTrials) return c;
+  int best = 0;
+  for (int c = 1; c < kConfigs; ++c)
+    if (ms[bucket][c] * jobs[bucket][best] < ms[bucket][best] * jobs[bucket][c]) best = c;
+  return best;
+}
+
+void GappedTuner::Record(int bucket, int config, double kernel_ms, double njobs) {
+  ms[bucket][config] += kernel_ms;
+  jobs[bucket][config] += njobs;
+  ++trials[bucket][config];
+}
+
 int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, uint32_t *dpos,
                           int *scores, uint32_t *cells, const uint16_t *keys, bool learned,
-                          AlignStats *st) {
+                          AlignStats *st, const uint32_t *bucket_off) {
   const size_t n = (size_t)nr + nf;
   if (n == 0) return 0;
   hipStream_t s = extend_stream_;
@@ -233,13 +281,17 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
   // the permutation, predicted costs and the sort scratch (the extension
   // kernels read the jobs through the permutation)
   const size_t a4 = AlignUp(n * 4, 256), a2 = AlignUp(n * 2, 256);
+  // the learned schedule with host lengths: the model orders the gated jobs
+  // inside their length bins (schedule.h)
+  const bool gated = model != nullptr && keys != nullptr;
   size_t bytes = 4 * a4;
   if (order) bytes += a2 + 2 * a4 + AlignUp(ScheduleTempBytes(n), 256);
+  if (gated) bytes += a2 + a4 + 256;
   if (gapped_ws_cap_ < bytes) {
-    if (gapped_ws_) (void)hipFree(gapped_ws_);
+    if (gapped_ws_) (void)gz::Free(gapped_ws_);
     gapped_ws_ = nullptr;
     gapped_ws_cap_ = 0;
-    GZ_TRY(hipMalloc(&gapped_ws_, bytes));
+    GZ_TRY(gz::Malloc(&gapped_ws_, bytes));
     gapped_ws_cap_ = bytes;
   }
   char *w = (char *)gapped_ws_;
@@ -261,13 +313,43 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
     d_pred = (float *)take(a4);
     d_tmp = take(AlignUp(ScheduleTempBytes(n), 256));
   }
+  uint16_t *d_len = nullptr;
+  uint32_t *d_index = nullptr, *d_count = nullptr;
+  if (gated) {
+    d_len = (uint16_t *)take(a2);
+    d_index = (uint32_t *)take(a4);
+    d_count = (uint32_t *)take(256);
+  }
   hipError_t e = hipMemcpyAsync(d_pq, qpos, n * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_pd, dpos, n * 4, hipMemcpyHostToDevice, s);
   uint32_t *rc = cells ? d_cells : nullptr;
+  bool gated_counted = false;
   if (e == hipSuccess && order) {
     // order the jobs by descending key inside each side, on the device
     e = hipEventRecord(ev_[2], s);
-    if (e == hipSuccess && model) {
+    if (e == hipSuccess && gated) {
+      // length bins for every job, the model's prediction bins for the
+      // jobs whose side may run long
+      e = hipMemcpyAsync(d_len, keys, n * 2, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) e = LengthBinKeys(d_len, (uint32_t)n, d_keys, s);
+      // the gated counts stay on the device: the launches are sized for
+      // every job of a side and read the count (no wait in the stream)
+      const uint32_t off[2] = {0, nr}, cnt[2] = {nr, nf};
+      for (int k = 0; k < 2 && e == hipSuccess; ++k)
+        e = GateJobs(d_len + off[k], cnt[k], learned_gate_, d_index + off[k], d_count + k, s);
+      for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = model->PredictIndexedOnDevice(dq_.packed, dc.packed, d_pq + off[k], d_pd + off[k],
+                                          d_index + off[k], cnt[k], k == 0, d_pred + off[k], s,
+                                          d_count + k);
+        if (e == hipSuccess)
+          e = AddPredictionBins(d_pred + off[k], d_index + off[k], cnt[k], d_keys + off[k], s,
+                                d_count + k);
+      }
+      uint32_t *hc = GatedCountsHost();
+      if (e == hipSuccess && !hc) e = hipErrorOutOfMemory;
+      if (e == hipSuccess) e = hipMemcpyAsync(hc, d_count, 8, hipMemcpyDeviceToHost, s);
+      gated_counted = true;
+    } else if (e == hipSuccess && model) {
       // the host lays out job nr + k as the forward side of job k (both
       // positions one later), so one feature pass serves both sides; were
       // that not so, only the order (never a result) would differ
@@ -284,8 +366,19 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
     } else if (e == hipSuccess) {
       e = hipMemcpyAsync(d_keys, keys, n * 2, hipMemcpyHostToDevice, s);
     }
-    if (e == hipSuccess) e = DeviceOrderByKey(d_keys, nr, d_perm, d_tmp, s);
-    if (e == hipSuccess) e = DeviceOrderByKey(d_keys + nr, nf, d_perm + nr, d_tmp, s);
+    if (bucket_off) {
+      // each side's buckets are ordered separately (positions in the
+      // permutation are bucket-relative)
+      for (int side = 0; side < 2 && e == hipSuccess; ++side)
+        for (int b = 0; b < kLengthBuckets && e == hipSuccess; ++b) {
+          const size_t o = (side ? nr : 0) + bucket_off[b];
+          e = DeviceOrderByKey(d_keys + o, bucket_off[b + 1] - bucket_off[b], d_perm + o, d_tmp,
+                               s);
+        }
+    } else {
+      if (e == hipSuccess) e = DeviceOrderByKey(d_keys, nr, d_perm, d_tmp, s);
+      if (e == hipSuccess) e = DeviceOrderByKey(d_keys + nr, nf, d_perm + nr, d_tmp, s);
+    }
     if (e == hipSuccess) e = hipEventRecord(ev_[3], s);
   }
   // each side's jobs run in the permutation's order (when ordered) and
@@ -302,20 +395,66 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
     return r == GZ_OK;
   };
   if (e == hipSuccess) e = hipEventRecord(ev_[0], s);
-  if (e == hipSuccess && nr > 0 && !side(0, nr, 1)) e = hipErrorUnknown;
-  if (e == hipSuccess && nf > 0 && !side(nr, nf, 0)) e = hipErrorUnknown;
+  int tuned[kLengthBuckets] = {-1, -1, -1, -1};
+  if (bucket_off && nr == nf && e == hipSuccess) {
+    // the workspace for the largest bucket, allocated before any capture
+    uint32_t most = 0;
+    for (int b = 0; b < kLengthBuckets; ++b) most = std::max(most, bucket_off[b + 1] - bucket_off[b]);
+    if (gz_gapped_extender_reserve(gapped_, most, &wrap) != GZ_OK) e = hipErrorUnknown;
+  }
+  if (bucket_off && nr == nf) {
+    // every query-length bucket with the configuration the tuner picks for
+    // it (first ring width, persistent work queue or static grid), each
+    // bucket timed on its own
+    for (int b = 0; b < kLengthBuckets && e == hipSuccess; ++b) {
+      const uint32_t o = bucket_off[b], c = bucket_off[b + 1] - o;
+      if (c == 0) continue;
+      if (!tune_ev_[2 * b]) {
+        e = hipEventCreate(&tune_ev_[2 * b]);
+        if (e == hipSuccess) e = hipEventCreate(&tune_ev_[2 * b + 1]);
+        if (e != hipSuccess) break;
+      }
+      std::lock_guard<std::mutex> lock(tuner_mutex_);
+      tuned[b] = tuner_.Pick(b);
+      if (gz_gapped_extender_set_narrow_ring(gapped_, GappedTuner::Narrow(tuned[b])) != GZ_OK ||
+          gz_gapped_extender_set_persistent(gapped_, GappedTuner::Persistent(tuned[b])) != GZ_OK)
+        e = hipErrorUnknown;
+      if (e == hipSuccess) e = hipEventRecord(tune_ev_[2 * b], s);
+      // the bucket's launch sequence (both sides: counter resets, the
+      // first ring, the 64-column ring, the global ring) as one graph
+      auto both = [&]() { return side(o, c, 1) && side(nr + o, c, 0); };
+      if (e == hipSuccess) {
+        if (use_graphs_) e = LaunchCaptured(b * GappedTuner::kConfigs + tuned[b], s, both);
+        else if (!both()) e = hipErrorUnknown;
+      }
+      if (e == hipSuccess) e = hipEventRecord(tune_ev_[2 * b + 1], s);
+    }
+  } else {
+    if (e == hipSuccess && nr > 0 && !side(0, nr, 1)) e = hipErrorUnknown;
+    if (e == hipSuccess && nf > 0 && !side(nr, nf, 0)) e = hipErrorUnknown;
+  }
   if (e == hipSuccess) e = hipEventRecord(ev_[1], s);
   if (e == hipSuccess) e = hipMemcpyAsync(qpos, d_pq, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(dpos, d_pd, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(scores, d_sc, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess && cells) e = hipMemcpyAsync(cells, d_cells, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && gated_counted) st->gated_jobs += (uint64_t)gated_counts_[0] + gated_counts_[1];
   uint32_t fails = 0;
   if (e == hipSuccess && (gz_gapped_extender_failures(gapped_, &fails) != GZ_OK || fails != 0))
     e = hipErrorUnknown;
   float ms = 0, ms2 = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[0], ev_[1]);
   if (e == hipSuccess && order) e = hipEventElapsedTime(&ms2, ev_[2], ev_[3]);
+  for (int b = 0; b < kLengthBuckets && e == hipSuccess; ++b) {
+    if (tuned[b] < 0) continue;
+    float bm = 0;
+    e = hipEventElapsedTime(&bm, tune_ev_[2 * b], tune_ev_[2 * b + 1]);
+    if (e == hipSuccess) {
+      std::lock_guard<std::mutex> lock(tuner_mutex_);
+      tuner_.Record(b, tuned[b], bm, 2.0 * (bucket_off[b + 1] - bucket_off[b]));
+    }
+  }
   if (e != hipSuccess) return (int)e;
   st->gapped_kernel_ms += ms;
   st->schedule_kernel_ms += ms2;
@@ -328,7 +467,7 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   const size_t n = jobs.size();
   out->resize(n);
   if (n == 0) return 0;
-  hipStream_t s = extend_stream_;
+  hipStream_t s = traceback_stream_;
   for (int i = 4; i < 6; ++i)
     if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
   const LaneCaps caps = TracebackCaps(max_query_length_, db_->max_length);
@@ -341,10 +480,10 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   const size_t a_outs = AlignUp(n * sizeof(TracebackOut), 256);
   const size_t bytes = a_jobs + a_outs + lanes * caps.bytes_per_lane;
   if (tb_ws_cap_ < bytes) {
-    if (tb_ws_) (void)hipFree(tb_ws_);
+    if (tb_ws_) (void)gz::Free(tb_ws_);
     tb_ws_ = nullptr;
     tb_ws_cap_ = 0;
-    GZ_TRY(hipMalloc(&tb_ws_, bytes));
+    GZ_TRY(gz::Malloc(&tb_ws_, bytes));
     tb_ws_cap_ = bytes;
   }
   TracebackJob *d_jobs = (TracebackJob *)tb_ws_;
@@ -368,15 +507,61 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   return 0;
 }
 
+hipError_t Aligner::LaunchCaptured(int slot, hipStream_t s, const std::function<bool()> &launch) {
+  hipGraph_t g = nullptr;
+  bool ok = false;
+  hipError_t ec = hipSuccess;
+  {
+    // no other thread of the library frees or allocates while the capture runs
+    std::lock_guard<std::recursive_mutex> lock(CaptureLock());
+    GZ_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    ok = launch();
+    ec = hipStreamEndCapture(s, &g);
+  }
+  if (!ok || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return ok ? ec : hipErrorUnknown;
+  }
+  // the slot's executable graph takes the new arguments when the launch
+  // sequence has the same shape, else it is instantiated again
+  hipGraphExec_t &x = graph_exec_[slot];
+  hipError_t e = hipSuccess;
+  if (x) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult r = hipGraphExecUpdateError;
+    if (hipGraphExecUpdate(x, g, &bad, &r) != hipSuccess || r != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    } else {
+      ++graph_updates_;
+    }
+  }
+  if (!x) {
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) ++graph_instantiations_;
+  }
+  if (e == hipSuccess) e = hipGraphLaunch(x, s);
+  (void)hipGraphDestroy(g);
+  return e;
+}
+
+uint32_t *Aligner::GatedCountsHost() {
+  if (!gated_counts_ &&
+      gz::HostMalloc((void **)&gated_counts_, 64, hipHostMallocDefault) != hipSuccess)
+    gated_counts_ = nullptr;
+  return gated_counts_;
+}
+
 uint8_t *Aligner::JobBuffer(size_t slot, size_t bytes) {
   if (pinned_jobs_.size() <= slot) pinned_jobs_.resize(slot + 1, std::make_pair(nullptr, 0));
   std::pair<void *, size_t> &b = pinned_jobs_[slot];
   if (b.second < bytes) {
-    if (b.first) (void)hipHostFree(b.first);
+    if (b.first) (void)gz::HostFree(b.first);
     b.first = nullptr;
     b.second = 0;
     const size_t want = bytes + bytes / 4;
-    if (hipHostMalloc(&b.first, want, hipHostMallocDefault) != hipSuccess) {
+    if (gz::HostMalloc(&b.first, want, hipHostMallocDefault) != hipSuccess) {
       b.first = nullptr;
       return PresearchDevice::JobBuffer(slot, bytes);  // pageable fallback
     }

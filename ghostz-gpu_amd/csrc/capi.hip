@@ -18,6 +18,7 @@
 #include "cost_model_gpu.h"
 #include "database.h"
 #include "gz_common.h"
+#include "database_io.h"
 #include "gz_runtime.h"
 #include "gz_scan.h"
 #include "queries.h"
@@ -28,6 +29,11 @@
 #include "traceback.h"
 
 using namespace gz;
+
+std::recursive_mutex &gz::CaptureLock() {
+  static std::recursive_mutex m;
+  return m;
+}
 
 struct gz_aligner_s {
   std::vector<int> matrix;
@@ -194,8 +200,8 @@ gz_status gz_query_lists_build(gz_queries_t h, gz_database_t db, int on_device,
     DeviceQueryLists dl;
     uint64_t entries = 0;
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_codes, std::max<size_t>(concat.size(), 16));
-    if (e == hipSuccess) e = hipMalloc((void **)&d_packed, PackedWords(concat.size()) * 8);
+    if (e == hipSuccess) e = gz::Malloc((void **)&d_codes, std::max<size_t>(concat.size(), 16));
+    if (e == hipSuccess) e = gz::Malloc((void **)&d_packed, PackedWords(concat.size()) * 8);
     if (e == hipSuccess)
       e = hipMemcpyAsync(d_codes, concat.data(), concat.size(), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
@@ -235,8 +241,8 @@ gz_status gz_query_lists_build(gz_queries_t h, gz_database_t db, int on_device,
     out->seconds = ms / 1e3;
     for (hipEvent_t x : ev)
       if (x) (void)hipEventDestroy(x);
-    if (d_codes) (void)hipFree(d_codes);
-    if (d_packed) (void)hipFree(d_packed);
+    if (d_codes) (void)gz::Free(d_codes);
+    if (d_packed) (void)gz::Free(d_packed);
     if (s) (void)hipStreamDestroy(s);
     if (e != hipSuccess) {
       SetError(std::string("gz_query_lists_build: ") + hipGetErrorString(e));
@@ -310,6 +316,29 @@ gz_status gz_database_read_fasta(const char *path, const gz_database_options *o,
   return GZ_OK;
 }
 
+gz_status gz_database_load(const char *prefix, gz_database_t *out) {
+  GZ_REQUIRE(prefix && out, "gz_database_load: null argument");
+  std::unique_ptr<gz_database_s> h(new gz_database_s());
+  std::string err;
+  if (!LoadDatabase(prefix, &h->db, &err)) {
+    SetError("gz_database_load: " + err);
+    return GZ_ERROR_IO;
+  }
+  FinishDatabase(h.get());
+  *out = h.release();
+  return GZ_OK;
+}
+
+gz_status gz_database_save(gz_database_t h, const char *prefix) {
+  GZ_REQUIRE(h && prefix, "gz_database_save: null argument");
+  std::string err;
+  if (!SaveDatabase(h->db, prefix, &err)) {
+    SetError("gz_database_save: " + err);
+    return GZ_ERROR_IO;
+  }
+  return GZ_OK;
+}
+
 gz_status gz_database_destroy(gz_database_t h) {
   delete h;
   return GZ_OK;
@@ -376,7 +405,7 @@ gz_status gz_build_results(const uint64_t *pq, const uint64_t *pd, const gz_trac
   void *ws = nullptr;
   const size_t a_jobs = AlignUp(n * sizeof(TracebackJob), 256);
   const size_t a_outs = AlignUp(n * sizeof(TracebackOut), 256);
-  GZ_HIP_CHECK(hipMalloc(&ws, a_jobs + a_outs + lanes * caps.bytes_per_lane));
+  GZ_HIP_CHECK(gz::Malloc(&ws, a_jobs + a_outs + lanes * caps.bytes_per_lane));
   hipError_t e = hipMemcpyAsync(ws, jobs, n * sizeof(TracebackJob), hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
     e = TracebackLaunch(pq, pd, (const TracebackJob *)ws, (uint32_t)n, d_matrix, nl, gap_open,
@@ -386,7 +415,7 @@ gz_status gz_build_results(const uint64_t *pq, const uint64_t *pd, const gz_trac
     e = hipMemcpyAsync(results, (char *)ws + a_jobs, n * sizeof(TracebackOut),
                        hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(ws);
+  (void)gz::Free(ws);
   GZ_HIP_CHECK(e);
   return GZ_OK;
 }
@@ -464,6 +493,31 @@ gz_status gz_aligner_set_database(gz_aligner_t h, gz_database_t db) {
   return GZ_OK;
 }
 
+gz_status gz_aligner_graph_counts(gz_aligner_t h, uint64_t *instantiations,
+                                  uint64_t *updates) {
+  GZ_REQUIRE(h && instantiations && updates, "gz_aligner_graph_counts: null argument");
+  h->aligner->GraphCounts(instantiations, updates);
+  return GZ_OK;
+}
+
+gz_status gz_aligner_tuner_state(gz_aligner_t h, int *chosen, double *ms_per_job,
+                                 uint32_t *trials) {
+  GZ_REQUIRE(h, "gz_aligner_tuner_state: null argument");
+  const GappedTuner t = h->aligner->Tuner();
+  for (int b = 0; b < kLengthBuckets; ++b) {
+    bool tried = false;
+    for (int c = 0; c < GappedTuner::kConfigs; ++c) {
+      tried = tried || t.trials[b][c] > 0;
+      if (ms_per_job)
+        ms_per_job[b * GappedTuner::kConfigs + c] =
+            t.jobs[b][c] > 0 ? t.ms[b][c] / t.jobs[b][c] : 0.0;
+      if (trials) trials[b * GappedTuner::kConfigs + c] = (uint32_t)t.trials[b][c];
+    }
+    if (chosen) chosen[b] = tried ? t.Pick(b) : -1;
+  }
+  return GZ_OK;
+}
+
 gz_status gz_aligner_set_cost_model(gz_aligner_t h, const float *weights, size_t count) {
   GZ_REQUIRE(h && weights, "gz_aligner_set_cost_model: null argument");
   CostModelWeights w;
@@ -522,6 +576,7 @@ gz_status gz_aligner_align(gz_aligner_t h, gz_queries_t q, char **output,
     stats->schedule_kernel_ms = st.schedule_kernel_ms;
     stats->traceback_kernel_ms = st.traceback_kernel_ms;
     stats->traceback_host_jobs = st.traceback_host_jobs;
+    stats->gated_jobs = st.gated_jobs;
   }
   return GZ_OK;
 }
@@ -549,16 +604,14 @@ gz_status gz_aligner_last_concatenated_queries(gz_aligner_t h, const uint8_t **c
   return GZ_OK;
 }
 
-gz_status gz_align_fasta(const char *queries_path, const char *database_path,
-                         const char *output_path, int translated,
-                         const gz_aligner_parameters *p,
-                         const gz_database_options *dbo) {
-  GZ_REQUIRE(queries_path && database_path && output_path, "gz_align_fasta: null argument");
+namespace {
+
+// Aligns a query FASTA against a database (either source) into a file.
+gz_status AlignInto(const char *queries_path, const char *output_path, int translated,
+                    const gz_aligner_parameters *p, gz_database_t db, gz_status st) {
   gz_query_options qo = {translated, 1, 0};
   gz_queries_t q = nullptr;
-  gz_database_t db = nullptr;
   gz_aligner_t a = nullptr;
-  gz_status st = gz_database_read_fasta(database_path, dbo, &db);
   if (st == GZ_OK) st = gz_queries_read_fasta(queries_path, &qo, &q);
   if (st == GZ_OK) st = gz_aligner_create(p, &a);
   if (st == GZ_OK) st = gz_aligner_set_database(a, db);
@@ -579,6 +632,38 @@ gz_status gz_align_fasta(const char *queries_path, const char *database_path,
   gz_queries_destroy(q);
   gz_database_destroy(db);
   return st;
+}
+
+}  // namespace
+
+gz_status gz_align_fasta(const char *queries_path, const char *database_path,
+                         const char *output_path, int translated,
+                         const gz_aligner_parameters *p,
+                         const gz_database_options *dbo) {
+  GZ_REQUIRE(queries_path && database_path && output_path, "gz_align_fasta: null argument");
+  gz_database_t db = nullptr;
+  const gz_status st = gz_database_read_fasta(database_path, dbo, &db);
+  return AlignInto(queries_path, output_path, translated, p, db, st);
+}
+
+gz_status gz_align_main(const char *queries_path, const char *database_prefix,
+                        const char *output_path, int translated,
+                        const gz_aligner_parameters *p) {
+  GZ_REQUIRE(queries_path && database_prefix && output_path, "gz_align_main: null argument");
+  // AlignMain::Run (src/align_main.cpp:46-62): both inputs must exist
+  std::ifstream qi(queries_path);
+  if (!qi) {
+    SetError("Query file does not exist.");
+    return GZ_ERROR_IO;
+  }
+  std::ifstream di((std::string(database_prefix) + ".inf").c_str());
+  if (!di) {
+    SetError("Database file does not exist.");
+    return GZ_ERROR_IO;
+  }
+  gz_database_t db = nullptr;
+  const gz_status st = gz_database_load(database_prefix, &db);
+  return AlignInto(queries_path, output_path, translated, p, db, st);
 }
 
 void gz_free(void *p) {

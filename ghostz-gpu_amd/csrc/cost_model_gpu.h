@@ -15,9 +15,13 @@ size_t FlatWeightCount();
 
 // Feature codes of n one-sided jobs straight from the packed sequences (the
 // same codes CostFeatures produces from unpacked ones): out is n x 128.
+// With `index`, row i holds job index[i] of the position arrays; with
+// d_count (device), only rows below *d_count - count_offset are written.
 hipError_t CostFeaturesOnDevice(const uint64_t *packed_queries, const uint64_t *packed_database,
                                 const uint32_t *qpos, const uint32_t *dpos, uint32_t n,
-                                bool reverse, uint8_t *out, hipStream_t stream);
+                                bool reverse, uint8_t *out, hipStream_t stream,
+                                const uint32_t *index = nullptr,
+                                const uint32_t *d_count = nullptr, uint32_t count_offset = 0);
 
 // The device predictions share one feature scratch buffer: a model serves one
 // stream at a time (each aligner owns its own model and calls it from its
@@ -37,6 +41,14 @@ class GpuCostModel : public CostModel {
   hipError_t PredictOnDevice(const uint64_t *packed_queries, const uint64_t *packed_database,
                              const uint32_t *d_qpos, const uint32_t *d_dpos, uint32_t n,
                              bool reverse, float *d_out, hipStream_t stream) const;
+  // Predictions for the jobs d_index[0..m) of one side (out[k] for job
+  // d_index[k]); with d_count (device), for the first *d_count of them only
+  // (m then bounds the count and sizes the launches).
+  hipError_t PredictIndexedOnDevice(const uint64_t *packed_queries,
+                                    const uint64_t *packed_database, const uint32_t *d_qpos,
+                                    const uint32_t *d_dpos, const uint32_t *d_index, uint32_t m,
+                                    bool reverse, float *d_out, hipStream_t stream,
+                                    const uint32_t *d_count = nullptr) const;
   // Both sides of n hits: job k extends backwards from (d_qpos[k], d_dpos[k])
   // and its forward side starts one position later.  The forward side's
   // windows are the backward side's in the other order (along the

@@ -10,6 +10,7 @@
 
 #include "cost_model_gpu.h"
 #include "gz_common.h"
+#include "gz_runtime.h"
 #include "gz_device.h"
 #include "mlp.h"
 #include "ungapped.h"
@@ -78,12 +79,19 @@ __device__ __forceinline__ void FeatureWindow(const uint64_t *packed, uint32_t s
 // window): windows along the extension (query, subject) then away from it
 // (query, subject).
 __global__ void CostFeatureKernel(const uint64_t *pq, const uint64_t *pd, const uint32_t *qpos,
-                                  const uint32_t *dpos, uint32_t n, int reverse, uint8_t *out) {
+                                  const uint32_t *dpos, const uint32_t *index, uint32_t n,
+                                  const uint32_t *count, uint32_t count_offset, int reverse,
+                                  uint8_t *out) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (count) {  // rows beyond *count - count_offset are not computed
+    const uint32_t c = *count;
+    n = c > count_offset ? min(n, c - count_offset) : 0u;
+  }
   if (t >= 4u * n) return;
   const uint32_t i = t >> 2, wi = t & 3u;
+  const uint32_t job = index ? index[i] : i;  // row i holds job index[i]
   const uint64_t *seq = (wi & 1u) ? pd : pq;
-  const uint32_t pos = (wi & 1u) ? dpos[i] : qpos[i];
+  const uint32_t pos = (wi & 1u) ? dpos[job] : qpos[job];
   // along: from pos in the extension's direction; away: from the next
   // position the other way
   const bool along = wi < 2u;
@@ -101,10 +109,11 @@ __global__ void CostFeatureKernel(const uint64_t *pq, const uint64_t *pd, const 
 
 hipError_t CostFeaturesOnDevice(const uint64_t *pq, const uint64_t *pd, const uint32_t *qpos,
                                 const uint32_t *dpos, uint32_t n, bool reverse, uint8_t *out,
-                                hipStream_t s) {
+                                hipStream_t s, const uint32_t *index, const uint32_t *d_count,
+                                uint32_t count_offset) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(CostFeatureKernel, dim3((4 * (size_t)n + 255) / 256), dim3(256), 0, s, pq, pd, qpos,
-                     dpos, n, reverse ? 1 : 0, out);
+  hipLaunchKernelGGL(CostFeatureKernel, dim3((4 * (size_t)n + 255) / 256), dim3(256), 0, s, pq, pd,
+                     qpos, dpos, index, n, d_count, count_offset, reverse ? 1 : 0, out);
   return hipGetLastError();
 }
 
@@ -113,7 +122,7 @@ struct GpuCostModel::Impl {
   uint8_t *codes = nullptr;
   size_t codes_cap = 0;
   ~Impl() {
-    if (codes) (void)hipFree(codes);
+    if (codes) (void)gz::Free(codes);
   }
 };
 
@@ -153,6 +162,33 @@ hipError_t GpuCostModel::PredictBothSidesOnDevice(const uint64_t *pq, const uint
   return Predict2(pq, pd, d_qpos, d_dpos, n, true, d_out_reverse, d_out_forward, s);
 }
 
+hipError_t GpuCostModel::PredictIndexedOnDevice(const uint64_t *pq, const uint64_t *pd,
+                                                const uint32_t *d_qpos, const uint32_t *d_dpos,
+                                                const uint32_t *d_index, uint32_t m,
+                                                bool reverse, float *d_out, hipStream_t s,
+                                                const uint32_t *d_count) const {
+  if (m == 0) return hipSuccess;
+  const size_t chunk = (size_t)1 << 22;
+  const size_t want = std::min<size_t>(m, chunk) * kCostResidues;
+  if (impl_->codes_cap < want) {
+    if (impl_->codes) (void)gz::Free(impl_->codes);
+    impl_->codes = nullptr;
+    impl_->codes_cap = 0;
+    const hipError_t e = gz::Malloc((void **)&impl_->codes, want);
+    if (e != hipSuccess) return e;
+    impl_->codes_cap = want;
+  }
+  for (size_t b = 0; b < m; b += chunk) {
+    const uint32_t k = (uint32_t)std::min<size_t>(chunk, m - b);
+    hipError_t e = CostFeaturesOnDevice(pq, pd, d_qpos, d_dpos, k, reverse, impl_->codes, s,
+                                        d_index + b, d_count, (uint32_t)b);
+    if (e == hipSuccess)
+      e = impl_->mlp.InferDevice(impl_->codes, k, d_out + b, s, 0, d_count, (uint32_t)b);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 // features of n jobs on side `reverse` into d_out; with d_out_other, also the
 // other side's predictions from the same rows, halves swapped
 hipError_t GpuCostModel::Predict2(const uint64_t *pq, const uint64_t *pd,
@@ -163,10 +199,10 @@ hipError_t GpuCostModel::Predict2(const uint64_t *pq, const uint64_t *pd,
   const size_t chunk = (size_t)1 << 22;
   const size_t want = std::min<size_t>(n, chunk) * kCostResidues;
   if (impl_->codes_cap < want) {
-    if (impl_->codes) (void)hipFree(impl_->codes);
+    if (impl_->codes) (void)gz::Free(impl_->codes);
     impl_->codes = nullptr;
     impl_->codes_cap = 0;
-    const hipError_t e = hipMalloc((void **)&impl_->codes, want);
+    const hipError_t e = gz::Malloc((void **)&impl_->codes, want);
     if (e != hipSuccess) return e;
     impl_->codes_cap = want;
   }

@@ -342,6 +342,131 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
   return true;
 }
 
+// GappedOneSide cut into a start and one call per DP row, so that a lane can
+// hold one extension's state across the rounds of a persistent kernel and
+// begin its next extension while the other lanes of its wave are still in
+// theirs.  Begin and Row run exactly the statements of GappedOneSide, in its
+// order; Row does one pass of its row loop.
+enum : int { kStepRunning = 0, kStepDone = 1, kStepTooWide = 2 };
+
+template <bool kReverse, class Ring>
+struct GappedStepper {
+  typename std::conditional<kReverse, ReverseCursor, ForwardCursor>::type dc;
+  uint32_t qpos, dpos, d, c;
+  int array_start, array_end, max_score, max_p0, max_p1, row;
+  uint32_t cells;
+
+  __device__ __forceinline__ uint32_t Code(const GappedParams &P, int col) const {
+    const int inc = kReverse ? -1 : 1;
+    return PackedCodeAt(P.queries, (uint32_t)((int)qpos + (col - 1) * inc));
+  }
+
+  __device__ int Begin(Ring &R, const GappedParams &P, uint32_t q, uint32_t dp) {
+    const int inc = kReverse ? -1 : 1;
+    const int W = (int)R.mask + 1;
+    qpos = q;
+    dpos = dp;
+    max_score = 0;
+    max_p0 = -inc;
+    max_p1 = -inc;
+    cells = 0;
+    row = 0;
+    dc.Init(P.database, dp);
+    d = dc.Next();
+    c = Code(P, 1);
+    if (c == P.delim || d == P.delim) return kStepDone;
+    const int gap_init = P.gap_open + P.gap_ext;
+    int score = -gap_init;
+    array_start = 0;
+    R.Set(0, make_int2(0, -gap_init));
+    int col = 1, ai;
+    for (ai = 1; c != P.delim; ++ai) {
+      if (score < -P.cutoff) break;
+      if (ai >= W) return kStepTooWide;
+      R.Set(ai, make_int2(score, score - gap_init));
+      R.Code(ai) = (uint8_t)c;
+      score -= P.gap_ext;
+      c = Code(P, ++col);
+    }
+    if (c != P.delim) {
+      if (ai >= W) return kStepTooWide;
+      R.Set(ai, make_int2(kInitScore, kInitScore));
+      R.Code(ai) = (uint8_t)c;
+      array_end = ai + 1;
+    } else {
+      array_end = ai;
+    }
+    return kStepRunning;
+  }
+
+  __device__ int Row(Ring &R, const GappedParams &P, const int8_t *sm) {
+    const int inc = kReverse ? -1 : 1;
+    const int W = (int)R.mask + 1;
+    const int gap_init = P.gap_open + P.gap_ext;
+    const int cutoff = P.cutoff;
+    if (d == P.delim) return kStepDone;
+    if (Ring::kShort && max_score > 16384) return kStepTooWide;
+    const int8_t *mrow = sm + d * kTableDim;
+    int2 first = R.Get(array_start);
+    int prev = first.x;
+    first.x = (max_score - first.y > cutoff) ? kInitScore : first.y;
+    int gap_row = first.x - gap_init;
+    first.y -= P.gap_ext;
+    R.Set(array_start, first);
+    int last = array_start;
+    const int as0 = array_start;
+    for (int ai = as0 + 1; ai < array_end; ++ai) {
+      ++cells;
+      int s = prev + mrow[R.Code(ai)];
+      int2 cell = R.Get(ai);
+      prev = cell.x;
+      const int gap_col = cell.y;
+      s = (s < gap_col) ? gap_col : s;
+      s = (s < gap_row) ? gap_row : s;
+      if (max_score - s > cutoff) {
+        if (array_start + 1 == ai) ++array_start;
+        R.Set(ai, make_int2(kInitScore, kInitScore));
+        gap_row = kInitScore;
+      } else {
+        last = ai;
+        if (s > max_score) {
+          max_score = s;
+          max_p0 = (ai - 1) * inc;
+          max_p1 = row * inc;
+        }
+        const int ng = s - gap_init;
+        R.Set(ai, make_int2(s, (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext));
+        gap_row = (ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext;
+      }
+    }
+    if (array_start + 1 == array_end) return kStepDone;
+    int p0col = array_end;
+    if (last < array_end - 1) {
+      array_end = last + 1;
+      c = Code(P, p0col);
+    } else {
+      c = Code(P, p0col);
+      while (gap_row >= (max_score - cutoff) && c != P.delim) {
+        if (array_end - array_start >= W) return kStepTooWide;
+        R.Set(array_end, make_int2(gap_row, gap_row - gap_init));
+        R.Code(array_end) = (uint8_t)c;
+        gap_row -= P.gap_ext;
+        ++array_end;
+        c = Code(P, ++p0col);
+      }
+    }
+    if (c != P.delim) {
+      if (array_end - array_start >= W) return kStepTooWide;
+      if (array_end == p0col) R.Code(array_end) = (uint8_t)c;
+      R.Set(array_end, make_int2(kInitScore, kInitScore));
+      ++array_end;
+    }
+    ++row;
+    d = dc.Next();
+    return kStepRunning;
+  }
+};
+
 constexpr int kGappedBlock = 64;
 
 // Jobs [begin, end) (through `order` when set: slot k runs job order[k],
@@ -399,6 +524,114 @@ __global__ void __launch_bounds__(kGappedBlock)
   }
 }
 
+// The batched form's persistent gapped extension: jobs come from a device
+// work queue (slot k of the queue is job order[k], or k), handed to the
+// waves 64 at a time with one atomic on the queue head; inside a wave each
+// lane takes the wave's next job as soon as its own extension ends, so a
+// wave runs as long as its lanes' summed work rather than its longest job
+// per round.  Extensions whose band outgrows the ring go to the overflow
+// list as in GappedKernel.  *head must be zero at the launch.
+template <bool kReverse, int W, bool kShort>
+__global__ void __launch_bounds__(kGappedBlock)
+    GappedQueueKernel(const uint32_t *order, uint32_t n, uint32_t *head, GappedParams P,
+                      const int *matrix, uint32_t nl, uint32_t *qpos, uint32_t *dpos,
+                      int *scores, uint32_t *cells, uint32_t *overflow_count,
+                      uint32_t *overflow_list, uint32_t *failed) {
+  using Ring = typename std::conditional<kShort, LaneRingLds16, LaneRingLds>::type;
+  using Cell = typename std::conditional<kShort, short2, int2>::type;
+  __shared__ int8_t sm[kTableDim * kTableDim];
+  __shared__ Cell ring_cells[W * kWave];
+  __shared__ uint8_t ring_codes[W * kWave];
+  StageScoreTable(sm, matrix, nl);
+  __syncthreads();
+  Ring R;
+  R.cells = ring_cells;
+  R.codes = ring_codes;
+  R.mask = W - 1;
+  R.lane = threadIdx.x;
+  const int lane = __lane_id();
+  GappedStepper<kReverse, Ring> st;
+  // lane state: 0 wants a job, 1 runs one, 2 the queue is empty
+  int state = 0;
+  uint32_t job = 0;
+  // the wave's current batch of queue slots: [batch, batch + kWave), of
+  // which `used` are handed out (wave-uniform)
+  uint32_t batch = 0, used = kWave;
+  for (;;) {
+    // hand out queue slots to the lanes that want a job
+    uint64_t want = __ballot(state == 0);
+    while (want != 0) {
+      if (used == kWave) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(head, (uint32_t)kWave);
+        batch = __shfl(b, 0, kWave);
+        used = 0;
+      }
+      const uint32_t take = min((uint32_t)__popcll(want), kWave - used);
+      const uint32_t rank = (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+      int r = kStepRunning;
+      bool finished = false;
+      if (((want >> lane) & 1ull) && rank < take) {
+        const uint32_t k = batch + used + rank;
+        if (k >= n) {
+          state = 2;
+        } else {
+          job = order ? order[k] : k;
+          if (job >= n) {  // an order entry out of range: counted, never read
+            atomicAdd(failed, 1u);
+            state = 1;
+            finished = true;
+            r = kStepRunning;
+          } else {
+            r = st.Begin(R, P, qpos[job], dpos[job]);
+            state = 1;
+            finished = r != kStepRunning;
+          }
+        }
+      }
+      used += take;
+      // extensions that end in Begin (a delimiter at the seed, or a first
+      // row wider than the ring) are written now and the lane asks again
+      if (finished && r == kStepDone) {
+        scores[job] = st.max_score;
+        qpos[job] = (uint32_t)((int)st.qpos + st.max_p0);
+        dpos[job] = (uint32_t)((int)st.dpos + st.max_p1);
+        if (cells) cells[job] = st.cells;
+      }
+      const bool over = finished && r == kStepTooWide;
+      const uint64_t om = __ballot(over);
+      if (om != 0) {
+        const int leader = __ffsll((unsigned long long)om) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(overflow_count, (uint32_t)__popcll(om));
+        base = __shfl(base, leader, kWave);
+        if (over) overflow_list[base + (uint32_t)__popcll(om & ((1ull << lane) - 1ull))] = job;
+      }
+      if (finished) state = 0;
+      want = __ballot(state == 0);
+    }
+    if (!__any(state == 1)) break;  // every lane has found the queue empty
+    int r = kStepRunning;
+    if (state == 1) r = st.Row(R, P, sm);
+    if (state == 1 && r == kStepDone) {
+      scores[job] = st.max_score;
+      qpos[job] = (uint32_t)((int)st.qpos + st.max_p0);
+      dpos[job] = (uint32_t)((int)st.dpos + st.max_p1);
+      if (cells) cells[job] = st.cells;
+    }
+    const bool over = state == 1 && r == kStepTooWide;
+    const uint64_t om = __ballot(over);
+    if (om != 0) {
+      const int leader = __ffsll((unsigned long long)om) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(overflow_count, (uint32_t)__popcll(om));
+      base = __shfl(base, leader, kWave);
+      if (over) overflow_list[base + (uint32_t)__popcll(om & ((1ull << lane) - 1ull))] = job;
+    }
+    if (state == 1 && r != kStepRunning) state = 0;
+  }
+}
+
 template <bool kReverse>
 __global__ void __launch_bounds__(kGappedBlock)
     GappedWideKernel(GappedParams P, const int *matrix, uint32_t nl,
@@ -442,6 +675,16 @@ __global__ void ConvertSeedsKernel(size_t size, const uint32_t *seed_positions,
   dout[i] = din[s];
 }
 
+// compute units of the current device
+inline unsigned Cus() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 256;
+  return (unsigned)cus;
+}
+
 inline unsigned GridFor(size_t n, unsigned block, unsigned cap = 65535u * 4) {
   size_t g = (n + block - 1) / block;
   if (g == 0) g = 1;
@@ -468,7 +711,7 @@ gz_status gz_set_device(int device) {
 gz_status gz_device_malloc(void **ptr, size_t bytes) {
   GZ_REQUIRE(ptr != nullptr, "gz_device_malloc: null out pointer");
   if (bytes == 0) bytes = 1;
-  hipError_t e = hipMalloc(ptr, bytes);
+  hipError_t e = gz::Malloc(ptr, bytes);
   if (e != hipSuccess) {
     SetError(std::string("hipMalloc: ") + hipGetErrorString(e));
     return GZ_ERROR_OUT_OF_MEMORY;
@@ -476,13 +719,13 @@ gz_status gz_device_malloc(void **ptr, size_t bytes) {
   return GZ_OK;
 }
 gz_status gz_device_free(void *ptr) {
-  if (ptr) GZ_HIP_CHECK(hipFree(ptr));
+  if (ptr) GZ_HIP_CHECK(gz::Free(ptr));
   return GZ_OK;
 }
 gz_status gz_host_malloc(void **ptr, size_t bytes) {
   GZ_REQUIRE(ptr != nullptr, "gz_host_malloc: null out pointer");
   if (bytes == 0) bytes = 1;
-  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+  hipError_t e = gz::HostMalloc(ptr, bytes, hipHostMallocDefault);
   if (e != hipSuccess) {
     SetError(std::string("hipHostMalloc: ") + hipGetErrorString(e));
     return GZ_ERROR_OUT_OF_MEMORY;
@@ -490,7 +733,7 @@ gz_status gz_host_malloc(void **ptr, size_t bytes) {
   return GZ_OK;
 }
 gz_status gz_host_free(void *ptr) {
-  if (ptr) GZ_HIP_CHECK(hipHostFree(ptr));
+  if (ptr) GZ_HIP_CHECK(gz::HostFree(ptr));
   return GZ_OK;
 }
 gz_status gz_stream_create(gz_stream_t *stream) {
@@ -748,9 +991,10 @@ struct gz_gapped_extender_s {
   int gap_open = 0, gap_ext = 0, cutoff = 0;
   uint32_t wide = 1u << 16;  // wide-ring columns (>= longest query + 2)
   uint32_t narrow = 32;      // columns of the first LDS ring (16 or 32)
+  int persistent = 0;        // first ring: one persistent launch fed from a work queue
   // workspace
   // [0] jobs out of the narrow ring, [1] failures, [2] jobs out of the
-  // 64-column ring
+  // 64-column ring, [3] the work queue's head
   uint32_t *counters = nullptr;
   uint32_t *overflow_list = nullptr;   // out of the narrow ring
   uint32_t *overflow_list2 = nullptr;  // out of the 64-column ring
@@ -762,11 +1006,11 @@ struct gz_gapped_extender_s {
   // allocation-free entry point's scratch on the device it ran on)
   uint32_t *failure_counter = nullptr;
   ~gz_gapped_extender_s() {
-    if (counters) (void)hipFree(counters);
-    if (overflow_list) (void)hipFree(overflow_list);
-    if (overflow_list2) (void)hipFree(overflow_list2);
-    if (wide_cells) (void)hipFree(wide_cells);
-    if (wide_codes) (void)hipFree(wide_codes);
+    if (counters) (void)gz::Free(counters);
+    if (overflow_list) (void)gz::Free(overflow_list);
+    if (overflow_list2) (void)gz::Free(overflow_list2);
+    if (wide_cells) (void)gz::Free(wide_cells);
+    if (wide_codes) (void)gz::Free(wide_codes);
   }
 };
 
@@ -860,7 +1104,29 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedWork &w, const Gapp
     if (short_cells) GZ_RING64(true, grid, 0, wide_first, nullptr, nullptr, order);
     else GZ_RING64(false, grid, 0, wide_first, nullptr, nullptr, order);
   }
-  if (size > wide_first) {
+  if (size > wide_first && h->persistent && wide_first == 0) {
+    // one persistent launch over every job, sized to the waves the device
+    // holds at once
+#define GZ_QUEUE(WIDTH, SHORT)                                                               \
+  {                                                                                          \
+    int per_cu = 1;                                                                          \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                        \
+            &per_cu, (const void *)GappedQueueKernel<kReverse, WIDTH, SHORT>, kGappedBlock,   \
+            0) != hipSuccess || per_cu < 1)                                                  \
+      per_cu = 1;                                                                            \
+    const size_t waves = std::min<size_t>((size_t)per_cu * Cus(), (size - 1) / kWave + 1);   \
+    hipLaunchKernelGGL((GappedQueueKernel<kReverse, WIDTH, SHORT>), dim3((unsigned)waves),    \
+                       dim3(kGappedBlock), 0, s, order, (uint32_t)size, w.counters + 3, P,    \
+                       h->matrix, h->number_letters, d_q, d_d, d_scores, d_cells, narrow_out, \
+                       w.overflow_list, failures);                                           \
+  }
+    if (h->narrow == 16) {
+      if (short_cells) GZ_QUEUE(16, true) else GZ_QUEUE(16, false)
+    } else {
+      if (short_cells) GZ_QUEUE(32, true) else GZ_QUEUE(32, false)
+    }
+#undef GZ_QUEUE
+  } else if (size > wide_first) {
     const dim3 grid(GridFor(size - wide_first, kGappedBlock, 256u * 64));
 #define GZ_NARROW(WIDTH, SHORT)                                                              \
   hipLaunchKernelGGL((GappedKernel<kReverse, WIDTH, SHORT>), grid, dim3(kGappedBlock), 0, s, \
@@ -907,6 +1173,41 @@ static gz_status GappedLaunchAll(gz_gapped_extender_t h, const GappedWork &w, si
   return GZ_OK;
 }
 
+// The run counters a launch sequence starts from: the overflow counts and
+// the queue head are zeroed, the failure count ([1]) is kept until read.  A
+// kernel, not a memset, so that the sequence can be captured in a graph.
+__global__ void ResetRunCountersKernel(uint32_t *counters, int all) {
+  if (threadIdx.x < 4 && (all || threadIdx.x != 1)) counters[threadIdx.x] = 0;
+}
+
+// Allocates the handle's workspace for runs of up to `size` jobs (no-op
+// when it is large enough).
+static gz_status GappedReserve(gz_gapped_extender_t h, size_t size, hipStream_t s) {
+  if (!h->counters) {
+    GZ_HIP_CHECK(gz::Malloc(&h->counters, 4 * sizeof(uint32_t)));
+    hipLaunchKernelGGL(ResetRunCountersKernel, dim3(1), dim3(64), 0, s, h->counters, 1);
+    GZ_HIP_CHECK(hipGetLastError());
+  }
+  if (h->overflow_capacity < size) {
+    if (h->overflow_list) GZ_HIP_CHECK(gz::Free(h->overflow_list));
+    if (h->overflow_list2) GZ_HIP_CHECK(gz::Free(h->overflow_list2));
+    h->overflow_list = h->overflow_list2 = nullptr;
+    h->overflow_capacity = 0;
+    size_t cap = std::max<size_t>(size, 1 << 16);
+    GZ_HIP_CHECK(gz::Malloc(&h->overflow_list, cap * sizeof(uint32_t)));
+    GZ_HIP_CHECK(gz::Malloc(&h->overflow_list2, cap * sizeof(uint32_t)));
+    h->overflow_capacity = cap;
+  }
+  if (h->wide_allocated < h->wide) {
+    if (h->wide_cells) GZ_HIP_CHECK(gz::Free(h->wide_cells));
+    if (h->wide_codes) GZ_HIP_CHECK(gz::Free(h->wide_codes));
+    GZ_HIP_CHECK(gz::Malloc(&h->wide_cells, (size_t)kWideSlots * h->wide * sizeof(int2)));
+    GZ_HIP_CHECK(gz::Malloc(&h->wide_codes, (size_t)kWideSlots * h->wide));
+    h->wide_allocated = h->wide;
+  }
+  return GZ_OK;
+}
+
 static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
                            uint32_t *d_q, uint32_t *d_d, int *d_scores,
                            uint32_t *d_cells, hipStream_t s, size_t wide_first = 0,
@@ -914,31 +1215,13 @@ static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
   GZ_REQUIRE(h->queries && h->database && h->matrix,
              "gapped extender: queries, database and score parameters must be set");
   GZ_REQUIRE(wide_first <= size, "gapped extender: wide_first beyond the jobs");
-  if (!h->counters) {
-    GZ_HIP_CHECK(hipMalloc(&h->counters, 3 * sizeof(uint32_t)));
-    GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, 3 * sizeof(uint32_t), s));
-  }
-  if (h->overflow_capacity < size) {
-    if (h->overflow_list) GZ_HIP_CHECK(hipFree(h->overflow_list));
-    if (h->overflow_list2) GZ_HIP_CHECK(hipFree(h->overflow_list2));
-    h->overflow_list = h->overflow_list2 = nullptr;
-    h->overflow_capacity = 0;
-    size_t cap = std::max<size_t>(size, 1 << 16);
-    GZ_HIP_CHECK(hipMalloc(&h->overflow_list, cap * sizeof(uint32_t)));
-    GZ_HIP_CHECK(hipMalloc(&h->overflow_list2, cap * sizeof(uint32_t)));
-    h->overflow_capacity = cap;
-  }
-  if (h->wide_allocated < h->wide) {
-    if (h->wide_cells) GZ_HIP_CHECK(hipFree(h->wide_cells));
-    if (h->wide_codes) GZ_HIP_CHECK(hipFree(h->wide_codes));
-    GZ_HIP_CHECK(hipMalloc(&h->wide_cells, (size_t)kWideSlots * h->wide * sizeof(int2)));
-    GZ_HIP_CHECK(hipMalloc(&h->wide_codes, (size_t)kWideSlots * h->wide));
-    h->wide_allocated = h->wide;
-  }
+  GZ_REQUIRE(size < (1ull << 31), "gapped extender: at most 2^31 - 1 jobs per call");
+  const gz_status rs = GappedReserve(h, size, s);
+  if (rs != GZ_OK) return rs;
   // the overflow counts of this run start at zero; failures (counters[1])
   // are kept until gz_gapped_extender_failures reads them
-  GZ_HIP_CHECK(hipMemsetAsync(h->counters, 0, sizeof(uint32_t), s));
-  GZ_HIP_CHECK(hipMemsetAsync(h->counters + 2, 0, sizeof(uint32_t), s));
+  hipLaunchKernelGGL(ResetRunCountersKernel, dim3(1), dim3(64), 0, s, h->counters, 0);
+  GZ_HIP_CHECK(hipGetLastError());
   const GappedWork w = {h->counters, h->overflow_list, h->overflow_list2, h->wide_cells,
                         h->wide_codes, h->wide};
   h->failure_counter = h->counters + 1;
@@ -1026,7 +1309,7 @@ gz_status ScratchOfCurrentDevice(uint8_t **out) {
 }
 
 __global__ void ClearCountersKernel(uint32_t *counters) {
-  if (threadIdx.x < 3) counters[threadIdx.x] = 0;
+  if (threadIdx.x < 4) counters[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -1074,6 +1357,17 @@ gz_status gz_gapped_extender_set_max_sequence_length(gz_gapped_extender_t h,
   uint32_t w = 64;
   while (w < length + 2) w <<= 1;
   h->wide = w;
+  return GZ_OK;
+}
+
+gz_status gz_gapped_extender_reserve(gz_gapped_extender_t h, size_t size, gz_stream_t stream) {
+  GZ_REQUIRE(h, "null handle");
+  return GappedReserve(h, size, S(stream));
+}
+
+gz_status gz_gapped_extender_set_persistent(gz_gapped_extender_t h, int persistent) {
+  GZ_REQUIRE(h, "null handle");
+  h->persistent = persistent ? 1 : 0;
   return GZ_OK;
 }
 

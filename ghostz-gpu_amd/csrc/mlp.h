@@ -34,8 +34,11 @@ class MlpDevice {
   // caller's stream (one persistent kernel for all three layers).
   // rotate_words: each 128-code row is read from its 32-bit word
   // rotate_words on, wrapping (16 swaps the row's two halves)
+  // With d_count (device), only the first min(n, *d_count - count_offset)
+  // rows are inferred: n sizes the launch, the device decides the rows.
   hipError_t InferDevice(const uint8_t *d_codes, size_t n, float *d_out, hipStream_t stream,
-                         uint32_t rotate_words = 0);
+                         uint32_t rotate_words = 0, const uint32_t *d_count = nullptr,
+                         uint32_t count_offset = 0);
   // Kernel-only forward time of the last Forward call (HIP events).
   float last_forward_ms = 0;
 

@@ -53,28 +53,73 @@ __device__ __forceinline__ float GeluGrad(float z) {
 
 enum { kEpiStore = 0, kEpiBiasGelu = 1, kEpiGeluGrad = 2 };
 
-// C[m][n] = sum_k A(m, k) * B(n, k), A(m,k) = A[m*sam + k*sak], B likewise.
+// C[m][n] = sum_k A(m, k) * B(n, k), A(m,k) = A[m*sam + k*sak], B likewise;
+// every operand has one unit stride (k for the forward and dX products, the
+// row for the weight gradients).  Blocks of four waves own 64 x 64 tiles of
+// C (32 x 32 per wave); K advances 32 at a time through LDS, where both
+// operand tiles are stored k-contiguous, each filled with 16-byte loads
+// along the operand's unit stride.
+constexpr int kGemmTile = 64, kGemmK = 32, kGemmPad = 8;
+
+__device__ __forceinline__ void GemmLoadTile(uint16_t (*T)[kGemmK + kGemmPad], const uint16_t *X,
+                                             long sr, long sk, int rows, int K, int r0, int k0) {
+  const int t = threadIdx.x;
+  if (sk == 1) {
+    // thread t: row r0 + t / 4, eight k's from k0 + (t % 4) * 8
+    const int r = t >> 2, kk = (t & 3) * 8;
+    const int gr = r0 + r, gk = k0 + kk;
+    const uint16_t *p = X + (long)gr * sr + gk;
+    if (gr < rows && gk + 8 <= K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+      *reinterpret_cast<uint4 *>(&T[r][kk]) = *reinterpret_cast<const uint4 *>(p);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        T[r][kk + j] = (gr < rows && gk + j < K) ? X[(long)gr * sr + gk + j] : (uint16_t)0;
+    }
+  } else {
+    // thread t: k0 + t / 8, eight rows from r0 + (t % 8) * 8 (unit row stride)
+    const int kk = t >> 3, r = (t & 7) * 8;
+    const int gr = r0 + r, gk = k0 + kk;
+    const uint16_t *p = X + gr + (long)gk * sk;
+    uint16_t v[8];
+    if (gk < K && gr + 8 <= rows && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+      const uint4 w = *reinterpret_cast<const uint4 *>(p);
+      memcpy(v, &w, 16);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        v[j] = (gk < K && gr + j < rows) ? X[gr + j + (long)gk * sk] : (uint16_t)0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) T[r + j][kk] = v[j];
+  }
+}
+
 template <int EPI>
 __global__ void __launch_bounds__(256)
     GemmKernel(int M, int N, int K, const uint16_t *A, long sam, long sak,
                const uint16_t *B, long sbn, long sbk, float *C, int ldc,
                const float *bias, uint16_t *Ch, const float *zsaved) {
+  __shared__ __align__(16) uint16_t As[kGemmTile][kGemmK + kGemmPad];
+  __shared__ __align__(16) uint16_t Bs[kGemmTile][kGemmK + kGemmPad];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m0 = blockIdx.y * 64 + (wave >> 1) * 32;
-  const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
-  const int am = m0 + (lane & 31), bn = n0 + (lane & 31);
+  const int mb = blockIdx.y * kGemmTile, nb = blockIdx.x * kGemmTile;
+  const int m0 = mb + (wave >> 1) * 32, n0 = nb + (wave & 1) * 32;
+  const int am = (wave >> 1) * 32 + (lane & 31), bn = (wave & 1) * 32 + (lane & 31);
   const int kq = (lane >> 5) * 4;
   float16_t acc;
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-  for (int k0 = 0; k0 < K; k0 += 8) {
-    short4_t a, b;
+  for (int k0 = 0; k0 < K; k0 += kGemmK) {
+    GemmLoadTile(As, A, sam, sak, M, K, mb, k0);
+    GemmLoadTile(Bs, B, sbn, sbk, N, K, nb, k0);
+    __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + kq + j;
-      a[j] = (am < M && k < K) ? (short)A[am * sam + k * sak] : (short)0;
-      b[j] = (bn < N && k < K) ? (short)B[bn * sbn + k * sbk] : (short)0;
+    for (int ks = 0; ks < kGemmK; ks += 8) {
+      short4_t a = *reinterpret_cast<const short4_t *>(&As[am][ks + kq]);
+      short4_t b = *reinterpret_cast<const short4_t *>(&Bs[bn][ks + kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    __syncthreads();
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -275,9 +320,14 @@ __device__ __forceinline__ void PutPiece(uint16_t *w1s, int i, uint4 v) {
 }
 
 __global__ void __launch_bounds__(kInferWaves * 64) __attribute__((amdgpu_waves_per_eu(2)))
-    InferKernel(const uint8_t *codes, uint32_t n, const uint16_t *w1h, const uint16_t *embh,
-                const float *b1, const uint16_t *w2h, const float *b2, const float *w3,
-                const float *b3, uint32_t rotate, float *y) {
+    InferKernel(const uint8_t *codes, uint32_t n, const uint32_t *count, uint32_t count_offset,
+                const uint16_t *w1h, const uint16_t *embh, const float *b1, const uint16_t *w2h,
+                const float *b2, const float *w3, const float *b3, uint32_t rotate, float *y) {
+  // with `count`, the rows are the first *count - count_offset (at most n)
+  if (count) {
+    const uint32_t c = *count;
+    n = c > count_offset ? min(n, c - count_offset) : 0u;
+  }
   __shared__ __align__(16) uint16_t w1s[2][kStageIn * kCostH1];     // two stages, 32 KB
   __shared__ __align__(16) uint16_t w2[kL2Steps * 2 * 64 * 8];      // [t][row block][lane][8]
   __shared__ __align__(16) uint16_t emb[kCostAlphabet * kCostEmbed];
@@ -409,15 +459,15 @@ void Gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, long sam, long 
 MlpDevice::MlpDevice() {
   count_ = kFlat;
   hipError_t e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc((void **)&master_, kFlat * 4);
-  if (e == hipSuccess) e = hipMalloc((void **)&grad_, kFlat * 4);
-  if (e == hipSuccess) e = hipMalloc((void **)&m_, kFlat * 4);
-  if (e == hipSuccess) e = hipMalloc((void **)&v_, kFlat * 4);
+  if (e == hipSuccess) e = gz::Malloc((void **)&master_, kFlat * 4);
+  if (e == hipSuccess) e = gz::Malloc((void **)&grad_, kFlat * 4);
+  if (e == hipSuccess) e = gz::Malloc((void **)&m_, kFlat * 4);
+  if (e == hipSuccess) e = gz::Malloc((void **)&v_, kFlat * 4);
   if (e == hipSuccess) e = hipMemset(m_, 0, kFlat * 4);
   if (e == hipSuccess) e = hipMemset(v_, 0, kFlat * 4);
-  if (e == hipSuccess) e = hipMalloc((void **)&w1h_, (size_t)kCostH1 * kCostIn * 2);
-  if (e == hipSuccess) e = hipMalloc((void **)&w2h_, (size_t)kCostH2 * kCostH1 * 2);
-  if (e == hipSuccess) e = hipMalloc((void **)&embh_, (size_t)kCostAlphabet * kCostEmbed * 2);
+  if (e == hipSuccess) e = gz::Malloc((void **)&w1h_, (size_t)kCostH1 * kCostIn * 2);
+  if (e == hipSuccess) e = gz::Malloc((void **)&w2h_, (size_t)kCostH2 * kCostH1 * 2);
+  if (e == hipSuccess) e = gz::Malloc((void **)&embh_, (size_t)kCostAlphabet * kCostEmbed * 2);
   init_status_ = e;
 }
 
@@ -426,7 +476,7 @@ MlpDevice::~MlpDevice() {
                 z1_,     h1_,   z2_,  h2_,   y_,    t_,    loss_, dz2_,   dz2h_,
                 dz1_,    dz1h_, dh1_, dx_};
   for (void *p : ps)
-    if (p) (void)hipFree(p);
+    if (p) (void)gz::Free(p);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -490,13 +540,13 @@ hipError_t MlpDevice::Reserve(size_t batch) {
                  (void **)&dz1h_,  (void **)&dh1_, (void **)&dx_};
   for (void **p : ps)
     if (*p) {
-      (void)hipFree(*p);
+      (void)gz::Free(*p);
       *p = nullptr;
     }
   const size_t B = batch;
   hipError_t e = hipSuccess;
   auto al = [&](void **p, size_t bytes) {
-    if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 64));
+    if (e == hipSuccess) e = gz::Malloc(p, std::max<size_t>(bytes, 64));
   };
   al((void **)&codes_, B * kCostResidues);
   al((void **)&x_, B * kCostIn * 2);
@@ -532,7 +582,8 @@ hipError_t MlpDevice::ForwardDevice(size_t n, size_t B) {
 }
 
 hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out,
-                                  hipStream_t s, uint32_t rotate_words) {
+                                  hipStream_t s, uint32_t rotate_words, const uint32_t *d_count,
+                                  uint32_t count_offset) {
   if (init_status_ != hipSuccess) return init_status_;
   if (n == 0) return hipSuccess;
   if (n > 0xFFFFFFFFull - kInferTile) return hipErrorInvalidValue;
@@ -541,8 +592,8 @@ hipError_t MlpDevice::InferDevice(const uint8_t *d_codes, size_t n, float *d_out
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const uint32_t tiles = (uint32_t)((n + kInferTile - 1) / kInferTile);
   hipLaunchKernelGGL(InferKernel, dim3(std::min<unsigned>(tiles, 2u * (unsigned)cus)),
-                     dim3(kInferWaves * 64), 0, s, d_codes, (uint32_t)n, w1h_, embh_,
-                     master_ + kOffB1, w2h_, master_ + kOffB2, master_ + kOffW3,
+                     dim3(kInferWaves * 64), 0, s, d_codes, (uint32_t)n, d_count, count_offset,
+                     w1h_, embh_, master_ + kOffB1, w2h_, master_ + kOffB2, master_ + kOffW3,
                      master_ + kOffB3, rotate_words & (kStages - 1), d_out);
   return hipGetLastError();
 }
@@ -553,8 +604,8 @@ hipError_t MlpDevice::Forward(const uint8_t *codes, size_t n, float *out, uint32
   uint8_t *d_codes = nullptr;
   float *d_out = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipError_t e = hipMalloc((void **)&d_codes, n * kCostResidues);
-  if (e == hipSuccess) e = hipMalloc((void **)&d_out, n * 4);
+  hipError_t e = gz::Malloc((void **)&d_codes, n * kCostResidues);
+  if (e == hipSuccess) e = gz::Malloc((void **)&d_out, n * 4);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e == hipSuccess)
@@ -568,8 +619,8 @@ hipError_t MlpDevice::Forward(const uint8_t *codes, size_t n, float *out, uint32
   if (e == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) last_forward_ms = ms;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  if (d_codes) (void)hipFree(d_codes);
-  if (d_out) (void)hipFree(d_out);
+  if (d_codes) (void)gz::Free(d_codes);
+  if (d_out) (void)gz::Free(d_out);
   return e;
 }
 

@@ -204,7 +204,8 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   // 3. gapped extension of both sides of the chained hits of [qb, qe)
   struct Block {
     size_t qb = 0, qe = 0, slot = 0;
-    std::vector<uint32_t> gstart;
+    std::vector<uint32_t> gstart;  // per query: its first job; [n] = all jobs
+    uint32_t bucket_off[kLengthBuckets + 1] = {0, 0, 0, 0, 0};
     // jobs [0, G) reverse sides, [G, 2G) forward sides: positions in,
     // extension ends and scores out
     uint32_t *aq = nullptr, *ad = nullptr;
@@ -214,8 +215,27 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   auto gapped = [&](size_t ci, Block &B) -> int {
     const size_t n = B.qe - B.qb;
     B.gstart.assign(n + 1, 0);
-    for (size_t k = 0; k < n; ++k)
-      B.gstart[k + 1] = B.gstart[k] + (uint32_t)chained[B.qb + k].size();
+    if (opt.batched) {
+      // the batched form lays each side's jobs out by query-length bucket
+      // (queries in order inside a bucket), so that the device can run every
+      // bucket with its own tuned configuration
+      uint32_t count[kLengthBuckets] = {0, 0, 0, 0};
+      for (size_t k = 0; k < n; ++k)
+        count[LengthBucket(q.Length(B.qb + k))] += (uint32_t)chained[B.qb + k].size();
+      for (int b = 0; b < kLengthBuckets; ++b) B.bucket_off[b + 1] = B.bucket_off[b] + count[b];
+      uint32_t cursor[kLengthBuckets];
+      for (int b = 0; b < kLengthBuckets; ++b) cursor[b] = B.bucket_off[b];
+      for (size_t k = 0; k < n; ++k) {
+        const int b = LengthBucket(q.Length(B.qb + k));
+        B.gstart[k] = cursor[b];
+        cursor[b] += (uint32_t)chained[B.qb + k].size();
+      }
+      B.gstart[n] = B.bucket_off[kLengthBuckets];
+    } else {
+      for (size_t k = 0; k < n; ++k)
+        B.gstart[k + 1] = B.gstart[k] + (uint32_t)chained[B.qb + k].size();
+      for (int b = 1; b <= kLengthBuckets; ++b) B.bucket_off[b] = B.gstart[n];
+    }
     const uint32_t G = B.gstart[n];
     B.G = G;
     if (G == 0) return 0;
@@ -224,8 +244,11 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     B.aq = reinterpret_cast<uint32_t *>(buf);
     B.ad = B.aq + jobs;
     B.sc = reinterpret_cast<int *>(B.ad + jobs);
+    // query side lengths: the length-sorted schedule's keys, and the length
+    // bins of the learned schedule
     std::vector<uint16_t> keys;
-    if (opt.schedule == Schedule::kLengthSorted) keys.resize(jobs);
+    if (opt.schedule == Schedule::kLengthSorted || opt.schedule == Schedule::kLearned)
+      keys.resize(jobs);
     ParallelBlocks(n, opt.batched ? std::max(1, threads / 4) : threads, 256,
                    [&](size_t b, size_t e) {
       for (size_t k = b; k < e; ++k) {
@@ -252,7 +275,8 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     std::vector<uint32_t> cells(opt.record_cells ? jobs : 0);
     const int r = dev->GappedExtend(ci, G, G, B.aq, B.ad, B.sc,
                                     cells.empty() ? nullptr : cells.data(),
-                                    keys.empty() ? nullptr : keys.data(), learned, st);
+                                    keys.empty() ? nullptr : keys.data(), learned, st,
+                                    opt.batched ? B.bucket_off : nullptr);
     if (r != 0) return r;
     if (!cells.empty()) {
       for (int side = 0; side < 2; ++side)
@@ -284,8 +308,9 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       for (size_t k = b; k < e; ++k) {
         const size_t qi = B.qb + k;
         tmp.clear();
-        for (uint32_t g = B.gstart[k]; g < B.gstart[k + 1]; ++g) {
-          const Hit &h = chained[qi][g - B.gstart[k]];
+        for (uint32_t j = 0; j < chained[qi].size(); ++j) {
+          const uint32_t g = B.gstart[k] + j;
+          const Hit &h = chained[qi][j];
           Presearched r;
           r.chunk = -1;
           r.subject = 0;
@@ -487,145 +512,152 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   double sec_seed = 0, sec_gapped = 0, sec_chain = 0, sec_add = 0, sec_build = 0,
          sec_build_host = 0, sec_output = 0;
   uint64_t seed_hits = 0, host_jobs = 0;
-  for (size_t ci = 0; ci < db.chunks.size(); ++ci) {
-    const DbChunk &chunk = db.chunks[ci];
-    const bool last = ci + 1 == db.chunks.size();
-    std::vector<SeedHitView> views(nb);
-    std::mutex mu;
-    std::condition_variable cv;
-    size_t seeded = 0, chained_upto = 0, gapped_upto = 0, added = 0, traced = 0;
-    int status = 0;
-    bool unsupported = false;
-    auto wait_for = [&](const size_t &counter, size_t value) {
-      std::unique_lock<std::mutex> lock(mu);
-      cv.wait(lock, [&] { return counter >= value || status != 0; });
-      return status == 0;
-    };
-    auto publish = [&](size_t &counter, size_t value, int r) {
-      {
-        std::lock_guard<std::mutex> lock(mu);
-        counter = value;
-        if (r != 0 && status == 0) status = r;
-      }
-      cv.notify_all();
-    };
-    // the seeder and the extender drive the device's two streams
-    std::thread seeder([&]() {
-      for (size_t k = 0; k < nb; ++k) {
-        // the view of block k reuses the buffer of block k - 2
-        if (k >= 2 && !wait_for(chained_upto, k - 1)) return;
-        const double t0 = Now();
-        const int r = dev->SeedHits(ci, (uint32_t)blocks[k].qb, (uint32_t)blocks[k].qe,
-                                    &views[k], st);
-        seed_hits += views[k].count;
-        sec_seed += Now() - t0;
-        trace.Add("seed", k, t0, Now());
-        publish(seeded, k + 1, r);
-        if (r != 0) return;
-      }
-    });
-    // the extender and the builder share the device's extension stream;
-    // their host work overlaps
-    std::thread extender([&]() {
-      for (size_t k = 0; k < nb; ++k) {
-        if (!wait_for(chained_upto, k + 1)) return;
-        const double t0 = Now();
-        const int r = gapped(ci, blocks[k]);
-        sec_gapped += Now() - t0;
-        trace.Add("gapped", k, t0, Now());
-        publish(gapped_upto, k + 1, r);
-        if (r != 0) return;
-      }
-    });
-    std::thread builder;
-    if (last) {
-      builder = std::thread([&]() {
-        for (size_t k = 0; k < nb; ++k) {
-          if (!wait_for(added, k + 1)) return;
-          const double t0 = Now();
-          int r = 0;
-          if (!unsupported) {
-            r = build_device(blocks[k].qb, blocks[k].qe, &block_parts[k]);
-            if (r == PresearchDevice::kTracebackUnsupported) {
-              unsupported = true;
-              r = 0;
-            }
-          }
-          sec_build += Now() - t0;
-          trace.Add("build_device", k, t0, Now());
-          publish(traced, k + 1, r);
-          if (r != 0) return;
-        }
-      });
+  // The pipeline runs over every (database chunk, block) pair as one
+  // sequence of tasks t = chunk * nb + block, so that the next chunk's
+  // seeding overlaps the previous chunk's last extensions instead of the
+  // pipeline draining at every chunk.  A block's host arrays (hits per query,
+  // job buffers) are reused by the same block of the next chunk, so its
+  // chain filter and gapped extension wait for the previous chunk's
+  // AddResults of the block; AddResults runs tasks in order, which keeps
+  // every query's merges in chunk order.  The result build and the output
+  // follow the last chunk's AddResults.
+  const size_t nchunks = db.chunks.size();
+  const size_t ntasks = nchunks * nb;
+  const size_t last_base = (nchunks - 1) * nb;
+  std::vector<SeedHitView> views(ntasks);
+  std::mutex mu;
+  std::condition_variable cv;
+  size_t seeded = 0, chained_upto = 0, gapped_upto = 0, added = 0, traced = 0;
+  int status = 0;
+  bool unsupported = false;
+  auto wait_for = [&](const size_t &counter, size_t value) {
+    std::unique_lock<std::mutex> lock(mu);
+    cv.wait(lock, [&] { return counter >= value || status != 0; });
+    return status == 0;
+  };
+  auto publish = [&](size_t &counter, size_t value, int r) {
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      counter = value;
+      if (r != 0 && status == 0) status = r;
     }
-    std::thread adder([&]() {
-      for (size_t b = 0; b < nb; ++b) {
-        if (!wait_for(gapped_upto, b + 1)) return;
-        const double t0 = Now();
-        add(ci, blocks[b]);
-        sec_add += Now() - t0;
-        trace.Add("add", b, t0, Now());
-        publish(added, b + 1, 0);
-      }
-    });
-    std::thread writer;
-    if (last) {
-      writer = std::thread([&]() {
-        for (size_t b = 0; b < nb; ++b) {
-          if (!wait_for(traced, b + 1)) return;
-          if (unsupported || rebuild) continue;
-          const double t0 = Now();
-          const size_t before = stale_chunks.size();
-          build_host(block_parts[b], &host_jobs, &stale_chunks);
-          std::vector<BuildPart>().swap(block_parts[b]);
-          if (stale_chunks.size() != before) {
-            rebuild = true;
-            continue;
-          }
-          SortResults(results, blocks[b].qb, blocks[b].qe);
-          const double t1 = Now();
-          sec_build_host += t1 - t0;
-          trace.Add("build_host", b, t0, t1);
-          // blocks are written in query order, straight into the output
-          WriteOutputText(output, q, db, results, blocks[b].qb, blocks[b].qe, threads);
-          // reserve for the whole query chunk, extrapolated from the text so
-          // far, so that the text moves at most once more rather than at
-          // every doubling
-          if (b + 1 < nb) {
-            const double share = (double)blocks[b].qe / (double)nq;
-            const size_t want =
-                output_before + (size_t)((double)(output->size() - output_before) / share * 1.15);
-            if (output->capacity() < want) output->reserve(want);
-          }
-          sec_output += Now() - t1;
-          trace.Add("output", b, t1, Now());
-        }
-      });
-    }
-    for (size_t b = 0; b < nb; ++b) {
-      if (!wait_for(seeded, b + 1)) break;
+    cv.notify_all();
+  };
+  // the seeder and the extender drive the device's two streams
+  std::thread seeder([&]() {
+    for (size_t t = 0; t < ntasks; ++t) {
+      const size_t ci = t / nb, k = t % nb;
+      // the view of task t reuses the buffer of task t - 2
+      if (t >= 2 && !wait_for(chained_upto, t - 1)) return;
       const double t0 = Now();
-      chain(chunk, views[b], blocks[b].qb, blocks[b].qb, blocks[b].qe);
-      sec_chain += Now() - t0;
-      trace.Add("chain", b, t0, Now());
-      publish(chained_upto, b + 1, 0);
+      const int r = dev->SeedHits(ci, (uint32_t)blocks[k].qb, (uint32_t)blocks[k].qe,
+                                  &views[t], st);
+      seed_hits += views[t].count;
+      sec_seed += Now() - t0;
+      trace.Add("seed", t, t0, Now());
+      publish(seeded, t + 1, r);
+      if (r != 0) return;
     }
-    seeder.join();
-    extender.join();
-    if (builder.joinable()) builder.join();
-    adder.join();
-    if (writer.joinable()) writer.join();
-    GZ_STEP(status);
-    if (last && (unsupported || rebuild)) {
-      // rebuild every result in the reference's order and write all queries
-      t = Now();
-      GZ_STEP(build_all());
-      sec_build += Now() - t;
-      t = Now();
-      output->resize(output_before);
-      WriteOutputText(output, q, db, results, 0, nq, threads);
-      sec_output += Now() - t;
+  });
+  // the extender and the builder drive their own device streams;
+  // their host work overlaps
+  std::thread extender([&]() {
+    for (size_t t = 0; t < ntasks; ++t) {
+      if (!wait_for(chained_upto, t + 1)) return;
+      const double t0 = Now();
+      const int r = gapped(t / nb, blocks[t % nb]);
+      sec_gapped += Now() - t0;
+      trace.Add("gapped", t, t0, Now());
+      publish(gapped_upto, t + 1, r);
+      if (r != 0) return;
     }
+  });
+  std::thread builder([&]() {
+    for (size_t k = 0; k < nb; ++k) {
+      if (!wait_for(added, last_base + k + 1)) return;
+      const double t0 = Now();
+      int r = 0;
+      if (!unsupported) {
+        r = build_device(blocks[k].qb, blocks[k].qe, &block_parts[k]);
+        if (r == PresearchDevice::kTracebackUnsupported) {
+          unsupported = true;
+          r = 0;
+        }
+      }
+      sec_build += Now() - t0;
+      trace.Add("build_device", k, t0, Now());
+      publish(traced, k + 1, r);
+      if (r != 0) return;
+    }
+  });
+  std::thread adder([&]() {
+    for (size_t t = 0; t < ntasks; ++t) {
+      if (!wait_for(gapped_upto, t + 1)) return;
+      const double t0 = Now();
+      add(t / nb, blocks[t % nb]);
+      sec_add += Now() - t0;
+      trace.Add("add", t, t0, Now());
+      publish(added, t + 1, 0);
+    }
+  });
+  std::thread writer([&]() {
+    for (size_t b = 0; b < nb; ++b) {
+      if (!wait_for(traced, b + 1)) return;
+      if (unsupported || rebuild) continue;
+      const double t0 = Now();
+      const size_t before = stale_chunks.size();
+      build_host(block_parts[b], &host_jobs, &stale_chunks);
+      std::vector<BuildPart>().swap(block_parts[b]);
+      if (stale_chunks.size() != before) {
+        rebuild = true;
+        continue;
+      }
+      SortResults(results, blocks[b].qb, blocks[b].qe);
+      const double t1 = Now();
+      sec_build_host += t1 - t0;
+      trace.Add("build_host", b, t0, t1);
+      // blocks are written in query order, straight into the output
+      WriteOutputText(output, q, db, results, blocks[b].qb, blocks[b].qe, threads);
+      // reserve for the whole query chunk, extrapolated from the text so
+      // far, so that the text moves at most once more rather than at
+      // every doubling
+      if (b + 1 < nb) {
+        const double share = (double)blocks[b].qe / (double)nq;
+        const size_t want =
+            output_before + (size_t)((double)(output->size() - output_before) / share * 1.15);
+        if (output->capacity() < want) output->reserve(want);
+      }
+      sec_output += Now() - t1;
+      trace.Add("output", b, t1, Now());
+    }
+  });
+  for (size_t t = 0; t < ntasks; ++t) {
+    if (!wait_for(seeded, t + 1)) break;
+    // the block's hit lists and job arrays are free once the previous
+    // chunk's AddResults of the block is done
+    if (t >= nb && !wait_for(added, t - nb + 1)) break;
+    const size_t ci = t / nb, k = t % nb;
+    const double t0 = Now();
+    chain(db.chunks[ci], views[t], blocks[k].qb, blocks[k].qb, blocks[k].qe);
+    sec_chain += Now() - t0;
+    trace.Add("chain", t, t0, Now());
+    publish(chained_upto, t + 1, 0);
+  }
+  seeder.join();
+  extender.join();
+  builder.join();
+  adder.join();
+  writer.join();
+  GZ_STEP(status);
+  if (unsupported || rebuild) {
+    // rebuild every result in the reference's order and write all queries
+    t = Now();
+    GZ_STEP(build_all());
+    sec_build += Now() - t;
+    t = Now();
+    output->resize(output_before);
+    WriteOutputText(output, q, db, results, 0, nq, threads);
+    sec_output += Now() - t;
   }
   dev->EndQueryChunk();
   st->seed_hits += seed_hits;

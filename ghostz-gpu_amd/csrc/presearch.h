@@ -65,6 +65,7 @@ struct AlignStats {
   double schedule_kernel_ms = 0;  // ordering of the gapped jobs on the device
   double traceback_kernel_ms = 0; // result build on the device
   uint64_t traceback_host_jobs = 0;  // results the device left to the host
+  uint64_t gated_jobs = 0;        // learned schedule: jobs the cost model predicted
   double seed_kernel_ms[2] = {0, 0};  // seed kernels: representatives, values
   uint64_t seed_launches[2] = {0, 0};
   uint64_t gapped_launches = 0;
@@ -89,6 +90,13 @@ struct AlignerOptions {
 
 // Order keys of gapped jobs are below this bound (larger key = more work).
 constexpr uint32_t kOrderKeyLimit = 4096;
+
+// Query-length buckets of the batched form's gapped stage (residues of the
+// framed query: <= 128, <= 512, <= 2048, longer).
+constexpr int kLengthBuckets = 4;
+inline int LengthBucket(uint32_t framed_length) {
+  return framed_length <= 130 ? 0 : framed_length <= 514 ? 1 : framed_length <= 2050 ? 2 : 3;
+}
 
 // One gapped extension job as traced for the learned scheduler.
 struct GappedTrace {
@@ -127,9 +135,13 @@ class PresearchDevice {
   // The jobs are independent; the device may run them in any order:
   // keys (optional, < kOrderKeyLimit) ask for descending-key order, learned
   // asks the device to predict the keys with its cost model.
+  // bucket_off (optional, kLengthBuckets + 1 entries): the jobs of each side
+  // are laid out by query-length bucket, bucket b holding side-relative jobs
+  // [bucket_off[b], bucket_off[b + 1]) (n_reverse == n_forward then).
   virtual int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward,
                            uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
-                           const uint16_t *keys, bool learned, AlignStats *st) = 0;
+                           const uint16_t *keys, bool learned, AlignStats *st,
+                           const uint32_t *bucket_off = nullptr) = 0;
   // Result build of `jobs` of database chunk `chunk` with the result cutoff.
   // Returns kTracebackUnsupported when the device does not build results.
   virtual int Tracebacks(size_t chunk, const std::vector<TracebackJob> &jobs, int cutoff,

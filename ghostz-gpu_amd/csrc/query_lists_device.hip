@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "gz_common.h"
+#include "gz_runtime.h"
 #include "gz_device.h"
 #include "gz_scan.h"
 #include "query_lists_device.h"
@@ -194,17 +195,17 @@ DeviceListBuilder::~DeviceListBuilder() {
                   &galt2_, &rank_of_group_, &list_size_, &out_hash_, &out_off_, &out_pos_,
                   &out_qid_, &out_check_};
   for (Buffer *b : bs)
-    if (b->p) (void)hipFree(b->p);
-  if (pinned_) (void)hipHostFree(pinned_);
+    if (b->p) (void)gz::Free(b->p);
+  if (pinned_) (void)gz::HostFree(pinned_);
 }
 
 hipError_t DeviceListBuilder::Reserve(Buffer &b, size_t bytes) {
   bytes = std::max<size_t>(AlignUp(bytes, 256), 256);
   if (b.cap >= bytes && b.p) return hipSuccess;
-  if (b.p) GZ_TRY(hipFree(b.p));
+  if (b.p) GZ_TRY(gz::Free(b.p));
   b.p = nullptr;
   b.cap = 0;
-  GZ_TRY(hipMalloc(&b.p, bytes));
+  GZ_TRY(gz::Malloc(&b.p, bytes));
   b.cap = bytes;
   return hipSuccess;
 }
@@ -227,7 +228,7 @@ hipError_t DeviceListBuilder::Build(const uint64_t *packed, uint64_t n_codes, co
   hp.max_length = hf.max_length;
   hp.shift = hf.shift;
   hp.threshold = hf.threshold;
-  if (!pinned_) GZ_TRY(hipHostMalloc(&pinned_, 64, hipHostMallocDefault));
+  if (!pinned_) GZ_TRY(gz::HostMalloc(&pinned_, 64, hipHostMallocDefault));
   uint32_t *hp32 = (uint32_t *)pinned_;
 
   // appearance order (blocks of 64 queries, reversed inside a block) and the

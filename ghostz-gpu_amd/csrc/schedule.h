@@ -27,4 +27,26 @@ hipError_t DeviceOrderByKey(const uint16_t *keys, uint32_t n, uint32_t *perm, vo
 hipError_t CostToKey(const float *cost, uint32_t n, float scale, uint16_t *keys,
                      hipStream_t stream);
 
+// The learned schedule's two-level keys.  A job's query side length (the
+// residues it may extend before the query's delimiter) bounds its DP cells
+// and orders most of the work; the cost model orders the jobs whose side is
+// long enough for the bound to leave room (at least `gate` residues), inside
+// their length bin:
+//   key = min(length, 63) << 6 | prediction bin,
+// the prediction bin being clamp(floor(4 * predicted log2 cells), 0, 63) for
+// a gated job and 0 otherwise.
+constexpr uint32_t kLengthBins = 64;
+// keys[j] = min(length[j], 63) << 6
+hipError_t LengthBinKeys(const uint16_t *length, uint32_t n, uint16_t *keys, hipStream_t stream);
+// Appends the indices j in [0, n) with length[j] >= gate to index (count at
+// *d_count, which the call clears first); the order of the list is
+// unspecified.
+hipError_t GateJobs(const uint16_t *length, uint32_t n, uint32_t gate, uint32_t *index,
+                    uint32_t *d_count, hipStream_t stream);
+// keys[index[k]] |= prediction bin of pred[k], k < m (k < *d_count when
+// d_count is given).
+hipError_t AddPredictionBins(const float *pred, const uint32_t *index, uint32_t m,
+                             uint16_t *keys, hipStream_t stream,
+                             const uint32_t *d_count = nullptr);
+
 }  // namespace gz

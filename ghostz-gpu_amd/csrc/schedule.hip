@@ -9,7 +9,7 @@ namespace gz {
 namespace {
 
 constexpr int kOrderThreads = 256;
-constexpr uint32_t kOrderChunk = 8192;  // jobs per block
+constexpr uint32_t kOrderChunk = 32768;  // jobs per block (few global bin atomics per job)
 
 __global__ void __launch_bounds__(kOrderThreads)
     BinHistogramKernel(const uint16_t *keys, uint32_t n, uint32_t *counts) {
@@ -79,6 +79,42 @@ __global__ void CostToKeyKernel(const float *cost, uint32_t n, float scale, uint
   keys[i] = (uint16_t)k;
 }
 
+__global__ void LengthBinKeysKernel(const uint16_t *length, uint32_t n, uint16_t *keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = (uint16_t)(min((uint32_t)length[i], kLengthBins - 1) << 6);
+}
+
+__global__ void ClearCountKernel(uint32_t *count) {
+  if (threadIdx.x == 0) *count = 0;
+}
+
+// wave-aggregated append: one atomic per wave
+__global__ void GateKernel(const uint16_t *length, uint32_t n, uint32_t gate, uint32_t *index,
+                           uint32_t *count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool take = i < n && (uint32_t)length[i] >= gate;
+  const uint64_t mask = __ballot(take);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, kWave);
+  if (take) index[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = i;
+}
+
+__global__ void PredictionBinKernel(const float *pred, const uint32_t *index, uint32_t m,
+                                    const uint32_t *count, uint16_t *keys) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (count) m = min(m, *count);
+  if (k >= m) return;
+  const float x = pred[k] * 4.f;
+  const int b = (x == x) ? (int)fminf(fmaxf(floorf(x), 0.f), (float)(kLengthBins - 1)) : 0;
+  const uint32_t j = index[k];
+  keys[j] = (uint16_t)(keys[j] | (uint32_t)b);
+}
+
 inline unsigned Blocks(uint32_t n, unsigned t = 256) { return (n + t - 1) / t; }
 
 }  // namespace
@@ -104,6 +140,29 @@ hipError_t DeviceOrderByKey(const uint16_t *keys, uint32_t n, uint32_t *perm, vo
 hipError_t CostToKey(const float *cost, uint32_t n, float scale, uint16_t *keys, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(CostToKeyKernel, dim3(Blocks(n)), dim3(256), 0, s, cost, n, scale, keys);
+  return hipGetLastError();
+}
+
+hipError_t LengthBinKeys(const uint16_t *length, uint32_t n, uint16_t *keys, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(LengthBinKeysKernel, dim3(Blocks(n)), dim3(256), 0, s, length, n, keys);
+  return hipGetLastError();
+}
+
+hipError_t GateJobs(const uint16_t *length, uint32_t n, uint32_t gate, uint32_t *index,
+                    uint32_t *d_count, hipStream_t s) {
+  hipLaunchKernelGGL(ClearCountKernel, dim3(1), dim3(64), 0, s, d_count);
+  if (n > 0)
+    hipLaunchKernelGGL(GateKernel, dim3(Blocks(n)), dim3(256), 0, s, length, n, gate, index,
+                       d_count);
+  return hipGetLastError();
+}
+
+hipError_t AddPredictionBins(const float *pred, const uint32_t *index, uint32_t m,
+                             uint16_t *keys, hipStream_t s, const uint32_t *d_count) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(PredictionBinKernel, dim3(Blocks(m)), dim3(256), 0, s, pred, index, m,
+                     d_count, keys);
   return hipGetLastError();
 }
 

@@ -54,6 +54,10 @@ struct SeedStageParams {
   int distance_threshold = 2;  // rep extension when d <= threshold
   int max_mismatches = 1;      // members when max(d - mm, 0) <= threshold
   bool clustering = true;
+  // byte ungapped score tables: every cutoff of the query chunk in
+  // [1, kMaxByteCutoff] and every matrix score in [-127, 127]; 16-bit tables
+  // otherwise
+  bool byte_tables = false;
 };
 
 struct SeedStageStats {

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "gz_common.h"
+#include "gz_runtime.h"
 #include "gz_device.h"
 #include "gz_scan.h"
 #include "search_device.h"
@@ -179,12 +180,24 @@ __global__ void TileRangeKernel(const uint64_t *active_off, uint32_t nactive,
   range[t] = make_uint2(a, lo - a);  // lists a .. lo-1
 }
 
-// The `kReverse` sides of the lane's items whose flag is set, one twelve-code
-// window per round: the lane starts its next item as soon as the current
-// walk stops.  Rounds repeat while any lane of the wave has a walk left.
-template <bool kReverse>
+// Item `second` of a lane's two windows, field by field (a select of whole
+// structs would go through private memory).
+__device__ __forceinline__ SeedWindow PickWindow(bool second, const SeedWindow (&s)[kSeedItems]) {
+  SeedWindow r;
+  r.w0 = second ? s[1].w0 : s[0].w0;
+  r.w1 = second ? s[1].w1 : s[0].w1;
+  r.w2 = second ? s[1].w2 : s[0].w2;
+  r.wi = second ? s[1].wi : s[0].wi;
+  r.k = second ? s[1].k : s[0].k;
+  return r;
+}
+
+// The `kReverse` sides of the lane's items whose flag is set, one query word
+// per round: the lane starts its next item as soon as the current walk
+// stops.  Rounds repeat while any lane of the wave has a walk left.
+template <bool kReverse, typename T>
 __device__ __forceinline__ void WalkItems(const uint64_t *qw, const uint64_t *dw,
-                                          const int16_t *tab,
+                                          const PairTables<T> *tab,
                                           const SeedWindow (&sq)[kSeedItems],
                                           const SeedWindow (&sd)[kSeedItems],
                                           const bool (&todo)[kSeedItems],
@@ -194,17 +207,17 @@ __device__ __forceinline__ void WalkItems(const uint64_t *qw, const uint64_t *dw
   score[0] = 0;
   score[1] = 0;
   int cur = todo[0] ? 0 : (todo[1] ? 1 : 2);
-  SideWalk w = SideStart<kReverse>(cur == 1 ? sq[1] : sq[0], cur == 1 ? sd[1] : sd[0]);
+  PairWalk w = PairStart<kReverse>(PickWindow(cur == 1, sq), PickWindow(cur == 1, sd));
   int cut = cur == 1 ? cutoff[1] : cutoff[0];
   for (;;) {
     const bool active = cur < 2;
     if (!__any(active)) break;
-    if (active && !SideStep<kReverse>(qw, dw, tab, cut, w)) {
+    if (active && !PairStep<kReverse>(qw, dw, tab, cut, w)) {
       if (cur == 0) {
         score[0] = w.best;
         cur = todo[1] ? 1 : 2;
         if (cur == 1) {
-          w = SideStart<kReverse>(sq[1], sd[1]);
+          w = PairStart<kReverse>(sq[1], sd[1]);
           cut = cutoff[1];
         }
       } else {
@@ -217,14 +230,14 @@ __device__ __forceinline__ void WalkItems(const uint64_t *qw, const uint64_t *dw
 
 // Register budgets: five waves per SIMD for the value kernel, four for the
 // representative kernel (whose member loop needs more).
-template <bool kValues>
+template <bool kValues, typename T>
 __global__ void __launch_bounds__(kSeedThreads)
     __attribute__((amdgpu_waves_per_eu(kValues ? 5 : 4)))
     SeedKernel(DeviceQueryLists q, DeviceDbChunk db, SeedStageParams p,
                const ListInfo *info, const uint32_t *active,
                const uint64_t *active_off, const uint2 *tile_range, uint64_t unit_begin,
                uint64_t unit_end, SeedHit *hits, uint32_t capacity, uint32_t *counters) {
-  __shared__ int16_t utab[kTableDim * kTableDim];
+  __shared__ PairTables<T> ptab;
   __shared__ uint8_t differs[kTableDim * kTableDim];
   // the tile's lists: unit offset, list rank, first entry, entries, first
   // representative / value, representatives
@@ -237,7 +250,7 @@ __global__ void __launch_bounds__(kSeedThreads)
   __shared__ SeedHit stage_lds[kSeedThreads / kWave][kWaveStage];
   HitStage stage{stage_lds[threadIdx.x / kWave], 0u};
   for (uint32_t i = threadIdx.x; i < kTableDim * kTableDim; i += blockDim.x) {
-    utab[i] = UngappedTableEntry(p.matrix, p.nl, i);
+    PairTablesFill(ptab, p.matrix, p.nl, i);
     if (!kValues) differs[i] = DiffersEntry(p.reduced_map, i);
   }
   const int lane = threadIdx.x & (kWave - 1);
@@ -353,12 +366,12 @@ __global__ void __launch_bounds__(kSeedThreads)
     // item as soon as the current walk stops, so a wave runs as long as its
     // longest sum of walks rather than the sum of its longest walks
     int sum[kSeedItems];
-    WalkItems<true>(q.packed, db.packed, utab, wq, wd, need, cutv, sum);
+    WalkItems<true>(q.packed, db.packed, &ptab, wq, wd, need, cutv, sum);
     bool fwd[kSeedItems];
     int fsum[kSeedItems];
 #pragma unroll
     for (int j = 0; j < kSeedItems; ++j) fwd[j] = need[j] && sum[j] <= trigv[j];
-    WalkItems<false>(q.packed, db.packed, utab, wq, wd, fwd, cutv, fsum);
+    WalkItems<false>(q.packed, db.packed, &ptab, wq, wd, fwd, cutv, fsum);
 #pragma unroll
     for (int j = 0; j < kSeedItems; ++j) {
       if (fwd[j]) sum[j] += fsum[j];
@@ -419,7 +432,7 @@ __global__ void __launch_bounds__(kSeedThreads)
           if (t < total) {
             mh.dpos = db.mem_pos[m];
             mh.member = 1 + k;
-            mp = UngappedWithTrigger(q.packed, mh.qpos, db.packed, mh.dpos, utab, mcut, mtrig) >
+            mp = UngappedWithTrigger(q.packed, mh.qpos, db.packed, mh.dpos, &ptab, mcut, mtrig) >
                  mtrig;
           }
           StageAppend(mp, mh, stage, hits, counters, capacity);
@@ -465,13 +478,13 @@ SeedStage::~SeedStage() {
   void *ps[] = {lookup_,  units_,  scan_tmp_,   active_,   hits_,
                 grouped_, qcount_, tile_range_, base_info_, entry_rec_};
   for (void *p : ps)
-    if (p) (void)hipFree(p);
+    if (p) (void)gz::Free(p);
   for (int k = 0; k < 2; ++k) {
-    if (host_hits_[k]) (void)hipHostFree(host_hits_[k]);
-    if (host_qstart_[k]) (void)hipHostFree(host_qstart_[k]);
+    if (host_hits_[k]) (void)gz::HostFree(host_hits_[k]);
+    if (host_qstart_[k]) (void)gz::HostFree(host_qstart_[k]);
   }
-  if (counters_) (void)hipFree(counters_);
-  if (pinned_) (void)hipHostFree(pinned_);
+  if (counters_) (void)gz::Free(counters_);
+  if (pinned_) (void)gz::HostFree(pinned_);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
 }
@@ -479,12 +492,12 @@ SeedStage::~SeedStage() {
 hipError_t SeedStage::Reserve(void **p, size_t *cap, size_t bytes) {
   if (*cap >= bytes && *p) return hipSuccess;
   if (*p) {
-    hipError_t e = hipFree(*p);
+    hipError_t e = gz::Free(*p);
     if (e != hipSuccess) return e;
     *p = nullptr;
   }
   bytes = std::max<size_t>(bytes, 256);
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = gz::Malloc(p, bytes);
   if (e == hipSuccess) *cap = bytes;
   return e;
 }
@@ -532,9 +545,9 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
   const uint32_t nq = qe - qb;
   const int slot = flip_;
   flip_ ^= 1;
-  if (!counters_) GZ_TRY(hipMalloc(&counters_, kCounters * sizeof(uint32_t)));
+  if (!counters_) GZ_TRY(gz::Malloc(&counters_, kCounters * sizeof(uint32_t)));
   if (!pinned_) {
-    GZ_TRY(hipHostMalloc(&pinned_, 64, hipHostMallocDefault));
+    GZ_TRY(gz::HostMalloc(&pinned_, 64, hipHostMallocDefault));
     pinned_cap_ = 64;
   }
   if (!ev_[0]) {
@@ -545,10 +558,10 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
   GZ_TRY(Reserve(&qcount_, &qcount_cap_, 2 * AlignUp((size_t)(nq + 1) * 4, 256) +
                                              DeviceScanTempBytes<uint32_t>(nq + 1)));
   if (host_qstart_cap_[slot] < (size_t)(nq + 1) * 4) {
-    if (host_qstart_[slot]) GZ_TRY(hipHostFree(host_qstart_[slot]));
+    if (host_qstart_[slot]) GZ_TRY(gz::HostFree(host_qstart_[slot]));
     host_qstart_[slot] = nullptr;
     host_qstart_cap_[slot] = 0;
-    GZ_TRY(hipHostMalloc(&host_qstart_[slot], (size_t)(nq + 1) * 4, hipHostMallocDefault));
+    GZ_TRY(gz::HostMalloc(&host_qstart_[slot], (size_t)(nq + 1) * 4, hipHostMallocDefault));
     host_qstart_cap_[slot] = (size_t)(nq + 1) * 4;
   }
   uint32_t *h_qstart = (uint32_t *)host_qstart_[slot];
@@ -634,14 +647,24 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
                            (uint2 *)tile_range_);
         GZ_TRY(hipGetLastError());
         GZ_TRY(hipEventRecord(ev_[0], s));
-        if (k == 0) {
-          hipLaunchKernelGGL((SeedKernel<false>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
-                             info, act[0], act_off[0], (const uint2 *)tile_range_, u0, u1,
-                             (SeedHit *)hits_, cap, counters_);
+        // byte score tables when every cutoff and score fits them
+        const uint2 *tr = (const uint2 *)tile_range_;
+        if (k == 0 && p.byte_tables) {
+          hipLaunchKernelGGL((SeedKernel<false, int8_t>), dim3(grid), dim3(kSeedThreads), 0, s,
+                             q, db, p, info, act[0], act_off[0], tr, u0, u1, (SeedHit *)hits_,
+                             cap, counters_);
+        } else if (k == 0) {
+          hipLaunchKernelGGL((SeedKernel<false, int16_t>), dim3(grid), dim3(kSeedThreads), 0, s,
+                             q, db, p, info, act[0], act_off[0], tr, u0, u1, (SeedHit *)hits_,
+                             cap, counters_);
+        } else if (p.byte_tables) {
+          hipLaunchKernelGGL((SeedKernel<true, int8_t>), dim3(grid), dim3(kSeedThreads), 0, s, q,
+                             db, p, info, act[1], act_off[1], tr, u0, u1, (SeedHit *)hits_, cap,
+                             counters_);
         } else {
-          hipLaunchKernelGGL((SeedKernel<true>), dim3(grid), dim3(kSeedThreads), 0, s, q, db, p,
-                             info, act[1], act_off[1], (const uint2 *)tile_range_, u0, u1,
-                             (SeedHit *)hits_, cap, counters_);
+          hipLaunchKernelGGL((SeedKernel<true, int16_t>), dim3(grid), dim3(kSeedThreads), 0, s,
+                             q, db, p, info, act[1], act_off[1], tr, u0, u1, (SeedHit *)hits_,
+                             cap, counters_);
         }
         GZ_TRY(hipGetLastError());
         GZ_TRY(hipEventRecord(ev_[1], s));
@@ -660,10 +683,10 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
           // hits of earlier ranges), rewind the count and redo the range
           const size_t keep = (size_t)total_hits * sizeof(SeedHit);
           void *bigger = nullptr;
-          GZ_TRY(hipMalloc(&bigger, std::max<size_t>((size_t)count * 2, 1024) * sizeof(SeedHit)));
+          GZ_TRY(gz::Malloc(&bigger, std::max<size_t>((size_t)count * 2, 1024) * sizeof(SeedHit)));
           if (keep) GZ_TRY(hipMemcpyAsync(bigger, hits_, keep, hipMemcpyDeviceToDevice, s));
           GZ_TRY(hipStreamSynchronize(s));
-          GZ_TRY(hipFree(hits_));
+          GZ_TRY(gz::Free(hits_));
           hits_ = bigger;
           hits_cap_ = std::max<size_t>((size_t)count * 2, 1024) * sizeof(SeedHit);
           hc[0] = (uint32_t)total_hits;
@@ -700,11 +723,11 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
     GZ_TRY(hipGetLastError());
     const size_t bytes = total_hits * sizeof(SeedHit);
     if (host_hits_cap_[slot] < bytes) {
-      if (host_hits_[slot]) GZ_TRY(hipHostFree(host_hits_[slot]));
+      if (host_hits_[slot]) GZ_TRY(gz::HostFree(host_hits_[slot]));
       host_hits_[slot] = nullptr;
       host_hits_cap_[slot] = 0;
       const size_t want = bytes + bytes / 4;
-      GZ_TRY(hipHostMalloc(&host_hits_[slot], want, hipHostMallocDefault));
+      GZ_TRY(gz::HostMalloc(&host_hits_[slot], want, hipHostMallocDefault));
       host_hits_cap_[slot] = want;
     }
     GZ_TRY(hipMemcpyAsync(host_hits_[slot], grouped_, bytes, hipMemcpyDeviceToHost, s));

@@ -32,7 +32,7 @@ constexpr int kStreams = 2;    // GpuStreamRunner::kNumberStream
 
 template <class T>
 hipError_t AllocArray(T **p, size_t n) {
-  return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+  return gz::Malloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
 }
 
 }  // namespace
@@ -46,7 +46,7 @@ struct gz_host_seeds_memory_s {
     void *ps[] = {query_seed_ids, query_seed_starts, query_ids, query_positions,
                   database_positions, result_values};
     for (void *p : ps)
-      if (p) (void)hipHostFree(p);
+      if (p) (void)gz::HostFree(p);
   }
 };
 
@@ -64,7 +64,7 @@ struct DeviceSeeds {
     void *ps[] = {query_seed_ids, query_seed_starts, query_ids, query_positions,
                   database_positions, result_values, temp, convert_temp};
     for (void *p : ps)
-      if (p) (void)hipFree(p);
+      if (p) (void)gz::Free(p);
     *this = DeviceSeeds();
   }
   hipError_t Init(size_t n) {
@@ -77,7 +77,7 @@ struct DeviceSeeds {
     if (e == hipSuccess) e = AllocArray(&query_positions, n);
     if (e == hipSuccess) e = AllocArray(&database_positions, n);
     if (e == hipSuccess) e = AllocArray(&result_values, n);
-    if (e == hipSuccess) e = hipMalloc(&temp, std::max<size_t>(temp_bytes, 16));
+    if (e == hipSuccess) e = gz::Malloc(&temp, std::max<size_t>(temp_bytes, 16));
     if (e == hipSuccess) e = AllocArray(&convert_temp, 2 * n);
     return e;
   }
@@ -123,7 +123,7 @@ gz_status gz_host_seeds_memory_create(size_t size, gz_host_seeds_memory_t *out) 
   void **ps[] = {(void **)&m->query_seed_ids, (void **)&m->query_seed_starts,
                  (void **)&m->query_ids, (void **)&m->query_positions,
                  (void **)&m->database_positions, (void **)&m->result_values};
-  for (void **p : ps) GZ_HIP_CHECK(hipHostMalloc(p, size * 4, hipHostMallocDefault));
+  for (void **p : ps) GZ_HIP_CHECK(gz::HostMalloc(p, size * 4, hipHostMallocDefault));
   *out = m.release();
   return GZ_OK;
 }

@@ -89,6 +89,8 @@ _SIGNATURES = {
         c_int, [c_void_p, c_size_t, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gz_gapped_extender_set_max_sequence_length": (c_int, [c_void_p, c_uint32]),
     "gz_gapped_extender_set_narrow_ring": (c_int, [c_void_p, c_uint32]),
+    "gz_gapped_extender_set_persistent": (c_int, [c_void_p, c_int]),
+    "gz_gapped_extender_reserve": (c_int, [c_void_p, c_size_t, c_void_p]),
     "gz_gapped_extender_overflows": (c_int, [c_void_p, u32p, u32p]),
     "gz_gapped_extender_failures": (c_int, [c_void_p, u32p]),
 }
@@ -281,7 +283,7 @@ class AlignStats(Structure):
         ("seed_value_kernel_ms", c_double), ("seed_rep_launches", c_uint64),
         ("seed_value_launches", c_uint64), ("gapped_launches", c_uint64),
         ("schedule_kernel_ms", c_double), ("traceback_kernel_ms", c_double),
-        ("traceback_host_jobs", c_uint64)]
+        ("traceback_host_jobs", c_uint64), ("gated_jobs", c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -313,6 +315,8 @@ _SIGNATURES.update({
                                    POINTER(DatabaseOptions), POINTER(c_void_p)]),
     "gz_database_read_fasta": (c_int, [c_char_p, POINTER(DatabaseOptions), POINTER(c_void_p)]),
     "gz_database_destroy": (c_int, [c_void_p]),
+    "gz_database_load": (c_int, [c_char_p, POINTER(c_void_p)]),
+    "gz_database_save": (c_int, [c_void_p, c_char_p]),
     "gz_database_info": (c_int, [c_void_p, POINTER(DatabaseInfo)]),
     "gz_database_chunk": (c_int, [c_void_p, c_uint32, POINTER(ChunkView)]),
     "gz_aligner_default_parameters": (None, [POINTER(AlignerParameters)]),
@@ -320,6 +324,8 @@ _SIGNATURES.update({
     "gz_aligner_destroy": (c_int, [c_void_p]),
     "gz_aligner_set_database": (c_int, [c_void_p, c_void_p]),
     "gz_aligner_set_cost_model": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "gz_aligner_tuner_state": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gz_aligner_graph_counts": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]),
     "gz_aligner_align": (c_int, [c_void_p, c_void_p, POINTER(c_void_p), POINTER(c_size_t),
                                  POINTER(AlignStats)]),
     "gz_aligner_traces": (c_uint64, [c_void_p, c_void_p, c_uint64]),
@@ -327,6 +333,7 @@ _SIGNATURES.update({
                                                      POINTER(c_uint64)]),
     "gz_align_fasta": (c_int, [c_char_p, c_char_p, c_char_p, c_int,
                                POINTER(AlignerParameters), POINTER(DatabaseOptions)]),
+    "gz_align_main": (c_int, [c_char_p, c_char_p, c_char_p, c_int, POINTER(AlignerParameters)]),
     "gz_free": (None, [c_void_p]),
     "gz_build_results": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_uint32, c_uint32,
                                  c_void_p, c_uint32, c_int, c_int, c_int, c_void_p, c_void_p]),
@@ -488,6 +495,19 @@ class Database:
         self.handle = h
         return self
 
+    @classmethod
+    def load(cls, prefix):
+        """Database::Load: <prefix>.inf and <prefix>_<i>.{inf,off,nam,seq,scp,sdp}."""
+        self = cls.__new__(cls)
+        h = c_void_p()
+        check(lib().gz_database_load(prefix.encode(), ctypes.byref(h)))
+        self.handle = h
+        return self
+
+    def save(self, prefix):
+        """Database::Save: the same file set."""
+        check(lib().gz_database_save(self.handle, prefix.encode()))
+
     def info(self):
         i = DatabaseInfo()
         check(lib().gz_database_info(self.handle, ctypes.byref(i)))
@@ -566,6 +586,21 @@ class Aligner:
         text = ctypes.string_at(out, n.value) if n.value else b""
         lib().gz_free(out)
         return text, st.as_dict()
+
+    def graph_counts(self):
+        """(graphs instantiated, launches that updated a graph) of the batched form."""
+        a, b = c_uint64(), c_uint64()
+        check(lib().gz_aligner_graph_counts(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def tuner_state(self):
+        """The batched form's autotuner: chosen configuration per query-length
+        bucket (-1: not seen), ms per job and trials per (bucket, config)."""
+        chosen = np.zeros(4, np.int32)
+        ms = np.zeros(16, np.float64)
+        trials = np.zeros(16, np.uint32)
+        check(lib().gz_aligner_tuner_state(self.handle, ptr(chosen), ptr(ms), ptr(trials)))
+        return chosen, ms.reshape(4, 4), trials.reshape(4, 4)
 
     def traces(self):
         n = lib().gz_aligner_traces(self.handle, None, 0)

@@ -184,6 +184,17 @@ gz_status gz_gapped_extender_extend_one_side_score_only_cells_async(
  * it move to a 64-column ring, then to a global-memory ring.  No reference
  * counterpart (the reference's single ring caps sequences at 128). */
 gz_status gz_gapped_extender_set_narrow_ring(gz_gapped_extender_t h, uint32_t columns);
+/* Allocates the extender's workspace for calls of up to `size` jobs now
+ * (calls allocate it on demand otherwise), so that a later call allocates
+ * nothing and its launch sequence can be captured in a graph.  No reference
+ * counterpart. */
+gz_status gz_gapped_extender_reserve(gz_gapped_extender_t h, size_t size, gz_stream_t stream);
+/* persistent != 0: the first ring's extensions run as one persistent launch
+ * (as many waves as the device holds) fed from a device work queue: each
+ * lane takes the next job as soon as its extension ends, instead of the
+ * static grid where a wave waits for its slowest lane.  No reference
+ * counterpart (the batched form's kernels); results do not depend on it. */
+gz_status gz_gapped_extender_set_persistent(gz_gapped_extender_t h, int persistent);
 /* Seeds of the last run that outgrew the narrow ring / the 64-column ring. */
 gz_status gz_gapped_extender_overflows(gz_gapped_extender_t h, uint32_t *narrow,
                                        uint32_t *ring64);
@@ -364,6 +375,14 @@ gz_status gz_database_create(const char *const *names, const char *residues,
                              const gz_database_options *options, gz_database_t *out);
 gz_status gz_database_read_fasta(const char *path, const gz_database_options *options,
                                  gz_database_t *out);
+/* Database::Load (src/database.h:229-237, :517-538) with every chunk
+ * resident (DatabaseChunk::Load, src/database_chunk.h:303-400): reads
+ * <prefix>.inf and <prefix>_<i>.{inf,off,nam,seq,scp,sdp}; GZ_ERROR_IO with
+ * the file named when one is missing, truncated or inconsistent. */
+gz_status gz_database_load(const char *prefix, gz_database_t *out);
+/* Database::Save (src/database.h:240-251, :540-554;
+ * DatabaseChunk::Save, src/database_chunk.h:267-278): the same file set. */
+gz_status gz_database_save(gz_database_t db, const char *prefix);
 gz_status gz_database_destroy(gz_database_t db);
 gz_status gz_database_info(gz_database_t db, gz_database_info_t *info);
 gz_status gz_database_chunk(gz_database_t db, uint32_t chunk,
@@ -426,6 +445,8 @@ typedef struct {
                                   cost model, counting sort, gathers) */
   double traceback_kernel_ms;  /* device result build */
   uint64_t traceback_host_jobs; /* results the device left to the host */
+  uint64_t gated_jobs;          /* learned schedule: jobs the cost model
+                                   predicted (query side >= the gate) */
 } gz_align_stats;
 typedef struct {
   uint32_t query_position; /* concatenated seed position */
@@ -445,6 +466,20 @@ gz_status gz_queries_cutoffs(gz_queries_t q, const gz_aligner_parameters *p,
 gz_status gz_aligner_create(const gz_aligner_parameters *p, gz_aligner_t *out);
 gz_status gz_aligner_destroy(gz_aligner_t a);
 gz_status gz_aligner_set_database(gz_aligner_t a, gz_database_t db);
+/* The batched form launches each query-length bucket's gapped sequence as a
+ * HIP graph (captured once per bucket and configuration, then updated with
+ * each call's arguments): how many graphs were instantiated and how many
+ * launches reused one through an update.  No reference counterpart. */
+gz_status gz_aligner_graph_counts(gz_aligner_t a, uint64_t *instantiations, uint64_t *updates);
+/* The batched form's autotuner (no reference counterpart): per query-length
+ * bucket b (<= 128, <= 512, <= 2048, longer residues), chosen[b] = the
+ * gapped configuration it uses next (-1: bucket not seen; configuration c:
+ * first ring 16 columns when c & 1, else 32; static grid when c & 2, else
+ * the persistent work queue), and for each configuration the measured
+ * kernel milliseconds per job and trials (4 x 4 arrays, bucket-major).  Any
+ * pointer may be NULL. */
+gz_status gz_aligner_tuner_state(gz_aligner_t a, int *chosen, double *ms_per_job,
+                                 uint32_t *trials);
 /* Flat fp32 weights of the cost model (embed, w1, b1, w2, b2, w3, b3). */
 gz_status gz_aligner_set_cost_model(gz_aligner_t a, const float *weights, size_t count);
 /* AlignerGpu::Align for in-memory queries: presearch, BuildResults and
@@ -455,6 +490,13 @@ gz_status gz_aligner_align(gz_aligner_t a, gz_queries_t q, char **output,
 uint64_t gz_aligner_traces(gz_aligner_t a, gz_gapped_trace *out, uint64_t capacity);
 gz_status gz_aligner_last_concatenated_queries(gz_aligner_t a, const uint8_t **codes,
                                                uint64_t *length);
+/* `ghostz-gpu aln -i <queries> -d <database prefix> -o <output>`
+ * (AlignMain::Run, src/align_main.cpp:34-80): the database files of
+ * gz_database_save / `ghostz-gpu db`; a missing query file or <prefix>.inf
+ * is GZ_ERROR_IO with the reference's message. */
+gz_status gz_align_main(const char *queries_path, const char *database_prefix,
+                        const char *output_path, int translated,
+                        const gz_aligner_parameters *p);
 /* `ghostz-gpu aln` on FASTA inputs (the database is built in memory). */
 gz_status gz_align_fasta(const char *queries_path, const char *database_path,
                          const char *output_path, int translated,

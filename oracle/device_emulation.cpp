@@ -126,8 +126,8 @@ class CpuDevice : public PresearchDevice {
   }
 
   int GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, uint32_t *dpos,
-                   int *scores, uint32_t *cells, const uint16_t *, bool,
-                   AlignStats *) override {
+                   int *scores, uint32_t *cells, const uint16_t *, bool, AlignStats *,
+                   const uint32_t *) override {
     const uint8_t *qs = concat_->data();
     const uint8_t *ds = db_.chunks[ci].seq.data();
     for (uint32_t k = 0; k < nr + nf; ++k) {
@@ -326,14 +326,22 @@ int gzh_align(const char *const *qnames, const char *qres, const uint64_t *qoffs
 void gzh_free(void *p) { free(p); }
 
 /* The product's word-at-a-time seed tests (ungapped.h) run on the host over
- * padded packed arrays, for checks against the oracle. */
+ * padded packed arrays, for checks against the oracle; table_bytes selects
+ * the byte (1) or 16-bit (2) score tables. */
 void gzh_ungapped_words(const uint64_t *qw, const uint64_t *dw, uint64_t n,
                         const uint32_t *qpos, const uint32_t *dpos, const int *matrix,
-                        uint32_t nl, const int *cutoff, const int *trigger, int *out) {
-  int16_t tab[kTableDim * kTableDim];
-  for (uint32_t i = 0; i < kTableDim * kTableDim; ++i) tab[i] = UngappedTableEntry(matrix, nl, i);
+                        uint32_t nl, const int *cutoff, const int *trigger, int table_bytes,
+                        int *out) {
+  PairTables<int8_t> t8;
+  PairTables<int16_t> t16;
+  for (uint32_t i = 0; i < kTableDim * kTableDim; ++i) {
+    PairTablesFill(t8, matrix, nl, i);
+    PairTablesFill(t16, matrix, nl, i);
+  }
   for (uint64_t k = 0; k < n; ++k)
-    out[k] = UngappedWithTrigger(qw, qpos[k], dw, dpos[k], tab, cutoff[k], trigger[k]);
+    out[k] = table_bytes == 1
+                 ? UngappedWithTrigger(qw, qpos[k], dw, dpos[k], &t8, cutoff[k], trigger[k])
+                 : UngappedWithTrigger(qw, qpos[k], dw, dpos[k], &t16, cutoff[k], trigger[k]);
 }
 
 void gzh_distance_words(const uint64_t *qw, const uint64_t *dw, uint64_t n,

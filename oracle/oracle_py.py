@@ -123,11 +123,62 @@ def ref():
         L.gzr_index_sizes.argtypes = [c_void_p, c_void_p]
         L.gzr_index_copy.argtypes = [c_void_p, c_void_p]
         L.gzr_index_free.argtypes = [c_void_p]
+        L.gzr_index_files.restype = None
+        L.gzr_index_files.argtypes = [c_void_p, c_uint32, c_int, c_uint32, c_int, c_void_p,
+                                      c_uint32, c_char_p, POINTER(c_void_p), POINTER(c_uint64),
+                                      POINTER(c_void_p), POINTER(c_uint64)]
+        L.gzr_free_bytes.argtypes = [c_void_p]
         _ref = L
     return _ref
 
 
 # ---- thin wrappers ------------------------------------------------------------
+def ref_index_files(seq, clustering, L, seed_threshold, matrix, nl,
+                    sets=b"A KR EDNQ C G H ILVM FYW P ST"):
+    """The reference's own .sdp and .scp bytes for one concatenated chunk
+    (SeedSearcherDatabaseParameters::Save / SaveCommonParameters)."""
+    R = ref()
+    seq = np.ascontiguousarray(seq, np.uint8)
+    m = np.ascontiguousarray(matrix, np.int32)
+    a, b = c_void_p(), c_void_p()
+    na, nb = c_uint64(), c_uint64()
+    R.gzr_index_files(ptr(seq), seq.size, int(clustering), L, seed_threshold, ptr(m), nl, sets,
+                      ctypes.byref(a), ctypes.byref(na), ctypes.byref(b), ctypes.byref(nb))
+    sdp = ctypes.string_at(a, na.value)
+    scp = ctypes.string_at(b, nb.value)
+    R.gzr_free_bytes(a)
+    R.gzr_free_bytes(b)
+    return sdp, scp
+
+
+def write_reference_database(prefix, chunks, max_length, total_length, total_sequences,
+                             clustering, L, seed_threshold, matrix, nl):
+    """Writes a database in the reference's file set: the chunk files that
+    DatabaseChunk::Save{Infomation,Offsets,Names,ConcatenatedSequence} write
+    (raw fields, src/database_chunk.h:403-447; one name per line), the
+    reference's own .sdp / .scp streams, and Database::SaveInformation's
+    <prefix>.inf (src/database.h:540-554).  chunks: (sequence codes, offsets,
+    names) per chunk."""
+    import struct
+    for i, (seq, offsets, names) in enumerate(chunks):
+        p = f"{prefix}_{i}"
+        seq = np.ascontiguousarray(seq, np.uint8)
+        with open(p + ".inf", "wb") as f:
+            f.write(struct.pack("<II", len(names), seq.size))
+        with open(p + ".off", "wb") as f:
+            f.write(np.ascontiguousarray(offsets, np.uint32).tobytes())
+        with open(p + ".nam", "w") as f:
+            for n in names:
+                f.write((n.decode() if isinstance(n, bytes) else n) + "\n")
+        with open(p + ".seq", "wb") as f:
+            f.write(seq.tobytes())
+        sdp, scp = ref_index_files(seq, clustering, L, seed_threshold, matrix, nl)
+        with open(p + ".sdp", "wb") as f:
+            f.write(sdp)
+        with open(p + ".scp", "wb") as f:
+            f.write(scp)
+    with open(prefix + ".inf", "wb") as f:
+        f.write(struct.pack("<iIQQB", len(chunks), max_length, total_length, total_sequences, 25))
 def gapped_one_side(lib_kind, seq, start, seq0_len, db, dstart, delim, reversed_, matrix, nl,
                     gap_open, gap_ext, cutoff, traceback=False):
     """Runs the oracle ('oracle') or the reference ('ref') gapped extender.
@@ -302,7 +353,8 @@ def hostpipe(path=None):
         L.gzh_force_stale.argtypes = [c_int]
         L.gzh_ungapped_words.restype = None
         L.gzh_ungapped_words.argtypes = [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p,
-                                         c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]
+                                         c_void_p, c_uint32, c_void_p, c_void_p, c_int,
+                                         c_void_p]
         L.gzh_distance_words.restype = None
         L.gzh_distance_words.argtypes = [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p,
                                          c_uint32, c_void_p, c_void_p]

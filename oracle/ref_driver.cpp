@@ -9,6 +9,7 @@
  * reference on randomised inputs, and to write the golden fixtures under
  * tests/golden/.  Nothing here is shipped or measured.
  */
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
 #include <string>
@@ -259,12 +260,11 @@ static void SkipHashFunction(std::istream &is) {
   f.Load(is);
 }
 
-void *gzr_build_index(const uint8_t *seq, uint32_t len, int clustering,
-                      uint32_t L, int seed_threshold, const int *matrix,
-                      uint32_t nl, const char *sets) {
+static void IndexBuildParameters(int clustering, uint32_t L, int seed_threshold,
+                                 const int *matrix, uint32_t nl, const char *sets,
+                                 SeedSearcherDatabaseParameters::BuildParameters &bp) {
   ProteinType t;
   AlphabetCoder coder(t);
-  SeedSearcherDatabaseParameters::BuildParameters bp;
   bp.clustering = clustering != 0;
   bp.subsequence_length = L;
   bp.sequence_delimiter = coder.GetMaxCode() + 1;
@@ -283,6 +283,38 @@ void *gzr_build_index(const uint8_t *seq, uint32_t len, int clustering,
     bp.similairty_code_map[c] = rc.Encode(coder.Decode(c));
   }
   bp.max_indexing_code = rc.GetMaxRegularLetterCode();
+}
+
+/* The bytes SeedSearcherDatabaseParameters::Save (the chunk's .sdp file)
+ * and SaveCommonParameters (its .scp file) write for one concatenated chunk
+ * (malloc'd; gzr_free_bytes). */
+void gzr_index_files(const uint8_t *seq, uint32_t len, int clustering, uint32_t L,
+                     int seed_threshold, const int *matrix, uint32_t nl, const char *sets,
+                     char **sdp, uint64_t *sdp_len, char **scp, uint64_t *scp_len) {
+  SeedSearcherDatabaseParameters::BuildParameters bp;
+  IndexBuildParameters(clustering, L, seed_threshold, matrix, nl, sets, bp);
+  std::vector<uint8_t> copy(seq, seq + len);
+  SeedSearcherDatabaseParameters p;
+  p.Build(&copy[0], len, bp);
+  std::stringstream a, b;
+  p.Save(a);
+  p.SaveCommonParameters(b);
+  const std::string sa = a.str(), sb = b.str();
+  *sdp = (char *)malloc(sa.size() + 1);
+  *scp = (char *)malloc(sb.size() + 1);
+  memcpy(*sdp, sa.data(), sa.size());
+  memcpy(*scp, sb.data(), sb.size());
+  *sdp_len = sa.size();
+  *scp_len = sb.size();
+}
+
+void gzr_free_bytes(char *p) { free(p); }
+
+void *gzr_build_index(const uint8_t *seq, uint32_t len, int clustering,
+                      uint32_t L, int seed_threshold, const int *matrix,
+                      uint32_t nl, const char *sets) {
+  SeedSearcherDatabaseParameters::BuildParameters bp;
+  IndexBuildParameters(clustering, L, seed_threshold, matrix, nl, sets, bp);
   std::vector<uint8_t> copy(seq, seq + len);
   SeedSearcherDatabaseParameters p;
   p.Build(&copy[0], len, bp);

@@ -3,7 +3,8 @@ code (oracle/_ref/libghostz_ref.so, compiled unmodified from the reference
 sources by oracle/Makefile).  Inputs are seeded; every fixture stores its
 inputs next to the reference outputs.
 
-  python tests/golden/make_golden.py        (needs oracle/_ref built)
+  python tests/golden/make_golden.py [fixture ...]   (needs oracle/_ref built;
+                                                     all fixtures by default)
 """
 import ctypes
 import os
@@ -21,6 +22,7 @@ import oracle_py  # noqa: E402
 from helpers import DELIM, related_pair_set, random_protein  # noqa: E402
 
 R = None
+REFERENCE = os.environ.get("GZ_REFERENCE", "/root/reference")
 
 
 def ref_matrix():
@@ -194,17 +196,42 @@ def db_index(path):
     np.savez_compressed(path, **out)
 
 
+def db_test_protein(directory):
+    """The reference test's test_protein.fa (test/test_protein.fa) as a
+    database in the reference's file set, chunked at 60 residues as
+    test/database_test.cpp:31 does: chunk files in the DatabaseChunk::Save
+    layout, .sdp / .scp from the reference's own SeedSearcherDatabaseParameters
+    (db defaults: -C T -L 10 -s 39)."""
+    import shutil
+    os.makedirs(directory, exist_ok=True)
+    fa = os.path.join(directory, "test_protein.fa")
+    shutil.copyfile(os.path.join(REFERENCE, "test", "test_protein.fa"), fa)
+    db = gz_native.Database.from_fasta(fa, chunk_size=60)
+    i = db.info()
+    chunks = []
+    for c in range(i.number_chunks):
+        ch = db.chunk(c)
+        chunks.append((ch["sequence"], ch["offsets"], ch["names"]))
+    m, nl = ref_matrix()
+    oracle_py.write_reference_database(os.path.join(directory, "test_protein"), chunks,
+                                       i.max_length, i.total_length, i.total_sequences,
+                                       True, 10, 39, m, nl)
+
+
 def main():
     global R
     if not oracle_py.ref_available():
         sys.exit("oracle/_ref is not built (the reference tree is needed once)")
     R = oracle_py.ref()
     R.gzr_read_matrix.restype = ctypes.c_int
-    gapped(os.path.join(HERE, "gapped_extender.npz"))
-    ungapped_distance_hash(os.path.join(HERE, "ungapped_distance_hash.npz"))
-    statistics(os.path.join(HERE, "statistics.npz"))
-    seg_translate(os.path.join(HERE, "seg_translate.npz"))
-    db_index(os.path.join(HERE, "db_index.npz"))
+    makers = {"gapped_extender.npz": gapped,
+              "ungapped_distance_hash.npz": ungapped_distance_hash,
+              "statistics.npz": statistics,
+              "seg_translate.npz": seg_translate,
+              "db_index.npz": db_index,
+              "db_test_protein": db_test_protein}
+    for name in sys.argv[1:] or list(makers):
+        makers[name](os.path.join(HERE, name))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -42,7 +42,7 @@ def test_struct_sizes_match_c():
     assert ctypes.sizeof(gz.ChunkView) == 128
     assert ctypes.sizeof(gz.AlignerParameters) == 64
     assert ctypes.sizeof(gz.GappedTrace) == 20
-    assert ctypes.sizeof(gz.AlignStats) == 224
+    assert ctypes.sizeof(gz.AlignStats) == 232
 
 
 def test_errors_are_reported_not_fatal():

@@ -8,7 +8,6 @@ import numpy as np
 
 import gz_native as gz
 import oracle_py
-import synthetic
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 DELIM = 25
@@ -100,10 +99,25 @@ def test_seg_translate_fixture_product():
         assert out.raw[:t] == g["frames"][fo[k]:fo[k + 1]].tobytes()
 
 
+def subjects_of_chunk(seq):
+    """Names, residues and offsets of the subjects in a chunk's concatenated
+    codes (leading delimiter, subject + delimiter ...), decoded to letters."""
+    t = gz.encode_table()
+    letters = np.zeros(32, np.uint8)
+    for c in range(ord("A"), ord("Z") + 1):
+        if t[c] < 25 and letters[t[c]] == 0:
+            letters[t[c]] = c
+    letters[t[ord("*")]] = ord("*")
+    cuts = np.flatnonzero(seq == DELIM)
+    parts = [seq[a + 1:b] for a, b in zip(cuts[:-1], cuts[1:])]
+    residues = letters[np.concatenate(parts)].tobytes()
+    offsets = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.uint64)
+    return [f"s{k}" for k in range(len(parts))], residues, offsets
+
+
 def test_database_index_fixture_product():
     g = load("db_index.npz")
-    (_, _, _), (dn, dr, do) = synthetic.protein_workload(20261206, 40, 8, qlen=(100, 200),
-                                                         homologs=6)
+    dn, dr, do = subjects_of_chunk(g["sequence"])
     for clustering in (1, 0):
         db = gz.Database(dn, dr, do, clustering=bool(clustering))
         ch = db.chunk(0)

@@ -37,6 +37,17 @@ def test_host_flow_is_clean_under_address_sanitizer():
     assert r.returncode == 0, r.stdout[-4000:]
 
 
+@pytest.mark.skipif(not os.path.exists(ASAN_LIB) or _libasan() is None,
+                    reason="AddressSanitizer build or runtime not available")
+def test_seed_word_walk_reads_stay_inside_the_packed_arrays():
+    # the pair walk's read-ahead on exactly sized arrays (ungapped.h)
+    env = dict(os.environ, LD_PRELOAD=_libasan(), ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "test_seed_words.py"),
+                        ASAN_LIB], env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=600)
+    assert r.returncode == 0 and "bounds cases done" in r.stdout, r.stdout[-4000:]
+
+
 @pytest.mark.skipif(not os.path.exists(TSAN_LIB) or _runtime("libtsan.so") is None,
                     reason="ThreadSanitizer build or runtime not available")
 def test_batched_pipeline_is_clean_under_thread_sanitizer():

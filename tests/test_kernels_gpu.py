@@ -170,7 +170,7 @@ def test_set_query_seeds_expansion(dev):
 
 
 def _gapped_gpu(dev, q, d, qpos, dpos, reverse, cutoff, gap_open=11, gap_ext=1, matrix=None,
-                nl=None, delim=DELIM, narrow=None, wide_first=None):
+                nl=None, delim=DELIM, narrow=None, wide_first=None, persistent=False):
     L = gz.lib()
     n = qpos.size
     _, qp = gz.pack_on_device(q)
@@ -184,6 +184,8 @@ def _gapped_gpu(dev, q, d, qpos, dpos, reverse, cutoff, gap_open=11, gap_ext=1, 
                                                        gap_ext, cutoff))
     if narrow is not None:
         gz.check(L.gz_gapped_extender_set_narrow_ring(h, narrow))
+    if persistent:
+        gz.check(L.gz_gapped_extender_set_persistent(h, 1))
     hq, hd = qpos.copy(), dpos.copy()
     hs = np.zeros(n, np.int32)
     d_q, d_d, d_s = gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n * 4), gz.DeviceBuffer(n * 4)
@@ -252,6 +254,27 @@ def test_gapped_kernel_ring_tiers(dev, narrow, wide_first):
         assert bad.size == 0, (bad[:5], got[bad[:5]], want[bad[:5]])
 
 
+@pytest.mark.parametrize("narrow", [16, 32])
+@pytest.mark.parametrize("cutoff", [26, 8200])
+def test_gapped_persistent_queue_vs_oracle(dev, narrow, cutoff):
+    """The persistent work-queue launch (lanes take the next job when their
+    extension ends) gives every job the oracle's score and end positions, in
+    both directions, with 16- and 32-bit cells (cutoff 8200 forces 32-bit
+    rings) and bands that outgrow the first ring."""
+    rng = np.random.default_rng(50 + narrow + cutoff)
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 2500, min_len=5, max_len=700,
+                                                identity=(0.3, 0.98), indel=0.05)
+    say("persistent queue: narrow", narrow, "cutoff", cutoff, "seeds", qpos.size)
+    for reverse in (False, True):
+        a, b = (qpos - 1, dpos - 1) if reverse else (qpos, dpos)
+        hs, hq, hd = _gapped_gpu(dev, q, d, a, b, reverse, cutoff, narrow=narrow,
+                                 persistent=True)
+        want = _gapped_oracle(q, d, a, b, reverse, cutoff, dev["matrix"], dev["nl"])
+        got = np.stack([hs, hq.astype(np.int64), hd.astype(np.int64)], 1)
+        bad = np.nonzero((got != want).any(1))[0]
+        assert bad.size == 0, (reverse, bad[:5], got[bad[:5]], want[bad[:5]])
+
+
 def test_gapped_kernel_ordered_jobs(dev):
     """Jobs run through an order array (the learned schedule's permutation)
     give every job the same score, end positions and DP cells as the
@@ -296,15 +319,24 @@ def test_gapped_kernel_ordered_jobs(dev):
 
     for reverse in (False, True):
         want = run(reverse, None)
-        for order in (rng.permutation(n), np.arange(n)[::-1]):
-            got = run(reverse, order)
+        for persistent in (0, 1):
+            # the static grid and the persistent work-queue launch
+            gz.check(L.gz_gapped_extender_set_persistent(h, persistent))
+            for order in (rng.permutation(n), np.arange(n)[::-1]):
+                got = run(reverse, order)
+                for g, w in zip(got, want):
+                    assert np.array_equal(g, w)
+            got = run(reverse, None)
             for g, w in zip(got, want):
                 assert np.array_equal(g, w)
+            gz.check(L.gz_gapped_extender_set_persistent(h, 0))
         assert failures() == 0
-    bad = rng.permutation(n)
-    bad[7] = n + 5  # out of range: that slot is skipped and counted
-    got = run(False, bad)
-    assert failures() == 1
+    for persistent in (0, 1):
+        gz.check(L.gz_gapped_extender_set_persistent(h, persistent))
+        bad = rng.permutation(n)
+        bad[7] = n + 5  # out of range: that slot is skipped and counted
+        got = run(False, bad)
+        assert failures() == 1
     L.gz_gapped_extender_destroy(h)
 
 

@@ -223,3 +223,20 @@ def test_seed_hit_buffer_growth_matches_oracle(matrix, monkeypatch):
     st, _ = run_case("hit-buffer-growth", q, db, m, nl,
                      schedules=(gz.SCHEDULE_PLAIN, gz.SCHEDULE_LEARNED))
     assert st["seed_hits"] > 64 * 20
+
+
+def test_uneven_queries_at_the_batched_shape_match_oracle(matrix):
+    """The uneven-length protein workload at a reduced size: 300 queries of
+    Pareto-tailed lengths (30 to 35,000 aa, median 250, 20% planted) against
+    5,000 proteins.  Long queries carry long homologs whose bands outgrow both
+    LDS rings; every form, including the batched persistent one, gives the
+    oracle's text."""
+    m, nl = matrix
+    (qn, qr, qo), (dn, dr, do) = synthetic.uneven_protein_workload(20261212, 5000, 300)
+    lens = np.diff(qo.astype(np.int64))
+    say("uneven 300 x 5000: max query", lens.max(), "median", np.median(lens))
+    assert lens.max() == 35000
+    q = gz.Queries(qn, qr, qo)
+    db = gz.Database(dn, dr, do)
+    run_case("uneven300x5000", q, db, m, nl,
+             schedules=(gz.SCHEDULE_PLAIN, gz.SCHEDULE_LEARNED))

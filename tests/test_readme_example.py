@@ -134,3 +134,29 @@ def test_gpu_forms_match_reference_output():
             got, _ = al.align(q)
             assert rows(got) == expected_rows(), (sch, bat)
             check_names(got)
+
+
+@pytest.mark.gpu
+def test_gpu_align_main_on_saved_database_matches_reference_output(tmp_path):
+    """`aln -d <prefix>` on the database files (gz_database_save of the
+    README database) gives the README's output."""
+    gz.check(gz.lib().gz_set_device(0))
+    prefix = str(tmp_path / "exdb")
+    gz.Database.from_fasta(DB).save(prefix)
+    out = str(tmp_path / "exout")
+    gz.check(gz.lib().gz_align_main(QUERIES.encode(), prefix.encode(), out.encode(), 1,
+                                    gz.default_parameters()))
+    text = open(out, "rb").read()
+    assert rows(text) == expected_rows()
+    check_names(text)
+
+
+def test_align_main_reports_missing_inputs(tmp_path):
+    """AlignMain::Run's checks (src/align_main.cpp:46-62), before any device
+    work."""
+    out = str(tmp_path / "o").encode()
+    p = gz.default_parameters()
+    st = gz.lib().gz_align_main(str(tmp_path / "none.fa").encode(), b"x", out, 1, p)
+    assert st == 4 and gz.lib().gz_last_error() == b"Query file does not exist."
+    st = gz.lib().gz_align_main(QUERIES.encode(), str(tmp_path / "nodb").encode(), out, 1, p)
+    assert st == 4 and gz.lib().gz_last_error() == b"Database file does not exist."
