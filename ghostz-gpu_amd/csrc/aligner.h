@@ -75,6 +75,12 @@ class Aligner : public PresearchDevice {
   int GappedExtend(size_t chunk, uint32_t n_reverse, uint32_t n_forward, uint32_t *qpos,
                    uint32_t *dpos, int *scores, uint32_t *cells, const uint16_t *keys,
                    bool learned, AlignStats *st, const uint32_t *bucket_off) override;
+  int GappedPrepare(int slot, size_t chunk, uint32_t n_reverse, uint32_t n_forward,
+                    const uint32_t *qpos, const uint32_t *dpos, const uint16_t *keys,
+                    bool learned, const uint32_t *bucket_off) override;
+  int GappedRun(int slot, size_t chunk, uint32_t n_reverse, uint32_t n_forward, uint32_t *qpos,
+                uint32_t *dpos, int *scores, uint32_t *cells, AlignStats *st,
+                const uint32_t *bucket_off) override;
   void GraphCounts(uint64_t *instantiations, uint64_t *updates) const {
     *instantiations = graph_instantiations_;
     *updates = graph_updates_;
@@ -112,20 +118,32 @@ class Aligner : public PresearchDevice {
   size_t lookup_chunk_ = kNoChunk;  // database chunk of the seed stage's lookup
   std::vector<void *> query_buffers_;
   unsigned seed_kinds_ = 3;
-  // timing events: [0, 4) the gapped stage's, [4, 6) the result build's (the
-  // two stages may run on different host threads)
+  // timing events: [0, 2) the gapped kernels', [4, 6) the result build's
+  // (the stages may run on different host threads)
   hipEvent_t ev_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  // gapped-stage workspace (grown on demand)
-  void *gapped_ws_ = nullptr;
-  size_t gapped_ws_cap_ = 0;
+  // The gapped stage's two pipeline slots: GappedPrepare uploads a slot's
+  // jobs and orders them on the schedule stream, GappedRun extends them on
+  // the extension stream once the slot's `ready` event has passed, so one
+  // slot's ordering overlaps the other's extension.
+  struct GappedSlot {
+    void *ws = nullptr;  // positions, scores, cells, keys, permutation, scratch
+    size_t cap = 0;
+    size_t n = 0;
+    bool order = false, gated = false;
+    uint32_t *pq = nullptr, *pd = nullptr, *cells = nullptr, *perm = nullptr;
+    int *sc = nullptr;
+    uint32_t *counts = nullptr;  // pinned: the two sides' gated job counts
+    hipEvent_t ready = nullptr, ev_begin = nullptr, ev_end = nullptr;
+  };
+  GappedSlot slots_[2];
+  hipStream_t schedule_stream_ = nullptr;
   // result-build workspace
   void *tb_ws_ = nullptr;
   size_t tb_ws_cap_ = 0;
   uint32_t max_query_length_ = 0;
   // pinned job buffers per pipeline slot
   std::vector<std::pair<void *, size_t> > pinned_jobs_;
-  // the learned schedule's gate (query side residues) and the pinned counts
-  // of gated jobs read back per call
+  // the learned schedule's gate (query side residues)
   uint32_t learned_gate_ = 44;
   GappedTuner tuner_;
   std::mutex tuner_mutex_;
@@ -136,8 +154,6 @@ class Aligner : public PresearchDevice {
   hipGraphExec_t graph_exec_[kLengthBuckets * GappedTuner::kConfigs] = {};
   uint64_t graph_updates_ = 0, graph_instantiations_ = 0;
   hipError_t LaunchCaptured(int slot, hipStream_t s, const std::function<bool()> &launch);
-  uint32_t *gated_counts_ = nullptr;
-  uint32_t *GatedCountsHost();
 };
 
 }  // namespace gz

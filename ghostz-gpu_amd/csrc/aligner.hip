@@ -80,9 +80,13 @@ Aligner::Aligner(const AlignerOptions &opt) : opt_(opt) {
 Aligner::~Aligner() {
   EndQueryChunk();
   chunks_.clear();
-  if (gapped_ws_) (void)gz::Free(gapped_ws_);
+  for (GappedSlot &S : slots_) {
+    if (S.ws) (void)gz::Free(S.ws);
+    if (S.counts) (void)gz::HostFree(S.counts);
+    for (hipEvent_t e : {S.ready, S.ev_begin, S.ev_end})
+      if (e) (void)hipEventDestroy(e);
+  }
   if (tb_ws_) (void)gz::Free(tb_ws_);
-  if (gated_counts_) (void)gz::HostFree(gated_counts_);
   for (auto &b : pinned_jobs_)
     if (b.first) (void)gz::HostFree(b.first);
   for (hipEvent_t e : ev_)
@@ -97,6 +101,7 @@ Aligner::~Aligner() {
   if (stream_) (void)hipStreamDestroy(stream_);
   if (extend_stream_) (void)hipStreamDestroy(extend_stream_);
   if (traceback_stream_) (void)hipStreamDestroy(traceback_stream_);
+  if (schedule_stream_) (void)hipStreamDestroy(schedule_stream_);
 }
 
 hipError_t Aligner::SetDatabase(const Database *db) {
@@ -105,6 +110,8 @@ hipError_t Aligner::SetDatabase(const Database *db) {
   if (!extend_stream_) GZ_TRY(hipStreamCreateWithFlags(&extend_stream_, hipStreamNonBlocking));
   if (!traceback_stream_)
     GZ_TRY(hipStreamCreateWithFlags(&traceback_stream_, hipStreamNonBlocking));
+  if (!schedule_stream_)
+    GZ_TRY(hipStreamCreateWithFlags(&schedule_stream_, hipStreamNonBlocking));
   chunks_.clear();
   for (const DbChunk &c : db->chunks) {
     std::unique_ptr<DeviceChunk> d(new DeviceChunk());
@@ -267,16 +274,31 @@ void GappedTuner::Record(int bucket, int config, double kernel_ms, double njobs)
 int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, uint32_t *dpos,
                           int *scores, uint32_t *cells, const uint16_t *keys, bool learned,
                           AlignStats *st, const uint32_t *bucket_off) {
-  const size_t n = (size_t)nr + nf;
-  if (n == 0) return 0;
-  hipStream_t s = extend_stream_;
-  for (int i = 0; i < 4; ++i)
-    if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
+  const int r = GappedPrepare(0, ci, nr, nf, qpos, dpos, keys, learned, bucket_off);
+  if (r != 0) return r;
+  return GappedRun(0, ci, nr, nf, qpos, dpos, scores, cells, st, bucket_off);
+}
+
+int Aligner::GappedPrepare(int slot, size_t ci, uint32_t nr, uint32_t nf, const uint32_t *qpos,
+                           const uint32_t *dpos, const uint16_t *keys, bool learned,
+                           const uint32_t *bucket_off) {
+  GappedSlot &S = slots_[slot];
+  S.n = (size_t)nr + nf;
+  S.order = false;
+  S.gated = false;
+  if (S.n == 0) return 0;
+  const size_t n = S.n;
+  hipStream_t s = schedule_stream_;
+  if (!S.ready) {
+    GZ_TRY(hipEventCreateWithFlags(&S.ready, hipEventDisableTiming));
+    GZ_TRY(hipEventCreate(&S.ev_begin));
+    GZ_TRY(hipEventCreate(&S.ev_end));
+  }
+  if (!S.counts) GZ_TRY(gz::HostMalloc(&S.counts, 64));
   const DeviceChunk &dc = *chunks_[ci];
-  if (gz_gapped_extender_set_database(gapped_, dc.packed) != GZ_OK) return (int)hipErrorUnknown;
   const GpuCostModel *model =
       learned ? dynamic_cast<const GpuCostModel *>(opt_.cost_model) : nullptr;
-  const bool order = keys != nullptr || model != nullptr;
+  S.order = keys != nullptr || model != nullptr;
   // workspace: positions, scores, cells; and for an ordered run the keys,
   // the permutation, predicted costs and the sort scratch (the extension
   // kernels read the jobs through the permutation)
@@ -285,31 +307,32 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
   // inside their length bins (schedule.h)
   const bool gated = model != nullptr && keys != nullptr;
   size_t bytes = 4 * a4;
-  if (order) bytes += a2 + 2 * a4 + AlignUp(ScheduleTempBytes(n), 256);
+  if (S.order) bytes += a2 + 2 * a4 + AlignUp(ScheduleTempBytes(n), 256);
   if (gated) bytes += a2 + a4 + 256;
-  if (gapped_ws_cap_ < bytes) {
-    if (gapped_ws_) (void)gz::Free(gapped_ws_);
-    gapped_ws_ = nullptr;
-    gapped_ws_cap_ = 0;
-    GZ_TRY(gz::Malloc(&gapped_ws_, bytes));
-    gapped_ws_cap_ = bytes;
+  if (S.cap < bytes) {
+    if (S.ws) (void)gz::Free(S.ws);
+    S.ws = nullptr;
+    S.cap = 0;
+    GZ_TRY(gz::Malloc(&S.ws, bytes));
+    S.cap = bytes;
   }
-  char *w = (char *)gapped_ws_;
+  char *w = (char *)S.ws;
   auto take = [&](size_t b) {
     char *p = w;
     w += b;
     return p;
   };
-  uint32_t *d_pq = (uint32_t *)take(a4), *d_pd = (uint32_t *)take(a4);
-  int *d_sc = (int *)take(a4);
-  uint32_t *d_cells = (uint32_t *)take(a4);
+  S.pq = (uint32_t *)take(a4);
+  S.pd = (uint32_t *)take(a4);
+  S.sc = (int *)take(a4);
+  S.cells = (uint32_t *)take(a4);
   uint16_t *d_keys = nullptr;
-  uint32_t *d_perm = nullptr;
   float *d_pred = nullptr;
   void *d_tmp = nullptr;
-  if (order) {
+  S.perm = nullptr;
+  if (S.order) {
     d_keys = (uint16_t *)take(a2);
-    d_perm = (uint32_t *)take(a4);
+    S.perm = (uint32_t *)take(a4);
     d_pred = (float *)take(a4);
     d_tmp = take(AlignUp(ScheduleTempBytes(n), 256));
   }
@@ -320,46 +343,42 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
     d_index = (uint32_t *)take(a4);
     d_count = (uint32_t *)take(256);
   }
-  hipError_t e = hipMemcpyAsync(d_pq, qpos, n * 4, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_pd, dpos, n * 4, hipMemcpyHostToDevice, s);
-  uint32_t *rc = cells ? d_cells : nullptr;
-  bool gated_counted = false;
-  if (e == hipSuccess && order) {
+  hipError_t e = hipMemcpyAsync(S.pq, qpos, n * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(S.pd, dpos, n * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && S.order) {
     // order the jobs by descending key inside each side, on the device
-    e = hipEventRecord(ev_[2], s);
+    e = hipEventRecord(S.ev_begin, s);
     if (e == hipSuccess && gated) {
       // length bins for every job, the model's prediction bins for the
       // jobs whose side may run long
       e = hipMemcpyAsync(d_len, keys, n * 2, hipMemcpyHostToDevice, s);
-      if (e == hipSuccess) e = LengthBinKeys(d_len, (uint32_t)n, d_keys, s);
       // the gated counts stay on the device: the launches are sized for
       // every job of a side and read the count (no wait in the stream)
       const uint32_t off[2] = {0, nr}, cnt[2] = {nr, nf};
       for (int k = 0; k < 2 && e == hipSuccess; ++k)
-        e = GateJobs(d_len + off[k], cnt[k], learned_gate_, d_index + off[k], d_count + k, s);
+        e = BinAndGateJobs(d_len + off[k], cnt[k], learned_gate_, d_keys + off[k],
+                           d_index + off[k], d_count + k, s);
       for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-        e = model->PredictIndexedOnDevice(dq_.packed, dc.packed, d_pq + off[k], d_pd + off[k],
+        e = model->PredictIndexedOnDevice(dq_.packed, dc.packed, S.pq + off[k], S.pd + off[k],
                                           d_index + off[k], cnt[k], k == 0, d_pred + off[k], s,
                                           d_count + k);
         if (e == hipSuccess)
           e = AddPredictionBins(d_pred + off[k], d_index + off[k], cnt[k], d_keys + off[k], s,
                                 d_count + k);
       }
-      uint32_t *hc = GatedCountsHost();
-      if (e == hipSuccess && !hc) e = hipErrorOutOfMemory;
-      if (e == hipSuccess) e = hipMemcpyAsync(hc, d_count, 8, hipMemcpyDeviceToHost, s);
-      gated_counted = true;
+      if (e == hipSuccess) e = hipMemcpyAsync(S.counts, d_count, 8, hipMemcpyDeviceToHost, s);
+      S.gated = true;
     } else if (e == hipSuccess && model) {
       // the host lays out job nr + k as the forward side of job k (both
       // positions one later), so one feature pass serves both sides; were
       // that not so, only the order (never a result) would differ
       if (nr == nf)
-        e = model->PredictBothSidesOnDevice(dq_.packed, dc.packed, d_pq, d_pd, nr, d_pred,
+        e = model->PredictBothSidesOnDevice(dq_.packed, dc.packed, S.pq, S.pd, nr, d_pred,
                                             d_pred + nr, s);
       else
-        e = model->PredictOnDevice(dq_.packed, dc.packed, d_pq, d_pd, nr, true, d_pred, s);
+        e = model->PredictOnDevice(dq_.packed, dc.packed, S.pq, S.pd, nr, true, d_pred, s);
       if (e == hipSuccess && nr != nf)
-        e = model->PredictOnDevice(dq_.packed, dc.packed, d_pq + nr, d_pd + nr, nf, false,
+        e = model->PredictOnDevice(dq_.packed, dc.packed, S.pq + nr, S.pd + nr, nf, false,
                                    d_pred + nr, s);
       // log2 cells in steps of 1/32
       if (e == hipSuccess) e = CostToKey(d_pred, (uint32_t)n, 32.f, d_keys, s);
@@ -372,15 +391,35 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
       for (int side = 0; side < 2 && e == hipSuccess; ++side)
         for (int b = 0; b < kLengthBuckets && e == hipSuccess; ++b) {
           const size_t o = (side ? nr : 0) + bucket_off[b];
-          e = DeviceOrderByKey(d_keys + o, bucket_off[b + 1] - bucket_off[b], d_perm + o, d_tmp,
+          e = DeviceOrderByKey(d_keys + o, bucket_off[b + 1] - bucket_off[b], S.perm + o, d_tmp,
                                s);
         }
     } else {
-      if (e == hipSuccess) e = DeviceOrderByKey(d_keys, nr, d_perm, d_tmp, s);
-      if (e == hipSuccess) e = DeviceOrderByKey(d_keys + nr, nf, d_perm + nr, d_tmp, s);
+      if (e == hipSuccess) e = DeviceOrderByKey(d_keys, nr, S.perm, d_tmp, s);
+      if (e == hipSuccess) e = DeviceOrderByKey(d_keys + nr, nf, S.perm + nr, d_tmp, s);
     }
-    if (e == hipSuccess) e = hipEventRecord(ev_[3], s);
+    if (e == hipSuccess) e = hipEventRecord(S.ev_end, s);
   }
+  // the extension stream waits for this event before it reads the slot
+  if (e == hipSuccess) e = hipEventRecord(S.ready, s);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos,
+                       uint32_t *dpos, int *scores, uint32_t *cells, AlignStats *st,
+                       const uint32_t *bucket_off) {
+  GappedSlot &S = slots_[slot];
+  const size_t n = S.n;
+  if (n == 0) return 0;
+  if (n != (size_t)nr + nf) return (int)hipErrorInvalidValue;
+  hipStream_t s = extend_stream_;
+  for (int i = 0; i < 2; ++i)
+    if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
+  const DeviceChunk &dc = *chunks_[ci];
+  if (gz_gapped_extender_set_database(gapped_, dc.packed) != GZ_OK) return (int)hipErrorUnknown;
+  const bool order = S.order;
+  uint32_t *rc = cells ? S.cells : nullptr;
+  hipError_t e = hipStreamWaitEvent(s, S.ready, 0);
   // each side's jobs run in the permutation's order (when ordered) and
   // leave their results at their own indices
   gz_stream_s wrap{s};
@@ -388,10 +427,10 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
     uint32_t *c = rc ? rc + off : nullptr;
     const gz_status r =
         order ? gz_gapped_extender_extend_one_side_score_only_ordered_async(
-                    gapped_, count, reverse, d_perm + off, d_pq + off, d_pd + off, d_sc + off,
+                    gapped_, count, reverse, S.perm + off, S.pq + off, S.pd + off, S.sc + off,
                     c, &wrap)
               : gz_gapped_extender_extend_one_side_score_only_cells_async(
-                    gapped_, count, reverse, d_pq + off, d_pd + off, d_sc + off, c, &wrap);
+                    gapped_, count, reverse, S.pq + off, S.pd + off, S.sc + off, c, &wrap);
     return r == GZ_OK;
   };
   if (e == hipSuccess) e = hipEventRecord(ev_[0], s);
@@ -434,18 +473,19 @@ int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, u
     if (e == hipSuccess && nf > 0 && !side(nr, nf, 0)) e = hipErrorUnknown;
   }
   if (e == hipSuccess) e = hipEventRecord(ev_[1], s);
-  if (e == hipSuccess) e = hipMemcpyAsync(qpos, d_pq, n * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(dpos, d_pd, n * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(scores, d_sc, n * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && cells) e = hipMemcpyAsync(cells, d_cells, n * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(qpos, S.pq, n * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dpos, S.pd, n * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(scores, S.sc, n * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && cells) e = hipMemcpyAsync(cells, S.cells, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e == hipSuccess && gated_counted) st->gated_jobs += (uint64_t)gated_counts_[0] + gated_counts_[1];
+  // the slot's schedule ran before its ready event, which this stream waited for
+  if (e == hipSuccess && S.gated) st->gated_jobs += (uint64_t)S.counts[0] + S.counts[1];
   uint32_t fails = 0;
   if (e == hipSuccess && (gz_gapped_extender_failures(gapped_, &fails) != GZ_OK || fails != 0))
     e = hipErrorUnknown;
   float ms = 0, ms2 = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[0], ev_[1]);
-  if (e == hipSuccess && order) e = hipEventElapsedTime(&ms2, ev_[2], ev_[3]);
+  if (e == hipSuccess && order) e = hipEventElapsedTime(&ms2, S.ev_begin, S.ev_end);
   for (int b = 0; b < kLengthBuckets && e == hipSuccess; ++b) {
     if (tuned[b] < 0) continue;
     float bm = 0;
@@ -546,12 +586,6 @@ hipError_t Aligner::LaunchCaptured(int slot, hipStream_t s, const std::function<
   return e;
 }
 
-uint32_t *Aligner::GatedCountsHost() {
-  if (!gated_counts_ &&
-      gz::HostMalloc((void **)&gated_counts_, 64, hipHostMallocDefault) != hipSuccess)
-    gated_counts_ = nullptr;
-  return gated_counts_;
-}
 
 uint8_t *Aligner::JobBuffer(size_t slot, size_t bytes) {
   if (pinned_jobs_.size() <= slot) pinned_jobs_.resize(slot + 1, std::make_pair(nullptr, 0));

@@ -77,32 +77,34 @@ __device__ __forceinline__ void FeatureWindow(const uint64_t *packed, uint32_t s
 
 // CostFeatures (cost_model.cpp) on packed codes, one thread per (seed,
 // window): windows along the extension (query, subject) then away from it
-// (query, subject).
+// (query, subject).  The grid is bounded and strides over the rows, so a
+// launch sized for more rows than the device-side count holds costs little.
 __global__ void CostFeatureKernel(const uint64_t *pq, const uint64_t *pd, const uint32_t *qpos,
                                   const uint32_t *dpos, const uint32_t *index, uint32_t n,
                                   const uint32_t *count, uint32_t count_offset, int reverse,
                                   uint8_t *out) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (count) {  // rows beyond *count - count_offset are not computed
     const uint32_t c = *count;
     n = c > count_offset ? min(n, c - count_offset) : 0u;
   }
-  if (t >= 4u * n) return;
-  const uint32_t i = t >> 2, wi = t & 3u;
-  const uint32_t job = index ? index[i] : i;  // row i holds job index[i]
-  const uint64_t *seq = (wi & 1u) ? pd : pq;
-  const uint32_t pos = (wi & 1u) ? dpos[job] : qpos[job];
-  // along: from pos in the extension's direction; away: from the next
-  // position the other way
-  const bool along = wi < 2u;
-  const bool backward = along == (reverse != 0);
-  const uint32_t start = along ? pos : (reverse ? pos + 1 : pos - 1);
-  uint32_t codes[8];
-  if (backward) FeatureWindow<true>(seq, start, codes);
-  else FeatureWindow<false>(seq, start, codes);
-  uint4 *dst = reinterpret_cast<uint4 *>(out + (size_t)i * kCostResidues + wi * kCostWindow);
-  dst[0] = make_uint4(codes[0], codes[1], codes[2], codes[3]);
-  dst[1] = make_uint4(codes[4], codes[5], codes[6], codes[7]);
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 4ull * n;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t i = (uint32_t)(t >> 2), wi = (uint32_t)t & 3u;
+    const uint32_t job = index ? index[i] : i;  // row i holds job index[i]
+    const uint64_t *seq = (wi & 1u) ? pd : pq;
+    const uint32_t pos = (wi & 1u) ? dpos[job] : qpos[job];
+    // along: from pos in the extension's direction; away: from the next
+    // position the other way
+    const bool along = wi < 2u;
+    const bool backward = along == (reverse != 0);
+    const uint32_t start = along ? pos : (reverse ? pos + 1 : pos - 1);
+    uint32_t codes[8];
+    if (backward) FeatureWindow<true>(seq, start, codes);
+    else FeatureWindow<false>(seq, start, codes);
+    uint4 *dst = reinterpret_cast<uint4 *>(out + (size_t)i * kCostResidues + wi * kCostWindow);
+    dst[0] = make_uint4(codes[0], codes[1], codes[2], codes[3]);
+    dst[1] = make_uint4(codes[4], codes[5], codes[6], codes[7]);
+  }
 }
 
 }  // namespace
@@ -112,8 +114,9 @@ hipError_t CostFeaturesOnDevice(const uint64_t *pq, const uint64_t *pd, const ui
                                 hipStream_t s, const uint32_t *index, const uint32_t *d_count,
                                 uint32_t count_offset) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(CostFeatureKernel, dim3((4 * (size_t)n + 255) / 256), dim3(256), 0, s, pq, pd,
-                     qpos, dpos, index, n, d_count, count_offset, reverse ? 1 : 0, out);
+  const size_t blocks = std::min<size_t>((4 * (size_t)n + 255) / 256, 4096);
+  hipLaunchKernelGGL(CostFeatureKernel, dim3((unsigned)blocks), dim3(256), 0, s, pq, pd, qpos,
+                     dpos, index, n, d_count, count_offset, reverse ? 1 : 0, out);
   return hipGetLastError();
 }
 

@@ -211,8 +211,13 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     uint32_t *aq = nullptr, *ad = nullptr;
     int *sc = nullptr;
     uint32_t G = 0;
+    std::vector<uint16_t> keys;  // query side lengths (ordered schedules)
   };
-  auto gapped = [&](size_t ci, Block &B) -> int {
+  // the order the device runs the jobs in never changes their results
+  const bool learned = opt.schedule == Schedule::kLearned && opt.cost_model != nullptr;
+  // 3a. the jobs of a block laid out and handed to the device (pipeline
+  // slot `slot`), which may start ordering them
+  auto gapped_prepare = [&](size_t ci, Block &B, int slot) -> int {
     const size_t n = B.qe - B.qb;
     B.gstart.assign(n + 1, 0);
     if (opt.batched) {
@@ -246,7 +251,8 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     B.sc = reinterpret_cast<int *>(B.ad + jobs);
     // query side lengths: the length-sorted schedule's keys, and the length
     // bins of the learned schedule
-    std::vector<uint16_t> keys;
+    std::vector<uint16_t> &keys = B.keys;
+    keys.clear();
     if (opt.schedule == Schedule::kLengthSorted || opt.schedule == Schedule::kLearned)
       keys.resize(jobs);
     ParallelBlocks(n, opt.batched ? std::max(1, threads / 4) : threads, 256,
@@ -270,13 +276,20 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
         }
       }
     });
-    // the order the device runs the jobs in never changes their results
-    const bool learned = opt.schedule == Schedule::kLearned && opt.cost_model != nullptr;
+    return dev->GappedPrepare(slot, ci, G, G, B.aq, B.ad, keys.empty() ? nullptr : keys.data(),
+                              learned, opt.batched ? B.bucket_off : nullptr);
+  };
+  // 3b. the block's gapped extensions: extension ends and scores back in
+  // the block's arrays
+  auto gapped_run = [&](size_t ci, Block &B, int slot) -> int {
+    const size_t n = B.qe - B.qb;
+    const uint32_t G = B.G;
+    if (G == 0) return 0;
+    const size_t jobs = 2 * (size_t)G;
     std::vector<uint32_t> cells(opt.record_cells ? jobs : 0);
-    const int r = dev->GappedExtend(ci, G, G, B.aq, B.ad, B.sc,
-                                    cells.empty() ? nullptr : cells.data(),
-                                    keys.empty() ? nullptr : keys.data(), learned, st,
-                                    opt.batched ? B.bucket_off : nullptr);
+    const int r = dev->GappedRun(slot, ci, G, G, B.aq, B.ad, B.sc,
+                                 cells.empty() ? nullptr : cells.data(), st,
+                                 opt.batched ? B.bucket_off : nullptr);
     if (r != 0) return r;
     if (!cells.empty()) {
       for (int side = 0; side < 2; ++side)
@@ -448,7 +461,8 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       Block all;
       all.qb = 0;
       all.qe = nq;
-      GZ_STEP(gapped(ci, all));
+      GZ_STEP(gapped_prepare(ci, all, 0));
+      GZ_STEP(gapped_run(ci, all, 0));
       st->seconds_gapped += Now() - t;
       t = Now();
       add(ci, all);
@@ -509,7 +523,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   std::vector<std::vector<BuildPart> > block_parts(nb);
   std::vector<size_t> stale_chunks;
   bool rebuild = false;  // a block could not be built on its own
-  double sec_seed = 0, sec_gapped = 0, sec_chain = 0, sec_add = 0, sec_build = 0,
+  double sec_seed = 0, sec_schedule = 0, sec_gapped = 0, sec_chain = 0, sec_add = 0, sec_build = 0,
          sec_build_host = 0, sec_output = 0;
   uint64_t seed_hits = 0, host_jobs = 0;
   // The pipeline runs over every (database chunk, block) pair as one
@@ -527,7 +541,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   std::vector<SeedHitView> views(ntasks);
   std::mutex mu;
   std::condition_variable cv;
-  size_t seeded = 0, chained_upto = 0, gapped_upto = 0, added = 0, traced = 0;
+  size_t seeded = 0, chained_upto = 0, prepared = 0, gapped_upto = 0, added = 0, traced = 0;
   int status = 0;
   bool unsupported = false;
   auto wait_for = [&](const size_t &counter, size_t value) {
@@ -559,13 +573,28 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       if (r != 0) return;
     }
   });
-  // the extender and the builder drive their own device streams;
-  // their host work overlaps
-  std::thread extender([&]() {
+  // the scheduler lays task t's jobs out and hands them to the device,
+  // which orders them on its own stream while the extender's task t - 1
+  // runs; the extender, the scheduler and the builder drive their own
+  // device streams, and their host work overlaps.  Task t uses the device's
+  // pipeline slot t % 2, free once task t - 2 has been extended.
+  std::thread scheduler([&]() {
     for (size_t t = 0; t < ntasks; ++t) {
       if (!wait_for(chained_upto, t + 1)) return;
+      if (t >= 2 && !wait_for(gapped_upto, t - 1)) return;
       const double t0 = Now();
-      const int r = gapped(t / nb, blocks[t % nb]);
+      const int r = gapped_prepare(t / nb, blocks[t % nb], (int)(t % 2));
+      sec_schedule += Now() - t0;
+      trace.Add("schedule", t, t0, Now());
+      publish(prepared, t + 1, r);
+      if (r != 0) return;
+    }
+  });
+  std::thread extender([&]() {
+    for (size_t t = 0; t < ntasks; ++t) {
+      if (!wait_for(prepared, t + 1)) return;
+      const double t0 = Now();
+      const int r = gapped_run(t / nb, blocks[t % nb], (int)(t % 2));
       sec_gapped += Now() - t0;
       trace.Add("gapped", t, t0, Now());
       publish(gapped_upto, t + 1, r);
@@ -644,6 +673,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     publish(chained_upto, t + 1, 0);
   }
   seeder.join();
+  scheduler.join();
   extender.join();
   builder.join();
   adder.join();
@@ -663,7 +693,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   st->seed_hits += seed_hits;
   st->seconds_seed += sec_seed;
   st->seconds_chain += sec_chain;
-  st->seconds_gapped += sec_gapped;
+  st->seconds_gapped += sec_gapped + sec_schedule;
   st->seconds_add += sec_add;
   st->seconds_build += sec_build + sec_build_host;
   st->seconds_output += sec_output;

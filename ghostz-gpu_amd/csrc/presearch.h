@@ -142,6 +142,31 @@ class PresearchDevice {
                            uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
                            const uint16_t *keys, bool learned, AlignStats *st,
                            const uint32_t *bucket_off = nullptr) = 0;
+  // GappedExtend in two halves for pipelined callers: GappedPrepare hands
+  // the jobs of pipeline slot `slot` (0 or 1) to the device, which may start
+  // ordering them; GappedRun extends them (same arguments) and writes the
+  // results back.  Prepare(slot) may overlap Run of the other slot; the
+  // host arrays stay untouched until Run(slot) returns.  The defaults run
+  // the whole extension in GappedRun.
+  virtual int GappedPrepare(int slot, size_t chunk, uint32_t n_reverse, uint32_t n_forward,
+                            const uint32_t *qpos, const uint32_t *dpos, const uint16_t *keys,
+                            bool learned, const uint32_t *bucket_off) {
+    (void)chunk;
+    (void)n_reverse;
+    (void)n_forward;
+    (void)qpos;
+    (void)dpos;
+    pending_keys_[slot & 1] = keys;
+    pending_learned_[slot & 1] = learned;
+    (void)bucket_off;
+    return 0;
+  }
+  virtual int GappedRun(int slot, size_t chunk, uint32_t n_reverse, uint32_t n_forward,
+                        uint32_t *qpos, uint32_t *dpos, int *scores, uint32_t *cells,
+                        AlignStats *st, const uint32_t *bucket_off) {
+    return GappedExtend(chunk, n_reverse, n_forward, qpos, dpos, scores, cells,
+                        pending_keys_[slot & 1], pending_learned_[slot & 1], st, bucket_off);
+  }
   // Result build of `jobs` of database chunk `chunk` with the result cutoff.
   // Returns kTracebackUnsupported when the device does not build results.
   virtual int Tracebacks(size_t chunk, const std::vector<TracebackJob> &jobs, int cutoff,
@@ -167,6 +192,8 @@ class PresearchDevice {
 
  private:
   std::vector<std::vector<uint64_t> > job_buffers_;
+  const uint16_t *pending_keys_[2] = {nullptr, nullptr};
+  bool pending_learned_[2] = {false, false};
 };
 
 // Aligns all queries (split into chunks by queries_chunk_size) against all

@@ -36,13 +36,11 @@ hipError_t CostToKey(const float *cost, uint32_t n, float scale, uint16_t *keys,
 // the prediction bin being clamp(floor(4 * predicted log2 cells), 0, 63) for
 // a gated job and 0 otherwise.
 constexpr uint32_t kLengthBins = 64;
-// keys[j] = min(length[j], 63) << 6
-hipError_t LengthBinKeys(const uint16_t *length, uint32_t n, uint16_t *keys, hipStream_t stream);
-// Appends the indices j in [0, n) with length[j] >= gate to index (count at
-// *d_count, which the call clears first); the order of the list is
-// unspecified.
-hipError_t GateJobs(const uint16_t *length, uint32_t n, uint32_t gate, uint32_t *index,
-                    uint32_t *d_count, hipStream_t stream);
+// keys[j] = min(length[j], 63) << 6 for j in [0, n), and the indices j with
+// length[j] >= gate appended to index (count at *d_count, which the call
+// clears first); the order of the list is unspecified.
+hipError_t BinAndGateJobs(const uint16_t *length, uint32_t n, uint32_t gate, uint16_t *keys,
+                          uint32_t *index, uint32_t *d_count, hipStream_t stream);
 // keys[index[k]] |= prediction bin of pred[k], k < m (k < *d_count when
 // d_count is given).
 hipError_t AddPredictionBins(const float *pred, const uint32_t *index, uint32_t m,

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gz_device.h"
+#include "gz_scan.h"
 #include "schedule.h"
 
 namespace gz {
@@ -79,29 +80,42 @@ __global__ void CostToKeyKernel(const float *cost, uint32_t n, float scale, uint
   keys[i] = (uint16_t)k;
 }
 
-__global__ void LengthBinKeysKernel(const uint16_t *length, uint32_t n, uint16_t *keys) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  keys[i] = (uint16_t)(min((uint32_t)length[i], kLengthBins - 1) << 6);
-}
-
 __global__ void ClearCountKernel(uint32_t *count) {
   if (threadIdx.x == 0) *count = 0;
 }
 
-// wave-aggregated append: one atomic per wave
-__global__ void GateKernel(const uint16_t *length, uint32_t n, uint32_t gate, uint32_t *index,
-                           uint32_t *count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool take = i < n && (uint32_t)length[i] >= gate;
-  const uint64_t mask = __ballot(take);
-  if (mask == 0) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader, kWave);
-  if (take) index[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = i;
+// Length bins and the gate of one side in one pass: every block takes
+// kGateChunk consecutive jobs (kGateItems per thread, strided so that the
+// loads coalesce), writes their length-bin keys, and appends the gated ones
+// with one atomic on the side's count per block (a block scan ranks them).
+constexpr uint32_t kGateItems = 16;
+constexpr uint32_t kGateChunk = 256 * kGateItems;
+
+__global__ void __launch_bounds__(256)
+    BinGateKernel(const uint16_t *length, uint32_t n, uint32_t gate, uint16_t *keys,
+                  uint32_t *index, uint32_t *count) {
+  __shared__ uint32_t lds[256 / kWave];
+  __shared__ uint32_t base_lds;
+  const uint32_t begin = blockIdx.x * kGateChunk;
+  uint32_t take = 0;  // bit r: item r of this thread is gated
+#pragma unroll
+  for (uint32_t r = 0; r < kGateItems; ++r) {
+    const uint32_t i = begin + r * 256 + threadIdx.x;
+    if (i < n) {
+      const uint32_t len = length[i];
+      keys[i] = (uint16_t)(min(len, kLengthBins - 1) << 6);
+      if (len >= gate) take |= 1u << r;
+    }
+  }
+  uint32_t inc, exc, total;
+  BlockScan<uint32_t, false>((uint32_t)__popc(take), lds, &inc, &exc, &total);
+  if (total == 0) return;  // block-uniform
+  if (threadIdx.x == 0) base_lds = atomicAdd(count, total);
+  __syncthreads();
+  uint32_t at = base_lds + exc;
+#pragma unroll
+  for (uint32_t r = 0; r < kGateItems; ++r)
+    if ((take >> r) & 1u) index[at++] = begin + r * 256 + threadIdx.x;
 }
 
 __global__ void PredictionBinKernel(const float *pred, const uint32_t *index, uint32_t m,
@@ -143,18 +157,12 @@ hipError_t CostToKey(const float *cost, uint32_t n, float scale, uint16_t *keys,
   return hipGetLastError();
 }
 
-hipError_t LengthBinKeys(const uint16_t *length, uint32_t n, uint16_t *keys, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(LengthBinKeysKernel, dim3(Blocks(n)), dim3(256), 0, s, length, n, keys);
-  return hipGetLastError();
-}
-
-hipError_t GateJobs(const uint16_t *length, uint32_t n, uint32_t gate, uint32_t *index,
-                    uint32_t *d_count, hipStream_t s) {
+hipError_t BinAndGateJobs(const uint16_t *length, uint32_t n, uint32_t gate, uint16_t *keys,
+                          uint32_t *index, uint32_t *d_count, hipStream_t s) {
   hipLaunchKernelGGL(ClearCountKernel, dim3(1), dim3(64), 0, s, d_count);
   if (n > 0)
-    hipLaunchKernelGGL(GateKernel, dim3(Blocks(n)), dim3(256), 0, s, length, n, gate, index,
-                       d_count);
+    hipLaunchKernelGGL(BinGateKernel, dim3((n + kGateChunk - 1) / kGateChunk), dim3(256), 0, s,
+                       length, n, gate, keys, index, d_count);
   return hipGetLastError();
 }
 
