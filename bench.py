@@ -130,10 +130,12 @@ def training_queries(gz, kind, rng, db_codes, n):
     return gz.Queries(names, res, offs)
 
 
-def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_000_000,
-                     kind="translated"):
+def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=0,
+                     kind="translated", batch=4096, lr=1e-3):
     """Logs extension traces with the plain schedule on training queries, then
-    trains the MLP on the GPU.  Returns (flat weights, report dict)."""
+    trains the MLP on the GPU (every logged trace unless max_traces caps them;
+    10% held out; the training set resident on the device).  Returns (flat
+    weights, report dict)."""
     q = training_queries(gz, kind, rng, db_codes, n_reads)
     al = gz.Aligner(db, record_traces=1)
     _, st = al.align(q)
@@ -151,32 +153,42 @@ def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_
                        "extensions": int(st["gapped_extensions"]), "kernel_ms": gk_ms,
                        "launches": int(st["gapped_launches"]),
                        "workload": f"{n_reads} training reads, plain order"}
-    if len(tr) > max_traces:  # a uniform sample keeps training to seconds
+    if max_traces and len(tr) > max_traces:  # a uniform sample
         tr = tr[np.sort(rng.choice(len(tr), max_traces, replace=False))]
     concat = al.last_concatenated_queries()
     chunk_seq = {c: db.chunk_sequence(c) for c in np.unique(tr[:, 2])} if len(tr) else {}
     feats = np.zeros((len(tr), 128), np.uint8)
+    t_feat = time.perf_counter()
+    jobs = []
     for c, seqc in chunk_seq.items():
         for rev in (0, 1):
             sel = np.nonzero((tr[:, 2] == c) & (tr[:, 3] == rev))[0]
-            if len(sel):
-                feats[sel] = gz.cost_features(concat, seqc, tr[sel, 0], tr[sel, 1], rev)
+            for part in np.array_split(sel, max(1, len(sel) // 250_000)):
+                if len(part):
+                    jobs.append((c, rev, part))
+
+    def featurize(job):
+        c, rev, part = job
+        feats[part] = gz.cost_features(concat, chunk_seq[c], tr[part, 0], tr[part, 1], rev)
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(host_threads()) as pool:
+        list(pool.map(featurize, jobs))
+    feat_s = time.perf_counter() - t_feat
     del chunk_seq
     target = np.log2(np.maximum(tr[:, 4], 1)).astype(np.float32)
     perm = rng.permutation(len(tr))
     n_test = max(1, len(tr) // 10)
     test, train = perm[:n_test], perm[n_test:]
     mlp = gz.Mlp(gz.Mlp.init_weights(seed))
-    batch = 1024
+    # the held-in rows, resident on the device; each epoch one permutation
+    train_feats, train_target = feats[train], target[train]
+    order = np.concatenate([rng.permutation(len(train)) for _ in range(epochs)])
+    steps = len(order) // batch
     t0 = time.perf_counter()
-    steps = 0
-    for ep in range(epochs):
-        order = rng.permutation(train)
-        for b in range(0, len(order) - batch + 1, batch):
-            idx = order[b:b + batch]
-            mlp.train_step(feats[idx], target[idx], 1e-3)
-            steps += 1
+    mean_loss, train_ms = mlp.train_epochs(train_feats, train_target, order, batch, lr)
     train_s = time.perf_counter() - t0
+    del train_feats, train_target, order
     pred = mlp.predict(feats[test])
     mae = float(np.abs(pred - target[test]).mean())
     base = float(np.abs(target[test] - target[train].mean()).mean())
@@ -189,10 +201,17 @@ def train_cost_model(gz, db, db_codes, rng, n_reads, epochs, seed, max_traces=2_
         tf = seeds_per_s * MLP_FLOP_PER_SEED / 1e12
         infer_roof = {"bound": "mfma_bf16", "achieved": tf, "peak": BF16_PEAK_TFLOPS,
                       "unit": "TFLOP/s", "frac": tf / BF16_PEAK_TFLOPS}
+    # forward + two backward products per seed
     train_flop = 3 * MLP_FLOP_PER_SEED * steps * batch
     report = {"traces_logged": int(all_traces), "traces": int(len(tr)), "held_out": int(n_test),
-              "epochs": epochs, "train_steps": steps, "train_seconds": round(train_s, 3),
-              "train_tflops": round(train_flop / train_s / 1e12, 3) if train_s > 0 else None,
+              "epochs": epochs, "batch": batch, "lr": lr, "train_steps": steps,
+              "feature_seconds": round(feat_s, 3),
+              "train_seconds": round(train_s, 3), "train_device_ms": round(train_ms, 1),
+              "train_tflops": round(train_flop / (train_ms / 1e3) / 1e12, 3)
+              if train_ms > 0 else None,
+              "train_mfma_frac": round(train_flop / (train_ms / 1e3) / 1e12 / BF16_PEAK_TFLOPS, 4)
+              if train_ms > 0 else None,
+              "mean_train_loss": round(mean_loss, 4),
               "mae_log2_cells": round(mae, 4), "mae_constant_predictor": round(base, 4),
               "inference_seeds_per_s": seeds_per_s, "inference_roofline": infer_roof}
     return mlp.weights(), report, gapped_roof
@@ -278,6 +297,12 @@ def main():
                     help="fraction of the workload (reads and database proteins)")
     ap.add_argument("--train-reads", type=int, default=0)
     ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--train-traces", type=int, default=0,
+                    help="cap on the logged traces the cost model trains on (0: all)")
+    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--train-lr", type=float, default=1e-3)
+    ap.add_argument("--train-only", action="store_true",
+                    help="stop after the cost model's training report")
     ap.add_argument("--compare-warmup", type=int, default=2,
                     help="warm-up steps of the plain and batched forms")
     ap.add_argument("--subset-reads", type=int, default=256,
@@ -377,9 +402,14 @@ def main():
         tr_reads = max(1000, int(tr_reads * min(1.0, args.scale * 10)))
         weights, learn, gapped_roof = train_cost_model(
             gz, db, db_codes, np.random.default_rng(args.seed + 1), tr_reads, args.epochs,
-            args.seed, kind=wl["kind"])
+            args.seed, max_traces=args.train_traces, kind=wl["kind"], batch=args.train_batch,
+            lr=args.train_lr)
     del db_codes
     log(f"cost model trained in {time.perf_counter() - t0:.1f} s: {learn}")
+    if args.train_only:
+        print(json.dumps({"workload": args.workload, "cost_model": learn,
+                          "gapped_plain_order": gapped_roof}), flush=True)
+        return
 
     pb = {"pipeline_blocks": args.pipeline_blocks} if args.pipeline_blocks else {}
     plain = gz.Aligner(db, schedule=gz.SCHEDULE_PLAIN)

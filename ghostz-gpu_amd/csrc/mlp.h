@@ -30,6 +30,14 @@ class MlpDevice {
   // returns the flat gradient (same layout as the flat weights).
   hipError_t TrainStep(const uint8_t *codes, const float *targets, size_t n, float lr,
                        float *loss, float *flat_grad);
+  // Training on a device-resident set: codes (n x 128) and targets are
+  // uploaded once, then every whole batch of `order` (m row indices, e.g.
+  // the epochs' permutations one after the other) is one TrainStep without
+  // a host round trip; *mean_loss is the mean of the steps' losses and
+  // *kernel_ms the device time of the steps (HIP events).
+  hipError_t TrainEpochs(const uint8_t *codes, const float *targets, size_t n,
+                         const uint32_t *order, size_t m, size_t batch, float lr,
+                         float *mean_loss, float *kernel_ms);
   // Forward on device-resident codes (n x 128) into device outputs, on the
   // caller's stream (one persistent kernel for all three layers).
   // rotate_words: each 128-code row is read from its 32-bit word
@@ -45,6 +53,7 @@ class MlpDevice {
  private:
   hipError_t Reserve(size_t batch);
   hipError_t ForwardDevice(size_t n, size_t padded);
+  hipError_t StepOnDevice(size_t n, size_t padded, float lr);
   void SyncBf16();
 
   hipError_t init_status_ = hipSuccess;

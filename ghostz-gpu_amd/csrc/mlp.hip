@@ -162,6 +162,26 @@ __global__ void OutputKernel(int B, const uint16_t *h2, const float *w3, const f
   y[b] = s + b3[0];
 }
 
+// Batch rows from a device-resident training set: row r < n is row
+// order[r] of (codes, targets); padding rows read code 20 and target 0.
+// One thread per 16 bytes of the batch's codes.
+__global__ void GatherRowsKernel(const uint8_t *codes, const float *targets,
+                                 const uint32_t *order, int n, int B, uint8_t *out_codes,
+                                 float *out_t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int kPieces = kCostResidues / 16;
+  if (i >= B * kPieces) return;
+  const int r = i / kPieces, k = i % kPieces;
+  uint4 v = make_uint4(0x14141414u, 0x14141414u, 0x14141414u, 0x14141414u);
+  if (r < n) v = reinterpret_cast<const uint4 *>(codes + (size_t)order[r] * kCostResidues)[k];
+  reinterpret_cast<uint4 *>(out_codes + (size_t)r * kCostResidues)[k] = v;
+  if (k == 0) out_t[r] = r < n ? targets[order[r]] : 0.f;
+}
+
+__global__ void AccumulateKernel(const float *x, float *sum) {
+  if (threadIdx.x == 0) *sum += *x;
+}
+
 // MSE loss over the first n rows; dY = 2 (y - t) / n, zero on padding.
 __global__ void LossKernel(int B, int n, const float *y, const float *t, float *dy,
                            float *loss) {
@@ -634,7 +654,73 @@ hipError_t MlpDevice::TrainStep(const uint8_t *codes, const float *targets, size
   (void)hipMemcpyAsync(codes_, codes, n * kCostResidues, hipMemcpyHostToDevice, s);
   (void)hipMemsetAsync(t_, 0, B * 4, s);
   (void)hipMemcpyAsync(t_, targets, n * 4, hipMemcpyHostToDevice, s);
-  e = ForwardDevice(n, B);
+  e = StepOnDevice(n, B, lr);
+  if (e != hipSuccess) return e;
+  if (loss) (void)hipMemcpyAsync(loss, loss_, 4, hipMemcpyDeviceToHost, s);
+  if (flat_grad) (void)hipMemcpyAsync(flat_grad, grad_, kFlat * 4, hipMemcpyDeviceToHost, s);
+  return hipStreamSynchronize(s);
+}
+
+hipError_t MlpDevice::TrainEpochs(const uint8_t *codes, const float *targets, size_t n,
+                                  const uint32_t *order, size_t m, size_t batch, float lr,
+                                  float *mean_loss, float *kernel_ms) {
+  if (mean_loss) *mean_loss = 0.f;
+  if (kernel_ms) *kernel_ms = 0.f;
+  if (n == 0 || m == 0 || batch == 0) return hipSuccess;
+  const size_t B = (batch + 63) / 64 * 64;
+  hipError_t e = Reserve(B);
+  if (e != hipSuccess) return e;
+  hipStream_t s = stream_;
+  uint8_t *d_codes = nullptr;
+  float *d_t = nullptr, *d_sum = nullptr;
+  uint32_t *d_order = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  e = gz::Malloc((void **)&d_codes, n * kCostResidues);
+  if (e == hipSuccess) e = gz::Malloc((void **)&d_t, n * 4);
+  if (e == hipSuccess) e = gz::Malloc((void **)&d_order, m * 4);
+  if (e == hipSuccess) e = gz::Malloc((void **)&d_sum, 256);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_codes, codes, n * kCostResidues, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_t, targets, n * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_order, order, m * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(d_sum, 0, 4, s);
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  size_t steps = 0;
+  // whole batches only (the remainder of the order is left out, as the
+  // per-step loop of the callers does)
+  for (size_t b = 0; e == hipSuccess && b + batch <= m; b += batch) {
+    hipLaunchKernelGGL(GatherRowsKernel, dim3((unsigned)Blocks(B * kCostResidues / 16)),
+                       dim3(256), 0, s, d_codes, d_t, d_order + b, (int)batch, (int)B, codes_,
+                       t_);
+    e = StepOnDevice(batch, B, lr);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(AccumulateKernel, dim3(1), dim3(64), 0, s, loss_, d_sum);
+      e = hipGetLastError();
+    }
+    ++steps;
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  float sum = 0.f;
+  if (e == hipSuccess) e = hipMemcpyAsync(&sum, d_sum, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && mean_loss && steps) *mean_loss = sum / (float)steps;
+  float ms = 0.f;
+  if (e == hipSuccess && kernel_ms && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+    *kernel_ms = ms;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (void *p : {(void *)d_codes, (void *)d_t, (void *)d_order, (void *)d_sum})
+    if (p) (void)gz::Free(p);
+  return e;
+}
+
+// One optimisation step on the batch in codes_ / t_ (n rows, B padded):
+// forward, loss, backward, Adam (lr != 0); nothing is read back.
+hipError_t MlpDevice::StepOnDevice(size_t n, size_t B, float lr) {
+  hipStream_t s = stream_;
+  hipError_t e = ForwardDevice(n, B);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(LossKernel, dim3(1), dim3(256), 0, s, (int)B, (int)n, y_, t_, y_, loss_);
   // y_ now holds dY
@@ -675,9 +761,7 @@ hipError_t MlpDevice::TrainStep(const uint8_t *codes, const float *targets, size
                        m_, v_, lr, b1, b2, c1, c2, 1e-8f);
     SyncBf16();
   }
-  if (loss) (void)hipMemcpyAsync(loss, loss_, 4, hipMemcpyDeviceToHost, s);
-  if (flat_grad) (void)hipMemcpyAsync(flat_grad, grad_, kFlat * 4, hipMemcpyDeviceToHost, s);
-  return hipStreamSynchronize(s);
+  return hipGetLastError();
 }
 
 }  // namespace gz
@@ -750,6 +834,18 @@ gz_status gz_mlp_train_step(gz_mlp_t h, const uint8_t *codes, const float *targe
                             float lr, float *loss, float *flat_grad) {
   GZ_REQUIRE(h && codes && targets && n > 0, "gz_mlp_train_step: bad argument");
   GZ_HIP_CHECK(h->dev.TrainStep(codes, targets, n, lr, loss, flat_grad));
+  return GZ_OK;
+}
+
+gz_status gz_mlp_train_epochs(gz_mlp_t h, const uint8_t *codes, const float *targets, size_t n,
+                              const uint32_t *order, size_t order_length, size_t batch, float lr,
+                              float *mean_loss, float *kernel_ms) {
+  GZ_REQUIRE(h && codes && targets && order && n > 0 && batch > 0,
+             "gz_mlp_train_epochs: bad argument");
+  for (size_t i = 0; i < order_length; ++i)
+    GZ_REQUIRE(order[i] < n, "gz_mlp_train_epochs: order index out of range");
+  GZ_HIP_CHECK(h->dev.TrainEpochs(codes, targets, n, order, order_length, batch, lr, mean_loss,
+                                  kernel_ms));
   return GZ_OK;
 }
 

@@ -348,6 +348,8 @@ _SIGNATURES.update({
     "gz_mlp_predict_swapped": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "gz_mlp_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_float,
                                   POINTER(c_float), c_void_p]),
+    "gz_mlp_train_epochs": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t,
+                                    c_size_t, c_float, POINTER(c_float), POINTER(c_float)]),
     "gz_cost_features": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int,
                                 c_void_p]),
     "gz_cost_features_packed_async": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
@@ -668,6 +670,19 @@ class Mlp:
         check(lib().gz_mlp_train_step(self.handle, ptr(codes), ptr(t), len(codes), lr,
                                       ctypes.byref(loss), ptr(g)))
         return (loss.value, g) if want_grad else loss.value
+
+    def train_epochs(self, codes, targets, order, batch, lr):
+        """Every whole batch of `order` (row indices into codes / targets) as
+        one training step, the set resident on the device.  Returns (mean
+        loss, device milliseconds)."""
+        codes = np.ascontiguousarray(codes, np.uint8).reshape(-1, 128)
+        t = np.ascontiguousarray(targets, np.float32)
+        o = np.ascontiguousarray(order, np.uint32)
+        loss, ms = c_float(), c_float()
+        check(lib().gz_mlp_train_epochs(self.handle, ptr(codes), ptr(t), len(codes), ptr(o),
+                                        o.size, batch, lr, ctypes.byref(loss),
+                                        ctypes.byref(ms)))
+        return loss.value, ms.value
 
     def __del__(self):
         try:

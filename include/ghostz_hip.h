@@ -610,6 +610,14 @@ gz_status gz_mlp_predict_swapped(gz_mlp_t m, const uint8_t *codes, size_t n, flo
 /* One MSE step: gradients (flat layout, optional out) and Adam (lr > 0). */
 gz_status gz_mlp_train_step(gz_mlp_t m, const uint8_t *codes, const float *targets,
                             size_t n, float lr, float *loss, float *flat_grad);
+/* The same steps over a training set held on the device: codes (n x 128)
+ * and targets uploaded once; each whole batch of `order` (row indices, for
+ * example several epochs' permutations one after the other) is one step
+ * with Adam at lr, with no host round trip between steps; *mean_loss = mean
+ * of the steps' losses, *kernel_ms = device time of the steps. */
+gz_status gz_mlp_train_epochs(gz_mlp_t m, const uint8_t *codes, const float *targets, size_t n,
+                              const uint32_t *order, size_t order_length, size_t batch, float lr,
+                              float *mean_loss, float *kernel_ms);
 /* 128 feature codes per seed from unpacked concatenated query / chunk codes. */
 void gz_cost_features(const uint8_t *queries, const uint8_t *database,
                       const uint32_t *query_positions, const uint32_t *database_positions,

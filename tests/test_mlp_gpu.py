@@ -109,6 +109,34 @@ def test_training_fits_a_learnable_target(device):
     assert mae < 0.5 * base
 
 
+def test_device_resident_training_matches_per_step_training(device):
+    """gz_mlp_train_epochs runs the steps of gz_mlp_train_step without host
+    round trips: the same batches give the same weights (the embedding
+    gradient sums with float atomics, so within 1e-3 relative), and a
+    batch that does not divide the order leaves the remainder out."""
+    rng = np.random.default_rng(12)
+    codes = random_codes(rng, 3000)
+    t = ((codes[:, :32] < 5).sum(axis=1) * 0.25).astype(np.float32)
+    t -= t.mean()
+    order = np.concatenate([rng.permutation(3000) for _ in range(2)]).astype(np.uint32)
+    batch = 512
+    a = gz.Mlp(gz.Mlp.init_weights(13))
+    losses = []
+    for b in range(0, len(order) - batch + 1, batch):
+        idx = order[b:b + batch]
+        losses.append(a.train_step(codes[idx], t[idx], 1e-3))
+    b_ = gz.Mlp(gz.Mlp.init_weights(13))
+    mean_loss, ms = b_.train_epochs(codes, t, order, batch, 1e-3)
+    wa, wb = a.weights(), b_.weights()
+    say("steps", len(losses), "mean loss", np.mean(losses), mean_loss, "device ms", ms)
+    assert len(losses) == len(order) // batch
+    assert np.allclose(wa, wb, rtol=1e-3, atol=1e-5)
+    assert abs(mean_loss - np.mean(losses)) <= 1e-3 * max(1.0, abs(np.mean(losses)))
+    assert ms > 0
+    with pytest.raises(gz.GzError):
+        b_.train_epochs(codes, t, np.array([0, 3000], np.uint32), 1, 1e-3)
+
+
 def test_device_features_match_host_features(device):
     """Feature codes gathered on the device from packed sequences equal the
     host's (the codes the network is trained on)."""
