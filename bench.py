@@ -452,7 +452,8 @@ def main():
                 f"{lines} lines, seed {st['seconds_seed']:.3f} s, gapped "
                 f"{st['seconds_gapped']:.3f} s (kernel {st['gapped_kernel_ms']:.1f} ms, "
                 f"schedule {st['schedule_kernel_ms']:.1f} ms, predicted jobs "
-                f"{st['gated_jobs']} of {st['gapped_extensions']})")
+                f"{st['gated_jobs']} of {st['gapped_extensions']}; seed units rep "
+                f"{st['units_rep']} value {st['units_value']})")
             del text
         return stats, times, aa
 

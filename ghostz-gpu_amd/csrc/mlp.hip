@@ -56,46 +56,62 @@ enum { kEpiStore = 0, kEpiBiasGelu = 1, kEpiGeluGrad = 2 };
 // C[m][n] = sum_k A(m, k) * B(n, k), A(m,k) = A[m*sam + k*sak], B likewise;
 // every operand has one unit stride (k for the forward and dX products, the
 // row for the weight gradients).  Blocks of four waves own 64 x 64 tiles of
-// C (32 x 32 per wave); K advances 32 at a time through LDS, where both
-// operand tiles are stored k-contiguous, each filled with 16-byte loads
-// along the operand's unit stride.
-constexpr int kGemmTile = 64, kGemmK = 32, kGemmPad = 8;
+// C (32 x 32 per wave, v_mfma_f32_32x32x16_bf16); K advances 64 at a time
+// through LDS, where both operand tiles are stored k-contiguous, each filled
+// with 16-byte loads along the operand's unit stride; the next K slice is
+// read into registers while the current one is multiplied.  The weight
+// gradients have K = batch and few tiles: they split K over gridDim.z and
+// add their partial tiles into C (which the caller has zeroed).
+constexpr int kGemmTile = 64, kGemmK = 64, kGemmPad = 8;
+typedef __bf16 gemm_bf16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ void GemmLoadTile(uint16_t (*T)[kGemmK + kGemmPad], const uint16_t *X,
-                                             long sr, long sk, int rows, int K, int r0, int k0) {
+struct GemmSlice {
+  uint4 v[2];  // this thread's 16 values of one operand's 64 x 64 slice
+};
+
+// Thread t's part of the slice of rows [r0, r0 + 64) x k [k0, k0 + 64):
+// with unit k stride, row r0 + t / 4 and k's (t % 4) * 16 .. + 16; with unit
+// row stride, k0 + t / 4 and rows (t % 4) * 16 .. + 16.  Out-of-range values
+// read 0.
+template <bool kUnitK>
+__device__ __forceinline__ void GemmLoadSlice(GemmSlice &g, const uint16_t *X, long sr, long sk,
+                                              int rows, int K, int r0, int k0) {
   const int t = threadIdx.x;
-  if (sk == 1) {
-    // thread t: row r0 + t / 4, eight k's from k0 + (t % 4) * 8
-    const int r = t >> 2, kk = (t & 3) * 8;
-    const int gr = r0 + r, gk = k0 + kk;
-    const uint16_t *p = X + (long)gr * sr + gk;
-    if (gr < rows && gk + 8 <= K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
-      *reinterpret_cast<uint4 *>(&T[r][kk]) = *reinterpret_cast<const uint4 *>(p);
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        T[r][kk + j] = (gr < rows && gk + j < K) ? X[(long)gr * sr + gk + j] : (uint16_t)0;
-    }
+  const int a = t >> 2, b = (t & 3) * 16;
+  const int gr = kUnitK ? r0 + a : r0 + b, gk = kUnitK ? k0 + b : k0 + a;
+  const uint16_t *p = X + (long)gr * sr + (long)gk * sk;
+  const bool fast = kUnitK ? (gr < rows && gk + 16 <= K) : (gk < K && gr + 16 <= rows);
+  if (fast && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+    g.v[0] = *reinterpret_cast<const uint4 *>(p);
+    g.v[1] = *reinterpret_cast<const uint4 *>(p + 8);
   } else {
-    // thread t: k0 + t / 8, eight rows from r0 + (t % 8) * 8 (unit row stride)
-    const int kk = t >> 3, r = (t & 7) * 8;
-    const int gr = r0 + r, gk = k0 + kk;
-    const uint16_t *p = X + gr + (long)gk * sk;
-    uint16_t v[8];
-    if (gk < K && gr + 8 <= rows && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
-      const uint4 w = *reinterpret_cast<const uint4 *>(p);
-      memcpy(v, &w, 16);
-    } else {
+    uint16_t h[16];
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        v[j] = (gk < K && gr + j < rows) ? X[gr + j + (long)gk * sk] : (uint16_t)0;
+    for (int j = 0; j < 16; ++j) {
+      const int rr = kUnitK ? gr : gr + j, kk = kUnitK ? gk + j : gk;
+      h[j] = (rr < rows && kk < K) ? X[(long)rr * sr + (long)kk * sk] : (uint16_t)0;
     }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) T[r + j][kk] = v[j];
+    memcpy(&g.v[0], h, 32);
   }
 }
 
-template <int EPI>
+template <bool kUnitK>
+__device__ __forceinline__ void GemmStoreSlice(uint16_t (*T)[kGemmK + kGemmPad],
+                                               const GemmSlice &g) {
+  const int t = threadIdx.x;
+  const int a = t >> 2, b = (t & 3) * 16;
+  if (kUnitK) {
+    *reinterpret_cast<uint4 *>(&T[a][b]) = g.v[0];
+    *reinterpret_cast<uint4 *>(&T[a][b + 8]) = g.v[1];
+  } else {
+    uint16_t h[16];
+    memcpy(h, &g.v[0], 32);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) T[b + j][a] = h[j];
+  }
+}
+
+template <int EPI, bool kUnitKA, bool kUnitKB>
 __global__ void __launch_bounds__(256)
     GemmKernel(int M, int N, int K, const uint16_t *A, long sam, long sak,
                const uint16_t *B, long sbn, long sbk, float *C, int ldc,
@@ -106,18 +122,31 @@ __global__ void __launch_bounds__(256)
   const int mb = blockIdx.y * kGemmTile, nb = blockIdx.x * kGemmTile;
   const int m0 = mb + (wave >> 1) * 32, n0 = nb + (wave & 1) * 32;
   const int am = (wave >> 1) * 32 + (lane & 31), bn = (wave & 1) * 32 + (lane & 31);
-  const int kq = (lane >> 5) * 4;
+  const int kq = (lane >> 5) * 8;
+  // this block's K range (split K: gridDim.z slices of whole 64-steps)
+  const int steps = (K + kGemmK - 1) / kGemmK;
+  const int per = (steps + (int)gridDim.z - 1) / (int)gridDim.z;
+  const int kb = (int)blockIdx.z * per * kGemmK, ke = min(K, kb + per * kGemmK);
   float16_t acc;
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-  for (int k0 = 0; k0 < K; k0 += kGemmK) {
-    GemmLoadTile(As, A, sam, sak, M, K, mb, k0);
-    GemmLoadTile(Bs, B, sbn, sbk, N, K, nb, k0);
+  GemmSlice ga, gb;
+  if (kb < ke) {
+    GemmLoadSlice<kUnitKA>(ga, A, sam, sak, M, ke, mb, kb);
+    GemmLoadSlice<kUnitKB>(gb, B, sbn, sbk, N, ke, nb, kb);
+  }
+  for (int k0 = kb; k0 < ke; k0 += kGemmK) {
+    GemmStoreSlice<kUnitKA>(As, ga);
+    GemmStoreSlice<kUnitKB>(Bs, gb);
     __syncthreads();
+    if (k0 + kGemmK < ke) {  // the next slice's loads overlap this slice's MFMAs
+      GemmLoadSlice<kUnitKA>(ga, A, sam, sak, M, ke, mb, k0 + kGemmK);
+      GemmLoadSlice<kUnitKB>(gb, B, sbn, sbk, N, ke, nb, k0 + kGemmK);
+    }
 #pragma unroll
-    for (int ks = 0; ks < kGemmK; ks += 8) {
-      short4_t a = *reinterpret_cast<const short4_t *>(&As[am][ks + kq]);
-      short4_t b = *reinterpret_cast<const short4_t *>(&Bs[bn][ks + kq]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int ks = 0; ks < kGemmK; ks += 16) {
+      const gemm_bf16x8 a = *reinterpret_cast<const gemm_bf16x8 *>(&As[am][ks + kq]);
+      const gemm_bf16x8 b = *reinterpret_cast<const gemm_bf16x8 *>(&Bs[bn][ks + kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
     __syncthreads();
   }
@@ -129,7 +158,8 @@ __global__ void __launch_bounds__(256)
       const long idx = (long)row * ldc + col;
       const float v = acc[r];
       if (EPI == kEpiStore) {
-        C[idx] = v;
+        if (gridDim.z > 1) atomicAdd(&C[idx], v);
+        else C[idx] = v;
       } else if (EPI == kEpiBiasGelu) {
         const float z = v + bias[col];
         C[idx] = z;
@@ -465,13 +495,32 @@ __global__ void __launch_bounds__(kInferWaves * 64) __attribute__((amdgpu_waves_
   }
 }
 
+// K splits for a weight gradient of `tiles` output tiles and K = batch:
+// about 512 blocks, each with at least four 64-steps of K
+inline int SplitK(int K, int tiles) {
+  const int steps = (K + kGemmK - 1) / kGemmK;
+  return std::max(1, std::min(512 / tiles, steps / 4));
+}
+
+// split_k > 1 only for kEpiStore into a zeroed C
 template <int EPI>
 void Gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, long sam, long sak,
           const uint16_t *B, long sbn, long sbk, float *C, int ldc, const float *bias = nullptr,
-          uint16_t *Ch = nullptr, const float *z = nullptr) {
-  dim3 grid((N + 63) / 64, (M + 63) / 64);
-  hipLaunchKernelGGL((GemmKernel<EPI>), grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbn,
-                     sbk, C, ldc, bias, Ch, z);
+          uint16_t *Ch = nullptr, const float *z = nullptr, int split_k = 1) {
+  dim3 grid((N + 63) / 64, (M + 63) / 64, split_k);
+  // every operand here has one unit stride: k or the row
+  if (sak == 1 && sbk == 1)
+    hipLaunchKernelGGL((GemmKernel<EPI, true, true>), grid, dim3(256), 0, s, M, N, K, A, sam,
+                       sak, B, sbn, sbk, C, ldc, bias, Ch, z);
+  else if (sak == 1)
+    hipLaunchKernelGGL((GemmKernel<EPI, true, false>), grid, dim3(256), 0, s, M, N, K, A, sam,
+                       sak, B, sbn, sbk, C, ldc, bias, Ch, z);
+  else if (sbk == 1)
+    hipLaunchKernelGGL((GemmKernel<EPI, false, true>), grid, dim3(256), 0, s, M, N, K, A, sam,
+                       sak, B, sbn, sbk, C, ldc, bias, Ch, z);
+  else
+    hipLaunchKernelGGL((GemmKernel<EPI, false, false>), grid, dim3(256), 0, s, M, N, K, A, sam,
+                       sak, B, sbn, sbk, C, ldc, bias, Ch, z);
 }
 
 }  // namespace
@@ -735,7 +784,7 @@ hipError_t MlpDevice::StepOnDevice(size_t n, size_t B, float lr) {
                      y_, z2_, master_ + kOffW3, dz2_, dz2h_);
   // dW2 = dZ2^T H1
   Gemm<kEpiStore>(s, kCostH2, kCostH1, (int)B, dz2h_, 1, kCostH2, h1_, 1, kCostH1,
-                  grad_ + kOffW2, kCostH1);
+                  grad_ + kOffW2, kCostH1, nullptr, nullptr, nullptr, SplitK((int)B, 4));
   hipLaunchKernelGGL((ColumnSumKernel<float>), dim3(1, row_split), dim3(256), 0, s, (int)B,
                      kCostH2, dz2_, (const float *)nullptr, grad_ + kOffB2);
   // dZ1 = (dZ2 W2) * gelu'(Z1)
@@ -743,7 +792,7 @@ hipError_t MlpDevice::StepOnDevice(size_t n, size_t B, float lr) {
                      kCostH1, nullptr, dz1h_, z1_);
   // dW1 = dZ1^T X
   Gemm<kEpiStore>(s, kCostH1, kCostIn, (int)B, dz1h_, 1, kCostH1, x_, 1, kCostIn,
-                  grad_ + kOffW1, kCostIn);
+                  grad_ + kOffW1, kCostIn, nullptr, nullptr, nullptr, SplitK((int)B, 64));
   hipLaunchKernelGGL((ColumnSumKernel<float>), dim3(kCostH1 / 64, row_split), dim3(256), 0, s,
                      (int)B, kCostH1, dz1_, (const float *)nullptr, grad_ + kOffB1);
   // dX = dZ1 W1, then the embedding gradient
