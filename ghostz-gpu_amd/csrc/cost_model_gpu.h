@@ -25,7 +25,7 @@ hipError_t CostFeaturesOnDevice(const uint64_t *packed_queries, const uint64_t *
 
 // The device predictions share one feature scratch buffer: a model serves one
 // stream at a time (each aligner owns its own model and calls it from its
-// extension stream only).  Concurrent calls on two streams would overwrite
+// schedule stream only, on one host thread).  Concurrent calls on two streams would overwrite
 // each other's features, which changes only the predicted order, never a
 // search result.
 class GpuCostModel : public CostModel {

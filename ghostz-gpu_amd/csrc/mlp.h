@@ -53,7 +53,7 @@ class MlpDevice {
  private:
   hipError_t Reserve(size_t batch);
   hipError_t ForwardDevice(size_t n, size_t padded);
-  hipError_t StepOnDevice(size_t n, size_t padded, float lr);
+  hipError_t StepOnDevice(size_t n, size_t padded, float lr, long base, const uint32_t *done);
   void SyncBf16();
 
   hipError_t init_status_ = hipSuccess;

@@ -192,15 +192,17 @@ __global__ void OutputKernel(int B, const uint16_t *h2, const float *w3, const f
   y[b] = s + b3[0];
 }
 
-// Batch rows from a device-resident training set: row r < n is row
-// order[r] of (codes, targets); padding rows read code 20 and target 0.
-// One thread per 16 bytes of the batch's codes.
+// Batch rows from a device-resident training set: row r < n of step *done
+// is row order_base[*done * n + r] of (codes, targets); padding rows read
+// code 20 and target 0.  One thread per 16 bytes of the batch's codes.
 __global__ void GatherRowsKernel(const uint8_t *codes, const float *targets,
-                                 const uint32_t *order, int n, int B, uint8_t *out_codes,
-                                 float *out_t) {
+                                 const uint32_t *order_base, const uint32_t *done, int n, int B,
+                                 uint8_t *out_codes, float *out_t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   constexpr int kPieces = kCostResidues / 16;
   if (i >= B * kPieces) return;
+  // the batch of step *done
+  const uint32_t *order = order_base + (size_t)*done * (size_t)n;
   const int r = i / kPieces, k = i % kPieces;
   uint4 v = make_uint4(0x14141414u, 0x14141414u, 0x14141414u, 0x14141414u);
   if (r < n) v = reinterpret_cast<const uint4 *>(codes + (size_t)order[r] * kCostResidues)[k];
@@ -208,8 +210,12 @@ __global__ void GatherRowsKernel(const uint8_t *codes, const float *targets,
   if (k == 0) out_t[r] = r < n ? targets[order[r]] : 0.f;
 }
 
-__global__ void AccumulateKernel(const float *x, float *sum) {
-  if (threadIdx.x == 0) *sum += *x;
+// End of a step: its loss into the sum, the step count up by one.
+__global__ void AdvanceKernel(const float *loss, float *sum, uint32_t *done) {
+  if (threadIdx.x == 0) {
+    *sum += *loss;
+    *done += 1;
+  }
 }
 
 // MSE loss over the first n rows; dY = 2 (y - t) / n, zero on padding.
@@ -299,6 +305,28 @@ __global__ void AdamKernel(size_t n, float *w, const float *g, float *m, float *
   m[i] = mi;
   v[i] = vi;
   w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Adam with the step number on the device: t = base + *done + 1 (a step
+// sequence replayed from a graph keeps its own count).
+__global__ void AdamCountedKernel(size_t n, float *w, const float *g, float *m, float *v,
+                                  float lr, float b1, float b2, float eps, long base,
+                                  const uint32_t *done) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float t = (float)(base + (long)*done + 1);
+  const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void ZeroFloatKernel(size_t n, float *p) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
 }
 
 __global__ void ToBf16Kernel(size_t n, const float *src, uint16_t *dst) {
@@ -703,7 +731,7 @@ hipError_t MlpDevice::TrainStep(const uint8_t *codes, const float *targets, size
   (void)hipMemcpyAsync(codes_, codes, n * kCostResidues, hipMemcpyHostToDevice, s);
   (void)hipMemsetAsync(t_, 0, B * 4, s);
   (void)hipMemcpyAsync(t_, targets, n * 4, hipMemcpyHostToDevice, s);
-  e = StepOnDevice(n, B, lr);
+  e = StepOnDevice(n, B, lr, 0, nullptr);
   if (e != hipSuccess) return e;
   if (loss) (void)hipMemcpyAsync(loss, loss_, 4, hipMemcpyDeviceToHost, s);
   if (flat_grad) (void)hipMemcpyAsync(flat_grad, grad_, kFlat * 4, hipMemcpyDeviceToHost, s);
@@ -722,42 +750,71 @@ hipError_t MlpDevice::TrainEpochs(const uint8_t *codes, const float *targets, si
   hipStream_t s = stream_;
   uint8_t *d_codes = nullptr;
   float *d_t = nullptr, *d_sum = nullptr;
-  uint32_t *d_order = nullptr;
+  uint32_t *d_order = nullptr, *d_done = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
   e = gz::Malloc((void **)&d_codes, n * kCostResidues);
   if (e == hipSuccess) e = gz::Malloc((void **)&d_t, n * 4);
   if (e == hipSuccess) e = gz::Malloc((void **)&d_order, m * 4);
   if (e == hipSuccess) e = gz::Malloc((void **)&d_sum, 256);
+  d_done = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(d_sum) + 128);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e == hipSuccess)
     e = hipMemcpyAsync(d_codes, codes, n * kCostResidues, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_t, targets, n * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_order, order, m * 4, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemsetAsync(d_sum, 0, 4, s);
+  if (e == hipSuccess) e = hipMemsetAsync(d_sum, 0, 256, s);
   if (e == hipSuccess) e = hipEventRecord(e0, s);
-  size_t steps = 0;
   // whole batches only (the remainder of the order is left out, as the
-  // per-step loop of the callers does)
-  for (size_t b = 0; e == hipSuccess && b + batch <= m; b += batch) {
+  // per-step loop of the callers does); every step reads its batch and its
+  // Adam step number from d_done, so a captured run of kGraphSteps steps is
+  // replayed as is
+  const size_t steps = m / batch;
+  const long base = step_;
+  auto step = [&]() -> hipError_t {
     hipLaunchKernelGGL(GatherRowsKernel, dim3((unsigned)Blocks(B * kCostResidues / 16)),
-                       dim3(256), 0, s, d_codes, d_t, d_order + b, (int)batch, (int)B, codes_,
-                       t_);
-    e = StepOnDevice(batch, B, lr);
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(AccumulateKernel, dim3(1), dim3(64), 0, s, loss_, d_sum);
-      e = hipGetLastError();
+                       dim3(256), 0, s, d_codes, d_t, d_order, d_done, (int)batch, (int)B,
+                       codes_, t_);
+    hipError_t r = StepOnDevice(batch, B, lr, base, d_done);
+    if (r == hipSuccess) {
+      hipLaunchKernelGGL(AdvanceKernel, dim3(1), dim3(64), 0, s, loss_, d_sum, d_done);
+      r = hipGetLastError();
     }
-    ++steps;
+    return r;
+  };
+  constexpr size_t kGraphSteps = 16;
+  size_t done = 0;
+  if (e == hipSuccess && steps >= 2 * kGraphSteps) {
+    {
+      std::lock_guard<std::recursive_mutex> lock(CaptureLock());
+      e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+      hipError_t ec = hipSuccess;
+      for (size_t k = 0; k < kGraphSteps && e == hipSuccess; ++k) e = step();
+      ec = hipStreamEndCapture(s, &graph);
+      if (e == hipSuccess) e = ec;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    // each replay waits for the previous one (one stream); the host
+    // synchronizes between replays so that a replay never races the next
+    for (; e == hipSuccess && done + kGraphSteps <= steps; done += kGraphSteps) {
+      e = hipGraphLaunch(exec, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
   }
+  for (; e == hipSuccess && done < steps; ++done) e = step();
   if (e == hipSuccess) e = hipEventRecord(e1, s);
   float sum = 0.f;
   if (e == hipSuccess) e = hipMemcpyAsync(&sum, d_sum, 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) step_ += (long)steps;
   if (e == hipSuccess && mean_loss && steps) *mean_loss = sum / (float)steps;
   float ms = 0.f;
   if (e == hipSuccess && kernel_ms && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
     *kernel_ms = ms;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   for (void *p : {(void *)d_codes, (void *)d_t, (void *)d_order, (void *)d_sum})
@@ -766,14 +823,17 @@ hipError_t MlpDevice::TrainEpochs(const uint8_t *codes, const float *targets, si
 }
 
 // One optimisation step on the batch in codes_ / t_ (n rows, B padded):
-// forward, loss, backward, Adam (lr != 0); nothing is read back.
-hipError_t MlpDevice::StepOnDevice(size_t n, size_t B, float lr) {
+// forward, loss, backward, Adam (lr != 0); nothing is read back.  With
+// done, Adam's step number is base + *done + 1 on the device (the host's
+// step count is the caller's to advance); without, the step is counted here.
+hipError_t MlpDevice::StepOnDevice(size_t n, size_t B, float lr, long base,
+                                   const uint32_t *done) {
   hipStream_t s = stream_;
   hipError_t e = ForwardDevice(n, B);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(LossKernel, dim3(1), dim3(256), 0, s, (int)B, (int)n, y_, t_, y_, loss_);
-  // y_ now holds dY
-  (void)hipMemsetAsync(grad_, 0, kFlat * 4, s);
+  // y_ now holds dY; the gradient is cleared by a kernel (graph-safe)
+  hipLaunchKernelGGL(ZeroFloatKernel, dim3(Blocks(kFlat)), dim3(256), 0, s, kFlat, grad_);
   const unsigned row_split = std::max(1u, std::min(64u, (unsigned)(B / 64)));
   // dW3 = H2^T dY, dB3 = sum dY, then dZ2
   hipLaunchKernelGGL((ColumnSumKernel<uint16_t>), dim3(1, row_split), dim3(256), 0, s, (int)B,
@@ -803,11 +863,16 @@ hipError_t MlpDevice::StepOnDevice(size_t n, size_t B, float lr) {
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (lr != 0.f) {
-    ++step_;
     const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.f - std::pow(b1, (float)step_), c2 = 1.f - std::pow(b2, (float)step_);
-    hipLaunchKernelGGL(AdamKernel, dim3(Blocks(kFlat)), dim3(256), 0, s, kFlat, master_, grad_,
-                       m_, v_, lr, b1, b2, c1, c2, 1e-8f);
+    if (done) {
+      hipLaunchKernelGGL(AdamCountedKernel, dim3(Blocks(kFlat)), dim3(256), 0, s, kFlat,
+                         master_, grad_, m_, v_, lr, b1, b2, 1e-8f, base, done);
+    } else {
+      ++step_;
+      const float c1 = 1.f - std::pow(b1, (float)step_), c2 = 1.f - std::pow(b2, (float)step_);
+      hipLaunchKernelGGL(AdamKernel, dim3(Blocks(kFlat)), dim3(256), 0, s, kFlat, master_,
+                         grad_, m_, v_, lr, b1, b2, c1, c2, 1e-8f);
+    }
     SyncBf16();
   }
   return hipGetLastError();
