@@ -113,10 +113,9 @@ def test_device_resident_training_matches_per_step_training(device):
     """gz_mlp_train_epochs runs the steps of gz_mlp_train_step without host
     round trips (runs of 16 steps replayed from one captured graph, here five
     replays and a remainder of 13 steps run directly): the same batches give
-    the same weights (the embedding gradient sums with float atomics, and
-    Adam's bias corrections are computed on the device, so within 1e-3
-    relative), and a batch that does not divide the order leaves the
-    remainder out."""
+    the same weights up to the order of float atomics (Adam's bias
+    corrections are computed on the device), and a batch that does not
+    divide the order leaves the remainder out."""
     rng = np.random.default_rng(12)
     codes = random_codes(rng, 3000)
     t = ((codes[:, :32] < 5).sum(axis=1) * 0.25).astype(np.float32)
@@ -133,7 +132,12 @@ def test_device_resident_training_matches_per_step_training(device):
     wa, wb = a.weights(), b_.weights()
     say("steps", len(losses), "mean loss", np.mean(losses), mean_loss, "device ms", ms)
     assert len(losses) == len(order) // batch
-    assert np.allclose(wa, wb, rtol=1e-3, atol=1e-5)
+    # the embedding gradient's float atomics sum in any order, and Adam turns
+    # a near-zero gradient's noise into whole lr-sized steps of either sign:
+    # the weight vectors agree to 1e-3 relative (L2), and no weight moves by
+    # more than ten steps of lr
+    assert np.linalg.norm(wa - wb) <= 1e-3 * np.linalg.norm(wa)
+    assert np.abs(wa - wb).max() <= 10 * 1e-3
     assert abs(mean_loss - np.mean(losses)) <= 1e-3 * max(1.0, abs(np.mean(losses)))
     assert ms > 0
     with pytest.raises(gz.GzError):
