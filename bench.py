@@ -449,7 +449,8 @@ def main():
             aa += q_aa[c]
             lines = text.count(b"\n")
             log(f"{label} step {i} (query chunk {c}): {times[-1]:.3f} s, "
-                f"{lines} lines, seed {st['seconds_seed']:.3f} s, gapped "
+                f"{lines} lines, seed {st['seconds_seed']:.3f} s, chain "
+                f"{st['seconds_chain']:.3f} s, build {st['seconds_build']:.3f} s, gapped "
                 f"{st['seconds_gapped']:.3f} s (kernel {st['gapped_kernel_ms']:.1f} ms, "
                 f"schedule {st['schedule_kernel_ms']:.1f} ms, predicted jobs "
                 f"{st['gated_jobs']} of {st['gapped_extensions']}; seed units rep "

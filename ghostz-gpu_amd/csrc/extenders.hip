@@ -285,29 +285,33 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
     R.Set(array_start, first);
     int last = array_start;
     const int as0 = array_start;
+    // the next column's code is read a column ahead (the ring index wraps,
+    // so the read past the band is harmless and unused), so a cell's three
+    // LDS reads are in flight together
+    uint32_t code_next = R.Code(as0 + 1);
     for (ai = as0 + 1; ai < array_end; ++ai) {
       ++cells;
-      int s = prev + mrow[R.Code(ai)];
-      int2 cell = R.Get(ai);
+      const int2 cell = R.Get(ai);
+      const uint32_t code = code_next;
+      code_next = R.Code(ai + 1);
+      int s = prev + mrow[code];
       prev = cell.x;
       const int gap_col = cell.y;
       s = (s < gap_col) ? gap_col : s;
       s = (s < gap_row) ? gap_row : s;
-      if (max_score - s > cutoff) {
-        if (array_start + 1 == ai) ++array_start;
-        R.Set(ai, make_int2(kInitScore, kInitScore));
-        gap_row = kInitScore;
-      } else {
-        last = ai;
-        if (s > max_score) {
-          max_score = s;
-          max_p0 = (ai - 1) * inc;
-          max_p1 = row * inc;
-        }
-        const int ng = s - gap_init;
-        R.Set(ai, make_int2(s, (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext));
-        gap_row = (ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext;
-      }
+      // the X-drop test and both of its outcomes as selects (lanes of a
+      // wave split between them nearly every cell)
+      const bool prune = max_score - s > cutoff;
+      array_start += (prune && array_start + 1 == ai) ? 1 : 0;
+      last = prune ? last : ai;
+      const bool better = !prune && s > max_score;
+      max_p0 = better ? (ai - 1) * inc : max_p0;
+      max_p1 = better ? row * inc : max_p1;
+      max_score = better ? s : max_score;
+      const int ng = s - gap_init;
+      const int gc = (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext;
+      R.Set(ai, prune ? make_int2(kInitScore, kInitScore) : make_int2(s, gc));
+      gap_row = prune ? kInitScore : ((ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext);
     }
     if (array_start + 1 == array_end) break;
     // seq0 position after the row loop is column array_end (old value)
@@ -415,29 +419,30 @@ struct GappedStepper {
     R.Set(array_start, first);
     int last = array_start;
     const int as0 = array_start;
+    uint32_t code_next = R.Code(as0 + 1);
     for (int ai = as0 + 1; ai < array_end; ++ai) {
       ++cells;
-      int s = prev + mrow[R.Code(ai)];
-      int2 cell = R.Get(ai);
+      const int2 cell = R.Get(ai);
+      const uint32_t code = code_next;
+      code_next = R.Code(ai + 1);
+      int s = prev + mrow[code];
       prev = cell.x;
       const int gap_col = cell.y;
       s = (s < gap_col) ? gap_col : s;
       s = (s < gap_row) ? gap_row : s;
-      if (max_score - s > cutoff) {
-        if (array_start + 1 == ai) ++array_start;
-        R.Set(ai, make_int2(kInitScore, kInitScore));
-        gap_row = kInitScore;
-      } else {
-        last = ai;
-        if (s > max_score) {
-          max_score = s;
-          max_p0 = (ai - 1) * inc;
-          max_p1 = row * inc;
-        }
-        const int ng = s - gap_init;
-        R.Set(ai, make_int2(s, (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext));
-        gap_row = (ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext;
-      }
+      // the X-drop test and both of its outcomes as selects (lanes of a
+      // wave split between them nearly every cell)
+      const bool prune = max_score - s > cutoff;
+      array_start += (prune && array_start + 1 == ai) ? 1 : 0;
+      last = prune ? last : ai;
+      const bool better = !prune && s > max_score;
+      max_p0 = better ? (ai - 1) * inc : max_p0;
+      max_p1 = better ? row * inc : max_p1;
+      max_score = better ? s : max_score;
+      const int ng = s - gap_init;
+      const int gc = (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext;
+      R.Set(ai, prune ? make_int2(kInitScore, kInitScore) : make_int2(s, gc));
+      gap_row = prune ? kInitScore : ((ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext);
     }
     if (array_start + 1 == array_end) return kStepDone;
     int p0col = array_end;
@@ -980,7 +985,21 @@ gz_status gz_ungapped_extender_extend_with_trigger_async(
 // LDS rings: most bands fit the narrow one (more waves per CU); the rest
 // move to the 64-column ring, then to the global-memory ring
 constexpr int kGappedRing = 64;
+// lanes of the global ring: the allocation-free entry point's fixed count,
+// and the handle's, chosen per longest query so that the ring takes at most
+// kWideBudget bytes (a long band runs on one lane, so the bands that leave
+// the LDS rings need many lanes: Pareto-long protein queries leave
+// thousands of them per call)
 constexpr uint32_t kWideSlots = 256;
+constexpr uint32_t kWideSlotsMax = 16384;
+constexpr size_t kWideBudget = (size_t)1 << 30;
+
+inline uint32_t WideSlotsFor(uint32_t wide) {
+  const size_t per = (size_t)(wide ? wide : 1) * (sizeof(int2) + 1);
+  size_t n = kWideBudget / per;
+  n = n < kWideSlots ? kWideSlots : (n > kWideSlotsMax ? kWideSlotsMax : n);
+  return (uint32_t)(n / 64 * 64);
+}
 
 struct gz_gapped_extender_s {
   uint32_t delimiter = kDelimiter;
@@ -1002,6 +1021,7 @@ struct gz_gapped_extender_s {
   int2 *wide_cells = nullptr;
   uint8_t *wide_codes = nullptr;
   uint32_t wide_allocated = 0;
+  uint32_t wide_slots = 0;  // lanes of the allocated global ring
   // the failure counter of the last run (its own workspace's, or the
   // allocation-free entry point's scratch on the device it ran on)
   uint32_t *failure_counter = nullptr;
@@ -1080,6 +1100,7 @@ struct GappedWork {
   int2 *wide_cells;
   uint8_t *wide_codes;
   uint32_t wide;
+  uint32_t wide_slots;  // lanes of the global ring (a multiple of 64)
 };
 
 template <bool kReverse>
@@ -1149,7 +1170,7 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedWork &w, const Gapp
     else GZ_RING64(false, dim3(256u * 4), 0, 0, w.overflow_list, narrow_out, none);
   }
 #undef GZ_RING64
-  hipLaunchKernelGGL((GappedWideKernel<kReverse>), dim3(kWideSlots / kGappedBlock),
+  hipLaunchKernelGGL((GappedWideKernel<kReverse>), dim3(w.wide_slots / kGappedBlock),
                      dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q, d_d,
                      d_scores, d_cells, wide_out, w.overflow_list2, w.wide_cells,
                      w.wide_codes, w.wide, failures);
@@ -1201,9 +1222,14 @@ static gz_status GappedReserve(gz_gapped_extender_t h, size_t size, hipStream_t 
   if (h->wide_allocated < h->wide) {
     if (h->wide_cells) GZ_HIP_CHECK(gz::Free(h->wide_cells));
     if (h->wide_codes) GZ_HIP_CHECK(gz::Free(h->wide_codes));
-    GZ_HIP_CHECK(gz::Malloc(&h->wide_cells, (size_t)kWideSlots * h->wide * sizeof(int2)));
-    GZ_HIP_CHECK(gz::Malloc(&h->wide_codes, (size_t)kWideSlots * h->wide));
+    h->wide_cells = nullptr;
+    h->wide_codes = nullptr;
+    h->wide_allocated = 0;
+    const uint32_t slots = WideSlotsFor(h->wide);
+    GZ_HIP_CHECK(gz::Malloc(&h->wide_cells, (size_t)slots * h->wide * sizeof(int2)));
+    GZ_HIP_CHECK(gz::Malloc(&h->wide_codes, (size_t)slots * h->wide));
     h->wide_allocated = h->wide;
+    h->wide_slots = slots;
   }
   return GZ_OK;
 }
@@ -1222,8 +1248,8 @@ static gz_status GappedRun(gz_gapped_extender_t h, size_t size, int reverse,
   // are kept until gz_gapped_extender_failures reads them
   hipLaunchKernelGGL(ResetRunCountersKernel, dim3(1), dim3(64), 0, s, h->counters, 0);
   GZ_HIP_CHECK(hipGetLastError());
-  const GappedWork w = {h->counters, h->overflow_list, h->overflow_list2, h->wide_cells,
-                        h->wide_codes, h->wide};
+  const GappedWork w = {h->counters,   h->overflow_list, h->overflow_list2, h->wide_cells,
+                        h->wide_codes, h->wide,          h->wide_slots};
   h->failure_counter = h->counters + 1;
   return GappedLaunchAll(h, w, size, reverse, d_q, d_d, d_scores, d_cells, s, wide_first, order);
 }
@@ -1343,6 +1369,7 @@ gz_status gz_gapped_extender_extend_one_side_score_only_noalloc_async(
   w.wide_cells = (int2 *)(base + kNoAllocCountersBytes + 2 * kNoAllocListBytes);
   w.wide_codes = base + kNoAllocCountersBytes + 2 * kNoAllocListBytes + kNoAllocCellsBytes;
   w.wide = h->wide;
+  w.wide_slots = kWideSlots;
   // this call's counts and failures start at zero (a fill kernel, no memset
   // node, so that the call can be captured into a graph)
   hipLaunchKernelGGL(ClearCountersKernel, dim3(1), dim3(64), 0, s, w.counters);

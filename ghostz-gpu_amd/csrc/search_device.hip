@@ -228,11 +228,11 @@ __device__ __forceinline__ void WalkItems(const uint64_t *qw, const uint64_t *dw
   }
 }
 
-// Register budgets: five waves per SIMD for the value kernel, four for the
-// representative kernel (whose member loop needs more).
+// Register budget: four waves per SIMD (at five the value kernel spills
+// its per-tile state).
 template <bool kValues, typename T>
 __global__ void __launch_bounds__(kSeedThreads)
-    __attribute__((amdgpu_waves_per_eu(kValues ? 5 : 4)))
+    __attribute__((amdgpu_waves_per_eu(4)))
     SeedKernel(DeviceQueryLists q, DeviceDbChunk db, SeedStageParams p,
                const ListInfo *info, const uint32_t *active,
                const uint64_t *active_off, const uint2 *tile_range, uint64_t unit_begin,
