@@ -508,8 +508,8 @@ def main():
     per_unit = 24.0
     achieved = per_unit * units / launches / (avg_ms / 1e3) / 1e9
     name = "SeedKernel<%s>" % ("false" if k == 0 else "true")
-    tp = TRAFFIC_PROFILES[args.workload]
-    measured = traffic_per_unit(tp, name) if args.scale == 1.0 else None
+    tp = TRAFFIC_PROFILES.get(args.workload)
+    measured = traffic_per_unit(tp, name) if tp and args.scale == 1.0 else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured * units / launches if measured else None,

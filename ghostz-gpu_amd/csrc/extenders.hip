@@ -986,13 +986,13 @@ gz_status gz_ungapped_extender_extend_with_trigger_async(
 // move to the 64-column ring, then to the global-memory ring
 constexpr int kGappedRing = 64;
 // lanes of the global ring: the allocation-free entry point's fixed count,
-// and the handle's, chosen per longest query so that the ring takes at most
+// and the handle's, one per job up to what the longest query allows within
 // kWideBudget bytes (a long band runs on one lane, so the bands that leave
 // the LDS rings need many lanes: Pareto-long protein queries leave
 // thousands of them per call)
 constexpr uint32_t kWideSlots = 256;
 constexpr uint32_t kWideSlotsMax = 16384;
-constexpr size_t kWideBudget = (size_t)1 << 30;
+constexpr size_t kWideBudget = (size_t)8 << 30;
 
 inline uint32_t WideSlotsFor(uint32_t wide) {
   const size_t per = (size_t)(wide ? wide : 1) * (sizeof(int2) + 1);
@@ -1219,13 +1219,18 @@ static gz_status GappedReserve(gz_gapped_extender_t h, size_t size, hipStream_t 
     GZ_HIP_CHECK(gz::Malloc(&h->overflow_list2, cap * sizeof(uint32_t)));
     h->overflow_capacity = cap;
   }
-  if (h->wide_allocated < h->wide) {
+  // lanes: one per job at most, at least kWideSlots, within the budget
+  const size_t by_jobs = (size + 63) / 64 * 64;
+  const uint32_t want = (uint32_t)std::max<size_t>(
+      kWideSlots, std::min<size_t>(WideSlotsFor(h->wide), by_jobs));
+  if (h->wide_allocated < h->wide || h->wide_slots < want) {
     if (h->wide_cells) GZ_HIP_CHECK(gz::Free(h->wide_cells));
     if (h->wide_codes) GZ_HIP_CHECK(gz::Free(h->wide_codes));
     h->wide_cells = nullptr;
     h->wide_codes = nullptr;
     h->wide_allocated = 0;
-    const uint32_t slots = WideSlotsFor(h->wide);
+    h->wide_slots = 0;
+    const uint32_t slots = want;
     GZ_HIP_CHECK(gz::Malloc(&h->wide_cells, (size_t)slots * h->wide * sizeof(int2)));
     GZ_HIP_CHECK(gz::Malloc(&h->wide_codes, (size_t)slots * h->wide));
     h->wide_allocated = h->wide;
