@@ -561,7 +561,8 @@ def main():
             "gapped_jobs": total(l_st, "gapped_extensions") // len(l_st)},
         "search_stats": {k: total(p_st, k) // len(p_st) for k in (
             "lists", "units_rep", "units_value", "seed_hits", "chained_hits",
-            "gapped_extensions", "results", "sequential_build_chunks")},
+            "gapped_extensions", "results", "sequential_build_chunks",
+            "traceback_host_jobs")},
         "stage_seconds": {k: round(total(p_st, k) / len(p_st), 4) for k in p_st[0]
                           if k.startswith("seconds_")},
         "cost_model": learn,

@@ -510,40 +510,59 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   hipStream_t s = traceback_stream_;
   for (int i = 4; i < 6; ++i)
     if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
+  // one pass over m jobs with the given per-lane capacities: lanes resident
+  // at once (up to seven 64-lane blocks per CU), bounded by a 12 GB scratch
+  // budget
+  auto pass = [&](const TracebackJob *hjobs, size_t m, const LaneCaps &caps,
+                  TracebackOut *hout) -> hipError_t {
+    const size_t budget = (size_t)12 << 30;
+    size_t lanes = std::min<size_t>((m + 63) / 64 * 64, (size_t)7 * 64 * 256);
+    lanes = std::max<size_t>(64, std::min(lanes, budget / caps.bytes_per_lane / 64 * 64));
+    const size_t a_jobs = AlignUp(m * sizeof(TracebackJob), 256);
+    const size_t a_outs = AlignUp(m * sizeof(TracebackOut), 256);
+    const size_t bytes = a_jobs + a_outs + lanes * caps.bytes_per_lane;
+    if (tb_ws_cap_ < bytes) {
+      if (tb_ws_) (void)gz::Free(tb_ws_);
+      tb_ws_ = nullptr;
+      tb_ws_cap_ = 0;
+      const hipError_t a = gz::Malloc(&tb_ws_, bytes);
+      if (a != hipSuccess) return a;
+      tb_ws_cap_ = bytes;
+    }
+    TracebackJob *d_jobs = (TracebackJob *)tb_ws_;
+    TracebackOut *d_outs = (TracebackOut *)((char *)tb_ws_ + a_jobs);
+    uint8_t *scratch = (uint8_t *)tb_ws_ + a_jobs + a_outs;
+    hipError_t e = hipMemcpyAsync(d_jobs, hjobs, m * sizeof(TracebackJob),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(ev_[4], s);
+    if (e == hipSuccess)
+      e = TracebackLaunch(dq_.packed, chunks_[ci]->packed, d_jobs, (uint32_t)m, d_matrix_,
+                          opt_.search.nl, opt_.search.gap_open, opt_.search.gap_extension,
+                          cutoff, scratch, caps, (uint32_t)lanes, d_outs, s);
+    if (e == hipSuccess) e = hipEventRecord(ev_[5], s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hout, d_outs, m * sizeof(TracebackOut), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[4], ev_[5]);
+    if (e == hipSuccess) st->traceback_kernel_ms += ms;
+    return e;
+  };
   const LaneCaps caps = TracebackCaps(max_query_length_, db_->max_length);
-  // lanes resident at once (up to seven 64-lane blocks per CU), bounded by a
-  // 12 GB scratch budget
-  const size_t budget = (size_t)12 << 30;
-  size_t lanes = std::min<size_t>((n + 63) / 64 * 64, (size_t)7 * 64 * 256);
-  lanes = std::max<size_t>(64, std::min(lanes, budget / caps.bytes_per_lane / 64 * 64));
-  const size_t a_jobs = AlignUp(n * sizeof(TracebackJob), 256);
-  const size_t a_outs = AlignUp(n * sizeof(TracebackOut), 256);
-  const size_t bytes = a_jobs + a_outs + lanes * caps.bytes_per_lane;
-  if (tb_ws_cap_ < bytes) {
-    if (tb_ws_) (void)gz::Free(tb_ws_);
-    tb_ws_ = nullptr;
-    tb_ws_cap_ = 0;
-    GZ_TRY(gz::Malloc(&tb_ws_, bytes));
-    tb_ws_cap_ = bytes;
+  GZ_TRY(pass(jobs.data(), n, caps, out->data()));
+  // results whose op rows outgrew the lane's arena run again with a large
+  // arena (fewer lanes); what still does not fit is left to the host
+  std::vector<uint32_t> big;
+  for (size_t k = 0; k < n; ++k)
+    if ((*out)[k].flags & kTracebackOverflow) big.push_back((uint32_t)k);
+  if (!big.empty()) {
+    const LaneCaps wide = TracebackCaps(max_query_length_, db_->max_length, kTracebackLargeArena);
+    std::vector<TracebackJob> again(big.size());
+    for (size_t i = 0; i < big.size(); ++i) again[i] = jobs[big[i]];
+    std::vector<TracebackOut> redo(big.size());
+    GZ_TRY(pass(again.data(), again.size(), wide, redo.data()));
+    for (size_t i = 0; i < big.size(); ++i) (*out)[big[i]] = redo[i];
   }
-  TracebackJob *d_jobs = (TracebackJob *)tb_ws_;
-  TracebackOut *d_outs = (TracebackOut *)((char *)tb_ws_ + a_jobs);
-  uint8_t *scratch = (uint8_t *)tb_ws_ + a_jobs + a_outs;
-  hipError_t e = hipMemcpyAsync(d_jobs, jobs.data(), n * sizeof(TracebackJob),
-                                hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipEventRecord(ev_[4], s);
-  if (e == hipSuccess)
-    e = TracebackLaunch(dq_.packed, chunks_[ci]->packed, d_jobs, (uint32_t)n, d_matrix_,
-                        opt_.search.nl, opt_.search.gap_open, opt_.search.gap_extension, cutoff,
-                        scratch, caps, (uint32_t)lanes, d_outs, s);
-  if (e == hipSuccess) e = hipEventRecord(ev_[5], s);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(out->data(), d_outs, n * sizeof(TracebackOut), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  float ms = 0;
-  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[4], ev_[5]);
-  if (e != hipSuccess) return (int)e;
-  st->traceback_kernel_ms += ms;
   return 0;
 }
 

@@ -410,13 +410,18 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
             r.end.d = o.end_d;
           }
         });
-        for (size_t k : redo) {
-          const uint32_t qi = ids[k].first, r = ids[k].second;
-          if (!BuildOneFresh(q, qi, chunk, pre[qi][r], P, result_cutoff, &results[qi][r])) {
-            stale = true;
-            break;
+        // results whose traceback outgrew a lane's arena, each rebuilt on
+        // its own (independent results: any thread, any order)
+        std::atomic<bool> redo_stale(false);
+        ParallelBlocks(redo.size(), threads, 1, [&](size_t b, size_t e) {
+          for (size_t i = b; i < e && !redo_stale.load(std::memory_order_relaxed); ++i) {
+            const size_t k = redo[i];
+            const uint32_t qi = ids[k].first, r = ids[k].second;
+            if (!BuildOneFresh(q, qi, chunk, pre[qi][r], P, result_cutoff, &results[qi][r]))
+              redo_stale.store(true);
           }
-        }
+        });
+        if (redo_stale.load()) stale = true;
       }
       if (stale) stale_chunks->push_back(part.ci);
     }

@@ -248,6 +248,7 @@ __global__ void __launch_bounds__(kSeedThreads)
   __shared__ uint32_t owner[kSeedTile];     // the tile list of each unit of the tile
   __shared__ uint32_t scan_lds[kSeedThreads / kWave];
   __shared__ SeedHit stage_lds[kSeedThreads / kWave][kWaveStage];
+  __shared__ uint4 rep_ex[kValues ? 1 : kSeedThreads / kWave][kValues ? 1 : kSeedItems * kWave];
   HitStage stage{stage_lds[threadIdx.x / kWave], 0u};
   for (uint32_t i = threadIdx.x; i < kTableDim * kTableDim; i += blockDim.x) {
     PairTablesFill(ptab, p.matrix, p.nl, i);
@@ -338,7 +339,9 @@ __global__ void __launch_bounds__(kSeedThreads)
     for (int j = 0; j < kSeedItems; ++j) {
       wq[j] = SeedWindow{0, 0, 0, 0, 0};
       wd[j] = SeedWindow{0, 0, 0, 0, 0};
-      if (valid[j]) {
+      // (representatives: only the distance check reads them here)
+      if (valid[j] && (kValues || ((rec[j].y >> 31) != 0 &&
+                                   p.subsequence_length <= kCodesPerWord))) {
         wq[j] = LoadSeedWindow(q.packed, rec[j].x);
         wd[j] = LoadSeedWindow(db.packed, dposv[j]);
       }
@@ -361,82 +364,98 @@ __global__ void __launch_bounds__(kSeedThreads)
         need[j] = dist[j] <= p.distance_threshold;
       }
     }
-    // the reverse sides of the lane's items, then the forward sides of those
-    // whose reverse score is within the trigger: a lane moves on to its next
-    // item as soon as the current walk stops, so a wave runs as long as its
-    // longest sum of walks rather than the sum of its longest walks
-    int sum[kSeedItems];
-    WalkItems<true>(q.packed, db.packed, &ptab, wq, wd, need, cutv, sum);
-    bool fwd[kSeedItems];
-    int fsum[kSeedItems];
+    if constexpr (kValues) {
+      // the reverse sides of the lane's items, then the forward sides of
+      // those whose reverse score is within the trigger: a lane moves on to
+      // its next item as soon as the current walk stops, so a wave runs as
+      // long as its longest sum of walks rather than the sum of its longest
+      // walks
+      int sum[kSeedItems];
+      WalkItems<true>(q.packed, db.packed, &ptab, wq, wd, need, cutv, sum);
+      bool fwd[kSeedItems];
+      int fsum[kSeedItems];
 #pragma unroll
-    for (int j = 0; j < kSeedItems; ++j) fwd[j] = need[j] && sum[j] <= trigv[j];
-    WalkItems<false>(q.packed, db.packed, &ptab, wq, wd, fwd, cutv, fsum);
+      for (int j = 0; j < kSeedItems; ++j) fwd[j] = need[j] && sum[j] <= trigv[j];
+      WalkItems<false>(q.packed, db.packed, &ptab, wq, wd, fwd, cutv, fsum);
 #pragma unroll
-    for (int j = 0; j < kSeedItems; ++j) {
-      if (fwd[j]) sum[j] += fsum[j];
-      SeedHit h = {0, 0, 0, 0, 0, 0, 0};
-      bool pass = false;
-      bool member_ok = false;
-      uint32_t mb = 0, me = 0;
-      int trig = 0, cut = 0;
-      if (valid[j]) {
-        const uint32_t lo = lidx[j], idx = ridx[j];
-        cut = cutv[j];
-        trig = trigv[j];
-        h.dpos = dposv[j];
-        h.qid = rec[j].y & 0x7fffffffu;
-        h.qpos = rec[j].x;
-        h.list = lrank[lo];
-        h.entry = eidx[j];
-        h.rv = kValues ? lnrep[lo] + idx : idx;
-        pass = need[j] && sum[j] > trig;
-        if (!kValues) {
+      for (int j = 0; j < kSeedItems; ++j) {
+        if (fwd[j]) sum[j] += fsum[j];
+        SeedHit h = {0, 0, 0, 0, 0, 0, 0};
+        if (valid[j]) {
+          h.dpos = dposv[j];
+          h.qid = rec[j].y & 0x7fffffffu;
+          h.qpos = rec[j].x;
+          h.list = lrank[lidx[j]];
+          h.entry = eidx[j];
+          h.rv = lnrep[lidx[j]] + ridx[j];
+        }
+        StageAppend(need[j] && sum[j] > trigv[j], h, stage, hits, counters, capacity);
+      }
+    } else {
+      // Representative units: a representative within the distance
+      // threshold is walked itself, and every member of one within the
+      // threshold plus the allowed mismatches is walked at the
+      // representative's query position.  Item j of a lane expands to cnt[j]
+      // walks (the representative's own first when it has one); the walks
+      // of the wave's 128 items are dealt out 64 at a time, one per lane,
+      // from a per-wave LDS record of each item (tile list, entry,
+      // representative index, first member).
+      uint4 *ex = rep_ex[threadIdx.x / kWave];
+      uint32_t cnt[kSeedItems];
+#pragma unroll
+      for (int j = 0; j < kSeedItems; ++j) {
+        bool member_ok = false;
+        uint32_t mb = 0, me = 0;
+        if (valid[j]) {
           const bool check = (rec[j].y >> 31) != 0;
           member_ok = !check || max(dist[j] - p.max_mismatches, 0) <= p.distance_threshold;
           if (member_ok) {
-            const uint32_t ri = lbeg[lo] + idx;
+            const uint32_t ri = lbeg[lidx[j]] + ridx[j];
             mb = db.mem_off[ri];
             me = db.mem_off[ri + 1];
           }
         }
+        cnt[j] = (need[j] ? 1u : 0u) + (member_ok ? me - mb : 0u);
+        ex[2 * lane + j] = make_uint4(lidx[j], eidx[j], ridx[j], mb);
       }
-      StageAppend(pass, h, stage, hits, counters, capacity);
-      if (!kValues) {
-        // the members of the wave's representatives, spread evenly over its
-        // lanes: item t belongs to the lane whose [first, first + count)
-        // holds it
-        const uint32_t count = member_ok ? me - mb : 0u;
-        const uint32_t incl = WaveInclusiveSum(count);
-        const uint32_t total = __shfl(incl, kWave - 1, kWave);
-        const uint32_t first = incl - count;
-        for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
-          const uint32_t t = t0 + (uint32_t)lane;
-          int owner = 0;  // first lane with incl > t
+      const uint32_t mine = cnt[0] + cnt[1];
+      const uint32_t incl = WaveInclusiveSum(mine);
+      const uint32_t total = __shfl(incl, kWave - 1, kWave);
+      // item 0's walk count, and which items walk their representative
+      const uint32_t split = cnt[0] << 2 | (need[0] ? 1u : 0u) | (need[1] ? 2u : 0u);
+      for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+        const uint32_t t = t0 + (uint32_t)lane;
+        int owner = 0;  // first lane with incl > t
 #pragma unroll
-          for (int step = kWave / 2; step >= 1; step >>= 1) {
-            const uint32_t v = __shfl(incl, owner + step - 1, kWave);
-            if (v <= t) owner += step;
-          }
-          owner = min(owner, kWave - 1);
-          SeedHit mh;
-          mh.qid = __shfl(h.qid, owner, kWave);
-          mh.qpos = __shfl(h.qpos, owner, kWave);
-          mh.list = __shfl(h.list, owner, kWave);
-          mh.rv = __shfl(h.rv, owner, kWave);
-          mh.entry = __shfl(h.entry, owner, kWave);
-          const uint32_t k = t - __shfl(first, owner, kWave);
-          const uint32_t m = __shfl(mb, owner, kWave) + k;
-          const int mcut = __shfl(cut, owner, kWave), mtrig = __shfl(trig, owner, kWave);
-          bool mp = false;
-          if (t < total) {
-            mh.dpos = db.mem_pos[m];
-            mh.member = 1 + k;
-            mp = UngappedWithTrigger(q.packed, mh.qpos, db.packed, mh.dpos, &ptab, mcut, mtrig) >
-                 mtrig;
-          }
-          StageAppend(mp, mh, stage, hits, counters, capacity);
+        for (int step = kWave / 2; step >= 1; step >>= 1) {
+          const uint32_t v = __shfl(incl, owner + step - 1, kWave);
+          if (v <= t) owner += step;
         }
+        owner = min(owner, kWave - 1);
+        const uint32_t first = __shfl(incl - mine, owner, kWave);
+        const uint32_t osplit = __shfl(split, owner, kWave);
+        SeedHit h = {0, 0, 0, 0, 0, 0, 0};
+        bool pass = false;
+        if (t < total) {
+          uint32_t r = t - first;  // the walk's index among the owner's
+          const uint32_t c0 = osplit >> 2;
+          const int j = r >= c0 ? 1 : 0;
+          if (j) r -= c0;
+          const uint32_t own = (osplit >> j) & 1u;  // the item walks its representative
+          const uint4 x = ex[2 * owner + j];        // tile list, entry, index, first member
+          const uint4 er = q.entry_rec[leb[x.x] + x.y];
+          const bool self = own != 0 && r == 0;
+          h.dpos = self ? db.rep_pos[lbeg[x.x] + x.z] : db.mem_pos[x.w + r - own];
+          h.member = self ? 0u : 1u + r - own;
+          h.qid = er.y & 0x7fffffffu;
+          h.qpos = er.x;
+          h.list = lrank[x.x];
+          h.rv = x.z;
+          h.entry = x.y;
+          const int cut = (int)er.z, trig = (int)er.w;
+          pass = UngappedWithTrigger(q.packed, h.qpos, db.packed, h.dpos, &ptab, cut, trig) > trig;
+        }
+        StageAppend(pass, h, stage, hits, counters, capacity);
       }
     }
   }

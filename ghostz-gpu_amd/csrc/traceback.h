@@ -15,7 +15,13 @@ struct LaneCaps {
   size_t bytes_per_lane = 0;
 };
 
-LaneCaps TracebackCaps(uint32_t max_query_length, uint32_t max_subject_length);
+// The op arena of a lane: the first pass's, and the one a result that
+// outgrew it runs again with.
+constexpr uint32_t kTracebackArena = 1u << 16;
+constexpr uint32_t kTracebackLargeArena = 1u << 25;
+
+LaneCaps TracebackCaps(uint32_t max_query_length, uint32_t max_subject_length,
+                       uint32_t arena = kTracebackArena);
 
 // One result per job; `lanes` (a multiple of 64) is the number of lanes the
 // scratch holds (lanes x caps.bytes_per_lane bytes).

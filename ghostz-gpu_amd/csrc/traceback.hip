@@ -401,11 +401,12 @@ __global__ void __launch_bounds__(kTbThreads)
 
 }  // namespace
 
-LaneCaps TracebackCaps(uint32_t max_query_length, uint32_t max_subject_length) {
+LaneCaps TracebackCaps(uint32_t max_query_length, uint32_t max_subject_length,
+                       uint32_t arena) {
   LaneCaps c;
   c.dp = max_query_length + 4;
   c.rows = max_subject_length + 4;
-  c.arena = 1u << 16;
+  c.arena = arena;
   c.ops = 2 * (max_query_length + max_subject_length) + 16;
   size_t b = (size_t)(c.dp + c.rows) * sizeof(int2) + c.arena + c.ops;
   c.bytes_per_lane = (b + 255) / 256 * 256;
