@@ -510,9 +510,8 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   hipStream_t s = traceback_stream_;
   for (int i = 4; i < 6; ++i)
     if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
-  // one pass over m jobs with the given per-lane capacities: lanes resident
-  // at once (up to seven 64-lane blocks per CU), bounded by a 12 GB scratch
-  // budget
+  // m jobs with the given per-lane capacities: lanes resident at once (up
+  // to seven 64-lane blocks per CU), bounded by a 12 GB scratch budget
   auto pass = [&](const TracebackJob *hjobs, size_t m, const LaneCaps &caps,
                   TracebackOut *hout) -> hipError_t {
     const size_t budget = (size_t)12 << 30;
@@ -550,19 +549,8 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
   };
   const LaneCaps caps = TracebackCaps(max_query_length_, db_->max_length);
   GZ_TRY(pass(jobs.data(), n, caps, out->data()));
-  // results whose op rows outgrew the lane's arena run again with a large
-  // arena (fewer lanes); what still does not fit is left to the host
-  std::vector<uint32_t> big;
-  for (size_t k = 0; k < n; ++k)
-    if ((*out)[k].flags & kTracebackOverflow) big.push_back((uint32_t)k);
-  if (!big.empty()) {
-    const LaneCaps wide = TracebackCaps(max_query_length_, db_->max_length, kTracebackLargeArena);
-    std::vector<TracebackJob> again(big.size());
-    for (size_t i = 0; i < big.size(); ++i) again[i] = jobs[big[i]];
-    std::vector<TracebackOut> redo(big.size());
-    GZ_TRY(pass(again.data(), again.size(), wide, redo.data()));
-    for (size_t i = 0; i < big.size(); ++i) (*out)[big[i]] = redo[i];
-  }
+  // results whose op rows outgrow the lane's arena are rebuilt on the host
+  // (on all threads): a long band on one lane runs far slower than on a core
   return 0;
 }
 

@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -176,6 +177,7 @@ struct LaneRingLds {
   int lane;
   __device__ __forceinline__ int2 Get(int i) const { return cells[(i & mask) * kWave + lane]; }
   __device__ __forceinline__ void Set(int i, int2 v) { cells[(i & mask) * kWave + lane] = v; }
+  __device__ __forceinline__ void SetInRange(int i, int2 v) { Set(i, v); }
   __device__ __forceinline__ uint8_t &Code(int i) {
     return codes[(i & mask) * kWave + lane];
   }
@@ -189,6 +191,9 @@ struct LaneRingLds {
 // be clamped there without changing a comparison.  And the extension leaves
 // the ring (as when its band outgrows it) once its best score passes 16384,
 // so stored values stay below 16384 + 127, the largest int8 matrix entry.
+// A row's cells are stored unclamped (SetInRange): a pruned cell stores
+// kInitScore, and a kept cell's score and gap value are at least
+// -(cutoff + gap open + extension) > -16384.
 struct LaneRingLds16 {
   static constexpr bool kShort = true;
   static constexpr int kLow = -16384;
@@ -204,6 +209,9 @@ struct LaneRingLds16 {
     cells[(i & mask) * kWave + lane] =
         make_short2((short)(v.x > kLow ? v.x : kLow), (short)(v.y > kLow ? v.y : kLow));
   }
+  __device__ __forceinline__ void SetInRange(int i, int2 v) {
+    cells[(i & mask) * kWave + lane] = make_short2((short)v.x, (short)v.y);
+  }
   __device__ __forceinline__ uint8_t &Code(int i) {
     return codes[(i & mask) * kWave + lane];
   }
@@ -216,6 +224,7 @@ struct LaneRingGlobal {
   uint32_t mask;
   __device__ __forceinline__ int2 Get(int i) const { return cells[i & mask]; }
   __device__ __forceinline__ void Set(int i, int2 v) { cells[i & mask] = v; }
+  __device__ __forceinline__ void SetInRange(int i, int2 v) { Set(i, v); }
   __device__ __forceinline__ uint8_t &Code(int i) { return codes[i & mask]; }
 };
 
@@ -289,8 +298,10 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
     // so the read past the band is harmless and unused), so a cell's three
     // LDS reads are in flight together
     uint32_t code_next = R.Code(as0 + 1);
+    // the row's best cell: the last column whose score rose above the best
+    int best_ai = -1;
+#pragma unroll 2
     for (ai = as0 + 1; ai < array_end; ++ai) {
-      ++cells;
       const int2 cell = R.Get(ai);
       const uint32_t code = code_next;
       code_next = R.Code(ai + 1);
@@ -305,13 +316,17 @@ __device__ bool GappedOneSide(Ring &R, const GappedParams &P, const int8_t *sm,
       array_start += (prune && array_start + 1 == ai) ? 1 : 0;
       last = prune ? last : ai;
       const bool better = !prune && s > max_score;
-      max_p0 = better ? (ai - 1) * inc : max_p0;
-      max_p1 = better ? row * inc : max_p1;
+      best_ai = better ? ai : best_ai;
       max_score = better ? s : max_score;
       const int ng = s - gap_init;
       const int gc = (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext;
-      R.Set(ai, prune ? make_int2(kInitScore, kInitScore) : make_int2(s, gc));
+      R.SetInRange(ai, prune ? make_int2(kInitScore, kInitScore) : make_int2(s, gc));
       gap_row = prune ? kInitScore : ((ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext);
+    }
+    cells += (uint32_t)max(array_end - as0 - 1, 0);
+    if (best_ai >= 0) {
+      max_p0 = (best_ai - 1) * inc;
+      max_p1 = row * inc;
     }
     if (array_start + 1 == array_end) break;
     // seq0 position after the row loop is column array_end (old value)
@@ -420,8 +435,9 @@ struct GappedStepper {
     int last = array_start;
     const int as0 = array_start;
     uint32_t code_next = R.Code(as0 + 1);
+    int best_ai = -1;
+#pragma unroll 2
     for (int ai = as0 + 1; ai < array_end; ++ai) {
-      ++cells;
       const int2 cell = R.Get(ai);
       const uint32_t code = code_next;
       code_next = R.Code(ai + 1);
@@ -436,13 +452,17 @@ struct GappedStepper {
       array_start += (prune && array_start + 1 == ai) ? 1 : 0;
       last = prune ? last : ai;
       const bool better = !prune && s > max_score;
-      max_p0 = better ? (ai - 1) * inc : max_p0;
-      max_p1 = better ? row * inc : max_p1;
+      best_ai = better ? ai : best_ai;
       max_score = better ? s : max_score;
       const int ng = s - gap_init;
       const int gc = (ng > gap_col - P.gap_ext) ? ng : gap_col - P.gap_ext;
-      R.Set(ai, prune ? make_int2(kInitScore, kInitScore) : make_int2(s, gc));
+      R.SetInRange(ai, prune ? make_int2(kInitScore, kInitScore) : make_int2(s, gc));
       gap_row = prune ? kInitScore : ((ng > gap_row - P.gap_ext) ? ng : gap_row - P.gap_ext);
+    }
+    cells += (uint32_t)max(array_end - as0 - 1, 0);
+    if (best_ai >= 0) {
+      max_p0 = (best_ai - 1) * inc;
+      max_p1 = row * inc;
     }
     if (array_start + 1 == array_end) return kStepDone;
     int p0col = array_end;
@@ -666,6 +686,223 @@ __global__ void __launch_bounds__(kGappedBlock)
       if (cells) cells[i] = nc;
     } else {
       atomicAdd(failed, 1u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gapped X-drop, one wave per extension (bands past the 64-column ring)
+// ---------------------------------------------------------------------------
+// GappedOneSide with a row's columns spread over the wave's lanes, 64 per
+// pass, the row kept in the wave's slot of the global ring.  Inside a row
+// the lane recurrence carries two values from column to column: the running
+// best score M and the horizontal gap g.  Both become wave scans:
+//   * g: with the reference's reset of g to kInitScore after a pruned cell
+//     dropped, g(a+1) = max(s'(a) - (open + ext), g(a) - ext), s'(a) being
+//     the cell's score before the horizontal gap (the diagonal step or the
+//     vertical gap), since a gap opened from a cell that a gap itself
+//     reached is never better (open >= 0); so g is the row's entry gap
+//     decayed, or an exclusive max-scan of s'(a) - (open + ext) + a ext
+//     less (a - 1) ext;
+//   * M before column a: the row's starting best or the exclusive max-scan
+//     of the row's scores (a pruned score is below M, so whether it counts
+//     changes nothing).
+// Dropping the reset is exact when kInitScore < -cutoff (the cutoff below
+// 1024): a cell is kept only with a score of at least M - cutoff >= -cutoff,
+// and every gap that runs through a pruned cell -- the dropped one, which
+// starts below that cell's score, and the reset one, which starts at
+// kInitScore -- stays below M - cutoff from there on, so it neither keeps a
+// cell nor sets a kept cell's score, nor the row's final gap when the band
+// grows past its end (which happens only from a kept last column).  Pruning,
+// kept scores, stored cells, the band bounds and the best cell are then
+// those of the lane form; the best cell is the last column whose score is
+// above every score before it (the first column holding the row's maximum).
+// Returns false when the band would not fit the ring.
+template <bool kReverse>
+__device__ bool WaveOneSide(int2 *ring, uint32_t mask, const GappedParams &P, const int8_t *sm,
+                            uint32_t qpos, uint32_t dpos, int *best_score, int *best_p0,
+                            int *best_p1, uint32_t *ncells) {
+  const int inc = kReverse ? -1 : 1;
+  const int W = (int)mask + 1;
+  const int lane = __lane_id();
+  constexpr int kNone = -(1 << 29);  // below every score (the scans' identity)
+  *best_score = 0;
+  *best_p0 = -inc;
+  *best_p1 = -inc;
+  *ncells = 0;
+  // column c (c >= 1) is the query code at qpos + (c - 1) * inc
+  auto code_at = [&](int col) -> uint32_t {
+    return PackedCodeAt(P.queries, (uint32_t)((int)qpos + (col - 1) * inc));
+  };
+  typename std::conditional<kReverse, ReverseCursor, ForwardCursor>::type dc;
+  dc.Init(P.database, dpos);
+  uint32_t d = dc.Next();
+  if (code_at(1) == P.delim || d == P.delim) return true;
+  const int gap_init = P.gap_open + P.gap_ext;
+  const int ge = P.gap_ext;
+  const int cutoff = P.cutoff;
+  // the first row: columns 1, 2, ... while the gap score -gap_init - (a - 1)
+  // ext is at least -cutoff and the column is not the query's delimiter;
+  // the first column that stops (a*) is the band's end, after a pruned
+  // column when it is not the delimiter.  A pass reads kPackPadCodes
+  // columns: every column before it is a residue, so its codes lie at most
+  // that far past a residue, inside the delimiter padding around the
+  // packed sequences.
+  if (lane == 0) ring[0] = make_int2(0, -gap_init);
+  int stop_col = 0;
+  bool stop_delim = false;
+  constexpr int kPass = (int)kPackPadCodes;
+  static_assert(kPass <= kWave, "one column per lane");
+  for (int a0 = 1;; a0 += kPass) {
+    if (a0 > W) return false;  // the loop would write column W
+    const int a = a0 + lane;
+    const bool mine = lane < kPass;
+    const int score = -gap_init - (a - 1) * ge;
+    const bool delim = mine && code_at(a) == P.delim;
+    const uint64_t stop = __ballot(mine && (delim || score < -cutoff));
+    const int k = stop ? __ffsll((unsigned long long)stop) - 1 : kPass;
+    if (a0 + k - 1 >= W) return false;  // the largest column the pass writes
+    if (lane < k) ring[a & mask] = make_int2(score, score - gap_init);
+    if (k < kPass) {
+      stop_col = a0 + k;
+      stop_delim = __shfl(delim ? 1 : 0, k, kWave) != 0;
+      break;
+    }
+  }
+  int array_start = 0, array_end = stop_col;
+  if (!stop_delim) {
+    if (stop_col >= W) return false;
+    if (lane == 0) ring[stop_col & mask] = make_int2(kInitScore, kInitScore);
+    array_end = stop_col + 1;
+  }
+  int max_score = 0, max_p0 = -inc, max_p1 = -inc;
+  uint32_t cells = 0;
+  for (int row = 0; d != P.delim; ++row, d = dc.Next()) {
+    // the previous row's cells (and the first row's) are written
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int8_t *mrow = sm + d * kTableDim;
+    int2 first = ring[array_start & mask];
+    const int prev0 = first.x;
+    first.x = (max_score - first.y > cutoff) ? kInitScore : first.y;
+    int gap_row = first.x - gap_init;
+    first.y -= ge;
+    if (lane == 0) ring[array_start & mask] = first;
+    const int as0 = array_start, ae = array_end;
+    int M = max_score;      // best score before the pass's first column
+    int gin = gap_row;      // horizontal gap into the pass's first column
+    int diag = prev0;       // the old score of the column before the pass
+    int first_kept = -1, last_kept = -1, best_col = -1;
+    for (int a0 = as0 + 1; a0 < ae; a0 += kWave) {
+      const int a = a0 + lane;
+      const bool on = a < ae;
+      const int2 cell = on ? ring[a & mask] : make_int2(kNone, kNone);
+      const uint32_t code = on ? code_at(a) : 0u;
+      const int up = __shfl_up(cell.x, 1, kWave);
+      const int prev = lane == 0 ? diag : up;
+      diag = __shfl(cell.x, kWave - 1, kWave);  // used only after a full pass
+      int sp = prev + mrow[code];
+      sp = sp < cell.y ? cell.y : sp;
+      // horizontal gap into column a
+      const int h = on ? sp - gap_init + lane * ge : kNone;
+      const int hx = __shfl_up(WaveInclusiveMax(h), 1, kWave);
+      int g = gin - lane * ge;
+      if (lane > 0) {
+        const int from = hx - (lane - 1) * ge;
+        g = g < from ? from : g;
+      }
+      const int sc = sp < g ? g : sp;
+      // best score before column a
+      const int mi = WaveInclusiveMax(on ? sc : kNone);
+      int mx = __shfl_up(mi, 1, kWave);
+      mx = lane == 0 ? M : (mx < M ? M : mx);
+      const bool prune = mx - sc > cutoff;
+      if (on) {
+        const int ng = sc - gap_init;
+        const int gc = (ng > cell.y - ge) ? ng : cell.y - ge;
+        ring[a & mask] = prune ? make_int2(kInitScore, kInitScore) : make_int2(sc, gc);
+      }
+      const uint64_t kept = __ballot(on && !prune);
+      if (kept) {
+        if (first_kept < 0) first_kept = a0 + __ffsll((unsigned long long)kept) - 1;
+        last_kept = a0 + 63 - __clzll((long long)kept);
+      }
+      const uint64_t better = __ballot(on && !prune && sc > mx);
+      if (better) best_col = a0 + 63 - __clzll((long long)better);
+      // the gap after column a, the pass's carries
+      const int gout = (sc - gap_init > g - ge) ? sc - gap_init : g - ge;
+      const int tail = (ae - a0 < kWave ? ae - a0 : kWave) - 1;
+      gin = __shfl(gout, tail, kWave);
+      const int pm = __shfl(mi, tail, kWave);
+      M = M < pm ? pm : M;
+    }
+    if (ae > as0 + 1) gap_row = gin;
+    cells += (uint32_t)(ae - as0 - 1 > 0 ? ae - as0 - 1 : 0);
+    array_start = first_kept >= 0 ? first_kept - 1 : ae - 1;
+    const int last = last_kept >= 0 ? last_kept : as0;
+    max_score = M;
+    if (best_col >= 0) {
+      max_p0 = (best_col - 1) * inc;
+      max_p1 = row * inc;
+    }
+    if (array_start + 1 == array_end) break;
+    // the band's end for the next row (as in GappedOneSide)
+    int p0col = array_end;
+    uint32_t c;
+    if (last < array_end - 1) {
+      array_end = last + 1;
+      c = code_at(p0col);
+    } else {
+      c = code_at(p0col);
+      while (gap_row >= (max_score - cutoff) && c != P.delim) {
+        if (array_end - array_start >= W) return false;
+        if (lane == 0) ring[array_end & mask] = make_int2(gap_row, gap_row - gap_init);
+        gap_row -= ge;
+        ++array_end;
+        c = code_at(++p0col);
+      }
+    }
+    if (c != P.delim) {
+      if (array_end - array_start >= W) return false;
+      if (lane == 0) ring[array_end & mask] = make_int2(kInitScore, kInitScore);
+      ++array_end;
+    }
+  }
+  *best_score = max_score;
+  *best_p0 = max_p0;
+  *best_p1 = max_p1;
+  *ncells = cells;
+  return true;
+}
+
+// The global ring's jobs, one wave per job (one wave per block), the
+// block's slot of the global ring as its row store.
+template <bool kReverse>
+__global__ void __launch_bounds__(kWave)
+    GappedWaveKernel(GappedParams P, const int *matrix, uint32_t nl, uint32_t *qpos,
+                     uint32_t *dpos, int *scores, uint32_t *cells, const uint32_t *list_count,
+                     const uint32_t *list, int2 *scratch_cells, uint32_t wide,
+                     uint32_t *failed) {
+  __shared__ int8_t sm[kTableDim * kTableDim];
+  StageScoreTable(sm, matrix, nl);
+  __syncthreads();
+  int2 *ring = scratch_cells + (size_t)blockIdx.x * wide;
+  const uint32_t n = *list_count;
+  for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
+    const uint32_t i = list[j];
+    const uint32_t q = qpos[i], d = dpos[i];
+    int s, p0, p1;
+    uint32_t nc;
+    const bool ok = WaveOneSide<kReverse>(ring, wide - 1, P, sm, q, d, &s, &p0, &p1, &nc);
+    if (threadIdx.x == 0) {
+      if (ok) {
+        scores[i] = s;
+        qpos[i] = (uint32_t)((int)q + p0);
+        dpos[i] = (uint32_t)((int)d + p1);
+        if (cells) cells[i] = nc;
+      } else {
+        atomicAdd(failed, 1u);
+      }
     }
   }
 }
@@ -1001,6 +1238,8 @@ inline uint32_t WideSlotsFor(uint32_t wide) {
   return (uint32_t)(n / 64 * 64);
 }
 
+enum : int { kWaveAuto = 0, kWaveNever = 1, kWaveDirect = 2 };
+
 struct gz_gapped_extender_s {
   uint32_t delimiter = kDelimiter;
   const uint64_t *queries = nullptr;
@@ -1011,6 +1250,10 @@ struct gz_gapped_extender_s {
   uint32_t wide = 1u << 16;  // wide-ring columns (>= longest query + 2)
   uint32_t narrow = 32;      // columns of the first LDS ring (16 or 32)
   int persistent = 0;        // first ring: one persistent launch fed from a work queue
+  // bands past the 64-column ring: kWaveAuto (a wave per band when exact),
+  // kWaveNever (a lane per band), kWaveDirect (as kWaveAuto, and the first
+  // ring's overflow skips the 64-column ring); GZ_GAPPED_WAVE=never|direct
+  int wave_mode = kWaveAuto;
   // workspace
   // [0] jobs out of the narrow ring, [1] failures, [2] jobs out of the
   // 64-column ring, [3] the work queue's head
@@ -1037,6 +1280,9 @@ struct gz_gapped_extender_s {
 gz_status gz_gapped_extender_create(gz_gapped_extender_t *h) {
   GZ_REQUIRE(h != nullptr, "null handle pointer");
   *h = new gz_gapped_extender_s();
+  const char *m = getenv("GZ_GAPPED_WAVE");
+  if (m && strcmp(m, "never") == 0) (*h)->wave_mode = kWaveNever;
+  if (m && strcmp(m, "direct") == 0) (*h)->wave_mode = kWaveDirect;
   return GZ_OK;
 }
 gz_status gz_gapped_extender_destroy(gz_gapped_extender_t h) {
@@ -1163,17 +1409,33 @@ static void GappedLaunch(gz_gapped_extender_t h, const GappedWork &w, const Gapp
     }
 #undef GZ_NARROW
   }
-  {
+  // bands past the 64-column ring: one wave per band when that is exact
+  // (the cutoff below -kInitScore and non-negative gap costs), else one lane
+  // per band; a wave takes a lane's slot of the global ring
+  const bool wave = h->wave_mode != kWaveNever && P.cutoff < -kInitScore && P.gap_open >= 0 &&
+                    P.gap_ext >= 0;
+  const dim3 wave_grid(std::min<uint32_t>(w.wide_slots, 4096u));
+  if (wave && h->wave_mode == kWaveDirect) {
+    // the narrow ring's overflow straight to the waves
+    hipLaunchKernelGGL((GappedWaveKernel<kReverse>), wave_grid, dim3(kWave), 0, s, P, h->matrix,
+                       h->number_letters, d_q, d_d, d_scores, d_cells, narrow_out,
+                       w.overflow_list, w.wide_cells, w.wide, failures);
+  } else {
     // the narrow ring's overflow; 16-bit cells leave LDS for more blocks
     const uint32_t *none = nullptr;
     if (short_cells) GZ_RING64(true, dim3(256u * 6), 0, 0, w.overflow_list, narrow_out, none);
     else GZ_RING64(false, dim3(256u * 4), 0, 0, w.overflow_list, narrow_out, none);
   }
 #undef GZ_RING64
-  hipLaunchKernelGGL((GappedWideKernel<kReverse>), dim3(w.wide_slots / kGappedBlock),
-                     dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q, d_d,
-                     d_scores, d_cells, wide_out, w.overflow_list2, w.wide_cells,
-                     w.wide_codes, w.wide, failures);
+  if (wave)
+    hipLaunchKernelGGL((GappedWaveKernel<kReverse>), wave_grid, dim3(kWave), 0, s, P, h->matrix,
+                       h->number_letters, d_q, d_d, d_scores, d_cells, wide_out,
+                       w.overflow_list2, w.wide_cells, w.wide, failures);
+  else
+    hipLaunchKernelGGL((GappedWideKernel<kReverse>), dim3(w.wide_slots / kGappedBlock),
+                       dim3(kGappedBlock), 0, s, P, h->matrix, h->number_letters, d_q, d_d,
+                       d_scores, d_cells, wide_out, w.overflow_list2, w.wide_cells,
+                       w.wide_codes, w.wide, failures);
 }
 }  // extern "C++"
 

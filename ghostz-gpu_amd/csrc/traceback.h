@@ -15,10 +15,8 @@ struct LaneCaps {
   size_t bytes_per_lane = 0;
 };
 
-// The op arena of a lane: the first pass's, and the one a result that
-// outgrew it runs again with.
+// The op arena of a lane.
 constexpr uint32_t kTracebackArena = 1u << 16;
-constexpr uint32_t kTracebackLargeArena = 1u << 25;
 
 LaneCaps TracebackCaps(uint32_t max_query_length, uint32_t max_subject_length,
                        uint32_t arena = kTracebackArena);

@@ -354,6 +354,36 @@ def test_gapped_kernel_wide_band(dev):
         assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("cutoff,gap_open,gap_ext", [(26, 11, 1), (52, 11, 1), (300, 5, 2),
+                                                     (1023, 11, 1), (40, 0, 1), (35, 8, 0)])
+def test_gapped_wave_per_band_vs_oracle(dev, monkeypatch, cutoff, gap_open, gap_ext):
+    """One wave per band (the global ring's jobs): with GZ_GAPPED_WAVE=direct
+    every band that outgrows a 16-column first ring runs on the wave kernel,
+    64 columns per pass.  Long homologous pairs with indels give bands of
+    many passes; cutoffs up to 1023 (the largest for which the wave form is
+    exact), gap opening 0 and gap extension 0 included.  Scores and end
+    positions are the oracle's, in both directions, and equal to the lane
+    form's (GZ_GAPPED_WAVE=never)."""
+    rng = np.random.default_rng(70 + cutoff + gap_open + gap_ext)
+    q, qo, d, do, qpos, dpos = related_pair_set(rng, 400, min_len=100, max_len=2500,
+                                                identity=(0.3, 0.97), indel=0.06)
+    say("wave per band: cutoff", cutoff, "gaps", gap_open, gap_ext, "seeds", qpos.size)
+    for reverse in (False, True):
+        a, b = (qpos - 1, dpos - 1) if reverse else (qpos, dpos)
+        monkeypatch.setenv("GZ_GAPPED_WAVE", "direct")
+        got = np.stack(_gapped_gpu(dev, q, d, a, b, reverse, cutoff, gap_open=gap_open,
+                                   gap_ext=gap_ext, narrow=16), 1).astype(np.int64)
+        monkeypatch.setenv("GZ_GAPPED_WAVE", "never")
+        lane = np.stack(_gapped_gpu(dev, q, d, a, b, reverse, cutoff, gap_open=gap_open,
+                                    gap_ext=gap_ext, narrow=16), 1).astype(np.int64)
+        monkeypatch.delenv("GZ_GAPPED_WAVE")
+        want = _gapped_oracle(q, d, a, b, reverse, cutoff, dev["matrix"], dev["nl"],
+                              gap_open=gap_open, gap_ext=gap_ext)
+        bad = np.nonzero((got != want).any(1))[0]
+        assert bad.size == 0, (reverse, bad[:5], got[bad[:5]], want[bad[:5]])
+        assert np.array_equal(lane, want)
+
+
 def test_gapped_kernel_scores_past_16_bits(dev):
     """Identical 7,000-residue sequences seeded in the middle: each side
     scores past 16384, where the 16-bit first ring hands the band on to the
