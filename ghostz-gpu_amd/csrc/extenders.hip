@@ -798,14 +798,13 @@ __device__ bool WaveOneSide(int2 *ring, uint32_t mask, const GappedParams &P, co
       const bool on = a < ae;
       const int2 cell = on ? ring[a & mask] : make_int2(kNone, kNone);
       const uint32_t code = on ? code_at(a) : 0u;
-      const int up = __shfl_up(cell.x, 1, kWave);
-      const int prev = lane == 0 ? diag : up;
-      diag = __shfl(cell.x, kWave - 1, kWave);  // used only after a full pass
+      const int prev = WaveShiftUp1(cell.x, diag);
+      diag = __builtin_amdgcn_readlane(cell.x, kWave - 1);  // used only after a full pass
       int sp = prev + mrow[code];
       sp = sp < cell.y ? cell.y : sp;
       // horizontal gap into column a
       const int h = on ? sp - gap_init + lane * ge : kNone;
-      const int hx = __shfl_up(WaveInclusiveMax(h), 1, kWave);
+      const int hx = WaveShiftUp1(WaveInclusiveMaxDpp(h, kNone), kNone);
       int g = gin - lane * ge;
       if (lane > 0) {
         const int from = hx - (lane - 1) * ge;
@@ -813,9 +812,9 @@ __device__ bool WaveOneSide(int2 *ring, uint32_t mask, const GappedParams &P, co
       }
       const int sc = sp < g ? g : sp;
       // best score before column a
-      const int mi = WaveInclusiveMax(on ? sc : kNone);
-      int mx = __shfl_up(mi, 1, kWave);
-      mx = lane == 0 ? M : (mx < M ? M : mx);
+      const int mi = WaveInclusiveMaxDpp(on ? sc : kNone, kNone);
+      int mx = WaveShiftUp1(mi, M);
+      mx = mx < M ? M : mx;
       const bool prune = mx - sc > cutoff;
       if (on) {
         const int ng = sc - gap_init;
@@ -832,8 +831,8 @@ __device__ bool WaveOneSide(int2 *ring, uint32_t mask, const GappedParams &P, co
       // the gap after column a, the pass's carries
       const int gout = (sc - gap_init > g - ge) ? sc - gap_init : g - ge;
       const int tail = (ae - a0 < kWave ? ae - a0 : kWave) - 1;
-      gin = __shfl(gout, tail, kWave);
-      const int pm = __shfl(mi, tail, kWave);
+      gin = __builtin_amdgcn_readlane(gout, tail);
+      const int pm = __builtin_amdgcn_readlane(mi, tail);
       M = M < pm ? pm : M;
     }
     if (ae > as0 + 1) gap_row = gin;

@@ -107,6 +107,32 @@ __device__ __forceinline__ T WaveInclusiveMax(T v) {
   return v;
 }
 
+// Inclusive max-scan of an int over the wave in DPP moves (no LDS):
+// Hillis-Steele inside each row of 16 lanes (row_shr 1, 2, 4, 8), then each
+// row's last lane carried into the rows after it (row_bcast 15 into rows 1
+// and 3, row_bcast 31 into rows 2 and 3).  A lane whose source lies outside
+// its row keeps `identity`, the smallest value the scan sees.
+__device__ __forceinline__ int WaveInclusiveMaxDpp(int v, int identity) {
+  int t = __builtin_amdgcn_update_dpp(identity, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v = v < t ? t : v;
+  t = __builtin_amdgcn_update_dpp(identity, v, 0x112, 0xf, 0xf, false);      // row_shr:2
+  v = v < t ? t : v;
+  t = __builtin_amdgcn_update_dpp(identity, v, 0x114, 0xf, 0xf, false);      // row_shr:4
+  v = v < t ? t : v;
+  t = __builtin_amdgcn_update_dpp(identity, v, 0x118, 0xf, 0xf, false);      // row_shr:8
+  v = v < t ? t : v;
+  t = __builtin_amdgcn_update_dpp(identity, v, 0x142, 0xa, 0xf, false);      // row_bcast:15
+  v = v < t ? t : v;
+  t = __builtin_amdgcn_update_dpp(identity, v, 0x143, 0xc, 0xf, false);      // row_bcast:31
+  v = v < t ? t : v;
+  return v;
+}
+
+// Lane i - 1's value (lane 0: fill), one DPP move (wave_shr:1).
+__device__ __forceinline__ int WaveShiftUp1(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xf, 0xf, false);
+}
+
 // Block-wide scan of one value per thread (unsigned types: identity 0).
 // lds must hold blockDim.x / 64 elements; blockDim.x a multiple of 64.
 template <typename T, bool kMax>
