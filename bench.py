@@ -74,7 +74,6 @@ def log(*a):
 # FETCH_SIZE / WRITE_SIZE and calibrated against a copy kernel
 TRAFFIC_PROFILES = {
     "reads1m_db4m": "profiles/hbm_traffic_translated_reads_1m_vs_db_4m.txt",
-    "reads200k_db1m": "profiles/hbm_traffic_translated_reads_200k_vs_db_1m.txt",
 }
 
 
