@@ -373,57 +373,62 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     }
     return 0;
   };
-  //    host part: copy the device's results; false when a traceback read a
-  //    byte an earlier extension left (the chunk must then be replayed in
-  //    the reference's order); host_jobs counts results redone on the host
+  //    host part: copy the device's results and rebuild on the host those
+  //    the device left (a traceback that outgrew its arena); results whose
+  //    traceback read a byte an earlier extension left are rebuilt with the
+  //    chunk's earlier extensions in the reference order when `whole` (the
+  //    parts hold every query of the chunk), else their chunk is listed in
+  //    stale_chunks (to be rebuilt whole); so is a chunk that rebuild
+  //    cannot decide.  host_jobs counts results redone on the host
   auto build_host = [&](std::vector<BuildPart> &parts, uint64_t *host_jobs,
-                        std::vector<size_t> *stale_chunks) {
+                        std::vector<size_t> *stale_chunks, bool whole) {
     for (BuildPart &part : parts) {
       const DbChunk &chunk = db.chunks[part.ci];
       const auto &ids = part.ids;
       const auto &outs = part.outs;
-      bool stale = false;
-      std::vector<size_t> redo;
+      std::vector<size_t> redo, flagged;
       for (size_t k = 0; k < ids.size(); ++k) {
-        if (outs[k].flags & kTracebackStale) stale = true;
-        if (outs[k].flags & kTracebackOverflow) redo.push_back(k);
+        if (outs[k].flags & kTracebackStale) flagged.push_back(k);
+        else if (outs[k].flags & kTracebackOverflow) redo.push_back(k);
       }
       *host_jobs += redo.size();
-      if (!stale) {
-        ParallelBlocks(ids.size(), threads, 4096, [&](size_t b, size_t e) {
-          for (size_t k = b; k < e; ++k) {
-            if (outs[k].flags) continue;
-            const Presearched &pr = pre[ids[k].first][ids[k].second];
-            const TracebackOut &o = outs[k];
-            AlignmentResult &r = results[ids[k].first][ids[k].second];
-            r.chunk = pr.chunk;
-            r.subject = pr.subject;
-            r.score = o.score;
-            r.length = o.length;
-            r.mismatches = o.mismatches;
-            r.gap_openings = o.gap_openings;
-            r.gap_extensions = o.gap_extensions;
-            r.hit = pr.hit;
-            r.start.q = o.start_q;
-            r.start.d = o.start_d;
-            r.end.q = o.end_q;
-            r.end.d = o.end_d;
-          }
-        });
-        // results whose traceback outgrew a lane's arena, each rebuilt on
-        // its own (independent results: any thread, any order)
-        std::atomic<bool> redo_stale(false);
-        ParallelBlocks(redo.size(), threads, 1, [&](size_t b, size_t e) {
-          for (size_t i = b; i < e && !redo_stale.load(std::memory_order_relaxed); ++i) {
-            const size_t k = redo[i];
-            const uint32_t qi = ids[k].first, r = ids[k].second;
-            if (!BuildOneFresh(q, qi, chunk, pre[qi][r], P, result_cutoff, &results[qi][r]))
-              redo_stale.store(true);
-          }
-        });
-        if (redo_stale.load()) stale = true;
-      }
-      if (stale) stale_chunks->push_back(part.ci);
+      ParallelBlocks(ids.size(), threads, 4096, [&](size_t b, size_t e) {
+        for (size_t k = b; k < e; ++k) {
+          if (outs[k].flags) continue;
+          const Presearched &pr = pre[ids[k].first][ids[k].second];
+          const TracebackOut &o = outs[k];
+          AlignmentResult &r = results[ids[k].first][ids[k].second];
+          r.chunk = pr.chunk;
+          r.subject = pr.subject;
+          r.score = o.score;
+          r.length = o.length;
+          r.mismatches = o.mismatches;
+          r.gap_openings = o.gap_openings;
+          r.gap_extensions = o.gap_extensions;
+          r.hit = pr.hit;
+          r.start.q = o.start_q;
+          r.start.d = o.start_d;
+          r.end.q = o.end_q;
+          r.end.d = o.end_d;
+        }
+      });
+      // results whose traceback outgrew a lane's arena, each rebuilt on its
+      // own (independent results: any thread, any order)
+      std::vector<uint8_t> redo_stale(redo.size(), 0);
+      ParallelBlocks(redo.size(), threads, 1, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) {
+          const size_t k = redo[i];
+          const uint32_t qi = ids[k].first, r = ids[k].second;
+          if (!BuildOneFresh(q, qi, chunk, pre[qi][r], P, result_cutoff, &results[qi][r]))
+            redo_stale[i] = 1;
+        }
+      });
+      for (size_t i = 0; i < redo.size(); ++i)
+        if (redo_stale[i]) flagged.push_back(redo[i]);
+      if (flagged.empty()) continue;
+      std::sort(flagged.begin(), flagged.end());
+      if (!whole || !ResolveStaleChunk(q, chunk, P, result_cutoff, pre, results, ids, flagged))
+        stale_chunks->push_back(part.ci);
     }
   };
   //    the whole query chunk, with the reference-order replay of every
@@ -439,7 +444,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     } else {
       GZ_STEP(rc);
       std::vector<size_t> stale;
-      build_host(parts, &st->traceback_host_jobs, &stale);
+      build_host(parts, &st->traceback_host_jobs, &stale, true);
       for (size_t ci : stale) {
         ++st->sequential_build_chunks;
         ReplayChunkHost(q, db.chunks[ci], P, result_cutoff, pre, results,
@@ -640,7 +645,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       if (unsupported || rebuild) continue;
       const double t0 = Now();
       const size_t before = stale_chunks.size();
-      build_host(block_parts[b], &host_jobs, &stale_chunks);
+      build_host(block_parts[b], &host_jobs, &stale_chunks, false);
       std::vector<BuildPart>().swap(block_parts[b]);
       if (stale_chunks.size() != before) {
         rebuild = true;

@@ -5,12 +5,14 @@
 #include <charconv>
 #include <atomic>
 #include <climits>
+#include <functional>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <sstream>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "gz_common.h"
@@ -245,6 +247,14 @@ struct Blocks {
 class Traceback {
  public:
   uint64_t stale = 0;
+  // Bytes of arena 0 that the current extension did not write come from
+  // `resolve` when set (it returns false when it cannot tell); `exotic`
+  // records an extension that used another arena or a byte past kArena,
+  // which the resolver does not model.
+  std::function<bool(int64_t idx, uint8_t *v)> resolve;
+  bool unresolved = false;
+  bool exotic = false;
+  int64_t max_put = -1;  // the current extension's highest arena-0 index written
 
   void Extend(const uint8_t *s0, uint32_t s0len, const uint8_t *s1, bool reversed,
               const int *sm, uint32_t nl, int gap_open, int gap_ext, int cutoff,
@@ -262,6 +272,7 @@ class Traceback {
     rows_base_.clear();
     rows_off_.clear();
     ++serial_;
+    max_put = -1;
     for (auto &a : arenas_) a.used = 0;
     const uint32_t extra = (gap_ext > 0 && cutoff != INT_MAX) ? cutoff / gap_ext + 3 : s0len + 3;
     uint32_t arr = Pick(extra);
@@ -428,16 +439,46 @@ class Traceback {
   void Put(uint32_t arr, int64_t idx, uint8_t v) {
     arenas_[arr].a[idx] = v;
     arenas_[arr].stamp[idx] = serial_;
+    if (arr != 0 || idx >= (int64_t)kArena) exotic = true;
+    else if (idx > max_put) max_put = idx;
   }
   uint8_t Get(uint32_t arr, int64_t idx) {
     Arena &a = arenas_[arr];
     if (idx < 0 || idx >= (int64_t)a.a.size()) {
       ++stale;
+      if (resolve) {
+        // before the arena or past it: 0 in any history while the index is
+        // below kArena (arena 0 never holds fewer bytes); otherwise undecided
+        if (arr != 0 || idx >= (int64_t)kArena) unresolved = true;
+      }
       return 0;
     }
-    if (a.stamp[idx] != serial_) ++stale;
+    if (a.stamp[idx] != serial_) {
+      ++stale;
+      if (resolve) {
+        uint8_t v = 0;
+        if (arr != 0 || idx >= (int64_t)kArena || !resolve(idx, &v)) unresolved = true;
+        return v;
+      }
+    }
     return a.a[idx];
   }
+
+ public:
+  // whether the current (last) extension wrote arena-0 byte idx, and its value
+  bool Wrote(int64_t idx, uint8_t *v) const {
+    if (arenas_.empty() || idx < 0 || idx >= (int64_t)arenas_[0].a.size()) return false;
+    if (arenas_[0].stamp[idx] != serial_) return false;
+    *v = arenas_[0].a[idx];
+    return true;
+  }
+  uint32_t Extensions() const { return serial_; }
+  bool WroteIndex(int64_t idx) const {
+    return !arenas_.empty() && idx >= 0 && idx < (int64_t)arenas_[0].a.size() &&
+           arenas_[0].stamp[idx] == serial_;
+  }
+
+ private:
 
   std::vector<Cell> dp_;
   std::vector<Arena> arenas_;
@@ -546,6 +587,96 @@ void ReplayChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters
       BuildOne(tb, q, qi, c, pre[qi][r], p, cutoff, &results[qi][r]);
     }
   }
+}
+
+bool ResolveStaleChunk(const QuerySet &q, const DbChunk &c, const SearchParameters &p,
+                       int cutoff, std::vector<std::vector<Presearched> > &pre,
+                       std::vector<std::vector<AlignmentResult> > &results,
+                       const std::vector<std::pair<uint32_t, uint32_t> > &ids,
+                       const std::vector<size_t> &flagged) {
+  constexpr uint32_t kMaxLookback = 256;
+  const size_t n = ids.size();
+  // ReplayChunkHost's order: blocks of 32, each from its back; extension
+  // 2t is the reverse side of the t-th result in that order, 2t + 1 its
+  // forward side
+  std::vector<size_t> order;
+  std::vector<size_t> place(n);
+  order.reserve(n);
+  for (size_t b = 0; b < n; b += 32) {
+    const size_t e = std::min(n, b + 32);
+    for (size_t k = e; k-- > b;) {
+      place[k] = order.size();
+      order.push_back(k);
+    }
+  }
+  // one extension on its own: the arena-0 bytes it writes (false when it
+  // uses more than arena 0's first kArena bytes)
+  struct Writes {
+    std::vector<uint8_t> bytes;
+    std::vector<uint8_t> wrote;
+  };
+  std::unordered_map<size_t, Writes> cache;
+  auto writes_of = [&](size_t x, const Writes **out) -> bool {
+    auto it = cache.find(x);
+    if (it != cache.end()) {
+      *out = &it->second;
+      return true;
+    }
+    const std::pair<uint32_t, uint32_t> id = ids[order[x / 2]];
+    const Presearched &pr = pre[id.first][id.second];
+    const uint8_t *qs = q.Sequence(id.first);
+    const uint32_t qlen = q.Length(id.first);
+    const uint8_t *ds = c.seq.data();
+    Traceback tb;
+    Blocks eb;
+    int sc, p0, p1;
+    if (x % 2 == 0)
+      tb.Extend(qs + pr.hit.q - 1, pr.hit.q - 1, ds + pr.hit.d - 1, true, p.matrix, p.nl,
+                p.gap_open, p.gap_extension, cutoff, &sc, &p0, &p1, &eb);
+    else
+      tb.Extend(qs + pr.hit.q, qlen - pr.hit.q, ds + pr.hit.d, false, p.matrix, p.nl,
+                p.gap_open, p.gap_extension, cutoff, &sc, &p0, &p1, &eb);
+    if (tb.exotic) return false;
+    Writes w;
+    const int64_t top = tb.max_put + 1;
+    w.bytes.resize((size_t)std::max<int64_t>(top, 0));
+    w.wrote.resize(w.bytes.size());
+    for (int64_t i = 0; i < top; ++i) {
+      uint8_t v = 0;
+      if (tb.Wrote(i, &v)) {
+        w.bytes[(size_t)i] = v;
+        w.wrote[(size_t)i] = 1;
+      }
+    }
+    if (cache.size() > 64) cache.clear();
+    *out = &cache.emplace(x, std::move(w)).first->second;
+    return true;
+  };
+  for (size_t k : flagged) {
+    const std::pair<uint32_t, uint32_t> id = ids[k];
+    const size_t t = place[k];
+    Traceback tb;
+    // byte idx as the extensions before the current one left it
+    tb.resolve = [&](int64_t idx, uint8_t *v) -> bool {
+      const size_t x = 2 * t + (tb.Extensions() - 1);
+      for (size_t back = 1; back <= x; ++back) {
+        if (back > kMaxLookback) return false;
+        const Writes *w = nullptr;
+        if (!writes_of(x - back, &w)) return false;
+        if ((size_t)idx < w->wrote.size() && w->wrote[(size_t)idx]) {
+          *v = w->bytes[(size_t)idx];
+          return true;
+        }
+      }
+      *v = 0;  // no earlier extension wrote it: the zeroed arena
+      return true;
+    };
+    AlignmentResult r;
+    BuildOne(tb, q, id.first, c, pre[id.first][id.second], p, cutoff, &r);
+    if (tb.unresolved || tb.exotic) return false;
+    results[id.first][id.second] = r;
+  }
+  return true;
 }
 
 void BuildChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters &p, int cutoff,

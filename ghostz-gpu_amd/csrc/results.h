@@ -149,6 +149,16 @@ void ReplayChunkHost(const QuerySet &q, const DbChunk &c, const SearchParameters
                      const std::vector<std::pair<uint32_t, uint32_t> > &ids);
 bool BuildOneFresh(const QuerySet &q, uint32_t qi, const DbChunk &c, const Presearched &pr,
                    const SearchParameters &p, int cutoff, AlignmentResult *out);
+// The results of chunk c at positions `flagged` of `ids` (the chunk's ids
+// in ChunkResultIds order) whose traceback read op bytes that the chunk's
+// earlier extensions left, rebuilt with those bytes: ReplayChunkHost's
+// result for them, without replaying the rest.  False when a read falls
+// outside what it models (the caller then replays the chunk).
+bool ResolveStaleChunk(const QuerySet &q, const DbChunk &c, const SearchParameters &p,
+                       int cutoff, std::vector<std::vector<Presearched> > &pre,
+                       std::vector<std::vector<AlignmentResult> > &results,
+                       const std::vector<std::pair<uint32_t, uint32_t> > &ids,
+                       const std::vector<size_t> &flagged);
 void SortResults(std::vector<std::vector<AlignmentResult> > &results);
 void SortResults(std::vector<std::vector<AlignmentResult> > &results, size_t qb, size_t qe);
 // The result build of one hit with a fresh extender on the framed query

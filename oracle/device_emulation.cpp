@@ -359,4 +359,87 @@ void gzh_distance_words(const uint64_t *qw, const uint64_t *dw, uint64_t n,
   }
 }
 
+/* The chunk replay in the reference's order (ReplayChunkHost) against the
+ * rebuild of only the results that read bytes left by earlier extensions
+ * (ResolveStaleChunk) on one database chunk.  Queries: codes with their
+ * delimiters and nq + 1 offsets (query i's leading delimiter; the last
+ * offset at the closing delimiter); dcodes: the chunk's codes; hits: (query,
+ * hit q, hit d) triples, grouped by query.  out[0] = results whose build on
+ * a fresh extender read such bytes, out[1] = 1 when ResolveStaleChunk
+ * decided them (0: it asked for the replay), out[2] = results of the two
+ * builds that differ, out[3] = flagged results whose fresh build differs
+ * from the replay's (where the earlier extensions' bytes mattered);
+ * results_out (when set): 9 ints per hit as gzo_build_chunk writes them,
+ * the product's results after the rebuild. */
+int gzh_stale_resolution(const uint8_t *qcodes, const uint64_t *qoff, uint32_t nq,
+                         const uint8_t *dcodes, uint32_t dn, const uint32_t *hits, uint32_t nh,
+                         const int *matrix, uint32_t nl, int gap_open, int gap_ext, int cutoff,
+                         uint64_t *out, int32_t *results_out) {
+  gz::QuerySet q;
+  q.codes.assign(qcodes, qcodes + qoff[nq] + 1);  // the last query's closing delimiter
+  q.offsets.assign(qoff, qoff + nq + 1);
+  q.names.assign(nq, "q");
+  gz::DbChunk c;
+  c.seq.assign(dcodes, dcodes + dn);
+  gz::SearchParameters p;
+  p.matrix = matrix;
+  p.nl = nl;
+  p.gap_open = gap_open;
+  p.gap_extension = gap_ext;
+  std::vector<std::vector<gz::Presearched> > pre(nq);
+  for (uint32_t i = 0; i < nh; ++i) {
+    gz::Presearched r = {};
+    r.chunk = 0;
+    r.hit.q = hits[3 * i + 1];
+    r.hit.d = hits[3 * i + 2];
+    pre[hits[3 * i]].push_back(r);
+  }
+  const auto ids = gz::ChunkResultIds(pre, 0);
+  std::vector<std::vector<gz::AlignmentResult> > want(nq), got(nq);
+  for (uint32_t i = 0; i < nq; ++i) {
+    want[i].resize(pre[i].size());
+    got[i].resize(pre[i].size());
+  }
+  gz::ReplayChunkHost(q, c, p, cutoff, pre, want, ids);
+  std::vector<size_t> flagged;
+  for (size_t k = 0; k < ids.size(); ++k)
+    if (!gz::BuildOneFresh(q, ids[k].first, c, pre[ids[k].first][ids[k].second], p, cutoff,
+                           &got[ids[k].first][ids[k].second]))
+      flagged.push_back(k);
+  auto same = [](const gz::AlignmentResult &a, const gz::AlignmentResult &b) {
+    return a.score == b.score && a.length == b.length && a.mismatches == b.mismatches &&
+           a.gap_openings == b.gap_openings && a.gap_extensions == b.gap_extensions &&
+           a.start.q == b.start.q && a.start.d == b.start.d && a.end.q == b.end.q &&
+           a.end.d == b.end.d;
+  };
+  uint64_t history = 0;  // flagged results whose fresh build is not the replay's
+  for (size_t k : flagged)
+    if (!same(got[ids[k].first][ids[k].second], want[ids[k].first][ids[k].second])) ++history;
+  const bool ok = gz::ResolveStaleChunk(q, c, p, cutoff, pre, got, ids, flagged);
+  uint64_t differ = 0;
+  for (uint32_t i = 0; i < nq; ++i)
+    for (size_t r = 0; r < want[i].size(); ++r)
+      if (!same(want[i][r], got[i][r])) ++differ;
+  out[0] = flagged.size();
+  out[1] = ok ? 1 : 0;
+  out[2] = differ;
+  out[3] = history;
+  if (results_out) {
+    for (size_t k = 0; k < ids.size(); ++k) {
+      const gz::AlignmentResult &r = got[ids[k].first][ids[k].second];
+      int32_t *o = results_out + 9 * k;
+      o[0] = r.score;
+      o[1] = (int32_t)r.length;
+      o[2] = (int32_t)r.mismatches;
+      o[3] = (int32_t)r.gap_openings;
+      o[4] = (int32_t)r.gap_extensions;
+      o[5] = (int32_t)r.start.q;
+      o[6] = (int32_t)r.start.d;
+      o[7] = (int32_t)r.end.q;
+      o[8] = (int32_t)r.end.d;
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -637,6 +637,60 @@ struct Result {
   Coordinate hit, start, end;
 };
 
+// One result of BuildResults (aligner_build_results_thread.h:97-150): both
+// sides re-extended with traceback through the extender's arena (reverse
+// side first), then AlignerCommon::BuildResult (src/aligner_common.h:276-343)
+// over the joined edit script.
+void BuildOneResult(const uint8_t *qs, uint32_t qlen, const uint8_t *ds, Coordinate hit,
+                    uint8_t delim, const int *sm, uint32_t nl, int gap_open, int gap_ext,
+                    int result_cutoff, GapArena *arena, Result *out) {
+  EditBlocks eb, tmp;
+  int sum = 0, score, p0, p1;
+  Coordinate start, end;
+  GappedOneSide<true>(qs + hit.q - 1, hit.q - 1, ds + hit.d - 1, delim, true, sm, nl, gap_open,
+                      gap_ext, result_cutoff, &score, &p0, &p1, &tmp, NULL, arena);
+  sum += score;
+  start.q = hit.q - 1 + p0;
+  start.d = hit.d - 1 + p1;
+  tmp.Reverse();
+  eb.Add(tmp);
+  GappedOneSide<true>(qs + hit.q, qlen - hit.q, ds + hit.d, delim, false, sm, nl, gap_open,
+                      gap_ext, result_cutoff, &score, &p0, &p1, &tmp, NULL, arena);
+  sum += score;
+  end.q = hit.q + p0;
+  end.d = hit.d + p1;
+  eb.Add(tmp);
+  std::vector<int> edits = eb.ToVector();
+  uint32_t qp = start.q, dp = start.d, m = 0, o = 0, e = 0;
+  int prev = kEditSub;
+  for (size_t x = 0; x < edits.size(); ++x) {
+    switch (edits[x]) {
+      case kEditSub:
+        if (qs[qp] != ds[dp]) ++m;
+        ++qp;
+        ++dp;
+        break;
+      case kEditGap0:
+        if (prev != kEditGap0) ++o; else ++e;
+        ++dp;
+        break;
+      case kEditGap1:
+        if (prev != kEditGap1) ++o; else ++e;
+        ++qp;
+        break;
+    }
+    prev = edits[x];
+  }
+  out->score = sum;
+  out->length = (uint32_t)edits.size();
+  out->mismatches = m;
+  out->gap_openings = o;
+  out->gap_extensions = e;
+  out->hit = hit;
+  out->start = start;
+  out->end = end;
+}
+
 // AlignerCommon::AlignmentComp (src/aligner_common.cpp:12-56)
 int AlignmentComp(int as, const Coordinate &a0, const Coordinate &a1, int bs,
                   const Coordinate &b0, const Coordinate &b1) {
@@ -982,6 +1036,43 @@ int gzo_gapped_one_side(const uint8_t *seq0, uint32_t seq0_length,
   return 0;
 }
 
+int gzo_build_chunk(const uint8_t *queries, const uint64_t *query_offsets,
+                    uint32_t number_queries, const uint8_t *chunk, const uint32_t *hits,
+                    uint32_t number_hits, uint8_t delimiter, const int *matrix, uint32_t nl,
+                    int gap_open, int gap_extension, int result_cutoff, int32_t *out) {
+  (void)number_queries;
+  // BuildResults of one chunk (src/aligner.cpp:264-348,
+  // aligner_build_results_thread.h:64-190): the results in query order,
+  // handed out in blocks of 32 and popped from the back of each block, all
+  // through one extender object
+  GapArena arena;
+  for (uint32_t block = 0; block < number_hits; block += 32) {
+    const uint32_t bend = std::min(number_hits, block + 32);
+    for (uint32_t k = bend; k-- > block;) {
+      const uint32_t qi = hits[3 * k];
+      const uint8_t *qs = queries + query_offsets[qi];
+      const uint32_t qlen = (uint32_t)(query_offsets[qi + 1] - query_offsets[qi]);
+      Coordinate hit;
+      hit.q = hits[3 * k + 1];
+      hit.d = hits[3 * k + 2];
+      Result r;
+      BuildOneResult(qs, qlen, chunk, hit, delimiter, matrix, nl, gap_open, gap_extension,
+                     result_cutoff, &arena, &r);
+      int32_t *o = out + 9 * (size_t)k;
+      o[0] = r.score;
+      o[1] = (int32_t)r.length;
+      o[2] = (int32_t)r.mismatches;
+      o[3] = (int32_t)r.gap_openings;
+      o[4] = (int32_t)r.gap_extensions;
+      o[5] = (int32_t)r.start.q;
+      o[6] = (int32_t)r.start.d;
+      o[7] = (int32_t)r.end.q;
+      o[8] = (int32_t)r.end.d;
+    }
+  }
+  return 0;
+}
+
 uint32_t gzo_chain_filter(const uint8_t *query, uint32_t qlen, int cutoff,
                           const uint8_t *database, uint8_t delimiter,
                           const int *matrix, uint32_t nl, uint32_t *hits,
@@ -1272,57 +1363,11 @@ int gzo_align(const gzo_queries *Qp, const gzo_database *Dp,
         const uint8_t *qs = Q.sequences + Q.offsets[qi];
         const uint32_t qlen = (uint32_t)(Q.offsets[qi + 1] - Q.offsets[qi]);
         const Presearched &pr = presearch[qi][ri];
-        EditBlocks eb, tmp;
-        int sum = 0, score, p0, p1;
-        Coordinate hit = pr.hit, start, end;
-        GappedOneSide<true>(qs + hit.q - 1, hit.q - 1, ds + hit.d - 1, delim,
-                            true, sm, nl, P.gap_open, P.gap_extension,
-                            result_cutoff, &score, &p0, &p1, &tmp, NULL, &arena);
-        sum += score;
-        start.q = hit.q - 1 + p0;
-        start.d = hit.d - 1 + p1;
-        tmp.Reverse();
-        eb.Add(tmp);
-        GappedOneSide<true>(qs + hit.q, qlen - hit.q, ds + hit.d, delim, false,
-                            sm, nl, P.gap_open, P.gap_extension, result_cutoff,
-                            &score, &p0, &p1, &tmp, NULL, &arena);
-        sum += score;
-        end.q = hit.q + p0;
-        end.d = hit.d + p1;
-        eb.Add(tmp);
-        std::vector<int> edits = eb.ToVector();
-        // AlignerCommon::BuildResult (src/aligner_common.h:276-343)
-        uint32_t qp = start.q, dp = start.d, m = 0, o = 0, e = 0;
-        int prev = kEditSub;
-        for (size_t x = 0; x < edits.size(); ++x) {
-          switch (edits[x]) {
-            case kEditSub:
-              if (qs[qp] != ds[dp]) ++m;
-              ++qp;
-              ++dp;
-              break;
-            case kEditGap0:
-              if (prev != kEditGap0) ++o; else ++e;
-              ++dp;
-              break;
-            case kEditGap1:
-              if (prev != kEditGap1) ++o; else ++e;
-              ++qp;
-              break;
-          }
-          prev = edits[x];
-        }
         Result &r = results[qi][ri];
+        BuildOneResult(qs, qlen, ds, pr.hit, delim, sm, nl, P.gap_open, P.gap_extension,
+                       result_cutoff, &arena, &r);
         r.chunk = pr.chunk;
         r.subject = pr.subject;
-        r.score = sum;
-        r.length = (uint32_t)edits.size();
-        r.mismatches = m;
-        r.gap_openings = o;
-        r.gap_extensions = e;
-        r.hit = hit;
-        r.start = start;
-        r.end = end;
       }
     }
     g_stats.stale_traceback_reads += arena.stale_reads;

@@ -68,6 +68,16 @@ int gzo_gapped_one_side(const uint8_t *seq0, uint32_t seq0_length,
 /* ChainFilter::Filter (src/chain_filter.cpp:24-99).  hits is an array of
  * (query_position, database_position) pairs, filtered in place; returns the
  * number kept. */
+/* BuildResults of one database chunk with one extender (the reference's
+ * order: results in query order, blocks of 32 popped from the back): hits
+ * are (query, hit q, hit d) triples grouped by query, queries' offsets point
+ * at each query's leading delimiter (nq + 1 of them).  out: 9 ints per hit
+ * (score, length, mismatches, gap openings, gap extensions, start q, start
+ * d, end q, end d), in the hits' order.  Returns 0. */
+int gzo_build_chunk(const uint8_t *queries, const uint64_t *query_offsets,
+                    uint32_t number_queries, const uint8_t *chunk, const uint32_t *hits,
+                    uint32_t number_hits, uint8_t delimiter, const int *matrix, uint32_t nl,
+                    int gap_open, int gap_extension, int result_cutoff, int32_t *out);
 uint32_t gzo_chain_filter(const uint8_t *query, uint32_t query_length,
                           int cutoff, const uint8_t *database,
                           uint8_t delimiter, const int *matrix,

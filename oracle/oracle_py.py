@@ -42,6 +42,10 @@ def oracle():
                                                 c_void_p, c_uint32, c_int, c_int]
         L.gzo_distance.restype = c_int
         L.gzo_distance.argtypes = [c_void_p, c_uint32, c_void_p, c_uint32, c_uint32, c_void_p]
+        L.gzo_build_chunk.restype = c_int
+        L.gzo_build_chunk.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_uint32,
+                                      c_uint8, c_void_p, c_uint32, c_int, c_int, c_int,
+                                      c_void_p]
         L.gzo_gapped_one_side.restype = c_int
         L.gzo_gapped_one_side.argtypes = [c_void_p, c_uint32, c_void_p, c_uint8, c_int,
                                           c_void_p, c_uint32, c_int, c_int, c_int,
@@ -360,10 +364,48 @@ def hostpipe(path=None):
                                          c_uint32, c_void_p, c_void_p]
         L.gzh_format_g6_check.restype = c_uint64
         L.gzh_format_g6_check.argtypes = [c_void_p, c_uint64, c_void_p, _P(c_uint64)]
+        L.gzh_stale_resolution.restype = c_int
+        L.gzh_stale_resolution.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32,
+                                           c_void_p, c_uint32, c_void_p, c_uint32, c_int, c_int,
+                                           c_int, c_void_p, c_void_p]
         if path:
             return L
         _hostpipe = L
     return _hostpipe
+
+
+def stale_resolution(qcodes, qoff, dcodes, hits, matrix, nl, gap_open, gap_ext, cutoff,
+                     lib=None):
+    """gzh_stale_resolution: (results read history bytes, resolved, differ from
+    the replay, history-dependent) for one chunk; qoff holds each query's
+    leading delimiter and, last, the closing delimiter."""
+    L = lib or hostpipe()
+    qc = np.ascontiguousarray(qcodes, np.uint8)
+    qo = np.ascontiguousarray(qoff, np.uint64)
+    dc = np.ascontiguousarray(dcodes, np.uint8)
+    h = np.ascontiguousarray(hits, np.uint32).ravel()
+    m = np.ascontiguousarray(matrix, np.int32)
+    out = np.zeros(4, np.uint64)
+    res = np.zeros((h.size // 3, 9), np.int32)
+    rc = L.gzh_stale_resolution(ptr(qc), ptr(qo), len(qo) - 1, ptr(dc), dc.size, ptr(h),
+                                h.size // 3, ptr(m), nl, gap_open, gap_ext, cutoff, ptr(out),
+                                ptr(res))
+    if rc != 0:
+        raise RuntimeError(f"gzh_stale_resolution failed ({rc})")
+    return [int(x) for x in out], res
+
+
+def build_chunk(qcodes, qoff, dcodes, hits, matrix, nl, gap_open, gap_ext, cutoff, delim):
+    """The oracle's BuildResults of one chunk (gzo_build_chunk): 9 ints per hit."""
+    qc = np.ascontiguousarray(qcodes, np.uint8)
+    qo = np.ascontiguousarray(qoff, np.uint64)
+    dc = np.ascontiguousarray(dcodes, np.uint8)
+    h = np.ascontiguousarray(hits, np.uint32).ravel()
+    m = np.ascontiguousarray(matrix, np.int32)
+    res = np.zeros((h.size // 3, 9), np.int32)
+    oracle().gzo_build_chunk(ptr(qc), ptr(qo), len(qo) - 1, ptr(dc), ptr(h), h.size // 3, delim,
+                             ptr(m), nl, gap_open, gap_ext, cutoff, ptr(res))
+    return res
 
 
 def host_align(queries, database, params, translated=False, filter=True, clustering=True,
