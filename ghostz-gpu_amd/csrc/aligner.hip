@@ -245,7 +245,8 @@ int Aligner::SeedHits(size_t ci, uint32_t qb, uint32_t qe, SeedHitView *hits,
     lookup_chunk_ = ci;
   }
   SeedStageStats ss;
-  const hipError_t e = seed_.Run(dq_, ddb, sp_, qb, qe, stream_, hits, &ss, seed_kinds_);
+  const hipError_t e =
+      seed_.Run(dq_, ddb, sp_, qb, qe, stream_, hits, &ss, seed_kinds_, /*compact=*/true);
   if (e != hipSuccess) return (int)e;
   st->units_rep += ss.units_rep;
   st->units_value += ss.units_value;

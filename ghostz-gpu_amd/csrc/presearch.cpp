@@ -89,7 +89,8 @@ struct KeyedHit {
 
 // (dpos << 32 | index) of n hits sorted by dpos into *keys (LSD radix over
 // the bytes of dpos that differ); false when two hits share a dpos.
-bool SortByPosition(const SeedHit *hits, uint32_t n, std::vector<uint64_t> *keys,
+template <class Hits>
+bool SortByPosition(const Hits *hits, uint32_t n, std::vector<uint64_t> *keys,
                     std::vector<uint64_t> *tmp) {
   keys->resize(n);
   tmp->resize(n);
@@ -155,35 +156,53 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   std::vector<std::vector<AlignmentResult> > results(nq);
 
   // 2. per-query enumeration order and chain filter of queries [qb, qe)
-  //    (hits: the seed view of a run that started at query hb)
+  //    (hits: the seed view of a run that started at query hb); a failed
+  //    copy of a compact view's records is left in chain_error
+  std::atomic<int> chain_error(0);
   auto chain = [&](const DbChunk &chunk, const SeedHitView &hits, size_t hb, size_t qb,
                    size_t qe) {
     ParallelBlocks(qe - qb, threads, 64, [&](size_t b, size_t e) {
       std::vector<uint64_t> scratch, by_d, by_d_tmp;
       std::vector<KeyedHit> keyed;
+      std::vector<SeedHit> fetched;
       for (size_t qi = qb + b; qi < qb + e; ++qi) {
         const uint32_t s0 = hits.qstart[qi - hb], s1 = hits.qstart[qi - hb + 1];
         if (s0 == s1) continue;
         // ChainFilter sorts by database position with an unstable sort.
         // When no two hits share a position that order is unique, and the
         // enumeration order it starts from does not matter.
-        if (SortByPosition(hits.hits + s0, s1 - s0, &by_d, &by_d_tmp)) {
+        const bool unique = hits.hits ? SortByPosition(hits.hits + s0, s1 - s0, &by_d, &by_d_tmp)
+                                      : SortByPosition(hits.positions + s0, s1 - s0, &by_d,
+                                                       &by_d_tmp);
+        if (unique) {
           std::vector<Hit> &hv = chained[qi];
           hv.resize(s1 - s0);
           for (uint32_t k = 0; k < s1 - s0; ++k) {
-            const SeedHit &h = hits.hits[s0 + (uint32_t)by_d[k]];
-            hv[k].q = h.qpos - (uint32_t)coff[qi];
-            hv[k].d = h.dpos;
+            const uint32_t x = s0 + (uint32_t)by_d[k];
+            hv[k].q = (hits.hits ? hits.hits[x].qpos : hits.positions[x].qpos) -
+                      (uint32_t)coff[qi];
+            hv[k].d = hits.hits ? hits.hits[x].dpos : hits.positions[x].dpos;
           }
           ChainFilterSorted(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix,
                             P.nl, hv, scratch);
           continue;
         }
         // the (list, rv, member, entry) keys are distinct, so sorting them
-        // gives exactly the one-thread enumeration order
+        // gives exactly the one-thread enumeration order (a compact view's
+        // records come from the device for such queries)
+        const SeedHit *rec = hits.hits ? hits.hits + s0 : nullptr;
+        if (!rec) {
+          fetched.resize(s1 - s0);
+          const int f = hits.fetch(s0, s1 - s0, fetched.data());
+          if (f != 0) {
+            chain_error.store(f);
+            continue;
+          }
+          rec = fetched.data();
+        }
         keyed.resize(s1 - s0);
         for (uint32_t k = 0; k < s1 - s0; ++k) {
-          const SeedHit &h = hits.hits[s0 + k];
+          const SeedHit &h = rec[k];
           keyed[k].hi = ((uint64_t)h.list << 32) | h.rv;
           keyed[k].lo = ((uint64_t)h.member << 32) | h.entry;
           keyed[k].q = h.qpos - (uint32_t)coff[qi];
@@ -466,6 +485,7 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
       st->seconds_seed += Now() - t;
       t = Now();
       chain(chunk, hits, 0, 0, nq);
+      GZ_STEP(chain_error.load());
       st->seconds_chain += Now() - t;
       t = Now();
       Block all;
@@ -680,7 +700,9 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
     chain(db.chunks[ci], views[t], blocks[k].qb, blocks[k].qb, blocks[k].qe);
     sec_chain += Now() - t0;
     trace.Add("chain", t, t0, Now());
-    publish(chained_upto, t + 1, 0);
+    const int rc = chain_error.load();
+    publish(chained_upto, t + 1, rc);
+    if (rc != 0) break;
   }
   seeder.join();
   scheduler.join();

@@ -9,6 +9,8 @@
 
 #include <stdint.h>
 
+#include <functional>
+
 #include <string>
 #include <vector>
 
@@ -32,12 +34,22 @@ struct SeedHit {
   uint32_t entry;  // entry index within the list
 };
 
+// A passing seed's query and database positions.
+struct HitPosition {
+  uint32_t qpos, dpos;
+};
+
 // Passing seeds of one database chunk for queries [qb, qe) grouped by query:
 // the hits of query qb + i are hits[qstart[i] .. qstart[i+1]) in no
 // particular order.  The arrays belong to the device stage and stay valid
-// until its second SeedHits call after this one.
+// until its second SeedHits call after this one.  A compact view carries
+// only the positions (hits == nullptr): positions[k] is hit k's, and
+// fetch(begin, count, out) copies the full records of hits [begin, begin +
+// count) (for queries whose order needs them).
 struct SeedHitView {
   const SeedHit *hits = nullptr;
+  const HitPosition *positions = nullptr;
+  std::function<int(size_t begin, size_t count, SeedHit *out)> fetch;
   size_t count = 0;
   const uint32_t *qstart = nullptr;  // number of queries + 1
 };

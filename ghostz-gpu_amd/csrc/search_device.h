@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <vector>
 
 #include "presearch.h"
@@ -90,7 +92,8 @@ class SeedStage {
   // and their members, bit 1 the non-clustered values.
   hipError_t Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
                  const SeedStageParams &p, uint32_t qb, uint32_t qe, hipStream_t stream,
-                 SeedHitView *hits, SeedStageStats *stats, unsigned kinds = 3);
+                 SeedHitView *hits, SeedStageStats *stats, unsigned kinds = 3,
+                 bool compact = false);
 
  private:
   hipError_t Reserve(void **p, size_t *cap, size_t bytes);
@@ -99,7 +102,12 @@ class SeedStage {
   void *scan_tmp_ = nullptr; size_t scan_tmp_cap_ = 0;
   void *active_ = nullptr; size_t active_cap_ = 0;
   void *hits_ = nullptr; size_t hits_cap_ = 0;
-  void *grouped_ = nullptr; size_t grouped_cap_ = 0;   // hits grouped by query
+  // hits grouped by query, per host buffer (a compact view's records stay
+  // on the device until that buffer's next Run)
+  void *grouped_[2] = {nullptr, nullptr}; size_t grouped_cap_[2] = {0, 0};
+  void *positions_ = nullptr; size_t positions_cap_ = 0;  // compact views' positions
+  hipStream_t fetch_stream_ = nullptr;  // copies of a compact view's records
+  std::mutex fetch_mutex_;
   void *tile_range_ = nullptr; size_t tile_range_cap_ = 0;  // per-tile list ranges
   void *qcount_ = nullptr; size_t qcount_cap_ = 0;     // per-query counts / cursors
   void *base_info_ = nullptr; size_t base_info_cap_ = 0;  // per-list database ranges

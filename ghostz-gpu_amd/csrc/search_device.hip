@@ -482,20 +482,25 @@ __global__ void QueryCountKernel(const SeedHit *hits, size_t n, uint32_t qb, uin
   if (i < n) atomicAdd(&count[hits[i].qid - qb], 1u);
 }
 
+// (positions, when set, receives each grouped hit's query and database
+// positions at the same index)
 __global__ void QueryScatterKernel(const SeedHit *hits, size_t n, uint32_t qb,
-                                   uint32_t *cursor, SeedHit *out) {
+                                   uint32_t *cursor, SeedHit *out, HitPosition *positions) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const SeedHit h = hits[i];
-    out[atomicAdd(&cursor[h.qid - qb], 1u)] = h;
+    const uint32_t k = atomicAdd(&cursor[h.qid - qb], 1u);
+    out[k] = h;
+    if (positions) positions[k] = HitPosition{h.qpos, h.dpos};
   }
 }
 
 }  // namespace
 
 SeedStage::~SeedStage() {
-  void *ps[] = {lookup_,  units_,  scan_tmp_,   active_,   hits_,
-                grouped_, qcount_, tile_range_, base_info_, entry_rec_};
+  void *ps[] = {lookup_,     units_,      scan_tmp_,  active_,    hits_,
+                grouped_[0], grouped_[1], positions_, qcount_,    tile_range_,
+                base_info_,  entry_rec_};
   for (void *p : ps)
     if (p) (void)gz::Free(p);
   for (int k = 0; k < 2; ++k) {
@@ -506,6 +511,7 @@ SeedStage::~SeedStage() {
   if (pinned_) (void)gz::HostFree(pinned_);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
+  if (fetch_stream_) (void)hipStreamDestroy(fetch_stream_);
 }
 
 hipError_t SeedStage::Reserve(void **p, size_t *cap, size_t bytes) {
@@ -556,7 +562,8 @@ hipError_t SeedStage::Lookup(const DeviceQueryLists &q, const DeviceDbChunk &db,
 
 hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
                           const SeedStageParams &p, uint32_t qb, uint32_t qe, hipStream_t s,
-                          SeedHitView *out, SeedStageStats *stats, unsigned kinds) {
+                          SeedHitView *out, SeedStageStats *stats, unsigned kinds,
+                          bool compact) {
   *out = SeedHitView();
   if (!looked_up_ || qb > qe || qe > q.nqueries || (qb & 63) != 0 ||
       ((qe & 63) != 0 && qe != q.nqueries))
@@ -726,7 +733,8 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
   hipLaunchKernelGGL(FillU32Kernel, dim3((nq + 1 + 255) / 256), dim3(256), 0, s, qcount,
                      (size_t)nq + 1, 0u);
   if (total_hits) {
-    GZ_TRY(Reserve(&grouped_, &grouped_cap_, total_hits * sizeof(SeedHit)));
+    GZ_TRY(Reserve(&grouped_[slot], &grouped_cap_[slot], total_hits * sizeof(SeedHit)));
+    if (compact) GZ_TRY(Reserve(&positions_, &positions_cap_, total_hits * sizeof(HitPosition)));
     const unsigned g = (unsigned)((total_hits + 255) / 256);
     hipLaunchKernelGGL(QueryCountKernel, dim3(g), dim3(256), 0, s, (const SeedHit *)hits_,
                        (size_t)total_hits, qb, qcount);
@@ -738,9 +746,11 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
     GZ_TRY(hipMemcpyAsync(qcount, qstart, (size_t)nq * 4, hipMemcpyDeviceToDevice, s));
     const unsigned g = (unsigned)((total_hits + 255) / 256);
     hipLaunchKernelGGL(QueryScatterKernel, dim3(g), dim3(256), 0, s, (const SeedHit *)hits_,
-                       (size_t)total_hits, qb, qcount, (SeedHit *)grouped_);
+                       (size_t)total_hits, qb, qcount, (SeedHit *)grouped_[slot],
+                       compact ? (HitPosition *)positions_ : nullptr);
     GZ_TRY(hipGetLastError());
-    const size_t bytes = total_hits * sizeof(SeedHit);
+    // a compact view moves 8 bytes per hit to the host instead of 28
+    const size_t bytes = total_hits * (compact ? sizeof(HitPosition) : sizeof(SeedHit));
     if (host_hits_cap_[slot] < bytes) {
       if (host_hits_[slot]) GZ_TRY(gz::HostFree(host_hits_[slot]));
       host_hits_[slot] = nullptr;
@@ -749,10 +759,27 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
       GZ_TRY(gz::HostMalloc(&host_hits_[slot], want, hipHostMallocDefault));
       host_hits_cap_[slot] = want;
     }
-    GZ_TRY(hipMemcpyAsync(host_hits_[slot], grouped_, bytes, hipMemcpyDeviceToHost, s));
+    GZ_TRY(hipMemcpyAsync(host_hits_[slot], compact ? positions_ : grouped_[slot], bytes,
+                          hipMemcpyDeviceToHost, s));
   }
   GZ_TRY(hipStreamSynchronize(s));
-  out->hits = (const SeedHit *)host_hits_[slot];
+  if (compact) {
+    out->positions = (const HitPosition *)host_hits_[slot];
+    const SeedHit *dev = (const SeedHit *)grouped_[slot];
+    out->fetch = [this, dev](size_t begin, size_t count, SeedHit *dst) -> int {
+      std::lock_guard<std::mutex> lock(fetch_mutex_);
+      if (!fetch_stream_) {
+        const hipError_t e = hipStreamCreateWithFlags(&fetch_stream_, hipStreamNonBlocking);
+        if (e != hipSuccess) return (int)e;
+      }
+      hipError_t e = hipMemcpyAsync(dst, dev + begin, count * sizeof(SeedHit),
+                                    hipMemcpyDeviceToHost, fetch_stream_);
+      if (e == hipSuccess) e = hipStreamSynchronize(fetch_stream_);
+      return (int)e;
+    };
+  } else {
+    out->hits = (const SeedHit *)host_hits_[slot];
+  }
   out->count = total_hits;
   out->qstart = h_qstart;
   return hipSuccess;

@@ -66,6 +66,18 @@ gz_status Search(gz_seed_searcher_t h, uint32_t qb, uint32_t qe, unsigned kinds,
     SetError(std::string("seed search: ") + hipGetErrorString((hipError_t)r));
     return GZ_ERROR_HIP;
   }
+  // the full records (a compact view keeps them on the device)
+  std::vector<SeedHit> full;
+  const SeedHit *hits = v.hits;
+  if (!hits && v.count) {
+    full.resize(v.count);
+    const int f = v.fetch(0, v.count, full.data());
+    if (f != 0) {
+      SetError(std::string("seed search: ") + hipGetErrorString((hipError_t)f));
+      return GZ_ERROR_HIP;
+    }
+    hits = full.data();
+  }
   // per query, the one-thread enumeration order: by (list rank,
   // representative or value index, member, entry), which are distinct
   const uint32_t n = qe - qb;
@@ -83,7 +95,7 @@ gz_status Search(gz_seed_searcher_t h, uint32_t qb, uint32_t qe, unsigned kinds,
     h->qoff[i] = s0;
     keyed.resize(s1 - s0);
     for (uint32_t k = s0; k < s1; ++k) {
-      const SeedHit &x = v.hits[k];
+      const SeedHit &x = hits[k];
       keyed[k - s0] = {((uint64_t)x.list << 32) | x.rv, ((uint64_t)x.member << 32) | x.entry,
                        x.qpos - (uint32_t)h->coff[qb + i], x.dpos};
     }
