@@ -120,6 +120,7 @@ class SeedStage {
   size_t host_qstart_cap_[2] = {0, 0};
   uint32_t *counters_ = nullptr;
   hipEvent_t ev_[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> launch_ev_;  // a pair per seed-kernel launch of a run
   void *pinned_ = nullptr; size_t pinned_cap_ = 0;
 };
 

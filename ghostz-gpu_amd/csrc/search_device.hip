@@ -22,7 +22,7 @@ constexpr int kSeedThreads = 256;
 constexpr int kSeedItems = 2;
 constexpr uint32_t kSeedTile = kSeedThreads * kSeedItems;
 constexpr uint64_t kUnitsPerLaunch = 1ull << 28;
-constexpr size_t kInitialHitCapacity = 1u << 22;
+constexpr size_t kInitialHitCapacity = 1u << 25;
 
 // The hit buffer's first size: kInitialHitCapacity, or GZ_SEED_HIT_CAPACITY
 // seeds when that variable is set (a small value makes every launch take the
@@ -511,6 +511,8 @@ SeedStage::~SeedStage() {
   if (pinned_) (void)gz::HostFree(pinned_);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : launch_ev_)
+    if (e) (void)hipEventDestroy(e);
   if (fetch_stream_) (void)hipStreamDestroy(fetch_stream_);
 }
 
@@ -658,71 +660,95 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
     GZ_TRY(hipGetDevice(&dev));
     int cus = 256;
     GZ_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    // passing seeds accumulate in hits_; counters_[0] is their running count
-    GZ_TRY(hipMemsetAsync(counters_, 0, kCounters * sizeof(uint32_t), s));
-    for (int k = 0; k < 2; ++k) {
-      if (totals[k] == 0 || !((kinds >> k) & 1)) continue;
-      for (uint64_t u0 = 0; u0 < totals[k];) {
-        const uint64_t u1 = std::min(totals[k], u0 + kUnitsPerLaunch);
-        const uint32_t cap = (uint32_t)std::min<size_t>(hits_cap_ / sizeof(SeedHit), 0xffffffffu);
-        const uint64_t tiles = (u1 - u0 + kSeedTile - 1) / kSeedTile;
-        const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)cus * 8);
-        GZ_TRY(Reserve(&tile_range_, &tile_range_cap_, tiles * sizeof(uint2)));
-        hipLaunchKernelGGL(TileRangeKernel, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0,
-                           s, act_off[k], nact[k], u0, u1, (uint32_t)tiles,
-                           (uint2 *)tile_range_);
-        GZ_TRY(hipGetLastError());
-        GZ_TRY(hipEventRecord(ev_[0], s));
-        // byte score tables when every cutoff and score fits them
-        const uint2 *tr = (const uint2 *)tile_range_;
-        if (k == 0 && p.byte_tables) {
-          hipLaunchKernelGGL((SeedKernel<false, int8_t>), dim3(grid), dim3(kSeedThreads), 0, s,
-                             q, db, p, info, act[0], act_off[0], tr, u0, u1, (SeedHit *)hits_,
-                             cap, counters_);
-        } else if (k == 0) {
-          hipLaunchKernelGGL((SeedKernel<false, int16_t>), dim3(grid), dim3(kSeedThreads), 0, s,
-                             q, db, p, info, act[0], act_off[0], tr, u0, u1, (SeedHit *)hits_,
-                             cap, counters_);
-        } else if (p.byte_tables) {
-          hipLaunchKernelGGL((SeedKernel<true, int8_t>), dim3(grid), dim3(kSeedThreads), 0, s, q,
-                             db, p, info, act[1], act_off[1], tr, u0, u1, (SeedHit *)hits_, cap,
-                             counters_);
-        } else {
-          hipLaunchKernelGGL((SeedKernel<true, int16_t>), dim3(grid), dim3(kSeedThreads), 0, s,
-                             q, db, p, info, act[1], act_off[1], tr, u0, u1, (SeedHit *)hits_,
-                             cap, counters_);
+    // passing seeds accumulate in hits_; counters_[0] is their running
+    // count.  Every range of both kinds is queued without a host round trip
+    // (a wait between launches would hand the seed kernels' compute units to
+    // the other streams' long kernels); the kernels never write past the
+    // buffer, and a run whose count passed it is redone whole with a buffer
+    // that holds it.
+    uint64_t most_tiles = 1;
+    for (int k = 0; k < 2; ++k)
+      if (totals[k] && ((kinds >> k) & 1))
+        most_tiles = std::max<uint64_t>(
+            most_tiles, (std::min<uint64_t>(totals[k], kUnitsPerLaunch) + kSeedTile - 1) / kSeedTile);
+    GZ_TRY(Reserve(&tile_range_, &tile_range_cap_, most_tiles * sizeof(uint2)));
+    for (;;) {
+      GZ_TRY(hipMemsetAsync(counters_, 0, kCounters * sizeof(uint32_t), s));
+      const uint32_t cap = (uint32_t)std::min<size_t>(hits_cap_ / sizeof(SeedHit), 0xffffffffu);
+      size_t nev = 0;
+      int kind_of[2] = {0, 0};
+      for (int k = 0; k < 2; ++k) {
+        if (totals[k] == 0 || !((kinds >> k) & 1)) continue;
+        for (uint64_t u0 = 0; u0 < totals[k];) {
+          const uint64_t u1 = std::min(totals[k], u0 + kUnitsPerLaunch);
+          const uint64_t tiles = (u1 - u0 + kSeedTile - 1) / kSeedTile;
+          const unsigned grid = (unsigned)std::min<uint64_t>(tiles, (uint64_t)cus * 8);
+          hipLaunchKernelGGL(TileRangeKernel, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0,
+                             s, act_off[k], nact[k], u0, u1, (uint32_t)tiles,
+                             (uint2 *)tile_range_);
+          GZ_TRY(hipGetLastError());
+          if (stats) {
+            while (launch_ev_.size() < 2 * nev + 2) {
+              hipEvent_t e = nullptr;
+              GZ_TRY(hipEventCreate(&e));
+              launch_ev_.push_back(e);
+            }
+            GZ_TRY(hipEventRecord(launch_ev_[2 * nev], s));
+          }
+          // byte score tables when every cutoff and score fits them
+          const uint2 *tr = (const uint2 *)tile_range_;
+          if (k == 0 && p.byte_tables) {
+            hipLaunchKernelGGL((SeedKernel<false, int8_t>), dim3(grid), dim3(kSeedThreads), 0,
+                               s, q, db, p, info, act[0], act_off[0], tr, u0, u1,
+                               (SeedHit *)hits_, cap, counters_);
+          } else if (k == 0) {
+            hipLaunchKernelGGL((SeedKernel<false, int16_t>), dim3(grid), dim3(kSeedThreads), 0,
+                               s, q, db, p, info, act[0], act_off[0], tr, u0, u1,
+                               (SeedHit *)hits_, cap, counters_);
+          } else if (p.byte_tables) {
+            hipLaunchKernelGGL((SeedKernel<true, int8_t>), dim3(grid), dim3(kSeedThreads), 0, s,
+                               q, db, p, info, act[1], act_off[1], tr, u0, u1,
+                               (SeedHit *)hits_, cap, counters_);
+          } else {
+            hipLaunchKernelGGL((SeedKernel<true, int16_t>), dim3(grid), dim3(kSeedThreads), 0,
+                               s, q, db, p, info, act[1], act_off[1], tr, u0, u1,
+                               (SeedHit *)hits_, cap, counters_);
+          }
+          GZ_TRY(hipGetLastError());
+          if (stats) {
+            GZ_TRY(hipEventRecord(launch_ev_[2 * nev + 1], s));
+            ++nev;
+            ++kind_of[k];
+          }
+          u0 = u1;
         }
-        GZ_TRY(hipGetLastError());
-        GZ_TRY(hipEventRecord(ev_[1], s));
-        uint32_t *hc = (uint32_t *)pinned_;
-        GZ_TRY(hipMemcpyAsync(hc, counters_, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        GZ_TRY(hipStreamSynchronize(s));
-        const uint32_t count = hc[0];
-        if (stats) {
-          float ms = 0;
-          GZ_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-          stats->kernel_ms[k] += ms;
-          stats->launches[k] += 1;
-        }
-        if (count > cap) {
-          // the buffer overflowed during this range: grow it (keeping the
-          // hits of earlier ranges), rewind the count and redo the range
-          const size_t keep = (size_t)total_hits * sizeof(SeedHit);
-          void *bigger = nullptr;
-          GZ_TRY(gz::Malloc(&bigger, std::max<size_t>((size_t)count * 2, 1024) * sizeof(SeedHit)));
-          if (keep) GZ_TRY(hipMemcpyAsync(bigger, hits_, keep, hipMemcpyDeviceToDevice, s));
-          GZ_TRY(hipStreamSynchronize(s));
-          GZ_TRY(gz::Free(hits_));
-          hits_ = bigger;
-          hits_cap_ = std::max<size_t>((size_t)count * 2, 1024) * sizeof(SeedHit);
-          hc[0] = (uint32_t)total_hits;
-          GZ_TRY(hipMemcpyAsync(counters_, hc, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-          GZ_TRY(hipStreamSynchronize(s));
-          continue;
-        }
-        total_hits = count;
-        u0 = u1;
       }
+      uint32_t *hc = (uint32_t *)pinned_;
+      GZ_TRY(hipMemcpyAsync(hc, counters_, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      GZ_TRY(hipStreamSynchronize(s));
+      const uint32_t count = hc[0];
+      if (count > cap) {
+        // the buffer overflowed: grow it to hold the run and redo it
+        GZ_TRY(gz::Free(hits_));
+        hits_ = nullptr;
+        hits_cap_ = 0;
+        const size_t want = ((size_t)count + (size_t)count / 4 + 1024) * sizeof(SeedHit);
+        GZ_TRY(gz::Malloc(&hits_, want));
+        hits_cap_ = want;
+        continue;
+      }
+      total_hits = count;
+      if (stats) {
+        size_t e = 0;
+        for (int k = 0; k < 2; ++k)
+          for (int j = 0; j < kind_of[k]; ++j, ++e) {
+            float ms = 0;
+            GZ_TRY(hipEventElapsedTime(&ms, launch_ev_[2 * e], launch_ev_[2 * e + 1]));
+            stats->kernel_ms[k] += ms;
+            stats->launches[k] += 1;
+          }
+      }
+      break;
     }
   }
   if (stats) stats->hits += total_hits;
