@@ -159,12 +159,36 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
   //    (hits: the seed view of a run that started at query hb); a failed
   //    copy of a compact view's records is left in chain_error
   std::atomic<int> chain_error(0);
+  // the keys (list, rv, member, entry) of a query's hits are distinct, so
+  // sorting them gives exactly the one-thread enumeration order
+  auto chain_records = [&](const DbChunk &chunk, size_t qi, const SeedHit *rec, uint32_t n,
+                           std::vector<KeyedHit> &keyed, std::vector<uint64_t> &scratch) {
+    keyed.resize(n);
+    for (uint32_t k = 0; k < n; ++k) {
+      const SeedHit &h = rec[k];
+      keyed[k].hi = ((uint64_t)h.list << 32) | h.rv;
+      keyed[k].lo = ((uint64_t)h.member << 32) | h.entry;
+      keyed[k].q = h.qpos - (uint32_t)coff[qi];
+      keyed[k].d = h.dpos;
+    }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<Hit> &hv = chained[qi];
+    hv.resize(n);
+    for (uint32_t k = 0; k < n; ++k) {
+      hv[k].q = keyed[k].q;
+      hv[k].d = keyed[k].d;
+    }
+    ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl, hv,
+                scratch);
+  };
   auto chain = [&](const DbChunk &chunk, const SeedHitView &hits, size_t hb, size_t qb,
                    size_t qe) {
+    // queries with two hits at one database position whose records are
+    // still on the device (a compact view)
+    std::vector<uint8_t> tied(qe - qb, 0);
     ParallelBlocks(qe - qb, threads, 64, [&](size_t b, size_t e) {
       std::vector<uint64_t> scratch, by_d, by_d_tmp;
       std::vector<KeyedHit> keyed;
-      std::vector<SeedHit> fetched;
       for (size_t qi = qb + b; qi < qb + e; ++qi) {
         const uint32_t s0 = hits.qstart[qi - hb], s1 = hits.qstart[qi - hb + 1];
         if (s0 == s1) continue;
@@ -185,38 +209,44 @@ int AlignQueryChunk(const QuerySet &q, const Database &db, const AlignerOptions 
           }
           ChainFilterSorted(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix,
                             P.nl, hv, scratch);
-          continue;
+        } else if (hits.hits) {
+          chain_records(chunk, qi, hits.hits + s0, s1 - s0, keyed, scratch);
+        } else {
+          tied[qi - qb] = 1;
         }
-        // the (list, rv, member, entry) keys are distinct, so sorting them
-        // gives exactly the one-thread enumeration order (a compact view's
-        // records come from the device for such queries)
-        const SeedHit *rec = hits.hits ? hits.hits + s0 : nullptr;
-        if (!rec) {
-          fetched.resize(s1 - s0);
-          const int f = hits.fetch(s0, s1 - s0, fetched.data());
-          if (f != 0) {
-            chain_error.store(f);
-            continue;
-          }
-          rec = fetched.data();
-        }
-        keyed.resize(s1 - s0);
-        for (uint32_t k = 0; k < s1 - s0; ++k) {
-          const SeedHit &h = rec[k];
-          keyed[k].hi = ((uint64_t)h.list << 32) | h.rv;
-          keyed[k].lo = ((uint64_t)h.member << 32) | h.entry;
-          keyed[k].q = h.qpos - (uint32_t)coff[qi];
-          keyed[k].d = h.dpos;
-        }
-        std::sort(keyed.begin(), keyed.end());
-        std::vector<Hit> &hv = chained[qi];
-        hv.resize(s1 - s0);
-        for (uint32_t k = 0; k < s1 - s0; ++k) {
-          hv[k].q = keyed[k].q;
-          hv[k].d = keyed[k].d;
-        }
-        ChainFilter(q.Sequence(qi), q.Length(qi), cut[qi], chunk.seq.data(), P.matrix, P.nl,
-                    hv, scratch);
+      }
+    });
+    if (hits.hits) return;
+    // the tied queries' records come from the device in one fetch (adjacent
+    // queries' ranges merged), then are ordered like above
+    std::vector<size_t> begin, count, at, which;
+    size_t total = 0;
+    for (size_t qi = qb; qi < qe; ++qi) {
+      if (!tied[qi - qb]) continue;
+      const size_t s0 = hits.qstart[qi - hb], s1 = hits.qstart[qi - hb + 1];
+      which.push_back(qi);
+      at.push_back(total);
+      if (!begin.empty() && begin.back() + count.back() == s0) count.back() += s1 - s0;
+      else {
+        begin.push_back(s0);
+        count.push_back(s1 - s0);
+      }
+      total += s1 - s0;
+    }
+    if (total == 0) return;
+    const SeedHit *fetched = nullptr;
+    const int f = hits.fetch(begin.size(), begin.data(), count.data(), &fetched);
+    if (f != 0) {
+      chain_error.store(f);
+      return;
+    }
+    ParallelBlocks(which.size(), threads, 4, [&](size_t b, size_t e) {
+      std::vector<uint64_t> scratch;
+      std::vector<KeyedHit> keyed;
+      for (size_t k = b; k < e; ++k) {
+        const size_t qi = which[k];
+        chain_records(chunk, qi, fetched + at[k], hits.qstart[qi - hb + 1] - hits.qstart[qi - hb],
+                      keyed, scratch);
       }
     });
   };

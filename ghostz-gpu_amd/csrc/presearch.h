@@ -44,12 +44,16 @@ struct HitPosition {
 // particular order.  The arrays belong to the device stage and stay valid
 // until its second SeedHits call after this one.  A compact view carries
 // only the positions (hits == nullptr): positions[k] is hit k's, and
-// fetch(begin, count, out) copies the full records of hits [begin, begin +
-// count) (for queries whose order needs them).
+// fetch(n, begin, count, &records) brings the full records of the n hit
+// ranges [begin[i], begin[i] + count[i]) to the host, one range after the
+// other, and returns (0 or a HIP error code) once they are all there, at
+// *records, which stays valid until the stage's next fetch; for the
+// queries whose order needs them.
 struct SeedHitView {
   const SeedHit *hits = nullptr;
   const HitPosition *positions = nullptr;
-  std::function<int(size_t begin, size_t count, SeedHit *out)> fetch;
+  std::function<int(size_t n, const size_t *begin, const size_t *count, const SeedHit **records)>
+      fetch;
   size_t count = 0;
   const uint32_t *qstart = nullptr;  // number of queries + 1
 };

@@ -107,6 +107,7 @@ class SeedStage {
   void *grouped_[2] = {nullptr, nullptr}; size_t grouped_cap_[2] = {0, 0};
   void *positions_ = nullptr; size_t positions_cap_ = 0;  // compact views' positions
   hipStream_t fetch_stream_ = nullptr;  // copies of a compact view's records
+  void *fetch_host_ = nullptr; size_t fetch_host_cap_ = 0;  // their pinned staging
   std::mutex fetch_mutex_;
   void *tile_range_ = nullptr; size_t tile_range_cap_ = 0;  // per-tile list ranges
   void *qcount_ = nullptr; size_t qcount_cap_ = 0;     // per-query counts / cursors

@@ -514,6 +514,7 @@ SeedStage::~SeedStage() {
   for (hipEvent_t e : launch_ev_)
     if (e) (void)hipEventDestroy(e);
   if (fetch_stream_) (void)hipStreamDestroy(fetch_stream_);
+  if (fetch_host_) (void)gz::HostFree(fetch_host_);
 }
 
 hipError_t SeedStage::Reserve(void **p, size_t *cap, size_t bytes) {
@@ -792,15 +793,30 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
   if (compact) {
     out->positions = (const HitPosition *)host_hits_[slot];
     const SeedHit *dev = (const SeedHit *)grouped_[slot];
-    out->fetch = [this, dev](size_t begin, size_t count, SeedHit *dst) -> int {
+    // every range of one call is copied into pinned staging before a single
+    // wait, and the records are read from there
+    out->fetch = [this, dev](size_t n, const size_t *begin, const size_t *count,
+                             const SeedHit **records) -> int {
       std::lock_guard<std::mutex> lock(fetch_mutex_);
-      if (!fetch_stream_) {
-        const hipError_t e = hipStreamCreateWithFlags(&fetch_stream_, hipStreamNonBlocking);
-        if (e != hipSuccess) return (int)e;
+      hipError_t e = hipSuccess;
+      if (!fetch_stream_) e = hipStreamCreateWithFlags(&fetch_stream_, hipStreamNonBlocking);
+      size_t total = 0;
+      for (size_t i = 0; i < n; ++i) total += count[i];
+      const size_t bytes = total * sizeof(SeedHit);
+      if (e == hipSuccess && fetch_host_cap_ < bytes) {
+        if (fetch_host_) e = gz::HostFree(fetch_host_);
+        fetch_host_ = nullptr;
+        fetch_host_cap_ = 0;
+        if (e == hipSuccess) e = gz::HostMalloc(&fetch_host_, bytes + bytes / 4, hipHostMallocDefault);
+        if (e == hipSuccess) fetch_host_cap_ = bytes + bytes / 4;
       }
-      hipError_t e = hipMemcpyAsync(dst, dev + begin, count * sizeof(SeedHit),
-                                    hipMemcpyDeviceToHost, fetch_stream_);
+      SeedHit *stage = (SeedHit *)fetch_host_;
+      for (size_t i = 0, at = 0; i < n && e == hipSuccess; at += count[i], ++i)
+        if (count[i])
+          e = hipMemcpyAsync(stage + at, dev + begin[i], count[i] * sizeof(SeedHit),
+                             hipMemcpyDeviceToHost, fetch_stream_);
       if (e == hipSuccess) e = hipStreamSynchronize(fetch_stream_);
+      *records = stage;
       return (int)e;
     };
   } else {

@@ -67,16 +67,14 @@ gz_status Search(gz_seed_searcher_t h, uint32_t qb, uint32_t qe, unsigned kinds,
     return GZ_ERROR_HIP;
   }
   // the full records (a compact view keeps them on the device)
-  std::vector<SeedHit> full;
   const SeedHit *hits = v.hits;
   if (!hits && v.count) {
-    full.resize(v.count);
-    const int f = v.fetch(0, v.count, full.data());
+    const size_t zero = 0;
+    const int f = v.fetch(1, &zero, &v.count, &hits);
     if (f != 0) {
       SetError(std::string("seed search: ") + hipGetErrorString((hipError_t)f));
       return GZ_ERROR_HIP;
     }
-    hits = full.data();
   }
   // per query, the one-thread enumeration order: by (list rank,
   // representative or value index, member, entry), which are distinct

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <charconv>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -119,9 +120,32 @@ class CpuDevice : public PresearchDevice {
     grouped.resize(hits->size());
     std::vector<uint32_t> fill(qstart.begin(), qstart.end() - 1);
     for (const SeedHit &h : *hits) grouped[fill[h.qid - qb]++] = h;
-    view->hits = grouped.data();
     view->count = grouped.size();
     view->qstart = qstart.data();
+    if (slot == 1) {
+      view->hits = grouped.data();
+      return 0;
+    }
+    // every other call serves a compact view like the product's: positions,
+    // and the records on request, by ranges
+    std::vector<HitPosition> &pos = positions_;
+    pos.resize(grouped.size());
+    for (size_t k = 0; k < grouped.size(); ++k) pos[k] = {grouped[k].qpos, grouped[k].dpos};
+    view->hits = nullptr;
+    view->positions = pos.data();
+    const SeedHit *rec = grouped.data();
+    const size_t n_rec = grouped.size();
+    view->fetch = [this, rec, n_rec](size_t n, const size_t *begin, const size_t *count,
+                                     const SeedHit **records) -> int {
+      std::lock_guard<std::mutex> lock(fetch_mutex_);
+      fetched_.clear();
+      for (size_t i = 0; i < n; ++i) {
+        if (begin[i] + count[i] > n_rec) return -3;
+        fetched_.insert(fetched_.end(), rec + begin[i], rec + begin[i] + count[i]);
+      }
+      *records = fetched_.data();
+      return 0;
+    };
     return 0;
   }
 
@@ -176,6 +200,9 @@ class CpuDevice : public PresearchDevice {
   std::vector<int> cut_, trig_;
   int presearch_cutoff_ = 0;
   std::vector<SeedHit> raw_, grouped_[2];
+  std::vector<HitPosition> positions_;  // the compact view's (slot 0 only)
+  std::vector<SeedHit> fetched_;        // the records of its last fetch
+  std::mutex fetch_mutex_;
   std::vector<uint32_t> qstart_[2];
   int flip_ = 0;
 };
