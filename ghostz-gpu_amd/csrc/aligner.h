@@ -133,6 +133,7 @@ class Aligner : public PresearchDevice {
     uint32_t *pq = nullptr, *pd = nullptr, *cells = nullptr, *perm = nullptr;
     int *sc = nullptr;
     uint32_t *counts = nullptr;  // pinned: the two sides' gated job counts
+    uint32_t *d_counts = nullptr;  // the same on the device
     hipEvent_t ready = nullptr, ev_begin = nullptr, ev_end = nullptr;
   };
   GappedSlot slots_[2];

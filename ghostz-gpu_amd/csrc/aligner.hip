@@ -367,7 +367,8 @@ int Aligner::GappedPrepare(int slot, size_t ci, uint32_t nr, uint32_t nf, const 
           e = AddPredictionBins(d_pred + off[k], d_index + off[k], cnt[k], d_keys + off[k], s,
                                 d_count + k);
       }
-      if (e == hipSuccess) e = hipMemcpyAsync(S.counts, d_count, 8, hipMemcpyDeviceToHost, s);
+      // the counts come back with the extension's results
+      S.d_counts = d_count;
       S.gated = true;
     } else if (e == hipSuccess && model) {
       // the host lays out job nr + k as the forward side of job k (both
@@ -474,6 +475,13 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
     if (e == hipSuccess && nf > 0 && !side(nr, nf, 0)) e = hipErrorUnknown;
   }
   if (e == hipSuccess) e = hipEventRecord(ev_[1], s);
+  // the copies back are queued once the kernels are done: a device-to-host
+  // copy queued behind a running kernel holds up the other streams' copies
+  // (the seed stage's hits, the chain filter's records) until that kernel
+  // ends
+  if (e == hipSuccess) e = hipEventSynchronize(ev_[1]);
+  if (e == hipSuccess && S.gated)
+    e = hipMemcpyAsync(S.counts, S.d_counts, 8, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(qpos, S.pq, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(dpos, S.pd, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(scores, S.sc, n * 4, hipMemcpyDeviceToHost, s);
@@ -540,6 +548,8 @@ int Aligner::Tracebacks(size_t ci, const std::vector<TracebackJob> &jobs, int cu
                           opt_.search.nl, opt_.search.gap_open, opt_.search.gap_extension,
                           cutoff, scratch, caps, (uint32_t)lanes, d_outs, s);
     if (e == hipSuccess) e = hipEventRecord(ev_[5], s);
+    // the outputs' copy is queued once the kernel is done (as in GappedRun)
+    if (e == hipSuccess) e = hipEventSynchronize(ev_[5]);
     if (e == hipSuccess)
       e = hipMemcpyAsync(hout, d_outs, m * sizeof(TracebackOut), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);

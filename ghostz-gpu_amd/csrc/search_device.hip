@@ -725,6 +725,10 @@ hipError_t SeedStage::Run(const DeviceQueryLists &q, const DeviceDbChunk &db,
         }
       }
       uint32_t *hc = (uint32_t *)pinned_;
+      // the count's copy is queued once the seed kernels are done: a
+      // device-to-host copy queued behind a running kernel holds up the
+      // other streams' copies until that kernel ends
+      GZ_TRY(hipStreamSynchronize(s));
       GZ_TRY(hipMemcpyAsync(hc, counters_, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
       GZ_TRY(hipStreamSynchronize(s));
       const uint32_t count = hc[0];
