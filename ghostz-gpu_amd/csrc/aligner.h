@@ -44,6 +44,18 @@ struct GappedTuner {
   static int Persistent(int config) { return (config & 2) ? 0 : 1; }
   int Pick(int bucket) const;
   void Record(int bucket, int config, double kernel_ms, double njobs);
+  // The task's layout: 0 = every bucket launched on its own with its
+  // configuration, 1 = one launch per side over all of the task's jobs (the
+  // fewest launches, each waiting once for its longest job).  Layout 0 is
+  // timed only on tasks whose buckets all ran their chosen configuration,
+  // layout 1 after that; then the faster per job is kept.
+  double layout_ms[2] = {};
+  double layout_jobs[2] = {};
+  int layout_trials[2] = {};
+  int PickLayout() const;
+  void RecordLayout(int layout, double kernel_ms, double njobs);
+  // the configuration a whole-side launch runs (32-column ring, static grid)
+  static constexpr int kWholeConfig = 2;
 };
 
 class Aligner : public PresearchDevice {
@@ -130,6 +142,7 @@ class Aligner : public PresearchDevice {
     size_t cap = 0;
     size_t n = 0;
     bool order = false, gated = false;
+    bool whole = false;  // one launch per side (GappedTuner::PickLayout)
     uint32_t *pq = nullptr, *pd = nullptr, *cells = nullptr, *perm = nullptr;
     int *sc = nullptr;
     uint32_t *counts = nullptr;  // pinned: the two sides' gated job counts
@@ -152,6 +165,7 @@ class Aligner : public PresearchDevice {
   // per (bucket, configuration): the executable graph of its launch
   // sequence, updated with each call's arguments
   bool use_graphs_ = true;
+  int forced_layout_ = -1;  // GappedTuner layout from GZ_GAPPED_LAYOUT (-1: tuned)
   hipGraphExec_t graph_exec_[kLengthBuckets * GappedTuner::kConfigs] = {};
   uint64_t graph_updates_ = 0, graph_instantiations_ = 0;
   hipError_t LaunchCaptured(int slot, hipStream_t s, const std::function<bool()> &launch);

@@ -75,6 +75,11 @@ Aligner::Aligner(const AlignerOptions &opt) : opt_(opt) {
   // (GZ_NO_GRAPHS=1 launches the same kernels directly, for comparison)
   const char *ng = getenv("GZ_NO_GRAPHS");
   use_graphs_ = !(ng && *ng && *ng != '0');
+  // the batched form's gapped layout per task is tuned (GZ_GAPPED_LAYOUT=
+  // buckets or whole fixes it, for comparison and tests)
+  const char *gl = getenv("GZ_GAPPED_LAYOUT");
+  if (gl && strcmp(gl, "buckets") == 0) forced_layout_ = 0;
+  if (gl && strcmp(gl, "whole") == 0) forced_layout_ = 1;
 }
 
 Aligner::~Aligner() {
@@ -272,6 +277,18 @@ void GappedTuner::Record(int bucket, int config, double kernel_ms, double njobs)
   ++trials[bucket][config];
 }
 
+int GappedTuner::PickLayout() const {
+  if (layout_trials[0] < kTrials) return 0;
+  if (layout_trials[1] < kTrials) return 1;
+  return layout_ms[1] * layout_jobs[0] < layout_ms[0] * layout_jobs[1] ? 1 : 0;
+}
+
+void GappedTuner::RecordLayout(int layout, double kernel_ms, double njobs) {
+  layout_ms[layout] += kernel_ms;
+  layout_jobs[layout] += njobs;
+  ++layout_trials[layout];
+}
+
 int Aligner::GappedExtend(size_t ci, uint32_t nr, uint32_t nf, uint32_t *qpos, uint32_t *dpos,
                           int *scores, uint32_t *cells, const uint16_t *keys, bool learned,
                           AlignStats *st, const uint32_t *bucket_off) {
@@ -287,7 +304,12 @@ int Aligner::GappedPrepare(int slot, size_t ci, uint32_t nr, uint32_t nf, const 
   S.n = (size_t)nr + nf;
   S.order = false;
   S.gated = false;
+  S.whole = false;
   if (S.n == 0) return 0;
+  if (bucket_off && nr == nf) {
+    std::lock_guard<std::mutex> lock(tuner_mutex_);
+    S.whole = forced_layout_ >= 0 ? forced_layout_ == 1 : tuner_.PickLayout() == 1;
+  }
   const size_t n = S.n;
   hipStream_t s = schedule_stream_;
   if (!S.ready) {
@@ -387,7 +409,7 @@ int Aligner::GappedPrepare(int slot, size_t ci, uint32_t nr, uint32_t nf, const 
     } else if (e == hipSuccess) {
       e = hipMemcpyAsync(d_keys, keys, n * 2, hipMemcpyHostToDevice, s);
     }
-    if (bucket_off) {
+    if (bucket_off && !S.whole) {
       // each side's buckets are ordered separately (positions in the
       // permutation are bucket-relative)
       for (int side = 0; side < 2 && e == hipSuccess; ++side)
@@ -437,13 +459,15 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
   };
   if (e == hipSuccess) e = hipEventRecord(ev_[0], s);
   int tuned[kLengthBuckets] = {-1, -1, -1, -1};
-  if (bucket_off && nr == nf && e == hipSuccess) {
+  bool settled = true;  // every bucket ran its chosen configuration
+  const bool bucketed = bucket_off && nr == nf && !S.whole;
+  if (bucketed && e == hipSuccess) {
     // the workspace for the largest bucket, allocated before any capture
     uint32_t most = 0;
     for (int b = 0; b < kLengthBuckets; ++b) most = std::max(most, bucket_off[b + 1] - bucket_off[b]);
     if (gz_gapped_extender_reserve(gapped_, most, &wrap) != GZ_OK) e = hipErrorUnknown;
   }
-  if (bucket_off && nr == nf) {
+  if (bucketed) {
     // every query-length bucket with the configuration the tuner picks for
     // it (first ring width, persistent work queue or static grid), each
     // bucket timed on its own
@@ -457,6 +481,8 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
       }
       std::lock_guard<std::mutex> lock(tuner_mutex_);
       tuned[b] = tuner_.Pick(b);
+      for (int c = 0; c < GappedTuner::kConfigs; ++c)
+        if (tuner_.trials[b][c] < GappedTuner::kTrials) settled = false;
       if (gz_gapped_extender_set_narrow_ring(gapped_, GappedTuner::Narrow(tuned[b])) != GZ_OK ||
           gz_gapped_extender_set_persistent(gapped_, GappedTuner::Persistent(tuned[b])) != GZ_OK)
         e = hipErrorUnknown;
@@ -471,6 +497,12 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
       if (e == hipSuccess) e = hipEventRecord(tune_ev_[2 * b + 1], s);
     }
   } else {
+    if (S.whole &&
+        (gz_gapped_extender_set_narrow_ring(
+             gapped_, GappedTuner::Narrow(GappedTuner::kWholeConfig)) != GZ_OK ||
+         gz_gapped_extender_set_persistent(
+             gapped_, GappedTuner::Persistent(GappedTuner::kWholeConfig)) != GZ_OK))
+      e = hipErrorUnknown;
     if (e == hipSuccess && nr > 0 && !side(0, nr, 1)) e = hipErrorUnknown;
     if (e == hipSuccess && nf > 0 && !side(nr, nf, 0)) e = hipErrorUnknown;
   }
@@ -503,6 +535,10 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
       std::lock_guard<std::mutex> lock(tuner_mutex_);
       tuner_.Record(b, tuned[b], bm, 2.0 * (bucket_off[b + 1] - bucket_off[b]));
     }
+  }
+  if (e == hipSuccess && bucket_off && nr == nf && (S.whole || settled)) {
+    std::lock_guard<std::mutex> lock(tuner_mutex_);
+    tuner_.RecordLayout(S.whole ? 1 : 0, ms, (double)n);
   }
   if (e != hipSuccess) return (int)e;
   st->gapped_kernel_ms += ms;

@@ -6,6 +6,7 @@ host) must write byte-identical tabular output to the oracle's gzo_align on
 the same prepared queries and database (whose preparation is checked against
 the reference's own code in test_host_prep.py).
 """
+import os
 import sys
 
 import numpy as np
@@ -83,9 +84,23 @@ def run_case(name, queries, db, m, nl, schedules=(gz.SCHEDULE_PLAIN,), chunk_siz
     want, ost = oracle_text(qv, db, m, nl, **kw)
     say("  oracle lines", want.count(b"\n"), ost)
     assert want.count(b"\n") > 0 or name.startswith("edge")
-    forms = [(sch, 0) for sch in schedules] + [(schedules[-1], 1)]
-    for sch, batched in forms:
-        al = gz.Aligner(db, schedule=sch, batched=batched, pipeline_blocks=6, **params)
+    # the batched form with its tuned gapped layout, and with every task's
+    # jobs in one launch per side (a layout the tuner reaches only after its
+    # trials)
+    forms = [(sch, 0, None) for sch in schedules] + [(schedules[-1], 1, None),
+                                                      (schedules[-1], 1, "whole")]
+    for sch, batched, layout in forms:
+        old = os.environ.get("GZ_GAPPED_LAYOUT")
+        if layout:
+            os.environ["GZ_GAPPED_LAYOUT"] = layout
+        try:
+            al = gz.Aligner(db, schedule=sch, batched=batched, pipeline_blocks=6, **params)
+        finally:
+            if layout:
+                if old is None:
+                    del os.environ["GZ_GAPPED_LAYOUT"]
+                else:
+                    os.environ["GZ_GAPPED_LAYOUT"] = old
         if sch == gz.SCHEDULE_LEARNED:
             al.set_cost_model(gz.Mlp.init_weights(7))
         got, st = al.align(queries)
