@@ -75,6 +75,7 @@ class Aligner : public PresearchDevice {
   // representatives and members, bit 1: non-clustered values; default both).
   const DeviceQueryLists &QueryLists() const { return dq_; }
   void SetSeedKinds(unsigned kinds) { seed_kinds_ = kinds; }
+  void SetConcurrentExtensions(bool two) override;
   const std::vector<uint8_t> &LastConcatenatedQueries() const { return concat_; }
 
   // PresearchDevice
@@ -112,7 +113,6 @@ class Aligner : public PresearchDevice {
   const Database *db_ = nullptr;
   std::vector<std::unique_ptr<DeviceChunk> > chunks_;
   hipStream_t stream_ = nullptr;         // uploads and the seed stage
-  hipStream_t extend_stream_ = nullptr;  // gapped extension
   // result build (tracebacks): its own stream, since the batched form runs
   // it on another thread while the extension stream may be capturing a graph
   hipStream_t traceback_stream_ = nullptr;
@@ -120,7 +120,9 @@ class Aligner : public PresearchDevice {
   uint8_t *d_reduced_ = nullptr;
   SeedStage seed_;
   DeviceListBuilder lists_;  // query hash lists of the resident chunk
-  gz_gapped_extender_s *gapped_ = nullptr;
+  // per pipeline slot: the gapped extender (its workspace) and the
+  // extension stream, so that two tasks' extensions can run at once
+  gz_gapped_extender_s *gapped_[2] = {nullptr, nullptr};
   std::vector<GappedTrace> traces_;
   std::vector<uint8_t> concat_;
   // resident query chunk
@@ -151,6 +153,12 @@ class Aligner : public PresearchDevice {
   };
   GappedSlot slots_[2];
   hipStream_t schedule_stream_ = nullptr;
+  // the extension streams: [0] always, [1] created for concurrent
+  // extensions (each slot then extends on its own stream and extender)
+  hipStream_t extend_streams_[2] = {nullptr, nullptr};
+  bool two_extensions_ = false;
+  hipEvent_t run_ev_[2][2] = {};  // per slot: around its gapped kernels
+  std::mutex stats_mutex_;        // AlignStats updates from both slots
   // result-build workspace
   void *tb_ws_ = nullptr;
   size_t tb_ws_cap_ = 0;
@@ -161,12 +169,12 @@ class Aligner : public PresearchDevice {
   uint32_t learned_gate_ = 44;
   GappedTuner tuner_;
   std::mutex tuner_mutex_;
-  hipEvent_t tune_ev_[2 * kLengthBuckets] = {};
+  hipEvent_t tune_ev_[2][2 * kLengthBuckets] = {};
   // per (bucket, configuration): the executable graph of its launch
   // sequence, updated with each call's arguments
   bool use_graphs_ = true;
   int forced_layout_ = -1;  // GappedTuner layout from GZ_GAPPED_LAYOUT (-1: tuned)
-  hipGraphExec_t graph_exec_[kLengthBuckets * GappedTuner::kConfigs] = {};
+  hipGraphExec_t graph_exec_[2 * kLengthBuckets * GappedTuner::kConfigs] = {};
   uint64_t graph_updates_ = 0, graph_instantiations_ = 0;
   hipError_t LaunchCaptured(int slot, hipStream_t s, const std::function<bool()> &launch);
 };

@@ -96,15 +96,21 @@ Aligner::~Aligner() {
     if (b.first) (void)gz::HostFree(b.first);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : tune_ev_)
-    if (e) (void)hipEventDestroy(e);
+  for (auto &slot : tune_ev_)
+    for (hipEvent_t e : slot)
+      if (e) (void)hipEventDestroy(e);
+  for (auto &slot : run_ev_)
+    for (hipEvent_t e : slot)
+      if (e) (void)hipEventDestroy(e);
   for (hipGraphExec_t x : graph_exec_)
     if (x) (void)hipGraphExecDestroy(x);
-  if (gapped_) gz_gapped_extender_destroy(gapped_);
+  for (gz_gapped_extender_s *g : gapped_)
+    if (g) gz_gapped_extender_destroy(g);
   if (d_matrix_) (void)gz::Free(d_matrix_);
   if (d_reduced_) (void)gz::Free(d_reduced_);
   if (stream_) (void)hipStreamDestroy(stream_);
-  if (extend_stream_) (void)hipStreamDestroy(extend_stream_);
+  for (hipStream_t x : extend_streams_)
+    if (x) (void)hipStreamDestroy(x);
   if (traceback_stream_) (void)hipStreamDestroy(traceback_stream_);
   if (schedule_stream_) (void)hipStreamDestroy(schedule_stream_);
 }
@@ -112,7 +118,8 @@ Aligner::~Aligner() {
 hipError_t Aligner::SetDatabase(const Database *db) {
   db_ = db;
   if (!stream_) GZ_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  if (!extend_stream_) GZ_TRY(hipStreamCreateWithFlags(&extend_stream_, hipStreamNonBlocking));
+  if (!extend_streams_[0])
+    GZ_TRY(hipStreamCreateWithFlags(&extend_streams_[0], hipStreamNonBlocking));
   if (!traceback_stream_)
     GZ_TRY(hipStreamCreateWithFlags(&traceback_stream_, hipStreamNonBlocking));
   if (!schedule_stream_)
@@ -137,10 +144,12 @@ hipError_t Aligner::SetDatabase(const Database *db) {
   if (d_reduced_) GZ_TRY(gz::Free(d_reduced_));
   GZ_TRY(Upload(&d_reduced_, rm, stream_));
   GZ_TRY(hipStreamSynchronize(stream_));
-  if (!gapped_ && gz_gapped_extender_create(&gapped_) != GZ_OK) return hipErrorUnknown;
   // the batched form runs the first ring as a persistent work-queue launch
-  if (gz_gapped_extender_set_persistent(gapped_, opt_.batched ? 1 : 0) != GZ_OK)
-    return hipErrorUnknown;
+  for (gz_gapped_extender_s *&g : gapped_) {
+    if (!g && gz_gapped_extender_create(&g) != GZ_OK) return hipErrorUnknown;
+    if (gz_gapped_extender_set_persistent(g, opt_.batched ? 1 : 0) != GZ_OK)
+      return hipErrorUnknown;
+  }
   return hipSuccess;
 }
 
@@ -212,13 +221,13 @@ int Aligner::BeginQueryChunk(const QuerySet &q, const std::vector<uint8_t> &conc
   sp_.max_mismatches = db_->max_mismatches;
   sp_.byte_tables = byte_tables;
 
-  if (gz_gapped_extender_set_max_sequence_length(gapped_, q.max_length) != GZ_OK ||
-      gz_gapped_extender_set_queries(gapped_, kDelimiter, d_q) != GZ_OK ||
-      gz_gapped_extender_set_score_parameters(gapped_, d_matrix_, opt_.search.nl,
-                                              opt_.search.gap_open,
-                                              opt_.search.gap_extension,
-                                              presearch_cutoff) != GZ_OK)
-    return (int)hipErrorUnknown;
+  for (gz_gapped_extender_s *g : gapped_)
+    if (gz_gapped_extender_set_max_sequence_length(g, q.max_length) != GZ_OK ||
+        gz_gapped_extender_set_queries(g, kDelimiter, d_q) != GZ_OK ||
+        gz_gapped_extender_set_score_parameters(g, d_matrix_, opt_.search.nl,
+                                                opt_.search.gap_open, opt_.search.gap_extension,
+                                                presearch_cutoff) != GZ_OK)
+      return (int)hipErrorUnknown;
   return 0;
 }
 
@@ -275,6 +284,15 @@ void GappedTuner::Record(int bucket, int config, double kernel_ms, double njobs)
   ms[bucket][config] += kernel_ms;
   jobs[bucket][config] += njobs;
   ++trials[bucket][config];
+}
+
+void Aligner::SetConcurrentExtensions(bool two) {
+  two_extensions_ = two;
+  // the second extension stream exists from here on (created before the
+  // pipeline's threads start)
+  if (two && !extend_streams_[1] &&
+      hipStreamCreateWithFlags(&extend_streams_[1], hipStreamNonBlocking) != hipSuccess)
+    extend_streams_[1] = nullptr;  // GappedRun creates it then, or reports the error
 }
 
 int GappedTuner::PickLayout() const {
@@ -436,11 +454,19 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
   const size_t n = S.n;
   if (n == 0) return 0;
   if (n != (size_t)nr + nf) return (int)hipErrorInvalidValue;
-  hipStream_t s = extend_stream_;
+  // with concurrent extensions each slot extends on its own stream with its
+  // own extender, so the other slot's task can run at the same time
+  const int x = two_extensions_ ? slot : 0;
+  if (!extend_streams_[x])
+    GZ_TRY(hipStreamCreateWithFlags(&extend_streams_[x], hipStreamNonBlocking));
+  hipStream_t s = extend_streams_[x];
+  gz_gapped_extender_s *gapped = gapped_[x];
+  hipEvent_t *ev = run_ev_[x];
+  hipEvent_t *tune_ev = tune_ev_[x];
   for (int i = 0; i < 2; ++i)
-    if (!ev_[i]) GZ_TRY(hipEventCreate(&ev_[i]));
+    if (!ev[i]) GZ_TRY(hipEventCreate(&ev[i]));
   const DeviceChunk &dc = *chunks_[ci];
-  if (gz_gapped_extender_set_database(gapped_, dc.packed) != GZ_OK) return (int)hipErrorUnknown;
+  if (gz_gapped_extender_set_database(gapped, dc.packed) != GZ_OK) return (int)hipErrorUnknown;
   const bool order = S.order;
   uint32_t *rc = cells ? S.cells : nullptr;
   hipError_t e = hipStreamWaitEvent(s, S.ready, 0);
@@ -451,13 +477,13 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
     uint32_t *c = rc ? rc + off : nullptr;
     const gz_status r =
         order ? gz_gapped_extender_extend_one_side_score_only_ordered_async(
-                    gapped_, count, reverse, S.perm + off, S.pq + off, S.pd + off, S.sc + off,
+                    gapped, count, reverse, S.perm + off, S.pq + off, S.pd + off, S.sc + off,
                     c, &wrap)
               : gz_gapped_extender_extend_one_side_score_only_cells_async(
-                    gapped_, count, reverse, S.pq + off, S.pd + off, S.sc + off, c, &wrap);
+                    gapped, count, reverse, S.pq + off, S.pd + off, S.sc + off, c, &wrap);
     return r == GZ_OK;
   };
-  if (e == hipSuccess) e = hipEventRecord(ev_[0], s);
+  if (e == hipSuccess) e = hipEventRecord(ev[0], s);
   int tuned[kLengthBuckets] = {-1, -1, -1, -1};
   bool settled = true;  // every bucket ran its chosen configuration
   const bool bucketed = bucket_off && nr == nf && !S.whole;
@@ -465,7 +491,7 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
     // the workspace for the largest bucket, allocated before any capture
     uint32_t most = 0;
     for (int b = 0; b < kLengthBuckets; ++b) most = std::max(most, bucket_off[b + 1] - bucket_off[b]);
-    if (gz_gapped_extender_reserve(gapped_, most, &wrap) != GZ_OK) e = hipErrorUnknown;
+    if (gz_gapped_extender_reserve(gapped, most, &wrap) != GZ_OK) e = hipErrorUnknown;
   }
   if (bucketed) {
     // every query-length bucket with the configuration the tuner picks for
@@ -474,44 +500,45 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
     for (int b = 0; b < kLengthBuckets && e == hipSuccess; ++b) {
       const uint32_t o = bucket_off[b], c = bucket_off[b + 1] - o;
       if (c == 0) continue;
-      if (!tune_ev_[2 * b]) {
-        e = hipEventCreate(&tune_ev_[2 * b]);
-        if (e == hipSuccess) e = hipEventCreate(&tune_ev_[2 * b + 1]);
+      if (!tune_ev[2 * b]) {
+        e = hipEventCreate(&tune_ev[2 * b]);
+        if (e == hipSuccess) e = hipEventCreate(&tune_ev[2 * b + 1]);
         if (e != hipSuccess) break;
       }
       std::lock_guard<std::mutex> lock(tuner_mutex_);
       tuned[b] = tuner_.Pick(b);
       for (int c = 0; c < GappedTuner::kConfigs; ++c)
         if (tuner_.trials[b][c] < GappedTuner::kTrials) settled = false;
-      if (gz_gapped_extender_set_narrow_ring(gapped_, GappedTuner::Narrow(tuned[b])) != GZ_OK ||
-          gz_gapped_extender_set_persistent(gapped_, GappedTuner::Persistent(tuned[b])) != GZ_OK)
+      if (gz_gapped_extender_set_narrow_ring(gapped, GappedTuner::Narrow(tuned[b])) != GZ_OK ||
+          gz_gapped_extender_set_persistent(gapped, GappedTuner::Persistent(tuned[b])) != GZ_OK)
         e = hipErrorUnknown;
-      if (e == hipSuccess) e = hipEventRecord(tune_ev_[2 * b], s);
+      if (e == hipSuccess) e = hipEventRecord(tune_ev[2 * b], s);
       // the bucket's launch sequence (both sides: counter resets, the
       // first ring, the 64-column ring, the global ring) as one graph
       auto both = [&]() { return side(o, c, 1) && side(nr + o, c, 0); };
       if (e == hipSuccess) {
-        if (use_graphs_) e = LaunchCaptured(b * GappedTuner::kConfigs + tuned[b], s, both);
+        const int key = (x * kLengthBuckets + b) * GappedTuner::kConfigs + tuned[b];
+        if (use_graphs_) e = LaunchCaptured(key, s, both);
         else if (!both()) e = hipErrorUnknown;
       }
-      if (e == hipSuccess) e = hipEventRecord(tune_ev_[2 * b + 1], s);
+      if (e == hipSuccess) e = hipEventRecord(tune_ev[2 * b + 1], s);
     }
   } else {
     if (S.whole &&
         (gz_gapped_extender_set_narrow_ring(
-             gapped_, GappedTuner::Narrow(GappedTuner::kWholeConfig)) != GZ_OK ||
+             gapped, GappedTuner::Narrow(GappedTuner::kWholeConfig)) != GZ_OK ||
          gz_gapped_extender_set_persistent(
-             gapped_, GappedTuner::Persistent(GappedTuner::kWholeConfig)) != GZ_OK))
+             gapped, GappedTuner::Persistent(GappedTuner::kWholeConfig)) != GZ_OK))
       e = hipErrorUnknown;
     if (e == hipSuccess && nr > 0 && !side(0, nr, 1)) e = hipErrorUnknown;
     if (e == hipSuccess && nf > 0 && !side(nr, nf, 0)) e = hipErrorUnknown;
   }
-  if (e == hipSuccess) e = hipEventRecord(ev_[1], s);
+  if (e == hipSuccess) e = hipEventRecord(ev[1], s);
   // the copies back are queued once the kernels are done: a device-to-host
   // copy queued behind a running kernel holds up the other streams' copies
   // (the seed stage's hits, the chain filter's records) until that kernel
   // ends
-  if (e == hipSuccess) e = hipEventSynchronize(ev_[1]);
+  if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
   if (e == hipSuccess && S.gated)
     e = hipMemcpyAsync(S.counts, S.d_counts, 8, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(qpos, S.pq, n * 4, hipMemcpyDeviceToHost, s);
@@ -520,17 +547,20 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
   if (e == hipSuccess && cells) e = hipMemcpyAsync(cells, S.cells, n * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   // the slot's schedule ran before its ready event, which this stream waited for
-  if (e == hipSuccess && S.gated) st->gated_jobs += (uint64_t)S.counts[0] + S.counts[1];
+  if (e == hipSuccess && S.gated) {
+    std::lock_guard<std::mutex> lock(stats_mutex_);
+    st->gated_jobs += (uint64_t)S.counts[0] + S.counts[1];
+  }
   uint32_t fails = 0;
-  if (e == hipSuccess && (gz_gapped_extender_failures(gapped_, &fails) != GZ_OK || fails != 0))
+  if (e == hipSuccess && (gz_gapped_extender_failures(gapped, &fails) != GZ_OK || fails != 0))
     e = hipErrorUnknown;
   float ms = 0, ms2 = 0;
-  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev_[0], ev_[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
   if (e == hipSuccess && order) e = hipEventElapsedTime(&ms2, S.ev_begin, S.ev_end);
   for (int b = 0; b < kLengthBuckets && e == hipSuccess; ++b) {
     if (tuned[b] < 0) continue;
     float bm = 0;
-    e = hipEventElapsedTime(&bm, tune_ev_[2 * b], tune_ev_[2 * b + 1]);
+    e = hipEventElapsedTime(&bm, tune_ev[2 * b], tune_ev[2 * b + 1]);
     if (e == hipSuccess) {
       std::lock_guard<std::mutex> lock(tuner_mutex_);
       tuner_.Record(b, tuned[b], bm, 2.0 * (bucket_off[b + 1] - bucket_off[b]));
@@ -541,6 +571,7 @@ int Aligner::GappedRun(int slot, size_t ci, uint32_t nr, uint32_t nf, uint32_t *
     tuner_.RecordLayout(S.whole ? 1 : 0, ms, (double)n);
   }
   if (e != hipSuccess) return (int)e;
+  std::lock_guard<std::mutex> lock(stats_mutex_);
   st->gapped_kernel_ms += ms;
   st->schedule_kernel_ms += ms2;
   st->gapped_launches += (nr > 0) + (nf > 0);

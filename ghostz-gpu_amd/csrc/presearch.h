@@ -161,9 +161,13 @@ class PresearchDevice {
   // GappedExtend in two halves for pipelined callers: GappedPrepare hands
   // the jobs of pipeline slot `slot` (0 or 1) to the device, which may start
   // ordering them; GappedRun extends them (same arguments) and writes the
-  // results back.  Prepare(slot) may overlap Run of the other slot; the
-  // host arrays stay untouched until Run(slot) returns.  The defaults run
-  // the whole extension in GappedRun.
+  // results back.  Prepare(slot) may overlap Run of the other slot, and the
+  // two slots' Runs may be called at once from two threads (GappedExtend
+  // must allow that); the host arrays stay untouched until Run(slot)
+  // returns.  The defaults run the whole extension in GappedRun.
+  // SetConcurrentExtensions(true) announces that the two slots' Runs will
+  // overlap (a device may then give each slot its own stream).
+  virtual void SetConcurrentExtensions(bool two) { (void)two; }
   virtual int GappedPrepare(int slot, size_t chunk, uint32_t n_reverse, uint32_t n_forward,
                             const uint32_t *qpos, const uint32_t *dpos, const uint16_t *keys,
                             bool learned, const uint32_t *bucket_off) {
