@@ -104,3 +104,11 @@ def test_small_many_blocks():
     check("small blocks", qd, dd, db_chunk_size=20_000, schedules=(0,), batched=(1,))
     check("small blocks stale", qd, dd, db_chunk_size=20_000, schedules=(0,), batched=(1,),
           force_stale=5)
+
+
+def test_small_blocks_two_extenders():
+    # queries past 2048 residues: the batched form runs two extender threads
+    # (even and odd tasks at once, published in task order)
+    qd, dd = synthetic.protein_workload(41, 150, 70, qlen=(1800, 2300))
+    assert np.diff(qd[2].astype(np.int64)).max() > 2048
+    check("two extenders", qd, dd, db_chunk_size=20_000, schedules=(0,), batched=(1,))

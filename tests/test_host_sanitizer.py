@@ -54,7 +54,7 @@ def test_batched_pipeline_is_clean_under_thread_sanitizer():
     # the batched form's seeder, extender, AddResults and writer threads
     env = dict(os.environ, LD_PRELOAD=_runtime("libtsan.so"), TSAN_OPTIONS="halt_on_error=1")
     r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "host_pipeline_cases.py"),
-                        TSAN_LIB, "test_small_many_blocks"],
+                        TSAN_LIB, "test_small_many_blocks", "test_small_blocks_two_extenders"],
                        env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                        timeout=900)
     assert r.returncode == 0, r.stdout[-4000:]
